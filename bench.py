@@ -1,0 +1,176 @@
+"""Benchmark of the swpc_3d time-stepping core on one MI355X.
+
+Workload: the plain 3-D viscoelastic method at 512 x 512 x 256 cells,
+dx = 0.25 km, dt = 10 ms, FP32; a 1-D layered medium (the reference's
+example lhm.dat layers, Qp 600 / Qs 300, 3 Zener bodies) with a 5 %-rms
+von Karman perturbation (a = 2 km, kappa = 0.5) from the seed, 20-cell
+ADE-PML, one double-couple Kupper source at 10 km depth, 64 receivers on
+the free surface.  A step is one full time step (stress + velocity
+half-steps, PML, source, receivers) over the whole grid; inputs are
+resident in HBM before the timed region.
+
+Prints one JSON line with cell-updates/s, the HBM roofline of the dominant
+kernel (HIP events on the library's own stream) and a CPU baseline: the
+oracle (C restatement of the reference loops, OpenMP, all host cores) on a
+bounded sample of the same grid.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
+# algorithmic bytes per cell-update (compulsory traffic, FP32, NM = 3)
+B_INT_STRESS = 58 * 4   # V 3r, medium 7r, R 18r/18w, S 6r/6w
+B_INT_VEL = 15 * 4      # S 6r, b 3r, V 3r/3w
+B_PML_STRESS = 38 * 4
+B_PML_VEL = 33 * 4
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=512)
+    ap.add_argument("--ny", type=int, default=512)
+    ap.add_argument("--nz", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the solver holds the whole domain on one MI355X "
+              "(multi-GPU sharding is design-only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from openswpc_amd import problem
+    from openswpc_amd.solver import Swpc3D
+
+    nt = args.warmup + args.steps
+    t0 = time.time()
+    case = problem.case3d(args.nx, args.ny, args.nz, 0.25, 0.01, nt, na=20, seed=args.seed,
+                          rmed=(0.05, 2.0, 0.5), src_depth=10.0, nst_side=8, rmed_tile=(128, 128, 128))
+    g = case.grid
+    log(f"setup {time.time() - t0:.1f} s: grid {g.nx}x{g.ny}x{g.nz}, vmax {case.medium.vmax:.2f} km/s, "
+        f"courant {problem.stability3(g, case.medium.vmax):.3f}, {len(case.stations)} receivers")
+
+    n_int = g.n_interior_cells()
+    n_pml = g.n_pml_cells()
+    ncell = g.nx * g.ny * g.nz
+    assert n_int + n_pml == ncell
+
+    sim = Swpc3D(g, case.medium, case.pml, case.glut, case.stations, case.ntdec_w, nt)
+    log(f"device memory {sim.memory() / 2**30:.2f} GiB")
+    sim.run(1, args.warmup)
+    sim.sync()
+    sim.profile(True)
+    t1 = time.perf_counter()
+    sim.run(args.warmup + 1, args.steps)
+    sim.sync()
+    wall = time.perf_counter() - t1
+    ms, cnt = sim.profile_read()
+    sim.profile(False)
+    vel, _ = sim.receivers()
+    vmax = float(np.abs(vel).max())
+    sim.close()
+    if not np.isfinite(vmax):
+        raise RuntimeError("non-finite receiver output")
+
+    ms_step = wall / args.steps * 1e3
+    cups = ncell * args.steps / wall
+    # dominant kernel: interior stress (class 0)
+    avg_stress = ms[0] / max(cnt[0], 1) * 1e-3
+    alg_stress = B_INT_STRESS * n_int
+    achieved = alg_stress / avg_stress
+    step_alg = (B_INT_STRESS + B_INT_VEL) * n_int + (B_PML_STRESS + B_PML_VEL) * n_pml
+    kern = {name: {"avg_us": ms[q] / max(cnt[q], 1) * 1e3, "launches": int(cnt[q])}
+            for q, name in enumerate(["stress_int", "stress_pml", "vel_int", "vel_pml", "other"])}
+    log(f"timed {args.steps} steps: {wall:.3f} s, {ms_step:.3f} ms/step, {cups / 1e9:.2f} G cell-updates/s, "
+        f"step alg {step_alg * args.steps / wall / 1e12:.2f} TB/s; kernels {kern}")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(case, args.cpu_steps)
+
+    out = {
+        "metric": "cell-updates/s",
+        "value": cups,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"swpc_3d plain viscoelastic {g.nx}x{g.ny}x{g.nz} dx=0.25km dt=10ms na=20 NM=3, "
+                        f"layered + 5% von Karman medium, 1 double couple, {len(case.stations)} receivers",
+            "nx": g.nx, "ny": g.ny, "nz": g.nz, "interior_cells": n_int, "pml_cells": n_pml,
+            "parallelism": "single GPU, whole domain",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k3_stress_int",
+            "achieved": achieved / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK,
+            "traffic": None,
+            "alg_bytes_per_launch": alg_stress,
+            "avg_launch_us": avg_stress * 1e6,
+        },
+        "step_roofline": {
+            "alg_bytes_per_step": step_alg,
+            "achieved_GBs": step_alg * args.steps / wall / 1e9,
+            "frac": step_alg * args.steps / wall / HBM_PEAK,
+        },
+        "kernels": kern,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(case, steps):
+    """Oracle FP32 (C/OpenMP restatement of the reference loops) on the same
+    grid for a bounded number of steps, all host cores."""
+    try:
+        from oracle.oracle import Oracle3D, olib
+    except Exception as e:  # pragma: no cover
+        log(f"cpu baseline unavailable: {e}")
+        return None
+    ncores = len(os.sched_getaffinity(0))
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    ref = Oracle3D(case, 32)
+    nth = olib(32).oracle_num_threads()
+    ref.run(1, 1)
+    t = time.perf_counter()
+    ref.run(2, steps)
+    dt = time.perf_counter() - t
+    g = case.grid
+    v = g.nx * g.ny * g.nz * steps / dt
+    log(f"cpu baseline: {steps} steps in {dt:.2f} s on {nth} threads ({ncores} cores visible): {v / 1e6:.1f} M cell-updates/s")
+    return {"value": v, "unit": "cell-updates/s", "cores": int(nth), "kind": "port",
+            "sample": f"{steps} full steps (after 1 untimed) of the same {g.nx}x{g.ny}x{g.nz} case, FP32 oracle"}
+
+
+if __name__ == "__main__":
+    main()

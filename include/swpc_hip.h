@@ -1,0 +1,218 @@
+/*
+ * swpc_hip.h -- C ABI of the MI355X-native OpenSWPC time-stepping core.
+ *
+ * The reference drives its hot path through parameterless Fortran module
+ * subroutines that read and write module-global arrays.  Each entry point
+ * below replaces one of them; the Fortran host reaches them through the
+ * bind(C) interfaces in include/swpc_hip.f90.
+ *
+ * Conventions
+ *   - Every call returns 0 on success and a negative code on error;
+ *     swpc_last_error() returns the text of the calling thread's last
+ *     error.  No exception crosses the ABI.
+ *   - Host arrays use the reference's Fortran layout: 3-D fields are
+ *     (k,i,j) with k fastest over the memory box [-2:n+3] in each axis
+ *     (src/swpc_3d/m_global.f90:297-302); P-SV fields are (k,i) over
+ *     [-2:nz+3] x [-2:nx+3] (src/swpc_psv/m_global.f90).  The library
+ *     copies them into its own x-fastest device layout; the caller keeps
+ *     ownership of host memory.
+ *   - Indices (isrc, ist, kfs, ...) are the reference's 1-based cell
+ *     numbers.
+ *   - All work of one context is ordered on one HIP stream.  Only the
+ *     get/vmax/sync calls wait for it.  A context is not thread-safe;
+ *     several contexts may coexist.
+ */
+#ifndef SWPC_HIP_H
+#define SWPC_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum swpc_precision {
+    SWPC_FP32 = 0,        /* state, memory variables and arithmetic in FP32 (reference built with MP = SP) */
+    SWPC_FP64 = 1,        /* the same scheme in FP64 */
+    SWPC_FP32_S16 = 2     /* FP32 arithmetic and V/S; memory variables and medium stored in 16 bit with per-array scale */
+};
+
+enum swpc_error {
+    SWPC_OK = 0,
+    SWPC_EINVAL = -1,     /* bad argument */
+    SWPC_EHIP = -2,       /* HIP runtime error */
+    SWPC_ESTATE = -3,     /* call out of order (e.g. step before set_medium) */
+    SWPC_ENOMEM = -4
+};
+
+/* Field identifiers for get_field / set_field. */
+enum swpc3d_field {
+    SWPC3D_VX = 0, SWPC3D_VY, SWPC3D_VZ,
+    SWPC3D_SXX, SWPC3D_SYY, SWPC3D_SZZ, SWPC3D_SYZ, SWPC3D_SXZ, SWPC3D_SXY,
+    /* memory variables, component-major: R{xx,yy,zz,yz,xz,xy} x m = 1..3 */
+    SWPC3D_R0 = 9,
+    /* ADE arrays of the PML, in the order axVx ayVx azVx axVy ayVy azVy
+       axVz ayVz azVz axSxx aySxy azSxz axSxy aySyy azSyz axSxz aySyz azSzz */
+    SWPC3D_ADE0 = 27,
+    SWPC3D_NFIELD = 45
+};
+
+enum swpc2d_field {
+    SWPC2D_VX = 0, SWPC2D_VZ, SWPC2D_SXX, SWPC2D_SZZ, SWPC2D_SXZ,
+    SWPC2D_R0 = 5,        /* R{xx,zz,xz} x m = 1..3 */
+    SWPC2D_ADE0 = 14,     /* axVx azVx axVz azVz axSxx azSxz axSxz azSzz */
+    SWPC2D_NFIELD = 22
+};
+
+const char *swpc_last_error(void);
+int swpc_version(void);
+/* Number of HIP devices visible (0 when no GPU); never fails. */
+int swpc_device_count(void);
+
+/* ------------------------------------------------------------------ */
+/* 3-D (swpc_3d)                                                       */
+/* ------------------------------------------------------------------ */
+
+typedef struct swpc3d_ctx swpc3d_ctx;
+
+typedef struct {
+    int nx, ny, nz;       /* grid cells (global; one GPU holds the domain) */
+    int na;               /* absorber thickness in cells (m_global.f90:166) */
+    int nm;               /* Zener bodies; the reference fixes NM = 3 (m_global.f90:31) */
+    int precision;        /* enum swpc_precision */
+    double dx, dy, dz;    /* grid widths [km] */
+    double dt;            /* time step [s] */
+    float ts[8];          /* relaxation times ts(1:nm) (m_fdtool.f90:691-727) */
+} swpc3d_desc;
+
+/* Replaces kernel__setup + memory_allocate (m_kernel.f90:30-73, 354-378)
+   and the absorber allocation of absorb_p__setup (m_absorb_p.f90:95-116). */
+int swpc3d_create(const swpc3d_desc *desc, swpc3d_ctx **out);
+int swpc3d_destroy(swpc3d_ctx *ctx);
+
+/* Medium arrays produced by medium__setup (m_medium.f90:36-467), all
+   real(SP) over the memory box (m_global.f90:39-42); replaces the
+   `!$acc enter data copyin` of main.f90:97-109. */
+int swpc3d_set_medium(swpc3d_ctx *ctx, const float *bx, const float *by, const float *bz,
+                      const float *lam, const float *mu, const float *taup, const float *taus,
+                      const float *muyz, const float *muxz, const float *muxy);
+
+/* Order-switch maps kfs_top/bot, kob_top/bot and kob over the (i,j)
+   memory box (m_medium.f90:342-397; used by m_kernel.f90:103-104 and
+   kernel__vmax m_kernel.f90:345). */
+int swpc3d_set_surface(swpc3d_ctx *ctx, const int *kfs_top, const int *kfs_bot,
+                       const int *kob_top, const int *kob_bot, const int *kob);
+
+/* Damping profiles g(4, 1:n) of absorb_p__setup (m_absorb_p.f90:75-93). */
+int swpc3d_set_pml(swpc3d_ctx *ctx, const float *gxc, const float *gxe, const float *gyc,
+                   const float *gye, const float *gzc, const float *gze);
+
+/* Stress-glut sources (source__setup m_source.f90:41-313): m6 =
+   (mxx,myy,mzz,myz,mxz,mxy) per source; sdrop[it-1][n] =
+   mo*momentrate(tbeg+(it-1/2)dt)*dt/(dx dy dz) for it = 1..nt, evaluated
+   on the host in FP64 (replaces the per-step evaluation at
+   m_source.f90:800-801). */
+int swpc3d_set_sources(swpc3d_ctx *ctx, int nsrc, const int *isrc, const int *jsrc, const int *ksrc,
+                       const double *m6, int nt, const double *sdrop);
+
+/* Body-force sources (m_source.f90:835-876): f3 = (fx,fy,fz)/M0 per
+   source; fdt[it-1][n] = momentrate(tbeg+it*dt)*dt/(dx dy dz)/2. */
+int swpc3d_set_bodyforce(swpc3d_ctx *ctx, int nsrc, const int *isrc, const int *jsrc, const int *ksrc,
+                         const double *f3, int nt, const double *fdt);
+
+/* Receivers of wav__setup/set_stinfo (m_wav.f90:54-274): 1-based cell
+   indices, decimation ntdec_w, ntw samples. */
+int swpc3d_set_receivers(swpc3d_ctx *ctx, int nst, const int *ist, const int *jst, const int *kst,
+                         int ntdec_w, int ntw);
+
+/* kernel__update_stress + absorb__update_stress (m_kernel.f90:142-326,
+   m_absorb_p.f90:313-505). */
+int swpc3d_update_stress(swpc3d_ctx *ctx);
+/* kernel__update_vel + absorb__update_vel (m_kernel.f90:75-140,
+   m_absorb_p.f90:126-311). */
+int swpc3d_update_vel(swpc3d_ctx *ctx);
+/* source__stressglut(it) (m_source.f90:775-833). */
+int swpc3d_stressglut(swpc3d_ctx *ctx, int it);
+/* source__bodyforce(it) (m_source.f90:835-876). */
+int swpc3d_bodyforce(swpc3d_ctx *ctx, int it);
+/* wav__store(it) for velocity and displacement (m_wav.f90:400-628);
+   displacement is accumulated in FP64. */
+int swpc3d_store_receivers(swpc3d_ctx *ctx, int it);
+/* One time step it in the order of main.f90:125-144: receivers,
+   stress (interior + PML), stress glut, velocity (interior + PML),
+   body force.  The halo exchange is absent: ghost cells stay in HBM. */
+int swpc3d_step(swpc3d_ctx *ctx, int it);
+/* Steps it0 .. it0+nsteps-1, enqueued without host synchronisation. */
+int swpc3d_run(swpc3d_ctx *ctx, int it0, int nsteps);
+
+/* Gathers (device -> host; these synchronise the context's stream).
+   vel[itw][n][3] are two-node averaged velocities in solver units
+   (z positive up), disp[itw][n][3] the FP64 displacement; either may be
+   NULL.  Multiply by M0*UC*1e9 for the reference's output units. */
+int swpc3d_get_receivers(swpc3d_ctx *ctx, float *vel, double *disp);
+int swpc3d_get_field(swpc3d_ctx *ctx, int field, double *out);
+int swpc3d_set_field(swpc3d_ctx *ctx, int field, const double *in);
+/* kernel__vmax (m_kernel.f90:328-352). */
+int swpc3d_vmax(swpc3d_ctx *ctx, float *out3);
+int swpc3d_sync(swpc3d_ctx *ctx);
+
+/* Per-kernel timing with HIP events recorded on the context's stream.
+   enable != 0 starts collection (and clears it); read returns, for each
+   kernel class (0 stress interior, 1 stress PML, 2 vel interior, 3 vel
+   PML, 4 other), the summed milliseconds and launch counts. */
+int swpc3d_profile(swpc3d_ctx *ctx, int enable);
+int swpc3d_profile_read(swpc3d_ctx *ctx, double *ms5, long long *count5);
+/* Device bytes held by the context. */
+int swpc3d_memory(swpc3d_ctx *ctx, long long *bytes);
+
+/* ------------------------------------------------------------------ */
+/* P-SV (swpc_psv)                                                     */
+/* ------------------------------------------------------------------ */
+
+typedef struct swpc2d_ctx swpc2d_ctx;
+
+typedef struct {
+    int nx, nz;
+    int na;
+    int nm;
+    int precision;
+    double dx, dz;
+    double dt;
+    float ts[8];
+} swpc2d_desc;
+
+int swpc2d_create(const swpc2d_desc *desc, swpc2d_ctx **out);
+int swpc2d_destroy(swpc2d_ctx *ctx);
+/* rho, lam, mu, taup, taus over the (k,i) memory box; the P-SV kernel
+   forms buoyancy and the harmonic mu_xz itself (src/swpc_psv/m_kernel.f90:118-119, 270-274). */
+int swpc2d_set_medium(swpc2d_ctx *ctx, const float *rho, const float *lam, const float *mu,
+                      const float *taup, const float *taus);
+/* maps over i in [-2:nx+3] */
+int swpc2d_set_surface(swpc2d_ctx *ctx, const int *kfs_top, const int *kfs_bot,
+                       const int *kob_top, const int *kob_bot, const int *kob);
+int swpc2d_set_pml(swpc2d_ctx *ctx, const float *gxc, const float *gxe, const float *gzc, const float *gze);
+/* m3 = (mxx, mzz, mxz); sdrop[it-1][n] = mo*momentrate(tbeg+(it-1/2)dt)*dt/(dx dz) */
+int swpc2d_set_sources(swpc2d_ctx *ctx, int nsrc, const int *isrc, const int *ksrc,
+                       const double *m3, int nt, const double *sdrop);
+/* f2 = (fx, fz)/M0; fdt[it-1][n] = momentrate(tbeg+it*dt)*dt/(dx dz)/2 */
+int swpc2d_set_bodyforce(swpc2d_ctx *ctx, int nsrc, const int *isrc, const int *ksrc,
+                         const double *f2, int nt, const double *fdt);
+int swpc2d_set_receivers(swpc2d_ctx *ctx, int nst, const int *ist, const int *kst, int ntdec_w, int ntw);
+int swpc2d_update_stress(swpc2d_ctx *ctx);
+int swpc2d_update_vel(swpc2d_ctx *ctx);
+int swpc2d_stressglut(swpc2d_ctx *ctx, int it);
+int swpc2d_bodyforce(swpc2d_ctx *ctx, int it);
+int swpc2d_store_receivers(swpc2d_ctx *ctx, int it);
+/* one step in the order of src/swpc_psv/main.f90:95-114 */
+int swpc2d_step(swpc2d_ctx *ctx, int it);
+int swpc2d_run(swpc2d_ctx *ctx, int it0, int nsteps);
+/* vel[itw][n][2], disp[itw][n][2] (x, z-up) */
+int swpc2d_get_receivers(swpc2d_ctx *ctx, float *vel, double *disp);
+int swpc2d_get_field(swpc2d_ctx *ctx, int field, double *out);
+int swpc2d_set_field(swpc2d_ctx *ctx, int field, const double *in);
+int swpc2d_vmax(swpc2d_ctx *ctx, float *out2);
+int swpc2d_sync(swpc2d_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SWPC_HIP_H */

@@ -1,0 +1,120 @@
+"""ctypes binding of the C ABI in include/swpc_hip.h (libswpc_hip.so).
+
+The library is built in-tree by ``openswpc_amd.build`` (or
+``__graft_entry__.build()``).  Loading fails loudly when it is missing:
+there is no CPU fallback on the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libswpc_hip.so"
+
+SWPC_FP32, SWPC_FP64, SWPC_FP32_S16 = 0, 1, 2
+
+P = C.c_void_p
+I = C.c_int
+D = C.c_double
+PI_ = C.POINTER(C.c_int)
+PF = C.POINTER(C.c_float)
+PD = C.POINTER(C.c_double)
+PLL = C.POINTER(C.c_longlong)
+
+
+class Desc3(C.Structure):
+    _fields_ = [("nx", I), ("ny", I), ("nz", I), ("na", I), ("nm", I), ("precision", I),
+                ("dx", D), ("dy", D), ("dz", D), ("dt", D), ("ts", C.c_float * 8)]
+
+
+class Desc2(C.Structure):
+    _fields_ = [("nx", I), ("nz", I), ("na", I), ("nm", I), ("precision", I),
+                ("dx", D), ("dz", D), ("dt", D), ("ts", C.c_float * 8)]
+
+
+# name -> (restype, argtypes); every entry point returns int status
+_PROTOS = {
+    "swpc_last_error": (C.c_char_p, []),
+    "swpc_version": (I, []),
+    "swpc_device_count": (I, []),
+    "swpc3d_create": (I, [C.POINTER(Desc3), C.POINTER(P)]),
+    "swpc3d_destroy": (I, [P]),
+    "swpc3d_set_medium": (I, [P] + [PF] * 10),
+    "swpc3d_set_surface": (I, [P] + [PI_] * 5),
+    "swpc3d_set_pml": (I, [P] + [PF] * 6),
+    "swpc3d_set_sources": (I, [P, I, PI_, PI_, PI_, PD, I, PD]),
+    "swpc3d_set_bodyforce": (I, [P, I, PI_, PI_, PI_, PD, I, PD]),
+    "swpc3d_set_receivers": (I, [P, I, PI_, PI_, PI_, I, I]),
+    "swpc3d_update_stress": (I, [P]),
+    "swpc3d_update_vel": (I, [P]),
+    "swpc3d_stressglut": (I, [P, I]),
+    "swpc3d_bodyforce": (I, [P, I]),
+    "swpc3d_store_receivers": (I, [P, I]),
+    "swpc3d_step": (I, [P, I]),
+    "swpc3d_run": (I, [P, I, I]),
+    "swpc3d_get_receivers": (I, [P, PF, PD]),
+    "swpc3d_get_field": (I, [P, I, PD]),
+    "swpc3d_set_field": (I, [P, I, PD]),
+    "swpc3d_vmax": (I, [P, PF]),
+    "swpc3d_sync": (I, [P]),
+    "swpc3d_profile": (I, [P, I]),
+    "swpc3d_profile_read": (I, [P, PD, PLL]),
+    "swpc3d_memory": (I, [P, PLL]),
+    "swpc2d_create": (I, [C.POINTER(Desc2), C.POINTER(P)]),
+    "swpc2d_destroy": (I, [P]),
+    "swpc2d_set_medium": (I, [P] + [PF] * 5),
+    "swpc2d_set_surface": (I, [P] + [PI_] * 5),
+    "swpc2d_set_pml": (I, [P] + [PF] * 4),
+    "swpc2d_set_sources": (I, [P, I, PI_, PI_, PD, I, PD]),
+    "swpc2d_set_bodyforce": (I, [P, I, PI_, PI_, PD, I, PD]),
+    "swpc2d_set_receivers": (I, [P, I, PI_, PI_, I, I]),
+    "swpc2d_update_stress": (I, [P]),
+    "swpc2d_update_vel": (I, [P]),
+    "swpc2d_stressglut": (I, [P, I]),
+    "swpc2d_bodyforce": (I, [P, I]),
+    "swpc2d_store_receivers": (I, [P, I]),
+    "swpc2d_step": (I, [P, I]),
+    "swpc2d_run": (I, [P, I, I]),
+    "swpc2d_get_receivers": (I, [P, PF, PD]),
+    "swpc2d_get_field": (I, [P, I, PD]),
+    "swpc2d_set_field": (I, [P, I, PD]),
+    "swpc2d_vmax": (I, [P, PF]),
+    "swpc2d_sync": (I, [P]),
+}
+
+_lib = None
+
+
+class SwpcError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libswpc_hip.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise SwpcError(f"{LIB_PATH} is missing: run `python -m openswpc_amd.build` "
+                            "(or __graft_entry__.build()) first; there is no CPU fallback")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _PROTOS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib().swpc_last_error().decode(errors="replace")
+        raise SwpcError(f"{what} failed ({rc}): {msg}")
+    return rc
+
+
+def ptr(a, ctype):
+    """Pointer to a contiguous numpy array (None passes NULL)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))

@@ -1,0 +1,874 @@
+// 2-D P-SV viscoelastic velocity-stress core (swpc_psv) for MI355X.
+//
+//   k2_stress_int  interior stress + memory variables, normal and shear
+//                  parts fused (src/swpc_psv/m_kernel.f90:139-306); the
+//                  harmonic mu_xz is formed in the kernel as in the reference
+//   k2_vel_int     interior velocity with in-kernel buoyancy
+//                  (src/swpc_psv/m_kernel.f90:76-137)
+//   k2_stress_pml / k2_vel_pml   ADE-CFS PML (src/swpc_psv/m_absorb_p.f90)
+//   k2_stressglut / k2_bodyforce / k2_receivers / k2_vmax
+//
+// One thread per cell in an x-fastest (k, i) layout: a P-SV grid has too
+// few columns to stream z per thread and still fill 256 CUs.
+#include "swpc_common.h"
+
+#include <algorithm>
+#include <memory>
+
+namespace swpc {
+
+constexpr int NM2 = 3;
+
+struct Box2 {
+    int i0, k0, ni, nk;
+    unsigned start;
+};
+
+struct Boxes2 {
+    Box2 b[3];
+    int n;
+    unsigned total;
+};
+
+template <typename T>
+struct K2 {
+    int px, nx, nz, na;
+    T *Vx, *Vz, *Sxx, *Szz, *Sxz;
+    T *R[9];    // Rxx(m) Rzz(m) Rxz(m)
+    T *A[8];    // axVx azVx axVz azVz axSxx azSxz axSxz azSzz
+    const float *rho, *lam, *mu, *taup, *taus;
+    const int4 *surf;   // per column i in [1, nx]
+    const T *gx, *gz;   // [8][n]
+    T e4[2][2], e2[2][2];
+    T r2[2];
+    T c1[NM2], c2[NM2], d1[NM2], d2;
+    T dt;
+    int ti0;
+    Boxes2 pml;
+};
+
+// Evaluated in IEEE single precision operation by operation, as the
+// reference's real(SP) expression: no contraction into FMA.
+__device__ inline float harm4(float nnn, float pnn, float npn, float ppn)
+{
+#pragma clang fp contract(off)
+    const float epsl = 1.1920928955078125e-07f;  // epsilon(1.0)
+    return 4 * nnn * pnn * npn * ppn / (nnn * pnn * npn + nnn * pnn * ppn + nnn * npn * ppn + pnn * npn * ppn + epsl);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k2_stress_int(const K2<T> a)
+{
+    const int i = a.ti0 + (int)(blockIdx.x * 64 + threadIdx.x);
+    const int k = 1 + (int)(blockIdx.y * 4 + threadIdx.y);
+    if (i <= a.na || i > a.nx - a.na || k > a.nz - a.na) return;
+    const int px = a.px;
+    const int c = at2(px, i, k);
+    const int4 fs = a.surf[i - 1];
+    const int o1 = (k - fs.x) * (fs.y - k), o2 = (k - fs.z) * (fs.w - k);
+    const bool lo = max(o1, o2) >= 0;
+    const T e0x = lo ? a.e2[0][0] : a.e4[0][0], e1x = lo ? a.e2[0][1] : a.e4[0][1];
+    const T e0z = lo ? a.e2[1][0] : a.e4[1][0], e1z = lo ? a.e2[1][1] : a.e4[1][1];
+    const T *__restrict__ Vx = a.Vx;
+    const T *__restrict__ Vz = a.Vz;
+    const T dt = a.dt;
+    const T vx0 = Vx[c], vz0 = Vz[c];
+
+    // normal (m_kernel.f90:196-222)
+    const T dxVx = (vx0 - Vx[c - 1]) * e0x - (Vx[c + 1] - Vx[c - 2]) * e1x;
+    const T dzVz = (vz0 - Vz[c - px]) * e0z - (Vz[c + px] - Vz[c - 2 * px]) * e1z;
+    const float mu00 = a.mu[c];
+    const T mu2 = 2 * (T)mu00;
+    const T lam2mu = (T)a.lam[c] + mu2;
+    const T taup1 = (T)a.taup[c], taus1 = (T)a.taus[c];
+    const T d2v2 = dxVx + dzVz;
+    const T f_Rxx = lam2mu * taup1 * d2v2 - mu2 * taus1 * dzVz;
+    const T f_Rzz = lam2mu * taup1 * d2v2 - mu2 * taus1 * dxVx;
+    T Rxx_n = 0, Rzz_n = 0;
+#pragma unroll
+    for (int m = 0; m < NM2; m++) {
+        T rxx = a.c1[m] * a.R[m][c] - a.c2[m] * f_Rxx * dt;
+        T rzz = a.c1[m] * a.R[3 + m][c] - a.c2[m] * f_Rzz * dt;
+        a.R[m][c] = rxx;
+        a.R[3 + m][c] = rzz;
+        Rxx_n += a.d1[m] * rxx;
+        Rzz_n += a.d1[m] * rzz;
+    }
+    const T taup_plus1 = 1 + taup1 * (1 + a.d2);
+    const T taus_plus1 = 1 + taus1 * (1 + a.d2);
+    a.Sxx[c] = a.Sxx[c] + (lam2mu * taup_plus1 * d2v2 - mu2 * taus_plus1 * dzVz + Rxx_n) * dt;
+    a.Szz[c] = a.Szz[c] + (lam2mu * taup_plus1 * d2v2 - mu2 * taus_plus1 * dxVx + Rzz_n) * dt;
+
+    // shear (m_kernel.f90:263-289)
+    const T dxVz = (Vz[c + 1] - vz0) * e0x - (Vz[c + 2] - Vz[c - 1]) * e1x;
+    const T dzVx = (Vx[c + px] - vx0) * e0z - (Vx[c + 2 * px] - Vx[c - px]) * e1z;
+    const T mu_xz = (T)harm4(mu00, a.mu[c + px], a.mu[c + 1], a.mu[c + px + 1]);
+    const T dxVz_dzVx = dxVz + dzVx;
+    const T f_Rxz = mu_xz * taus1 * dxVz_dzVx;
+    T Rxz_n = 0;
+#pragma unroll
+    for (int m = 0; m < NM2; m++) {
+        T rxz = a.c1[m] * a.R[6 + m][c] - a.c2[m] * f_Rxz * dt;
+        a.R[6 + m][c] = rxz;
+        Rxz_n += a.d1[m] * rxz;
+    }
+    a.Sxz[c] = a.Sxz[c] + (mu_xz * taus_plus1 * dxVz_dzVx + Rxz_n) * dt;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k2_vel_int(const K2<T> a)
+{
+    const int i = a.ti0 + (int)(blockIdx.x * 64 + threadIdx.x);
+    const int k = 1 + (int)(blockIdx.y * 4 + threadIdx.y);
+    if (i <= a.na || i > a.nx - a.na || k > a.nz - a.na) return;
+    const int px = a.px;
+    const int c = at2(px, i, k);
+    const int4 fs = a.surf[i - 1];
+    const int o1 = (k - fs.x) * (fs.y - k), o2 = (k - fs.z) * (fs.w - k);
+    const bool lo = max(o1, o2) >= 0;
+    const T e0x = lo ? a.e2[0][0] : a.e4[0][0], e1x = lo ? a.e2[0][1] : a.e4[0][1];
+    const T e0z = lo ? a.e2[1][0] : a.e4[1][0], e1z = lo ? a.e2[1][1] : a.e4[1][1];
+    const T *__restrict__ Sxx = a.Sxx;
+    const T *__restrict__ Szz = a.Szz;
+    const T *__restrict__ Sxz = a.Sxz;
+    // m_kernel.f90:110-123
+    const T sxz0 = Sxz[c];
+    const T dxSxx = (Sxx[c + 1] - Sxx[c]) * e0x - (Sxx[c + 2] - Sxx[c - 1]) * e1x;
+    const T dzSzz = (Szz[c + px] - Szz[c]) * e0z - (Szz[c + 2 * px] - Szz[c - px]) * e1z;
+    const T dxSxz = (sxz0 - Sxz[c - 1]) * e0x - (Sxz[c + 1] - Sxz[c - 2]) * e1x;
+    const T dzSxz = (sxz0 - Sxz[c - px]) * e0z - (Sxz[c + px] - Sxz[c - 2 * px]) * e1z;
+    const float r0 = a.rho[c];
+    const float bx = 2.0f / (r0 + a.rho[c + 1]);
+    const float bz = 2.0f / (r0 + a.rho[c + px]);
+    a.Vx[c] = a.Vx[c] + (T)bx * (dxSxx + dzSxz) * a.dt;
+    a.Vz[c] = a.Vz[c] + (T)bz * (dxSxz + dzSzz) * a.dt;
+}
+
+__device__ inline bool pml_cell2(const Boxes2 &B, unsigned id, int &i, int &k)
+{
+    if (id >= B.total) return false;
+    int b = 0;
+#pragma unroll
+    for (int q = 1; q < 3; q++)
+        if (q < B.n && id >= B.b[q].start) b = q;
+    const Box2 &x = B.b[b];
+    unsigned l = id - x.start;
+    i = x.i0 + (int)(l % (unsigned)x.ni);
+    k = x.k0 + (int)(l / (unsigned)x.ni);
+    return true;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k2_stress_pml(const K2<T> a)
+{
+    int i, k;
+    if (!pml_cell2(a.pml, blockIdx.x * 256u + threadIdx.x, i, k)) return;
+    const int px = a.px, nx = a.nx, nz = a.nz;
+    const int c = at2(px, i, k);
+    const T *__restrict__ Vx = a.Vx;
+    const T *__restrict__ Vz = a.Vz;
+    const T gxc1 = a.gx[0 * nx + i - 1], gxc2 = a.gx[1 * nx + i - 1], gxc3 = a.gx[2 * nx + i - 1], gxc4 = a.gx[3 * nx + i - 1];
+    const T gxe1 = a.gx[4 * nx + i - 1], gxe2 = a.gx[5 * nx + i - 1], gxe3 = a.gx[6 * nx + i - 1], gxe4 = a.gx[7 * nx + i - 1];
+    const T gzc1 = a.gz[0 * nz + k - 1], gzc2 = a.gz[1 * nz + k - 1], gzc3 = a.gz[2 * nz + k - 1], gzc4 = a.gz[3 * nz + k - 1];
+    const T gze1 = a.gz[4 * nz + k - 1], gze2 = a.gz[5 * nz + k - 1], gze3 = a.gz[6 * nz + k - 1], gze4 = a.gz[7 * nz + k - 1];
+    const T r20x = a.r2[0], r20z = a.r2[1], dt = a.dt;
+    const T vx0 = Vx[c], vz0 = Vz[c];
+    // src/swpc_psv/m_absorb_p.f90:347-384
+    const T dxVx = (vx0 - Vx[c - 1]) * r20x;
+    const T dzVz = (vz0 - Vz[c - px]) * r20z;
+    const float mu00 = a.mu[c];
+    const T lam2mu_R = (T)(a.lam[c] + 2 * mu00);
+    const T lam_R = lam2mu_R - 2 * (T)mu00;
+    const T axvx = a.A[0][c], azvz = a.A[3][c];
+    const T dxVx_ade = gxc1 * dxVx + gxc2 * axvx;
+    const T dzVz_ade = gzc1 * dzVz + gzc2 * azvz;
+    a.Sxx[c] = a.Sxx[c] + (lam2mu_R * dxVx_ade + lam_R * dzVz_ade) * dt;
+    a.Szz[c] = a.Szz[c] + (lam2mu_R * dzVz_ade + lam_R * dxVx_ade) * dt;
+    a.A[0][c] = gxc3 * axvx + gxc4 * dxVx * dt;
+    a.A[3][c] = gzc3 * azvz + gzc4 * dzVz * dt;
+    // shear (m_absorb_p.f90:453-476)
+    const T dzVx = (Vx[c + px] - vx0) * r20z;
+    const T dxVz = (Vz[c + 1] - vz0) * r20x;
+    const T muxz = (T)harm4(mu00, a.mu[c + px], a.mu[c + 1], a.mu[c + px + 1]);
+    const T azvx = a.A[1][c], axvz = a.A[2][c];
+    a.Sxz[c] = a.Sxz[c] + muxz * (gxe1 * dxVz + gze1 * dzVx + gxe2 * axvz + gze2 * azvx) * dt;
+    a.A[1][c] = gze3 * azvx + gze4 * dzVx * dt;
+    a.A[2][c] = gxe3 * axvz + gxe4 * dxVz * dt;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k2_vel_pml(const K2<T> a)
+{
+    int i, k;
+    if (!pml_cell2(a.pml, blockIdx.x * 256u + threadIdx.x, i, k)) return;
+    const int px = a.px, nx = a.nx, nz = a.nz;
+    const int c = at2(px, i, k);
+    const T gxc1 = a.gx[0 * nx + i - 1], gxc2 = a.gx[1 * nx + i - 1], gxc3 = a.gx[2 * nx + i - 1], gxc4 = a.gx[3 * nx + i - 1];
+    const T gxe1 = a.gx[4 * nx + i - 1], gxe2 = a.gx[5 * nx + i - 1], gxe3 = a.gx[6 * nx + i - 1], gxe4 = a.gx[7 * nx + i - 1];
+    const T gzc1 = a.gz[0 * nz + k - 1], gzc2 = a.gz[1 * nz + k - 1], gzc3 = a.gz[2 * nz + k - 1], gzc4 = a.gz[3 * nz + k - 1];
+    const T gze1 = a.gz[4 * nz + k - 1], gze2 = a.gz[5 * nz + k - 1], gze3 = a.gz[6 * nz + k - 1], gze4 = a.gz[7 * nz + k - 1];
+    const T r20x = a.r2[0], r20z = a.r2[1], dt = a.dt;
+    // src/swpc_psv/m_absorb_p.f90:177-201
+    const T sxx0 = a.Sxx[c], szz0 = a.Szz[c], sxz0 = a.Sxz[c];
+    const T dxSxx = (a.Sxx[c + 1] - sxx0) * r20x;
+    const T dzSzz = (a.Szz[c + px] - szz0) * r20z;
+    const T dxSxz = (sxz0 - a.Sxz[c - 1]) * r20x;
+    const T dzSxz = (sxz0 - a.Sxz[c - px]) * r20z;
+    const float r0 = a.rho[c];
+    const float bx = 2.0f / (r0 + a.rho[c + 1]);
+    const float bz = 2.0f / (r0 + a.rho[c + px]);
+    const T axsxx = a.A[4][c], azsxz = a.A[5][c], axsxz = a.A[6][c], azszz = a.A[7][c];
+    a.Vx[c] = a.Vx[c] + (T)bx * (gxe1 * dxSxx + gzc1 * dzSxz + gxe2 * axsxx + gzc2 * azsxz) * dt;
+    a.Vz[c] = a.Vz[c] + (T)bz * (gxc1 * dxSxz + gze1 * dzSzz + gxc2 * axsxz + gze2 * azszz) * dt;
+    a.A[4][c] = gxe3 * axsxx + gxe4 * dxSxx * dt;
+    a.A[5][c] = gzc3 * azsxz + gzc4 * dzSxz * dt;
+    a.A[6][c] = gxc3 * axsxz + gxc4 * dxSxz * dt;
+    a.A[7][c] = gze3 * azszz + gze4 * dzSzz * dt;
+}
+
+template <typename T>
+__global__ void k2_stressglut(const K2<T> a, int nsrc, const int *__restrict__ src, const double *__restrict__ m3,
+                              const double *__restrict__ sdrop)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int px = a.px;
+    for (int n = 0; n < nsrc; n++) {
+        const int c = at2(px, src[2 * n], src[2 * n + 1]);
+        const T sd = (T)sdrop[n];
+        const T mxx = (T)m3[3 * n], mzz = (T)m3[3 * n + 1], mxz = (T)m3[3 * n + 2];
+        a.Sxx[c] -= mxx * sd;
+        a.Szz[c] -= mzz * sd;
+        a.Sxz[c] -= mxz * sd / 4;
+        a.Sxz[c - px] -= mxz * sd / 4;
+        a.Sxz[c - 1] -= mxz * sd / 4;
+        a.Sxz[c - 1 - px] -= mxz * sd / 4;
+    }
+}
+
+template <typename T>
+__global__ void k2_bodyforce(const K2<T> a, int nsrc, const int *__restrict__ src, const double *__restrict__ f2,
+                             const double *__restrict__ fdt)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int px = a.px;
+    for (int n = 0; n < nsrc; n++) {
+        const int c = at2(px, src[2 * n], src[2 * n + 1]);
+        const T f = (T)fdt[n] / (T)a.rho[c];
+        const T fx = (T)f2[2 * n], fz = (T)f2[2 * n + 1];
+        a.Vx[c] += fx * f;
+        a.Vx[c - 1] += fx * f;
+        a.Vz[c] += fz * f;
+        a.Vz[c - px] += fz * f;
+    }
+}
+
+template <typename T>
+__global__ void k2_receivers(const K2<T> a, int nst, const int *__restrict__ st, double *__restrict__ uacc,
+                             float *__restrict__ vel_out, double *__restrict__ disp_out, int store)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nst) return;
+    const int px = a.px;
+    const int c = at2(px, st[2 * n], st[2 * n + 1]);
+    const T vx = a.Vx[c] + a.Vx[c - 1];
+    const T vz = a.Vz[c] + a.Vz[c - px];
+    const double hdt = 0.5 * (double)a.dt;
+    const double ux = uacc[2 * n] + (double)vx * hdt;
+    const double uz = uacc[2 * n + 1] - (double)vz * hdt;
+    uacc[2 * n] = ux;
+    uacc[2 * n + 1] = uz;
+    if (store) {
+        vel_out[2 * n] = (float)(vx / 2);
+        vel_out[2 * n + 1] = (float)(-vz / 2);
+        disp_out[2 * n] = ux;
+        disp_out[2 * n + 1] = uz;
+    }
+}
+
+__global__ void k2_fill_u32(unsigned *p, int n, unsigned v)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+template <typename T>
+__global__ void k2_vmax(const K2<T> a, const int *__restrict__ kob, unsigned *__restrict__ out)
+{
+    const int i = a.na + 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    float m0 = 0, m1 = 0;
+    if (i <= a.nx - a.na) {
+        const int c = at2(a.px, i, kob[i - 1] + 1);
+        m0 = fabsf((float)a.Vx[c]);
+        m1 = fabsf((float)a.Vz[c]);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        m0 = fmaxf(m0, __shfl_xor(m0, off));
+        m1 = fmaxf(m1, __shfl_xor(m1, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(out + 0, __float_as_uint(m0));
+        atomicMax(out + 1, __float_as_uint(m1));
+    }
+}
+
+}  // namespace swpc
+
+struct swpc2d_ctx {
+    swpc2d_desc d;
+    swpc::Lay2 L;
+    size_t esz;
+    hipStream_t stream = nullptr;
+    long long bytes = 0;
+    swpc::DBuf V[2], S[3], R[9], A[8];
+    swpc::DBuf med[5];  // rho lam mu taup taus
+    swpc::DBuf surf, kob, gx, gz;
+    swpc::DBuf src, m3, sdrop, bsrc, f2, fdt;
+    swpc::DBuf st, uacc, wvel, wdisp, vmax;
+    int nsrc = 0, nt_src = 0, nbf = 0, nt_bf = 0;
+    int nst = 0, ntdec_w = 1, ntw = 0;
+    bool have_medium = false, have_surface = false, have_pml = false;
+    swpc::Boxes2 boxes;
+    int ti0 = 1, ntx = 0, ntz = 0;
+};
+
+namespace swpc {
+
+template <typename T>
+static K2<T> make_args2(swpc2d_ctx *c)
+{
+    K2<T> a;
+    memset(&a, 0, sizeof a);
+    const swpc2d_desc &d = c->d;
+    a.px = c->L.px; a.nx = d.nx; a.nz = d.nz; a.na = d.na;
+    a.Vx = (T *)c->V[0].p; a.Vz = (T *)c->V[1].p;
+    a.Sxx = (T *)c->S[0].p; a.Szz = (T *)c->S[1].p; a.Sxz = (T *)c->S[2].p;
+    for (int q = 0; q < 9; q++) a.R[q] = (T *)c->R[q].p;
+    for (int q = 0; q < 8; q++) a.A[q] = (T *)c->A[q].p;
+    a.rho = (const float *)c->med[0].p; a.lam = (const float *)c->med[1].p; a.mu = (const float *)c->med[2].p;
+    a.taup = (const float *)c->med[3].p; a.taus = (const float *)c->med[4].p;
+    a.surf = (const int4 *)c->surf.p;
+    a.gx = (const T *)c->gx.p; a.gz = (const T *)c->gz.p;
+    const double h[2] = {d.dx, d.dz};
+    for (int q = 0; q < 2; q++) {
+        T hh = (T)h[q];
+        T rc40 = (T)17.0 / (T)16.0 / hh, rc41 = (T)1.0 / (T)48.0 / hh;
+        T rd40 = (T)(-1.0) / (T)16.0 / hh, rd41 = (T)(-1.0) / (T)48.0 / hh;
+        a.e4[q][0] = rc40 - rd40; a.e4[q][1] = rc41 - rd41;
+        a.e2[q][0] = rc40 + rd40; a.e2[q][1] = rc41 + rd41;
+    }
+    // r20 is real(SP) in the P-SV absorber (src/swpc_psv/m_absorb_p.f90:69-71)
+    a.r2[0] = (T)(1.0 / d.dx);
+    a.r2[1] = (T)(1.0 / d.dz);
+    T t = (T)(float)d.dt, s = 0;
+    for (int m = 0; m < NM2; m++) {
+        T tm = (T)d.ts[m];
+        a.c1[m] = (2 * tm - t) / (2 * tm + t);
+        a.c2[m] = (T)2 / (2 * tm + t) / NM2;
+        s += t / (2 * tm - t);
+        a.d1[m] = 2 * tm / (2 * tm - t);
+    }
+    a.d2 = s / NM2;
+    a.dt = t;
+    a.ti0 = c->ti0;
+    a.pml = c->boxes;
+    return a;
+}
+
+static bool fp64(const swpc2d_ctx *c) { return c->d.precision == SWPC_FP64; }
+
+static int ready2(const swpc2d_ctx *c)
+{
+    if (!c) return fail(SWPC_EINVAL, "null context");
+    if (!c->have_medium || !c->have_surface || !c->have_pml)
+        return fail(SWPC_ESTATE, "set_medium, set_surface and set_pml must precede time stepping");
+    return SWPC_OK;
+}
+
+template <typename T>
+static int l2_stress(swpc2d_ctx *c)
+{
+    K2<T> a = make_args2<T>(c);
+    if (c->ntx > 0 && c->ntz > 0) {
+        hipLaunchKernelGGL((k2_stress_int<T>), dim3(c->ntx, c->ntz), dim3(64, 4), 0, c->stream, a);
+        SWPC_CHECK_LAUNCH();
+    }
+    if (c->boxes.total > 0) {
+        hipLaunchKernelGGL((k2_stress_pml<T>), dim3((c->boxes.total + 255) / 256), dim3(256), 0, c->stream, a);
+        SWPC_CHECK_LAUNCH();
+    }
+    return SWPC_OK;
+}
+
+template <typename T>
+static int l2_vel(swpc2d_ctx *c)
+{
+    K2<T> a = make_args2<T>(c);
+    if (c->ntx > 0 && c->ntz > 0) {
+        hipLaunchKernelGGL((k2_vel_int<T>), dim3(c->ntx, c->ntz), dim3(64, 4), 0, c->stream, a);
+        SWPC_CHECK_LAUNCH();
+    }
+    if (c->boxes.total > 0) {
+        hipLaunchKernelGGL((k2_vel_pml<T>), dim3((c->boxes.total + 255) / 256), dim3(256), 0, c->stream, a);
+        SWPC_CHECK_LAUNCH();
+    }
+    return SWPC_OK;
+}
+
+template <typename T>
+static int l2_glut(swpc2d_ctx *c, int it)
+{
+    if (c->nsrc == 0) return SWPC_OK;
+    if (it < 1 || it > c->nt_src) return fail(SWPC_EINVAL, "stressglut: it=%d outside 1..%d", it, c->nt_src);
+    K2<T> a = make_args2<T>(c);
+    hipLaunchKernelGGL((k2_stressglut<T>), dim3(1), dim3(64), 0, c->stream, a, c->nsrc, (const int *)c->src.p,
+                       (const double *)c->m3.p, (const double *)c->sdrop.p + (size_t)(it - 1) * c->nsrc);
+    SWPC_CHECK_LAUNCH();
+    return SWPC_OK;
+}
+
+template <typename T>
+static int l2_bf(swpc2d_ctx *c, int it)
+{
+    if (c->nbf == 0) return SWPC_OK;
+    if (it < 1 || it > c->nt_bf) return fail(SWPC_EINVAL, "bodyforce: it=%d outside 1..%d", it, c->nt_bf);
+    K2<T> a = make_args2<T>(c);
+    hipLaunchKernelGGL((k2_bodyforce<T>), dim3(1), dim3(64), 0, c->stream, a, c->nbf, (const int *)c->bsrc.p,
+                       (const double *)c->f2.p, (const double *)c->fdt.p + (size_t)(it - 1) * c->nbf);
+    SWPC_CHECK_LAUNCH();
+    return SWPC_OK;
+}
+
+template <typename T>
+static int l2_recv(swpc2d_ctx *c, int it)
+{
+    if (c->nst == 0) return SWPC_OK;
+    int store = ((it - 1) % c->ntdec_w) == 0;
+    int itw = (it - 1) / c->ntdec_w;
+    if (store && itw >= c->ntw) store = 0;
+    K2<T> a = make_args2<T>(c);
+    size_t off = (size_t)(store ? itw : 0) * c->nst * 2;
+    hipLaunchKernelGGL((k2_receivers<T>), dim3((c->nst + 63) / 64), dim3(64), 0, c->stream, a, c->nst,
+                       (const int *)c->st.p, (double *)c->uacc.p, (float *)c->wvel.p + off,
+                       (double *)c->wdisp.p + off, store);
+    SWPC_CHECK_LAUNCH();
+    return SWPC_OK;
+}
+
+// P-SV step order (src/swpc_psv/main.f90:95-114): receivers, stress,
+// absorber, stress glut, velocity, body force, absorber.  The additive
+// body force commutes with the absorber update.
+static int step2(swpc2d_ctx *c, int it)
+{
+    int r;
+    if (fp64(c)) {
+        if ((r = l2_recv<double>(c, it))) return r;
+        if ((r = l2_stress<double>(c))) return r;
+        if ((r = l2_glut<double>(c, it))) return r;
+        if ((r = l2_vel<double>(c))) return r;
+        if ((r = l2_bf<double>(c, it))) return r;
+    } else {
+        if ((r = l2_recv<float>(c, it))) return r;
+        if ((r = l2_stress<float>(c))) return r;
+        if ((r = l2_glut<float>(c, it))) return r;
+        if ((r = l2_vel<float>(c))) return r;
+        if ((r = l2_bf<float>(c, it))) return r;
+    }
+    return SWPC_OK;
+}
+
+template <typename S, typename D>
+static void box2_to_dev(const Lay2 &L, const S *src, D *dst)
+{
+    const int nkb = L.nz + 6;
+    for (int k = -2; k <= L.nz + 3; k++)
+        for (int i = -2; i <= L.nx + 3; i++) dst[at2(L.px, i, k)] = (D)src[(size_t)(k + 2) + (size_t)nkb * (size_t)(i + 2)];
+}
+
+template <typename S, typename D>
+static void dev_to_box2(const Lay2 &L, const S *src, D *dst)
+{
+    const int nkb = L.nz + 6;
+    for (int k = -2; k <= L.nz + 3; k++)
+        for (int i = -2; i <= L.nx + 3; i++) dst[(size_t)(k + 2) + (size_t)nkb * (size_t)(i + 2)] = (D)src[at2(L.px, i, k)];
+}
+
+static DBuf *field_buf2(swpc2d_ctx *c, int f)
+{
+    if (f >= SWPC2D_VX && f <= SWPC2D_VZ) return &c->V[f];
+    if (f >= SWPC2D_SXX && f <= SWPC2D_SXZ) return &c->S[f - SWPC2D_SXX];
+    if (f >= SWPC2D_R0 && f < SWPC2D_ADE0) return &c->R[f - SWPC2D_R0];
+    if (f >= SWPC2D_ADE0 && f < SWPC2D_NFIELD) return &c->A[f - SWPC2D_ADE0];
+    return nullptr;
+}
+
+static int upload_profile2(swpc2d_ctx *c, DBuf &dst, const float *gc, const float *ge, int n)
+{
+    std::vector<double> h((size_t)8 * n);
+    for (int q = 0; q < 4; q++)
+        for (int i = 0; i < n; i++) {
+            h[(size_t)q * n + i] = gc[q + 4 * i];
+            h[(size_t)(4 + q) * n + i] = ge[q + 4 * i];
+        }
+    if (int r = dalloc(dst, (size_t)8 * n * c->esz, c->bytes)) return r;
+    if (fp64(c)) {
+        SWPC_HIP(hipMemcpy(dst.p, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(h.begin(), h.end());
+        SWPC_HIP(hipMemcpy(dst.p, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+    }
+    return SWPC_OK;
+}
+
+}  // namespace swpc
+
+using namespace swpc;
+
+extern "C" {
+
+int swpc2d_create(const swpc2d_desc *desc, swpc2d_ctx **out)
+{
+    clear_error();
+    if (!desc || !out) return fail(SWPC_EINVAL, "null argument");
+    *out = nullptr;
+    const swpc2d_desc &d = *desc;
+    if (d.nx < 8 || d.nz < 8) return fail(SWPC_EINVAL, "grid %dx%d too small", d.nx, d.nz);
+    if (d.na < 0 || 2 * d.na >= d.nx || d.na >= d.nz) return fail(SWPC_EINVAL, "absorber thickness na=%d does not fit", d.na);
+    if (d.nm != NM2) return fail(SWPC_EINVAL, "nm=%d: the kernels are built for NM = 3", d.nm);
+    if (d.precision != SWPC_FP32 && d.precision != SWPC_FP64)
+        return fail(SWPC_EINVAL, "precision mode %d not supported by this build", d.precision);
+    if (!(d.dx > 0 && d.dz > 0 && d.dt > 0)) return fail(SWPC_EINVAL, "non-positive dx/dz/dt");
+    for (int m = 0; m < d.nm; m++)
+        if (!(d.ts[m] > 0)) return fail(SWPC_EINVAL, "ts(%d) must be positive", m + 1);
+    Lay2 L = make_lay2(d.nx, d.nz);
+    if (L.vol >= (1LL << 31)) return fail(SWPC_EINVAL, "grid too large");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(SWPC_EHIP, "no HIP device available");
+    std::unique_ptr<swpc2d_ctx> c(new swpc2d_ctx());
+    c->d = d;
+    c->L = L;
+    c->esz = d.precision == SWPC_FP64 ? 8 : 4;
+    SWPC_HIP(hipStreamCreate(&c->stream));
+    const size_t vb = (size_t)L.vol * c->esz;
+    int r;
+    for (auto &b : c->V) if ((r = dalloc(b, vb, c->bytes))) return r;
+    for (auto &b : c->S) if ((r = dalloc(b, vb, c->bytes))) return r;
+    for (auto &b : c->R) if ((r = dalloc(b, vb, c->bytes))) return r;
+    for (auto &b : c->A) if ((r = dalloc(b, vb, c->bytes))) return r;
+    for (auto &b : c->med) if ((r = dalloc(b, (size_t)L.vol * 4, c->bytes))) return r;
+    if ((r = dalloc(c->surf, (size_t)d.nx * 16, c->bytes))) return r;
+    if ((r = dalloc(c->kob, (size_t)d.nx * 4, c->bytes))) return r;
+    if ((r = dalloc(c->vmax, 64, c->bytes))) return r;
+    const int ilo = d.na + 1, ihi = d.nx - d.na;
+    c->ti0 = 1 + ((ilo - 1) / 64) * 64;
+    c->ntx = ihi >= ilo ? (ihi - c->ti0) / 64 + 1 : 0;
+    const int nk = d.nz - d.na;
+    c->ntz = nk > 0 ? (nk + 3) / 4 : 0;
+    Boxes2 &B = c->boxes;
+    memset(&B, 0, sizeof B);
+    auto add = [&](int i0, int i1, int k0, int k1) {
+        int ni = i1 - i0 + 1, nk = k1 - k0 + 1;
+        if (ni <= 0 || nk <= 0) return;
+        Box2 &x = B.b[B.n++];
+        x.i0 = i0; x.k0 = k0; x.ni = ni; x.nk = nk;
+        x.start = B.total;
+        B.total += (unsigned)ni * (unsigned)nk;
+    };
+    if (d.na > 0) {
+        add(1, d.na, 1, d.nz);
+        add(d.nx - d.na + 1, d.nx, 1, d.nz);
+        add(d.na + 1, d.nx - d.na, d.nz - d.na + 1, d.nz);
+    }
+    *out = c.release();
+    return SWPC_OK;
+}
+
+int swpc2d_destroy(swpc2d_ctx *c)
+{
+    if (!c) return SWPC_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &b : c->V) dfree(b);
+    for (auto &b : c->S) dfree(b);
+    for (auto &b : c->R) dfree(b);
+    for (auto &b : c->A) dfree(b);
+    for (auto &b : c->med) dfree(b);
+    DBuf *rest[] = {&c->surf, &c->kob, &c->gx, &c->gz, &c->src, &c->m3, &c->sdrop, &c->bsrc, &c->f2,
+                    &c->fdt, &c->st, &c->uacc, &c->wvel, &c->wdisp, &c->vmax};
+    for (auto b : rest) dfree(*b);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SWPC_OK;
+}
+
+int swpc2d_set_medium(swpc2d_ctx *c, const float *rho, const float *lam, const float *mu, const float *taup,
+                      const float *taus)
+{
+    clear_error();
+    if (!c || !rho || !lam || !mu || !taup || !taus) return fail(SWPC_EINVAL, "null argument");
+    const float *in[5] = {rho, lam, mu, taup, taus};
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    std::vector<float> h((size_t)c->L.vol, 0.0f);
+    for (int q = 0; q < 5; q++) {
+        box2_to_dev(c->L, in[q], h.data());
+        SWPC_HIP(hipMemcpy(c->med[q].p, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+    }
+    c->have_medium = true;
+    return SWPC_OK;
+}
+
+int swpc2d_set_surface(swpc2d_ctx *c, const int *kfs_top, const int *kfs_bot, const int *kob_top, const int *kob_bot,
+                       const int *kob)
+{
+    clear_error();
+    if (!c || !kfs_top || !kfs_bot || !kob_top || !kob_bot || !kob) return fail(SWPC_EINVAL, "null argument");
+    const int nx = c->d.nx;
+    std::vector<int> s4((size_t)nx * 4), kb(nx);
+    for (int i = 1; i <= nx; i++) {
+        size_t h = (size_t)(i + 2);
+        s4[4 * (i - 1) + 0] = kfs_top[h];
+        s4[4 * (i - 1) + 1] = kfs_bot[h];
+        s4[4 * (i - 1) + 2] = kob_top[h];
+        s4[4 * (i - 1) + 3] = kob_bot[h];
+        kb[i - 1] = kob[h];
+        if (kob[h] + 1 < -2 || kob[h] + 1 > c->d.nz + 3) return fail(SWPC_EINVAL, "kob(%d)=%d outside the memory box", i, kob[h]);
+    }
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    SWPC_HIP(hipMemcpy(c->surf.p, s4.data(), s4.size() * 4, hipMemcpyHostToDevice));
+    SWPC_HIP(hipMemcpy(c->kob.p, kb.data(), kb.size() * 4, hipMemcpyHostToDevice));
+    c->have_surface = true;
+    return SWPC_OK;
+}
+
+int swpc2d_set_pml(swpc2d_ctx *c, const float *gxc, const float *gxe, const float *gzc, const float *gze)
+{
+    clear_error();
+    if (!c || !gxc || !gxe || !gzc || !gze) return fail(SWPC_EINVAL, "null argument");
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    int r;
+    if ((r = upload_profile2(c, c->gx, gxc, gxe, c->d.nx))) return r;
+    if ((r = upload_profile2(c, c->gz, gzc, gze, c->d.nz))) return r;
+    c->have_pml = true;
+    return SWPC_OK;
+}
+
+static int pack_src2(swpc2d_ctx *c, int nsrc, const int *is, const int *ks, std::vector<int> &pack)
+{
+    pack.resize((size_t)2 * nsrc);
+    for (int n = 0; n < nsrc; n++) {
+        if (is[n] < -1 || is[n] > c->d.nx + 3 || ks[n] < -1 || ks[n] > c->d.nz + 3)
+            return fail(SWPC_EINVAL, "source %d at (%d,%d) outside the memory box", n + 1, is[n], ks[n]);
+        pack[2 * n] = is[n];
+        pack[2 * n + 1] = ks[n];
+    }
+    return SWPC_OK;
+}
+
+int swpc2d_set_sources(swpc2d_ctx *c, int nsrc, const int *isrc, const int *ksrc, const double *m3, int nt,
+                       const double *sdrop)
+{
+    clear_error();
+    if (!c || nsrc < 0 || nt < 0) return fail(SWPC_EINVAL, "bad argument");
+    if (nsrc > 0 && (!isrc || !ksrc || !m3 || !sdrop || nt < 1)) return fail(SWPC_EINVAL, "null source array");
+    std::vector<int> pack;
+    if (int r = pack_src2(c, nsrc, isrc, ksrc, pack)) return r;
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    int r;
+    if ((r = dalloc(c->src, pack.size() * 4, c->bytes))) return r;
+    if ((r = dalloc(c->m3, (size_t)3 * nsrc * 8, c->bytes))) return r;
+    if ((r = dalloc(c->sdrop, (size_t)nt * nsrc * 8, c->bytes))) return r;
+    if (nsrc > 0) {
+        SWPC_HIP(hipMemcpy(c->src.p, pack.data(), pack.size() * 4, hipMemcpyHostToDevice));
+        SWPC_HIP(hipMemcpy(c->m3.p, m3, (size_t)3 * nsrc * 8, hipMemcpyHostToDevice));
+        SWPC_HIP(hipMemcpy(c->sdrop.p, sdrop, (size_t)nt * nsrc * 8, hipMemcpyHostToDevice));
+    }
+    c->nsrc = nsrc;
+    c->nt_src = nt;
+    return SWPC_OK;
+}
+
+int swpc2d_set_bodyforce(swpc2d_ctx *c, int nsrc, const int *isrc, const int *ksrc, const double *f2, int nt,
+                         const double *fdt)
+{
+    clear_error();
+    if (!c || nsrc < 0 || nt < 0) return fail(SWPC_EINVAL, "bad argument");
+    if (nsrc > 0 && (!isrc || !ksrc || !f2 || !fdt || nt < 1)) return fail(SWPC_EINVAL, "null source array");
+    std::vector<int> pack;
+    if (int r = pack_src2(c, nsrc, isrc, ksrc, pack)) return r;
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    int r;
+    if ((r = dalloc(c->bsrc, pack.size() * 4, c->bytes))) return r;
+    if ((r = dalloc(c->f2, (size_t)2 * nsrc * 8, c->bytes))) return r;
+    if ((r = dalloc(c->fdt, (size_t)nt * nsrc * 8, c->bytes))) return r;
+    if (nsrc > 0) {
+        SWPC_HIP(hipMemcpy(c->bsrc.p, pack.data(), pack.size() * 4, hipMemcpyHostToDevice));
+        SWPC_HIP(hipMemcpy(c->f2.p, f2, (size_t)2 * nsrc * 8, hipMemcpyHostToDevice));
+        SWPC_HIP(hipMemcpy(c->fdt.p, fdt, (size_t)nt * nsrc * 8, hipMemcpyHostToDevice));
+    }
+    c->nbf = nsrc;
+    c->nt_bf = nt;
+    return SWPC_OK;
+}
+
+int swpc2d_set_receivers(swpc2d_ctx *c, int nst, const int *ist, const int *kst, int ntdec_w, int ntw)
+{
+    clear_error();
+    if (!c || nst < 0 || ntdec_w < 1 || ntw < 0) return fail(SWPC_EINVAL, "bad argument");
+    if (nst > 0 && (!ist || !kst)) return fail(SWPC_EINVAL, "null station array");
+    std::vector<int> pack((size_t)2 * nst);
+    for (int n = 0; n < nst; n++) {
+        if (ist[n] < 1 || ist[n] > c->d.nx || kst[n] < 1 || kst[n] > c->d.nz)
+            return fail(SWPC_EINVAL, "station %d at (%d,%d) outside the grid", n + 1, ist[n], kst[n]);
+        pack[2 * n] = ist[n];
+        pack[2 * n + 1] = kst[n];
+    }
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    int r;
+    if ((r = dalloc(c->st, pack.size() * 4, c->bytes))) return r;
+    if ((r = dalloc(c->uacc, (size_t)2 * nst * 8, c->bytes))) return r;
+    if ((r = dalloc(c->wvel, (size_t)2 * nst * std::max(ntw, 1) * 4, c->bytes))) return r;
+    if ((r = dalloc(c->wdisp, (size_t)2 * nst * std::max(ntw, 1) * 8, c->bytes))) return r;
+    if (nst > 0) SWPC_HIP(hipMemcpy(c->st.p, pack.data(), pack.size() * 4, hipMemcpyHostToDevice));
+    c->nst = nst;
+    c->ntdec_w = ntdec_w;
+    c->ntw = ntw;
+    return SWPC_OK;
+}
+
+int swpc2d_update_stress(swpc2d_ctx *c)
+{
+    clear_error();
+    if (int r = ready2(c)) return r;
+    return fp64(c) ? l2_stress<double>(c) : l2_stress<float>(c);
+}
+
+int swpc2d_update_vel(swpc2d_ctx *c)
+{
+    clear_error();
+    if (int r = ready2(c)) return r;
+    return fp64(c) ? l2_vel<double>(c) : l2_vel<float>(c);
+}
+
+int swpc2d_stressglut(swpc2d_ctx *c, int it)
+{
+    clear_error();
+    if (int r = ready2(c)) return r;
+    return fp64(c) ? l2_glut<double>(c, it) : l2_glut<float>(c, it);
+}
+
+int swpc2d_bodyforce(swpc2d_ctx *c, int it)
+{
+    clear_error();
+    if (int r = ready2(c)) return r;
+    return fp64(c) ? l2_bf<double>(c, it) : l2_bf<float>(c, it);
+}
+
+int swpc2d_store_receivers(swpc2d_ctx *c, int it)
+{
+    clear_error();
+    if (int r = ready2(c)) return r;
+    if (it < 1) return fail(SWPC_EINVAL, "it must be >= 1");
+    return fp64(c) ? l2_recv<double>(c, it) : l2_recv<float>(c, it);
+}
+
+int swpc2d_step(swpc2d_ctx *c, int it)
+{
+    clear_error();
+    if (int r = ready2(c)) return r;
+    if (it < 1) return fail(SWPC_EINVAL, "it must be >= 1");
+    return step2(c, it);
+}
+
+int swpc2d_run(swpc2d_ctx *c, int it0, int nsteps)
+{
+    clear_error();
+    if (int r = ready2(c)) return r;
+    if (it0 < 1 || nsteps < 0) return fail(SWPC_EINVAL, "bad step range");
+    for (int it = it0; it < it0 + nsteps; it++)
+        if (int r = step2(c, it)) return r;
+    return SWPC_OK;
+}
+
+int swpc2d_get_receivers(swpc2d_ctx *c, float *vel, double *disp)
+{
+    clear_error();
+    if (!c) return fail(SWPC_EINVAL, "null context");
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    size_t n = (size_t)2 * c->nst * c->ntw;
+    if (n == 0) return SWPC_OK;
+    if (vel) SWPC_HIP(hipMemcpy(vel, c->wvel.p, n * 4, hipMemcpyDeviceToHost));
+    if (disp) SWPC_HIP(hipMemcpy(disp, c->wdisp.p, n * 8, hipMemcpyDeviceToHost));
+    return SWPC_OK;
+}
+
+int swpc2d_get_field(swpc2d_ctx *c, int f, double *out)
+{
+    clear_error();
+    if (!c || !out) return fail(SWPC_EINVAL, "null argument");
+    DBuf *b = field_buf2(c, f);
+    if (!b) return fail(SWPC_EINVAL, "unknown field %d", f);
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    if (fp64(c)) {
+        std::vector<double> h((size_t)c->L.vol);
+        SWPC_HIP(hipMemcpy(h.data(), b->p, h.size() * 8, hipMemcpyDeviceToHost));
+        dev_to_box2(c->L, h.data(), out);
+    } else {
+        std::vector<float> h((size_t)c->L.vol);
+        SWPC_HIP(hipMemcpy(h.data(), b->p, h.size() * 4, hipMemcpyDeviceToHost));
+        dev_to_box2(c->L, h.data(), out);
+    }
+    return SWPC_OK;
+}
+
+int swpc2d_set_field(swpc2d_ctx *c, int f, const double *in)
+{
+    clear_error();
+    if (!c || !in) return fail(SWPC_EINVAL, "null argument");
+    DBuf *b = field_buf2(c, f);
+    if (!b) return fail(SWPC_EINVAL, "unknown field %d", f);
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    if (fp64(c)) {
+        std::vector<double> h((size_t)c->L.vol, 0.0);
+        box2_to_dev(c->L, in, h.data());
+        SWPC_HIP(hipMemcpy(b->p, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> h((size_t)c->L.vol, 0.0f);
+        box2_to_dev(c->L, in, h.data());
+        SWPC_HIP(hipMemcpy(b->p, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+    }
+    return SWPC_OK;
+}
+
+int swpc2d_vmax(swpc2d_ctx *c, float *out2)
+{
+    clear_error();
+    if (int r = ready2(c)) return r;
+    if (!out2) return fail(SWPC_EINVAL, "null argument");
+    hipLaunchKernelGGL(k2_fill_u32, dim3(1), dim3(64), 0, c->stream, (unsigned *)c->vmax.p, 2, 0u);
+    SWPC_CHECK_LAUNCH();
+    int ni = c->d.nx - 2 * c->d.na;
+    if (ni > 0) {
+        if (fp64(c)) {
+            hipLaunchKernelGGL((k2_vmax<double>), dim3((ni + 255) / 256), dim3(256), 0, c->stream, make_args2<double>(c),
+                               (const int *)c->kob.p, (unsigned *)c->vmax.p);
+        } else {
+            hipLaunchKernelGGL((k2_vmax<float>), dim3((ni + 255) / 256), dim3(256), 0, c->stream, make_args2<float>(c),
+                               (const int *)c->kob.p, (unsigned *)c->vmax.p);
+        }
+        SWPC_CHECK_LAUNCH();
+    }
+    unsigned u[2];
+    SWPC_HIP(hipMemcpyAsync(u, c->vmax.p, 8, hipMemcpyDeviceToHost, c->stream));
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    memcpy(&out2[0], &u[0], 4);
+    memcpy(&out2[1], &u[1], 4);
+    return SWPC_OK;
+}
+
+int swpc2d_sync(swpc2d_ctx *c)
+{
+    clear_error();
+    if (!c) return fail(SWPC_EINVAL, "null context");
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    return SWPC_OK;
+}
+
+}  // extern "C"

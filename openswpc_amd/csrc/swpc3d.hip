@@ -1,0 +1,1157 @@
+// 3-D viscoelastic velocity-stress core (swpc_3d) for MI355X.
+//
+// Kernels
+//   k3_stress_int  interior stress: 4th/2nd-order staggered derivatives of
+//                  V and the generalized-Zener memory-variable update,
+//                  normal and shear parts fused in one pass
+//                  (src/swpc_3d/m_kernel.f90:142-326)
+//   k3_vel_int     interior velocity (m_kernel.f90:75-140)
+//   k3_stress_pml  ADE-CFS PML stress (m_absorb_p.f90:423-503)
+//   k3_vel_pml     ADE-CFS PML velocity (m_absorb_p.f90:247-309)
+//   k3_stressglut / k3_bodyforce / k3_receivers / k3_vmax
+//
+// The interior kernels give each thread one (i,j) column of a 64 x BY
+// tile and stream it through z: the z-neighbours of the stencil sit in a
+// register window that shifts by one plane per step, the x-neighbours
+// come from the same wavefront's coalesced row and the y-neighbours from
+// the rows of the same workgroup (L1/L2).  Every array is read and written
+// once per cell, in 128-byte aligned rows.  The PML kernels enumerate the
+// five boxes of the absorber region (two x faces, two y faces, the
+// bottom slab) with one thread per cell.  There is no MFMA: the stencil
+// is a gather, not a contraction, and is bound by HBM bandwidth.
+#include "swpc_common.h"
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+
+namespace swpc {
+
+constexpr int NM3 = 3;  // Zener bodies; the reference fixes NM = 3 (m_global.f90:31)
+
+struct Box3 {
+    int i0, j0, k0, ni, nj, nk;
+    unsigned start;  // first flat cell id of this box
+};
+
+struct Boxes3 {
+    Box3 b[5];
+    int n;
+    unsigned total;
+};
+
+template <typename T>
+struct K3 {
+    int px, sxy, nx, ny, nz, na;
+    T *V[3];    // Vx Vy Vz
+    T *S[6];    // Sxx Syy Szz Syz Sxz Sxy
+    T *R[18];   // Rxx(m) Ryy(m) Rzz(m) Ryz(m) Rxz(m) Rxy(m), m = 0..2
+    T *A[18];   // ADE arrays (order of enum swpc3d_field)
+    const float *bx, *by, *bz, *lam, *mu, *taup, *taus, *muyz, *muxz, *muxy;
+    const int4 *surf;   // (kfs_top, kfs_bot, kob_top, kob_bot) per column (i,j) in [1,nx]x[1,ny]
+    const T *gx, *gy, *gz;  // profiles [8][n]: c1..c4 at centres then e1..e4 at edges
+    T e4[3][2], e2[3][2];   // (re40, re41) for isign = -1 (4th) and +1 (2nd), per axis
+    T r2[3];                // 1/dx, 1/dy, 1/dz
+    T c1[NM3], c2[NM3], d1[NM3], d2;
+    T dt;
+    int ti0;  // i of the first interior tile
+    int kc;   // z-chunk length of the interior kernels
+    Boxes3 pml;
+};
+
+// ---------------------------------------------------------------------------
+// interior stress
+// ---------------------------------------------------------------------------
+template <typename T, int BY>
+__global__ void __launch_bounds__(64 * BY) k3_stress_int(const K3<T> a)
+{
+    const int i = a.ti0 + (int)blockIdx.x * 64 + (int)threadIdx.x;
+    const int j = a.na + 1 + (int)blockIdx.y * BY + (int)threadIdx.y;
+    if (i <= a.na || i > a.nx - a.na || j > a.ny - a.na) return;
+    const int kb = 1 + (int)blockIdx.z * a.kc;
+    const int ke = min(kb + a.kc - 1, a.nz - a.na);
+    const int4 fs = a.surf[(j - 1) * a.nx + (i - 1)];
+    const int px = a.px, sxy = a.sxy;
+
+    const T *__restrict__ Vx = a.V[0];
+    const T *__restrict__ Vy = a.V[1];
+    const T *__restrict__ Vz = a.V[2];
+    T *__restrict__ Sxx = a.S[0];
+    T *__restrict__ Syy = a.S[1];
+    T *__restrict__ Szz = a.S[2];
+    T *__restrict__ Syz = a.S[3];
+    T *__restrict__ Sxz = a.S[4];
+    T *__restrict__ Sxy = a.S[5];
+    const float *__restrict__ lam = a.lam;
+    const float *__restrict__ mu = a.mu;
+    const float *__restrict__ taup = a.taup;
+    const float *__restrict__ taus = a.taus;
+    const float *__restrict__ muyz = a.muyz;
+    const float *__restrict__ muxz = a.muxz;
+    const float *__restrict__ muxy = a.muxy;
+
+    int c = at3(px, sxy, i, j, kb);
+    T vxm1 = Vx[c - sxy], vx0 = Vx[c], vxp1 = Vx[c + sxy];
+    T vym1 = Vy[c - sxy], vy0 = Vy[c], vyp1 = Vy[c + sxy];
+    T vzm2 = Vz[c - 2 * sxy], vzm1 = Vz[c - sxy], vz0 = Vz[c];
+    const T dt = a.dt;
+
+    for (int k = kb; k <= ke; ++k, c += sxy) {
+        const T vxp2 = Vx[c + 2 * sxy];
+        const T vyp2 = Vy[c + 2 * sxy];
+        const T vzp1 = Vz[c + sxy];
+
+        // order switch of m_kernel.f90:185-186
+        const int o1 = (k - fs.x) * (fs.y - k), o2 = (k - fs.z) * (fs.w - k);
+        const bool lo = max(o1, o2) >= 0;
+        const T e0x = lo ? a.e2[0][0] : a.e4[0][0], e1x = lo ? a.e2[0][1] : a.e4[0][1];
+        const T e0y = lo ? a.e2[1][0] : a.e4[1][0], e1y = lo ? a.e2[1][1] : a.e4[1][1];
+        const T e0z = lo ? a.e2[2][0] : a.e4[2][0], e1z = lo ? a.e2[2][1] : a.e4[2][1];
+
+        const T dxVx = (vx0 - Vx[c - 1]) * e0x - (Vx[c + 1] - Vx[c - 2]) * e1x;
+        const T dyVy = (vy0 - Vy[c - px]) * e0y - (Vy[c + px] - Vy[c - 2 * px]) * e1y;
+        const T dzVz = (vz0 - vzm1) * e0z - (vzp1 - vzm2) * e1z;
+
+        const T dxVy_dyVx = (Vy[c + 1] - vy0) * e0x - (Vy[c + 2] - Vy[c - 1]) * e1x
+                          + (Vx[c + px] - vx0) * e0y - (Vx[c + 2 * px] - Vx[c - px]) * e1y;
+        const T dxVz_dzVx = (Vz[c + 1] - vz0) * e0x - (Vz[c + 2] - Vz[c - 1]) * e1x
+                          + (vxp1 - vx0) * e0z - (vxp2 - vxm1) * e1z;
+        const T dyVz_dzVy = (Vz[c + px] - vz0) * e0y - (Vz[c + 2 * px] - Vz[c - px]) * e1y
+                          + (vyp1 - vy0) * e0z - (vyp2 - vym1) * e1z;
+
+        // normal part (m_kernel.f90:199-235)
+        const T mu2 = 2 * (T)mu[c];
+        const T lam2mu = (T)lam[c] + mu2;
+        const T taup1 = (T)taup[c], taus1 = (T)taus[c];
+        const T d3v3 = dxVx + dyVy + dzVz;
+        const T dyVy_dzVz = dyVy + dzVz, dxVx_dzVz = dxVx + dzVz, dxVx_dyVy = dxVx + dyVy;
+        const T fxx = lam2mu * taup1 * d3v3 - mu2 * taus1 * dyVy_dzVz;
+        const T fyy = lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dzVz;
+        const T fzz = lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dyVy;
+        T Rxx_n = 0, Ryy_n = 0, Rzz_n = 0;
+#pragma unroll
+        for (int m = 0; m < NM3; m++) {
+            T rxx = a.c1[m] * a.R[0 + m][c] - a.c2[m] * fxx * dt;
+            T ryy = a.c1[m] * a.R[3 + m][c] - a.c2[m] * fyy * dt;
+            T rzz = a.c1[m] * a.R[6 + m][c] - a.c2[m] * fzz * dt;
+            a.R[0 + m][c] = rxx;
+            a.R[3 + m][c] = ryy;
+            a.R[6 + m][c] = rzz;
+            Rxx_n += a.d1[m] * rxx;
+            Ryy_n += a.d1[m] * ryy;
+            Rzz_n += a.d1[m] * rzz;
+        }
+        const T taup_plus1 = 1 + taup1 * (1 + a.d2);
+        const T taus_plus1 = 1 + taus1 * (1 + a.d2);
+        Sxx[c] = Sxx[c] + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dyVy_dzVz + Rxx_n) * dt;
+        Syy[c] = Syy[c] + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dzVz + Ryy_n) * dt;
+        Szz[c] = Szz[c] + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dyVy + Rzz_n) * dt;
+
+        // shear part (m_kernel.f90:289-311)
+        const T myz = (T)muyz[c], mxz = (T)muxz[c], mxy = (T)muxy[c];
+        const T fyz = myz * taus1 * dyVz_dzVy;
+        const T fxz = mxz * taus1 * dxVz_dzVx;
+        const T fxy = mxy * taus1 * dxVy_dyVx;
+        T Ryz_n = 0, Rxz_n = 0, Rxy_n = 0;
+#pragma unroll
+        for (int m = 0; m < NM3; m++) {
+            T ryz = a.c1[m] * a.R[9 + m][c] - a.c2[m] * fyz * dt;
+            T rxz = a.c1[m] * a.R[12 + m][c] - a.c2[m] * fxz * dt;
+            T rxy = a.c1[m] * a.R[15 + m][c] - a.c2[m] * fxy * dt;
+            a.R[9 + m][c] = ryz;
+            a.R[12 + m][c] = rxz;
+            a.R[15 + m][c] = rxy;
+            Ryz_n += a.d1[m] * ryz;
+            Rxz_n += a.d1[m] * rxz;
+            Rxy_n += a.d1[m] * rxy;
+        }
+        Syz[c] = Syz[c] + (myz * taus_plus1 * dyVz_dzVy + Ryz_n) * dt;
+        Sxz[c] = Sxz[c] + (mxz * taus_plus1 * dxVz_dzVx + Rxz_n) * dt;
+        Sxy[c] = Sxy[c] + (mxy * taus_plus1 * dxVy_dyVx + Rxy_n) * dt;
+
+        vxm1 = vx0; vx0 = vxp1; vxp1 = vxp2;
+        vym1 = vy0; vy0 = vyp1; vyp1 = vyp2;
+        vzm2 = vzm1; vzm1 = vz0; vz0 = vzp1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// interior velocity
+// ---------------------------------------------------------------------------
+template <typename T, int BY>
+__global__ void __launch_bounds__(64 * BY) k3_vel_int(const K3<T> a)
+{
+    const int i = a.ti0 + (int)blockIdx.x * 64 + (int)threadIdx.x;
+    const int j = a.na + 1 + (int)blockIdx.y * BY + (int)threadIdx.y;
+    if (i <= a.na || i > a.nx - a.na || j > a.ny - a.na) return;
+    const int kb = 1 + (int)blockIdx.z * a.kc;
+    const int ke = min(kb + a.kc - 1, a.nz - a.na);
+    const int4 fs = a.surf[(j - 1) * a.nx + (i - 1)];
+    const int px = a.px, sxy = a.sxy;
+
+    T *__restrict__ Vx = a.V[0];
+    T *__restrict__ Vy = a.V[1];
+    T *__restrict__ Vz = a.V[2];
+    const T *__restrict__ Sxx = a.S[0];
+    const T *__restrict__ Syy = a.S[1];
+    const T *__restrict__ Szz = a.S[2];
+    const T *__restrict__ Syz = a.S[3];
+    const T *__restrict__ Sxz = a.S[4];
+    const T *__restrict__ Sxy = a.S[5];
+    const float *__restrict__ bx = a.bx;
+    const float *__restrict__ by = a.by;
+    const float *__restrict__ bz = a.bz;
+
+    int c = at3(px, sxy, i, j, kb);
+    T xzm2 = Sxz[c - 2 * sxy], xzm1 = Sxz[c - sxy], xz0 = Sxz[c];
+    T yzm2 = Syz[c - 2 * sxy], yzm1 = Syz[c - sxy], yz0 = Syz[c];
+    T zzm1 = Szz[c - sxy], zz0 = Szz[c], zzp1 = Szz[c + sxy];
+    const T dt = a.dt;
+
+    for (int k = kb; k <= ke; ++k, c += sxy) {
+        const T xzp1 = Sxz[c + sxy];
+        const T yzp1 = Syz[c + sxy];
+        const T zzp2 = Szz[c + 2 * sxy];
+
+        const int o1 = (k - fs.x) * (fs.y - k), o2 = (k - fs.z) * (fs.w - k);
+        const bool lo = max(o1, o2) >= 0;
+        const T e0x = lo ? a.e2[0][0] : a.e4[0][0], e1x = lo ? a.e2[0][1] : a.e4[0][1];
+        const T e0y = lo ? a.e2[1][0] : a.e4[1][0], e1y = lo ? a.e2[1][1] : a.e4[1][1];
+        const T e0z = lo ? a.e2[2][0] : a.e4[2][0], e1z = lo ? a.e2[2][1] : a.e4[2][1];
+
+        const T sxy0 = Sxy[c];
+        // m_kernel.f90:113-121
+        const T d3Sx3 = (Sxx[c + 1] - Sxx[c]) * e0x - (Sxx[c + 2] - Sxx[c - 1]) * e1x
+                      + (sxy0 - Sxy[c - px]) * e0y - (Sxy[c + px] - Sxy[c - 2 * px]) * e1y
+                      + (xz0 - xzm1) * e0z - (xzp1 - xzm2) * e1z;
+        const T d3Sy3 = (sxy0 - Sxy[c - 1]) * e0x - (Sxy[c + 1] - Sxy[c - 2]) * e1x
+                      + (Syy[c + px] - Syy[c]) * e0y - (Syy[c + 2 * px] - Syy[c - px]) * e1y
+                      + (yz0 - yzm1) * e0z - (yzp1 - yzm2) * e1z;
+        const T d3Sz3 = (xz0 - Sxz[c - 1]) * e0x - (Sxz[c + 1] - Sxz[c - 2]) * e1x
+                      + (yz0 - Syz[c - px]) * e0y - (Syz[c + px] - Syz[c - 2 * px]) * e1y
+                      + (zzp1 - zz0) * e0z - (zzp2 - zzm1) * e1z;
+
+        Vx[c] = Vx[c] + (T)bx[c] * d3Sx3 * dt;
+        Vy[c] = Vy[c] + (T)by[c] * d3Sy3 * dt;
+        Vz[c] = Vz[c] + (T)bz[c] * d3Sz3 * dt;
+
+        xzm2 = xzm1; xzm1 = xz0; xz0 = xzp1;
+        yzm2 = yzm1; yzm1 = yz0; yz0 = yzp1;
+        zzm1 = zz0; zz0 = zzp1; zzp1 = zzp2;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PML (ADE-CFS, 2nd order, elastic)
+// ---------------------------------------------------------------------------
+__device__ inline bool pml_cell(const Boxes3 &B, unsigned id, int &i, int &j, int &k)
+{
+    if (id >= B.total) return false;
+    int b = 0;
+#pragma unroll
+    for (int q = 1; q < 5; q++)
+        if (q < B.n && id >= B.b[q].start) b = q;
+    const Box3 &x = B.b[b];
+    unsigned l = id - x.start;
+    unsigned li = l % (unsigned)x.ni;
+    unsigned t = l / (unsigned)x.ni;
+    unsigned lj = t % (unsigned)x.nj;
+    unsigned lk = t / (unsigned)x.nj;
+    i = x.i0 + (int)li;
+    j = x.j0 + (int)lj;
+    k = x.k0 + (int)lk;
+    return true;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k3_stress_pml(const K3<T> a)
+{
+    int i, j, k;
+    if (!pml_cell(a.pml, blockIdx.x * 256u + threadIdx.x, i, j, k)) return;
+    const int px = a.px, sxy = a.sxy;
+    const int c = at3(px, sxy, i, j, k);
+    const T *__restrict__ Vx = a.V[0];
+    const T *__restrict__ Vy = a.V[1];
+    const T *__restrict__ Vz = a.V[2];
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const T gxc1 = a.gx[0 * nx + i - 1], gxc2 = a.gx[1 * nx + i - 1], gxc3 = a.gx[2 * nx + i - 1], gxc4 = a.gx[3 * nx + i - 1];
+    const T gxe1 = a.gx[4 * nx + i - 1], gxe2 = a.gx[5 * nx + i - 1], gxe3 = a.gx[6 * nx + i - 1], gxe4 = a.gx[7 * nx + i - 1];
+    const T gyc1 = a.gy[0 * ny + j - 1], gyc2 = a.gy[1 * ny + j - 1], gyc3 = a.gy[2 * ny + j - 1], gyc4 = a.gy[3 * ny + j - 1];
+    const T gye1 = a.gy[4 * ny + j - 1], gye2 = a.gy[5 * ny + j - 1], gye3 = a.gy[6 * ny + j - 1], gye4 = a.gy[7 * ny + j - 1];
+    const T gzc1 = a.gz[0 * nz + k - 1], gzc2 = a.gz[1 * nz + k - 1], gzc3 = a.gz[2 * nz + k - 1], gzc4 = a.gz[3 * nz + k - 1];
+    const T gze1 = a.gz[4 * nz + k - 1], gze2 = a.gz[5 * nz + k - 1], gze3 = a.gz[6 * nz + k - 1], gze4 = a.gz[7 * nz + k - 1];
+    const T r20x = a.r2[0], r20y = a.r2[1], r20z = a.r2[2];
+    const T dt = a.dt;
+    T *__restrict__ axVx = a.A[0];
+    T *__restrict__ ayVx = a.A[1];
+    T *__restrict__ azVx = a.A[2];
+    T *__restrict__ axVy = a.A[3];
+    T *__restrict__ ayVy = a.A[4];
+    T *__restrict__ azVy = a.A[5];
+    T *__restrict__ axVz = a.A[6];
+    T *__restrict__ ayVz = a.A[7];
+    T *__restrict__ azVz = a.A[8];
+
+    const T vx0 = Vx[c], vy0 = Vy[c], vz0 = Vz[c];
+    // normal (m_absorb_p.f90:445-462)
+    const T dxVx = (vx0 - Vx[c - 1]) * r20x;
+    const T dyVy = (vy0 - Vy[c - px]) * r20y;
+    const T dzVz = (vz0 - Vz[c - sxy]) * r20z;
+    const T lam2mu_R = (T)(a.lam[c] + 2 * a.mu[c]);
+    const T lam_R = lam2mu_R - 2 * (T)a.mu[c];
+    const T axvx = axVx[c], ayvy = ayVy[c], azvz = azVz[c];
+    const T dxVx_ade = gxc1 * dxVx + gxc2 * axvx;
+    const T dyVy_ade = gyc1 * dyVy + gyc2 * ayvy;
+    const T dzVz_ade = gzc1 * dzVz + gzc2 * azvz;
+    a.S[0][c] = a.S[0][c] + (lam2mu_R * dxVx_ade + lam_R * (dyVy_ade + dzVz_ade)) * dt;
+    a.S[1][c] = a.S[1][c] + (lam2mu_R * dyVy_ade + lam_R * (dxVx_ade + dzVz_ade)) * dt;
+    a.S[2][c] = a.S[2][c] + (lam2mu_R * dzVz_ade + lam_R * (dxVx_ade + dyVy_ade)) * dt;
+    axVx[c] = gxc3 * axvx + gxc4 * dxVx * dt;
+    ayVy[c] = gyc3 * ayvy + gyc4 * dyVy * dt;
+    azVz[c] = gzc3 * azvz + gzc4 * dzVz * dt;
+
+    // shear (m_absorb_p.f90:469-491)
+    const T dxVy = (Vy[c + 1] - vy0) * r20x;
+    const T dxVz = (Vz[c + 1] - vz0) * r20x;
+    const T dyVx = (Vx[c + px] - vx0) * r20y;
+    const T dyVz = (Vz[c + px] - vz0) * r20y;
+    const T dzVx = (Vx[c + sxy] - vx0) * r20z;
+    const T dzVy = (Vy[c + sxy] - vy0) * r20z;
+    const T ayvx = ayVx[c], azvx = azVx[c], axvy = axVy[c], azvy = azVy[c], axvz = axVz[c], ayvz = ayVz[c];
+    a.S[3][c] = a.S[3][c] + (T)a.muyz[c] * (gye1 * dyVz + gze1 * dzVy + gye2 * ayvz + gze2 * azvy) * dt;
+    a.S[4][c] = a.S[4][c] + (T)a.muxz[c] * (gxe1 * dxVz + gze1 * dzVx + gxe2 * axvz + gze2 * azvx) * dt;
+    a.S[5][c] = a.S[5][c] + (T)a.muxy[c] * (gxe1 * dxVy + gye1 * dyVx + gxe2 * axvy + gye2 * ayvx) * dt;
+    ayVx[c] = gye3 * ayvx + gye4 * dyVx * dt;
+    azVx[c] = gze3 * azvx + gze4 * dzVx * dt;
+    axVy[c] = gxe3 * axvy + gxe4 * dxVy * dt;
+    azVy[c] = gze3 * azvy + gze4 * dzVy * dt;
+    axVz[c] = gxe3 * axvz + gxe4 * dxVz * dt;
+    ayVz[c] = gye3 * ayvz + gye4 * dyVz * dt;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k3_vel_pml(const K3<T> a)
+{
+    int i, j, k;
+    if (!pml_cell(a.pml, blockIdx.x * 256u + threadIdx.x, i, j, k)) return;
+    const int px = a.px, sxy = a.sxy;
+    const int c = at3(px, sxy, i, j, k);
+    const T *__restrict__ Sxx = a.S[0];
+    const T *__restrict__ Syy = a.S[1];
+    const T *__restrict__ Szz = a.S[2];
+    const T *__restrict__ Syz = a.S[3];
+    const T *__restrict__ Sxz = a.S[4];
+    const T *__restrict__ Sxy = a.S[5];
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const T gxc1 = a.gx[0 * nx + i - 1], gxc2 = a.gx[1 * nx + i - 1], gxc3 = a.gx[2 * nx + i - 1], gxc4 = a.gx[3 * nx + i - 1];
+    const T gxe1 = a.gx[4 * nx + i - 1], gxe2 = a.gx[5 * nx + i - 1], gxe3 = a.gx[6 * nx + i - 1], gxe4 = a.gx[7 * nx + i - 1];
+    const T gyc1 = a.gy[0 * ny + j - 1], gyc2 = a.gy[1 * ny + j - 1], gyc3 = a.gy[2 * ny + j - 1], gyc4 = a.gy[3 * ny + j - 1];
+    const T gye1 = a.gy[4 * ny + j - 1], gye2 = a.gy[5 * ny + j - 1], gye3 = a.gy[6 * ny + j - 1], gye4 = a.gy[7 * ny + j - 1];
+    const T gzc1 = a.gz[0 * nz + k - 1], gzc2 = a.gz[1 * nz + k - 1], gzc3 = a.gz[2 * nz + k - 1], gzc4 = a.gz[3 * nz + k - 1];
+    const T gze1 = a.gz[4 * nz + k - 1], gze2 = a.gz[5 * nz + k - 1], gze3 = a.gz[6 * nz + k - 1], gze4 = a.gz[7 * nz + k - 1];
+    const T r20x = a.r2[0], r20y = a.r2[1], r20z = a.r2[2];
+    const T dt = a.dt;
+    T *__restrict__ axSxx = a.A[9];
+    T *__restrict__ aySxy = a.A[10];
+    T *__restrict__ azSxz = a.A[11];
+    T *__restrict__ axSxy = a.A[12];
+    T *__restrict__ aySyy = a.A[13];
+    T *__restrict__ azSyz = a.A[14];
+    T *__restrict__ axSxz = a.A[15];
+    T *__restrict__ aySyz = a.A[16];
+    T *__restrict__ azSzz = a.A[17];
+
+    // m_absorb_p.f90:266-298
+    const T sxx0 = Sxx[c], syy0 = Syy[c], szz0 = Szz[c], syz0 = Syz[c], sxz0 = Sxz[c], sxy0 = Sxy[c];
+    const T dxSxx = (Sxx[c + 1] - sxx0) * r20x;
+    const T dySyy = (Syy[c + px] - syy0) * r20y;
+    const T dzSzz = (Szz[c + sxy] - szz0) * r20z;
+    const T dySyz = (syz0 - Syz[c - px]) * r20y;
+    const T dzSyz = (syz0 - Syz[c - sxy]) * r20z;
+    const T dxSxz = (sxz0 - Sxz[c - 1]) * r20x;
+    const T dzSxz = (sxz0 - Sxz[c - sxy]) * r20z;
+    const T dxSxy = (sxy0 - Sxy[c - 1]) * r20x;
+    const T dySxy = (sxy0 - Sxy[c - px]) * r20y;
+    const T axsxx = axSxx[c], aysxy = aySxy[c], azsxz = azSxz[c];
+    const T axsxy = axSxy[c], aysyy = aySyy[c], azsyz = azSyz[c];
+    const T axsxz = axSxz[c], aysyz = aySyz[c], azszz = azSzz[c];
+    a.V[0][c] = a.V[0][c] + (T)a.bx[c] * (gxe1 * dxSxx + gyc1 * dySxy + gzc1 * dzSxz
+                                          + gxe2 * axsxx + gyc2 * aysxy + gzc2 * azsxz) * dt;
+    a.V[1][c] = a.V[1][c] + (T)a.by[c] * (gxc1 * dxSxy + gye1 * dySyy + gzc1 * dzSyz
+                                          + gxc2 * axsxy + gye2 * aysyy + gzc2 * azsyz) * dt;
+    a.V[2][c] = a.V[2][c] + (T)a.bz[c] * (gxc1 * dxSxz + gyc1 * dySyz + gze1 * dzSzz
+                                          + gxc2 * axsxz + gyc2 * aysyz + gze2 * azszz) * dt;
+    axSxx[c] = gxe3 * axsxx + gxe4 * dxSxx * dt;
+    aySxy[c] = gyc3 * aysxy + gyc4 * dySxy * dt;
+    azSxz[c] = gzc3 * azsxz + gzc4 * dzSxz * dt;
+    axSxy[c] = gxc3 * axsxy + gxc4 * dxSxy * dt;
+    aySyy[c] = gye3 * aysyy + gye4 * dySyy * dt;
+    azSyz[c] = gzc3 * azsyz + gzc4 * dzSyz * dt;
+    axSxz[c] = gxc3 * axsxz + gxc4 * dxSxz * dt;
+    aySyz[c] = gyc3 * aysyz + gyc4 * dySyz * dt;
+    azSzz[c] = gze3 * azszz + gze4 * dzSzz * dt;
+}
+
+// ---------------------------------------------------------------------------
+// sources, receivers, vmax
+// ---------------------------------------------------------------------------
+// Sources are applied in the reference's sequential order by one thread
+// (m_source.f90:796 `loop seq`): several sources may share a cell.
+template <typename T>
+__global__ void k3_stressglut(const K3<T> a, int nsrc, const int *__restrict__ src, const double *__restrict__ m6,
+                              const double *__restrict__ sdrop)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int px = a.px, sxy = a.sxy;
+    for (int n = 0; n < nsrc; n++) {
+        const int ii = src[3 * n], jj = src[3 * n + 1], kk = src[3 * n + 2];
+        const T sd = (T)sdrop[n];
+        const T mxx = (T)m6[6 * n + 0], myy = (T)m6[6 * n + 1], mzz = (T)m6[6 * n + 2];
+        const T myz = (T)m6[6 * n + 3], mxz = (T)m6[6 * n + 4], mxy = (T)m6[6 * n + 5];
+        const int c = at3(px, sxy, ii, jj, kk);
+        a.S[0][c] -= mxx * sd;
+        a.S[1][c] -= myy * sd;
+        a.S[2][c] -= mzz * sd;
+        a.S[5][c] -= mxy * sd / 4;
+        a.S[5][c - px] -= mxy * sd / 4;
+        a.S[5][c - 1] -= mxy * sd / 4;
+        a.S[5][c - 1 - px] -= mxy * sd / 4;
+        a.S[4][c] -= mxz * sd / 4;
+        a.S[4][c - sxy] -= mxz * sd / 4;
+        a.S[4][c - 1] -= mxz * sd / 4;
+        a.S[4][c - 1 - sxy] -= mxz * sd / 4;
+        a.S[3][c] -= myz * sd / 4;
+        a.S[3][c - sxy] -= myz * sd / 4;
+        a.S[3][c - px] -= myz * sd / 4;
+        a.S[3][c - px - sxy] -= myz * sd / 4;
+    }
+}
+
+template <typename T>
+__global__ void k3_bodyforce(const K3<T> a, int nsrc, const int *__restrict__ src, const double *__restrict__ f3,
+                             const double *__restrict__ fdt)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int px = a.px, sxy = a.sxy;
+    for (int n = 0; n < nsrc; n++) {
+        const int ii = src[3 * n], jj = src[3 * n + 1], kk = src[3 * n + 2];
+        const T f = (T)fdt[n];
+        const T fx = (T)f3[3 * n], fy = (T)f3[3 * n + 1], fz = (T)f3[3 * n + 2];
+        const int c = at3(px, sxy, ii, jj, kk);
+        a.V[0][c] += (T)a.bx[c] * fx * f;
+        a.V[0][c - 1] += (T)a.bx[c - 1] * fx * f;
+        a.V[1][c] += (T)a.by[c] * fy * f;
+        a.V[1][c - px] += (T)a.by[c - px] * fy * f;
+        a.V[2][c] += (T)a.bz[c] * fz * f;
+        a.V[2][c - sxy] += (T)a.bz[c - sxy] * fz * f;
+    }
+}
+
+// wav__store (m_wav.f90:436-447, 524-535): disp accumulates every step in
+// FP64; when store != 0 the sample itw is written.
+template <typename T>
+__global__ void k3_receivers(const K3<T> a, int nst, const int *__restrict__ st, double *__restrict__ uacc,
+                             float *__restrict__ vel_out, double *__restrict__ disp_out, int store)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nst) return;
+    const int px = a.px, sxy = a.sxy;
+    const int c = at3(px, sxy, st[3 * n], st[3 * n + 1], st[3 * n + 2]);
+    const T vx = a.V[0][c] + a.V[0][c - 1];
+    const T vy = a.V[1][c] + a.V[1][c - px];
+    const T vz = a.V[2][c] + a.V[2][c - sxy];
+    const double hdt = 0.5 * (double)a.dt;
+    double ux = uacc[3 * n] + (double)vx * hdt;
+    double uy = uacc[3 * n + 1] + (double)vy * hdt;
+    double uz = uacc[3 * n + 2] - (double)vz * hdt;
+    uacc[3 * n] = ux;
+    uacc[3 * n + 1] = uy;
+    uacc[3 * n + 2] = uz;
+    if (store) {
+        vel_out[3 * n] = (float)(vx / 2);
+        vel_out[3 * n + 1] = (float)(vy / 2);
+        vel_out[3 * n + 2] = (float)(-vz / 2);
+        disp_out[3 * n] = ux;
+        disp_out[3 * n + 1] = uy;
+        disp_out[3 * n + 2] = uz;
+    }
+}
+
+__global__ void k_fill_u32(unsigned *p, int n, unsigned v)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+template <typename T>
+__global__ void k3_vmax(const K3<T> a, const int *__restrict__ kob, unsigned *__restrict__ out)
+{
+    const int margin = 5;
+    const int i = a.na + margin + 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int j = a.na + margin + 1 + (int)blockIdx.y;
+    float m0 = 0, m1 = 0, m2 = 0;
+    if (i <= a.nx - a.na - margin && j <= a.ny - a.na - margin) {
+        const int k = kob[(j - 1) * a.nx + (i - 1)] + 1;
+        const int c = at3(a.px, a.sxy, i, j, k);
+        m0 = fabsf((float)a.V[0][c]);
+        m1 = fabsf((float)a.V[1][c]);
+        m2 = fabsf((float)a.V[2][c]);
+    }
+    // wavefront max, then one atomic per wave (non-negative floats order as integers)
+    for (int off = 32; off > 0; off >>= 1) {
+        m0 = fmaxf(m0, __shfl_xor(m0, off));
+        m1 = fmaxf(m1, __shfl_xor(m1, off));
+        m2 = fmaxf(m2, __shfl_xor(m2, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(out + 0, __float_as_uint(m0));
+        atomicMax(out + 1, __float_as_uint(m1));
+        atomicMax(out + 2, __float_as_uint(m2));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct Ctx3Base {
+    virtual ~Ctx3Base() {}
+};
+
+}  // namespace swpc
+
+struct swpc3d_ctx {
+    swpc3d_desc d;
+    swpc::Lay3 L;
+    size_t esz;  // bytes per state element (4 or 8)
+    hipStream_t stream = nullptr;
+    long long bytes = 0;
+    swpc::DBuf V[3], S[6], R[18], A[18];
+    swpc::DBuf med[10];     // bx by bz lam mu taup taus muyz muxz muxy
+    swpc::DBuf surf, kob, gx, gy, gz;
+    swpc::DBuf src, m6, sdrop, bsrc, f3, fdt;
+    swpc::DBuf st, uacc, wvel, wdisp, vmax;
+    int nsrc = 0, nt_src = 0, nbf = 0, nt_bf = 0;
+    int nst = 0, ntdec_w = 1, ntw = 0;
+    bool have_medium = false, have_surface = false, have_pml = false;
+    swpc::Boxes3 boxes;
+    int ti0 = 1, ntx = 0, nty = 0, kc = 1, nkc = 1;
+    swpc::Prof prof;
+};
+
+namespace swpc {
+
+static const int BY3 = 4;
+
+template <typename T>
+static K3<T> make_args(swpc3d_ctx *c)
+{
+    K3<T> a;
+    memset(&a, 0, sizeof a);
+    const swpc3d_desc &d = c->d;
+    a.px = c->L.px; a.sxy = c->L.sxy; a.nx = d.nx; a.ny = d.ny; a.nz = d.nz; a.na = d.na;
+    for (int q = 0; q < 3; q++) a.V[q] = (T *)c->V[q].p;
+    for (int q = 0; q < 6; q++) a.S[q] = (T *)c->S[q].p;
+    for (int q = 0; q < 18; q++) a.R[q] = (T *)c->R[q].p;
+    for (int q = 0; q < 18; q++) a.A[q] = (T *)c->A[q].p;
+    const float **m[10] = {&a.bx, &a.by, &a.bz, &a.lam, &a.mu, &a.taup, &a.taus, &a.muyz, &a.muxz, &a.muxy};
+    for (int q = 0; q < 10; q++) *m[q] = (const float *)c->med[q].p;
+    a.surf = (const int4 *)c->surf.p;
+    a.gx = (const T *)c->gx.p; a.gy = (const T *)c->gy.p; a.gz = (const T *)c->gz.p;
+    // m_kernel.f90:43-54 evaluated in T, then re40 = rc40 + isign*rd40
+    const double h[3] = {d.dx, d.dy, d.dz};
+    for (int q = 0; q < 3; q++) {
+        T hh = (T)h[q];
+        T rc40 = (T)17.0 / (T)16.0 / hh, rc41 = (T)1.0 / (T)48.0 / hh;
+        T rd40 = (T)(-1.0) / (T)16.0 / hh, rd41 = (T)(-1.0) / (T)48.0 / hh;
+        a.e4[q][0] = rc40 + (-1) * rd40; a.e4[q][1] = rc41 + (-1) * rd41;
+        a.e2[q][0] = rc40 + (+1) * rd40; a.e2[q][1] = rc41 + (+1) * rd41;
+        a.r2[q] = (T)1.0 / hh;
+    }
+    // m_kernel.f90:58-67
+    T t = (T)(float)d.dt, s = 0;
+    for (int m = 0; m < NM3; m++) {
+        T tm = (T)d.ts[m];
+        a.c1[m] = (2 * tm - t) / (2 * tm + t);
+        a.c2[m] = (T)2 / (2 * tm + t) / NM3;
+        s += t / (2 * tm - t);
+        a.d1[m] = 2 * tm / (2 * tm - t);
+    }
+    a.d2 = s / NM3;
+    a.dt = t;
+    a.ti0 = c->ti0;
+    a.kc = c->kc;
+    a.pml = c->boxes;
+    return a;
+}
+
+template <typename T>
+static int launch_stress(swpc3d_ctx *c)
+{
+    K3<T> a = make_args<T>(c);
+    hipEvent_t e0;
+    if (c->ntx > 0 && c->nty > 0) {
+        if (int r = prof_begin(c->prof, 0, c->stream, &e0)) return r;
+        hipLaunchKernelGGL((k3_stress_int<T, BY3>), dim3(c->ntx, c->nty, c->nkc), dim3(64, BY3), 0, c->stream, a);
+        SWPC_CHECK_LAUNCH();
+        if (int r = prof_end(c->prof, 0, c->stream, e0)) return r;
+    }
+    if (c->boxes.total > 0) {
+        if (int r = prof_begin(c->prof, 1, c->stream, &e0)) return r;
+        hipLaunchKernelGGL((k3_stress_pml<T>), dim3((c->boxes.total + 255) / 256), dim3(256), 0, c->stream, a);
+        SWPC_CHECK_LAUNCH();
+        if (int r = prof_end(c->prof, 1, c->stream, e0)) return r;
+    }
+    return SWPC_OK;
+}
+
+template <typename T>
+static int launch_vel(swpc3d_ctx *c)
+{
+    K3<T> a = make_args<T>(c);
+    hipEvent_t e0;
+    if (c->ntx > 0 && c->nty > 0) {
+        if (int r = prof_begin(c->prof, 2, c->stream, &e0)) return r;
+        hipLaunchKernelGGL((k3_vel_int<T, BY3>), dim3(c->ntx, c->nty, c->nkc), dim3(64, BY3), 0, c->stream, a);
+        SWPC_CHECK_LAUNCH();
+        if (int r = prof_end(c->prof, 2, c->stream, e0)) return r;
+    }
+    if (c->boxes.total > 0) {
+        if (int r = prof_begin(c->prof, 3, c->stream, &e0)) return r;
+        hipLaunchKernelGGL((k3_vel_pml<T>), dim3((c->boxes.total + 255) / 256), dim3(256), 0, c->stream, a);
+        SWPC_CHECK_LAUNCH();
+        if (int r = prof_end(c->prof, 3, c->stream, e0)) return r;
+    }
+    return SWPC_OK;
+}
+
+template <typename T>
+static int launch_glut(swpc3d_ctx *c, int it)
+{
+    if (c->nsrc == 0) return SWPC_OK;
+    if (it < 1 || it > c->nt_src) return fail(SWPC_EINVAL, "stressglut: it=%d outside 1..%d", it, c->nt_src);
+    K3<T> a = make_args<T>(c);
+    hipLaunchKernelGGL((k3_stressglut<T>), dim3(1), dim3(64), 0, c->stream, a, c->nsrc, (const int *)c->src.p,
+                       (const double *)c->m6.p, (const double *)c->sdrop.p + (size_t)(it - 1) * c->nsrc);
+    SWPC_CHECK_LAUNCH();
+    return SWPC_OK;
+}
+
+template <typename T>
+static int launch_bf(swpc3d_ctx *c, int it)
+{
+    if (c->nbf == 0) return SWPC_OK;
+    if (it < 1 || it > c->nt_bf) return fail(SWPC_EINVAL, "bodyforce: it=%d outside 1..%d", it, c->nt_bf);
+    K3<T> a = make_args<T>(c);
+    hipLaunchKernelGGL((k3_bodyforce<T>), dim3(1), dim3(64), 0, c->stream, a, c->nbf, (const int *)c->bsrc.p,
+                       (const double *)c->f3.p, (const double *)c->fdt.p + (size_t)(it - 1) * c->nbf);
+    SWPC_CHECK_LAUNCH();
+    return SWPC_OK;
+}
+
+template <typename T>
+static int launch_recv(swpc3d_ctx *c, int it)
+{
+    if (c->nst == 0) return SWPC_OK;
+    int store = ((it - 1) % c->ntdec_w) == 0;
+    int itw = (it - 1) / c->ntdec_w;
+    if (store && itw >= c->ntw) store = 0;
+    K3<T> a = make_args<T>(c);
+    size_t off = (size_t)(store ? itw : 0) * c->nst * 3;
+    hipLaunchKernelGGL((k3_receivers<T>), dim3((c->nst + 63) / 64), dim3(64), 0, c->stream, a, c->nst,
+                       (const int *)c->st.p, (double *)c->uacc.p, (float *)c->wvel.p + off,
+                       (double *)c->wdisp.p + off, store);
+    SWPC_CHECK_LAUNCH();
+    return SWPC_OK;
+}
+
+template <typename T>
+static int launch_vmax(swpc3d_ctx *c)
+{
+    K3<T> a = make_args<T>(c);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, (unsigned *)c->vmax.p, 3, 0u);
+    SWPC_CHECK_LAUNCH();
+    int ni = c->d.nx - 2 * (c->d.na + 5), nj = c->d.ny - 2 * (c->d.na + 5);
+    if (ni > 0 && nj > 0) {
+        hipLaunchKernelGGL((k3_vmax<T>), dim3((ni + 255) / 256, nj), dim3(256), 0, c->stream, a,
+                           (const int *)c->kob.p, (unsigned *)c->vmax.p);
+        SWPC_CHECK_LAUNCH();
+    }
+    return SWPC_OK;
+}
+
+static bool fp64(const swpc3d_ctx *c) { return c->d.precision == SWPC_FP64; }
+
+static int ready(const swpc3d_ctx *c)
+{
+    if (!c) return fail(SWPC_EINVAL, "null context");
+    if (!c->have_medium || !c->have_surface || !c->have_pml)
+        return fail(SWPC_ESTATE, "set_medium, set_surface and set_pml must precede time stepping");
+    return SWPC_OK;
+}
+
+static int step(swpc3d_ctx *c, int it)
+{
+    int r;
+    if (fp64(c)) {
+        if ((r = launch_recv<double>(c, it))) return r;
+        if ((r = launch_stress<double>(c))) return r;
+        if ((r = launch_glut<double>(c, it))) return r;
+        if ((r = launch_vel<double>(c))) return r;
+        if ((r = launch_bf<double>(c, it))) return r;
+    } else {
+        if ((r = launch_recv<float>(c, it))) return r;
+        if ((r = launch_stress<float>(c))) return r;
+        if ((r = launch_glut<float>(c, it))) return r;
+        if ((r = launch_vel<float>(c))) return r;
+        if ((r = launch_bf<float>(c, it))) return r;
+    }
+    return SWPC_OK;
+}
+
+// Fortran (k,i,j) memory box -> device layout (row-blocked transposition).
+template <typename S, typename D>
+static void box_to_dev(const Lay3 &L, const S *src, D *dst)
+{
+    const int nkb = L.nz + 6, nib = L.nx + 6;
+    for (int j = -2; j <= L.ny + 3; j++)
+        for (int k0 = -2; k0 <= L.nz + 3; k0 += 16)
+            for (int i = -2; i <= L.nx + 3; i++) {
+                const S *s = src + (size_t)(k0 + 2) + (size_t)nkb * ((size_t)(i + 2) + (size_t)nib * (size_t)(j + 2));
+                int k1 = std::min(k0 + 15, L.nz + 3);
+                for (int k = k0; k <= k1; k++) dst[at3(L.px, L.sxy, i, j, k)] = (D)s[k - k0];
+            }
+}
+
+template <typename S, typename D>
+static void dev_to_box(const Lay3 &L, const S *src, D *dst)
+{
+    const int nkb = L.nz + 6, nib = L.nx + 6;
+    for (int j = -2; j <= L.ny + 3; j++)
+        for (int k0 = -2; k0 <= L.nz + 3; k0 += 16)
+            for (int i = -2; i <= L.nx + 3; i++) {
+                D *o = dst + (size_t)(k0 + 2) + (size_t)nkb * ((size_t)(i + 2) + (size_t)nib * (size_t)(j + 2));
+                int k1 = std::min(k0 + 15, L.nz + 3);
+                for (int k = k0; k <= k1; k++) o[k - k0] = (D)src[at3(L.px, L.sxy, i, j, k)];
+            }
+}
+
+static DBuf *field_buf(swpc3d_ctx *c, int f)
+{
+    if (f >= SWPC3D_VX && f <= SWPC3D_VZ) return &c->V[f];
+    if (f >= SWPC3D_SXX && f <= SWPC3D_SXY) return &c->S[f - SWPC3D_SXX];
+    if (f >= SWPC3D_R0 && f < SWPC3D_ADE0) return &c->R[f - SWPC3D_R0];
+    if (f >= SWPC3D_ADE0 && f < SWPC3D_NFIELD) return &c->A[f - SWPC3D_ADE0];
+    return nullptr;
+}
+
+static int upload_profile(swpc3d_ctx *c, DBuf &dst, const float *gc, const float *ge, int n)
+{
+    // host (4, 1:n) pairs -> device [8][n] in T
+    std::vector<double> h((size_t)8 * n);
+    for (int q = 0; q < 4; q++)
+        for (int i = 0; i < n; i++) {
+            h[(size_t)q * n + i] = gc[q + 4 * i];
+            h[(size_t)(4 + q) * n + i] = ge[q + 4 * i];
+        }
+    if (int r = dalloc(dst, (size_t)8 * n * c->esz, c->bytes)) return r;
+    if (fp64(c)) {
+        SWPC_HIP(hipMemcpy(dst.p, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(h.begin(), h.end());
+        SWPC_HIP(hipMemcpy(dst.p, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+    }
+    return SWPC_OK;
+}
+
+}  // namespace swpc
+
+using namespace swpc;
+
+extern "C" {
+
+int swpc3d_create(const swpc3d_desc *desc, swpc3d_ctx **out)
+{
+    clear_error();
+    if (!desc || !out) return fail(SWPC_EINVAL, "null argument");
+    *out = nullptr;
+    const swpc3d_desc &d = *desc;
+    if (d.nx < 8 || d.ny < 8 || d.nz < 8) return fail(SWPC_EINVAL, "grid %dx%dx%d too small", d.nx, d.ny, d.nz);
+    if (d.na < 0 || 2 * d.na >= d.nx || 2 * d.na >= d.ny || d.na >= d.nz)
+        return fail(SWPC_EINVAL, "absorber thickness na=%d does not fit the grid", d.na);
+    if (d.nm != NM3) return fail(SWPC_EINVAL, "nm=%d: the kernels are built for NM = 3", d.nm);
+    if (d.precision != SWPC_FP32 && d.precision != SWPC_FP64)
+        return fail(SWPC_EINVAL, "precision mode %d not supported by this build", d.precision);
+    if (!(d.dx > 0 && d.dy > 0 && d.dz > 0 && d.dt > 0)) return fail(SWPC_EINVAL, "non-positive dx/dy/dz/dt");
+    for (int m = 0; m < d.nm; m++)
+        if (!(d.ts[m] > 0)) return fail(SWPC_EINVAL, "ts(%d) must be positive", m + 1);
+    Lay3 L = make_lay3(d.nx, d.ny, d.nz);
+    if (L.vol >= (1LL << 31)) return fail(SWPC_EINVAL, "grid too large for 32-bit element offsets");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(SWPC_EHIP, "no HIP device available");
+
+    std::unique_ptr<swpc3d_ctx> c(new swpc3d_ctx());
+    c->d = d;
+    c->L = L;
+    c->esz = d.precision == SWPC_FP64 ? 8 : 4;
+    SWPC_HIP(hipStreamCreate(&c->stream));
+    const size_t vb = (size_t)L.vol * c->esz;
+    int r;
+    for (auto &b : c->V) if ((r = dalloc(b, vb, c->bytes))) return r;
+    for (auto &b : c->S) if ((r = dalloc(b, vb, c->bytes))) return r;
+    for (auto &b : c->R) if ((r = dalloc(b, vb, c->bytes))) return r;
+    for (auto &b : c->A) if ((r = dalloc(b, vb, c->bytes))) return r;
+    for (auto &b : c->med) if ((r = dalloc(b, (size_t)L.vol * 4, c->bytes))) return r;
+    if ((r = dalloc(c->surf, (size_t)d.nx * d.ny * 16, c->bytes))) return r;
+    if ((r = dalloc(c->kob, (size_t)d.nx * d.ny * 4, c->bytes))) return r;
+    if ((r = dalloc(c->vmax, 64, c->bytes))) return r;
+
+    // interior tiling: 64-wide tiles aligned to i = 1 + 64 t
+    const int ilo = d.na + 1, ihi = d.nx - d.na;
+    c->ti0 = 1 + ((ilo - 1) / 64) * 64;
+    c->ntx = ihi >= ilo ? (ihi - c->ti0) / 64 + 1 : 0;
+    const int nj = d.ny - 2 * d.na;
+    c->nty = nj > 0 ? (nj + BY3 - 1) / BY3 : 0;
+    const int nk = d.nz - d.na;
+    // z chunks: enough workgroups to fill the 256 CUs several times over
+    long long cols = (long long)c->ntx * c->nty;
+    int want = (int)std::max(1LL, std::min((long long)nk / 16, (4096 + cols - 1) / std::max(1LL, cols)));
+    c->kc = (nk + want - 1) / want;
+    c->nkc = (nk + c->kc - 1) / c->kc;
+
+    // absorber boxes (kbeg_a, m_global.f90:336-345)
+    Boxes3 &B = c->boxes;
+    memset(&B, 0, sizeof B);
+    auto add = [&](int i0, int i1, int j0, int j1, int k0, int k1) {
+        int ni = i1 - i0 + 1, nj = j1 - j0 + 1, nk = k1 - k0 + 1;
+        if (ni <= 0 || nj <= 0 || nk <= 0) return;
+        Box3 &x = B.b[B.n++];
+        x.i0 = i0; x.j0 = j0; x.k0 = k0; x.ni = ni; x.nj = nj; x.nk = nk;
+        x.start = B.total;
+        B.total += (unsigned)ni * (unsigned)nj * (unsigned)nk;
+    };
+    if (d.na > 0) {
+        add(1, d.na, 1, d.ny, 1, d.nz);
+        add(d.nx - d.na + 1, d.nx, 1, d.ny, 1, d.nz);
+        add(d.na + 1, d.nx - d.na, 1, d.na, 1, d.nz);
+        add(d.na + 1, d.nx - d.na, d.ny - d.na + 1, d.ny, 1, d.nz);
+        add(d.na + 1, d.nx - d.na, d.na + 1, d.ny - d.na, d.nz - d.na + 1, d.nz);
+    }
+    *out = c.release();
+    return SWPC_OK;
+}
+
+int swpc3d_destroy(swpc3d_ctx *c)
+{
+    if (!c) return SWPC_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &b : c->V) dfree(b);
+    for (auto &b : c->S) dfree(b);
+    for (auto &b : c->R) dfree(b);
+    for (auto &b : c->A) dfree(b);
+    for (auto &b : c->med) dfree(b);
+    DBuf *rest[] = {&c->surf, &c->kob, &c->gx, &c->gy, &c->gz, &c->src, &c->m6, &c->sdrop, &c->bsrc, &c->f3,
+                    &c->fdt, &c->st, &c->uacc, &c->wvel, &c->wdisp, &c->vmax};
+    for (auto b : rest) dfree(*b);
+    c->prof.reset();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SWPC_OK;
+}
+
+int swpc3d_set_medium(swpc3d_ctx *c, const float *bx, const float *by, const float *bz, const float *lam,
+                      const float *mu, const float *taup, const float *taus, const float *muyz, const float *muxz,
+                      const float *muxy)
+{
+    clear_error();
+    if (!c) return fail(SWPC_EINVAL, "null context");
+    const float *in[10] = {bx, by, bz, lam, mu, taup, taus, muyz, muxz, muxy};
+    for (int q = 0; q < 10; q++)
+        if (!in[q]) return fail(SWPC_EINVAL, "medium array %d is null", q);
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    std::vector<float> h((size_t)c->L.vol, 0.0f);
+    for (int q = 0; q < 10; q++) {
+        box_to_dev(c->L, in[q], h.data());
+        SWPC_HIP(hipMemcpy(c->med[q].p, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+    }
+    c->have_medium = true;
+    return SWPC_OK;
+}
+
+int swpc3d_set_surface(swpc3d_ctx *c, const int *kfs_top, const int *kfs_bot, const int *kob_top, const int *kob_bot,
+                       const int *kob)
+{
+    clear_error();
+    if (!c || !kfs_top || !kfs_bot || !kob_top || !kob_bot || !kob) return fail(SWPC_EINVAL, "null argument");
+    const int nx = c->d.nx, ny = c->d.ny, nib = nx + 6;
+    std::vector<int> s4((size_t)nx * ny * 4), kb((size_t)nx * ny);
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+            size_t h = (size_t)(i + 2) + (size_t)nib * (size_t)(j + 2), o = (size_t)(j - 1) * nx + (i - 1);
+            s4[4 * o + 0] = kfs_top[h];
+            s4[4 * o + 1] = kfs_bot[h];
+            s4[4 * o + 2] = kob_top[h];
+            s4[4 * o + 3] = kob_bot[h];
+            kb[o] = kob[h];
+            if (kob[h] + 1 < -2 || kob[h] + 1 > c->d.nz + 3)
+                return fail(SWPC_EINVAL, "kob(%d,%d)=%d outside the memory box", i, j, kob[h]);
+        }
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    SWPC_HIP(hipMemcpy(c->surf.p, s4.data(), s4.size() * 4, hipMemcpyHostToDevice));
+    SWPC_HIP(hipMemcpy(c->kob.p, kb.data(), kb.size() * 4, hipMemcpyHostToDevice));
+    c->have_surface = true;
+    return SWPC_OK;
+}
+
+int swpc3d_set_pml(swpc3d_ctx *c, const float *gxc, const float *gxe, const float *gyc, const float *gye,
+                   const float *gzc, const float *gze)
+{
+    clear_error();
+    if (!c || !gxc || !gxe || !gyc || !gye || !gzc || !gze) return fail(SWPC_EINVAL, "null argument");
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    int r;
+    if ((r = upload_profile(c, c->gx, gxc, gxe, c->d.nx))) return r;
+    if ((r = upload_profile(c, c->gy, gyc, gye, c->d.ny))) return r;
+    if ((r = upload_profile(c, c->gz, gzc, gze, c->d.nz))) return r;
+    c->have_pml = true;
+    return SWPC_OK;
+}
+
+static int check_src(swpc3d_ctx *c, int nsrc, const int *is, const int *js, const int *ks, std::vector<int> &pack)
+{
+    pack.resize((size_t)3 * nsrc);
+    for (int n = 0; n < nsrc; n++) {
+        // the stress glut touches i-1, j-1, k-1; the reference accepts
+        // sources over [beg-2, end+3] (m_source.f90:233-235)
+        if (is[n] < -1 || is[n] > c->d.nx + 3 || js[n] < -1 || js[n] > c->d.ny + 3 || ks[n] < -1 ||
+            ks[n] > c->d.nz + 3)
+            return fail(SWPC_EINVAL, "source %d at (%d,%d,%d) outside the memory box", n + 1, is[n], js[n], ks[n]);
+        pack[3 * n] = is[n];
+        pack[3 * n + 1] = js[n];
+        pack[3 * n + 2] = ks[n];
+    }
+    return SWPC_OK;
+}
+
+int swpc3d_set_sources(swpc3d_ctx *c, int nsrc, const int *isrc, const int *jsrc, const int *ksrc, const double *m6,
+                       int nt, const double *sdrop)
+{
+    clear_error();
+    if (!c || nsrc < 0 || nt < 0) return fail(SWPC_EINVAL, "bad argument");
+    if (nsrc > 0 && (!isrc || !jsrc || !ksrc || !m6 || !sdrop || nt < 1)) return fail(SWPC_EINVAL, "null source array");
+    std::vector<int> pack;
+    if (int r = check_src(c, nsrc, isrc, jsrc, ksrc, pack)) return r;
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    int r;
+    if ((r = dalloc(c->src, pack.size() * 4, c->bytes))) return r;
+    if ((r = dalloc(c->m6, (size_t)6 * nsrc * 8, c->bytes))) return r;
+    if ((r = dalloc(c->sdrop, (size_t)nt * nsrc * 8, c->bytes))) return r;
+    if (nsrc > 0) {
+        SWPC_HIP(hipMemcpy(c->src.p, pack.data(), pack.size() * 4, hipMemcpyHostToDevice));
+        SWPC_HIP(hipMemcpy(c->m6.p, m6, (size_t)6 * nsrc * 8, hipMemcpyHostToDevice));
+        SWPC_HIP(hipMemcpy(c->sdrop.p, sdrop, (size_t)nt * nsrc * 8, hipMemcpyHostToDevice));
+    }
+    c->nsrc = nsrc;
+    c->nt_src = nt;
+    return SWPC_OK;
+}
+
+int swpc3d_set_bodyforce(swpc3d_ctx *c, int nsrc, const int *isrc, const int *jsrc, const int *ksrc, const double *f3,
+                         int nt, const double *fdt)
+{
+    clear_error();
+    if (!c || nsrc < 0 || nt < 0) return fail(SWPC_EINVAL, "bad argument");
+    if (nsrc > 0 && (!isrc || !jsrc || !ksrc || !f3 || !fdt || nt < 1)) return fail(SWPC_EINVAL, "null source array");
+    std::vector<int> pack;
+    if (int r = check_src(c, nsrc, isrc, jsrc, ksrc, pack)) return r;
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    int r;
+    if ((r = dalloc(c->bsrc, pack.size() * 4, c->bytes))) return r;
+    if ((r = dalloc(c->f3, (size_t)3 * nsrc * 8, c->bytes))) return r;
+    if ((r = dalloc(c->fdt, (size_t)nt * nsrc * 8, c->bytes))) return r;
+    if (nsrc > 0) {
+        SWPC_HIP(hipMemcpy(c->bsrc.p, pack.data(), pack.size() * 4, hipMemcpyHostToDevice));
+        SWPC_HIP(hipMemcpy(c->f3.p, f3, (size_t)3 * nsrc * 8, hipMemcpyHostToDevice));
+        SWPC_HIP(hipMemcpy(c->fdt.p, fdt, (size_t)nt * nsrc * 8, hipMemcpyHostToDevice));
+    }
+    c->nbf = nsrc;
+    c->nt_bf = nt;
+    return SWPC_OK;
+}
+
+int swpc3d_set_receivers(swpc3d_ctx *c, int nst, const int *ist, const int *jst, const int *kst, int ntdec_w, int ntw)
+{
+    clear_error();
+    if (!c || nst < 0 || ntdec_w < 1 || ntw < 0) return fail(SWPC_EINVAL, "bad argument");
+    if (nst > 0 && (!ist || !jst || !kst)) return fail(SWPC_EINVAL, "null station array");
+    std::vector<int> pack((size_t)3 * nst);
+    for (int n = 0; n < nst; n++) {
+        if (ist[n] < 1 || ist[n] > c->d.nx || jst[n] < 1 || jst[n] > c->d.ny || kst[n] < 1 || kst[n] > c->d.nz)
+            return fail(SWPC_EINVAL, "station %d at (%d,%d,%d) outside the grid", n + 1, ist[n], jst[n], kst[n]);
+        pack[3 * n] = ist[n];
+        pack[3 * n + 1] = jst[n];
+        pack[3 * n + 2] = kst[n];
+    }
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    int r;
+    if ((r = dalloc(c->st, pack.size() * 4, c->bytes))) return r;
+    if ((r = dalloc(c->uacc, (size_t)3 * nst * 8, c->bytes))) return r;
+    if ((r = dalloc(c->wvel, (size_t)3 * nst * std::max(ntw, 1) * 4, c->bytes))) return r;
+    if ((r = dalloc(c->wdisp, (size_t)3 * nst * std::max(ntw, 1) * 8, c->bytes))) return r;
+    if (nst > 0) SWPC_HIP(hipMemcpy(c->st.p, pack.data(), pack.size() * 4, hipMemcpyHostToDevice));
+    c->nst = nst;
+    c->ntdec_w = ntdec_w;
+    c->ntw = ntw;
+    return SWPC_OK;
+}
+
+int swpc3d_update_stress(swpc3d_ctx *c)
+{
+    clear_error();
+    if (int r = ready(c)) return r;
+    return fp64(c) ? launch_stress<double>(c) : launch_stress<float>(c);
+}
+
+int swpc3d_update_vel(swpc3d_ctx *c)
+{
+    clear_error();
+    if (int r = ready(c)) return r;
+    return fp64(c) ? launch_vel<double>(c) : launch_vel<float>(c);
+}
+
+int swpc3d_stressglut(swpc3d_ctx *c, int it)
+{
+    clear_error();
+    if (int r = ready(c)) return r;
+    return fp64(c) ? launch_glut<double>(c, it) : launch_glut<float>(c, it);
+}
+
+int swpc3d_bodyforce(swpc3d_ctx *c, int it)
+{
+    clear_error();
+    if (int r = ready(c)) return r;
+    return fp64(c) ? launch_bf<double>(c, it) : launch_bf<float>(c, it);
+}
+
+int swpc3d_store_receivers(swpc3d_ctx *c, int it)
+{
+    clear_error();
+    if (int r = ready(c)) return r;
+    if (it < 1) return fail(SWPC_EINVAL, "it must be >= 1");
+    return fp64(c) ? launch_recv<double>(c, it) : launch_recv<float>(c, it);
+}
+
+int swpc3d_step(swpc3d_ctx *c, int it)
+{
+    clear_error();
+    if (int r = ready(c)) return r;
+    if (it < 1) return fail(SWPC_EINVAL, "it must be >= 1");
+    return step(c, it);
+}
+
+int swpc3d_run(swpc3d_ctx *c, int it0, int nsteps)
+{
+    clear_error();
+    if (int r = ready(c)) return r;
+    if (it0 < 1 || nsteps < 0) return fail(SWPC_EINVAL, "bad step range");
+    for (int it = it0; it < it0 + nsteps; it++)
+        if (int r = step(c, it)) return r;
+    return SWPC_OK;
+}
+
+int swpc3d_get_receivers(swpc3d_ctx *c, float *vel, double *disp)
+{
+    clear_error();
+    if (!c) return fail(SWPC_EINVAL, "null context");
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    size_t n = (size_t)3 * c->nst * c->ntw;
+    if (n == 0) return SWPC_OK;
+    if (vel) SWPC_HIP(hipMemcpy(vel, c->wvel.p, n * 4, hipMemcpyDeviceToHost));
+    if (disp) SWPC_HIP(hipMemcpy(disp, c->wdisp.p, n * 8, hipMemcpyDeviceToHost));
+    return SWPC_OK;
+}
+
+int swpc3d_get_field(swpc3d_ctx *c, int f, double *out)
+{
+    clear_error();
+    if (!c || !out) return fail(SWPC_EINVAL, "null argument");
+    DBuf *b = field_buf(c, f);
+    if (!b) return fail(SWPC_EINVAL, "unknown field %d", f);
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    if (fp64(c)) {
+        std::vector<double> h((size_t)c->L.vol);
+        SWPC_HIP(hipMemcpy(h.data(), b->p, h.size() * 8, hipMemcpyDeviceToHost));
+        dev_to_box(c->L, h.data(), out);
+    } else {
+        std::vector<float> h((size_t)c->L.vol);
+        SWPC_HIP(hipMemcpy(h.data(), b->p, h.size() * 4, hipMemcpyDeviceToHost));
+        dev_to_box(c->L, h.data(), out);
+    }
+    return SWPC_OK;
+}
+
+int swpc3d_set_field(swpc3d_ctx *c, int f, const double *in)
+{
+    clear_error();
+    if (!c || !in) return fail(SWPC_EINVAL, "null argument");
+    DBuf *b = field_buf(c, f);
+    if (!b) return fail(SWPC_EINVAL, "unknown field %d", f);
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    if (fp64(c)) {
+        std::vector<double> h((size_t)c->L.vol, 0.0);
+        box_to_dev(c->L, in, h.data());
+        SWPC_HIP(hipMemcpy(b->p, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> h((size_t)c->L.vol, 0.0f);
+        box_to_dev(c->L, in, h.data());
+        SWPC_HIP(hipMemcpy(b->p, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+    }
+    return SWPC_OK;
+}
+
+int swpc3d_vmax(swpc3d_ctx *c, float *out3)
+{
+    clear_error();
+    if (int r = ready(c)) return r;
+    if (!out3) return fail(SWPC_EINVAL, "null argument");
+    int r = fp64(c) ? launch_vmax<double>(c) : launch_vmax<float>(c);
+    if (r) return r;
+    unsigned u[3];
+    SWPC_HIP(hipMemcpyAsync(u, c->vmax.p, 12, hipMemcpyDeviceToHost, c->stream));
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    for (int q = 0; q < 3; q++) memcpy(&out3[q], &u[q], 4);
+    return SWPC_OK;
+}
+
+int swpc3d_sync(swpc3d_ctx *c)
+{
+    clear_error();
+    if (!c) return fail(SWPC_EINVAL, "null context");
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    return SWPC_OK;
+}
+
+int swpc3d_profile(swpc3d_ctx *c, int enable)
+{
+    clear_error();
+    if (!c) return fail(SWPC_EINVAL, "null context");
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    c->prof.reset();
+    c->prof.on = enable != 0;
+    return SWPC_OK;
+}
+
+int swpc3d_profile_read(swpc3d_ctx *c, double *ms5, long long *count5)
+{
+    clear_error();
+    if (!c) return fail(SWPC_EINVAL, "null context");
+    return prof_read(c->prof, ms5, count5);
+}
+
+int swpc3d_memory(swpc3d_ctx *c, long long *bytes)
+{
+    if (!c || !bytes) return fail(SWPC_EINVAL, "null argument");
+    *bytes = c->bytes;
+    return SWPC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,113 @@
+// Shared host/device helpers of the swpc HIP library: error reporting,
+// device layout and host<->device transposition of the reference's
+// Fortran-ordered arrays.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/swpc_hip.h"
+
+namespace swpc {
+
+int fail(int code, const char *fmt, ...);
+void clear_error();
+
+#define SWPC_HIP(x)                                                                                   \
+    do {                                                                                              \
+        hipError_t e_ = (x);                                                                          \
+        if (e_ != hipSuccess)                                                                         \
+            return ::swpc::fail(SWPC_EHIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_),        \
+                                __FILE__, __LINE__);                                                  \
+    } while (0)
+
+#define SWPC_CHECK_LAUNCH()                                                                           \
+    do {                                                                                              \
+        hipError_t e_ = hipGetLastError();                                                            \
+        if (e_ != hipSuccess)                                                                         \
+            return ::swpc::fail(SWPC_EHIP, "kernel launch failed: %s (%s:%d)", hipGetErrorString(e_), \
+                                __FILE__, __LINE__);                                                  \
+    } while (0)
+
+// Device layout.  Cells carry the reference's 1-based indices; x (i) is
+// the fastest axis.  Cell i = 1 sits at element offset IO + 1 = 32 of its
+// row so that rows of 64 interior cells start on a 128-byte boundary for
+// FP32.  The memory box of the reference, [-2, n+3] per axis, is kept so
+// that the ghost cells that replace the MPI halo stay zero in HBM.
+constexpr int IO = 31;  // element offset of i = 0
+constexpr int JO = 2;   // row offset of j = 0 (j = -2 is row 0)
+constexpr int KO = 2;   // plane offset of k = 0
+
+struct Lay3 {
+    int nx, ny, nz;
+    int px;       // elements per row
+    int py;       // rows per plane (ny + 6)
+    int pz;       // planes (nz + 6)
+    int sxy;      // px * py
+    long long vol;
+};
+
+inline Lay3 make_lay3(int nx, int ny, int nz)
+{
+    Lay3 L;
+    L.nx = nx; L.ny = ny; L.nz = nz;
+    int need = IO + nx + 4;                // i = nx + 3 is the last element
+    L.px = (need + 31) / 32 * 32;
+    L.py = ny + 6;
+    L.pz = nz + 6;
+    L.sxy = L.px * L.py;
+    L.vol = (long long)L.sxy * L.pz;
+    return L;
+}
+
+__host__ __device__ inline int at3(int px, int sxy, int i, int j, int k)
+{
+    return (k + KO) * sxy + (j + JO) * px + (i + IO);
+}
+
+struct Lay2 {
+    int nx, nz;
+    int px;       // elements per row (x)
+    int pz;       // rows (nz + 6)
+    long long vol;
+};
+
+inline Lay2 make_lay2(int nx, int nz)
+{
+    Lay2 L;
+    L.nx = nx; L.nz = nz;
+    int need = IO + nx + 4;
+    L.px = (need + 31) / 32 * 32;
+    L.pz = nz + 6;
+    L.vol = (long long)L.px * L.pz;
+    return L;
+}
+
+__host__ __device__ inline int at2(int px, int i, int k) { return (k + KO) * px + (i + IO); }
+
+// Device buffer owned by a context.
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+int dalloc(DBuf &b, size_t bytes, long long &total);
+void dfree(DBuf &b);
+
+// Event-pair timing per kernel class.
+struct Prof {
+    bool on = false;
+    std::vector<hipEvent_t> ev[5];
+    void reset();
+    ~Prof() { reset(); }
+};
+
+int prof_begin(Prof &p, int cls, hipStream_t s, hipEvent_t *out);
+int prof_end(Prof &p, int cls, hipStream_t s, hipEvent_t start);
+int prof_read(Prof &p, double *ms5, long long *count5);
+
+}  // namespace swpc

@@ -1,0 +1,107 @@
+// Error reporting, allocation and event timing shared by the 3-D and
+// P-SV contexts.
+#include "swpc_common.h"
+
+namespace swpc {
+
+static thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+void clear_error() { g_err.clear(); }
+
+int dalloc(DBuf &b, size_t bytes, long long &total)
+{
+    dfree(b);
+    if (bytes == 0) return SWPC_OK;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(SWPC_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    e = hipMemset(b.p, 0, bytes);
+    if (e != hipSuccess) return fail(SWPC_EHIP, "hipMemset failed: %s", hipGetErrorString(e));
+    b.bytes = bytes;
+    total += (long long)bytes;
+    return SWPC_OK;
+}
+
+void dfree(DBuf &b)
+{
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+void Prof::reset()
+{
+    for (auto &v : ev) {
+        for (auto e : v) (void)hipEventDestroy(e);
+        v.clear();
+    }
+}
+
+int prof_begin(Prof &p, int cls, hipStream_t s, hipEvent_t *out)
+{
+    *out = nullptr;
+    if (!p.on) return SWPC_OK;
+    hipEvent_t e;
+    SWPC_HIP(hipEventCreate(&e));
+    SWPC_HIP(hipEventRecord(e, s));
+    p.ev[cls].push_back(e);
+    *out = e;
+    return SWPC_OK;
+}
+
+int prof_end(Prof &p, int cls, hipStream_t s, hipEvent_t start)
+{
+    if (!p.on || !start) return SWPC_OK;
+    hipEvent_t e;
+    SWPC_HIP(hipEventCreate(&e));
+    SWPC_HIP(hipEventRecord(e, s));
+    p.ev[cls].push_back(e);
+    return SWPC_OK;
+}
+
+int prof_read(Prof &p, double *ms5, long long *count5)
+{
+    for (int c = 0; c < 5; c++) {
+        double sum = 0;
+        auto &v = p.ev[c];
+        for (size_t n = 0; n + 1 < v.size(); n += 2) {
+            SWPC_HIP(hipEventSynchronize(v[n + 1]));
+            float ms = 0;
+            SWPC_HIP(hipEventElapsedTime(&ms, v[n], v[n + 1]));
+            sum += ms;
+        }
+        if (ms5) ms5[c] = sum;
+        if (count5) count5[c] = (long long)(v.size() / 2);
+    }
+    return SWPC_OK;
+}
+
+}  // namespace swpc
+
+extern "C" {
+
+const char *swpc_last_error(void) { return swpc::g_err.c_str(); }
+
+int swpc_version(void) { return 10000; }
+
+int swpc_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+}  // extern "C"

@@ -1,0 +1,299 @@
+"""Host-side mirror of the reference's hot-path module interface, driving the
+HIP kernels through the C ABI.
+
+``Swpc3D`` / ``SwpcPSV`` own one device context each and expose the
+reference's per-step routines under their reference names
+(``kernel__update_stress``, ``absorb__update_stress`` folded in,
+``source__stressglut``, ``kernel__update_vel``, ``source__bodyforce``,
+``wav__store``, ``kernel__vmax``) plus ``step``/``run`` for whole steps in
+the order of src/swpc_3d/main.f90:125-144 (P-SV: src/swpc_psv/main.f90:95-114).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib, ptr
+
+UC3 = 1e-15   # src/swpc_3d/m_global.f90:29
+UC2 = 1e-12   # src/swpc_psv/m_global.f90:29
+
+FIELDS3 = ["Vx", "Vy", "Vz", "Sxx", "Syy", "Szz", "Syz", "Sxz", "Sxy"] + \
+    [f"R{c}{m}" for c in ("xx", "yy", "zz", "yz", "xz", "xy") for m in (1, 2, 3)] + \
+    ["axVx", "ayVx", "azVx", "axVy", "ayVy", "azVy", "axVz", "ayVz", "azVz",
+     "axSxx", "aySxy", "azSxz", "axSxy", "aySyy", "azSyz", "axSxz", "aySyz", "azSzz"]
+FIELDS2 = ["Vx", "Vz", "Sxx", "Szz", "Sxz"] + \
+    [f"R{c}{m}" for c in ("xx", "zz", "xz") for m in (1, 2, 3)] + \
+    ["axVx", "azVx", "axVz", "azVz", "axSxx", "azSxz", "axSxz", "azSzz"]
+
+PRECISION = {"fp32": _lib.SWPC_FP32, "fp64": _lib.SWPC_FP64, "fp32_s16": _lib.SWPC_FP32_S16}
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Swpc3D:
+    """One 3-D simulation resident on the GPU."""
+
+    def __init__(self, grid, medium, pml, glut=None, stations=None, ntdec_w=1, nt=None,
+                 precision="fp32", bodyforce=None):
+        self.g = grid
+        self.nt = nt
+        self.precision = precision
+        L = lib()
+        d = _lib.Desc3()
+        d.nx, d.ny, d.nz, d.na, d.nm = grid.nx, grid.ny, grid.nz, grid.na, 3
+        d.precision = PRECISION[precision]
+        d.dx, d.dy, d.dz, d.dt = grid.dx, grid.dy, grid.dz, grid.dt
+        for m in range(3):
+            d.ts[m] = float(medium.ts[m])
+        h = C.c_void_p()
+        check(L.swpc3d_create(C.byref(d), C.byref(h)), "swpc3d_create")
+        self.h = h
+        m = medium
+        self._keep = [_f32(a) for a in (m.bx, m.by, m.bz, m.lam, m.mu, m.taup, m.taus, m.muyz, m.muxz, m.muxy)]
+        check(L.swpc3d_set_medium(h, *[ptr(a, C.c_float) for a in self._keep]), "set_medium")
+        s = [_i32(a) for a in (m.kfs_top, m.kfs_bot, m.kob_top, m.kob_bot, m.kob)]
+        check(L.swpc3d_set_surface(h, *[ptr(a, C.c_int) for a in s]), "set_surface")
+        self._keep = None
+        gp = [_f32(a) for a in pml]
+        check(L.swpc3d_set_pml(h, *[ptr(a, C.c_float) for a in gp]), "set_pml")
+        self.M0 = 1.0
+        if glut is not None and len(glut.idx):
+            idx = _i32(glut.idx)
+            i, j, k = _i32(idx[:, 0]), _i32(idx[:, 1]), _i32(idx[:, 2])
+            tab = _f64(glut.table)
+            check(L.swpc3d_set_sources(h, len(idx), ptr(i, C.c_int), ptr(j, C.c_int), ptr(k, C.c_int),
+                                       ptr(_f64(glut.m), C.c_double), tab.shape[0], ptr(tab, C.c_double)),
+                  "set_sources")
+            self.M0 = glut.M0
+        if bodyforce is not None and len(bodyforce.idx):
+            idx = _i32(bodyforce.idx)
+            i, j, k = _i32(idx[:, 0]), _i32(idx[:, 1]), _i32(idx[:, 2])
+            tab = _f64(bodyforce.table)
+            check(L.swpc3d_set_bodyforce(h, len(idx), ptr(i, C.c_int), ptr(j, C.c_int), ptr(k, C.c_int),
+                                         ptr(_f64(bodyforce.m), C.c_double), tab.shape[0], ptr(tab, C.c_double)),
+                  "set_bodyforce")
+            self.M0 = bodyforce.M0
+        self.nst = 0
+        self.ntw = 0
+        self.ntdec_w = ntdec_w
+        if stations is not None and len(stations):
+            st = _i32(stations)
+            from .source import ntw_of
+            self.nst = len(st)
+            self.ntw = ntw_of(nt, ntdec_w)
+            check(L.swpc3d_set_receivers(h, self.nst, ptr(_i32(st[:, 0]), C.c_int), ptr(_i32(st[:, 1]), C.c_int),
+                                         ptr(_i32(st[:, 2]), C.c_int), ntdec_w, self.ntw), "set_receivers")
+
+    # -- reference routine names ------------------------------------------
+    def kernel__update_stress(self):
+        check(lib().swpc3d_update_stress(self.h), "update_stress")
+
+    def kernel__update_vel(self):
+        check(lib().swpc3d_update_vel(self.h), "update_vel")
+
+    def source__stressglut(self, it):
+        check(lib().swpc3d_stressglut(self.h, it), "stressglut")
+
+    def source__bodyforce(self, it):
+        check(lib().swpc3d_bodyforce(self.h, it), "bodyforce")
+
+    def wav__store(self, it):
+        check(lib().swpc3d_store_receivers(self.h, it), "store_receivers")
+
+    def kernel__vmax(self):
+        out = np.zeros(3, np.float32)
+        check(lib().swpc3d_vmax(self.h, ptr(out, C.c_float)), "vmax")
+        return out
+
+    # -- whole steps ------------------------------------------------------
+    def step(self, it):
+        check(lib().swpc3d_step(self.h, it), "step")
+
+    def run(self, it0, nsteps):
+        check(lib().swpc3d_run(self.h, it0, nsteps), "run")
+
+    def sync(self):
+        check(lib().swpc3d_sync(self.h), "sync")
+
+    # -- gathers ------------------------------------------------------------
+    def receivers(self, scaled=False):
+        """(vel[ntw, nst, 3] float32, disp[ntw, nst, 3] float64); scaled=True
+        multiplies by M0*UC*1e9 (the reference's nm/s and nm)."""
+        vel = np.zeros((self.ntw, self.nst, 3), np.float32)
+        disp = np.zeros((self.ntw, self.nst, 3), np.float64)
+        if self.nst:
+            check(lib().swpc3d_get_receivers(self.h, ptr(vel, C.c_float), ptr(disp, C.c_double)), "get_receivers")
+        if scaled:
+            s = self.M0 * UC3 * 1e9
+            return (vel * np.float32(s)).astype(np.float32), disp * s
+        return vel, disp
+
+    def field(self, name):
+        out = np.zeros(self.g.shape, np.float64)
+        check(lib().swpc3d_get_field(self.h, FIELDS3.index(name), ptr(out, C.c_double)), "get_field")
+        return out
+
+    def set_field(self, name, a):
+        a = _f64(a)
+        assert a.shape == self.g.shape
+        check(lib().swpc3d_set_field(self.h, FIELDS3.index(name), ptr(a, C.c_double)), "set_field")
+
+    def profile(self, enable=True):
+        check(lib().swpc3d_profile(self.h, 1 if enable else 0), "profile")
+
+    def profile_read(self):
+        ms = np.zeros(5, np.float64)
+        cnt = np.zeros(5, np.int64)
+        check(lib().swpc3d_profile_read(self.h, ptr(ms, C.c_double), ptr(cnt, C.c_longlong)), "profile_read")
+        return ms, cnt
+
+    def memory(self):
+        b = C.c_longlong()
+        check(lib().swpc3d_memory(self.h, C.byref(b)), "memory")
+        return b.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().swpc3d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class SwpcPSV:
+    """One 2-D P-SV simulation resident on the GPU."""
+
+    def __init__(self, grid, medium, pml, glut=None, stations=None, ntdec_w=1, nt=None,
+                 precision="fp32", bodyforce=None):
+        self.g = grid
+        self.nt = nt
+        L = lib()
+        d = _lib.Desc2()
+        d.nx, d.nz, d.na, d.nm = grid.nx, grid.nz, grid.na, 3
+        d.precision = PRECISION[precision]
+        d.dx, d.dz, d.dt = grid.dx, grid.dz, grid.dt
+        for m in range(3):
+            d.ts[m] = float(medium.ts[m])
+        h = C.c_void_p()
+        check(L.swpc2d_create(C.byref(d), C.byref(h)), "swpc2d_create")
+        self.h = h
+        m = medium
+        arrs = [_f32(a) for a in (m.rho, m.lam, m.mu, m.taup, m.taus)]
+        check(L.swpc2d_set_medium(h, *[ptr(a, C.c_float) for a in arrs]), "set_medium")
+        s = [_i32(a) for a in (m.kfs_top, m.kfs_bot, m.kob_top, m.kob_bot, m.kob)]
+        check(L.swpc2d_set_surface(h, *[ptr(a, C.c_int) for a in s]), "set_surface")
+        gp = [_f32(a) for a in pml]
+        check(L.swpc2d_set_pml(h, *[ptr(a, C.c_float) for a in gp]), "set_pml")
+        self.M0 = 1.0
+        if glut is not None and len(glut.idx):
+            idx = _i32(glut.idx)
+            tab = _f64(glut.table)
+            check(L.swpc2d_set_sources(h, len(idx), ptr(_i32(idx[:, 0]), C.c_int), ptr(_i32(idx[:, 1]), C.c_int),
+                                       ptr(_f64(glut.m), C.c_double), tab.shape[0], ptr(tab, C.c_double)),
+                  "set_sources")
+            self.M0 = glut.M0
+        if bodyforce is not None and len(bodyforce.idx):
+            idx = _i32(bodyforce.idx)
+            tab = _f64(bodyforce.table)
+            check(L.swpc2d_set_bodyforce(h, len(idx), ptr(_i32(idx[:, 0]), C.c_int), ptr(_i32(idx[:, 1]), C.c_int),
+                                         ptr(_f64(bodyforce.m), C.c_double), tab.shape[0], ptr(tab, C.c_double)),
+                  "set_bodyforce")
+            self.M0 = bodyforce.M0
+        self.nst = 0
+        self.ntw = 0
+        if stations is not None and len(stations):
+            st = _i32(stations)
+            from .source import ntw_of
+            self.nst = len(st)
+            self.ntw = ntw_of(nt, ntdec_w)
+            check(L.swpc2d_set_receivers(h, self.nst, ptr(_i32(st[:, 0]), C.c_int), ptr(_i32(st[:, 1]), C.c_int),
+                                         ntdec_w, self.ntw), "set_receivers")
+
+    def kernel__update_stress(self):
+        check(lib().swpc2d_update_stress(self.h), "update_stress")
+
+    def kernel__update_vel(self):
+        check(lib().swpc2d_update_vel(self.h), "update_vel")
+
+    def source__stressglut(self, it):
+        check(lib().swpc2d_stressglut(self.h, it), "stressglut")
+
+    def source__bodyforce(self, it):
+        check(lib().swpc2d_bodyforce(self.h, it), "bodyforce")
+
+    def wav__store(self, it):
+        check(lib().swpc2d_store_receivers(self.h, it), "store_receivers")
+
+    def kernel__vmax(self):
+        out = np.zeros(2, np.float32)
+        check(lib().swpc2d_vmax(self.h, ptr(out, C.c_float)), "vmax")
+        return out
+
+    def step(self, it):
+        check(lib().swpc2d_step(self.h, it), "step")
+
+    def run(self, it0, nsteps):
+        check(lib().swpc2d_run(self.h, it0, nsteps), "run")
+
+    def sync(self):
+        check(lib().swpc2d_sync(self.h), "sync")
+
+    def receivers(self, scaled=False):
+        vel = np.zeros((self.ntw, self.nst, 2), np.float32)
+        disp = np.zeros((self.ntw, self.nst, 2), np.float64)
+        if self.nst:
+            check(lib().swpc2d_get_receivers(self.h, ptr(vel, C.c_float), ptr(disp, C.c_double)), "get_receivers")
+        if scaled:
+            s = self.M0 * UC2 * 1e9
+            return (vel * np.float32(s)).astype(np.float32), disp * s
+        return vel, disp
+
+    def field(self, name):
+        out = np.zeros(self.g.shape, np.float64)
+        check(lib().swpc2d_get_field(self.h, FIELDS2.index(name), ptr(out, C.c_double)), "get_field")
+        return out
+
+    def set_field(self, name, a):
+        a = _f64(a)
+        assert a.shape == self.g.shape
+        check(lib().swpc2d_set_field(self.h, FIELDS2.index(name), ptr(a, C.c_double)), "set_field")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().swpc2d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

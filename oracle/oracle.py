@@ -1,0 +1,322 @@
+"""ctypes harness of the CPU oracle (oracle/swpc_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke()
+as the checker, and by bench.py's cpu_baseline leg.  Never by the product.
+
+``Oracle3D`` / ``Oracle2D`` own numpy arrays in the reference's Fortran
+layout (memory box, k fastest) and advance them with the C restatement of
+the reference loops in FP32 (liboracle32.so) or FP64 (liboracle64.so).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+_libs = {}
+
+
+def olib(bits=32):
+    if bits not in _libs:
+        p = HERE / f"liboracle{bits}.so"
+        if not p.exists():
+            raise RuntimeError(f"{p} missing: run `make -C oracle`")
+        _libs[bits] = C.CDLL(str(p))
+    return _libs[bits]
+
+
+def ref_lib():
+    """oracle/_ref/libswpcref.so (reference Fortran modules) or None."""
+    p = HERE / "_ref" / "libswpcref.so"
+    if not p.exists():
+        return None
+    if "ref" not in _libs:
+        _libs["ref"] = C.CDLL(str(p))
+    return _libs["ref"]
+
+
+_P = C.c_void_p
+_PF = C.POINTER(C.c_float)
+_PI = C.POINTER(C.c_int)
+
+
+def _o3d_struct(real):
+    PR = C.POINTER(real)
+
+    class O3D(C.Structure):
+        _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("na", C.c_int), ("nm", C.c_int),
+                    ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("dt", C.c_float)] + \
+                   [(n, PR) for n in ("Vx", "Vy", "Vz", "Sxx", "Syy", "Szz", "Syz", "Sxz", "Sxy")] + \
+                   [(n, PR) for n in ("Rxx", "Ryy", "Rzz", "Ryz", "Rxz", "Rxy")] + \
+                   [(n, _PF) for n in ("bx", "by", "bz", "lam", "mu", "taup", "taus", "muyz", "muxz", "muxy")] + \
+                   [(n, _PI) for n in ("kfs_top", "kfs_bot", "kob_top", "kob_bot", "kob")] + \
+                   [(n, _PF) for n in ("gxc", "gxe", "gyc", "gye", "gzc", "gze")] + \
+                   [("ade", PR * 18), ("ts", _PF)]
+    return O3D
+
+
+def _o2d_struct(real):
+    PR = C.POINTER(real)
+
+    class O2D(C.Structure):
+        _fields_ = [("nx", C.c_int), ("nz", C.c_int), ("na", C.c_int), ("nm", C.c_int),
+                    ("dx", C.c_double), ("dz", C.c_double), ("dt", C.c_float)] + \
+                   [(n, PR) for n in ("Vx", "Vz", "Sxx", "Szz", "Sxz")] + \
+                   [(n, PR) for n in ("Rxx", "Rzz", "Rxz")] + \
+                   [(n, _PF) for n in ("rho", "lam", "mu", "taup", "taus")] + \
+                   [(n, _PI) for n in ("kfs_top", "kfs_bot", "kob_top", "kob_bot", "kob")] + \
+                   [(n, _PF) for n in ("gxc", "gxe", "gzc", "gze")] + \
+                   [("ade", PR * 8), ("ts", _PF)]
+    return O2D
+
+
+FIELDS3 = ["Vx", "Vy", "Vz", "Sxx", "Syy", "Szz", "Syz", "Sxz", "Sxy"]
+R3 = ["Rxx", "Ryy", "Rzz", "Ryz", "Rxz", "Rxy"]
+ADE3 = ["axVx", "ayVx", "azVx", "axVy", "ayVy", "azVy", "axVz", "ayVz", "azVz",
+        "axSxx", "aySxy", "azSxz", "axSxy", "aySyy", "azSyz", "axSxz", "aySyz", "azSzz"]
+FIELDS2 = ["Vx", "Vz", "Sxx", "Szz", "Sxz"]
+R2 = ["Rxx", "Rzz", "Rxz"]
+ADE2 = ["axVx", "azVx", "axVz", "azVz", "axSxx", "azSxz", "axSxz", "azSzz"]
+
+
+class Oracle3D:
+    def __init__(self, case, bits=32):
+        self.case = case
+        self.bits = bits
+        self.L = olib(bits)
+        real = C.c_float if bits == 32 else C.c_double
+        self.dtype = np.float32 if bits == 32 else np.float64
+        g, m = case.grid, case.medium
+        shp = g.shape
+        self.f = {n: np.zeros(shp, self.dtype) for n in FIELDS3}
+        self.R = {n: np.zeros(shp + (3,), self.dtype) for n in R3}   # (m, box), m fastest
+        self.A = {n: np.zeros(shp, self.dtype) for n in ADE3}
+        self.med = {n: np.ascontiguousarray(getattr(m, n), np.float32)
+                    for n in ("bx", "by", "bz", "lam", "mu", "taup", "taus", "muyz", "muxz", "muxy")}
+        self.surf = {n: np.ascontiguousarray(getattr(m, n), np.int32)
+                     for n in ("kfs_top", "kfs_bot", "kob_top", "kob_bot", "kob")}
+        self.prof = [np.ascontiguousarray(a, np.float32) for a in case.pml]
+        self.ts = np.ascontiguousarray(m.ts, np.float32)
+        S = _o3d_struct(real)
+        s = S()
+        s.nx, s.ny, s.nz, s.na, s.nm = g.nx, g.ny, g.nz, g.na, 3
+        s.dx, s.dy, s.dz, s.dt = g.dx, g.dy, g.dz, g.dt
+        PR = C.POINTER(real)
+        for n in FIELDS3:
+            setattr(s, n, self.f[n].ctypes.data_as(PR))
+        for n in R3:
+            setattr(s, n, self.R[n].ctypes.data_as(PR))
+        for n, a in self.med.items():
+            setattr(s, n, a.ctypes.data_as(_PF))
+        for n, a in self.surf.items():
+            setattr(s, n, a.ctypes.data_as(_PI))
+        for n, a in zip(("gxc", "gxe", "gyc", "gye", "gzc", "gze"), self.prof):
+            setattr(s, n, a.ctypes.data_as(_PF))
+        for q, n in enumerate(ADE3):
+            s.ade[q] = self.A[n].ctypes.data_as(PR)
+        s.ts = self.ts.ctypes.data_as(_PF)
+        self.s = s
+        gl = case.glut
+        self.nsrc = len(gl.idx) if gl is not None else 0
+        if self.nsrc:
+            self.isrc = np.ascontiguousarray(gl.idx[:, 0], np.int32)
+            self.jsrc = np.ascontiguousarray(gl.idx[:, 1], np.int32)
+            self.ksrc = np.ascontiguousarray(gl.idx[:, 2], np.int32)
+            self.m6 = np.ascontiguousarray(gl.m, np.float64)
+            self.tab = np.ascontiguousarray(gl.table, np.float64)
+        st = case.stations
+        self.nst = len(st)
+        if self.nst:
+            self.ist = np.ascontiguousarray(st[:, 0], np.int32)
+            self.jst = np.ascontiguousarray(st[:, 1], np.int32)
+            self.kst = np.ascontiguousarray(st[:, 2], np.int32)
+            from openswpc_amd.source import ntw_of
+            self.ntw = ntw_of(case.nt, case.ntdec_w)
+            self.vel = np.zeros((self.ntw, self.nst, 3), np.float32)
+            self.disp = np.zeros((self.ntw, self.nst, 3), np.float64)
+            self.uacc = np.zeros((self.nst, 3), np.float64)
+
+    def _ps(self):
+        return C.byref(self.s)
+
+    def receivers(self, it):
+        if not self.nst:
+            return
+        dec = self.case.ntdec_w
+        store = (it - 1) % dec == 0 and (it - 1) // dec < self.ntw
+        vel = np.zeros((self.nst, 3), np.float32)
+        self.L.o3d_receivers(self._ps(), self.nst, self.ist.ctypes.data_as(_PI), self.jst.ctypes.data_as(_PI),
+                             self.kst.ctypes.data_as(_PI), vel.ctypes.data_as(_PF),
+                             self.uacc.ctypes.data_as(C.POINTER(C.c_double)))
+        if store:
+            itw = (it - 1) // dec
+            self.vel[itw] = vel
+            self.disp[itw] = self.uacc
+
+    def update_stress(self):
+        self.L.o3d_update_stress(self._ps())
+        self.L.o3d_absorb_stress(self._ps())
+
+    def update_vel(self):
+        self.L.o3d_update_vel(self._ps())
+        self.L.o3d_absorb_vel(self._ps())
+
+    def stressglut(self, it):
+        if self.nsrc:
+            row = np.ascontiguousarray(self.tab[it - 1])
+            PD = C.POINTER(C.c_double)
+            self.L.o3d_stressglut(self._ps(), self.nsrc, self.isrc.ctypes.data_as(_PI), self.jsrc.ctypes.data_as(_PI),
+                                  self.ksrc.ctypes.data_as(_PI), self.m6.ctypes.data_as(PD), row.ctypes.data_as(PD))
+
+    def step(self, it):
+        """main.f90:125-144 order (receivers sampled before the updates)."""
+        self.receivers(it)
+        self.update_stress()
+        self.stressglut(it)
+        self.update_vel()
+
+    def run(self, it0, n):
+        for it in range(it0, it0 + n):
+            self.step(it)
+
+    def field(self, name):
+        if name in self.f:
+            return self.f[name]
+        if name in self.A:
+            return self.A[name]
+        c, m = name[:3], int(name[3:])
+        return self.R[c][..., m - 1]
+
+    def set_field(self, name, a):
+        if name in self.f:
+            self.f[name][...] = a
+        elif name in self.A:
+            self.A[name][...] = a
+        else:
+            c, m = name[:3], int(name[3:])
+            self.R[c][..., m - 1] = a
+
+    def vmax(self):
+        out = np.zeros(3, np.float32)
+        self.L.o3d_vmax(self._ps(), out.ctypes.data_as(_PF))
+        return out
+
+
+class Oracle2D:
+    def __init__(self, case, bits=32):
+        self.case = case
+        self.bits = bits
+        self.L = olib(bits)
+        real = C.c_float if bits == 32 else C.c_double
+        self.dtype = np.float32 if bits == 32 else np.float64
+        g, m = case.grid, case.medium
+        shp = g.shape
+        self.f = {n: np.zeros(shp, self.dtype) for n in FIELDS2}
+        self.R = {n: np.zeros(shp + (3,), self.dtype) for n in R2}
+        self.A = {n: np.zeros(shp, self.dtype) for n in ADE2}
+        self.med = {n: np.ascontiguousarray(getattr(m, n), np.float32) for n in ("rho", "lam", "mu", "taup", "taus")}
+        self.surf = {n: np.ascontiguousarray(getattr(m, n), np.int32)
+                     for n in ("kfs_top", "kfs_bot", "kob_top", "kob_bot", "kob")}
+        self.prof = [np.ascontiguousarray(a, np.float32) for a in case.pml]
+        self.ts = np.ascontiguousarray(m.ts, np.float32)
+        S = _o2d_struct(real)
+        s = S()
+        s.nx, s.nz, s.na, s.nm = g.nx, g.nz, g.na, 3
+        s.dx, s.dz, s.dt = g.dx, g.dz, g.dt
+        PR = C.POINTER(real)
+        for n in FIELDS2:
+            setattr(s, n, self.f[n].ctypes.data_as(PR))
+        for n in R2:
+            setattr(s, n, self.R[n].ctypes.data_as(PR))
+        for n, a in self.med.items():
+            setattr(s, n, a.ctypes.data_as(_PF))
+        for n, a in self.surf.items():
+            setattr(s, n, a.ctypes.data_as(_PI))
+        for n, a in zip(("gxc", "gxe", "gzc", "gze"), self.prof):
+            setattr(s, n, a.ctypes.data_as(_PF))
+        for q, n in enumerate(ADE2):
+            s.ade[q] = self.A[n].ctypes.data_as(PR)
+        s.ts = self.ts.ctypes.data_as(_PF)
+        self.s = s
+        gl = case.glut
+        self.nsrc = len(gl.idx) if gl is not None else 0
+        if self.nsrc:
+            self.isrc = np.ascontiguousarray(gl.idx[:, 0], np.int32)
+            self.ksrc = np.ascontiguousarray(gl.idx[:, 1], np.int32)
+            self.m3 = np.ascontiguousarray(gl.m, np.float64)
+            self.tab = np.ascontiguousarray(gl.table, np.float64)
+        st = case.stations
+        self.nst = len(st)
+        if self.nst:
+            self.ist = np.ascontiguousarray(st[:, 0], np.int32)
+            self.kst = np.ascontiguousarray(st[:, 1], np.int32)
+            from openswpc_amd.source import ntw_of
+            self.ntw = ntw_of(case.nt, case.ntdec_w)
+            self.vel = np.zeros((self.ntw, self.nst, 2), np.float32)
+            self.disp = np.zeros((self.ntw, self.nst, 2), np.float64)
+            self.uacc = np.zeros((self.nst, 2), np.float64)
+
+    def _ps(self):
+        return C.byref(self.s)
+
+    def receivers(self, it):
+        if not self.nst:
+            return
+        dec = self.case.ntdec_w
+        store = (it - 1) % dec == 0 and (it - 1) // dec < self.ntw
+        vel = np.zeros((self.nst, 2), np.float32)
+        self.L.o2d_receivers(self._ps(), self.nst, self.ist.ctypes.data_as(_PI), self.kst.ctypes.data_as(_PI),
+                             vel.ctypes.data_as(_PF), self.uacc.ctypes.data_as(C.POINTER(C.c_double)))
+        if store:
+            itw = (it - 1) // dec
+            self.vel[itw] = vel
+            self.disp[itw] = self.uacc
+
+    def update_stress(self):
+        self.L.o2d_update_stress(self._ps())
+        self.L.o2d_absorb_stress(self._ps())
+
+    def update_vel(self):
+        self.L.o2d_update_vel(self._ps())
+        self.L.o2d_absorb_vel(self._ps())
+
+    def stressglut(self, it):
+        if self.nsrc:
+            row = np.ascontiguousarray(self.tab[it - 1])
+            PD = C.POINTER(C.c_double)
+            self.L.o2d_stressglut(self._ps(), self.nsrc, self.isrc.ctypes.data_as(_PI), self.ksrc.ctypes.data_as(_PI),
+                                  self.m3.ctypes.data_as(PD), row.ctypes.data_as(PD))
+
+    def step(self, it):
+        self.receivers(it)
+        self.update_stress()
+        self.stressglut(it)
+        self.update_vel()
+
+    def run(self, it0, n):
+        for it in range(it0, it0 + n):
+            self.step(it)
+
+    def field(self, name):
+        if name in self.f:
+            return self.f[name]
+        if name in self.A:
+            return self.A[name]
+        c, m = name[:3], int(name[3:])
+        return self.R[c][..., m - 1]
+
+    def set_field(self, name, a):
+        if name in self.f:
+            self.f[name][...] = a
+        elif name in self.A:
+            self.A[name][...] = a
+        else:
+            c, m = name[:3], int(name[3:])
+            self.R[c][..., m - 1] = a
+
+    def vmax(self):
+        out = np.zeros(2, np.float32)
+        self.L.o2d_vmax(self._ps(), out.ctypes.data_as(_PF))
+        return out
