@@ -66,6 +66,11 @@ const char *swpc_last_error(void);
 int swpc_version(void);
 /* Number of HIP devices visible (0 when no GPU); never fails. */
 int swpc_device_count(void);
+/* Measured device copy bandwidth: `iters` copies of `bytes` bytes with a
+   float4 grid-stride kernel; *gbs = (read + write bytes) / time in GB/s.
+   A diagnostic with no counterpart in the reference: it is the measured
+   ceiling the stencil kernels are reported against. */
+int swpc_membw(long long bytes, int iters, double *gbs);
 
 /* ------------------------------------------------------------------ */
 /* 3-D (swpc_3d)                                                       */
@@ -162,6 +167,13 @@ int swpc3d_profile(swpc3d_ctx *ctx, int enable);
 int swpc3d_profile_read(swpc3d_ctx *ctx, double *ms5, long long *count5);
 /* Device bytes held by the context. */
 int swpc3d_memory(swpc3d_ctx *ctx, long long *bytes);
+/* Launch configuration of the interior kernels (no counterpart in the
+   reference; results do not depend on it).  p[0..n-1] overrides, in
+   order: stress tile rows (2/4/8), stress waves-per-SIMD hint (0/3/4),
+   split normal/shear stress into two kernels (0/1), velocity tile rows,
+   velocity waves hint (0/5), XCD-contiguous workgroup order (0/1),
+   stress z-chunk, velocity z-chunk (0 = automatic). */
+int swpc3d_set_tuning(swpc3d_ctx *ctx, const int *p, int n);
 
 /* ------------------------------------------------------------------ */
 /* P-SV (swpc_psv)                                                     */

@@ -38,6 +38,7 @@ _PROTOS = {
     "swpc_last_error": (C.c_char_p, []),
     "swpc_version": (I, []),
     "swpc_device_count": (I, []),
+    "swpc_membw": (I, [C.c_longlong, I, PD]),
     "swpc3d_create": (I, [C.POINTER(Desc3), C.POINTER(P)]),
     "swpc3d_destroy": (I, [P]),
     "swpc3d_set_medium": (I, [P] + [PF] * 10),
@@ -61,6 +62,7 @@ _PROTOS = {
     "swpc3d_profile": (I, [P, I]),
     "swpc3d_profile_read": (I, [P, PD, PLL]),
     "swpc3d_memory": (I, [P, PLL]),
+    "swpc3d_set_tuning": (I, [P, PI_, I]),
     "swpc2d_create": (I, [C.POINTER(Desc2), C.POINTER(P)]),
     "swpc2d_destroy": (I, [P]),
     "swpc2d_set_medium": (I, [P] + [PF] * 5),

@@ -56,19 +56,49 @@ struct K3 {
     T dt;
     int ti0;  // i of the first interior tile
     int kc;   // z-chunk length of the interior kernels
+    int ntx, nty, nkc;          // interior tiles along x, y and z chunks
+    unsigned nblk, nblk_pad;    // ntx*nty*nkc and its round-up to a multiple of 8
+    int xcd;                    // remap workgroup ids XCD-contiguously
     Boxes3 pml;
 };
 
 // ---------------------------------------------------------------------------
+// interior tiles
+// ---------------------------------------------------------------------------
+// The interior box is cut into 64 x BY column tiles and nkc z-chunks,
+// enumerated by one linear workgroup id.  Workgroups are dealt to the 8
+// XCDs round-robin; with xcd != 0 the id is remapped so that each XCD owns
+// one contiguous run of tiles and its L2 holds the y-neighbour rows that
+// adjacent tiles share.
+template <typename T>
+__device__ inline bool tile_of(const K3<T> &a, int &bx, int &by, int &bz)
+{
+    const unsigned b = blockIdx.x;
+    unsigned L = b;
+    if (a.xcd) {
+        const unsigned per = a.nblk_pad >> 3;
+        L = (b & 7u) * per + (b >> 3);
+    }
+    if (L >= a.nblk) return false;
+    bx = (int)(L % (unsigned)a.ntx);
+    const unsigned t = L / (unsigned)a.ntx;
+    by = (int)(t % (unsigned)a.nty);
+    bz = (int)(t / (unsigned)a.nty);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
 // interior stress
 // ---------------------------------------------------------------------------
-template <typename T, int BY>
-__global__ void __launch_bounds__(64 * BY) k3_stress_int(const K3<T> a)
+// PART 0: normal and shear fused (V and taus read once per cell);
+// PART 1: normal only; PART 2: shear only (the reference's two loops).
+template <typename T, int BY, int PART>
+__device__ __forceinline__ void stress_column(const K3<T> &a, int bx, int by, int bz)
 {
-    const int i = a.ti0 + (int)blockIdx.x * 64 + (int)threadIdx.x;
-    const int j = a.na + 1 + (int)blockIdx.y * BY + (int)threadIdx.y;
+    const int i = a.ti0 + bx * 64 + (int)threadIdx.x;
+    const int j = a.na + 1 + by * BY + (int)threadIdx.y;
     if (i <= a.na || i > a.nx - a.na || j > a.ny - a.na) return;
-    const int kb = 1 + (int)blockIdx.z * a.kc;
+    const int kb = 1 + bz * a.kc;
     const int ke = min(kb + a.kc - 1, a.nz - a.na);
     const int4 fs = a.surf[(j - 1) * a.nx + (i - 1)];
     const int px = a.px, sxy = a.sxy;
@@ -89,16 +119,25 @@ __global__ void __launch_bounds__(64 * BY) k3_stress_int(const K3<T> a)
     const float *__restrict__ muyz = a.muyz;
     const float *__restrict__ muxz = a.muxz;
     const float *__restrict__ muxy = a.muxy;
+    constexpr bool NORMAL = PART != 2, SHEAR = PART != 1;
 
     int c = at3(px, sxy, i, j, kb);
-    T vxm1 = Vx[c - sxy], vx0 = Vx[c], vxp1 = Vx[c + sxy];
-    T vym1 = Vy[c - sxy], vy0 = Vy[c], vyp1 = Vy[c + sxy];
-    T vzm2 = Vz[c - 2 * sxy], vzm1 = Vz[c - sxy], vz0 = Vz[c];
+    T vxm1 = 0, vx0 = Vx[c], vxp1 = 0;
+    T vym1 = 0, vy0 = Vy[c], vyp1 = 0;
+    T vzm2 = 0, vzm1 = Vz[c - sxy], vz0 = Vz[c];
+    if (SHEAR) {
+        vxm1 = Vx[c - sxy]; vxp1 = Vx[c + sxy];
+        vym1 = Vy[c - sxy]; vyp1 = Vy[c + sxy];
+    }
+    if (NORMAL) vzm2 = Vz[c - 2 * sxy];
     const T dt = a.dt;
 
     for (int k = kb; k <= ke; ++k, c += sxy) {
-        const T vxp2 = Vx[c + 2 * sxy];
-        const T vyp2 = Vy[c + 2 * sxy];
+        T vxp2 = 0, vyp2 = 0;
+        if (SHEAR) {
+            vxp2 = Vx[c + 2 * sxy];
+            vyp2 = Vy[c + 2 * sxy];
+        }
         const T vzp1 = Vz[c + sxy];
 
         // order switch of m_kernel.f90:185-186
@@ -107,84 +146,105 @@ __global__ void __launch_bounds__(64 * BY) k3_stress_int(const K3<T> a)
         const T e0x = lo ? a.e2[0][0] : a.e4[0][0], e1x = lo ? a.e2[0][1] : a.e4[0][1];
         const T e0y = lo ? a.e2[1][0] : a.e4[1][0], e1y = lo ? a.e2[1][1] : a.e4[1][1];
         const T e0z = lo ? a.e2[2][0] : a.e4[2][0], e1z = lo ? a.e2[2][1] : a.e4[2][1];
-
-        const T dxVx = (vx0 - Vx[c - 1]) * e0x - (Vx[c + 1] - Vx[c - 2]) * e1x;
-        const T dyVy = (vy0 - Vy[c - px]) * e0y - (Vy[c + px] - Vy[c - 2 * px]) * e1y;
-        const T dzVz = (vz0 - vzm1) * e0z - (vzp1 - vzm2) * e1z;
-
-        const T dxVy_dyVx = (Vy[c + 1] - vy0) * e0x - (Vy[c + 2] - Vy[c - 1]) * e1x
-                          + (Vx[c + px] - vx0) * e0y - (Vx[c + 2 * px] - Vx[c - px]) * e1y;
-        const T dxVz_dzVx = (Vz[c + 1] - vz0) * e0x - (Vz[c + 2] - Vz[c - 1]) * e1x
-                          + (vxp1 - vx0) * e0z - (vxp2 - vxm1) * e1z;
-        const T dyVz_dzVy = (Vz[c + px] - vz0) * e0y - (Vz[c + 2 * px] - Vz[c - px]) * e1y
-                          + (vyp1 - vy0) * e0z - (vyp2 - vym1) * e1z;
-
-        // normal part (m_kernel.f90:199-235)
-        const T mu2 = 2 * (T)mu[c];
-        const T lam2mu = (T)lam[c] + mu2;
-        const T taup1 = (T)taup[c], taus1 = (T)taus[c];
-        const T d3v3 = dxVx + dyVy + dzVz;
-        const T dyVy_dzVz = dyVy + dzVz, dxVx_dzVz = dxVx + dzVz, dxVx_dyVy = dxVx + dyVy;
-        const T fxx = lam2mu * taup1 * d3v3 - mu2 * taus1 * dyVy_dzVz;
-        const T fyy = lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dzVz;
-        const T fzz = lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dyVy;
-        T Rxx_n = 0, Ryy_n = 0, Rzz_n = 0;
-#pragma unroll
-        for (int m = 0; m < NM3; m++) {
-            T rxx = a.c1[m] * a.R[0 + m][c] - a.c2[m] * fxx * dt;
-            T ryy = a.c1[m] * a.R[3 + m][c] - a.c2[m] * fyy * dt;
-            T rzz = a.c1[m] * a.R[6 + m][c] - a.c2[m] * fzz * dt;
-            a.R[0 + m][c] = rxx;
-            a.R[3 + m][c] = ryy;
-            a.R[6 + m][c] = rzz;
-            Rxx_n += a.d1[m] * rxx;
-            Ryy_n += a.d1[m] * ryy;
-            Rzz_n += a.d1[m] * rzz;
-        }
-        const T taup_plus1 = 1 + taup1 * (1 + a.d2);
+        const T taus1 = (T)taus[c];
         const T taus_plus1 = 1 + taus1 * (1 + a.d2);
-        Sxx[c] = Sxx[c] + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dyVy_dzVz + Rxx_n) * dt;
-        Syy[c] = Syy[c] + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dzVz + Ryy_n) * dt;
-        Szz[c] = Szz[c] + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dyVy + Rzz_n) * dt;
 
-        // shear part (m_kernel.f90:289-311)
-        const T myz = (T)muyz[c], mxz = (T)muxz[c], mxy = (T)muxy[c];
-        const T fyz = myz * taus1 * dyVz_dzVy;
-        const T fxz = mxz * taus1 * dxVz_dzVx;
-        const T fxy = mxy * taus1 * dxVy_dyVx;
-        T Ryz_n = 0, Rxz_n = 0, Rxy_n = 0;
+        if (NORMAL) {
+            // m_kernel.f90:195-235
+            const T dxVx = (vx0 - Vx[c - 1]) * e0x - (Vx[c + 1] - Vx[c - 2]) * e1x;
+            const T dyVy = (vy0 - Vy[c - px]) * e0y - (Vy[c + px] - Vy[c - 2 * px]) * e1y;
+            const T dzVz = (vz0 - vzm1) * e0z - (vzp1 - vzm2) * e1z;
+            const T mu2 = 2 * (T)mu[c];
+            const T lam2mu = (T)lam[c] + mu2;
+            const T taup1 = (T)taup[c];
+            const T d3v3 = dxVx + dyVy + dzVz;
+            const T dyVy_dzVz = dyVy + dzVz, dxVx_dzVz = dxVx + dzVz, dxVx_dyVy = dxVx + dyVy;
+            const T fxx = lam2mu * taup1 * d3v3 - mu2 * taus1 * dyVy_dzVz;
+            const T fyy = lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dzVz;
+            const T fzz = lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dyVy;
+            T Rxx_n = 0, Ryy_n = 0, Rzz_n = 0;
 #pragma unroll
-        for (int m = 0; m < NM3; m++) {
-            T ryz = a.c1[m] * a.R[9 + m][c] - a.c2[m] * fyz * dt;
-            T rxz = a.c1[m] * a.R[12 + m][c] - a.c2[m] * fxz * dt;
-            T rxy = a.c1[m] * a.R[15 + m][c] - a.c2[m] * fxy * dt;
-            a.R[9 + m][c] = ryz;
-            a.R[12 + m][c] = rxz;
-            a.R[15 + m][c] = rxy;
-            Ryz_n += a.d1[m] * ryz;
-            Rxz_n += a.d1[m] * rxz;
-            Rxy_n += a.d1[m] * rxy;
+            for (int m = 0; m < NM3; m++) {
+                T rxx = a.c1[m] * a.R[0 + m][c] - a.c2[m] * fxx * dt;
+                T ryy = a.c1[m] * a.R[3 + m][c] - a.c2[m] * fyy * dt;
+                T rzz = a.c1[m] * a.R[6 + m][c] - a.c2[m] * fzz * dt;
+                a.R[0 + m][c] = rxx;
+                a.R[3 + m][c] = ryy;
+                a.R[6 + m][c] = rzz;
+                Rxx_n += a.d1[m] * rxx;
+                Ryy_n += a.d1[m] * ryy;
+                Rzz_n += a.d1[m] * rzz;
+            }
+            const T taup_plus1 = 1 + taup1 * (1 + a.d2);
+            Sxx[c] = Sxx[c] + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dyVy_dzVz + Rxx_n) * dt;
+            Syy[c] = Syy[c] + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dzVz + Ryy_n) * dt;
+            Szz[c] = Szz[c] + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dyVy + Rzz_n) * dt;
         }
-        Syz[c] = Syz[c] + (myz * taus_plus1 * dyVz_dzVy + Ryz_n) * dt;
-        Sxz[c] = Sxz[c] + (mxz * taus_plus1 * dxVz_dzVx + Rxz_n) * dt;
-        Sxy[c] = Sxy[c] + (mxy * taus_plus1 * dxVy_dyVx + Rxy_n) * dt;
+        if (SHEAR) {
+            // m_kernel.f90:282-311
+            const T dxVy_dyVx = (Vy[c + 1] - vy0) * e0x - (Vy[c + 2] - Vy[c - 1]) * e1x
+                              + (Vx[c + px] - vx0) * e0y - (Vx[c + 2 * px] - Vx[c - px]) * e1y;
+            const T dxVz_dzVx = (Vz[c + 1] - vz0) * e0x - (Vz[c + 2] - Vz[c - 1]) * e1x
+                              + (vxp1 - vx0) * e0z - (vxp2 - vxm1) * e1z;
+            const T dyVz_dzVy = (Vz[c + px] - vz0) * e0y - (Vz[c + 2 * px] - Vz[c - px]) * e1y
+                              + (vyp1 - vy0) * e0z - (vyp2 - vym1) * e1z;
+            const T myz = (T)muyz[c], mxz = (T)muxz[c], mxy = (T)muxy[c];
+            const T fyz = myz * taus1 * dyVz_dzVy;
+            const T fxz = mxz * taus1 * dxVz_dzVx;
+            const T fxy = mxy * taus1 * dxVy_dyVx;
+            T Ryz_n = 0, Rxz_n = 0, Rxy_n = 0;
+#pragma unroll
+            for (int m = 0; m < NM3; m++) {
+                T ryz = a.c1[m] * a.R[9 + m][c] - a.c2[m] * fyz * dt;
+                T rxz = a.c1[m] * a.R[12 + m][c] - a.c2[m] * fxz * dt;
+                T rxy = a.c1[m] * a.R[15 + m][c] - a.c2[m] * fxy * dt;
+                a.R[9 + m][c] = ryz;
+                a.R[12 + m][c] = rxz;
+                a.R[15 + m][c] = rxy;
+                Ryz_n += a.d1[m] * ryz;
+                Rxz_n += a.d1[m] * rxz;
+                Rxy_n += a.d1[m] * rxy;
+            }
+            Syz[c] = Syz[c] + (myz * taus_plus1 * dyVz_dzVy + Ryz_n) * dt;
+            Sxz[c] = Sxz[c] + (mxz * taus_plus1 * dxVz_dzVx + Rxz_n) * dt;
+            Sxy[c] = Sxy[c] + (mxy * taus_plus1 * dxVy_dyVx + Rxy_n) * dt;
+        }
 
         vxm1 = vx0; vx0 = vxp1; vxp1 = vxp2;
         vym1 = vy0; vy0 = vyp1; vyp1 = vyp2;
         vzm2 = vzm1; vzm1 = vz0; vz0 = vzp1;
+        if (!SHEAR) {
+            // the normal part needs Vx and Vy only at the centre plane
+            vx0 = Vx[c + sxy];
+            vy0 = Vy[c + sxy];
+        }
     }
+}
+
+// WPE > 0 asks the compiler for at least WPE waves per SIMD (a register
+// budget of 512/WPE VGPRs).
+template <typename T, int BY, int PART, int WPE>
+__global__ void __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8)))
+k3_stress_int(const K3<T> a)
+{
+    int bx, by, bz;
+    if (!tile_of(a, bx, by, bz)) return;
+    stress_column<T, BY, PART>(a, bx, by, bz);
 }
 
 // ---------------------------------------------------------------------------
 // interior velocity
 // ---------------------------------------------------------------------------
-template <typename T, int BY>
-__global__ void __launch_bounds__(64 * BY) k3_vel_int(const K3<T> a)
+template <typename T, int BY, int WPE>
+__global__ void __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8)))
+k3_vel_int(const K3<T> a)
 {
-    const int i = a.ti0 + (int)blockIdx.x * 64 + (int)threadIdx.x;
-    const int j = a.na + 1 + (int)blockIdx.y * BY + (int)threadIdx.y;
+    int bx, by, bz;
+    if (!tile_of(a, bx, by, bz)) return;
+    const int i = a.ti0 + bx * 64 + (int)threadIdx.x;
+    const int j = a.na + 1 + by * BY + (int)threadIdx.y;
     if (i <= a.na || i > a.nx - a.na || j > a.ny - a.na) return;
-    const int kb = 1 + (int)blockIdx.z * a.kc;
+    const int kb = 1 + bz * a.kc;
     const int ke = min(kb + a.kc - 1, a.nz - a.na);
     const int4 fs = a.surf[(j - 1) * a.nx + (i - 1)];
     const int px = a.px, sxy = a.sxy;
@@ -198,9 +258,9 @@ __global__ void __launch_bounds__(64 * BY) k3_vel_int(const K3<T> a)
     const T *__restrict__ Syz = a.S[3];
     const T *__restrict__ Sxz = a.S[4];
     const T *__restrict__ Sxy = a.S[5];
-    const float *__restrict__ bx = a.bx;
-    const float *__restrict__ by = a.by;
-    const float *__restrict__ bz = a.bz;
+    const float *__restrict__ bxm = a.bx;
+    const float *__restrict__ bym = a.by;
+    const float *__restrict__ bzm = a.bz;
 
     int c = at3(px, sxy, i, j, kb);
     T xzm2 = Sxz[c - 2 * sxy], xzm1 = Sxz[c - sxy], xz0 = Sxz[c];
@@ -231,9 +291,9 @@ __global__ void __launch_bounds__(64 * BY) k3_vel_int(const K3<T> a)
                       + (yz0 - Syz[c - px]) * e0y - (Syz[c + px] - Syz[c - 2 * px]) * e1y
                       + (zzp1 - zz0) * e0z - (zzp2 - zzm1) * e1z;
 
-        Vx[c] = Vx[c] + (T)bx[c] * d3Sx3 * dt;
-        Vy[c] = Vy[c] + (T)by[c] * d3Sy3 * dt;
-        Vz[c] = Vz[c] + (T)bz[c] * d3Sz3 * dt;
+        Vx[c] = Vx[c] + (T)bxm[c] * d3Sx3 * dt;
+        Vy[c] = Vy[c] + (T)bym[c] * d3Sy3 * dt;
+        Vz[c] = Vz[c] + (T)bzm[c] * d3Sz3 * dt;
 
         xzm2 = xzm1; xzm1 = xz0; xz0 = xzp1;
         yzm2 = yzm1; yzm1 = yz0; yz0 = yzp1;
@@ -534,13 +594,13 @@ struct swpc3d_ctx {
     int nst = 0, ntdec_w = 1, ntw = 0;
     bool have_medium = false, have_surface = false, have_pml = false;
     swpc::Boxes3 boxes;
-    int ti0 = 1, ntx = 0, nty = 0, kc = 1, nkc = 1;
+    int ti0 = 1;
+    // interior launch configuration (swpc3d_set_tuning)
+    int s_by = 4, s_wpe = 0, s_split = 0, v_by = 4, v_wpe = 0, xcd = 1, s_kc = 0, v_kc = 0;
     swpc::Prof prof;
 };
 
 namespace swpc {
-
-static const int BY3 = 4;
 
 template <typename T>
 static K3<T> make_args(swpc3d_ctx *c)
@@ -579,19 +639,93 @@ static K3<T> make_args(swpc3d_ctx *c)
     a.d2 = s / NM3;
     a.dt = t;
     a.ti0 = c->ti0;
-    a.kc = c->kc;
     a.pml = c->boxes;
+    a.xcd = c->xcd;
     return a;
+}
+
+// Interior tiling for a 64 x by tile and a z-chunk length kc (0 = chosen
+// so that the launch holds at least ~4096 workgroups, 16 per CU).
+template <typename T>
+static void set_tiles(const swpc3d_ctx *c, K3<T> &a, int by, int kc)
+{
+    const swpc3d_desc &d = c->d;
+    const int ihi = d.nx - d.na;
+    a.ntx = ihi >= d.na + 1 ? (ihi - c->ti0) / 64 + 1 : 0;
+    const int nj = d.ny - 2 * d.na;
+    a.nty = nj > 0 ? (nj + by - 1) / by : 0;
+    const int nk = d.nz - d.na;
+    if (kc <= 0) {
+        long long cols = std::max(1LL, (long long)a.ntx * a.nty);
+        int want = (int)std::max(1LL, std::min((long long)std::max(1, nk / 16), (4096 + cols - 1) / cols));
+        kc = (nk + want - 1) / want;
+    }
+    a.kc = std::max(1, kc);
+    a.nkc = nk > 0 ? (nk + a.kc - 1) / a.kc : 0;
+    a.nblk = (unsigned)a.ntx * (unsigned)a.nty * (unsigned)a.nkc;
+    a.nblk_pad = (a.nblk + 7u) & ~7u;
+}
+
+template <typename T, int BY, int PART, int WPE>
+static void go_stress(const K3<T> &a, hipStream_t s)
+{
+    hipLaunchKernelGGL((k3_stress_int<T, BY, PART, WPE>), dim3(a.xcd ? a.nblk_pad : a.nblk), dim3(64, BY), 0, s, a);
+}
+
+template <typename T, int BY, int WPE>
+static void go_vel(const K3<T> &a, hipStream_t s)
+{
+    hipLaunchKernelGGL((k3_vel_int<T, BY, WPE>), dim3(a.xcd ? a.nblk_pad : a.nblk), dim3(64, BY), 0, s, a);
+}
+
+// Instantiated interior variants.  FP64 always runs the 64 x 4 fused form
+// (the FP64 scheme is the accuracy reference, not a tuned path).
+template <typename T, int PART>
+static bool pick_stress(int by, int wpe, const K3<T> &a, hipStream_t s)
+{
+    if constexpr (sizeof(T) == 8) {
+        go_stress<T, 4, PART, 0>(a, s);
+        return true;
+    } else {
+#define SWPC_S(BY_, W_) if (by == BY_ && wpe == W_) { go_stress<T, BY_, PART, W_>(a, s); return true; }
+        SWPC_S(2, 0) SWPC_S(2, 4) SWPC_S(4, 0) SWPC_S(4, 3) SWPC_S(4, 4) SWPC_S(8, 0) SWPC_S(8, 4)
+#undef SWPC_S
+        return false;
+    }
+}
+
+template <typename T>
+static bool pick_vel(int by, int wpe, const K3<T> &a, hipStream_t s)
+{
+    if constexpr (sizeof(T) == 8) {
+        go_vel<T, 4, 0>(a, s);
+        return true;
+    } else {
+#define SWPC_V(BY_, W_) if (by == BY_ && wpe == W_) { go_vel<T, BY_, W_>(a, s); return true; }
+        SWPC_V(2, 0) SWPC_V(4, 0) SWPC_V(4, 5) SWPC_V(8, 0) SWPC_V(8, 5)
+#undef SWPC_V
+        return false;
+    }
 }
 
 template <typename T>
 static int launch_stress(swpc3d_ctx *c)
 {
     K3<T> a = make_args<T>(c);
+    const bool f64 = sizeof(T) == 8;
+    const int by = f64 ? 4 : c->s_by, wpe = f64 ? 0 : c->s_wpe;
+    set_tiles(c, a, by, c->s_kc);
     hipEvent_t e0;
-    if (c->ntx > 0 && c->nty > 0) {
+    if (a.nblk > 0) {
         if (int r = prof_begin(c->prof, 0, c->stream, &e0)) return r;
-        hipLaunchKernelGGL((k3_stress_int<T, BY3>), dim3(c->ntx, c->nty, c->nkc), dim3(64, BY3), 0, c->stream, a);
+        bool ok;
+        if (c->s_split && !f64) {
+            ok = pick_stress<T, 1>(by, wpe, a, c->stream);
+            if (ok) ok = pick_stress<T, 2>(by, wpe, a, c->stream);
+        } else {
+            ok = pick_stress<T, 0>(by, wpe, a, c->stream);
+        }
+        if (!ok) return fail(SWPC_EINVAL, "stress variant by=%d wpe=%d not built", c->s_by, c->s_wpe);
         SWPC_CHECK_LAUNCH();
         if (int r = prof_end(c->prof, 0, c->stream, e0)) return r;
     }
@@ -608,10 +742,14 @@ template <typename T>
 static int launch_vel(swpc3d_ctx *c)
 {
     K3<T> a = make_args<T>(c);
+    const bool f64 = sizeof(T) == 8;
+    const int by = f64 ? 4 : c->v_by, wpe = f64 ? 0 : c->v_wpe;
+    set_tiles(c, a, by, c->v_kc);
     hipEvent_t e0;
-    if (c->ntx > 0 && c->nty > 0) {
+    if (a.nblk > 0) {
         if (int r = prof_begin(c->prof, 2, c->stream, &e0)) return r;
-        hipLaunchKernelGGL((k3_vel_int<T, BY3>), dim3(c->ntx, c->nty, c->nkc), dim3(64, BY3), 0, c->stream, a);
+        if (!pick_vel<T>(by, wpe, a, c->stream))
+            return fail(SWPC_EINVAL, "velocity variant by=%d wpe=%d not built", c->v_by, c->v_wpe);
         SWPC_CHECK_LAUNCH();
         if (int r = prof_end(c->prof, 2, c->stream, e0)) return r;
     }
@@ -805,18 +943,8 @@ int swpc3d_create(const swpc3d_desc *desc, swpc3d_ctx **out)
     if ((r = dalloc(c->kob, (size_t)d.nx * d.ny * 4, c->bytes))) return r;
     if ((r = dalloc(c->vmax, 64, c->bytes))) return r;
 
-    // interior tiling: 64-wide tiles aligned to i = 1 + 64 t
-    const int ilo = d.na + 1, ihi = d.nx - d.na;
-    c->ti0 = 1 + ((ilo - 1) / 64) * 64;
-    c->ntx = ihi >= ilo ? (ihi - c->ti0) / 64 + 1 : 0;
-    const int nj = d.ny - 2 * d.na;
-    c->nty = nj > 0 ? (nj + BY3 - 1) / BY3 : 0;
-    const int nk = d.nz - d.na;
-    // z chunks: enough workgroups to fill the 256 CUs several times over
-    long long cols = (long long)c->ntx * c->nty;
-    int want = (int)std::max(1LL, std::min((long long)nk / 16, (4096 + cols - 1) / std::max(1LL, cols)));
-    c->kc = (nk + want - 1) / want;
-    c->nkc = (nk + c->kc - 1) / c->kc;
+    // interior tiles: 64 wide, aligned to i = 1 + 64 t (128-byte rows)
+    c->ti0 = 1 + (d.na / 64) * 64;
 
     // absorber boxes (kbeg_a, m_global.f90:336-345)
     Boxes3 &B = c->boxes;
@@ -1145,6 +1273,18 @@ int swpc3d_profile_read(swpc3d_ctx *c, double *ms5, long long *count5)
     clear_error();
     if (!c) return fail(SWPC_EINVAL, "null context");
     return prof_read(c->prof, ms5, count5);
+}
+
+int swpc3d_set_tuning(swpc3d_ctx *c, const int *p, int n)
+{
+    clear_error();
+    if (!c || (n > 0 && !p)) return fail(SWPC_EINVAL, "null argument");
+    int v[8] = {c->s_by, c->s_wpe, c->s_split, c->v_by, c->v_wpe, c->xcd, c->s_kc, c->v_kc};
+    for (int q = 0; q < n && q < 8; q++) v[q] = p[q];
+    if (v[6] < 0 || v[7] < 0) return fail(SWPC_EINVAL, "negative z-chunk");
+    c->s_by = v[0]; c->s_wpe = v[1]; c->s_split = v[2] != 0; c->v_by = v[3]; c->v_wpe = v[4];
+    c->xcd = v[5] != 0; c->s_kc = v[6]; c->v_kc = v[7];
+    return SWPC_OK;
 }
 
 int swpc3d_memory(swpc3d_ctx *c, long long *bytes)

@@ -2,6 +2,8 @@
 // P-SV contexts.
 #include "swpc_common.h"
 
+#include <algorithm>
+
 namespace swpc {
 
 static thread_local std::string g_err;
@@ -105,3 +107,49 @@ int swpc_device_count(void)
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Stream-copy probe: the measured HBM bandwidth that the stencil kernels are
+// compared against (and the scale for rocprof FETCH/WRITE counters).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_stream_copy(const float4 *__restrict__ a, float4 *__restrict__ b, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (; i < n; i += stride) b[i] = a[i];
+}
+
+extern "C" int swpc_membw(long long bytes, int iters, double *gbs)
+{
+    swpc::clear_error();
+    if (bytes < 1024 || iters < 1 || !gbs) return swpc::fail(SWPC_EINVAL, "bad argument");
+    size_t n = (size_t)bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = SWPC_OK;
+    if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess) {
+        rc = swpc::fail(SWPC_ENOMEM, "hipMalloc failed");
+    } else {
+        (void)hipMemset(a, 0, n * 16);
+        int blocks = (int)std::min<size_t>((n + 255) / 256, 256 * 32);
+        hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, 0, (const float4 *)a, (float4 *)b, n);
+        (void)hipEventCreate(&e0);
+        (void)hipEventCreate(&e1);
+        (void)hipEventRecord(e0, 0);
+        for (int q = 0; q < iters; q++)
+            hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, 0, (const float4 *)a, (float4 *)b, n);
+        (void)hipEventRecord(e1, 0);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+            rc = swpc::fail(SWPC_EHIP, "copy probe failed");
+        } else {
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            *gbs = 2.0 * (double)n * 16 * iters / (ms * 1e-3) / 1e9;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
+}

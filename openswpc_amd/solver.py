@@ -160,6 +160,11 @@ class Swpc3D:
         check(lib().swpc3d_profile_read(self.h, ptr(ms, C.c_double), ptr(cnt, C.c_longlong)), "profile_read")
         return ms, cnt
 
+    def set_tuning(self, *p):
+        """Interior launch configuration, see swpc3d_set_tuning."""
+        a = np.array(p, np.int32)
+        check(lib().swpc3d_set_tuning(self.h, ptr(a, C.c_int), len(a)), "set_tuning")
+
     def memory(self):
         b = C.c_longlong()
         check(lib().swpc3d_memory(self.h, C.byref(b)), "memory")

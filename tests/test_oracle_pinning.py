@@ -1,0 +1,354 @@
+"""Pins of the CPU oracle by what the mathematics fixes.
+
+The reference's kernel modules cannot be built here (they need MPI), so the
+oracle's restatement of them is checked against:
+  * exactness of the staggered operators: 4th order on cubic fields, 2nd
+    order on linear fields in the free-surface band;
+  * the generalized-Zener body: its first-step response and the relaxed
+    modulus in steady state under a constant strain rate;
+  * the ADE-PML update reducing to the plain 2nd-order update where the
+    damping vanishes, and the closed-form damping profile;
+  * symmetries of an explosion at the centre of a homogeneous medium (a
+    transposed axis or a wrong sign breaks them);
+  * the analytic whole-space Green's function of an explosion, and the P
+    speed of a plane wave in the P-SV solver.
+A dropped term, a wrong sign or index, or a transposed operand fails one.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from openswpc_amd import fdtool, medium as md, pml, problem
+from openswpc_amd.source import PointSource, stress_glut2, stress_glut3
+from oracle.oracle import Oracle2D, Oracle3D
+
+ISO = tuple(np.float32(1 / math.sqrt(3)) if q < 3 else np.float32(0) for q in range(6))
+
+
+def homog_case3(nx=32, ny=30, nz=28, na=4, dx=0.5, dt=0.04, q=1e10, vp=6.0, vs=3.5, rho=2.7, nt=4):
+    g = md.Grid3D(nx, ny, nz, dx, dx, dx, dt, na=na, zbeg=-dx * 3)
+    rho_, lam, mu, qp, qs = md.layered3(g, [md.Layer(-1000.0, rho, vp, vs, q, q)])
+    m = md.medium_setup3(g, rho_, lam, mu, qp, qs)
+    glut = stress_glut3(g, [], nt)
+    return problem.Case3D(g, m, pml.pml3(g, 1.0), glut, np.zeros((0, 3), np.int32), [], nt, 1, (0, 0, 0))
+
+
+def coords3(g):
+    """Cell-centre coordinates (x, y, z), centred on the grid, on the (j, i, k) box."""
+    x = (np.arange(-2, g.nx + 4) - g.nx / 2)[None, :, None] * g.dx
+    y = (np.arange(-2, g.ny + 4) - g.ny / 2)[:, None, None] * g.dy
+    z = (np.arange(-2, g.nz + 4) - g.nz / 2)[None, None, :] * g.dz
+    return x, y, z
+
+
+def interior_mask(g, kmin):
+    jj = np.arange(-2, g.ny + 4)[:, None, None]
+    ii = np.arange(-2, g.nx + 4)[None, :, None]
+    kk = np.arange(-2, g.nz + 4)[None, None, :]
+    return ((ii >= g.na + 1) & (ii <= g.nx - g.na) & (jj >= g.na + 1) & (jj <= g.ny - g.na)
+            & (kk >= kmin) & (kk <= g.nz - g.na))
+
+
+class Cubic:
+    """Random cubic polynomial in (x, y, z) with its analytic gradient."""
+
+    def __init__(self, seed):
+        rng = np.random.default_rng(seed)
+        self.terms = [(a, b, c, rng.standard_normal()) for a in range(4) for b in range(4) for c in range(4)
+                      if a + b + c <= 3]
+
+    def __call__(self, x, y, z):
+        return sum(w * x ** a * y ** b * z ** c for a, b, c, w in self.terms)
+
+    def d(self, ax, x, y, z):
+        out = 0.0
+        for a, b, c, w in self.terms:
+            e = (a, b, c)[ax]
+            if e == 0:
+                continue
+            p = [a, b, c]
+            p[ax] -= 1
+            out = out + w * e * x ** p[0] * y ** p[1] * z ** p[2]
+        return out
+
+
+def test_velocity_stencil_exact_on_cubics():
+    case = homog_case3()
+    g = case.grid
+    o = Oracle3D(case, 64)
+    for b in ("bx", "by", "bz"):
+        o.med[b][...] = 1.0
+    x, y, z = coords3(g)
+    h = g.dx / 2
+    F = {n: Cubic(q) for q, n in enumerate(("Sxx", "Syy", "Szz", "Sxy", "Sxz", "Syz"))}
+    pos = {"Sxx": (x, y, z), "Syy": (x, y, z), "Szz": (x, y, z),
+           "Sxy": (x + h, y + h, z), "Sxz": (x + h, y, z + h), "Syz": (x, y + h, z + h)}
+    for n, f in F.items():
+        o.set_field(n, f(*pos[n]))
+    o.L.o3d_update_vel(o._ps())
+    dt = float(np.float32(g.dt))
+    vx = (x + h, y, z)
+    vy = (x, y + h, z)
+    vz = (x, y, z + h)
+    ex = {"Vx": F["Sxx"].d(0, *vx) + F["Sxy"].d(1, *vx) + F["Sxz"].d(2, *vx),
+          "Vy": F["Sxy"].d(0, *vy) + F["Syy"].d(1, *vy) + F["Syz"].d(2, *vy),
+          "Vz": F["Sxz"].d(0, *vz) + F["Syz"].d(1, *vz) + F["Szz"].d(2, *vz)}
+    m = interior_mask(g, 3)  # k = 1, 2 are the 2nd-order band (kfs = 0)
+    for n in ("Vx", "Vy", "Vz"):
+        e = np.broadcast_to(ex[n], o.f[n].shape)
+        assert np.allclose(o.f[n][m] / dt, e[m], rtol=1e-9, atol=1e-9 * np.abs(e[m]).max()), n
+
+
+def test_stress_stencils_exact_on_cubics_elastic():
+    """tau = 0: S_ii += (lam theta + 2 mu dV_i/dx_i) dt and
+    S_ij += mu_ij (dV_i/dx_j + dV_j/dx_i) dt exactly on cubic V."""
+    case = homog_case3()
+    g = case.grid
+    o = Oracle3D(case, 64)
+    for n in ("taup", "taus"):
+        o.med[n][...] = 0.0
+    lam = float(o.med["lam"][10, 10, 10])
+    mu = float(o.med["mu"][10, 10, 10])
+    mij = {n: float(o.med[n][10, 10, 10]) for n in ("muyz", "muxz", "muxy")}
+    x, y, z = coords3(g)
+    h = g.dx / 2
+    G = {n: Cubic(10 + q) for q, n in enumerate(("Vx", "Vy", "Vz"))}
+    pos = {"Vx": (x + h, y, z), "Vy": (x, y + h, z), "Vz": (x, y, z + h)}
+    for n, f in G.items():
+        o.set_field(n, f(*pos[n]))
+    o.L.o3d_update_stress(o._ps())
+    dt = float(np.float32(g.dt))
+    c = (x, y, z)
+    dxVx, dyVy, dzVz = G["Vx"].d(0, *c), G["Vy"].d(1, *c), G["Vz"].d(2, *c)
+    th = dxVx + dyVy + dzVz
+    pyz, pxz, pxy = (x, y + h, z + h), (x + h, y, z + h), (x + h, y + h, z)
+    ex = {"Sxx": lam * th + 2 * mu * dxVx, "Syy": lam * th + 2 * mu * dyVy, "Szz": lam * th + 2 * mu * dzVz,
+          "Syz": mij["muyz"] * (G["Vz"].d(1, *pyz) + G["Vy"].d(2, *pyz)),
+          "Sxz": mij["muxz"] * (G["Vz"].d(0, *pxz) + G["Vx"].d(2, *pxz)),
+          "Sxy": mij["muxy"] * (G["Vy"].d(0, *pxy) + G["Vx"].d(1, *pxy))}
+    m = interior_mask(g, 3)
+    for n, e in ex.items():
+        e = np.broadcast_to(e, o.f[n].shape)
+        assert np.allclose(o.f[n][m] / dt, e[m], rtol=1e-9, atol=1e-9 * np.abs(e[m]).max()), n
+    assert np.all(o.R["Rxx"] == 0)
+
+
+def test_second_order_band_exact_on_linear_and_not_cubic():
+    """In the free-surface band [kfs-2, kfs+2] the operator is (1, 0)/dx:
+    exact on linear fields and different from the 4th order on cubics."""
+    g = md.Grid3D(32, 30, 30, 0.5, 0.5, 0.5, 0.04, na=4, zbeg=-2.0)
+    rho, lam, mu, qp, qs = md.layered3(g, [md.Layer(0.0, 2.7, 6.0, 3.5, 1e30, 1e30)])
+    m = md.medium_setup3(g, rho, lam, mu, qp, qs)
+    glut = stress_glut3(g, [], 2)
+    case = problem.Case3D(g, m, pml.pml3(g, 1.0), glut, np.zeros((0, 3), np.int32), [], 2, 1, (0, 0, 0))
+    kfs = int(m.kfs[10, 10])
+    assert kfs == 4  # cells k <= 4 have z = -2 + (k - 1/2) 0.5 < 0 (air)
+    kb = kfs + 2     # inside the band
+    k4 = kfs + 6     # below the band: 4th order
+    z = (np.arange(-2, g.nz + 4) * g.dz)[None, None, :] * np.ones(g.shape)
+    dt = float(np.float32(g.dt))
+    for field, expect_band in ((2.0 * z, lambda k: 2.0), (z ** 3, None)):
+        o = Oracle3D(case, 64)
+        for b in ("bx", "by", "bz"):
+            o.med[b][...] = 1.0
+        o.set_field("Szz", field)
+        o.L.o3d_update_vel(o._ps())
+        got_b = o.f["Vz"][12, 12, kb + 2] / dt
+        got_4 = o.f["Vz"][12, 12, k4 + 2] / dt
+        if expect_band is not None:
+            assert np.isclose(got_b, 2.0, rtol=1e-12) and np.isclose(got_4, 2.0, rtol=1e-12)
+        else:
+            d = g.dz
+            second = (((kb + 1) * d) ** 3 - (kb * d) ** 3) / d
+            exact = 3 * ((kb + 0.5) * d) ** 2
+            assert np.isclose(got_b, second, rtol=1e-12)
+            assert not np.isclose(got_b, exact, rtol=1e-3)
+            assert np.isclose(got_4, 3 * ((k4 + 0.5) * d) ** 2, rtol=1e-12)
+
+
+def test_zener_first_step_and_relaxed_modulus():
+    """Constant strain rate dVx/dx: the first step is the unrelaxed response;
+    in steady state dSyy/dt = lambda * rate (relaxed modulus)."""
+    case = homog_case3(q=50.0)
+    g = case.grid
+    o = Oracle3D(case, 64)
+    x, y, z = coords3(g)
+    rate = 1e-3
+    o.set_field("Vx", rate * x)
+    lam2mu = float(o.med["lam"][10, 10, 10]) + 2 * float(o.med["mu"][10, 10, 10])
+    mu2 = 2 * float(o.med["mu"][10, 10, 10])
+    taup = float(o.med["taup"][10, 10, 10])
+    taus = float(o.med["taus"][10, 10, 10])
+    assert taup > 0.01 and taus > 0.01
+    c1, c2, d1, d2 = (np.zeros(3) for _ in range(4))
+    PD = C.POINTER(C.c_double)
+    o.L.oracle_zener(3, o.ts.ctypes.data_as(C.POINTER(C.c_float)), C.c_float(g.dt), c1.ctypes.data_as(PD),
+                     c2.ctypes.data_as(PD), d1.ctypes.data_as(PD), d2.ctypes.data_as(PD))
+    # closed forms of kernel__setup (m_kernel.f90:58-67)
+    ts, dt = o.ts.astype(float), float(np.float32(g.dt))
+    assert np.allclose(c1, (2 * ts - dt) / (2 * ts + dt), rtol=1e-7)
+    assert np.allclose(d1, 2 * ts / (2 * ts - dt), rtol=1e-7)
+    syy = []
+    for _ in range(4000):
+        o.L.o3d_update_stress(o._ps())
+        syy.append(float(o.f["Syy"][10, 10, 10]))
+    f = (lam2mu * taup - mu2 * taus) * rate
+    first = (lam2mu * (1 + taup * (1 + d2[0])) - mu2 * (1 + taus * (1 + d2[0]))) * rate \
+        + sum(d1[m] * (-c2[m] * f * dt) for m in range(3))
+    assert np.isclose(syy[0] / dt, first, rtol=1e-10)
+    relaxed = (lam2mu - mu2) * rate
+    assert np.isclose((syy[-1] - syy[-2]) / dt, relaxed, rtol=1e-6)
+    assert abs(syy[0] / dt - relaxed) > 1e-2 * abs(relaxed)
+
+
+def test_pml_profile_closed_forms():
+    g = md.Grid3D(60, 60, 60, 0.5, 0.5, 0.5, 0.04, na=10)
+    fcut = 1.0
+    gxc, gxe, gyc, gye, gzc, gze = pml.pml3(g, fcut)
+    a0 = math.pi * fcut
+    h = g.dt / 2
+    assert np.allclose(gxc[30], [1.0, -1.0 / (1 + h * a0), (1 - h * a0) / (1 + h * a0), 0.0], rtol=1e-6)
+    assert np.all(gxc[:10, 3] > 0) and np.all(gxc[-10:, 3] > 0) and np.all(gxc[10:-10, 3] == 0)
+    assert np.allclose(gxc[:10], gxc[::-1][:10], rtol=1e-5)  # the two faces mirror each other
+    # na = 10: R0 = 10^(-(1 - 1)/log10(2) - 3) = 1e-3
+    H = 10 * 0.5
+    d0 = -(1 / (2 * H)) * 2 * 6.0 * math.log(1e-3)
+    xx = (g.xbeg + H) - float(g.xc[1 + 2])
+    d, a, b = d0 * xx / H, a0 * (1 - xx / H), 1 + 6 * (xx / H) ** 2
+    assert np.isclose(gxc[0, 3], (d / b) / (1 + h * (a + d / b)), rtol=1e-5)
+    assert np.isclose(gxc[0, 0], ((1 + h * a) / b) / (1 + h * (a + d / b)), rtol=1e-5)
+
+
+def test_pml_update_reduces_to_second_order_where_undamped():
+    case = homog_case3(q=1e30)
+    g = case.grid
+    o = Oracle3D(case, 64)
+    for p in o.prof:
+        p[:, 0] = 1.0
+        p[:, 3] = 0.0
+    x, y, z = coords3(g)
+    o.set_field("Vx", 0.3 * x + 0.1 * y)
+    o.set_field("Vz", -0.2 * z + 0.4 * x)
+    o.L.o3d_absorb_stress(o._ps())
+    lam = float(o.med["lam"][3, 3, 10])
+    mu = float(o.med["mu"][3, 3, 10])
+    dt = float(np.float32(g.dt))
+    assert np.isclose(o.f["Sxx"][3, 3, 10], ((lam + 2 * mu) * 0.3 + lam * (-0.2)) * dt, rtol=1e-9)
+    mxz = float(o.med["muxz"][3, 3, 10])
+    assert np.isclose(o.f["Sxz"][3, 3, 10], mxz * 0.4 * dt, rtol=1e-9)
+    mxy = float(o.med["muxy"][3, 3, 10])
+    assert np.isclose(o.f["Sxy"][3, 3, 10], mxy * 0.1 * dt, rtol=1e-9)
+    for n in ("axVx", "azVz", "azVx", "ayVx"):
+        assert np.all(o.A[n] == 0)
+
+
+def explosion_case(n, na, nt, dx=0.5, dt=0.03, tr=1.0):
+    """Odd n: the source sits exactly on the centre cell of a homogeneous
+    elastic whole space (no air)."""
+    src = PointSource(0.0, 0.0, 0.0, 0.0, tr, 1e15, ISO)
+    beg = -(n / 2) * dx
+    g = md.Grid3D(n, n, n, dx, dx, dx, dt, na=na, xbeg=beg, ybeg=beg, zbeg=beg)
+    rho, lam, mu, qp, qs = md.layered3(g, [md.Layer(-1000.0, 2.7, 6.0, 3.5, 1e30, 1e30)])
+    m = md.medium_setup3(g, rho, lam, mu, qp, qs, benchmark=True)
+    glut = stress_glut3(g, [src], nt)
+    return problem.Case3D(g, m, pml.pml3(g, glut.fcut), glut, np.zeros((0, 3), np.int32), [], nt, 1, (0, 0, 0))
+
+
+def test_explosion_symmetries():
+    """Before the wavefront reaches the absorber (whose interior/PML split
+    is not mirror-symmetric for staggered nodes), an explosion at the
+    centre cell gives Vx(k,i,j) = Vy(k,j,i), Vx odd and Vy even about the
+    source in x."""
+    n = 41
+    case = explosion_case(n, 6, 24)
+    g = case.grid
+    is_, js, ks = (int(v) for v in case.glut.idx[0])
+    assert is_ == js == ks == (n + 1) // 2
+    o = Oracle3D(case, 64)
+    o.run(1, 24)
+    Vx, Vy = o.f["Vx"], o.f["Vy"]
+    scale = np.abs(Vx).max()
+    assert scale > 0
+    assert np.allclose(Vx, np.swapaxes(Vy, 0, 1), atol=1e-12 * scale)
+    w = 12
+    i = np.arange(is_ - w, is_ + w)
+    a = Vx[:, i + 2, :]
+    b = Vx[:, (2 * is_ - i - 1) + 2, :]
+    # rounding differs between mirrored cells (summation order); an index
+    # error would show at O(1)
+    assert np.allclose(a, -b, atol=1e-9 * scale)
+    i2 = np.arange(is_ - w, is_ + w + 1)
+    assert np.allclose(Vy[:, i2 + 2, :], Vy[:, (2 * is_ - i2) + 2, :], atol=1e-9 * scale)
+    # the wavefront has not reached the absorber yet
+    assert np.abs(Vx[:, : is_ - w + 2, :]).max() < 1e-4 * scale
+
+
+def test_explosion_matches_analytic_whole_space():
+    """Radial velocity of an isotropic source M(t) m delta_ij, m = 1/sqrt(3):
+    v_r = m/(4 pi rho a^2) [Mdot(t - r/a)/r^2 + Mddot(t - r/a)/(a r)]
+    (Aki & Richards eq. 4.29 for an explosion), at 3 receivers on +x.
+
+    Time labels follow the reference: the velocity after step n is the
+    sample at t = n dt (m_wav.f90:524-535 stores it at the start of step
+    n+1), while the stress glut of step n uses the moment rate at
+    (n - 1/2) dt (m_source.f90:800) for a stress increment centred at
+    (n - 1) dt.  The discrete solution therefore leads the continuous one
+    by dt/2, which the comparison includes."""
+    n, na, nt = 73, 12, 120
+    case = explosion_case(n, na, nt, tr=2.0)
+    g = case.grid
+    o = Oracle3D(case, 64)
+    is_, js, ks = (int(v) for v in case.glut.idx[0])
+    recv = [is_ + 8, is_ + 12, is_ + 16]
+    tr = []
+    for it in range(1, nt + 1):
+        o.step(it)
+        tr.append([0.5 * (o.f["Vx"][js + 2, r + 2, ks + 2] + o.f["Vx"][js + 2, r + 1, ks + 2]) for r in recv])
+    tr = np.array(tr)
+    alpha, rho = 6.0, 2.7
+    dt = float(np.float32(g.dt))
+    t = np.arange(1, nt + 1) * dt + dt / 2
+    m = 1 / math.sqrt(3)
+
+    def Md(tt):
+        return np.array([float(fdtool.kupper(v, 0.0, 2.0)) for v in tt])
+
+    def Mdd(tt, e=1e-4):
+        return (Md(tt + e) - Md(tt - e)) / (2 * e)
+
+    for q, r in enumerate(recv):
+        R = (r - is_) * g.dx
+        tau = t - R / alpha
+        v = m * (Md(tau) / R ** 2 + Mdd(tau) / (alpha * R)) / (4 * math.pi * rho * alpha ** 2)
+        err = np.linalg.norm(tr[:, q] - v) / np.linalg.norm(v)
+        assert err < 0.02, (R, err)
+        assert abs(np.abs(tr[:, q]).max() / np.abs(v).max() - 1) < 0.02
+
+
+def test_psv_plane_wave_speed():
+    """A down-going plane P wave in the P-SV solver travels at alpha."""
+    g = md.Grid2D(100, 200, 0.1, 0.1, 0.005, na=4, zbeg=-10.0)
+    rho, lam, mu, qp, qs = md.layered2(g, [md.Layer(-1000.0, 2.7, 6.0, 3.5, 1e30, 1e30)])
+    m = md.medium_setup2(g, rho, lam, mu, qp, qs, benchmark=True)
+    glut = stress_glut2(g, [], 4)
+    case = problem.Case2D(g, m, pml.pml2(g, 1.0), glut, np.zeros((0, 2), np.int32), [], 4, 1, (0, 0, 0))
+    o = Oracle2D(case, 64)
+    k = np.arange(-2, g.nz + 4).astype(float)
+    z0, w = 60.0, 4.0
+    ones = np.ones((g.nx + 6, 1))
+    a, b = 6.0, 3.5
+    o.set_field("Vz", np.exp(-((k + 0.5 - z0) / w) ** 2)[None, :] * ones)
+    s = -2.7 * a * np.exp(-((k - z0) / w) ** 2)[None, :] * ones
+    o.set_field("Szz", s)
+    o.set_field("Sxx", s * (1 - 2 * b * b / (a * a)))
+    nsteps = 100
+    for _ in range(nsteps):
+        o.update_stress()
+        o.update_vel()
+    prof = o.f["Vz"][50 + 2, :]
+    peak = float(np.argmax(prof) - 2) + 0.5
+    expected = z0 + a * nsteps * g.dt / g.dz
+    assert abs(peak - expected) <= 1.5
+    # no up-going wave of comparable size
+    assert prof[: int(z0)].max() < 0.05 * prof.max()
