@@ -172,7 +172,10 @@ int swpc3d_memory(swpc3d_ctx *ctx, long long *bytes);
    order: stress tile rows (2/4/8), stress waves-per-SIMD hint (0/3/4),
    split normal/shear stress into two kernels (0/1), velocity tile rows,
    velocity waves hint (0/5), XCD-contiguous workgroup order (0/1),
-   stress z-chunk, velocity z-chunk (0 = automatic). */
+   stress z-chunk, velocity z-chunk (0 = automatic), 4-cell-per-thread
+   kernels for FP32 and reduced storage (0/1, default 1), their tile
+   rows (1/2/4), and 4-cell-per-thread PML kernels (0/1, default 0).  The first seven apply to the one-cell-per-thread
+   kernels, which FP64 always uses. */
 int swpc3d_set_tuning(swpc3d_ctx *ctx, const int *p, int n);
 
 /* ------------------------------------------------------------------ */

@@ -21,6 +21,9 @@
 // is a gather, not a contraction, and is bound by HBM bandwidth.
 #include "swpc_common.h"
 
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
 #include <algorithm>
 #include <cmath>
 #include <memory>
@@ -47,9 +50,17 @@ struct K3 {
     T *S[6];    // Sxx Syy Szz Syz Sxz Sxy
     T *R[18];   // Rxx(m) Ryy(m) Rzz(m) Ryz(m) Rxz(m) Rxy(m), m = 0..2
     T *A[18];   // ADE arrays (order of enum swpc3d_field)
-    const float *bx, *by, *bz, *lam, *mu, *taup, *taus, *muyz, *muxz, *muxy;
+    const float *med[10];   // bx by bz lam mu taup taus muyz muxz muxy (the reference's real(SP) arrays)
+    // reduced storage (SWPC_FP32_S16): FP32 arithmetic, V and S in FP32
+    const __half *hmed[10];     // medium: FP16 residual about a per-plane FP32 base
+    const float2 *mbs[10];      // per-plane (base, scale), indexed k + KO
+    __half *hR[18];             // memory variables stored as R * r_inv
+    float r_scale, r_inv;
+    __hip_bfloat16 *hA[18];     // ADE arrays of the PML
     const int4 *surf;   // (kfs_top, kfs_bot, kob_top, kob_bot) per column (i,j) in [1,nx]x[1,ny]
     const T *gx, *gy, *gz;  // profiles [8][n]: c1..c4 at centres then e1..e4 at edges
+    const float *gxv;       // x profiles for the 4-wide PML: [8][gxp], 16-byte aligned rows
+    int gxp;
     T e4[3][2], e2[3][2];   // (re40, re41) for isign = -1 (4th) and +1 (2nd), per axis
     T r2[3];                // 1/dx, 1/dy, 1/dz
     T c1[NM3], c2[NM3], d1[NM3], d2;
@@ -61,6 +72,52 @@ struct K3 {
     int xcd;                    // remap workgroup ids XCD-contiguously
     Boxes3 pml;
 };
+
+enum { MBX = 0, MBY, MBZ, MLAM, MMU, MTAUP, MTAUS, MMUYZ, MMUXZ, MMUXY };
+
+// Medium value at cell c of plane k.  Reduced storage reconstructs
+// base(k) + scale(k) * q; a plane whose residual is identically zero
+// (a layered medium) is not read at all.
+template <bool S16, typename T>
+__device__ __forceinline__ T medv(const K3<T> &a, int q, int c, int k)
+{
+    if constexpr (S16) {
+        const float2 bs = a.mbs[q][k + KO];
+        float v = bs.x;
+        if (bs.y != 0.0f) v += bs.y * __half2float(a.hmed[q][c]);
+        return (T)v;
+    } else {
+        return (T)a.med[q][c];
+    }
+}
+
+template <bool S16, typename T>
+__device__ __forceinline__ T ldR(const K3<T> &a, int q, int c)
+{
+    if constexpr (S16) return (T)(__half2float(a.hR[q][c]) * a.r_scale);
+    else return a.R[q][c];
+}
+
+template <bool S16, typename T>
+__device__ __forceinline__ void stR(const K3<T> &a, int q, int c, T v)
+{
+    if constexpr (S16) a.hR[q][c] = __float2half_rn(fminf(fmaxf((float)v * a.r_inv, -65504.0f), 65504.0f));
+    else a.R[q][c] = v;
+}
+
+template <bool S16, typename T>
+__device__ __forceinline__ T ldA(const K3<T> &a, int q, int c)
+{
+    if constexpr (S16) return (T)__bfloat162float(a.hA[q][c]);
+    else return a.A[q][c];
+}
+
+template <bool S16, typename T>
+__device__ __forceinline__ void stA(const K3<T> &a, int q, int c, T v)
+{
+    if constexpr (S16) a.hA[q][c] = __float2bfloat16((float)v);
+    else a.A[q][c] = v;
+}
 
 // ---------------------------------------------------------------------------
 // interior tiles
@@ -92,7 +149,7 @@ __device__ inline bool tile_of(const K3<T> &a, int &bx, int &by, int &bz)
 // ---------------------------------------------------------------------------
 // PART 0: normal and shear fused (V and taus read once per cell);
 // PART 1: normal only; PART 2: shear only (the reference's two loops).
-template <typename T, int BY, int PART>
+template <typename T, int BY, int PART, bool S16>
 __device__ __forceinline__ void stress_column(const K3<T> &a, int bx, int by, int bz)
 {
     const int i = a.ti0 + bx * 64 + (int)threadIdx.x;
@@ -112,13 +169,6 @@ __device__ __forceinline__ void stress_column(const K3<T> &a, int bx, int by, in
     T *__restrict__ Syz = a.S[3];
     T *__restrict__ Sxz = a.S[4];
     T *__restrict__ Sxy = a.S[5];
-    const float *__restrict__ lam = a.lam;
-    const float *__restrict__ mu = a.mu;
-    const float *__restrict__ taup = a.taup;
-    const float *__restrict__ taus = a.taus;
-    const float *__restrict__ muyz = a.muyz;
-    const float *__restrict__ muxz = a.muxz;
-    const float *__restrict__ muxy = a.muxy;
     constexpr bool NORMAL = PART != 2, SHEAR = PART != 1;
 
     int c = at3(px, sxy, i, j, kb);
@@ -146,7 +196,7 @@ __device__ __forceinline__ void stress_column(const K3<T> &a, int bx, int by, in
         const T e0x = lo ? a.e2[0][0] : a.e4[0][0], e1x = lo ? a.e2[0][1] : a.e4[0][1];
         const T e0y = lo ? a.e2[1][0] : a.e4[1][0], e1y = lo ? a.e2[1][1] : a.e4[1][1];
         const T e0z = lo ? a.e2[2][0] : a.e4[2][0], e1z = lo ? a.e2[2][1] : a.e4[2][1];
-        const T taus1 = (T)taus[c];
+        const T taus1 = medv<S16>(a, MTAUS, c, k);
         const T taus_plus1 = 1 + taus1 * (1 + a.d2);
 
         if (NORMAL) {
@@ -154,9 +204,9 @@ __device__ __forceinline__ void stress_column(const K3<T> &a, int bx, int by, in
             const T dxVx = (vx0 - Vx[c - 1]) * e0x - (Vx[c + 1] - Vx[c - 2]) * e1x;
             const T dyVy = (vy0 - Vy[c - px]) * e0y - (Vy[c + px] - Vy[c - 2 * px]) * e1y;
             const T dzVz = (vz0 - vzm1) * e0z - (vzp1 - vzm2) * e1z;
-            const T mu2 = 2 * (T)mu[c];
-            const T lam2mu = (T)lam[c] + mu2;
-            const T taup1 = (T)taup[c];
+            const T mu2 = 2 * medv<S16>(a, MMU, c, k);
+            const T lam2mu = medv<S16>(a, MLAM, c, k) + mu2;
+            const T taup1 = medv<S16>(a, MTAUP, c, k);
             const T d3v3 = dxVx + dyVy + dzVz;
             const T dyVy_dzVz = dyVy + dzVz, dxVx_dzVz = dxVx + dzVz, dxVx_dyVy = dxVx + dyVy;
             const T fxx = lam2mu * taup1 * d3v3 - mu2 * taus1 * dyVy_dzVz;
@@ -165,12 +215,12 @@ __device__ __forceinline__ void stress_column(const K3<T> &a, int bx, int by, in
             T Rxx_n = 0, Ryy_n = 0, Rzz_n = 0;
 #pragma unroll
             for (int m = 0; m < NM3; m++) {
-                T rxx = a.c1[m] * a.R[0 + m][c] - a.c2[m] * fxx * dt;
-                T ryy = a.c1[m] * a.R[3 + m][c] - a.c2[m] * fyy * dt;
-                T rzz = a.c1[m] * a.R[6 + m][c] - a.c2[m] * fzz * dt;
-                a.R[0 + m][c] = rxx;
-                a.R[3 + m][c] = ryy;
-                a.R[6 + m][c] = rzz;
+                T rxx = a.c1[m] * ldR<S16>(a, 0 + m, c) - a.c2[m] * fxx * dt;
+                T ryy = a.c1[m] * ldR<S16>(a, 3 + m, c) - a.c2[m] * fyy * dt;
+                T rzz = a.c1[m] * ldR<S16>(a, 6 + m, c) - a.c2[m] * fzz * dt;
+                stR<S16>(a, 0 + m, c, rxx);
+                stR<S16>(a, 3 + m, c, ryy);
+                stR<S16>(a, 6 + m, c, rzz);
                 Rxx_n += a.d1[m] * rxx;
                 Ryy_n += a.d1[m] * ryy;
                 Rzz_n += a.d1[m] * rzz;
@@ -188,19 +238,19 @@ __device__ __forceinline__ void stress_column(const K3<T> &a, int bx, int by, in
                               + (vxp1 - vx0) * e0z - (vxp2 - vxm1) * e1z;
             const T dyVz_dzVy = (Vz[c + px] - vz0) * e0y - (Vz[c + 2 * px] - Vz[c - px]) * e1y
                               + (vyp1 - vy0) * e0z - (vyp2 - vym1) * e1z;
-            const T myz = (T)muyz[c], mxz = (T)muxz[c], mxy = (T)muxy[c];
+            const T myz = medv<S16>(a, MMUYZ, c, k), mxz = medv<S16>(a, MMUXZ, c, k), mxy = medv<S16>(a, MMUXY, c, k);
             const T fyz = myz * taus1 * dyVz_dzVy;
             const T fxz = mxz * taus1 * dxVz_dzVx;
             const T fxy = mxy * taus1 * dxVy_dyVx;
             T Ryz_n = 0, Rxz_n = 0, Rxy_n = 0;
 #pragma unroll
             for (int m = 0; m < NM3; m++) {
-                T ryz = a.c1[m] * a.R[9 + m][c] - a.c2[m] * fyz * dt;
-                T rxz = a.c1[m] * a.R[12 + m][c] - a.c2[m] * fxz * dt;
-                T rxy = a.c1[m] * a.R[15 + m][c] - a.c2[m] * fxy * dt;
-                a.R[9 + m][c] = ryz;
-                a.R[12 + m][c] = rxz;
-                a.R[15 + m][c] = rxy;
+                T ryz = a.c1[m] * ldR<S16>(a, 9 + m, c) - a.c2[m] * fyz * dt;
+                T rxz = a.c1[m] * ldR<S16>(a, 12 + m, c) - a.c2[m] * fxz * dt;
+                T rxy = a.c1[m] * ldR<S16>(a, 15 + m, c) - a.c2[m] * fxy * dt;
+                stR<S16>(a, 9 + m, c, ryz);
+                stR<S16>(a, 12 + m, c, rxz);
+                stR<S16>(a, 15 + m, c, rxy);
                 Ryz_n += a.d1[m] * ryz;
                 Rxz_n += a.d1[m] * rxz;
                 Rxy_n += a.d1[m] * rxy;
@@ -223,19 +273,19 @@ __device__ __forceinline__ void stress_column(const K3<T> &a, int bx, int by, in
 
 // WPE > 0 asks the compiler for at least WPE waves per SIMD (a register
 // budget of 512/WPE VGPRs).
-template <typename T, int BY, int PART, int WPE>
+template <typename T, int BY, int PART, int WPE, bool S16>
 __global__ void __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8)))
 k3_stress_int(const K3<T> a)
 {
     int bx, by, bz;
     if (!tile_of(a, bx, by, bz)) return;
-    stress_column<T, BY, PART>(a, bx, by, bz);
+    stress_column<T, BY, PART, S16>(a, bx, by, bz);
 }
 
 // ---------------------------------------------------------------------------
 // interior velocity
 // ---------------------------------------------------------------------------
-template <typename T, int BY, int WPE>
+template <typename T, int BY, int WPE, bool S16>
 __global__ void __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8)))
 k3_vel_int(const K3<T> a)
 {
@@ -258,9 +308,6 @@ k3_vel_int(const K3<T> a)
     const T *__restrict__ Syz = a.S[3];
     const T *__restrict__ Sxz = a.S[4];
     const T *__restrict__ Sxy = a.S[5];
-    const float *__restrict__ bxm = a.bx;
-    const float *__restrict__ bym = a.by;
-    const float *__restrict__ bzm = a.bz;
 
     int c = at3(px, sxy, i, j, kb);
     T xzm2 = Sxz[c - 2 * sxy], xzm1 = Sxz[c - sxy], xz0 = Sxz[c];
@@ -291,14 +338,343 @@ k3_vel_int(const K3<T> a)
                       + (yz0 - Syz[c - px]) * e0y - (Syz[c + px] - Syz[c - 2 * px]) * e1y
                       + (zzp1 - zz0) * e0z - (zzp2 - zzm1) * e1z;
 
-        Vx[c] = Vx[c] + (T)bxm[c] * d3Sx3 * dt;
-        Vy[c] = Vy[c] + (T)bym[c] * d3Sy3 * dt;
-        Vz[c] = Vz[c] + (T)bzm[c] * d3Sz3 * dt;
+        Vx[c] = Vx[c] + medv<S16>(a, MBX, c, k) * d3Sx3 * dt;
+        Vy[c] = Vy[c] + medv<S16>(a, MBY, c, k) * d3Sy3 * dt;
+        Vz[c] = Vz[c] + medv<S16>(a, MBZ, c, k) * d3Sz3 * dt;
 
         xzm2 = xzm1; xzm1 = xz0; xz0 = xzp1;
         yzm2 = yzm1; yzm1 = yz0; yz0 = yzp1;
         zzm1 = zz0; zz0 = zzp1; zzp1 = zzp2;
     }
+}
+
+// ---------------------------------------------------------------------------
+// 4-wide interior kernels (FP32 arithmetic, FP32 or reduced storage)
+// ---------------------------------------------------------------------------
+// Each thread owns 4 consecutive x-cells of one (j, k) row, so every
+// wavefront moves 1 KiB per streamed array with one 16-byte (non-temporal)
+// access per lane; a wavefront covers a 256-cell x tile.  The grid is flat
+// over k (one plane per workgroup row of tiles); the stencil's x
+// neighbours come from 8-byte halo loads of the adjacent cells, y and z
+// neighbours from 16-byte loads of the adjacent rows and planes, which the
+// L2 and the Infinity Cache serve.  With xcd != 0 every XCD owns one
+// contiguous band of y tiles and walks it plane by plane, so the planes
+// k-2..k+2 that a band needs stay in that XCD's L2.
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f4v ld4(const float *p) { return *(const f4v *)p; }
+__device__ __forceinline__ f4v ld4nt(const float *p) { return __builtin_nontemporal_load((const f4v *)p); }
+__device__ __forceinline__ void st4nt(float *p, f4v v) { __builtin_nontemporal_store(v, (f4v *)p); }
+__device__ __forceinline__ float2 ld2(const float *p) { return *(const float2 *)p; }
+
+typedef _Float16 h4v __attribute__((ext_vector_type(4)));
+
+// four FP16 values (8 bytes, non-temporal) widened to FP32
+__device__ __forceinline__ f4v h4(const __half *p)
+{
+    const h4v h = __builtin_nontemporal_load((const h4v *)p);
+    return __builtin_convertvector(h, f4v);
+}
+
+// four FP32 values rounded to nearest FP16 and stored non-temporally
+__device__ __forceinline__ void sth4(__half *p, f4v v)
+{
+    const h4v h = __builtin_convertvector(v, h4v);
+    __builtin_nontemporal_store(h, (h4v *)p);
+}
+
+// medium value of 4 cells of plane k
+template <bool S16>
+__device__ __forceinline__ f4v med4(const K3<float> &a, int q, int c, int k)
+{
+    if constexpr (S16) {
+        const float2 bs = a.mbs[q][k + KO];
+        f4v v = {bs.x, bs.x, bs.x, bs.x};
+        if (bs.y != 0.0f) v += bs.y * h4(a.hmed[q] + c);
+        return v;
+    } else {
+        return ld4nt(a.med[q] + c);
+    }
+}
+
+template <bool S16>
+__device__ __forceinline__ f4v ldR4(const K3<float> &a, int q, int c)
+{
+    if constexpr (S16) return h4(a.hR[q] + c) * a.r_scale;
+    else return ld4nt(a.R[q] + c);
+}
+
+// masked stores: all four cells, or the valid ones one by one
+__device__ __forceinline__ void st4m(float *p, f4v v, unsigned m)
+{
+    if (m == 15u) {
+        st4nt(p, v);
+    } else {
+        if (m & 1u) p[0] = v.x;
+        if (m & 2u) p[1] = v.y;
+        if (m & 4u) p[2] = v.z;
+        if (m & 8u) p[3] = v.w;
+    }
+}
+
+template <bool S16>
+__device__ __forceinline__ void stR4(const K3<float> &a, int q, int c, f4v v, unsigned m)
+{
+    if constexpr (S16) {
+        const float lim = 65504.0f;
+        v *= a.r_inv;
+        v.x = fminf(fmaxf(v.x, -lim), lim);
+        v.y = fminf(fmaxf(v.y, -lim), lim);
+        v.z = fminf(fmaxf(v.z, -lim), lim);
+        v.w = fminf(fmaxf(v.w, -lim), lim);
+        if (m == 15u) {
+            sth4(a.hR[q] + c, v);
+        } else {
+            if (m & 1u) a.hR[q][c] = __float2half_rn(v.x);
+            if (m & 2u) a.hR[q][c + 1] = __float2half_rn(v.y);
+            if (m & 4u) a.hR[q][c + 2] = __float2half_rn(v.z);
+            if (m & 8u) a.hR[q][c + 3] = __float2half_rn(v.w);
+        }
+    } else {
+        st4m(a.R[q] + c, v, m);
+    }
+}
+
+// Tile decode of the 4-wide kernels: thread -> (i0, j, k) and the mask of
+// its 4 cells inside the interior box.
+__device__ __forceinline__ bool x4_cell(const K3<float> &a, int by, int &i0, int &j, int &k, unsigned &m)
+{
+    const unsigned b = blockIdx.x;
+    unsigned xt, yt, kk;
+    if (a.xcd) {
+        const unsigned nyc = ((unsigned)a.nty + 7u) >> 3;
+        const unsigned loc = b >> 3;
+        xt = loc % (unsigned)a.ntx;
+        const unsigned t = loc / (unsigned)a.ntx;
+        yt = (b & 7u) * nyc + t % nyc;
+        kk = t / nyc;
+        if (yt >= (unsigned)a.nty) return false;
+    } else {
+        xt = b % (unsigned)a.ntx;
+        const unsigned t = b / (unsigned)a.ntx;
+        yt = t % (unsigned)a.nty;
+        kk = t / (unsigned)a.nty;
+    }
+    k = 1 + (int)kk;
+    if (k > a.nz - a.na) return false;
+    i0 = a.ti0 + (int)xt * 256 + (int)threadIdx.x * 4;
+    j = a.na + 1 + (int)yt * by + (int)threadIdx.y;
+    if (j > a.ny - a.na) return false;
+    m = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        if (i0 + q > a.na && i0 + q <= a.nx - a.na) m |= 1u << q;
+    return m != 0;
+}
+
+struct Ord {
+    float e0x, e1x, e0y, e1y, e0z, e1z;
+};
+
+__device__ __forceinline__ Ord order4(const K3<float> &a, int i, int j, int k)
+{
+    i = min(max(i, 1), a.nx);
+    const int4 fs = a.surf[(j - 1) * a.nx + (i - 1)];
+    const int o1 = (k - fs.x) * (fs.y - k), o2 = (k - fs.z) * (fs.w - k);
+    const bool lo = max(o1, o2) >= 0;
+    Ord o;
+    o.e0x = lo ? a.e2[0][0] : a.e4[0][0];
+    o.e1x = lo ? a.e2[0][1] : a.e4[0][1];
+    o.e0y = lo ? a.e2[1][0] : a.e4[1][0];
+    o.e1y = lo ? a.e2[1][1] : a.e4[1][1];
+    o.e0z = lo ? a.e2[2][0] : a.e4[2][0];
+    o.e1z = lo ? a.e2[2][1] : a.e4[2][1];
+    return o;
+}
+
+// normal stresses and their memory variables (m_kernel.f90:176-239)
+template <int BY, bool S16>
+__global__ void __launch_bounds__(64 * BY) k3x_normal(const K3<float> a)
+{
+    int i0, j, k;
+    unsigned msk;
+    if (!x4_cell(a, BY, i0, j, k, msk)) return;
+    const int px = a.px, sxy = a.sxy;
+    const int c = at3(px, sxy, i0, j, k);
+    const float *__restrict__ Vx = a.V[0];
+    const float *__restrict__ Vy = a.V[1];
+    const float *__restrict__ Vz = a.V[2];
+    const f4v vx = ld4(Vx + c);
+    const float2 vxl = ld2(Vx + c - 2);
+    const float vxr = Vx[c + 4];
+    const f4v vy = ld4(Vy + c), vym1 = ld4(Vy + c - px), vym2 = ld4(Vy + c - 2 * px), vyp1 = ld4(Vy + c + px);
+    const f4v vz = ld4(Vz + c), vzm1 = ld4(Vz + c - sxy), vzm2 = ld4(Vz + c - 2 * sxy), vzp1 = ld4(Vz + c + sxy);
+    const f4v lam = med4<S16>(a, MLAM, c, k), mu = med4<S16>(a, MMU, c, k);
+    const f4v taup = med4<S16>(a, MTAUP, c, k), taus = med4<S16>(a, MTAUS, c, k);
+    f4v R[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) R[q] = ldR4<S16>(a, q, c);
+    f4v sxx = ld4nt(a.S[0] + c), syy = ld4nt(a.S[1] + c), szz = ld4nt(a.S[2] + c);
+    const float xs[7] = {vxl.x, vxl.y, vx.x, vx.y, vx.z, vx.w, vxr};
+    const float dt = a.dt;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const Ord o = order4(a, i0 + q, j, k);
+        const float dxVx = (xs[q + 2] - xs[q + 1]) * o.e0x - (xs[q + 3] - xs[q]) * o.e1x;
+        const float dyVy = (vy[q] - vym1[q]) * o.e0y - (vyp1[q] - vym2[q]) * o.e1y;
+        const float dzVz = (vz[q] - vzm1[q]) * o.e0z - (vzp1[q] - vzm2[q]) * o.e1z;
+        const float mu2 = 2 * mu[q];
+        const float lam2mu = lam[q] + mu2;
+        const float taup1 = taup[q], taus1 = taus[q];
+        const float d3v3 = dxVx + dyVy + dzVz;
+        const float dyVy_dzVz = dyVy + dzVz, dxVx_dzVz = dxVx + dzVz, dxVx_dyVy = dxVx + dyVy;
+        const float fxx = lam2mu * taup1 * d3v3 - mu2 * taus1 * dyVy_dzVz;
+        const float fyy = lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dzVz;
+        const float fzz = lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dyVy;
+        float Rxx_n = 0, Ryy_n = 0, Rzz_n = 0;
+#pragma unroll
+        for (int mm = 0; mm < NM3; mm++) {
+            const float rxx = a.c1[mm] * R[0 + mm][q] - a.c2[mm] * fxx * dt;
+            const float ryy = a.c1[mm] * R[3 + mm][q] - a.c2[mm] * fyy * dt;
+            const float rzz = a.c1[mm] * R[6 + mm][q] - a.c2[mm] * fzz * dt;
+            R[0 + mm][q] = rxx;
+            R[3 + mm][q] = ryy;
+            R[6 + mm][q] = rzz;
+            Rxx_n += a.d1[mm] * rxx;
+            Ryy_n += a.d1[mm] * ryy;
+            Rzz_n += a.d1[mm] * rzz;
+        }
+        const float taup_plus1 = 1 + taup1 * (1 + a.d2);
+        const float taus_plus1 = 1 + taus1 * (1 + a.d2);
+        sxx[q] = sxx[q] + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dyVy_dzVz + Rxx_n) * dt;
+        syy[q] = syy[q] + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dzVz + Ryy_n) * dt;
+        szz[q] = szz[q] + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dyVy + Rzz_n) * dt;
+    }
+#pragma unroll
+    for (int q = 0; q < 9; q++) stR4<S16>(a, q, c, R[q], msk);
+    st4m(a.S[0] + c, sxx, msk);
+    st4m(a.S[1] + c, syy, msk);
+    st4m(a.S[2] + c, szz, msk);
+}
+
+// shear stresses and their memory variables (m_kernel.f90:263-315)
+template <int BY, bool S16>
+__global__ void __launch_bounds__(64 * BY) k3x_shear(const K3<float> a)
+{
+    int i0, j, k;
+    unsigned msk;
+    if (!x4_cell(a, BY, i0, j, k, msk)) return;
+    const int px = a.px, sxy = a.sxy;
+    const int c = at3(px, sxy, i0, j, k);
+    const float *__restrict__ Vx = a.V[0];
+    const float *__restrict__ Vy = a.V[1];
+    const float *__restrict__ Vz = a.V[2];
+    const f4v vx = ld4(Vx + c), vxyp1 = ld4(Vx + c + px), vxyp2 = ld4(Vx + c + 2 * px), vxym1 = ld4(Vx + c - px);
+    const f4v vxzp1 = ld4(Vx + c + sxy), vxzp2 = ld4(Vx + c + 2 * sxy), vxzm1 = ld4(Vx + c - sxy);
+    const f4v vy = ld4(Vy + c), vyzp1 = ld4(Vy + c + sxy), vyzp2 = ld4(Vy + c + 2 * sxy), vyzm1 = ld4(Vy + c - sxy);
+    const float2 vyl = ld2(Vy + c - 2), vyr = ld2(Vy + c + 4);
+    const f4v vz = ld4(Vz + c), vzyp1 = ld4(Vz + c + px), vzyp2 = ld4(Vz + c + 2 * px), vzym1 = ld4(Vz + c - px);
+    const float2 vzl = ld2(Vz + c - 2), vzr = ld2(Vz + c + 4);
+    const f4v taus = med4<S16>(a, MTAUS, c, k);
+    const f4v myz = med4<S16>(a, MMUYZ, c, k), mxz = med4<S16>(a, MMUXZ, c, k), mxy = med4<S16>(a, MMUXY, c, k);
+    f4v R[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) R[q] = ldR4<S16>(a, 9 + q, c);
+    f4v syz = ld4nt(a.S[3] + c), sxz = ld4nt(a.S[4] + c), sxy_ = ld4nt(a.S[5] + c);
+    // x rows i0-1 .. i0+5 of Vy and Vz
+    const float ys[7] = {vyl.y, vy.x, vy.y, vy.z, vy.w, vyr.x, vyr.y};
+    const float zs[7] = {vzl.y, vz.x, vz.y, vz.z, vz.w, vzr.x, vzr.y};
+    const float dt = a.dt;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const Ord o = order4(a, i0 + q, j, k);
+        const float dxVy_dyVx = (ys[q + 2] - ys[q + 1]) * o.e0x - (ys[q + 3] - ys[q]) * o.e1x
+                              + (vxyp1[q] - vx[q]) * o.e0y - (vxyp2[q] - vxym1[q]) * o.e1y;
+        const float dxVz_dzVx = (zs[q + 2] - zs[q + 1]) * o.e0x - (zs[q + 3] - zs[q]) * o.e1x
+                              + (vxzp1[q] - vx[q]) * o.e0z - (vxzp2[q] - vxzm1[q]) * o.e1z;
+        const float dyVz_dzVy = (vzyp1[q] - vz[q]) * o.e0y - (vzyp2[q] - vzym1[q]) * o.e1y
+                              + (vyzp1[q] - vy[q]) * o.e0z - (vyzp2[q] - vyzm1[q]) * o.e1z;
+        const float taus1 = taus[q];
+        const float fyz = myz[q] * taus1 * dyVz_dzVy;
+        const float fxz = mxz[q] * taus1 * dxVz_dzVx;
+        const float fxy = mxy[q] * taus1 * dxVy_dyVx;
+        float Ryz_n = 0, Rxz_n = 0, Rxy_n = 0;
+#pragma unroll
+        for (int mm = 0; mm < NM3; mm++) {
+            const float ryz = a.c1[mm] * R[0 + mm][q] - a.c2[mm] * fyz * dt;
+            const float rxz = a.c1[mm] * R[3 + mm][q] - a.c2[mm] * fxz * dt;
+            const float rxy = a.c1[mm] * R[6 + mm][q] - a.c2[mm] * fxy * dt;
+            R[0 + mm][q] = ryz;
+            R[3 + mm][q] = rxz;
+            R[6 + mm][q] = rxy;
+            Ryz_n += a.d1[mm] * ryz;
+            Rxz_n += a.d1[mm] * rxz;
+            Rxy_n += a.d1[mm] * rxy;
+        }
+        const float taus_plus1 = 1 + taus1 * (1 + a.d2);
+        syz[q] = syz[q] + (myz[q] * taus_plus1 * dyVz_dzVy + Ryz_n) * dt;
+        sxz[q] = sxz[q] + (mxz[q] * taus_plus1 * dxVz_dzVx + Rxz_n) * dt;
+        sxy_[q] = sxy_[q] + (mxy[q] * taus_plus1 * dxVy_dyVx + Rxy_n) * dt;
+    }
+#pragma unroll
+    for (int q = 0; q < 9; q++) stR4<S16>(a, 9 + q, c, R[q], msk);
+    st4m(a.S[3] + c, syz, msk);
+    st4m(a.S[4] + c, sxz, msk);
+    st4m(a.S[5] + c, sxy_, msk);
+}
+
+// velocity (m_kernel.f90:99-129)
+template <int BY, bool S16>
+__global__ void __launch_bounds__(64 * BY) k3x_vel(const K3<float> a)
+{
+    int i0, j, k;
+    unsigned msk;
+    if (!x4_cell(a, BY, i0, j, k, msk)) return;
+    const int px = a.px, sxy = a.sxy;
+    const int c = at3(px, sxy, i0, j, k);
+    const float *__restrict__ Sxx = a.S[0];
+    const float *__restrict__ Syy = a.S[1];
+    const float *__restrict__ Szz = a.S[2];
+    const float *__restrict__ Syz = a.S[3];
+    const float *__restrict__ Sxz = a.S[4];
+    const float *__restrict__ Sxy = a.S[5];
+    const f4v sxx = ld4(Sxx + c);
+    const float2 sxxl = ld2(Sxx + c - 2), sxxr = ld2(Sxx + c + 4);
+    const f4v syy = ld4(Syy + c), syyp1 = ld4(Syy + c + px), syyp2 = ld4(Syy + c + 2 * px), syym1 = ld4(Syy + c - px);
+    const f4v szz = ld4(Szz + c), szzp1 = ld4(Szz + c + sxy), szzp2 = ld4(Szz + c + 2 * sxy), szzm1 = ld4(Szz + c - sxy);
+    const f4v sxyv = ld4(Sxy + c), sxyym1 = ld4(Sxy + c - px), sxyym2 = ld4(Sxy + c - 2 * px), sxyyp1 = ld4(Sxy + c + px);
+    const float2 sxyl = ld2(Sxy + c - 2), sxyr = ld2(Sxy + c + 4);
+    const f4v sxzv = ld4(Sxz + c), sxzzm1 = ld4(Sxz + c - sxy), sxzzm2 = ld4(Sxz + c - 2 * sxy), sxzzp1 = ld4(Sxz + c + sxy);
+    const float2 sxzl = ld2(Sxz + c - 2), sxzr = ld2(Sxz + c + 4);
+    const f4v syzv = ld4(Syz + c), syzym1 = ld4(Syz + c - px), syzym2 = ld4(Syz + c - 2 * px), syzyp1 = ld4(Syz + c + px);
+    const f4v syzzm1 = ld4(Syz + c - sxy), syzzm2 = ld4(Syz + c - 2 * sxy), syzzp1 = ld4(Syz + c + sxy);
+    const f4v bx = med4<S16>(a, MBX, c, k), by = med4<S16>(a, MBY, c, k), bz = med4<S16>(a, MBZ, c, k);
+    f4v vx = ld4nt(a.V[0] + c), vy = ld4nt(a.V[1] + c), vz = ld4nt(a.V[2] + c);
+    // x rows i0-2 .. i0+5
+    const float xx[8] = {sxxl.x, sxxl.y, sxx.x, sxx.y, sxx.z, sxx.w, sxxr.x, sxxr.y};
+    const float xy[8] = {sxyl.x, sxyl.y, sxyv.x, sxyv.y, sxyv.z, sxyv.w, sxyr.x, sxyr.y};
+    const float xz[8] = {sxzl.x, sxzl.y, sxzv.x, sxzv.y, sxzv.z, sxzv.w, sxzr.x, sxzr.y};
+    const float dt = a.dt;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const Ord o = order4(a, i0 + q, j, k);
+        const int x = q + 2;  // index of cell i0+q in the x rows
+        const float d3Sx3 = (xx[x + 1] - xx[x]) * o.e0x - (xx[x + 2] - xx[x - 1]) * o.e1x
+                          + (sxyv[q] - sxyym1[q]) * o.e0y - (sxyyp1[q] - sxyym2[q]) * o.e1y
+                          + (sxzv[q] - sxzzm1[q]) * o.e0z - (sxzzp1[q] - sxzzm2[q]) * o.e1z;
+        const float d3Sy3 = (xy[x] - xy[x - 1]) * o.e0x - (xy[x + 1] - xy[x - 2]) * o.e1x
+                          + (syyp1[q] - syy[q]) * o.e0y - (syyp2[q] - syym1[q]) * o.e1y
+                          + (syzv[q] - syzzm1[q]) * o.e0z - (syzzp1[q] - syzzm2[q]) * o.e1z;
+        const float d3Sz3 = (xz[x] - xz[x - 1]) * o.e0x - (xz[x + 1] - xz[x - 2]) * o.e1x
+                          + (syzv[q] - syzym1[q]) * o.e0y - (syzyp1[q] - syzym2[q]) * o.e1y
+                          + (szzp1[q] - szz[q]) * o.e0z - (szzp2[q] - szzm1[q]) * o.e1z;
+        vx[q] = vx[q] + bx[q] * d3Sx3 * dt;
+        vy[q] = vy[q] + by[q] * d3Sy3 * dt;
+        vz[q] = vz[q] + bz[q] * d3Sz3 * dt;
+    }
+    st4m(a.V[0] + c, vx, msk);
+    st4m(a.V[1] + c, vy, msk);
+    st4m(a.V[2] + c, vz, msk);
 }
 
 // ---------------------------------------------------------------------------
@@ -323,7 +699,7 @@ __device__ inline bool pml_cell(const Boxes3 &B, unsigned id, int &i, int &j, in
     return true;
 }
 
-template <typename T>
+template <typename T, bool S16>
 __global__ void __launch_bounds__(256) k3_stress_pml(const K3<T> a)
 {
     int i, j, k;
@@ -342,33 +718,26 @@ __global__ void __launch_bounds__(256) k3_stress_pml(const K3<T> a)
     const T gze1 = a.gz[4 * nz + k - 1], gze2 = a.gz[5 * nz + k - 1], gze3 = a.gz[6 * nz + k - 1], gze4 = a.gz[7 * nz + k - 1];
     const T r20x = a.r2[0], r20y = a.r2[1], r20z = a.r2[2];
     const T dt = a.dt;
-    T *__restrict__ axVx = a.A[0];
-    T *__restrict__ ayVx = a.A[1];
-    T *__restrict__ azVx = a.A[2];
-    T *__restrict__ axVy = a.A[3];
-    T *__restrict__ ayVy = a.A[4];
-    T *__restrict__ azVy = a.A[5];
-    T *__restrict__ axVz = a.A[6];
-    T *__restrict__ ayVz = a.A[7];
-    T *__restrict__ azVz = a.A[8];
-
     const T vx0 = Vx[c], vy0 = Vy[c], vz0 = Vz[c];
     // normal (m_absorb_p.f90:445-462)
     const T dxVx = (vx0 - Vx[c - 1]) * r20x;
     const T dyVy = (vy0 - Vy[c - px]) * r20y;
     const T dzVz = (vz0 - Vz[c - sxy]) * r20z;
-    const T lam2mu_R = (T)(a.lam[c] + 2 * a.mu[c]);
-    const T lam_R = lam2mu_R - 2 * (T)a.mu[c];
-    const T axvx = axVx[c], ayvy = ayVy[c], azvz = azVz[c];
+    const T mu0 = medv<S16>(a, MMU, c, k);
+    T lam2mu_R;
+    if constexpr (S16) lam2mu_R = medv<S16>(a, MLAM, c, k) + 2 * mu0;
+    else lam2mu_R = (T)(a.med[MLAM][c] + 2 * a.med[MMU][c]);
+    const T lam_R = lam2mu_R - 2 * mu0;
+    const T axvx = ldA<S16>(a, 0, c), ayvy = ldA<S16>(a, 4, c), azvz = ldA<S16>(a, 8, c);
     const T dxVx_ade = gxc1 * dxVx + gxc2 * axvx;
     const T dyVy_ade = gyc1 * dyVy + gyc2 * ayvy;
     const T dzVz_ade = gzc1 * dzVz + gzc2 * azvz;
     a.S[0][c] = a.S[0][c] + (lam2mu_R * dxVx_ade + lam_R * (dyVy_ade + dzVz_ade)) * dt;
     a.S[1][c] = a.S[1][c] + (lam2mu_R * dyVy_ade + lam_R * (dxVx_ade + dzVz_ade)) * dt;
     a.S[2][c] = a.S[2][c] + (lam2mu_R * dzVz_ade + lam_R * (dxVx_ade + dyVy_ade)) * dt;
-    axVx[c] = gxc3 * axvx + gxc4 * dxVx * dt;
-    ayVy[c] = gyc3 * ayvy + gyc4 * dyVy * dt;
-    azVz[c] = gzc3 * azvz + gzc4 * dzVz * dt;
+    stA<S16>(a, 0, c, gxc3 * axvx + gxc4 * dxVx * dt);
+    stA<S16>(a, 4, c, gyc3 * ayvy + gyc4 * dyVy * dt);
+    stA<S16>(a, 8, c, gzc3 * azvz + gzc4 * dzVz * dt);
 
     // shear (m_absorb_p.f90:469-491)
     const T dxVy = (Vy[c + 1] - vy0) * r20x;
@@ -377,19 +746,20 @@ __global__ void __launch_bounds__(256) k3_stress_pml(const K3<T> a)
     const T dyVz = (Vz[c + px] - vz0) * r20y;
     const T dzVx = (Vx[c + sxy] - vx0) * r20z;
     const T dzVy = (Vy[c + sxy] - vy0) * r20z;
-    const T ayvx = ayVx[c], azvx = azVx[c], axvy = axVy[c], azvy = azVy[c], axvz = axVz[c], ayvz = ayVz[c];
-    a.S[3][c] = a.S[3][c] + (T)a.muyz[c] * (gye1 * dyVz + gze1 * dzVy + gye2 * ayvz + gze2 * azvy) * dt;
-    a.S[4][c] = a.S[4][c] + (T)a.muxz[c] * (gxe1 * dxVz + gze1 * dzVx + gxe2 * axvz + gze2 * azvx) * dt;
-    a.S[5][c] = a.S[5][c] + (T)a.muxy[c] * (gxe1 * dxVy + gye1 * dyVx + gxe2 * axvy + gye2 * ayvx) * dt;
-    ayVx[c] = gye3 * ayvx + gye4 * dyVx * dt;
-    azVx[c] = gze3 * azvx + gze4 * dzVx * dt;
-    axVy[c] = gxe3 * axvy + gxe4 * dxVy * dt;
-    azVy[c] = gze3 * azvy + gze4 * dzVy * dt;
-    axVz[c] = gxe3 * axvz + gxe4 * dxVz * dt;
-    ayVz[c] = gye3 * ayvz + gye4 * dyVz * dt;
+    const T ayvx = ldA<S16>(a, 1, c), azvx = ldA<S16>(a, 2, c), axvy = ldA<S16>(a, 3, c);
+    const T azvy = ldA<S16>(a, 5, c), axvz = ldA<S16>(a, 6, c), ayvz = ldA<S16>(a, 7, c);
+    a.S[3][c] = a.S[3][c] + medv<S16>(a, MMUYZ, c, k) * (gye1 * dyVz + gze1 * dzVy + gye2 * ayvz + gze2 * azvy) * dt;
+    a.S[4][c] = a.S[4][c] + medv<S16>(a, MMUXZ, c, k) * (gxe1 * dxVz + gze1 * dzVx + gxe2 * axvz + gze2 * azvx) * dt;
+    a.S[5][c] = a.S[5][c] + medv<S16>(a, MMUXY, c, k) * (gxe1 * dxVy + gye1 * dyVx + gxe2 * axvy + gye2 * ayvx) * dt;
+    stA<S16>(a, 1, c, gye3 * ayvx + gye4 * dyVx * dt);
+    stA<S16>(a, 2, c, gze3 * azvx + gze4 * dzVx * dt);
+    stA<S16>(a, 3, c, gxe3 * axvy + gxe4 * dxVy * dt);
+    stA<S16>(a, 5, c, gze3 * azvy + gze4 * dzVy * dt);
+    stA<S16>(a, 6, c, gxe3 * axvz + gxe4 * dxVz * dt);
+    stA<S16>(a, 7, c, gye3 * ayvz + gye4 * dyVz * dt);
 }
 
-template <typename T>
+template <typename T, bool S16>
 __global__ void __launch_bounds__(256) k3_vel_pml(const K3<T> a)
 {
     int i, j, k;
@@ -411,16 +781,6 @@ __global__ void __launch_bounds__(256) k3_vel_pml(const K3<T> a)
     const T gze1 = a.gz[4 * nz + k - 1], gze2 = a.gz[5 * nz + k - 1], gze3 = a.gz[6 * nz + k - 1], gze4 = a.gz[7 * nz + k - 1];
     const T r20x = a.r2[0], r20y = a.r2[1], r20z = a.r2[2];
     const T dt = a.dt;
-    T *__restrict__ axSxx = a.A[9];
-    T *__restrict__ aySxy = a.A[10];
-    T *__restrict__ azSxz = a.A[11];
-    T *__restrict__ axSxy = a.A[12];
-    T *__restrict__ aySyy = a.A[13];
-    T *__restrict__ azSyz = a.A[14];
-    T *__restrict__ axSxz = a.A[15];
-    T *__restrict__ aySyz = a.A[16];
-    T *__restrict__ azSzz = a.A[17];
-
     // m_absorb_p.f90:266-298
     const T sxx0 = Sxx[c], syy0 = Syy[c], szz0 = Szz[c], syz0 = Syz[c], sxz0 = Sxz[c], sxy0 = Sxy[c];
     const T dxSxx = (Sxx[c + 1] - sxx0) * r20x;
@@ -432,24 +792,274 @@ __global__ void __launch_bounds__(256) k3_vel_pml(const K3<T> a)
     const T dzSxz = (sxz0 - Sxz[c - sxy]) * r20z;
     const T dxSxy = (sxy0 - Sxy[c - 1]) * r20x;
     const T dySxy = (sxy0 - Sxy[c - px]) * r20y;
-    const T axsxx = axSxx[c], aysxy = aySxy[c], azsxz = azSxz[c];
-    const T axsxy = axSxy[c], aysyy = aySyy[c], azsyz = azSyz[c];
-    const T axsxz = axSxz[c], aysyz = aySyz[c], azszz = azSzz[c];
-    a.V[0][c] = a.V[0][c] + (T)a.bx[c] * (gxe1 * dxSxx + gyc1 * dySxy + gzc1 * dzSxz
-                                          + gxe2 * axsxx + gyc2 * aysxy + gzc2 * azsxz) * dt;
-    a.V[1][c] = a.V[1][c] + (T)a.by[c] * (gxc1 * dxSxy + gye1 * dySyy + gzc1 * dzSyz
-                                          + gxc2 * axsxy + gye2 * aysyy + gzc2 * azsyz) * dt;
-    a.V[2][c] = a.V[2][c] + (T)a.bz[c] * (gxc1 * dxSxz + gyc1 * dySyz + gze1 * dzSzz
-                                          + gxc2 * axsxz + gyc2 * aysyz + gze2 * azszz) * dt;
-    axSxx[c] = gxe3 * axsxx + gxe4 * dxSxx * dt;
-    aySxy[c] = gyc3 * aysxy + gyc4 * dySxy * dt;
-    azSxz[c] = gzc3 * azsxz + gzc4 * dzSxz * dt;
-    axSxy[c] = gxc3 * axsxy + gxc4 * dxSxy * dt;
-    aySyy[c] = gye3 * aysyy + gye4 * dySyy * dt;
-    azSyz[c] = gzc3 * azsyz + gzc4 * dzSyz * dt;
-    axSxz[c] = gxc3 * axsxz + gxc4 * dxSxz * dt;
-    aySyz[c] = gyc3 * aysyz + gyc4 * dySyz * dt;
-    azSzz[c] = gze3 * azszz + gze4 * dzSzz * dt;
+    const T axsxx = ldA<S16>(a, 9, c), aysxy = ldA<S16>(a, 10, c), azsxz = ldA<S16>(a, 11, c);
+    const T axsxy = ldA<S16>(a, 12, c), aysyy = ldA<S16>(a, 13, c), azsyz = ldA<S16>(a, 14, c);
+    const T axsxz = ldA<S16>(a, 15, c), aysyz = ldA<S16>(a, 16, c), azszz = ldA<S16>(a, 17, c);
+    a.V[0][c] = a.V[0][c] + medv<S16>(a, MBX, c, k) * (gxe1 * dxSxx + gyc1 * dySxy + gzc1 * dzSxz
+                                                      + gxe2 * axsxx + gyc2 * aysxy + gzc2 * azsxz) * dt;
+    a.V[1][c] = a.V[1][c] + medv<S16>(a, MBY, c, k) * (gxc1 * dxSxy + gye1 * dySyy + gzc1 * dzSyz
+                                                      + gxc2 * axsxy + gye2 * aysyy + gzc2 * azsyz) * dt;
+    a.V[2][c] = a.V[2][c] + medv<S16>(a, MBZ, c, k) * (gxc1 * dxSxz + gyc1 * dySyz + gze1 * dzSzz
+                                                      + gxc2 * axsxz + gyc2 * aysyz + gze2 * azszz) * dt;
+    stA<S16>(a, 9, c, gxe3 * axsxx + gxe4 * dxSxx * dt);
+    stA<S16>(a, 10, c, gyc3 * aysxy + gyc4 * dySxy * dt);
+    stA<S16>(a, 11, c, gzc3 * azsxz + gzc4 * dzSxz * dt);
+    stA<S16>(a, 12, c, gxc3 * axsxy + gxc4 * dxSxy * dt);
+    stA<S16>(a, 13, c, gye3 * aysyy + gye4 * dySyy * dt);
+    stA<S16>(a, 14, c, gzc3 * azsyz + gzc4 * dzSyz * dt);
+    stA<S16>(a, 15, c, gxc3 * axsxz + gxc4 * dxSxz * dt);
+    stA<S16>(a, 16, c, gyc3 * aysyz + gyc4 * dySyz * dt);
+    stA<S16>(a, 17, c, gze3 * azszz + gze4 * dzSzz * dt);
+}
+
+// ---------------------------------------------------------------------------
+// 4-wide PML kernels (FP32 arithmetic, FP32 or reduced storage)
+// ---------------------------------------------------------------------------
+// The absorber boxes are enumerated in groups of 4 x-cells aligned to
+// i = 1 + 4m (16-byte aligned rows); cells of a group outside its box are
+// masked.  Profiles along x are read as float4, along y and z as scalars.
+struct Box4 {
+    int g0, ngx, i0, i1, j0, nj, k0;
+    unsigned start;
+};
+
+struct Boxes4 {
+    Box4 b[5];
+    int n;
+    unsigned total;
+};
+
+typedef unsigned short u4v __attribute__((ext_vector_type(4)));
+
+template <bool S16>
+__device__ __forceinline__ f4v ldA4(const K3<float> &a, int q, int c)
+{
+    if constexpr (S16) {
+        const u4v r = __builtin_nontemporal_load((const u4v *)(a.hA[q] + c));
+        f4v v;
+        v.x = __uint_as_float((unsigned)r.x << 16);
+        v.y = __uint_as_float((unsigned)r.y << 16);
+        v.z = __uint_as_float((unsigned)r.z << 16);
+        v.w = __uint_as_float((unsigned)r.w << 16);
+        return v;
+    } else {
+        return ld4nt(a.A[q] + c);
+    }
+}
+
+__device__ __forceinline__ unsigned short bf16_rne(float f)
+{
+    unsigned u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+template <bool S16>
+__device__ __forceinline__ void stA4(const K3<float> &a, int q, int c, f4v v, unsigned m)
+{
+    if constexpr (S16) {
+        u4v r;
+        r.x = bf16_rne(v.x);
+        r.y = bf16_rne(v.y);
+        r.z = bf16_rne(v.z);
+        r.w = bf16_rne(v.w);
+        unsigned short *p = (unsigned short *)(a.hA[q] + c);
+        if (m == 15u) {
+            __builtin_nontemporal_store(r, (u4v *)p);
+        } else {
+            if (m & 1u) p[0] = r.x;
+            if (m & 2u) p[1] = r.y;
+            if (m & 4u) p[2] = r.z;
+            if (m & 8u) p[3] = r.w;
+        }
+    } else {
+        st4m(a.A[q] + c, v, m);
+    }
+}
+
+__device__ __forceinline__ bool pml4_cell(const Boxes4 &B, unsigned id, int &i0, int &j, int &k, unsigned &m)
+{
+    if (id >= B.total) return false;
+    int b = 0;
+#pragma unroll
+    for (int q = 1; q < 5; q++)
+        if (q < B.n && id >= B.b[q].start) b = q;
+    const Box4 &x = B.b[b];
+    const unsigned l = id - x.start;
+    const unsigned gx = l % (unsigned)x.ngx;
+    const unsigned t = l / (unsigned)x.ngx;
+    j = x.j0 + (int)(t % (unsigned)x.nj);
+    k = x.k0 + (int)(t / (unsigned)x.nj);
+    i0 = 1 + 4 * (x.g0 + (int)gx);
+    m = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        if (i0 + q >= x.i0 && i0 + q <= x.i1) m |= 1u << q;
+    return true;
+}
+
+// profile row q (0..7) of x at cells i0..i0+3 (profiles padded to 4)
+__device__ __forceinline__ f4v gx4(const K3<float> &a, int q, int i0) { return ld4(a.gxv + q * a.gxp + (i0 - 1)); }
+
+template <bool S16>
+__global__ void __launch_bounds__(256) k3x_stress_pml(const K3<float> a, const Boxes4 B)
+{
+    int i0, j, k;
+    unsigned msk;
+    if (!pml4_cell(B, blockIdx.x * 256u + threadIdx.x, i0, j, k, msk)) return;
+    const int px = a.px, sxy = a.sxy, ny = a.ny, nz = a.nz;
+    const int c = at3(px, sxy, i0, j, k);
+    const float *__restrict__ Vx = a.V[0];
+    const float *__restrict__ Vy = a.V[1];
+    const float *__restrict__ Vz = a.V[2];
+    const f4v gxc1 = gx4(a, 0, i0), gxc2 = gx4(a, 1, i0), gxc3 = gx4(a, 2, i0), gxc4 = gx4(a, 3, i0);
+    const f4v gxe1 = gx4(a, 4, i0), gxe2 = gx4(a, 5, i0), gxe3 = gx4(a, 6, i0), gxe4 = gx4(a, 7, i0);
+    const float gyc1 = a.gy[0 * ny + j - 1], gyc2 = a.gy[1 * ny + j - 1], gyc3 = a.gy[2 * ny + j - 1], gyc4 = a.gy[3 * ny + j - 1];
+    const float gye1 = a.gy[4 * ny + j - 1], gye2 = a.gy[5 * ny + j - 1], gye3 = a.gy[6 * ny + j - 1], gye4 = a.gy[7 * ny + j - 1];
+    const float gzc1 = a.gz[0 * nz + k - 1], gzc2 = a.gz[1 * nz + k - 1], gzc3 = a.gz[2 * nz + k - 1], gzc4 = a.gz[3 * nz + k - 1];
+    const float gze1 = a.gz[4 * nz + k - 1], gze2 = a.gz[5 * nz + k - 1], gze3 = a.gz[6 * nz + k - 1], gze4 = a.gz[7 * nz + k - 1];
+    const float r20x = a.r2[0], r20y = a.r2[1], r20z = a.r2[2], dt = a.dt;
+
+    const f4v vx = ld4(Vx + c), vxyp = ld4(Vx + c + px), vxzp = ld4(Vx + c + sxy);
+    const float vxl = Vx[c - 1];
+    const f4v vy = ld4(Vy + c), vyym = ld4(Vy + c - px), vyzp = ld4(Vy + c + sxy);
+    const float vyr = Vy[c + 4];
+    const f4v vz = ld4(Vz + c), vzzm = ld4(Vz + c - sxy), vzyp = ld4(Vz + c + px);
+    const float vzr = Vz[c + 4];
+    const f4v lam = med4<S16>(a, MLAM, c, k), mu = med4<S16>(a, MMU, c, k);
+    const f4v myz = med4<S16>(a, MMUYZ, c, k), mxz = med4<S16>(a, MMUXZ, c, k), mxy = med4<S16>(a, MMUXY, c, k);
+    f4v s0 = ld4nt(a.S[0] + c), s1 = ld4nt(a.S[1] + c), s2 = ld4nt(a.S[2] + c);
+    f4v s3 = ld4nt(a.S[3] + c), s4 = ld4nt(a.S[4] + c), s5 = ld4nt(a.S[5] + c);
+    f4v A[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) A[q] = ldA4<S16>(a, q, c);
+    // x rows: Vx at i0-1..i0+3, Vy and Vz at i0..i0+4
+    const float xvx[5] = {vxl, vx.x, vx.y, vx.z, vx.w};
+    const float xvy[5] = {vy.x, vy.y, vy.z, vy.w, vyr};
+    const float xvz[5] = {vz.x, vz.y, vz.z, vz.w, vzr};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        // normal (m_absorb_p.f90:445-462)
+        const float dxVx = (xvx[q + 1] - xvx[q]) * r20x;
+        const float dyVy = (vy[q] - vyym[q]) * r20y;
+        const float dzVz = (vz[q] - vzzm[q]) * r20z;
+        float lam2mu_R;
+        if constexpr (S16) lam2mu_R = lam[q] + 2 * mu[q];
+        else lam2mu_R = lam[q] + 2 * mu[q];
+        const float lam_R = lam2mu_R - 2 * mu[q];
+        const float axvx = A[0][q], ayvy = A[4][q], azvz = A[8][q];
+        const float dxVx_ade = gxc1[q] * dxVx + gxc2[q] * axvx;
+        const float dyVy_ade = gyc1 * dyVy + gyc2 * ayvy;
+        const float dzVz_ade = gzc1 * dzVz + gzc2 * azvz;
+        s0[q] = s0[q] + (lam2mu_R * dxVx_ade + lam_R * (dyVy_ade + dzVz_ade)) * dt;
+        s1[q] = s1[q] + (lam2mu_R * dyVy_ade + lam_R * (dxVx_ade + dzVz_ade)) * dt;
+        s2[q] = s2[q] + (lam2mu_R * dzVz_ade + lam_R * (dxVx_ade + dyVy_ade)) * dt;
+        A[0][q] = gxc3[q] * axvx + gxc4[q] * dxVx * dt;
+        A[4][q] = gyc3 * ayvy + gyc4 * dyVy * dt;
+        A[8][q] = gzc3 * azvz + gzc4 * dzVz * dt;
+        // shear (m_absorb_p.f90:469-491)
+        const float dxVy = (xvy[q + 1] - xvy[q]) * r20x;
+        const float dxVz = (xvz[q + 1] - xvz[q]) * r20x;
+        const float dyVx = (vxyp[q] - vx[q]) * r20y;
+        const float dyVz = (vzyp[q] - vz[q]) * r20y;
+        const float dzVx = (vxzp[q] - vx[q]) * r20z;
+        const float dzVy = (vyzp[q] - vy[q]) * r20z;
+        const float ayvx = A[1][q], azvx = A[2][q], axvy = A[3][q], azvy = A[5][q], axvz = A[6][q], ayvz = A[7][q];
+        s3[q] = s3[q] + myz[q] * (gye1 * dyVz + gze1 * dzVy + gye2 * ayvz + gze2 * azvy) * dt;
+        s4[q] = s4[q] + mxz[q] * (gxe1[q] * dxVz + gze1 * dzVx + gxe2[q] * axvz + gze2 * azvx) * dt;
+        s5[q] = s5[q] + mxy[q] * (gxe1[q] * dxVy + gye1 * dyVx + gxe2[q] * axvy + gye2 * ayvx) * dt;
+        A[1][q] = gye3 * ayvx + gye4 * dyVx * dt;
+        A[2][q] = gze3 * azvx + gze4 * dzVx * dt;
+        A[3][q] = gxe3[q] * axvy + gxe4[q] * dxVy * dt;
+        A[5][q] = gze3 * azvy + gze4 * dzVy * dt;
+        A[6][q] = gxe3[q] * axvz + gxe4[q] * dxVz * dt;
+        A[7][q] = gye3 * ayvz + gye4 * dyVz * dt;
+    }
+    st4m(a.S[0] + c, s0, msk);
+    st4m(a.S[1] + c, s1, msk);
+    st4m(a.S[2] + c, s2, msk);
+    st4m(a.S[3] + c, s3, msk);
+    st4m(a.S[4] + c, s4, msk);
+    st4m(a.S[5] + c, s5, msk);
+#pragma unroll
+    for (int q = 0; q < 9; q++) stA4<S16>(a, q, c, A[q], msk);
+}
+
+template <bool S16>
+__global__ void __launch_bounds__(256) k3x_vel_pml(const K3<float> a, const Boxes4 B)
+{
+    int i0, j, k;
+    unsigned msk;
+    if (!pml4_cell(B, blockIdx.x * 256u + threadIdx.x, i0, j, k, msk)) return;
+    const int px = a.px, sxy = a.sxy, ny = a.ny, nz = a.nz;
+    const int c = at3(px, sxy, i0, j, k);
+    const float *__restrict__ Sxx = a.S[0];
+    const float *__restrict__ Syy = a.S[1];
+    const float *__restrict__ Szz = a.S[2];
+    const float *__restrict__ Syz = a.S[3];
+    const float *__restrict__ Sxz = a.S[4];
+    const float *__restrict__ Sxy = a.S[5];
+    const f4v gxc1 = gx4(a, 0, i0), gxc2 = gx4(a, 1, i0), gxc3 = gx4(a, 2, i0), gxc4 = gx4(a, 3, i0);
+    const f4v gxe1 = gx4(a, 4, i0), gxe2 = gx4(a, 5, i0), gxe3 = gx4(a, 6, i0), gxe4 = gx4(a, 7, i0);
+    const float gyc1 = a.gy[0 * ny + j - 1], gyc2 = a.gy[1 * ny + j - 1], gyc3 = a.gy[2 * ny + j - 1], gyc4 = a.gy[3 * ny + j - 1];
+    const float gye1 = a.gy[4 * ny + j - 1], gye2 = a.gy[5 * ny + j - 1], gye3 = a.gy[6 * ny + j - 1], gye4 = a.gy[7 * ny + j - 1];
+    const float gzc1 = a.gz[0 * nz + k - 1], gzc2 = a.gz[1 * nz + k - 1], gzc3 = a.gz[2 * nz + k - 1], gzc4 = a.gz[3 * nz + k - 1];
+    const float gze1 = a.gz[4 * nz + k - 1], gze2 = a.gz[5 * nz + k - 1], gze3 = a.gz[6 * nz + k - 1], gze4 = a.gz[7 * nz + k - 1];
+    const float r20x = a.r2[0], r20y = a.r2[1], r20z = a.r2[2], dt = a.dt;
+
+    const f4v sxx = ld4(Sxx + c);
+    const float sxxr = Sxx[c + 4];
+    const f4v syy = ld4(Syy + c), syyp = ld4(Syy + c + px);
+    const f4v szz = ld4(Szz + c), szzp = ld4(Szz + c + sxy);
+    const f4v syz = ld4(Syz + c), syzym = ld4(Syz + c - px), syzzm = ld4(Syz + c - sxy);
+    const f4v sxz = ld4(Sxz + c), sxzzm = ld4(Sxz + c - sxy);
+    const float sxzl = Sxz[c - 1];
+    const f4v sxy_ = ld4(Sxy + c), sxyym = ld4(Sxy + c - px);
+    const float sxyl = Sxy[c - 1];
+    const f4v bx = med4<S16>(a, MBX, c, k), by = med4<S16>(a, MBY, c, k), bz = med4<S16>(a, MBZ, c, k);
+    f4v vx = ld4nt(a.V[0] + c), vy = ld4nt(a.V[1] + c), vz = ld4nt(a.V[2] + c);
+    f4v A[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) A[q] = ldA4<S16>(a, 9 + q, c);
+    const float xxx[5] = {sxx.x, sxx.y, sxx.z, sxx.w, sxxr};
+    const float xxz[5] = {sxzl, sxz.x, sxz.y, sxz.z, sxz.w};
+    const float xxy[5] = {sxyl, sxy_.x, sxy_.y, sxy_.z, sxy_.w};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        // m_absorb_p.f90:266-298
+        const float dxSxx = (xxx[q + 1] - xxx[q]) * r20x;
+        const float dySyy = (syyp[q] - syy[q]) * r20y;
+        const float dzSzz = (szzp[q] - szz[q]) * r20z;
+        const float dySyz = (syz[q] - syzym[q]) * r20y;
+        const float dzSyz = (syz[q] - syzzm[q]) * r20z;
+        const float dxSxz = (xxz[q + 1] - xxz[q]) * r20x;
+        const float dzSxz = (sxz[q] - sxzzm[q]) * r20z;
+        const float dxSxy = (xxy[q + 1] - xxy[q]) * r20x;
+        const float dySxy = (sxy_[q] - sxyym[q]) * r20y;
+        const float axsxx = A[0][q], aysxy = A[1][q], azsxz = A[2][q];
+        const float axsxy = A[3][q], aysyy = A[4][q], azsyz = A[5][q];
+        const float axsxz = A[6][q], aysyz = A[7][q], azszz = A[8][q];
+        vx[q] = vx[q] + bx[q] * (gxe1[q] * dxSxx + gyc1 * dySxy + gzc1 * dzSxz
+                                 + gxe2[q] * axsxx + gyc2 * aysxy + gzc2 * azsxz) * dt;
+        vy[q] = vy[q] + by[q] * (gxc1[q] * dxSxy + gye1 * dySyy + gzc1 * dzSyz
+                                 + gxc2[q] * axsxy + gye2 * aysyy + gzc2 * azsyz) * dt;
+        vz[q] = vz[q] + bz[q] * (gxc1[q] * dxSxz + gyc1 * dySyz + gze1 * dzSzz
+                                 + gxc2[q] * axsxz + gyc2 * aysyz + gze2 * azszz) * dt;
+        A[0][q] = gxe3[q] * axsxx + gxe4[q] * dxSxx * dt;
+        A[1][q] = gyc3 * aysxy + gyc4 * dySxy * dt;
+        A[2][q] = gzc3 * azsxz + gzc4 * dzSxz * dt;
+        A[3][q] = gxc3[q] * axsxy + gxc4[q] * dxSxy * dt;
+        A[4][q] = gye3 * aysyy + gye4 * dySyy * dt;
+        A[5][q] = gzc3 * azsyz + gzc4 * dzSyz * dt;
+        A[6][q] = gxc3[q] * axsxz + gxc4[q] * dxSxz * dt;
+        A[7][q] = gyc3 * aysyz + gyc4 * dySyz * dt;
+        A[8][q] = gze3 * azszz + gze4 * dzSzz * dt;
+    }
+    st4m(a.V[0] + c, vx, msk);
+    st4m(a.V[1] + c, vy, msk);
+    st4m(a.V[2] + c, vz, msk);
+#pragma unroll
+    for (int q = 0; q < 9; q++) stA4<S16>(a, 9 + q, c, A[q], msk);
 }
 
 // ---------------------------------------------------------------------------
@@ -487,7 +1097,7 @@ __global__ void k3_stressglut(const K3<T> a, int nsrc, const int *__restrict__ s
     }
 }
 
-template <typename T>
+template <typename T, bool S16>
 __global__ void k3_bodyforce(const K3<T> a, int nsrc, const int *__restrict__ src, const double *__restrict__ f3,
                              const double *__restrict__ fdt)
 {
@@ -498,12 +1108,12 @@ __global__ void k3_bodyforce(const K3<T> a, int nsrc, const int *__restrict__ sr
         const T f = (T)fdt[n];
         const T fx = (T)f3[3 * n], fy = (T)f3[3 * n + 1], fz = (T)f3[3 * n + 2];
         const int c = at3(px, sxy, ii, jj, kk);
-        a.V[0][c] += (T)a.bx[c] * fx * f;
-        a.V[0][c - 1] += (T)a.bx[c - 1] * fx * f;
-        a.V[1][c] += (T)a.by[c] * fy * f;
-        a.V[1][c - px] += (T)a.by[c - px] * fy * f;
-        a.V[2][c] += (T)a.bz[c] * fz * f;
-        a.V[2][c - sxy] += (T)a.bz[c - sxy] * fz * f;
+        a.V[0][c] += medv<S16>(a, MBX, c, kk) * fx * f;
+        a.V[0][c - 1] += medv<S16>(a, MBX, c - 1, kk) * fx * f;
+        a.V[1][c] += medv<S16>(a, MBY, c, kk) * fy * f;
+        a.V[1][c - px] += medv<S16>(a, MBY, c - px, kk) * fy * f;
+        a.V[2][c] += medv<S16>(a, MBZ, c, kk) * fz * f;
+        a.V[2][c - sxy] += medv<S16>(a, MBZ, c - sxy, kk - 1) * fz * f;
     }
 }
 
@@ -587,16 +1197,23 @@ struct swpc3d_ctx {
     long long bytes = 0;
     swpc::DBuf V[3], S[6], R[18], A[18];
     swpc::DBuf med[10];     // bx by bz lam mu taup taus muyz muxz muxy
-    swpc::DBuf surf, kob, gx, gy, gz;
+    swpc::DBuf mbs[10];     // reduced storage: per-plane (base, scale)
+    bool s16 = false;       // SWPC_FP32_S16
+    float r_scale = 1.0f;   // memory-variable scale of the reduced storage
+    swpc::DBuf surf, kob, gx, gy, gz, gxv;
     swpc::DBuf src, m6, sdrop, bsrc, f3, fdt;
     swpc::DBuf st, uacc, wvel, wdisp, vmax;
     int nsrc = 0, nt_src = 0, nbf = 0, nt_bf = 0;
     int nst = 0, ntdec_w = 1, ntw = 0;
     bool have_medium = false, have_surface = false, have_pml = false;
     swpc::Boxes3 boxes;
+    swpc::Boxes4 boxes4;
+    int gxp = 0;
     int ti0 = 1;
     // interior launch configuration (swpc3d_set_tuning)
-    int s_by = 4, s_wpe = 0, s_split = 0, v_by = 4, v_wpe = 0, xcd = 1, s_kc = 0, v_kc = 0;
+    int s_by = 4, s_wpe = 3, s_split = 0, v_by = 4, v_wpe = 5, xcd = 0, s_kc = 32, v_kc = 0;
+    int x4 = 1, x4_by = 2;  // 4-wide FP32 / reduced-storage interior kernels
+    int pml4 = 0;           // 4-wide PML kernels (fewer waves per SIMD; off by default)
     swpc::Prof prof;
 };
 
@@ -613,10 +1230,21 @@ static K3<T> make_args(swpc3d_ctx *c)
     for (int q = 0; q < 6; q++) a.S[q] = (T *)c->S[q].p;
     for (int q = 0; q < 18; q++) a.R[q] = (T *)c->R[q].p;
     for (int q = 0; q < 18; q++) a.A[q] = (T *)c->A[q].p;
-    const float **m[10] = {&a.bx, &a.by, &a.bz, &a.lam, &a.mu, &a.taup, &a.taus, &a.muyz, &a.muxz, &a.muxy};
-    for (int q = 0; q < 10; q++) *m[q] = (const float *)c->med[q].p;
+    for (int q = 0; q < 10; q++) a.med[q] = (const float *)c->med[q].p;
+    if (c->s16) {
+        for (int q = 0; q < 10; q++) {
+            a.hmed[q] = (const __half *)c->med[q].p;
+            a.mbs[q] = (const float2 *)c->mbs[q].p;
+        }
+        for (int q = 0; q < 18; q++) a.hR[q] = (__half *)c->R[q].p;
+        for (int q = 0; q < 18; q++) a.hA[q] = (__hip_bfloat16 *)c->A[q].p;
+        a.r_scale = c->r_scale;
+        a.r_inv = 1.0f / c->r_scale;
+    }
     a.surf = (const int4 *)c->surf.p;
     a.gx = (const T *)c->gx.p; a.gy = (const T *)c->gy.p; a.gz = (const T *)c->gz.p;
+    a.gxv = (const float *)c->gxv.p;
+    a.gxp = c->gxp;
     // m_kernel.f90:43-54 evaluated in T, then re40 = rc40 + isign*rd40
     const double h[3] = {d.dx, d.dy, d.dz};
     for (int q = 0; q < 3; q++) {
@@ -666,16 +1294,34 @@ static void set_tiles(const swpc3d_ctx *c, K3<T> &a, int by, int kc)
     a.nblk_pad = (a.nblk + 7u) & ~7u;
 }
 
-template <typename T, int BY, int PART, int WPE>
+template <typename T, int BY, int PART, int WPE, bool S16 = false>
 static void go_stress(const K3<T> &a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k3_stress_int<T, BY, PART, WPE>), dim3(a.xcd ? a.nblk_pad : a.nblk), dim3(64, BY), 0, s, a);
+    hipLaunchKernelGGL((k3_stress_int<T, BY, PART, WPE, S16>), dim3(a.xcd ? a.nblk_pad : a.nblk), dim3(64, BY), 0, s, a);
 }
 
-template <typename T, int BY, int WPE>
+template <typename T, int BY, int WPE, bool S16 = false>
 static void go_vel(const K3<T> &a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k3_vel_int<T, BY, WPE>), dim3(a.xcd ? a.nblk_pad : a.nblk), dim3(64, BY), 0, s, a);
+    hipLaunchKernelGGL((k3_vel_int<T, BY, WPE, S16>), dim3(a.xcd ? a.nblk_pad : a.nblk), dim3(64, BY), 0, s, a);
+}
+
+// Reduced-storage interior variants (FP32 arithmetic).
+template <int PART>
+static bool pick_stress16(int by, int wpe, const K3<float> &a, hipStream_t s)
+{
+#define SWPC_S(BY_, W_) if (by == BY_ && wpe == W_) { go_stress<float, BY_, PART, W_, true>(a, s); return true; }
+    SWPC_S(2, 0) SWPC_S(4, 0) SWPC_S(4, 4) SWPC_S(8, 0)
+#undef SWPC_S
+    return false;
+}
+
+static bool pick_vel16(int by, int wpe, const K3<float> &a, hipStream_t s)
+{
+#define SWPC_V(BY_, W_) if (by == BY_ && wpe == W_) { go_vel<float, BY_, W_, true>(a, s); return true; }
+    SWPC_V(2, 0) SWPC_V(4, 0) SWPC_V(8, 0)
+#undef SWPC_V
+    return false;
 }
 
 // Instantiated interior variants.  FP64 always runs the 64 x 4 fused form
@@ -708,6 +1354,67 @@ static bool pick_vel(int by, int wpe, const K3<T> &a, hipStream_t s)
     }
 }
 
+// Tiling of the 4-wide kernels (256-cell x tiles, by rows, one plane per
+// workgroup row); returns the number of workgroups to launch.
+static unsigned x4_tiles(const swpc3d_ctx *c, K3<float> &a, int by)
+{
+    const swpc3d_desc &d = c->d;
+    a.ti0 = 1 + (d.na / 256) * 256;
+    const int ihi = d.nx - d.na;
+    a.ntx = ihi >= d.na + 1 ? (ihi - a.ti0) / 256 + 1 : 0;
+    const int nj = d.ny - 2 * d.na;
+    a.nty = nj > 0 ? (nj + by - 1) / by : 0;
+    const int nk = d.nz - d.na;
+    a.nkc = nk;
+    a.xcd = c->xcd;
+    if (a.ntx <= 0 || a.nty <= 0 || nk <= 0) return 0;
+    if (c->xcd) return 8u * (unsigned)a.ntx * (((unsigned)a.nty + 7u) / 8u) * (unsigned)nk;
+    return (unsigned)a.ntx * (unsigned)a.nty * (unsigned)nk;
+}
+
+template <int BY>
+static void go_x4_stress(const K3<float> &a, unsigned nb, bool s16, hipStream_t s)
+{
+    if (s16) {
+        hipLaunchKernelGGL((k3x_normal<BY, true>), dim3(nb), dim3(64, BY), 0, s, a);
+        hipLaunchKernelGGL((k3x_shear<BY, true>), dim3(nb), dim3(64, BY), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((k3x_normal<BY, false>), dim3(nb), dim3(64, BY), 0, s, a);
+        hipLaunchKernelGGL((k3x_shear<BY, false>), dim3(nb), dim3(64, BY), 0, s, a);
+    }
+}
+
+template <int BY>
+static void go_x4_vel(const K3<float> &a, unsigned nb, bool s16, hipStream_t s)
+{
+    if (s16) hipLaunchKernelGGL((k3x_vel<BY, true>), dim3(nb), dim3(64, BY), 0, s, a);
+    else hipLaunchKernelGGL((k3x_vel<BY, false>), dim3(nb), dim3(64, BY), 0, s, a);
+}
+
+static bool x4_stress(const swpc3d_ctx *c, K3<float> a, hipStream_t s)
+{
+    const unsigned nb = x4_tiles(c, a, c->x4_by);
+    if (nb == 0) return true;
+    switch (c->x4_by) {
+    case 1: go_x4_stress<1>(a, nb, c->s16, s); return true;
+    case 2: go_x4_stress<2>(a, nb, c->s16, s); return true;
+    case 4: go_x4_stress<4>(a, nb, c->s16, s); return true;
+    default: return false;
+    }
+}
+
+static bool x4_vel(const swpc3d_ctx *c, K3<float> a, hipStream_t s)
+{
+    const unsigned nb = x4_tiles(c, a, c->x4_by);
+    if (nb == 0) return true;
+    switch (c->x4_by) {
+    case 1: go_x4_vel<1>(a, nb, c->s16, s); return true;
+    case 2: go_x4_vel<2>(a, nb, c->s16, s); return true;
+    case 4: go_x4_vel<4>(a, nb, c->s16, s); return true;
+    default: return false;
+    }
+}
+
 template <typename T>
 static int launch_stress(swpc3d_ctx *c)
 {
@@ -719,9 +1426,22 @@ static int launch_stress(swpc3d_ctx *c)
     if (a.nblk > 0) {
         if (int r = prof_begin(c->prof, 0, c->stream, &e0)) return r;
         bool ok;
-        if (c->s_split && !f64) {
-            ok = pick_stress<T, 1>(by, wpe, a, c->stream);
-            if (ok) ok = pick_stress<T, 2>(by, wpe, a, c->stream);
+        if constexpr (sizeof(T) == 4) {
+            if (c->x4) {
+                ok = x4_stress(c, a, c->stream);
+            } else if (c->s16) {
+                if (c->s_split) {
+                    ok = pick_stress16<1>(by, wpe, a, c->stream);
+                    if (ok) ok = pick_stress16<2>(by, wpe, a, c->stream);
+                } else {
+                    ok = pick_stress16<0>(by, wpe, a, c->stream);
+                }
+            } else if (c->s_split) {
+                ok = pick_stress<T, 1>(by, wpe, a, c->stream);
+                if (ok) ok = pick_stress<T, 2>(by, wpe, a, c->stream);
+            } else {
+                ok = pick_stress<T, 0>(by, wpe, a, c->stream);
+            }
         } else {
             ok = pick_stress<T, 0>(by, wpe, a, c->stream);
         }
@@ -731,7 +1451,21 @@ static int launch_stress(swpc3d_ctx *c)
     }
     if (c->boxes.total > 0) {
         if (int r = prof_begin(c->prof, 1, c->stream, &e0)) return r;
-        hipLaunchKernelGGL((k3_stress_pml<T>), dim3((c->boxes.total + 255) / 256), dim3(256), 0, c->stream, a);
+        bool done = false;
+        if constexpr (sizeof(T) == 4) {
+            if (c->pml4) {
+                const unsigned nb = (c->boxes4.total + 255) / 256;
+                if (c->s16) hipLaunchKernelGGL((k3x_stress_pml<true>), dim3(nb), dim3(256), 0, c->stream, a, c->boxes4);
+                else hipLaunchKernelGGL((k3x_stress_pml<false>), dim3(nb), dim3(256), 0, c->stream, a, c->boxes4);
+                done = true;
+            }
+        }
+        if (!done) {
+            if (c->s16)
+                hipLaunchKernelGGL((k3_stress_pml<T, true>), dim3((c->boxes.total + 255) / 256), dim3(256), 0, c->stream, a);
+            else
+                hipLaunchKernelGGL((k3_stress_pml<T, false>), dim3((c->boxes.total + 255) / 256), dim3(256), 0, c->stream, a);
+        }
         SWPC_CHECK_LAUNCH();
         if (int r = prof_end(c->prof, 1, c->stream, e0)) return r;
     }
@@ -748,14 +1482,34 @@ static int launch_vel(swpc3d_ctx *c)
     hipEvent_t e0;
     if (a.nblk > 0) {
         if (int r = prof_begin(c->prof, 2, c->stream, &e0)) return r;
-        if (!pick_vel<T>(by, wpe, a, c->stream))
+        bool ok;
+        if constexpr (sizeof(T) == 4) {
+            if (c->x4) ok = x4_vel(c, a, c->stream);
+            else ok = c->s16 ? pick_vel16(by, wpe, a, c->stream) : pick_vel<T>(by, wpe, a, c->stream);
+        }
+        else ok = pick_vel<T>(by, wpe, a, c->stream);
+        if (!ok)
             return fail(SWPC_EINVAL, "velocity variant by=%d wpe=%d not built", c->v_by, c->v_wpe);
         SWPC_CHECK_LAUNCH();
         if (int r = prof_end(c->prof, 2, c->stream, e0)) return r;
     }
     if (c->boxes.total > 0) {
         if (int r = prof_begin(c->prof, 3, c->stream, &e0)) return r;
-        hipLaunchKernelGGL((k3_vel_pml<T>), dim3((c->boxes.total + 255) / 256), dim3(256), 0, c->stream, a);
+        bool done = false;
+        if constexpr (sizeof(T) == 4) {
+            if (c->pml4) {
+                const unsigned nb = (c->boxes4.total + 255) / 256;
+                if (c->s16) hipLaunchKernelGGL((k3x_vel_pml<true>), dim3(nb), dim3(256), 0, c->stream, a, c->boxes4);
+                else hipLaunchKernelGGL((k3x_vel_pml<false>), dim3(nb), dim3(256), 0, c->stream, a, c->boxes4);
+                done = true;
+            }
+        }
+        if (!done) {
+            if (c->s16)
+                hipLaunchKernelGGL((k3_vel_pml<T, true>), dim3((c->boxes.total + 255) / 256), dim3(256), 0, c->stream, a);
+            else
+                hipLaunchKernelGGL((k3_vel_pml<T, false>), dim3((c->boxes.total + 255) / 256), dim3(256), 0, c->stream, a);
+        }
         SWPC_CHECK_LAUNCH();
         if (int r = prof_end(c->prof, 3, c->stream, e0)) return r;
     }
@@ -780,8 +1534,12 @@ static int launch_bf(swpc3d_ctx *c, int it)
     if (c->nbf == 0) return SWPC_OK;
     if (it < 1 || it > c->nt_bf) return fail(SWPC_EINVAL, "bodyforce: it=%d outside 1..%d", it, c->nt_bf);
     K3<T> a = make_args<T>(c);
-    hipLaunchKernelGGL((k3_bodyforce<T>), dim3(1), dim3(64), 0, c->stream, a, c->nbf, (const int *)c->bsrc.p,
-                       (const double *)c->f3.p, (const double *)c->fdt.p + (size_t)(it - 1) * c->nbf);
+    if (c->s16)
+        hipLaunchKernelGGL((k3_bodyforce<T, true>), dim3(1), dim3(64), 0, c->stream, a, c->nbf, (const int *)c->bsrc.p,
+                           (const double *)c->f3.p, (const double *)c->fdt.p + (size_t)(it - 1) * c->nbf);
+    else
+        hipLaunchKernelGGL((k3_bodyforce<T, false>), dim3(1), dim3(64), 0, c->stream, a, c->nbf, (const int *)c->bsrc.p,
+                           (const double *)c->f3.p, (const double *)c->fdt.p + (size_t)(it - 1) * c->nbf);
     SWPC_CHECK_LAUNCH();
     return SWPC_OK;
 }
@@ -917,7 +1675,7 @@ int swpc3d_create(const swpc3d_desc *desc, swpc3d_ctx **out)
     if (d.na < 0 || 2 * d.na >= d.nx || 2 * d.na >= d.ny || d.na >= d.nz)
         return fail(SWPC_EINVAL, "absorber thickness na=%d does not fit the grid", d.na);
     if (d.nm != NM3) return fail(SWPC_EINVAL, "nm=%d: the kernels are built for NM = 3", d.nm);
-    if (d.precision != SWPC_FP32 && d.precision != SWPC_FP64)
+    if (d.precision != SWPC_FP32 && d.precision != SWPC_FP64 && d.precision != SWPC_FP32_S16)
         return fail(SWPC_EINVAL, "precision mode %d not supported by this build", d.precision);
     if (!(d.dx > 0 && d.dy > 0 && d.dz > 0 && d.dt > 0)) return fail(SWPC_EINVAL, "non-positive dx/dy/dz/dt");
     for (int m = 0; m < d.nm; m++)
@@ -931,14 +1689,18 @@ int swpc3d_create(const swpc3d_desc *desc, swpc3d_ctx **out)
     c->d = d;
     c->L = L;
     c->esz = d.precision == SWPC_FP64 ? 8 : 4;
+    c->s16 = d.precision == SWPC_FP32_S16;
     SWPC_HIP(hipStreamCreate(&c->stream));
     const size_t vb = (size_t)L.vol * c->esz;
     int r;
     for (auto &b : c->V) if ((r = dalloc(b, vb, c->bytes))) return r;
     for (auto &b : c->S) if ((r = dalloc(b, vb, c->bytes))) return r;
-    for (auto &b : c->R) if ((r = dalloc(b, vb, c->bytes))) return r;
-    for (auto &b : c->A) if ((r = dalloc(b, vb, c->bytes))) return r;
-    for (auto &b : c->med) if ((r = dalloc(b, (size_t)L.vol * 4, c->bytes))) return r;
+    const size_t vb16 = (size_t)L.vol * 2;
+    for (auto &b : c->R) if ((r = dalloc(b, c->s16 ? vb16 : vb, c->bytes))) return r;
+    for (auto &b : c->A) if ((r = dalloc(b, c->s16 ? vb16 : vb, c->bytes))) return r;
+    for (auto &b : c->med) if ((r = dalloc(b, c->s16 ? vb16 : (size_t)L.vol * 4, c->bytes))) return r;
+    if (c->s16)
+        for (auto &b : c->mbs) if ((r = dalloc(b, (size_t)L.pz * 8, c->bytes))) return r;
     if ((r = dalloc(c->surf, (size_t)d.nx * d.ny * 16, c->bytes))) return r;
     if ((r = dalloc(c->kob, (size_t)d.nx * d.ny * 4, c->bytes))) return r;
     if ((r = dalloc(c->vmax, 64, c->bytes))) return r;
@@ -964,6 +1726,22 @@ int swpc3d_create(const swpc3d_desc *desc, swpc3d_ctx **out)
         add(d.na + 1, d.nx - d.na, d.ny - d.na + 1, d.ny, 1, d.nz);
         add(d.na + 1, d.nx - d.na, d.na + 1, d.ny - d.na, d.nz - d.na + 1, d.nz);
     }
+    // the same boxes in groups of 4 x-cells for the 4-wide PML kernels
+    Boxes4 &B4 = c->boxes4;
+    memset(&B4, 0, sizeof B4);
+    for (int q = 0; q < B.n; q++) {
+        const Box3 &x = B.b[q];
+        Box4 &y = B4.b[B4.n++];
+        y.i0 = x.i0;
+        y.i1 = x.i0 + x.ni - 1;
+        y.g0 = (y.i0 - 1) / 4;
+        y.ngx = (y.i1 - 1) / 4 - y.g0 + 1;
+        y.j0 = x.j0;
+        y.nj = x.nj;
+        y.k0 = x.k0;
+        y.start = B4.total;
+        B4.total += (unsigned)y.ngx * (unsigned)x.nj * (unsigned)x.nk;
+    }
     *out = c.release();
     return SWPC_OK;
 }
@@ -977,7 +1755,8 @@ int swpc3d_destroy(swpc3d_ctx *c)
     for (auto &b : c->R) dfree(b);
     for (auto &b : c->A) dfree(b);
     for (auto &b : c->med) dfree(b);
-    DBuf *rest[] = {&c->surf, &c->kob, &c->gx, &c->gy, &c->gz, &c->src, &c->m6, &c->sdrop, &c->bsrc, &c->f3,
+    for (auto &b : c->mbs) dfree(b);
+    DBuf *rest[] = {&c->surf, &c->kob, &c->gx, &c->gy, &c->gz, &c->gxv, &c->src, &c->m6, &c->sdrop, &c->bsrc, &c->f3,
                     &c->fdt, &c->st, &c->uacc, &c->wvel, &c->wdisp, &c->vmax};
     for (auto b : rest) dfree(*b);
     c->prof.reset();
@@ -999,7 +1778,36 @@ int swpc3d_set_medium(swpc3d_ctx *c, const float *bx, const float *by, const flo
     std::vector<float> h((size_t)c->L.vol, 0.0f);
     for (int q = 0; q < 10; q++) {
         box_to_dev(c->L, in[q], h.data());
-        SWPC_HIP(hipMemcpy(c->med[q].p, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+        if (!c->s16) {
+            SWPC_HIP(hipMemcpy(c->med[q].p, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+            continue;
+        }
+        // per-plane base (mean over the cells the kernels read, i, j in
+        // [1, n]) and an FP16 residual scaled so that the largest maps to 2^14
+        std::vector<__half> hq((size_t)c->L.vol, __float2half(0.0f));
+        std::vector<float2> bs((size_t)c->L.pz, make_float2(0.0f, 0.0f));
+        const Lay3 &L = c->L;
+        for (int k = 1; k <= L.nz; k++) {
+            double sum = 0;
+            for (int j = 1; j <= L.ny; j++)
+                for (int i = 1; i <= L.nx; i++) sum += h[at3(L.px, L.sxy, i, j, k)];
+            const float base = (float)(sum / ((double)L.nx * L.ny));
+            float mx = 0;
+            for (int j = 1; j <= L.ny; j++)
+                for (int i = 1; i <= L.nx; i++) mx = std::max(mx, std::fabs(h[at3(L.px, L.sxy, i, j, k)] - base));
+            const float scale = mx > 0 ? mx / 16384.0f : 0.0f;
+            bs[k + KO] = make_float2(base, scale);
+            if (scale > 0)
+                for (int j = -2; j <= L.ny + 3; j++)
+                    for (int i = -2; i <= L.nx + 3; i++) {
+                        const int e = at3(L.px, L.sxy, i, j, k);
+                        float v = (h[e] - base) / scale;
+                        v = std::min(std::max(v, -65504.0f), 65504.0f);
+                        hq[e] = __float2half(v);
+                    }
+        }
+        SWPC_HIP(hipMemcpy(c->med[q].p, hq.data(), hq.size() * 2, hipMemcpyHostToDevice));
+        SWPC_HIP(hipMemcpy(c->mbs[q].p, bs.data(), bs.size() * 8, hipMemcpyHostToDevice));
     }
     c->have_medium = true;
     return SWPC_OK;
@@ -1038,6 +1846,19 @@ int swpc3d_set_pml(swpc3d_ctx *c, const float *gxc, const float *gxe, const floa
     SWPC_HIP(hipStreamSynchronize(c->stream));
     int r;
     if ((r = upload_profile(c, c->gx, gxc, gxe, c->d.nx))) return r;
+    {
+        // padded FP32 copy of the x profiles for the 4-wide PML kernels
+        const int n = c->d.nx;
+        c->gxp = (n + 3) / 4 * 4 + 4;
+        std::vector<float> h((size_t)8 * c->gxp, 0.0f);
+        for (int q = 0; q < 4; q++)
+            for (int i = 0; i < n; i++) {
+                h[(size_t)q * c->gxp + i] = gxc[q + 4 * i];
+                h[(size_t)(4 + q) * c->gxp + i] = gxe[q + 4 * i];
+            }
+        if ((r = dalloc(c->gxv, h.size() * 4, c->bytes))) return r;
+        SWPC_HIP(hipMemcpy(c->gxv.p, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+    }
     if ((r = upload_profile(c, c->gy, gyc, gye, c->d.ny))) return r;
     if ((r = upload_profile(c, c->gz, gzc, gze, c->d.nz))) return r;
     c->have_pml = true;
@@ -1080,6 +1901,21 @@ int swpc3d_set_sources(swpc3d_ctx *c, int nsrc, const int *isrc, const int *jsrc
     }
     c->nsrc = nsrc;
     c->nt_src = nt;
+    if (c->s16 && nsrc > 0) {
+        // The stress at any cell stays below the total stress glut injected
+        // at the source, sum_t |sdrop| max|m|; memory variables are smaller
+        // still (of order tau w S).  Map 32x that bound to 2^14 in FP16,
+        // as a power of two so that the scaling itself is exact.
+        double bound = 0, mmax = 0;
+        for (int it = 0; it < nt; it++) {
+            double mx = 0;
+            for (int n = 0; n < nsrc; n++) mx = std::max(mx, std::fabs(sdrop[(size_t)it * nsrc + n]));
+            bound += mx;
+        }
+        for (int q = 0; q < 6 * nsrc; q++) mmax = std::max(mmax, std::fabs(m6[q]));
+        bound *= mmax * 32.0;
+        if (bound > 0) c->r_scale = (float)std::ldexp(1.0, (int)std::ceil(std::log2(bound / 16384.0)));
+    }
     return SWPC_OK;
 }
 
@@ -1205,13 +2041,29 @@ int swpc3d_get_field(swpc3d_ctx *c, int f, double *out)
     DBuf *b = field_buf(c, f);
     if (!b) return fail(SWPC_EINVAL, "unknown field %d", f);
     SWPC_HIP(hipStreamSynchronize(c->stream));
+    const size_t n = (size_t)c->L.vol;
     if (fp64(c)) {
-        std::vector<double> h((size_t)c->L.vol);
-        SWPC_HIP(hipMemcpy(h.data(), b->p, h.size() * 8, hipMemcpyDeviceToHost));
+        std::vector<double> h(n);
+        SWPC_HIP(hipMemcpy(h.data(), b->p, n * 8, hipMemcpyDeviceToHost));
+        dev_to_box(c->L, h.data(), out);
+    } else if (c->s16 && f >= SWPC3D_R0) {
+        std::vector<uint16_t> raw(n);
+        SWPC_HIP(hipMemcpy(raw.data(), b->p, n * 2, hipMemcpyDeviceToHost));
+        std::vector<float> h(n);
+        for (size_t e = 0; e < n; e++) {
+            if (f < SWPC3D_ADE0) {
+                __half_raw hr;
+                hr.x = raw[e];
+                h[e] = __half2float(__half(hr)) * c->r_scale;
+            } else {
+                uint32_t u = (uint32_t)raw[e] << 16;
+                memcpy(&h[e], &u, 4);
+            }
+        }
         dev_to_box(c->L, h.data(), out);
     } else {
-        std::vector<float> h((size_t)c->L.vol);
-        SWPC_HIP(hipMemcpy(h.data(), b->p, h.size() * 4, hipMemcpyDeviceToHost));
+        std::vector<float> h(n);
+        SWPC_HIP(hipMemcpy(h.data(), b->p, n * 4, hipMemcpyDeviceToHost));
         dev_to_box(c->L, h.data(), out);
     }
     return SWPC_OK;
@@ -1224,14 +2076,32 @@ int swpc3d_set_field(swpc3d_ctx *c, int f, const double *in)
     DBuf *b = field_buf(c, f);
     if (!b) return fail(SWPC_EINVAL, "unknown field %d", f);
     SWPC_HIP(hipStreamSynchronize(c->stream));
+    const size_t n = (size_t)c->L.vol;
     if (fp64(c)) {
-        std::vector<double> h((size_t)c->L.vol, 0.0);
+        std::vector<double> h(n, 0.0);
         box_to_dev(c->L, in, h.data());
-        SWPC_HIP(hipMemcpy(b->p, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+        SWPC_HIP(hipMemcpy(b->p, h.data(), n * 8, hipMemcpyHostToDevice));
+    } else if (c->s16 && f >= SWPC3D_R0) {
+        std::vector<float> h(n, 0.0f);
+        box_to_dev(c->L, in, h.data());
+        std::vector<uint16_t> raw(n);
+        for (size_t e = 0; e < n; e++) {
+            if (f < SWPC3D_ADE0) {
+                __half_raw hr = __half_raw(__float2half(h[e] / c->r_scale));
+                raw[e] = hr.x;
+            } else {
+                // round to nearest even, as __float2bfloat16
+                uint32_t u;
+                memcpy(&u, &h[e], 4);
+                u += 0x7FFFu + ((u >> 16) & 1u);
+                raw[e] = (uint16_t)(u >> 16);
+            }
+        }
+        SWPC_HIP(hipMemcpy(b->p, raw.data(), n * 2, hipMemcpyHostToDevice));
     } else {
-        std::vector<float> h((size_t)c->L.vol, 0.0f);
+        std::vector<float> h(n, 0.0f);
         box_to_dev(c->L, in, h.data());
-        SWPC_HIP(hipMemcpy(b->p, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+        SWPC_HIP(hipMemcpy(b->p, h.data(), n * 4, hipMemcpyHostToDevice));
     }
     return SWPC_OK;
 }
@@ -1279,11 +2149,12 @@ int swpc3d_set_tuning(swpc3d_ctx *c, const int *p, int n)
 {
     clear_error();
     if (!c || (n > 0 && !p)) return fail(SWPC_EINVAL, "null argument");
-    int v[8] = {c->s_by, c->s_wpe, c->s_split, c->v_by, c->v_wpe, c->xcd, c->s_kc, c->v_kc};
-    for (int q = 0; q < n && q < 8; q++) v[q] = p[q];
+    int v[11] = {c->s_by, c->s_wpe, c->s_split, c->v_by, c->v_wpe, c->xcd, c->s_kc, c->v_kc, c->x4, c->x4_by, c->pml4};
+    for (int q = 0; q < n && q < 11; q++) v[q] = p[q];
     if (v[6] < 0 || v[7] < 0) return fail(SWPC_EINVAL, "negative z-chunk");
+    if (v[9] != 1 && v[9] != 2 && v[9] != 4) return fail(SWPC_EINVAL, "4-wide tile rows must be 1, 2 or 4");
     c->s_by = v[0]; c->s_wpe = v[1]; c->s_split = v[2] != 0; c->v_by = v[3]; c->v_wpe = v[4];
-    c->xcd = v[5] != 0; c->s_kc = v[6]; c->v_kc = v[7];
+    c->xcd = v[5] != 0; c->s_kc = v[6]; c->v_kc = v[7]; c->x4 = v[8] != 0; c->x4_by = v[9]; c->pml4 = v[10] != 0;
     return SWPC_OK;
 }
 
