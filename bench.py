@@ -51,6 +51,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-chosen", action="store_true")
+    ap.add_argument("--misfit-steps", type=int, default=500)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -74,34 +76,39 @@ def main():
     ncell = g.nx * g.ny * g.nz
     assert n_int + n_pml == ncell
 
-    sim = Swpc3D(g, case.medium, case.pml, case.glut, case.stations, case.ntdec_w, nt)
-    log(f"device memory {sim.memory() / 2**30:.2f} GiB")
-    sim.run(1, args.warmup)
-    sim.sync()
-    sim.profile(True)
-    t1 = time.perf_counter()
-    sim.run(args.warmup + 1, args.steps)
-    sim.sync()
-    wall = time.perf_counter() - t1
-    ms, cnt = sim.profile_read()
-    sim.profile(False)
-    vel, _ = sim.receivers()
-    vmax = float(np.abs(vel).max())
-    sim.close()
-    if not np.isfinite(vmax):
-        raise RuntimeError("non-finite receiver output")
+    import ctypes as C
+    from openswpc_amd._lib import lib
+    gbs = C.c_double(0)
+    copy_bw = None
+    if lib().swpc_membw(1 << 30, 20, C.byref(gbs)) == 0:
+        copy_bw = gbs.value
+        log(f"measured copy bandwidth (1 GiB float4 copy): {copy_bw:.0f} GB/s")
 
+    plain = timed_run(case, "fp32", args.warmup, args.steps)
+    wall, ms, cnt = plain["wall"], plain["ms"], plain["cnt"]
     ms_step = wall / args.steps * 1e3
     cups = ncell * args.steps / wall
-    # dominant kernel: interior stress (class 0)
+    # dominant kernel: the interior stress update (class 0: the normal- and
+    # shear-stress launches of one step)
     avg_stress = ms[0] / max(cnt[0], 1) * 1e-3
     alg_stress = B_INT_STRESS * n_int
     achieved = alg_stress / avg_stress
     step_alg = (B_INT_STRESS + B_INT_VEL) * n_int + (B_PML_STRESS + B_PML_VEL) * n_pml
     kern = {name: {"avg_us": ms[q] / max(cnt[q], 1) * 1e3, "launches": int(cnt[q])}
             for q, name in enumerate(["stress_int", "stress_pml", "vel_int", "vel_pml", "other"])}
-    log(f"timed {args.steps} steps: {wall:.3f} s, {ms_step:.3f} ms/step, {cups / 1e9:.2f} G cell-updates/s, "
-        f"step alg {step_alg * args.steps / wall / 1e12:.2f} TB/s; kernels {kern}")
+    log(f"plain: {args.steps} steps in {wall:.3f} s, {ms_step:.3f} ms/step, {cups / 1e9:.2f} G cell-updates/s, "
+        f"step {step_alg * args.steps / wall / 1e12:.2f} TB/s algorithmic; kernels {kern}")
+
+    chosen = None
+    if not args.no_chosen:
+        red = timed_run(case, "fp32_s16", args.warmup, args.steps)
+        ms_red = red["wall"] / args.steps * 1e3
+        mis = misfit(case, args.misfit_steps)
+        chosen = {"form": "reduced storage: FP32 arithmetic and V/S, FP16 memory variables (power-of-two scale), "
+                          "BF16 PML memory, FP16 medium residuals about FP32 per-plane bases; 20-cell ADE-PML",
+                  "ms_per_step": ms_red, "value": ncell * args.steps / red["wall"],
+                  "speedup_vs_plain": ms_step / ms_red, **mis}
+        log(f"chosen: {ms_red:.3f} ms/step, speedup {ms_step / ms_red:.2f}x; misfit {mis}")
 
     cpu = None
     if not args.no_cpu:
@@ -128,14 +135,15 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k3_stress_int",
+            "kernel": "interior stress update (k3x_normal + k3x_shear launches)",
             "achieved": achieved / 1e9,
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
-            "traffic": None,
+            "traffic": traffic_from_profile(),
             "alg_bytes_per_launch": alg_stress,
             "avg_launch_us": avg_stress * 1e6,
+            "measured_copy_GBs": copy_bw,
         },
         "step_roofline": {
             "alg_bytes_per_step": step_alg,
@@ -143,9 +151,65 @@ def main():
             "frac": step_alg * args.steps / wall / HBM_PEAK,
         },
         "kernels": kern,
+        "chosen": chosen,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def timed_run(case, precision, warmup, steps):
+    """Build the context, run warmup steps, then time `steps` steps with the
+    per-kernel HIP events enabled; inputs are resident in HBM throughout."""
+    from openswpc_amd.solver import Swpc3D
+    g = case.grid
+    nt = warmup + steps
+    sim = Swpc3D(g, case.medium, case.pml, case.glut, case.stations, case.ntdec_w, nt, precision=precision)
+    log(f"{precision}: device memory {sim.memory() / 2**30:.2f} GiB")
+    sim.run(1, warmup)
+    sim.sync()
+    sim.profile(True)
+    t1 = time.perf_counter()
+    sim.run(warmup + 1, steps)
+    sim.sync()
+    wall = time.perf_counter() - t1
+    ms, cnt = sim.profile_read()
+    sim.profile(False)
+    vel, _ = sim.receivers()
+    sim.close()
+    if not np.all(np.isfinite(vel)):
+        raise RuntimeError(f"{precision}: non-finite receiver output")
+    return {"wall": wall, "ms": ms, "cnt": cnt}
+
+
+def misfit(case, steps):
+    """Receiver-trace relative L2 misfit of the plain FP32 and the reduced
+    form against the FP64 run of the same scheme, over the first `steps`
+    steps of the benchmark case."""
+    from openswpc_amd.solver import Swpc3D
+    g = case.grid
+    tr = {}
+    for prec in ("fp64", "fp32", "fp32_s16"):
+        sim = Swpc3D(g, case.medium, case.pml, case.glut, case.stations, case.ntdec_w, steps, precision=prec)
+        sim.run(1, steps)
+        tr[prec] = sim.receivers()[0].astype(np.float64)
+        sim.close()
+    ref = tr["fp64"]
+    n = np.linalg.norm(ref)
+    return {"misfit_steps": steps,
+            "misfit_vs_fp64": float(np.linalg.norm(tr["fp32_s16"] - ref) / n),
+            "plain_misfit_vs_fp64": float(np.linalg.norm(tr["fp32"] - ref) / n)}
+
+
+def traffic_from_profile():
+    """HBM bytes per interior-stress update from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary, if present."""
+    p = ROOT / "profiles" / "stress_int_512x512x256_traffic.json"
+    if not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text()).get("bytes_per_step_update")
+    except Exception:
+        return None
 
 
 def cpu_baseline(case, steps):
