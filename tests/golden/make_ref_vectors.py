@@ -88,6 +88,98 @@ def main():
     p = Path(__file__).with_name("ref_fdtool.json")
     p.write_text(json.dumps(out, indent=1))
     print("wrote", p)
+    sac_vectors(L, rng)
+    readini_vectors(L)
+
+
+def sac_vectors(L, rng):
+    """SAC files written by sac__init + sac__write (m_sac.f90) for a few
+    header/data sets, stored as hex bytes."""
+    import tempfile
+    cases = []
+    deltas = [0.05, 0.01, 0.003, 0.1 * 3, 0.0123456789, 1.0 / 3.0]
+    for n, delta in enumerate(deltas):
+        npts = [1, 7, 16, 33, 5, 64][n]
+        fh = [delta, -1.5 + n, 20.0 + n, -3.25, 4.5, 139.7, 35.6, 140.1, 36.2, 10.5 + n, 1 / np.sqrt(3.0)]
+        ih = [npts, [7, 6, 5][n % 3], 2025 + n, 1 + 60 * n]
+        kst = ["ST01", "N.KI2H", "ABCDEFGHIJ", "", "X", "stn-9"][n]
+        kcm = ["Vx", "Uz", "Sxy", "Vy", "Exz", "Vz"][n]
+        dat = (rng.standard_normal(npts) * 10.0 ** (n - 3)).astype(np.float32)
+        with tempfile.TemporaryDirectory() as td:
+            fn = (Path(td) / "t.sac").as_posix().encode()
+            a = kst.encode()[:8].ljust(8)
+            b = kcm.encode()[:8].ljust(8)
+            L.ref_wsac(fn, len(fn), (C.c_double * 11)(*fh), (C.c_int * 4)(*ih), a, b, npts,
+                       dat.ctypes.data_as(C.POINTER(C.c_float)))
+            raw = Path(fn.decode()).read_bytes()
+        cases.append({"fh": fh, "ih": ih, "kstnm": kst, "kcmpnm": kcm, "data": [float(v) for v in dat],
+                      "bytes": raw.hex()})
+    p = Path(__file__).with_name("ref_sac.json")
+    p.write_text(json.dumps({"fh_order": ["delta", "b", "e", "depmin", "depmax", "stlo", "stla", "evlo", "evla",
+                                          "evdp", "user0"],
+                             "ih_order": ["npts", "idep", "nzyear", "nzjday"], "cases": cases}, indent=1))
+    print("wrote", p)
+
+
+
+INI_TEXT = """# parameter file exercising readini's parsing
+  ! indented comment
+title = 'swpc test'   ! quoted value with a blank
+odir = ./out/dir      # unquoted: the item ends at the slash
+nx= 512, 33
+nxs   =  9
+dx = 1.5d-1
+dt = 5e-3
+fq = .5
+big = 1.0E+02
+neg = -12
+ndig  =  007
+fl1 = .true.
+fl2 = T
+fl3 = false
+fl4 = .F.
+fl5 = .t.
+str2 = "dq ""x"" y"
+str3 = a,b
+str4 = ${SWPC_TEST_DIR}/x${SWPC_TEST_USER}y
+str5 = pre{X}post
+str6 = '${SWPC_TEST_DIR} and more'
+tab\t= 7
+nx = 1000
+ sp_key    =    3.25
+#nz = 99
+nz = 64
+dx2 = 2.
+"""
+INI_ENV = {"SWPC_TEST_DIR": "/data/swpc", "SWPC_TEST_USER": "me"}
+INI_QUERIES = [("title", 3), ("odir", 3), ("str2", 3), ("str3", 3), ("str4", 3), ("str5", 3), ("str6", 3),
+               ("missing", 3), ("nx", 0), ("nxs", 0), ("neg", 0), ("ndig", 0), ("nz", 0), ("tab", 0),
+               ("missing", 0), ("dx", 1), ("dx", 2), ("dt", 1), ("dt", 2), ("fq", 1), ("big", 2), ("sp_key", 1),
+               ("sp_key", 2), ("dx2", 2), ("missing", 2), ("fl1", 4), ("fl2", 4), ("fl3", 4), ("fl4", 4),
+               ("fl5", 4), ("missing", 4), ("n", 0), ("title ", 3)]
+
+
+def readini_vectors(L):
+    """Values returned by readini (m_readini.f90) for INI_TEXT."""
+    import os
+    import tempfile
+    os.environ.update(INI_ENV)
+    out = []
+    with tempfile.TemporaryDirectory() as td:
+        fn = (Path(td) / "t.inf").as_posix()
+        Path(fn).write_text(INI_TEXT)
+        fb = fn.encode()
+        for key, kind in INI_QUERIES:
+            i, r, d, l = C.c_int(-7), C.c_float(2.5), C.c_double(2.5), C.c_int(1)
+            s = C.create_string_buffer(b"dflt", 256)
+            kb = key.encode()
+            L.ref_readini(fb, len(fb), kb, len(kb), kind, C.byref(i), C.byref(r), C.byref(d), s, C.byref(l))
+            val = [i.value, r.value, d.value, s.value.decode().rstrip(" "), bool(l.value)][kind]
+            out.append({"key": key, "kind": kind, "value": val})
+    p = Path(__file__).with_name("ref_readini.json")
+    p.write_text(json.dumps({"text": INI_TEXT, "env": INI_ENV, "defaults": {"i": -7, "r": 2.5, "c": "dflt", "l": True},
+                             "queries": out}, indent=1))
+    print("wrote", p)
 
 
 if __name__ == "__main__":
