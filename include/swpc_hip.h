@@ -55,6 +55,14 @@ enum swpc3d_field {
     SWPC3D_NFIELD = 45
 };
 
+/* Snapshot products of snap__write (src/swpc_3d/m_snap.f90:911-2185;
+   P-SV src/swpc_psv/m_snap.f90:419-650). */
+enum swpc_snap_kind {
+    SWPC_SNAP_PS = 0,     /* div, rot_x, rot_y, rot_z (3-D) or div, rot (P-SV), x UC*M0*1e-3 */
+    SWPC_SNAP_V = 1,      /* velocity x UC*M0 */
+    SWPC_SNAP_U = 2       /* running sum of velocity x UC*M0*dt, advanced every step */
+};
+
 enum swpc2d_field {
     SWPC2D_VX = 0, SWPC2D_VZ, SWPC2D_SXX, SWPC2D_SZZ, SWPC2D_SXZ,
     SWPC2D_R0 = 5,        /* R{xx,zz,xz} x m = 1..3 */
@@ -141,9 +149,27 @@ int swpc3d_bodyforce(swpc3d_ctx *ctx, int it);
 /* wav__store(it) for velocity and displacement (m_wav.f90:400-628);
    displacement is accumulated in FP64. */
 int swpc3d_store_receivers(swpc3d_ctx *ctx, int it);
+/* A snapshot stream of snap__setup/snap__write (m_snap.f90:97-323,
+   911-2185) over an explicit list of cells: ijk[3*npts] holds 1-based
+   (i,j,k) per output point, with 1 <= i <= nx, 1 <= j <= ny,
+   -1 <= k <= nz+2 (the decimated yz/xz/xy section, or the kfs+1 / kob+1
+   surfaces for fs/ob).  A frame of nvar x npts floats (nvar = 4 for 'ps',
+   3 for 'v' and 'u', variable-major) is produced at every it with
+   mod(it-1, ntdec_s) = 0 and kept in a device ring of nbuf frames;
+   'u' accumulates at every step.  uc, m0 and dt are the reference's
+   real(SP) UC, M0 and dt.  *id receives the stream's handle. */
+int swpc3d_add_snapshot(swpc3d_ctx *ctx, int kind, int npts, const int *ijk, int ntdec_s, int nbuf,
+                        float uc, float m0, float dt, int *id);
+/* snap__write(it) alone (step() already calls it). */
+int swpc3d_snapshots(swpc3d_ctx *ctx, int it);
+/* Copy frame (0-based: it = frame*ntdec_s + 1) of stream id to out
+   (synchronises); fails if the frame is not yet written or has been
+   overwritten in the ring. */
+int swpc3d_get_snapshot(swpc3d_ctx *ctx, int id, int frame, float *out);
 /* One time step it in the order of main.f90:125-144: receivers,
-   stress (interior + PML), stress glut, velocity (interior + PML),
-   body force.  The halo exchange is absent: ghost cells stay in HBM. */
+   snapshots, stress (interior + PML), stress glut, velocity (interior +
+   PML), body force.  The halo exchange is absent: ghost cells stay in
+   HBM. */
 int swpc3d_step(swpc3d_ctx *ctx, int it);
 /* Steps it0 .. it0+nsteps-1, enqueued without host synchronisation. */
 int swpc3d_run(swpc3d_ctx *ctx, int it0, int nsteps);
@@ -217,6 +243,13 @@ int swpc2d_stressglut(swpc2d_ctx *ctx, int it);
 int swpc2d_bodyforce(swpc2d_ctx *ctx, int it);
 int swpc2d_store_receivers(swpc2d_ctx *ctx, int it);
 /* one step in the order of src/swpc_psv/main.f90:95-114 */
+/* P-SV snapshot streams (src/swpc_psv/m_snap.f90:80-160, 419-650) over
+   1-based (i,k) cells ik[2*npts]; frames of 2 x npts floats (div, rot /
+   Vx, Vz / Ux, Uz).  Same conventions as swpc3d_add_snapshot. */
+int swpc2d_add_snapshot(swpc2d_ctx *ctx, int kind, int npts, const int *ik, int ntdec_s, int nbuf,
+                        float uc, float m0, float dt, int *id);
+int swpc2d_snapshots(swpc2d_ctx *ctx, int it);
+int swpc2d_get_snapshot(swpc2d_ctx *ctx, int id, int frame, float *out);
 int swpc2d_step(swpc2d_ctx *ctx, int it);
 int swpc2d_run(swpc2d_ctx *ctx, int it0, int nsteps);
 /* vel[itw][n][2], disp[itw][n][2] (x, z-up) */

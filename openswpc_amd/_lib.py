@@ -52,6 +52,9 @@ _PROTOS = {
     "swpc3d_stressglut": (I, [P, I]),
     "swpc3d_bodyforce": (I, [P, I]),
     "swpc3d_store_receivers": (I, [P, I]),
+    "swpc3d_add_snapshot": (I, [P, I, I, PI_, I, I, C.c_float, C.c_float, C.c_float, PI_]),
+    "swpc3d_snapshots": (I, [P, I]),
+    "swpc3d_get_snapshot": (I, [P, I, I, PF]),
     "swpc3d_step": (I, [P, I]),
     "swpc3d_run": (I, [P, I, I]),
     "swpc3d_get_receivers": (I, [P, PF, PD]),
@@ -76,6 +79,9 @@ _PROTOS = {
     "swpc2d_stressglut": (I, [P, I]),
     "swpc2d_bodyforce": (I, [P, I]),
     "swpc2d_store_receivers": (I, [P, I]),
+    "swpc2d_add_snapshot": (I, [P, I, I, PI_, I, I, C.c_float, C.c_float, C.c_float, PI_]),
+    "swpc2d_snapshots": (I, [P, I]),
+    "swpc2d_get_snapshot": (I, [P, I, I, PF]),
     "swpc2d_step": (I, [P, I]),
     "swpc2d_run": (I, [P, I, I]),
     "swpc2d_get_receivers": (I, [P, PF, PD]),

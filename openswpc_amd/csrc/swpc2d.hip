@@ -311,6 +311,48 @@ __global__ void k2_vmax(const K2<T> a, const int *__restrict__ kob, unsigned *__
     }
 }
 
+// P-SV snapshot products at a list of (i,k) cells (src/swpc_psv/m_snap.f90):
+// kind 0 'ps' follows wbuf_xz_ps (461-494): div and rot masked by lam and
+// by the harmonic mu_xz, then x UC*M0*1e-3; kind 1 'v' wbuf_xz_v (552-553);
+// kind 2 'u' wbuf_xz_u (612-613), advanced at every call.  Variable-major
+// output out[n*npts + p]; the reference's operation order, no contraction.
+template <typename T>
+__global__ void k2_snap(const K2<T> a, int kind, int npts, const int2 *__restrict__ pts, float uc, float m0, float dt,
+                        float *__restrict__ acc, float *__restrict__ out, int store)
+{
+#pragma clang fp contract(off)
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npts) return;
+    const int2 q = pts[p];
+    const int px = a.px;
+    const int c = at2(px, q.x, q.y);
+    const T *vx = a.Vx, *vz = a.Vz;
+    if (kind == 0) {
+        const T rx = a.r2[0], rz = a.r2[1];
+        float dv = (float)((vx[c] - vx[c - 1]) * rx + (vz[c] - vz[c - px]) * rz);
+        float rt = (float)((vx[c + px] - vx[c]) * rz - (vz[c + 1] - vz[c]) * rx);
+        const float eps = 1.1920929e-07f;  // epsilon(1.0)
+        const float muxz = harm4(a.mu[c], a.mu[c + px], a.mu[c + 1], a.mu[c + px + 1]);
+        const float lam = a.lam[c];
+        dv = dv * lam / (fabsf(lam) + eps);
+        rt = rt * muxz / fabsf(muxz + eps);
+        out[p] = dv * uc * m0 * 1e-3f;
+        out[npts + p] = rt * uc * m0 * 1e-3f;
+    } else if (kind == 1) {
+        out[p] = (float)(vx[c] * (T)uc * (T)m0);
+        out[npts + p] = (float)(vz[c] * (T)uc * (T)m0);
+    } else {
+        const float ux = (float)((T)acc[p] + vx[c] * (T)uc * (T)m0 * (T)dt);
+        const float uz = (float)((T)acc[npts + p] + vz[c] * (T)uc * (T)m0 * (T)dt);
+        acc[p] = ux;
+        acc[npts + p] = uz;
+        if (store) {
+            out[p] = ux;
+            out[npts + p] = uz;
+        }
+    }
+}
+
 }  // namespace swpc
 
 struct swpc2d_ctx {
@@ -329,6 +371,13 @@ struct swpc2d_ctx {
     bool have_medium = false, have_surface = false, have_pml = false;
     swpc::Boxes2 boxes;
     int ti0 = 1, ntx = 0, ntz = 0;
+    struct Snap {
+        int kind, npts, ntdec, nbuf;
+        float uc, m0, dt;
+        swpc::DBuf pts, acc, frames;
+        long long last = -1;
+    };
+    std::vector<Snap> snaps;
 };
 
 namespace swpc {
@@ -454,19 +503,39 @@ static int l2_recv(swpc2d_ctx *c, int it)
     return SWPC_OK;
 }
 
-// P-SV step order (src/swpc_psv/main.f90:95-114): receivers, stress,
+template <typename T>
+static int l2_snap(swpc2d_ctx *c, int it)
+{
+    if (c->snaps.empty()) return SWPC_OK;
+    K2<T> a = make_args2<T>(c);
+    for (auto &sn : c->snaps) {
+        const bool due = ((it - 1) % sn.ntdec) == 0;
+        if (!due && sn.kind != 2) continue;
+        const long long frame = (it - 1) / sn.ntdec;
+        float *out = (float *)sn.frames.p + (size_t)(frame % sn.nbuf) * 2 * sn.npts;
+        hipLaunchKernelGGL((k2_snap<T>), dim3((sn.npts + 255) / 256), dim3(256), 0, c->stream, a, sn.kind, sn.npts,
+                           (const int2 *)sn.pts.p, sn.uc, sn.m0, sn.dt, (float *)sn.acc.p, out, due ? 1 : 0);
+        SWPC_CHECK_LAUNCH();
+        if (due) sn.last = std::max(sn.last, frame);
+    }
+    return SWPC_OK;
+}
+
+// P-SV step order (src/swpc_psv/main.f90:95-114): snapshots, receivers, stress,
 // absorber, stress glut, velocity, body force, absorber.  The additive
 // body force commutes with the absorber update.
 static int step2(swpc2d_ctx *c, int it)
 {
     int r;
     if (fp64(c)) {
+        if ((r = l2_snap<double>(c, it))) return r;
         if ((r = l2_recv<double>(c, it))) return r;
         if ((r = l2_stress<double>(c))) return r;
         if ((r = l2_glut<double>(c, it))) return r;
         if ((r = l2_vel<double>(c))) return r;
         if ((r = l2_bf<double>(c, it))) return r;
     } else {
+        if ((r = l2_snap<float>(c, it))) return r;
         if ((r = l2_recv<float>(c, it))) return r;
         if ((r = l2_stress<float>(c))) return r;
         if ((r = l2_glut<float>(c, it))) return r;
@@ -594,6 +663,11 @@ int swpc2d_destroy(swpc2d_ctx *c)
     DBuf *rest[] = {&c->surf, &c->kob, &c->gx, &c->gz, &c->src, &c->m3, &c->sdrop, &c->bsrc, &c->f2,
                     &c->fdt, &c->st, &c->uacc, &c->wvel, &c->wdisp, &c->vmax};
     for (auto b : rest) dfree(*b);
+    for (auto &sn : c->snaps) {
+        dfree(sn.pts);
+        dfree(sn.acc);
+        dfree(sn.frames);
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SWPC_OK;
@@ -767,6 +841,62 @@ int swpc2d_store_receivers(swpc2d_ctx *c, int it)
     if (int r = ready2(c)) return r;
     if (it < 1) return fail(SWPC_EINVAL, "it must be >= 1");
     return fp64(c) ? l2_recv<double>(c, it) : l2_recv<float>(c, it);
+}
+
+int swpc2d_add_snapshot(swpc2d_ctx *c, int kind, int npts, const int *ik, int ntdec_s, int nbuf, float uc, float m0,
+                        float dt, int *id)
+{
+    clear_error();
+    if (!c || !ik || !id) return fail(SWPC_EINVAL, "null argument");
+    if (kind < SWPC_SNAP_PS || kind > SWPC_SNAP_U) return fail(SWPC_EINVAL, "unknown snapshot kind %d", kind);
+    if (npts < 1 || ntdec_s < 1 || nbuf < 1) return fail(SWPC_EINVAL, "npts, ntdec_s and nbuf must be >= 1");
+    if ((long long)2 * npts * nbuf >= (1LL << 31)) return fail(SWPC_EINVAL, "snapshot store too large");
+    std::vector<int> p2((size_t)2 * npts);
+    for (int n = 0; n < npts; n++) {
+        const int i = ik[2 * n], k = ik[2 * n + 1];
+        if (i < -1 || i > c->d.nx + 2 || k < -1 || k > c->d.nz + 2)
+            return fail(SWPC_EINVAL, "snapshot point %d at (%d,%d) outside the grid", n, i, k);
+        p2[2 * n] = i;
+        p2[2 * n + 1] = k;
+    }
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    swpc2d_ctx::Snap sn;
+    sn.kind = kind; sn.npts = npts; sn.ntdec = ntdec_s; sn.nbuf = nbuf; sn.uc = uc; sn.m0 = m0; sn.dt = dt;
+    int r;
+    if ((r = dalloc(sn.pts, p2.size() * 4, c->bytes))) return r;
+    if ((r = dalloc(sn.frames, (size_t)2 * npts * nbuf * 4, c->bytes))) { dfree(sn.pts); return r; }
+    if (kind == SWPC_SNAP_U && (r = dalloc(sn.acc, (size_t)2 * npts * 4, c->bytes))) {
+        dfree(sn.pts);
+        dfree(sn.frames);
+        return r;
+    }
+    SWPC_HIP(hipMemcpy(sn.pts.p, p2.data(), p2.size() * 4, hipMemcpyHostToDevice));
+    c->snaps.push_back(sn);
+    *id = (int)c->snaps.size() - 1;
+    return SWPC_OK;
+}
+
+int swpc2d_snapshots(swpc2d_ctx *c, int it)
+{
+    clear_error();
+    if (int r = ready2(c)) return r;
+    if (it < 1) return fail(SWPC_EINVAL, "it must be >= 1");
+    return fp64(c) ? l2_snap<double>(c, it) : l2_snap<float>(c, it);
+}
+
+int swpc2d_get_snapshot(swpc2d_ctx *c, int id, int frame, float *out)
+{
+    clear_error();
+    if (!c || !out) return fail(SWPC_EINVAL, "null argument");
+    if (id < 0 || id >= (int)c->snaps.size()) return fail(SWPC_EINVAL, "unknown snapshot id %d", id);
+    const auto &sn = c->snaps[id];
+    if (frame < 0 || frame > sn.last || frame <= sn.last - sn.nbuf)
+        return fail(SWPC_EINVAL, "snapshot %d frame %d not held (latest %lld, %d buffered)", id, frame, sn.last,
+                    sn.nbuf);
+    const size_t n = (size_t)2 * sn.npts;
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    SWPC_HIP(hipMemcpy(out, (const float *)sn.frames.p + (size_t)(frame % sn.nbuf) * n, n * 4, hipMemcpyDeviceToHost));
+    return SWPC_OK;
 }
 
 int swpc2d_step(swpc2d_ctx *c, int it)

@@ -1180,6 +1180,56 @@ __global__ void k3_vmax(const K3<T> a, const int *__restrict__ kob, unsigned *__
     }
 }
 
+// Snapshot products at a list of cells (snap__write and its wbuf_* loops,
+// m_snap.f90:911-2185).  kind 0 ('ps'): divergence and rotation from
+// one-sided 2nd-order differences, masked by lam / mu_ij and scaled by
+// UC*M0*1e-3 (m_snap.f90:1007-1027); kind 1 ('v'): V*UC*M0 (1469-1472);
+// kind 2 ('u'): running sum of V*UC*M0*dt, advanced at every call
+// (2046-2048).  Output planes are variable-major: out[n*npts + p].
+// Evaluated in the reference's order without contraction.
+template <typename T, bool S16>
+__global__ void k3_snap(const K3<T> a, int kind, int npts, const int4 *__restrict__ pts, float uc, float m0, float dt,
+                        float *__restrict__ acc, float *__restrict__ out, int store)
+{
+#pragma clang fp contract(off)
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npts) return;
+    const int4 q = pts[p];
+    const int px = a.px, sxy = a.sxy;
+    const int c = at3(px, sxy, q.x, q.y, q.z);
+    const T *vx = a.V[0], *vy = a.V[1], *vz = a.V[2];
+    if (kind == 0) {
+        const T rx = a.r2[0], ry = a.r2[1], rz = a.r2[2];
+        float dv = (float)((vx[c] - vx[c - 1]) * rx + (vy[c] - vy[c - px]) * ry + (vz[c] - vz[c - sxy]) * rz);
+        float r1 = (float)((vz[c + px] - vz[c]) * ry - (vy[c + sxy] - vy[c]) * rz);
+        float r2 = (float)((vx[c + sxy] - vx[c]) * rz - (vz[c + 1] - vz[c]) * rx);
+        float r3 = (float)((vy[c + 1] - vy[c]) * rx - (vx[c + px] - vx[c]) * ry);
+        const float eps = 1.1920929e-07f;  // epsilon(1.0)
+        const float lam = (float)medv<S16>(a, MLAM, c, q.z), myz = (float)medv<S16>(a, MMUYZ, c, q.z);
+        const float mxz = (float)medv<S16>(a, MMUXZ, c, q.z), mxy = (float)medv<S16>(a, MMUXY, c, q.z);
+        dv = dv * lam / fabsf(lam + eps);
+        r1 = r1 * myz / fabsf(myz + eps);
+        r2 = r2 * mxz / fabsf(mxz + eps);
+        r3 = r3 * mxy / fabsf(mxy + eps);
+        out[p] = dv * uc * m0 * 1e-3f;
+        out[npts + p] = r1 * uc * m0 * 1e-3f;
+        out[2 * npts + p] = r2 * uc * m0 * 1e-3f;
+        out[3 * npts + p] = r3 * uc * m0 * 1e-3f;
+    } else if (kind == 1) {
+        out[p] = (float)(vx[c] * (T)uc * (T)m0);
+        out[npts + p] = (float)(vy[c] * (T)uc * (T)m0);
+        out[2 * npts + p] = (float)(vz[c] * (T)uc * (T)m0);
+    } else {
+        const T *v[3] = {vx, vy, vz};
+#pragma unroll
+        for (int n = 0; n < 3; n++) {
+            const float u = (float)((T)acc[n * npts + p] + v[n][c] * (T)uc * (T)m0 * (T)dt);
+            acc[n * npts + p] = u;
+            if (store) out[n * npts + p] = u;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
@@ -1215,6 +1265,13 @@ struct swpc3d_ctx {
     int x4 = 1, x4_by = 2;  // 4-wide FP32 / reduced-storage interior kernels
     int pml4 = 0;           // 4-wide PML kernels (fewer waves per SIMD; off by default)
     swpc::Prof prof;
+    struct Snap {
+        int kind, npts, ntdec, nbuf;
+        float uc, m0, dt;
+        swpc::DBuf pts, acc, frames;
+        long long last = -1;  // latest frame index written
+    };
+    std::vector<Snap> snaps;
 };
 
 namespace swpc {
@@ -1575,6 +1632,29 @@ static int launch_vmax(swpc3d_ctx *c)
     return SWPC_OK;
 }
 
+template <typename T>
+static int launch_snap(swpc3d_ctx *c, int it)
+{
+    if (c->snaps.empty()) return SWPC_OK;
+    K3<T> a = make_args<T>(c);
+    for (auto &sn : c->snaps) {
+        const bool due = ((it - 1) % sn.ntdec) == 0;
+        if (!due && sn.kind != 2) continue;
+        const long long frame = (it - 1) / sn.ntdec;
+        const int nvar = sn.kind == 0 ? 4 : 3;
+        float *out = (float *)sn.frames.p + (size_t)(frame % sn.nbuf) * nvar * sn.npts;
+        if (c->s16)
+            hipLaunchKernelGGL((k3_snap<T, true>), dim3((sn.npts + 255) / 256), dim3(256), 0, c->stream, a, sn.kind,
+                               sn.npts, (const int4 *)sn.pts.p, sn.uc, sn.m0, sn.dt, (float *)sn.acc.p, out, due ? 1 : 0);
+        else
+            hipLaunchKernelGGL((k3_snap<T, false>), dim3((sn.npts + 255) / 256), dim3(256), 0, c->stream, a, sn.kind,
+                               sn.npts, (const int4 *)sn.pts.p, sn.uc, sn.m0, sn.dt, (float *)sn.acc.p, out, due ? 1 : 0);
+        SWPC_CHECK_LAUNCH();
+        if (due) sn.last = std::max(sn.last, frame);
+    }
+    return SWPC_OK;
+}
+
 static bool fp64(const swpc3d_ctx *c) { return c->d.precision == SWPC_FP64; }
 
 static int ready(const swpc3d_ctx *c)
@@ -1590,12 +1670,14 @@ static int step(swpc3d_ctx *c, int it)
     int r;
     if (fp64(c)) {
         if ((r = launch_recv<double>(c, it))) return r;
+        if ((r = launch_snap<double>(c, it))) return r;
         if ((r = launch_stress<double>(c))) return r;
         if ((r = launch_glut<double>(c, it))) return r;
         if ((r = launch_vel<double>(c))) return r;
         if ((r = launch_bf<double>(c, it))) return r;
     } else {
         if ((r = launch_recv<float>(c, it))) return r;
+        if ((r = launch_snap<float>(c, it))) return r;
         if ((r = launch_stress<float>(c))) return r;
         if ((r = launch_glut<float>(c, it))) return r;
         if ((r = launch_vel<float>(c))) return r;
@@ -1759,6 +1841,11 @@ int swpc3d_destroy(swpc3d_ctx *c)
     DBuf *rest[] = {&c->surf, &c->kob, &c->gx, &c->gy, &c->gz, &c->gxv, &c->src, &c->m6, &c->sdrop, &c->bsrc, &c->f3,
                     &c->fdt, &c->st, &c->uacc, &c->wvel, &c->wdisp, &c->vmax};
     for (auto b : rest) dfree(*b);
+    for (auto &sn : c->snaps) {
+        dfree(sn.pts);
+        dfree(sn.acc);
+        dfree(sn.frames);
+    }
     c->prof.reset();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -2002,6 +2089,64 @@ int swpc3d_store_receivers(swpc3d_ctx *c, int it)
     if (int r = ready(c)) return r;
     if (it < 1) return fail(SWPC_EINVAL, "it must be >= 1");
     return fp64(c) ? launch_recv<double>(c, it) : launch_recv<float>(c, it);
+}
+
+int swpc3d_add_snapshot(swpc3d_ctx *c, int kind, int npts, const int *ijk, int ntdec_s, int nbuf, float uc, float m0,
+                        float dt, int *id)
+{
+    clear_error();
+    if (!c || !ijk || !id) return fail(SWPC_EINVAL, "null argument");
+    if (kind < SWPC_SNAP_PS || kind > SWPC_SNAP_U) return fail(SWPC_EINVAL, "unknown snapshot kind %d", kind);
+    if (npts < 1 || ntdec_s < 1 || nbuf < 1) return fail(SWPC_EINVAL, "npts, ntdec_s and nbuf must be >= 1");
+    const int nvar = kind == SWPC_SNAP_PS ? 4 : 3;
+    if ((long long)nvar * npts * nbuf >= (1LL << 31)) return fail(SWPC_EINVAL, "snapshot store too large");
+    std::vector<int> p4((size_t)4 * npts);
+    for (int n = 0; n < npts; n++) {
+        const int i = ijk[3 * n], j = ijk[3 * n + 1], k = ijk[3 * n + 2];
+        // the differences reach one cell either side
+        if (i < 1 || i > c->d.nx || j < 1 || j > c->d.ny || k < -1 || k > c->d.nz + 2)
+            return fail(SWPC_EINVAL, "snapshot point %d at (%d,%d,%d) outside the grid", n, i, j, k);
+        p4[4 * n] = i; p4[4 * n + 1] = j; p4[4 * n + 2] = k; p4[4 * n + 3] = 0;
+    }
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    swpc3d_ctx::Snap sn;
+    sn.kind = kind; sn.npts = npts; sn.ntdec = ntdec_s; sn.nbuf = nbuf; sn.uc = uc; sn.m0 = m0; sn.dt = dt;
+    int r;
+    if ((r = dalloc(sn.pts, p4.size() * 4, c->bytes))) return r;
+    if ((r = dalloc(sn.frames, (size_t)nvar * npts * nbuf * 4, c->bytes))) { dfree(sn.pts); return r; }
+    if (kind == SWPC_SNAP_U && (r = dalloc(sn.acc, (size_t)3 * npts * 4, c->bytes))) {
+        dfree(sn.pts);
+        dfree(sn.frames);
+        return r;
+    }
+    SWPC_HIP(hipMemcpy(sn.pts.p, p4.data(), p4.size() * 4, hipMemcpyHostToDevice));
+    c->snaps.push_back(sn);
+    *id = (int)c->snaps.size() - 1;
+    return SWPC_OK;
+}
+
+int swpc3d_snapshots(swpc3d_ctx *c, int it)
+{
+    clear_error();
+    if (int r = ready(c)) return r;
+    if (it < 1) return fail(SWPC_EINVAL, "it must be >= 1");
+    return fp64(c) ? launch_snap<double>(c, it) : launch_snap<float>(c, it);
+}
+
+int swpc3d_get_snapshot(swpc3d_ctx *c, int id, int frame, float *out)
+{
+    clear_error();
+    if (!c || !out) return fail(SWPC_EINVAL, "null argument");
+    if (id < 0 || id >= (int)c->snaps.size()) return fail(SWPC_EINVAL, "unknown snapshot id %d", id);
+    const auto &sn = c->snaps[id];
+    if (frame < 0 || frame > sn.last || frame <= sn.last - sn.nbuf)
+        return fail(SWPC_EINVAL, "snapshot %d frame %d not held (latest %lld, %d buffered)", id, frame, sn.last,
+                    sn.nbuf);
+    const int nvar = sn.kind == SWPC_SNAP_PS ? 4 : 3;
+    const size_t n = (size_t)nvar * sn.npts;
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    SWPC_HIP(hipMemcpy(out, (const float *)sn.frames.p + (size_t)(frame % sn.nbuf) * n, n * 4, hipMemcpyDeviceToHost));
+    return SWPC_OK;
 }
 
 int swpc3d_step(swpc3d_ctx *c, int it)

@@ -141,6 +141,20 @@ class Swpc3D:
             return (vel * np.float32(s)).astype(np.float32), disp * s
         return vel, disp
 
+    def add_snapshot(self, kind, ijk, ntdec_s, nbuf, uc, m0, dt):
+        """Register a snapshot stream over 1-based (i,j,k) cells (snap__write,
+        m_snap.f90); returns its id."""
+        ijk = _i32(np.asarray(ijk).reshape(-1, 3))
+        sid = C.c_int()
+        check(lib().swpc3d_add_snapshot(self.h, int(kind), len(ijk), ptr(ijk, C.c_int), int(ntdec_s), int(nbuf),
+                                        float(uc), float(m0), float(dt), C.byref(sid)), "add_snapshot")
+        return sid.value
+
+    def get_snapshot(self, sid, frame, nvar, npts):
+        out = np.zeros((nvar, npts), np.float32)
+        check(lib().swpc3d_get_snapshot(self.h, int(sid), int(frame), ptr(out, C.c_float)), "get_snapshot")
+        return out
+
     def field(self, name):
         out = np.zeros(self.g.shape, np.float64)
         check(lib().swpc3d_get_field(self.h, FIELDS3.index(name), ptr(out, C.c_double)), "get_field")
@@ -275,6 +289,20 @@ class SwpcPSV:
             s = self.M0 * UC2 * 1e9
             return (vel * np.float32(s)).astype(np.float32), disp * s
         return vel, disp
+
+    def add_snapshot(self, kind, ik, ntdec_s, nbuf, uc, m0, dt):
+        """Register a P-SV snapshot stream over 1-based (i,k) cells
+        (src/swpc_psv/m_snap.f90); returns its id."""
+        ik = _i32(np.asarray(ik).reshape(-1, 2))
+        sid = C.c_int()
+        check(lib().swpc2d_add_snapshot(self.h, int(kind), len(ik), ptr(ik, C.c_int), int(ntdec_s), int(nbuf),
+                                        float(uc), float(m0), float(dt), C.byref(sid)), "add_snapshot")
+        return sid.value
+
+    def get_snapshot(self, sid, frame, nvar, npts):
+        out = np.zeros((nvar, npts), np.float32)
+        check(lib().swpc2d_get_snapshot(self.h, int(sid), int(frame), ptr(out, C.c_float)), "get_snapshot")
+        return out
 
     def field(self, name):
         out = np.zeros(self.g.shape, np.float64)

@@ -170,9 +170,35 @@ class Oracle3D:
             self.L.o3d_stressglut(self._ps(), self.nsrc, self.isrc.ctypes.data_as(_PI), self.jsrc.ctypes.data_as(_PI),
                                   self.ksrc.ctypes.data_as(_PI), self.m6.ctypes.data_as(PD), row.ctypes.data_as(PD))
 
+    def add_snapshot(self, kind, ijk, ntdec_s, uc, m0):
+        """A snapshot stream over 1-based (i,j,k) cells (m_snap.f90); frames
+        are kept in self.snaps[id]['frames']."""
+        ijk = np.ascontiguousarray(ijk, np.int32).reshape(-1, 3)
+        n = len(ijk)
+        nvar = 4 if kind == 0 else 3
+        if not hasattr(self, "snaps"):
+            self.snaps = []
+        self.snaps.append({"kind": kind, "ijk": ijk, "ntdec": ntdec_s, "uc": np.float32(uc), "m0": np.float32(m0),
+                           "acc": np.zeros(3 * n, np.float32), "nvar": nvar, "frames": []})
+        return len(self.snaps) - 1
+
+    def snapshots(self, it):
+        for sn in getattr(self, "snaps", []):
+            due = (it - 1) % sn["ntdec"] == 0
+            if not due and sn["kind"] != 2:
+                continue
+            n = len(sn["ijk"])
+            out = np.zeros(sn["nvar"] * n, np.float32)
+            self.L.o3d_snapshot(self._ps(), sn["kind"], n, sn["ijk"].ctypes.data_as(_PI), C.c_float(sn["uc"]),
+                                C.c_float(sn["m0"]), sn["acc"].ctypes.data_as(_PF), out.ctypes.data_as(_PF))
+            if due:
+                sn["frames"].append(out.reshape(sn["nvar"], n))
+
     def step(self, it):
-        """main.f90:125-144 order (receivers sampled before the updates)."""
+        """main.f90:125-144 order (receivers and snapshots sampled before
+        the updates)."""
         self.receivers(it)
+        self.snapshots(it)
         self.update_stress()
         self.stressglut(it)
         self.update_vel()
@@ -289,7 +315,30 @@ class Oracle2D:
             self.L.o2d_stressglut(self._ps(), self.nsrc, self.isrc.ctypes.data_as(_PI), self.ksrc.ctypes.data_as(_PI),
                                   self.m3.ctypes.data_as(PD), row.ctypes.data_as(PD))
 
+    def add_snapshot(self, kind, ik, ntdec_s, uc, m0):
+        """A P-SV snapshot stream over 1-based (i,k) cells (m_snap.f90)."""
+        ik = np.ascontiguousarray(ik, np.int32).reshape(-1, 2)
+        if not hasattr(self, "snaps"):
+            self.snaps = []
+        self.snaps.append({"kind": kind, "ik": ik, "ntdec": ntdec_s, "uc": np.float32(uc), "m0": np.float32(m0),
+                           "acc": np.zeros(2 * len(ik), np.float32), "frames": []})
+        return len(self.snaps) - 1
+
+    def snapshots(self, it):
+        for sn in getattr(self, "snaps", []):
+            due = (it - 1) % sn["ntdec"] == 0
+            if not due and sn["kind"] != 2:
+                continue
+            n = len(sn["ik"])
+            out = np.zeros(2 * n, np.float32)
+            self.L.o2d_snapshot(self._ps(), sn["kind"], n, sn["ik"].ctypes.data_as(_PI), C.c_float(sn["uc"]),
+                                C.c_float(sn["m0"]), sn["acc"].ctypes.data_as(_PF), out.ctypes.data_as(_PF))
+            if due:
+                sn["frames"].append(out.reshape(2, n))
+
     def step(self, it):
+        """src/swpc_psv/main.f90:95-114 order."""
+        self.snapshots(it)
         self.receivers(it)
         self.update_stress()
         self.stressglut(it)

@@ -436,6 +436,50 @@ void o3d_vmax(o3d_t *s, float *out3)
     out3[0] = xm; out3[1] = ym; out3[2] = zm;
 }
 
+/* Snapshot products on a cell list (snap__write, src/swpc_3d/m_snap.f90):
+   kind 0 'ps' follows wbuf_xy_ps/wbuf_fs_ps (1183-1216, 1278-1304),
+   kind 1 'v' wbuf_xy_v (1600-1606), kind 2 'u' wbuf_fs_u (2041-2049; the
+   running sum acc[3*npts] is advanced at every call).  ijk holds 1-based
+   (i,j,k) triplets; out is variable-major (n*npts + p). */
+void o3d_snapshot(o3d_t *s, int kind, int npts, const int *ijk, float uc, float m0, float *acc, float *out)
+{
+    const REAL r20x = (REAL)1.0 / (REAL)s->dx, r20y = (REAL)1.0 / (REAL)s->dy, r20z = (REAL)1.0 / (REAL)s->dz;
+    const float eps = 1.1920929e-07f; /* epsilon(1.0) */
+    for (int p = 0; p < npts; p++) {
+        const int i = ijk[3 * p], j = ijk[3 * p + 1], k = ijk[3 * p + 2];
+        const size_t c = I3(s, k, i, j);
+        if (kind == 0) {
+            float div = (float)((s->Vx[c] - s->Vx[I3(s, k, i - 1, j)]) * r20x
+                              + (s->Vy[c] - s->Vy[I3(s, k, i, j - 1)]) * r20y
+                              + (s->Vz[c] - s->Vz[I3(s, k - 1, i, j)]) * r20z);
+            float rot_x = (float)((s->Vz[I3(s, k, i, j + 1)] - s->Vz[c]) * r20y
+                                - (s->Vy[I3(s, k + 1, i, j)] - s->Vy[c]) * r20z);
+            float rot_y = (float)((s->Vx[I3(s, k + 1, i, j)] - s->Vx[c]) * r20z
+                                - (s->Vz[I3(s, k, i + 1, j)] - s->Vz[c]) * r20x);
+            float rot_z = (float)((s->Vy[I3(s, k, i + 1, j)] - s->Vy[c]) * r20x
+                                - (s->Vx[I3(s, k, i, j + 1)] - s->Vx[c]) * r20y);
+            div = div * s->lam[c] / fabsf(s->lam[c] + eps);
+            rot_x = rot_x * s->muyz[c] / fabsf(s->muyz[c] + eps);
+            rot_y = rot_y * s->muxz[c] / fabsf(s->muxz[c] + eps);
+            rot_z = rot_z * s->muxy[c] / fabsf(s->muxy[c] + eps);
+            out[p] = div * uc * m0 * 1e-3f;
+            out[npts + p] = rot_x * uc * m0 * 1e-3f;
+            out[2 * npts + p] = rot_y * uc * m0 * 1e-3f;
+            out[3 * npts + p] = rot_z * uc * m0 * 1e-3f;
+        } else {
+            const REAL v[3] = {s->Vx[c], s->Vy[c], s->Vz[c]};
+            for (int n = 0; n < 3; n++) {
+                if (kind == 1) {
+                    out[n * npts + p] = (float)(v[n] * (REAL)uc * (REAL)m0);
+                } else {
+                    acc[n * npts + p] = (float)((REAL)acc[n * npts + p] + v[n] * (REAL)uc * (REAL)m0 * (REAL)s->dt);
+                    out[n * npts + p] = acc[n * npts + p];
+                }
+            }
+        }
+    }
+}
+
 /* One full step in the order of src/swpc_3d/main.f90:125-144 (receivers
    are sampled by the caller before the updates). */
 void o3d_step(o3d_t *s, int nsrc, const int *isrc, const int *jsrc, const int *ksrc,
@@ -659,6 +703,39 @@ void o2d_bodyforce(o2d_t *s, int nsrc, const int *isrc, const int *ksrc, const d
 }
 
 /* src/swpc_psv/m_wav.f90:170-240 (v and u). */
+/* P-SV snapshot products on an (i,k) cell list (src/swpc_psv/m_snap.f90:
+   wbuf_xz_ps 461-494, wbuf_xz_v 552-553, wbuf_xz_u 612-613). */
+void o2d_snapshot(o2d_t *s, int kind, int npts, const int *ik, float uc, float m0, float *acc, float *out)
+{
+    const REAL r20x = (REAL)1.0 / (REAL)s->dx, r20z = (REAL)1.0 / (REAL)s->dz;
+    const float eps = 1.1920929e-07f; /* epsilon(1.0) */
+    for (int p = 0; p < npts; p++) {
+        const int i = ik[2 * p], k = ik[2 * p + 1];
+        const size_t c = J2(s, k, i);
+        if (kind == 0) {
+            float div = (float)((s->Vx[c] - s->Vx[J2(s, k, i - 1)]) * r20x + (s->Vz[c] - s->Vz[J2(s, k - 1, i)]) * r20z);
+            float rot = (float)((s->Vx[J2(s, k + 1, i)] - s->Vx[c]) * r20z - (s->Vz[J2(s, k, i + 1)] - s->Vz[c]) * r20x);
+            float nnn = s->mu[c], pnn = s->mu[J2(s, k + 1, i)], npn = s->mu[J2(s, k, i + 1)], ppn = s->mu[J2(s, k + 1, i + 1)];
+            float mu_xz = 4 * nnn * pnn * npn * ppn
+                        / (nnn * pnn * npn + nnn * pnn * ppn + nnn * npn * ppn + pnn * npn * ppn + eps);
+            div = div * s->lam[c] / (fabsf(s->lam[c]) + eps);
+            rot = rot * mu_xz / fabsf(mu_xz + eps);
+            out[p] = div * uc * m0 * 1e-3f;
+            out[npts + p] = rot * uc * m0 * 1e-3f;
+        } else {
+            const REAL v[2] = {s->Vx[c], s->Vz[c]};
+            for (int n = 0; n < 2; n++) {
+                if (kind == 1) {
+                    out[n * npts + p] = (float)(v[n] * (REAL)uc * (REAL)m0);
+                } else {
+                    acc[n * npts + p] = (float)((REAL)acc[n * npts + p] + v[n] * (REAL)uc * (REAL)m0 * (REAL)s->dt);
+                    out[n * npts + p] = acc[n * npts + p];
+                }
+            }
+        }
+    }
+}
+
 void o2d_receivers(o2d_t *s, int nst, const int *ist, const int *kst, float *vel, double *disp)
 {
     for (int n = 0; n < nst; n++) {
