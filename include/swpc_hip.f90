@@ -1,0 +1,334 @@
+!! swpc_hip.f90 -- Fortran bind(C) interfaces to include/swpc_hip.h.
+!!
+!! A Fortran host (the reference's main.f90 and its modules) reaches the
+!! MI355X time-stepping core through this module.  Every routine is a
+!! function returning 0 on success and a negative code on error; the text
+!! of the last error is swpc_errmsg().  Arrays are passed in the
+!! reference's own layout, (k,i,j) over the memory box [-2:n+3]
+!! (src/swpc_3d/m_global.f90:297-302); an array section such as
+!! lam(-2:nz+3, -2:nx+3, -2:ny+3) of a padded allocation is passed as a
+!! contiguous copy by the assumed-size dummies below.
+module swpc_hip
+
+    use iso_c_binding
+    implicit none
+    private
+
+    integer(c_int), parameter, public :: SWPC_FP32 = 0, SWPC_FP64 = 1, SWPC_FP32_S16 = 2
+    integer(c_int), parameter, public :: SWPC_OK = 0, SWPC_EINVAL = -1, SWPC_EHIP = -2, SWPC_ESTATE = -3, &
+                                         SWPC_ENOMEM = -4
+    integer(c_int), parameter, public :: SWPC_SNAP_PS = 0, SWPC_SNAP_V = 1, SWPC_SNAP_U = 2
+
+    type, bind(C), public :: swpc3d_desc
+        integer(c_int) :: nx, ny, nz, na, nm, precision
+        real(c_double) :: dx, dy, dz, dt
+        real(c_float)  :: ts(8)
+    end type swpc3d_desc
+
+    type, bind(C), public :: swpc2d_desc
+        integer(c_int) :: nx, nz, na, nm, precision
+        real(c_double) :: dx, dz, dt
+        real(c_float)  :: ts(8)
+    end type swpc2d_desc
+
+    public :: swpc_errmsg
+    public :: swpc_version, swpc_device_count
+    public :: swpc3d_create, swpc3d_destroy, swpc3d_set_medium, swpc3d_set_surface, swpc3d_set_pml
+    public :: swpc3d_set_sources, swpc3d_set_bodyforce, swpc3d_set_receivers
+    public :: swpc3d_update_stress, swpc3d_update_vel, swpc3d_stressglut, swpc3d_bodyforce
+    public :: swpc3d_store_receivers, swpc3d_add_snapshot, swpc3d_snapshots, swpc3d_get_snapshot
+    public :: swpc3d_step, swpc3d_run, swpc3d_get_receivers, swpc3d_vmax, swpc3d_sync
+    public :: swpc2d_create, swpc2d_destroy, swpc2d_set_medium, swpc2d_set_surface, swpc2d_set_pml
+    public :: swpc2d_set_sources, swpc2d_set_receivers, swpc2d_update_stress, swpc2d_update_vel
+    public :: swpc2d_stressglut, swpc2d_store_receivers, swpc2d_step, swpc2d_run, swpc2d_get_receivers
+    public :: swpc2d_vmax, swpc2d_sync
+
+    interface
+
+        function swpc_last_error() result(p) bind(C, name='swpc_last_error')
+            import :: c_ptr
+            type(c_ptr) :: p
+        end function
+
+        integer(c_int) function swpc_version() bind(C, name='swpc_version')
+            import :: c_int
+        end function
+
+        integer(c_int) function swpc_device_count() bind(C, name='swpc_device_count')
+            import :: c_int
+        end function
+
+        ! ---------------------------------------------------------------- 3-D
+
+        integer(c_int) function swpc3d_create(desc, ctx) bind(C, name='swpc3d_create')
+            import :: c_int, c_ptr, swpc3d_desc
+            type(swpc3d_desc), intent(in) :: desc
+            type(c_ptr), intent(out) :: ctx
+        end function
+
+        integer(c_int) function swpc3d_destroy(ctx) bind(C, name='swpc3d_destroy')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+
+        !! medium__setup arrays (m_medium.f90:36-467)
+        integer(c_int) function swpc3d_set_medium(ctx, bx, by, bz, lam, mu, taup, taus, muyz, muxz, muxy) &
+            bind(C, name='swpc3d_set_medium')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            real(c_float), intent(in) :: bx(*), by(*), bz(*), lam(*), mu(*), taup(*), taus(*)
+            real(c_float), intent(in) :: muyz(*), muxz(*), muxy(*)
+        end function
+
+        !! kfs_top, kfs_bot, kob_top, kob_bot, kob over (i,j) of the memory box
+        integer(c_int) function swpc3d_set_surface(ctx, kfs_top, kfs_bot, kob_top, kob_bot, kob) &
+            bind(C, name='swpc3d_set_surface')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), intent(in) :: kfs_top(*), kfs_bot(*), kob_top(*), kob_bot(*), kob(*)
+        end function
+
+        !! damping profiles g(4, 1:n) at centres and edges (m_absorb_p.f90:75-93)
+        integer(c_int) function swpc3d_set_pml(ctx, gxc, gxe, gyc, gye, gzc, gze) bind(C, name='swpc3d_set_pml')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            real(c_float), intent(in) :: gxc(*), gxe(*), gyc(*), gye(*), gzc(*), gze(*)
+        end function
+
+        !! m6(6, nsrc); sdrop(nsrc, nt) in FP64 (m_source.f90:775-833)
+        integer(c_int) function swpc3d_set_sources(ctx, nsrc, isrc, jsrc, ksrc, m6, nt, sdrop) &
+            bind(C, name='swpc3d_set_sources')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nsrc, nt
+            integer(c_int), intent(in) :: isrc(*), jsrc(*), ksrc(*)
+            real(c_double), intent(in) :: m6(*), sdrop(*)
+        end function
+
+        integer(c_int) function swpc3d_set_bodyforce(ctx, nsrc, isrc, jsrc, ksrc, f3, nt, fdt) &
+            bind(C, name='swpc3d_set_bodyforce')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nsrc, nt
+            integer(c_int), intent(in) :: isrc(*), jsrc(*), ksrc(*)
+            real(c_double), intent(in) :: f3(*), fdt(*)
+        end function
+
+        integer(c_int) function swpc3d_set_receivers(ctx, nst, ist, jst, kst, ntdec_w, ntw) &
+            bind(C, name='swpc3d_set_receivers')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nst, ntdec_w, ntw
+            integer(c_int), intent(in) :: ist(*), jst(*), kst(*)
+        end function
+
+        integer(c_int) function swpc3d_update_stress(ctx) bind(C, name='swpc3d_update_stress')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+
+        integer(c_int) function swpc3d_update_vel(ctx) bind(C, name='swpc3d_update_vel')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+
+        integer(c_int) function swpc3d_stressglut(ctx, it) bind(C, name='swpc3d_stressglut')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it
+        end function
+
+        integer(c_int) function swpc3d_bodyforce(ctx, it) bind(C, name='swpc3d_bodyforce')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it
+        end function
+
+        integer(c_int) function swpc3d_store_receivers(ctx, it) bind(C, name='swpc3d_store_receivers')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it
+        end function
+
+        !! ijk(3, npts) 1-based cells of a snapshot section (snap__write, m_snap.f90)
+        integer(c_int) function swpc3d_add_snapshot(ctx, kind, npts, ijk, ntdec_s, nbuf, uc, m0, dt, id) &
+            bind(C, name='swpc3d_add_snapshot')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: kind, npts, ntdec_s, nbuf
+            integer(c_int), intent(in) :: ijk(*)
+            real(c_float), value :: uc, m0, dt
+            integer(c_int), intent(out) :: id
+        end function
+
+        integer(c_int) function swpc3d_snapshots(ctx, it) bind(C, name='swpc3d_snapshots')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it
+        end function
+
+        !! out(npts, nvar) of frame (0-based)
+        integer(c_int) function swpc3d_get_snapshot(ctx, id, frame, out) bind(C, name='swpc3d_get_snapshot')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: id, frame
+            real(c_float), intent(out) :: out(*)
+        end function
+
+        integer(c_int) function swpc3d_step(ctx, it) bind(C, name='swpc3d_step')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it
+        end function
+
+        integer(c_int) function swpc3d_run(ctx, it0, nsteps) bind(C, name='swpc3d_run')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it0, nsteps
+        end function
+
+        !! vel(3, nst, ntw) in solver units (z positive up), disp(3, nst, ntw) FP64
+        integer(c_int) function swpc3d_get_receivers(ctx, vel, disp) bind(C, name='swpc3d_get_receivers')
+            import :: c_int, c_ptr, c_float, c_double
+            type(c_ptr), value :: ctx
+            real(c_float), intent(out) :: vel(*)
+            real(c_double), intent(out) :: disp(*)
+        end function
+
+        integer(c_int) function swpc3d_vmax(ctx, out3) bind(C, name='swpc3d_vmax')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            real(c_float), intent(out) :: out3(3)
+        end function
+
+        integer(c_int) function swpc3d_sync(ctx) bind(C, name='swpc3d_sync')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+
+        ! ---------------------------------------------------------------- P-SV
+
+        integer(c_int) function swpc2d_create(desc, ctx) bind(C, name='swpc2d_create')
+            import :: c_int, c_ptr, swpc2d_desc
+            type(swpc2d_desc), intent(in) :: desc
+            type(c_ptr), intent(out) :: ctx
+        end function
+
+        integer(c_int) function swpc2d_destroy(ctx) bind(C, name='swpc2d_destroy')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+
+        integer(c_int) function swpc2d_set_medium(ctx, rho, lam, mu, taup, taus) bind(C, name='swpc2d_set_medium')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            real(c_float), intent(in) :: rho(*), lam(*), mu(*), taup(*), taus(*)
+        end function
+
+        integer(c_int) function swpc2d_set_surface(ctx, kfs_top, kfs_bot, kob_top, kob_bot, kob) &
+            bind(C, name='swpc2d_set_surface')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), intent(in) :: kfs_top(*), kfs_bot(*), kob_top(*), kob_bot(*), kob(*)
+        end function
+
+        integer(c_int) function swpc2d_set_pml(ctx, gxc, gxe, gzc, gze) bind(C, name='swpc2d_set_pml')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            real(c_float), intent(in) :: gxc(*), gxe(*), gzc(*), gze(*)
+        end function
+
+        integer(c_int) function swpc2d_set_sources(ctx, nsrc, isrc, ksrc, m3, nt, sdrop) &
+            bind(C, name='swpc2d_set_sources')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nsrc, nt
+            integer(c_int), intent(in) :: isrc(*), ksrc(*)
+            real(c_double), intent(in) :: m3(*), sdrop(*)
+        end function
+
+        integer(c_int) function swpc2d_set_receivers(ctx, nst, ist, kst, ntdec_w, ntw) &
+            bind(C, name='swpc2d_set_receivers')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nst, ntdec_w, ntw
+            integer(c_int), intent(in) :: ist(*), kst(*)
+        end function
+
+        integer(c_int) function swpc2d_update_stress(ctx) bind(C, name='swpc2d_update_stress')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+
+        integer(c_int) function swpc2d_update_vel(ctx) bind(C, name='swpc2d_update_vel')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+
+        integer(c_int) function swpc2d_stressglut(ctx, it) bind(C, name='swpc2d_stressglut')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it
+        end function
+
+        integer(c_int) function swpc2d_store_receivers(ctx, it) bind(C, name='swpc2d_store_receivers')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it
+        end function
+
+        integer(c_int) function swpc2d_step(ctx, it) bind(C, name='swpc2d_step')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it
+        end function
+
+        integer(c_int) function swpc2d_run(ctx, it0, nsteps) bind(C, name='swpc2d_run')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it0, nsteps
+        end function
+
+        integer(c_int) function swpc2d_get_receivers(ctx, vel, disp) bind(C, name='swpc2d_get_receivers')
+            import :: c_int, c_ptr, c_float, c_double
+            type(c_ptr), value :: ctx
+            real(c_float), intent(out) :: vel(*)
+            real(c_double), intent(out) :: disp(*)
+        end function
+
+        integer(c_int) function swpc2d_vmax(ctx, out2) bind(C, name='swpc2d_vmax')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            real(c_float), intent(out) :: out2(2)
+        end function
+
+        integer(c_int) function swpc2d_sync(ctx) bind(C, name='swpc2d_sync')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+
+    end interface
+
+contains
+
+    !! Text of the calling thread's last error.
+    function swpc_errmsg() result(msg)
+        character(len=:), allocatable :: msg
+        character(kind=c_char), pointer :: s(:)
+        type(c_ptr) :: p
+        integer :: n
+        interface
+            function c_strlen(str) result(n) bind(C, name='strlen')
+                import :: c_ptr, c_size_t
+                type(c_ptr), value :: str
+                integer(c_size_t) :: n
+            end function
+        end interface
+        p = swpc_last_error()
+        n = int(c_strlen(p))
+        call c_f_pointer(p, s, [n])
+        allocate (character(len=n) :: msg)
+        msg = transfer(s(1:n), msg)
+    end function swpc_errmsg
+
+end module swpc_hip

@@ -1,7 +1,9 @@
 """Build recipe for the HIP library (gfx950) and the CPU oracle.
 
 ``build_library()`` compiles every ``csrc/*.hip`` with hipcc for gfx950 and
-links ``openswpc_amd/libswpc_hip.so`` in-tree.  ``build_oracle()`` runs the
+links ``openswpc_amd/libswpc_hip.so`` in-tree.  ``build_fortran_host()``
+compiles the Fortran bindings (``include/swpc_hip.f90``) and the Fortran
+time-loop host (``fhost/``) with amdflang into ``fhost/swpc3d_fdrive``.  ``build_oracle()`` runs the
 Makefile under ``oracle/`` (the C restatement used by the tests, and, when
 the reference sources are present, the amdflang build of the reference's
 shared Fortran modules into ``oracle/_ref``).
@@ -55,6 +57,30 @@ def build_library(verbose: bool = False) -> Path:
     return LIB
 
 
+FLANG = os.environ.get("SWPC_FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FDRIVE = PKG / "fhost" / "swpc3d_fdrive"
+
+
+def build_fortran_host(verbose: bool = False):
+    """Fortran bindings + drop-in host; linked against libswpc_hip.so with
+    an rpath to the package directory.  Returns the executable, or None
+    where no Fortran compiler is installed."""
+    if not Path(FLANG).exists():
+        return None
+    mod = BUILD_DIR / "fmod"
+    mod.mkdir(parents=True, exist_ok=True)
+    srcs = [ROOT / "include" / "swpc_hip.f90", PKG / "fhost" / "m_kernel_hip.f90", PKG / "fhost" / "swpc3d_fdrive.f90"]
+    objs = []
+    for src in srcs:
+        obj = mod / (src.stem + ".o")
+        _run([FLANG, "-O2", "-fPIC", "-module-dir", str(mod), "-I", str(mod), "-c", str(src), "-o", str(obj)])
+        objs.append(obj)
+    tmp = FDRIVE.with_name(FDRIVE.name + ".tmp")
+    _run([FLANG, "-o", str(tmp), *map(str, objs), f"-L{PKG}", "-lswpc_hip", "-Wl,-rpath,$ORIGIN/.."])
+    os.replace(tmp, FDRIVE)
+    return FDRIVE
+
+
 def build_oracle(verbose: bool = False) -> None:
     """Build oracle/liboracle32.so, liboracle64.so and (if possible) oracle/_ref."""
     out = _run(["make", "-C", str(ROOT / "oracle"), "-j4"])
@@ -64,6 +90,7 @@ def build_oracle(verbose: bool = False) -> None:
 
 def main(argv=None):
     build_library(verbose=True)
+    build_fortran_host(verbose=True)
     build_oracle(verbose=True)
     print("built", LIB)
 
