@@ -88,8 +88,8 @@ def main():
     wall, ms, cnt = plain["wall"], plain["ms"], plain["cnt"]
     ms_step = wall / args.steps * 1e3
     cups = ncell * args.steps / wall
-    # dominant kernel: the interior stress update (class 0: the normal- and
-    # shear-stress launches of one step)
+    # dominant kernel: the interior stress update (class 0: one launch of
+    # the one-pass normal + shear kernel per step)
     avg_stress = ms[0] / max(cnt[0], 1) * 1e-3
     alg_stress = B_INT_STRESS * n_int
     achieved = alg_stress / avg_stress
@@ -135,7 +135,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "interior stress update (k3x_normal + k3x_shear launches)",
+            "kernel": "interior stress update (k3x_stress, one launch per step)",
             "achieved": achieved / 1e9,
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",

@@ -75,7 +75,7 @@ int swpc_version(void);
 /* Number of HIP devices visible (0 when no GPU); never fails. */
 int swpc_device_count(void);
 /* Measured device copy bandwidth: `iters` copies of `bytes` bytes with a
-   float4 grid-stride kernel; *gbs = (read + write bytes) / time in GB/s.
+   16-byte non-temporal copy kernel; *gbs = (read + write bytes) / time in GB/s.
    A diagnostic with no counterpart in the reference: it is the measured
    ceiling the stencil kernels are reported against. */
 int swpc_membw(long long bytes, int iters, double *gbs);
@@ -193,15 +193,25 @@ int swpc3d_profile(swpc3d_ctx *ctx, int enable);
 int swpc3d_profile_read(swpc3d_ctx *ctx, double *ms5, long long *count5);
 /* Device bytes held by the context. */
 int swpc3d_memory(swpc3d_ctx *ctx, long long *bytes);
-/* Launch configuration of the interior kernels (no counterpart in the
-   reference; results do not depend on it).  p[0..n-1] overrides, in
-   order: stress tile rows (2/4/8), stress waves-per-SIMD hint (0/3/4),
-   split normal/shear stress into two kernels (0/1), velocity tile rows,
-   velocity waves hint (0/5), XCD-contiguous workgroup order (0/1),
-   stress z-chunk, velocity z-chunk (0 = automatic), 4-cell-per-thread
-   kernels for FP32 and reduced storage (0/1, default 1), their tile
-   rows (1/2/4), and 4-cell-per-thread PML kernels (0/1, default 0).  The first seven apply to the one-cell-per-thread
-   kernels, which FP64 always uses. */
+/* Launch configuration of the kernels (no counterpart in the reference;
+   results do not depend on it beyond FP32 rounding order).  p[0..n-1]
+   overrides, in order:
+     [0..7]  one-cell-per-thread kernels (FP64 always uses them): stress
+             tile rows (2/4/8), stress waves-per-SIMD hint (0/3/4), split
+             normal/shear stress (0/1), velocity tile rows, velocity waves
+             hint (0/5), XCD-contiguous workgroup order (0/1), stress
+             z-chunk, velocity z-chunk (0 = automatic);
+     [8]     4-cell-per-thread kernels for FP32 and reduced storage (0/1,
+             default 1); [9] their tile rows (1/2/4);
+     [10]    4-cell-per-thread PML kernels (0/1, default 0);
+     [11]    4-wide stress form: 0 normal + shear launches, 1 one pass with
+             staged loads, 2 one pass (default 1 for FP32, 0 for 16-bit
+             storage); [12] its waves hint (0/3/4);
+     [13]    PML kernels on a second stream, concurrent with the interior
+             (0/1, default 0);
+     [14]    4-wide velocity z-marching chunk (0 = one plane per workgroup;
+             default 8 for FP32, 16 for 16-bit storage);
+     [15]    4-wide one-pass z-marching stress chunk (0 = off, default). */
 int swpc3d_set_tuning(swpc3d_ctx *ctx, const int *p, int n);
 
 /* ------------------------------------------------------------------ */

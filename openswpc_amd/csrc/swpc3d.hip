@@ -70,6 +70,7 @@ struct K3 {
     int ntx, nty, nkc;          // interior tiles along x, y and z chunks
     unsigned nblk, nblk_pad;    // ntx*nty*nkc and its round-up to a multiple of 8
     int xcd;                    // remap workgroup ids XCD-contiguously
+    int ade_sparse;             // ADE arrays hold zeros wherever their axis is undamped
     Boxes3 pml;
 };
 
@@ -623,6 +624,288 @@ __global__ void __launch_bounds__(64 * BY) k3x_shear(const K3<float> a)
     st4m(a.S[5] + c, sxy_, msk);
 }
 
+// Normal and shear stresses in one pass (m_kernel.f90:142-326): the
+// velocity differences of both parts are formed first from one set of V
+// loads, then the normal and the shear memory variables are streamed in
+// turn.  BAR = 1 keeps the compiler from hoisting the shear loads above the
+// normal stores, which bounds the registers held at once.
+template <int BY, bool S16, int BAR, int WPE>
+__global__ void __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8)))
+k3x_stress(const K3<float> a)
+{
+    int i0, j, k;
+    unsigned msk;
+    if (!x4_cell(a, BY, i0, j, k, msk)) return;
+    const int px = a.px, sxy = a.sxy;
+    const int c = at3(px, sxy, i0, j, k);
+    const float *__restrict__ Vx = a.V[0];
+    const float *__restrict__ Vy = a.V[1];
+    const float *__restrict__ Vz = a.V[2];
+    f4v dxVx, dyVy, dzVz, dxVy_dyVx, dxVz_dzVx, dyVz_dzVy;
+    {
+        const f4v vx = ld4(Vx + c);
+        const float2 vxl = ld2(Vx + c - 2);
+        const float vxr = Vx[c + 4];
+        const f4v vxyp1 = ld4(Vx + c + px), vxyp2 = ld4(Vx + c + 2 * px), vxym1 = ld4(Vx + c - px);
+        const f4v vxzp1 = ld4(Vx + c + sxy), vxzp2 = ld4(Vx + c + 2 * sxy), vxzm1 = ld4(Vx + c - sxy);
+        const f4v vy = ld4(Vy + c), vym1 = ld4(Vy + c - px), vym2 = ld4(Vy + c - 2 * px), vyp1 = ld4(Vy + c + px);
+        const f4v vyzp1 = ld4(Vy + c + sxy), vyzp2 = ld4(Vy + c + 2 * sxy), vyzm1 = ld4(Vy + c - sxy);
+        const float2 vyl = ld2(Vy + c - 2), vyr = ld2(Vy + c + 4);
+        const f4v vz = ld4(Vz + c), vzm1 = ld4(Vz + c - sxy), vzm2 = ld4(Vz + c - 2 * sxy), vzp1 = ld4(Vz + c + sxy);
+        const f4v vzyp1 = ld4(Vz + c + px), vzyp2 = ld4(Vz + c + 2 * px), vzym1 = ld4(Vz + c - px);
+        const float2 vzl = ld2(Vz + c - 2), vzr = ld2(Vz + c + 4);
+        // x rows: Vx i0-2 .. i0+4, Vy and Vz i0-1 .. i0+5
+        const float xs[7] = {vxl.x, vxl.y, vx.x, vx.y, vx.z, vx.w, vxr};
+        const float ys[7] = {vyl.y, vy.x, vy.y, vy.z, vy.w, vyr.x, vyr.y};
+        const float zs[7] = {vzl.y, vz.x, vz.y, vz.z, vz.w, vzr.x, vzr.y};
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const Ord o = order4(a, i0 + q, j, k);
+            dxVx[q] = (xs[q + 2] - xs[q + 1]) * o.e0x - (xs[q + 3] - xs[q]) * o.e1x;
+            dyVy[q] = (vy[q] - vym1[q]) * o.e0y - (vyp1[q] - vym2[q]) * o.e1y;
+            dzVz[q] = (vz[q] - vzm1[q]) * o.e0z - (vzp1[q] - vzm2[q]) * o.e1z;
+            dxVy_dyVx[q] = (ys[q + 2] - ys[q + 1]) * o.e0x - (ys[q + 3] - ys[q]) * o.e1x
+                         + (vxyp1[q] - vx[q]) * o.e0y - (vxyp2[q] - vxym1[q]) * o.e1y;
+            dxVz_dzVx[q] = (zs[q + 2] - zs[q + 1]) * o.e0x - (zs[q + 3] - zs[q]) * o.e1x
+                         + (vxzp1[q] - vx[q]) * o.e0z - (vxzp2[q] - vxzm1[q]) * o.e1z;
+            dyVz_dzVy[q] = (vzyp1[q] - vz[q]) * o.e0y - (vzyp2[q] - vzym1[q]) * o.e1y
+                         + (vyzp1[q] - vy[q]) * o.e0z - (vyzp2[q] - vyzm1[q]) * o.e1z;
+        }
+    }
+    const float dt = a.dt;
+    const f4v taus = med4<S16>(a, MTAUS, c, k);
+    {
+        const f4v lam = med4<S16>(a, MLAM, c, k), mu = med4<S16>(a, MMU, c, k), taup = med4<S16>(a, MTAUP, c, k);
+        f4v R[9];
+#pragma unroll
+        for (int q = 0; q < 9; q++) R[q] = ldR4<S16>(a, q, c);
+        f4v sxx = ld4nt(a.S[0] + c), syy = ld4nt(a.S[1] + c), szz = ld4nt(a.S[2] + c);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float mu2 = 2 * mu[q];
+            const float lam2mu = lam[q] + mu2;
+            const float taup1 = taup[q], taus1 = taus[q];
+            const float d3v3 = dxVx[q] + dyVy[q] + dzVz[q];
+            const float dyVy_dzVz = dyVy[q] + dzVz[q], dxVx_dzVz = dxVx[q] + dzVz[q], dxVx_dyVy = dxVx[q] + dyVy[q];
+            const float fxx = lam2mu * taup1 * d3v3 - mu2 * taus1 * dyVy_dzVz;
+            const float fyy = lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dzVz;
+            const float fzz = lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dyVy;
+            float Rxx_n = 0, Ryy_n = 0, Rzz_n = 0;
+#pragma unroll
+            for (int mm = 0; mm < NM3; mm++) {
+                const float rxx = a.c1[mm] * R[0 + mm][q] - a.c2[mm] * fxx * dt;
+                const float ryy = a.c1[mm] * R[3 + mm][q] - a.c2[mm] * fyy * dt;
+                const float rzz = a.c1[mm] * R[6 + mm][q] - a.c2[mm] * fzz * dt;
+                R[0 + mm][q] = rxx;
+                R[3 + mm][q] = ryy;
+                R[6 + mm][q] = rzz;
+                Rxx_n += a.d1[mm] * rxx;
+                Ryy_n += a.d1[mm] * ryy;
+                Rzz_n += a.d1[mm] * rzz;
+            }
+            const float taup_plus1 = 1 + taup1 * (1 + a.d2);
+            const float taus_plus1 = 1 + taus1 * (1 + a.d2);
+            sxx[q] = sxx[q] + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dyVy_dzVz + Rxx_n) * dt;
+            syy[q] = syy[q] + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dzVz + Ryy_n) * dt;
+            szz[q] = szz[q] + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dyVy + Rzz_n) * dt;
+        }
+#pragma unroll
+        for (int q = 0; q < 9; q++) stR4<S16>(a, q, c, R[q], msk);
+        st4m(a.S[0] + c, sxx, msk);
+        st4m(a.S[1] + c, syy, msk);
+        st4m(a.S[2] + c, szz, msk);
+    }
+    if constexpr (BAR != 0) __builtin_amdgcn_sched_barrier(0);
+    {
+        const f4v myz = med4<S16>(a, MMUYZ, c, k), mxz = med4<S16>(a, MMUXZ, c, k), mxy = med4<S16>(a, MMUXY, c, k);
+        f4v R[9];
+#pragma unroll
+        for (int q = 0; q < 9; q++) R[q] = ldR4<S16>(a, 9 + q, c);
+        f4v syz = ld4nt(a.S[3] + c), sxz = ld4nt(a.S[4] + c), sxy_ = ld4nt(a.S[5] + c);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float taus1 = taus[q];
+            const float fyz = myz[q] * taus1 * dyVz_dzVy[q];
+            const float fxz = mxz[q] * taus1 * dxVz_dzVx[q];
+            const float fxy = mxy[q] * taus1 * dxVy_dyVx[q];
+            float Ryz_n = 0, Rxz_n = 0, Rxy_n = 0;
+#pragma unroll
+            for (int mm = 0; mm < NM3; mm++) {
+                const float ryz = a.c1[mm] * R[0 + mm][q] - a.c2[mm] * fyz * dt;
+                const float rxz = a.c1[mm] * R[3 + mm][q] - a.c2[mm] * fxz * dt;
+                const float rxy = a.c1[mm] * R[6 + mm][q] - a.c2[mm] * fxy * dt;
+                R[0 + mm][q] = ryz;
+                R[3 + mm][q] = rxz;
+                R[6 + mm][q] = rxy;
+                Ryz_n += a.d1[mm] * ryz;
+                Rxz_n += a.d1[mm] * rxz;
+                Rxy_n += a.d1[mm] * rxy;
+            }
+            const float taus_plus1 = 1 + taus1 * (1 + a.d2);
+            syz[q] = syz[q] + (myz[q] * taus_plus1 * dyVz_dzVy[q] + Ryz_n) * dt;
+            sxz[q] = sxz[q] + (mxz[q] * taus_plus1 * dxVz_dzVx[q] + Rxz_n) * dt;
+            sxy_[q] = sxy_[q] + (mxy[q] * taus_plus1 * dxVy_dyVx[q] + Rxy_n) * dt;
+        }
+#pragma unroll
+        for (int q = 0; q < 9; q++) stR4<S16>(a, 9 + q, c, R[q], msk);
+        st4m(a.S[3] + c, syz, msk);
+        st4m(a.S[4] + c, sxz, msk);
+        st4m(a.S[5] + c, sxy_, msk);
+    }
+}
+
+// One-pass stress update, z-marching: each thread owns 4 x-cells of one
+// row and walks a chunk of a.kc planes.  The z-stencils of Vz (planes
+// k-2..k+1), Vx and Vy (k-1..k+2) come from register windows, one new
+// plane of each per step; the memory variables, stresses and medium stream
+// through once.  Workgroups: 256-cell x tiles x BY rows x z-chunks.
+template <int BY, bool S16>
+__global__ void __launch_bounds__(64 * BY) k3x_stressz(const K3<float> a)
+{
+    const unsigned b = blockIdx.x;
+    const unsigned xt = b % (unsigned)a.ntx;
+    const unsigned t = b / (unsigned)a.ntx;
+    const unsigned yt = t % (unsigned)a.nty;
+    const unsigned zc = t / (unsigned)a.nty;
+    const int i0 = a.ti0 + (int)xt * 256 + (int)threadIdx.x * 4;
+    const int j = a.na + 1 + (int)yt * BY + (int)threadIdx.y;
+    if (j > a.ny - a.na) return;
+    unsigned msk = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        if (i0 + q > a.na && i0 + q <= a.nx - a.na) msk |= 1u << q;
+    if (msk == 0) return;
+    const int kb = 1 + (int)zc * a.kc;
+    const int ke = min(kb + a.kc - 1, a.nz - a.na);
+    if (kb > ke) return;
+    const int px = a.px, sxy = a.sxy;
+    const float *__restrict__ Vx = a.V[0];
+    const float *__restrict__ Vy = a.V[1];
+    const float *__restrict__ Vz = a.V[2];
+    int4 fs[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) fs[q] = a.surf[(j - 1) * a.nx + (min(max(i0 + q, 1), a.nx) - 1)];
+    int c = at3(px, sxy, i0, j, kb);
+    f4v vz_m2 = ld4(Vz + c - 2 * sxy), vz_m1 = ld4(Vz + c - sxy), vz_0 = ld4(Vz + c);
+    f4v vx_m1 = ld4(Vx + c - sxy), vx_0 = ld4(Vx + c), vx_p1 = ld4(Vx + c + sxy);
+    f4v vy_m1 = ld4(Vy + c - sxy), vy_0 = ld4(Vy + c), vy_p1 = ld4(Vy + c + sxy);
+    const float dt = a.dt;
+    for (int k = kb; k <= ke; k++, c += sxy) {
+        const f4v vz_p1 = ld4(Vz + c + sxy), vx_p2 = ld4(Vx + c + 2 * sxy), vy_p2 = ld4(Vy + c + 2 * sxy);
+        f4v dxVx, dyVy, dzVz, dxVy_dyVx, dxVz_dzVx, dyVz_dzVy;
+        {
+            const float2 vxl = ld2(Vx + c - 2);
+            const float vxr = Vx[c + 4];
+            const f4v vxyp1 = ld4(Vx + c + px), vxyp2 = ld4(Vx + c + 2 * px), vxym1 = ld4(Vx + c - px);
+            const f4v vym1 = ld4(Vy + c - px), vym2 = ld4(Vy + c - 2 * px), vyp1 = ld4(Vy + c + px);
+            const float2 vyl = ld2(Vy + c - 2), vyr = ld2(Vy + c + 4);
+            const f4v vzyp1 = ld4(Vz + c + px), vzyp2 = ld4(Vz + c + 2 * px), vzym1 = ld4(Vz + c - px);
+            const float2 vzl = ld2(Vz + c - 2), vzr = ld2(Vz + c + 4);
+            const float xs[7] = {vxl.x, vxl.y, vx_0.x, vx_0.y, vx_0.z, vx_0.w, vxr};
+            const float ys[7] = {vyl.y, vy_0.x, vy_0.y, vy_0.z, vy_0.w, vyr.x, vyr.y};
+            const float zs[7] = {vzl.y, vz_0.x, vz_0.y, vz_0.z, vz_0.w, vzr.x, vzr.y};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int o1 = (k - fs[q].x) * (fs[q].y - k), o2 = (k - fs[q].z) * (fs[q].w - k);
+                const bool lo = max(o1, o2) >= 0;
+                const float e0x = lo ? a.e2[0][0] : a.e4[0][0], e1x = lo ? a.e2[0][1] : a.e4[0][1];
+                const float e0y = lo ? a.e2[1][0] : a.e4[1][0], e1y = lo ? a.e2[1][1] : a.e4[1][1];
+                const float e0z = lo ? a.e2[2][0] : a.e4[2][0], e1z = lo ? a.e2[2][1] : a.e4[2][1];
+                dxVx[q] = (xs[q + 2] - xs[q + 1]) * e0x - (xs[q + 3] - xs[q]) * e1x;
+                dyVy[q] = (vy_0[q] - vym1[q]) * e0y - (vyp1[q] - vym2[q]) * e1y;
+                dzVz[q] = (vz_0[q] - vz_m1[q]) * e0z - (vz_p1[q] - vz_m2[q]) * e1z;
+                dxVy_dyVx[q] = (ys[q + 2] - ys[q + 1]) * e0x - (ys[q + 3] - ys[q]) * e1x
+                             + (vxyp1[q] - vx_0[q]) * e0y - (vxyp2[q] - vxym1[q]) * e1y;
+                dxVz_dzVx[q] = (zs[q + 2] - zs[q + 1]) * e0x - (zs[q + 3] - zs[q]) * e1x
+                             + (vx_p1[q] - vx_0[q]) * e0z - (vx_p2[q] - vx_m1[q]) * e1z;
+                dyVz_dzVy[q] = (vzyp1[q] - vz_0[q]) * e0y - (vzyp2[q] - vzym1[q]) * e1y
+                             + (vy_p1[q] - vy_0[q]) * e0z - (vy_p2[q] - vy_m1[q]) * e1z;
+            }
+        }
+        const f4v taus = med4<S16>(a, MTAUS, c, k);
+        {
+            const f4v lam = med4<S16>(a, MLAM, c, k), mu = med4<S16>(a, MMU, c, k), taup = med4<S16>(a, MTAUP, c, k);
+            f4v R[9];
+#pragma unroll
+            for (int q = 0; q < 9; q++) R[q] = ldR4<S16>(a, q, c);
+            f4v sxx = ld4nt(a.S[0] + c), syy = ld4nt(a.S[1] + c), szz = ld4nt(a.S[2] + c);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const float mu2 = 2 * mu[q];
+                const float lam2mu = lam[q] + mu2;
+                const float taup1 = taup[q], taus1 = taus[q];
+                const float d3v3 = dxVx[q] + dyVy[q] + dzVz[q];
+                const float dyVy_dzVz = dyVy[q] + dzVz[q], dxVx_dzVz = dxVx[q] + dzVz[q], dxVx_dyVy = dxVx[q] + dyVy[q];
+                const float fxx = lam2mu * taup1 * d3v3 - mu2 * taus1 * dyVy_dzVz;
+                const float fyy = lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dzVz;
+                const float fzz = lam2mu * taup1 * d3v3 - mu2 * taus1 * dxVx_dyVy;
+                float Rxx_n = 0, Ryy_n = 0, Rzz_n = 0;
+#pragma unroll
+                for (int mm = 0; mm < NM3; mm++) {
+                    const float rxx = a.c1[mm] * R[0 + mm][q] - a.c2[mm] * fxx * dt;
+                    const float ryy = a.c1[mm] * R[3 + mm][q] - a.c2[mm] * fyy * dt;
+                    const float rzz = a.c1[mm] * R[6 + mm][q] - a.c2[mm] * fzz * dt;
+                    R[0 + mm][q] = rxx;
+                    R[3 + mm][q] = ryy;
+                    R[6 + mm][q] = rzz;
+                    Rxx_n += a.d1[mm] * rxx;
+                    Ryy_n += a.d1[mm] * ryy;
+                    Rzz_n += a.d1[mm] * rzz;
+                }
+                const float taup_plus1 = 1 + taup1 * (1 + a.d2);
+                const float taus_plus1 = 1 + taus1 * (1 + a.d2);
+                sxx[q] = sxx[q] + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dyVy_dzVz + Rxx_n) * dt;
+                syy[q] = syy[q] + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dzVz + Ryy_n) * dt;
+                szz[q] = szz[q] + (lam2mu * taup_plus1 * d3v3 - mu2 * taus_plus1 * dxVx_dyVy + Rzz_n) * dt;
+            }
+#pragma unroll
+            for (int q = 0; q < 9; q++) stR4<S16>(a, q, c, R[q], msk);
+            st4m(a.S[0] + c, sxx, msk);
+            st4m(a.S[1] + c, syy, msk);
+            st4m(a.S[2] + c, szz, msk);
+        }
+        {
+            const f4v myz = med4<S16>(a, MMUYZ, c, k), mxz = med4<S16>(a, MMUXZ, c, k), mxy = med4<S16>(a, MMUXY, c, k);
+            f4v R[9];
+#pragma unroll
+            for (int q = 0; q < 9; q++) R[q] = ldR4<S16>(a, 9 + q, c);
+            f4v syz = ld4nt(a.S[3] + c), sxz = ld4nt(a.S[4] + c), sxy_ = ld4nt(a.S[5] + c);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const float taus1 = taus[q];
+                const float fyz = myz[q] * taus1 * dyVz_dzVy[q];
+                const float fxz = mxz[q] * taus1 * dxVz_dzVx[q];
+                const float fxy = mxy[q] * taus1 * dxVy_dyVx[q];
+                float Ryz_n = 0, Rxz_n = 0, Rxy_n = 0;
+#pragma unroll
+                for (int mm = 0; mm < NM3; mm++) {
+                    const float ryz = a.c1[mm] * R[0 + mm][q] - a.c2[mm] * fyz * dt;
+                    const float rxz = a.c1[mm] * R[3 + mm][q] - a.c2[mm] * fxz * dt;
+                    const float rxy = a.c1[mm] * R[6 + mm][q] - a.c2[mm] * fxy * dt;
+                    R[0 + mm][q] = ryz;
+                    R[3 + mm][q] = rxz;
+                    R[6 + mm][q] = rxy;
+                    Ryz_n += a.d1[mm] * ryz;
+                    Rxz_n += a.d1[mm] * rxz;
+                    Rxy_n += a.d1[mm] * rxy;
+                }
+                const float taus_plus1 = 1 + taus1 * (1 + a.d2);
+                syz[q] = syz[q] + (myz[q] * taus_plus1 * dyVz_dzVy[q] + Ryz_n) * dt;
+                sxz[q] = sxz[q] + (mxz[q] * taus_plus1 * dxVz_dzVx[q] + Rxz_n) * dt;
+                sxy_[q] = sxy_[q] + (mxy[q] * taus_plus1 * dxVy_dyVx[q] + Rxy_n) * dt;
+            }
+#pragma unroll
+            for (int q = 0; q < 9; q++) stR4<S16>(a, 9 + q, c, R[q], msk);
+            st4m(a.S[3] + c, syz, msk);
+            st4m(a.S[4] + c, sxz, msk);
+            st4m(a.S[5] + c, sxy_, msk);
+        }
+        vz_m2 = vz_m1; vz_m1 = vz_0; vz_0 = vz_p1;
+        vx_m1 = vx_0; vx_0 = vx_p1; vx_p1 = vx_p2;
+        vy_m1 = vy_0; vy_0 = vy_p1; vy_p1 = vy_p2;
+    }
+}
+
 // velocity (m_kernel.f90:99-129)
 template <int BY, bool S16>
 __global__ void __launch_bounds__(64 * BY) k3x_vel(const K3<float> a)
@@ -677,6 +960,90 @@ __global__ void __launch_bounds__(64 * BY) k3x_vel(const K3<float> a)
     st4m(a.V[2] + c, vz, msk);
 }
 
+// Velocity, z-marching: each thread owns 4 x-cells of one row and walks a
+// chunk of a.kc planes.  The z-stencils of Szz, Sxz and Syz come from a
+// register window (one new plane per step), so each of their planes is
+// read once per chunk instead of four times; y- and x-neighbours are read
+// as in k3x_vel.  Workgroups: 256-cell x tiles x BY rows x z-chunks.
+template <int BY, bool S16>
+__global__ void __launch_bounds__(64 * BY) k3x_velz(const K3<float> a)
+{
+    const unsigned b = blockIdx.x;
+    const unsigned xt = b % (unsigned)a.ntx;
+    const unsigned t = b / (unsigned)a.ntx;
+    const unsigned yt = t % (unsigned)a.nty;
+    const unsigned zc = t / (unsigned)a.nty;
+    const int i0 = a.ti0 + (int)xt * 256 + (int)threadIdx.x * 4;
+    const int j = a.na + 1 + (int)yt * BY + (int)threadIdx.y;
+    if (j > a.ny - a.na) return;
+    unsigned msk = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        if (i0 + q > a.na && i0 + q <= a.nx - a.na) msk |= 1u << q;
+    if (msk == 0) return;
+    const int kb = 1 + (int)zc * a.kc;
+    const int ke = min(kb + a.kc - 1, a.nz - a.na);
+    if (kb > ke) return;
+    const int px = a.px, sxy = a.sxy;
+    const float *__restrict__ Sxx = a.S[0];
+    const float *__restrict__ Syy = a.S[1];
+    const float *__restrict__ Szz = a.S[2];
+    const float *__restrict__ Syz = a.S[3];
+    const float *__restrict__ Sxz = a.S[4];
+    const float *__restrict__ Sxy = a.S[5];
+    int4 fs[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) fs[q] = a.surf[(j - 1) * a.nx + (min(max(i0 + q, 1), a.nx) - 1)];
+    int c = at3(px, sxy, i0, j, kb);
+    f4v zz_m1 = ld4(Szz + c - sxy), zz_0 = ld4(Szz + c), zz_p1 = ld4(Szz + c + sxy);
+    f4v xz_m2 = ld4(Sxz + c - 2 * sxy), xz_m1 = ld4(Sxz + c - sxy), xz_0 = ld4(Sxz + c);
+    f4v yz_m2 = ld4(Syz + c - 2 * sxy), yz_m1 = ld4(Syz + c - sxy), yz_0 = ld4(Syz + c);
+    const float dt = a.dt;
+    for (int k = kb; k <= ke; k++, c += sxy) {
+        const f4v zz_p2 = ld4(Szz + c + 2 * sxy), xz_p1 = ld4(Sxz + c + sxy), yz_p1 = ld4(Syz + c + sxy);
+        const f4v sxx = ld4(Sxx + c);
+        const float2 sxxl = ld2(Sxx + c - 2), sxxr = ld2(Sxx + c + 4);
+        const f4v syy = ld4(Syy + c), syyp1 = ld4(Syy + c + px), syyp2 = ld4(Syy + c + 2 * px), syym1 = ld4(Syy + c - px);
+        const f4v sxyv = ld4(Sxy + c), sxyym1 = ld4(Sxy + c - px), sxyym2 = ld4(Sxy + c - 2 * px), sxyyp1 = ld4(Sxy + c + px);
+        const float2 sxyl = ld2(Sxy + c - 2), sxyr = ld2(Sxy + c + 4);
+        const float2 sxzl = ld2(Sxz + c - 2), sxzr = ld2(Sxz + c + 4);
+        const f4v syzym1 = ld4(Syz + c - px), syzym2 = ld4(Syz + c - 2 * px), syzyp1 = ld4(Syz + c + px);
+        const f4v bx = med4<S16>(a, MBX, c, k), by = med4<S16>(a, MBY, c, k), bz = med4<S16>(a, MBZ, c, k);
+        f4v vx = ld4nt(a.V[0] + c), vy = ld4nt(a.V[1] + c), vz = ld4nt(a.V[2] + c);
+        // x rows i0-2 .. i0+5
+        const float xx[8] = {sxxl.x, sxxl.y, sxx.x, sxx.y, sxx.z, sxx.w, sxxr.x, sxxr.y};
+        const float xy[8] = {sxyl.x, sxyl.y, sxyv.x, sxyv.y, sxyv.z, sxyv.w, sxyr.x, sxyr.y};
+        const float xz[8] = {sxzl.x, sxzl.y, xz_0.x, xz_0.y, xz_0.z, xz_0.w, sxzr.x, sxzr.y};
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int o1 = (k - fs[q].x) * (fs[q].y - k), o2 = (k - fs[q].z) * (fs[q].w - k);
+            const bool lo = max(o1, o2) >= 0;
+            const float e0x = lo ? a.e2[0][0] : a.e4[0][0], e1x = lo ? a.e2[0][1] : a.e4[0][1];
+            const float e0y = lo ? a.e2[1][0] : a.e4[1][0], e1y = lo ? a.e2[1][1] : a.e4[1][1];
+            const float e0z = lo ? a.e2[2][0] : a.e4[2][0], e1z = lo ? a.e2[2][1] : a.e4[2][1];
+            const int x = q + 2;  // index of cell i0+q in the x rows
+            const float d3Sx3 = (xx[x + 1] - xx[x]) * e0x - (xx[x + 2] - xx[x - 1]) * e1x
+                              + (sxyv[q] - sxyym1[q]) * e0y - (sxyyp1[q] - sxyym2[q]) * e1y
+                              + (xz_0[q] - xz_m1[q]) * e0z - (xz_p1[q] - xz_m2[q]) * e1z;
+            const float d3Sy3 = (xy[x] - xy[x - 1]) * e0x - (xy[x + 1] - xy[x - 2]) * e1x
+                              + (syyp1[q] - syy[q]) * e0y - (syyp2[q] - syym1[q]) * e1y
+                              + (yz_0[q] - yz_m1[q]) * e0z - (yz_p1[q] - yz_m2[q]) * e1z;
+            const float d3Sz3 = (xz[x] - xz[x - 1]) * e0x - (xz[x + 1] - xz[x - 2]) * e1x
+                              + (yz_0[q] - syzym1[q]) * e0y - (syzyp1[q] - syzym2[q]) * e1y
+                              + (zz_p1[q] - zz_0[q]) * e0z - (zz_p2[q] - zz_m1[q]) * e1z;
+            vx[q] = vx[q] + bx[q] * d3Sx3 * dt;
+            vy[q] = vy[q] + by[q] * d3Sy3 * dt;
+            vz[q] = vz[q] + bz[q] * d3Sz3 * dt;
+        }
+        st4m(a.V[0] + c, vx, msk);
+        st4m(a.V[1] + c, vy, msk);
+        st4m(a.V[2] + c, vz, msk);
+        zz_m1 = zz_0; zz_0 = zz_p1; zz_p1 = zz_p2;
+        xz_m2 = xz_m1; xz_m1 = xz_0; xz_0 = xz_p1;
+        yz_m2 = yz_m1; yz_m1 = yz_0; yz_0 = yz_p1;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // PML (ADE-CFS, 2nd order, elastic)
 // ---------------------------------------------------------------------------
@@ -718,6 +1085,14 @@ __global__ void __launch_bounds__(256) k3_stress_pml(const K3<T> a)
     const T gze1 = a.gz[4 * nz + k - 1], gze2 = a.gz[5 * nz + k - 1], gze3 = a.gz[6 * nz + k - 1], gze4 = a.gz[7 * nz + k - 1];
     const T r20x = a.r2[0], r20y = a.r2[1], r20z = a.r2[2];
     const T dt = a.dt;
+    // Along an axis where this cell is undamped (d = 0, so g4 = 0 at the
+    // centre and the edge) the ADE update a = g3 a + g4 dv dt keeps a zero
+    // array zero, and its term g2 a adds zero.  While the context knows the
+    // arrays started from zero (ade_sparse), such ADE values are neither
+    // read nor written; the arithmetic is unchanged.
+    const bool lx = !a.ade_sparse || gxc4 != (T)0 || gxe4 != (T)0;
+    const bool ly = !a.ade_sparse || gyc4 != (T)0 || gye4 != (T)0;
+    const bool lz = !a.ade_sparse || gzc4 != (T)0 || gze4 != (T)0;
     const T vx0 = Vx[c], vy0 = Vy[c], vz0 = Vz[c];
     // normal (m_absorb_p.f90:445-462)
     const T dxVx = (vx0 - Vx[c - 1]) * r20x;
@@ -728,16 +1103,17 @@ __global__ void __launch_bounds__(256) k3_stress_pml(const K3<T> a)
     if constexpr (S16) lam2mu_R = medv<S16>(a, MLAM, c, k) + 2 * mu0;
     else lam2mu_R = (T)(a.med[MLAM][c] + 2 * a.med[MMU][c]);
     const T lam_R = lam2mu_R - 2 * mu0;
-    const T axvx = ldA<S16>(a, 0, c), ayvy = ldA<S16>(a, 4, c), azvz = ldA<S16>(a, 8, c);
+    const T axvx = lx ? ldA<S16>(a, 0, c) : (T)0, ayvy = ly ? ldA<S16>(a, 4, c) : (T)0;
+    const T azvz = lz ? ldA<S16>(a, 8, c) : (T)0;
     const T dxVx_ade = gxc1 * dxVx + gxc2 * axvx;
     const T dyVy_ade = gyc1 * dyVy + gyc2 * ayvy;
     const T dzVz_ade = gzc1 * dzVz + gzc2 * azvz;
     a.S[0][c] = a.S[0][c] + (lam2mu_R * dxVx_ade + lam_R * (dyVy_ade + dzVz_ade)) * dt;
     a.S[1][c] = a.S[1][c] + (lam2mu_R * dyVy_ade + lam_R * (dxVx_ade + dzVz_ade)) * dt;
     a.S[2][c] = a.S[2][c] + (lam2mu_R * dzVz_ade + lam_R * (dxVx_ade + dyVy_ade)) * dt;
-    stA<S16>(a, 0, c, gxc3 * axvx + gxc4 * dxVx * dt);
-    stA<S16>(a, 4, c, gyc3 * ayvy + gyc4 * dyVy * dt);
-    stA<S16>(a, 8, c, gzc3 * azvz + gzc4 * dzVz * dt);
+    if (lx) stA<S16>(a, 0, c, gxc3 * axvx + gxc4 * dxVx * dt);
+    if (ly) stA<S16>(a, 4, c, gyc3 * ayvy + gyc4 * dyVy * dt);
+    if (lz) stA<S16>(a, 8, c, gzc3 * azvz + gzc4 * dzVz * dt);
 
     // shear (m_absorb_p.f90:469-491)
     const T dxVy = (Vy[c + 1] - vy0) * r20x;
@@ -746,17 +1122,18 @@ __global__ void __launch_bounds__(256) k3_stress_pml(const K3<T> a)
     const T dyVz = (Vz[c + px] - vz0) * r20y;
     const T dzVx = (Vx[c + sxy] - vx0) * r20z;
     const T dzVy = (Vy[c + sxy] - vy0) * r20z;
-    const T ayvx = ldA<S16>(a, 1, c), azvx = ldA<S16>(a, 2, c), axvy = ldA<S16>(a, 3, c);
-    const T azvy = ldA<S16>(a, 5, c), axvz = ldA<S16>(a, 6, c), ayvz = ldA<S16>(a, 7, c);
+    const T ayvx = ly ? ldA<S16>(a, 1, c) : (T)0, azvx = lz ? ldA<S16>(a, 2, c) : (T)0;
+    const T axvy = lx ? ldA<S16>(a, 3, c) : (T)0, azvy = lz ? ldA<S16>(a, 5, c) : (T)0;
+    const T axvz = lx ? ldA<S16>(a, 6, c) : (T)0, ayvz = ly ? ldA<S16>(a, 7, c) : (T)0;
     a.S[3][c] = a.S[3][c] + medv<S16>(a, MMUYZ, c, k) * (gye1 * dyVz + gze1 * dzVy + gye2 * ayvz + gze2 * azvy) * dt;
     a.S[4][c] = a.S[4][c] + medv<S16>(a, MMUXZ, c, k) * (gxe1 * dxVz + gze1 * dzVx + gxe2 * axvz + gze2 * azvx) * dt;
     a.S[5][c] = a.S[5][c] + medv<S16>(a, MMUXY, c, k) * (gxe1 * dxVy + gye1 * dyVx + gxe2 * axvy + gye2 * ayvx) * dt;
-    stA<S16>(a, 1, c, gye3 * ayvx + gye4 * dyVx * dt);
-    stA<S16>(a, 2, c, gze3 * azvx + gze4 * dzVx * dt);
-    stA<S16>(a, 3, c, gxe3 * axvy + gxe4 * dxVy * dt);
-    stA<S16>(a, 5, c, gze3 * azvy + gze4 * dzVy * dt);
-    stA<S16>(a, 6, c, gxe3 * axvz + gxe4 * dxVz * dt);
-    stA<S16>(a, 7, c, gye3 * ayvz + gye4 * dyVz * dt);
+    if (ly) stA<S16>(a, 1, c, gye3 * ayvx + gye4 * dyVx * dt);
+    if (lz) stA<S16>(a, 2, c, gze3 * azvx + gze4 * dzVx * dt);
+    if (lx) stA<S16>(a, 3, c, gxe3 * axvy + gxe4 * dxVy * dt);
+    if (lz) stA<S16>(a, 5, c, gze3 * azvy + gze4 * dzVy * dt);
+    if (lx) stA<S16>(a, 6, c, gxe3 * axvz + gxe4 * dxVz * dt);
+    if (ly) stA<S16>(a, 7, c, gye3 * ayvz + gye4 * dyVz * dt);
 }
 
 template <typename T, bool S16>
@@ -781,6 +1158,10 @@ __global__ void __launch_bounds__(256) k3_vel_pml(const K3<T> a)
     const T gze1 = a.gz[4 * nz + k - 1], gze2 = a.gz[5 * nz + k - 1], gze3 = a.gz[6 * nz + k - 1], gze4 = a.gz[7 * nz + k - 1];
     const T r20x = a.r2[0], r20y = a.r2[1], r20z = a.r2[2];
     const T dt = a.dt;
+    // undamped axes: see k3_stress_pml
+    const bool lx = !a.ade_sparse || gxc4 != (T)0 || gxe4 != (T)0;
+    const bool ly = !a.ade_sparse || gyc4 != (T)0 || gye4 != (T)0;
+    const bool lz = !a.ade_sparse || gzc4 != (T)0 || gze4 != (T)0;
     // m_absorb_p.f90:266-298
     const T sxx0 = Sxx[c], syy0 = Syy[c], szz0 = Szz[c], syz0 = Syz[c], sxz0 = Sxz[c], sxy0 = Sxy[c];
     const T dxSxx = (Sxx[c + 1] - sxx0) * r20x;
@@ -792,24 +1173,26 @@ __global__ void __launch_bounds__(256) k3_vel_pml(const K3<T> a)
     const T dzSxz = (sxz0 - Sxz[c - sxy]) * r20z;
     const T dxSxy = (sxy0 - Sxy[c - 1]) * r20x;
     const T dySxy = (sxy0 - Sxy[c - px]) * r20y;
-    const T axsxx = ldA<S16>(a, 9, c), aysxy = ldA<S16>(a, 10, c), azsxz = ldA<S16>(a, 11, c);
-    const T axsxy = ldA<S16>(a, 12, c), aysyy = ldA<S16>(a, 13, c), azsyz = ldA<S16>(a, 14, c);
-    const T axsxz = ldA<S16>(a, 15, c), aysyz = ldA<S16>(a, 16, c), azszz = ldA<S16>(a, 17, c);
+    const T axsxx = lx ? ldA<S16>(a, 9, c) : (T)0, aysxy = ly ? ldA<S16>(a, 10, c) : (T)0;
+    const T azsxz = lz ? ldA<S16>(a, 11, c) : (T)0, axsxy = lx ? ldA<S16>(a, 12, c) : (T)0;
+    const T aysyy = ly ? ldA<S16>(a, 13, c) : (T)0, azsyz = lz ? ldA<S16>(a, 14, c) : (T)0;
+    const T axsxz = lx ? ldA<S16>(a, 15, c) : (T)0, aysyz = ly ? ldA<S16>(a, 16, c) : (T)0;
+    const T azszz = lz ? ldA<S16>(a, 17, c) : (T)0;
     a.V[0][c] = a.V[0][c] + medv<S16>(a, MBX, c, k) * (gxe1 * dxSxx + gyc1 * dySxy + gzc1 * dzSxz
                                                       + gxe2 * axsxx + gyc2 * aysxy + gzc2 * azsxz) * dt;
     a.V[1][c] = a.V[1][c] + medv<S16>(a, MBY, c, k) * (gxc1 * dxSxy + gye1 * dySyy + gzc1 * dzSyz
                                                       + gxc2 * axsxy + gye2 * aysyy + gzc2 * azsyz) * dt;
     a.V[2][c] = a.V[2][c] + medv<S16>(a, MBZ, c, k) * (gxc1 * dxSxz + gyc1 * dySyz + gze1 * dzSzz
                                                       + gxc2 * axsxz + gyc2 * aysyz + gze2 * azszz) * dt;
-    stA<S16>(a, 9, c, gxe3 * axsxx + gxe4 * dxSxx * dt);
-    stA<S16>(a, 10, c, gyc3 * aysxy + gyc4 * dySxy * dt);
-    stA<S16>(a, 11, c, gzc3 * azsxz + gzc4 * dzSxz * dt);
-    stA<S16>(a, 12, c, gxc3 * axsxy + gxc4 * dxSxy * dt);
-    stA<S16>(a, 13, c, gye3 * aysyy + gye4 * dySyy * dt);
-    stA<S16>(a, 14, c, gzc3 * azsyz + gzc4 * dzSyz * dt);
-    stA<S16>(a, 15, c, gxc3 * axsxz + gxc4 * dxSxz * dt);
-    stA<S16>(a, 16, c, gyc3 * aysyz + gyc4 * dySyz * dt);
-    stA<S16>(a, 17, c, gze3 * azszz + gze4 * dzSzz * dt);
+    if (lx) stA<S16>(a, 9, c, gxe3 * axsxx + gxe4 * dxSxx * dt);
+    if (ly) stA<S16>(a, 10, c, gyc3 * aysxy + gyc4 * dySxy * dt);
+    if (lz) stA<S16>(a, 11, c, gzc3 * azsxz + gzc4 * dzSxz * dt);
+    if (lx) stA<S16>(a, 12, c, gxc3 * axsxy + gxc4 * dxSxy * dt);
+    if (ly) stA<S16>(a, 13, c, gye3 * aysyy + gye4 * dySyy * dt);
+    if (lz) stA<S16>(a, 14, c, gzc3 * azsyz + gzc4 * dzSyz * dt);
+    if (lx) stA<S16>(a, 15, c, gxc3 * axsxz + gxc4 * dxSxz * dt);
+    if (ly) stA<S16>(a, 16, c, gyc3 * aysyz + gyc4 * dySyz * dt);
+    if (lz) stA<S16>(a, 17, c, gze3 * azszz + gze4 * dzSzz * dt);
 }
 
 // ---------------------------------------------------------------------------
@@ -1263,7 +1646,18 @@ struct swpc3d_ctx {
     // interior launch configuration (swpc3d_set_tuning)
     int s_by = 4, s_wpe = 3, s_split = 0, v_by = 4, v_wpe = 5, xcd = 0, s_kc = 32, v_kc = 0;
     int x4 = 1, x4_by = 2;  // 4-wide FP32 / reduced-storage interior kernels
+    int x4_form = 0;        // 4-wide stress: 0 normal + shear launches, 1 one pass (staged), 2 one pass (free)
+    int x4_wpe = 0;         // waves-per-SIMD hint of the one-pass kernel (0, 3, 4)
+    int x4_vz = 0;          // 4-wide velocity: 0 one plane per workgroup row, > 0 z-marching chunk length
+    int x4_sz = 0;          // 4-wide stress: > 0 one-pass z-marching kernel with this chunk length
     int pml4 = 0;           // 4-wide PML kernels (fewer waves per SIMD; off by default)
+    bool ade_sparse = true; // no ADE array has been set from outside since creation
+    // The PML kernels run on a second stream, concurrently with the
+    // interior kernels of the same half-step (they write disjoint cells and
+    // read the same inputs); events fork and join the two streams.
+    int conc = 0;
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     swpc::Prof prof;
     struct Snap {
         int kind, npts, ntdec, nbuf;
@@ -1326,6 +1720,7 @@ static K3<T> make_args(swpc3d_ctx *c)
     a.ti0 = c->ti0;
     a.pml = c->boxes;
     a.xcd = c->xcd;
+    a.ade_sparse = c->ade_sparse ? 1 : 0;
     return a;
 }
 
@@ -1429,6 +1824,13 @@ static unsigned x4_tiles(const swpc3d_ctx *c, K3<float> &a, int by)
     return (unsigned)a.ntx * (unsigned)a.nty * (unsigned)nk;
 }
 
+template <int BAR, int WPE>
+static void go_x4_fused(const K3<float> &a, unsigned nb, bool s16, hipStream_t s)
+{
+    if (s16) hipLaunchKernelGGL((k3x_stress<2, true, BAR, WPE>), dim3(nb), dim3(64, 2), 0, s, a);
+    else hipLaunchKernelGGL((k3x_stress<2, false, BAR, WPE>), dim3(nb), dim3(64, 2), 0, s, a);
+}
+
 template <int BY>
 static void go_x4_stress(const K3<float> &a, unsigned nb, bool s16, hipStream_t s)
 {
@@ -1450,6 +1852,30 @@ static void go_x4_vel(const K3<float> &a, unsigned nb, bool s16, hipStream_t s)
 
 static bool x4_stress(const swpc3d_ctx *c, K3<float> a, hipStream_t s)
 {
+    if (c->x4_sz > 0) {
+        // z-marching one-pass stress: 64 x 2 tiles, chunks of x4_sz planes
+        x4_tiles(c, a, 2);
+        const int nk = c->d.nz - c->d.na;
+        a.kc = c->x4_sz;
+        a.nkc = (nk + a.kc - 1) / a.kc;
+        if (a.ntx <= 0 || a.nty <= 0 || nk <= 0) return true;
+        const unsigned nb = (unsigned)a.ntx * (unsigned)a.nty * (unsigned)a.nkc;
+        if (c->s16) hipLaunchKernelGGL((k3x_stressz<2, true>), dim3(nb), dim3(64, 2), 0, s, a);
+        else hipLaunchKernelGGL((k3x_stressz<2, false>), dim3(nb), dim3(64, 2), 0, s, a);
+        return true;
+    }
+    if (c->x4_form != 0) {
+        // one-pass normal + shear kernel (64 x 2 tiles)
+        const unsigned nb = x4_tiles(c, a, 2);
+        if (nb == 0) return true;
+        const bool bar = c->x4_form == 1;
+        switch (c->x4_wpe) {
+        case 0: bar ? go_x4_fused<1, 0>(a, nb, c->s16, s) : go_x4_fused<0, 0>(a, nb, c->s16, s); return true;
+        case 3: bar ? go_x4_fused<1, 3>(a, nb, c->s16, s) : go_x4_fused<0, 3>(a, nb, c->s16, s); return true;
+        case 4: bar ? go_x4_fused<1, 4>(a, nb, c->s16, s) : go_x4_fused<0, 4>(a, nb, c->s16, s); return true;
+        default: return false;
+        }
+    }
     const unsigned nb = x4_tiles(c, a, c->x4_by);
     if (nb == 0) return true;
     switch (c->x4_by) {
@@ -1462,6 +1888,18 @@ static bool x4_stress(const swpc3d_ctx *c, K3<float> a, hipStream_t s)
 
 static bool x4_vel(const swpc3d_ctx *c, K3<float> a, hipStream_t s)
 {
+    if (c->x4_vz > 0) {
+        // z-marching 4-wide velocity: 64 x 2 tiles, chunks of x4_vz planes
+        x4_tiles(c, a, 2);
+        const int nk = c->d.nz - c->d.na;
+        a.kc = c->x4_vz;
+        a.nkc = (nk + a.kc - 1) / a.kc;
+        if (a.ntx <= 0 || a.nty <= 0 || nk <= 0) return true;
+        const unsigned nb = (unsigned)a.ntx * (unsigned)a.nty * (unsigned)a.nkc;
+        if (c->s16) hipLaunchKernelGGL((k3x_velz<2, true>), dim3(nb), dim3(64, 2), 0, s, a);
+        else hipLaunchKernelGGL((k3x_velz<2, false>), dim3(nb), dim3(64, 2), 0, s, a);
+        return true;
+    }
     const unsigned nb = x4_tiles(c, a, c->x4_by);
     if (nb == 0) return true;
     switch (c->x4_by) {
@@ -1472,6 +1910,26 @@ static bool x4_vel(const swpc3d_ctx *c, K3<float> a, hipStream_t s)
     }
 }
 
+// Second stream for the PML kernels: it waits for everything enqueued on
+// the main stream so far; join() makes the main stream wait for it.
+static int fork_side(swpc3d_ctx *c, hipStream_t &ps)
+{
+    ps = c->stream;
+    if (!c->conc || c->boxes.total == 0) return SWPC_OK;
+    SWPC_HIP(hipEventRecord(c->ev_fork, c->stream));
+    SWPC_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    ps = c->side;
+    return SWPC_OK;
+}
+
+static int join_side(swpc3d_ctx *c, hipStream_t ps)
+{
+    if (ps == c->stream) return SWPC_OK;
+    SWPC_HIP(hipEventRecord(c->ev_join, ps));
+    SWPC_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    return SWPC_OK;
+}
+
 template <typename T>
 static int launch_stress(swpc3d_ctx *c)
 {
@@ -1479,7 +1937,29 @@ static int launch_stress(swpc3d_ctx *c)
     const bool f64 = sizeof(T) == 8;
     const int by = f64 ? 4 : c->s_by, wpe = f64 ? 0 : c->s_wpe;
     set_tiles(c, a, by, c->s_kc);
+    hipStream_t ps;
+    if (int r = fork_side(c, ps)) return r;
     hipEvent_t e0;
+    if (c->boxes.total > 0) {
+        if (int r = prof_begin(c->prof, 1, ps, &e0)) return r;
+        bool done = false;
+        if constexpr (sizeof(T) == 4) {
+            if (c->pml4) {
+                const unsigned nb = (c->boxes4.total + 255) / 256;
+                if (c->s16) hipLaunchKernelGGL((k3x_stress_pml<true>), dim3(nb), dim3(256), 0, ps, a, c->boxes4);
+                else hipLaunchKernelGGL((k3x_stress_pml<false>), dim3(nb), dim3(256), 0, ps, a, c->boxes4);
+                done = true;
+            }
+        }
+        if (!done) {
+            if (c->s16)
+                hipLaunchKernelGGL((k3_stress_pml<T, true>), dim3((c->boxes.total + 255) / 256), dim3(256), 0, ps, a);
+            else
+                hipLaunchKernelGGL((k3_stress_pml<T, false>), dim3((c->boxes.total + 255) / 256), dim3(256), 0, ps, a);
+        }
+        SWPC_CHECK_LAUNCH();
+        if (int r = prof_end(c->prof, 1, ps, e0)) return r;
+    }
     if (a.nblk > 0) {
         if (int r = prof_begin(c->prof, 0, c->stream, &e0)) return r;
         bool ok;
@@ -1506,27 +1986,7 @@ static int launch_stress(swpc3d_ctx *c)
         SWPC_CHECK_LAUNCH();
         if (int r = prof_end(c->prof, 0, c->stream, e0)) return r;
     }
-    if (c->boxes.total > 0) {
-        if (int r = prof_begin(c->prof, 1, c->stream, &e0)) return r;
-        bool done = false;
-        if constexpr (sizeof(T) == 4) {
-            if (c->pml4) {
-                const unsigned nb = (c->boxes4.total + 255) / 256;
-                if (c->s16) hipLaunchKernelGGL((k3x_stress_pml<true>), dim3(nb), dim3(256), 0, c->stream, a, c->boxes4);
-                else hipLaunchKernelGGL((k3x_stress_pml<false>), dim3(nb), dim3(256), 0, c->stream, a, c->boxes4);
-                done = true;
-            }
-        }
-        if (!done) {
-            if (c->s16)
-                hipLaunchKernelGGL((k3_stress_pml<T, true>), dim3((c->boxes.total + 255) / 256), dim3(256), 0, c->stream, a);
-            else
-                hipLaunchKernelGGL((k3_stress_pml<T, false>), dim3((c->boxes.total + 255) / 256), dim3(256), 0, c->stream, a);
-        }
-        SWPC_CHECK_LAUNCH();
-        if (int r = prof_end(c->prof, 1, c->stream, e0)) return r;
-    }
-    return SWPC_OK;
+    return join_side(c, ps);
 }
 
 template <typename T>
@@ -1536,7 +1996,29 @@ static int launch_vel(swpc3d_ctx *c)
     const bool f64 = sizeof(T) == 8;
     const int by = f64 ? 4 : c->v_by, wpe = f64 ? 0 : c->v_wpe;
     set_tiles(c, a, by, c->v_kc);
+    hipStream_t ps;
+    if (int r = fork_side(c, ps)) return r;
     hipEvent_t e0;
+    if (c->boxes.total > 0) {
+        if (int r = prof_begin(c->prof, 3, ps, &e0)) return r;
+        bool done = false;
+        if constexpr (sizeof(T) == 4) {
+            if (c->pml4) {
+                const unsigned nb = (c->boxes4.total + 255) / 256;
+                if (c->s16) hipLaunchKernelGGL((k3x_vel_pml<true>), dim3(nb), dim3(256), 0, ps, a, c->boxes4);
+                else hipLaunchKernelGGL((k3x_vel_pml<false>), dim3(nb), dim3(256), 0, ps, a, c->boxes4);
+                done = true;
+            }
+        }
+        if (!done) {
+            if (c->s16)
+                hipLaunchKernelGGL((k3_vel_pml<T, true>), dim3((c->boxes.total + 255) / 256), dim3(256), 0, ps, a);
+            else
+                hipLaunchKernelGGL((k3_vel_pml<T, false>), dim3((c->boxes.total + 255) / 256), dim3(256), 0, ps, a);
+        }
+        SWPC_CHECK_LAUNCH();
+        if (int r = prof_end(c->prof, 3, ps, e0)) return r;
+    }
     if (a.nblk > 0) {
         if (int r = prof_begin(c->prof, 2, c->stream, &e0)) return r;
         bool ok;
@@ -1550,27 +2032,7 @@ static int launch_vel(swpc3d_ctx *c)
         SWPC_CHECK_LAUNCH();
         if (int r = prof_end(c->prof, 2, c->stream, e0)) return r;
     }
-    if (c->boxes.total > 0) {
-        if (int r = prof_begin(c->prof, 3, c->stream, &e0)) return r;
-        bool done = false;
-        if constexpr (sizeof(T) == 4) {
-            if (c->pml4) {
-                const unsigned nb = (c->boxes4.total + 255) / 256;
-                if (c->s16) hipLaunchKernelGGL((k3x_vel_pml<true>), dim3(nb), dim3(256), 0, c->stream, a, c->boxes4);
-                else hipLaunchKernelGGL((k3x_vel_pml<false>), dim3(nb), dim3(256), 0, c->stream, a, c->boxes4);
-                done = true;
-            }
-        }
-        if (!done) {
-            if (c->s16)
-                hipLaunchKernelGGL((k3_vel_pml<T, true>), dim3((c->boxes.total + 255) / 256), dim3(256), 0, c->stream, a);
-            else
-                hipLaunchKernelGGL((k3_vel_pml<T, false>), dim3((c->boxes.total + 255) / 256), dim3(256), 0, c->stream, a);
-        }
-        SWPC_CHECK_LAUNCH();
-        if (int r = prof_end(c->prof, 3, c->stream, e0)) return r;
-    }
-    return SWPC_OK;
+    return join_side(c, ps);
 }
 
 template <typename T>
@@ -1772,7 +2234,14 @@ int swpc3d_create(const swpc3d_desc *desc, swpc3d_ctx **out)
     c->L = L;
     c->esz = d.precision == SWPC_FP64 ? 8 : 4;
     c->s16 = d.precision == SWPC_FP32_S16;
+    // measured defaults at 512 x 512 x 256 on MI355X: one-pass stress for
+    // FP32 (the two-launch form for 16-bit storage), z-marching velocity
+    c->x4_form = c->s16 ? 0 : 1;
+    c->x4_vz = c->s16 ? 16 : 8;
     SWPC_HIP(hipStreamCreate(&c->stream));
+    SWPC_HIP(hipStreamCreate(&c->side));
+    SWPC_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    SWPC_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     const size_t vb = (size_t)L.vol * c->esz;
     int r;
     for (auto &b : c->V) if ((r = dalloc(b, vb, c->bytes))) return r;
@@ -1847,6 +2316,10 @@ int swpc3d_destroy(swpc3d_ctx *c)
         dfree(sn.frames);
     }
     c->prof.reset();
+    if (c->side) (void)hipStreamSynchronize(c->side);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SWPC_OK;
@@ -2220,6 +2693,9 @@ int swpc3d_set_field(swpc3d_ctx *c, int f, const double *in)
     if (!c || !in) return fail(SWPC_EINVAL, "null argument");
     DBuf *b = field_buf(c, f);
     if (!b) return fail(SWPC_EINVAL, "unknown field %d", f);
+    // an ADE array set from outside may be non-zero where its axis is
+    // undamped: the PML kernels then read and write every ADE value
+    if (f >= SWPC3D_ADE0) c->ade_sparse = false;
     SWPC_HIP(hipStreamSynchronize(c->stream));
     const size_t n = (size_t)c->L.vol;
     if (fp64(c)) {
@@ -2294,12 +2770,17 @@ int swpc3d_set_tuning(swpc3d_ctx *c, const int *p, int n)
 {
     clear_error();
     if (!c || (n > 0 && !p)) return fail(SWPC_EINVAL, "null argument");
-    int v[11] = {c->s_by, c->s_wpe, c->s_split, c->v_by, c->v_wpe, c->xcd, c->s_kc, c->v_kc, c->x4, c->x4_by, c->pml4};
-    for (int q = 0; q < n && q < 11; q++) v[q] = p[q];
+    int v[16] = {c->s_by, c->s_wpe, c->s_split, c->v_by, c->v_wpe, c->xcd, c->s_kc, c->v_kc, c->x4, c->x4_by, c->pml4,
+                 c->x4_form, c->x4_wpe, c->conc, c->x4_vz, c->x4_sz};
+    for (int q = 0; q < n && q < 16; q++) v[q] = p[q];
+    if (v[14] < 0 || v[15] < 0) return fail(SWPC_EINVAL, "negative z-chunk");
     if (v[6] < 0 || v[7] < 0) return fail(SWPC_EINVAL, "negative z-chunk");
     if (v[9] != 1 && v[9] != 2 && v[9] != 4) return fail(SWPC_EINVAL, "4-wide tile rows must be 1, 2 or 4");
+    if (v[11] < 0 || v[11] > 2) return fail(SWPC_EINVAL, "4-wide stress form must be 0, 1 or 2");
+    if (v[12] != 0 && v[12] != 3 && v[12] != 4) return fail(SWPC_EINVAL, "waves hint must be 0, 3 or 4");
     c->s_by = v[0]; c->s_wpe = v[1]; c->s_split = v[2] != 0; c->v_by = v[3]; c->v_wpe = v[4];
     c->xcd = v[5] != 0; c->s_kc = v[6]; c->v_kc = v[7]; c->x4 = v[8] != 0; c->x4_by = v[9]; c->pml4 = v[10] != 0;
+    c->x4_form = v[11]; c->x4_wpe = v[12]; c->conc = v[13] != 0; c->x4_vz = v[14]; c->x4_sz = v[15];
     return SWPC_OK;
 }
 

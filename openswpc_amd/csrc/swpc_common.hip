@@ -112,11 +112,15 @@ int swpc_device_count(void)
 // Stream-copy probe: the measured HBM bandwidth that the stencil kernels are
 // compared against (and the scale for rocprof FETCH/WRITE counters).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_stream_copy(const float4 *__restrict__ a, float4 *__restrict__ b, size_t n)
+// One 16-byte non-temporal load and store per thread and pass, the access
+// form of the stencil kernels' streamed arrays.
+typedef float f4c __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256) k_stream_copy(const f4c *__restrict__ a, f4c *__restrict__ b, size_t n)
 {
     size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * 256;
-    for (; i < n; i += stride) b[i] = a[i];
+    for (; i < n; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(a + i), b + i);
 }
 
 extern "C" int swpc_membw(long long bytes, int iters, double *gbs)
@@ -131,13 +135,13 @@ extern "C" int swpc_membw(long long bytes, int iters, double *gbs)
         rc = swpc::fail(SWPC_ENOMEM, "hipMalloc failed");
     } else {
         (void)hipMemset(a, 0, n * 16);
-        int blocks = (int)std::min<size_t>((n + 255) / 256, 256 * 32);
-        hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, 0, (const float4 *)a, (float4 *)b, n);
+        int blocks = (int)std::min<size_t>((n + 255) / 256, (size_t)1 << 20);
+        hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, 0, (const f4c *)a, (f4c *)b, n);
         (void)hipEventCreate(&e0);
         (void)hipEventCreate(&e1);
         (void)hipEventRecord(e0, 0);
         for (int q = 0; q < iters; q++)
-            hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, 0, (const float4 *)a, (float4 *)b, n);
+            hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, 0, (const f4c *)a, (f4c *)b, n);
         (void)hipEventRecord(e1, 0);
         if (hipEventSynchronize(e1) != hipSuccess) {
             rc = swpc::fail(SWPC_EHIP, "copy probe failed");

@@ -157,3 +157,25 @@ def test_determinism():
             sim.run(1, 30)
             out.append(sim.field("Vz"))
     assert np.array_equal(out[0], out[1])
+
+
+def test_3d_undamped_ade_skip_is_exact():
+    """The PML kernels leave ADE arrays of undamped axes untouched while they
+    are known to be zero; forcing the full update (by setting an ADE array
+    from the host) must give bit-identical results."""
+    nt = 60
+    case = problem.case3d(64, 56, 48, 0.5, 0.04, nt, na=8, src_depth=5.0, nst_side=3, zbeg=-2.0)
+    say("3-D ADE skip vs full PML update")
+    out = []
+    for full in (False, True):
+        for prec in ("fp32", "fp64", "fp32_s16"):
+            with Swpc3D(case.grid, case.medium, case.pml, case.glut, case.stations, 1, nt, precision=prec) as sim:
+                if full:
+                    sim.set_field("axVx", np.zeros(case.grid.shape))
+                sim.run(1, nt)
+                out.append((sim.receivers()[0], sim.field("Vx"), sim.field("aySxy")))
+    for q in range(3):
+        a, b = out[q], out[q + 3]
+        assert np.abs(a[0]).max() > 0
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y), q
