@@ -34,6 +34,7 @@ B_INT_STRESS = 58 * 4   # V 3r, medium 7r, R 18r/18w, S 6r/6w
 B_INT_VEL = 15 * 4      # S 6r, b 3r, V 3r/3w
 B_PML_STRESS = 38 * 4
 B_PML_VEL = 33 * 4
+CHOSEN = "fp32_r16"    # the reduced-storage form reported beside the plain one
 
 
 def log(msg):
@@ -52,7 +53,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-chosen", action="store_true")
-    ap.add_argument("--misfit-steps", type=int, default=500)
+    ap.add_argument("--misfit-steps", type=int, default=0,
+                    help="steps of the misfit runs (0: the full warmup + timed length)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -101,11 +103,11 @@ def main():
 
     chosen = None
     if not args.no_chosen:
-        red = timed_run(case, "fp32_s16", args.warmup, args.steps)
+        red = timed_run(case, CHOSEN, args.warmup, args.steps)
         ms_red = red["wall"] / args.steps * 1e3
-        mis = misfit(case, args.misfit_steps)
-        chosen = {"form": "reduced storage: FP32 arithmetic and V/S, FP16 memory variables (power-of-two scale), "
-                          "BF16 PML memory, FP16 medium residuals about FP32 per-plane bases; 20-cell ADE-PML",
+        mis = misfit(case, args.misfit_steps or nt)
+        chosen = {"form": "reduced storage (fp32_r16): FP32 arithmetic, V, S and medium; memory variables in FP16 "
+                          "with a power-of-two scale, PML ADE arrays in BF16; 20-cell ADE-PML",
                   "ms_per_step": ms_red, "value": ncell * args.steps / red["wall"],
                   "speedup_vs_plain": ms_step / ms_red, **mis}
         log(f"chosen: {ms_red:.3f} ms/step, speedup {ms_step / ms_red:.2f}x; misfit {mis}")
@@ -188,7 +190,7 @@ def misfit(case, steps):
     from openswpc_amd.solver import Swpc3D
     g = case.grid
     tr = {}
-    for prec in ("fp64", "fp32", "fp32_s16"):
+    for prec in ("fp64", "fp32", CHOSEN):
         sim = Swpc3D(g, case.medium, case.pml, case.glut, case.stations, case.ntdec_w, steps, precision=prec)
         sim.run(1, steps)
         tr[prec] = sim.receivers()[0].astype(np.float64)
@@ -196,7 +198,7 @@ def misfit(case, steps):
     ref = tr["fp64"]
     n = np.linalg.norm(ref)
     return {"misfit_steps": steps,
-            "misfit_vs_fp64": float(np.linalg.norm(tr["fp32_s16"] - ref) / n),
+            "misfit_vs_fp64": float(np.linalg.norm(tr[CHOSEN] - ref) / n),
             "plain_misfit_vs_fp64": float(np.linalg.norm(tr["fp32"] - ref) / n)}
 
 

@@ -14,7 +14,7 @@ module swpc_hip
     implicit none
     private
 
-    integer(c_int), parameter, public :: SWPC_FP32 = 0, SWPC_FP64 = 1, SWPC_FP32_S16 = 2
+    integer(c_int), parameter, public :: SWPC_FP32 = 0, SWPC_FP64 = 1, SWPC_FP32_S16 = 2, SWPC_FP32_R16 = 3
     integer(c_int), parameter, public :: SWPC_OK = 0, SWPC_EINVAL = -1, SWPC_EHIP = -2, SWPC_ESTATE = -3, &
                                          SWPC_ENOMEM = -4
     integer(c_int), parameter, public :: SWPC_SNAP_PS = 0, SWPC_SNAP_V = 1, SWPC_SNAP_U = 2
@@ -33,7 +33,7 @@ module swpc_hip
 
     public :: swpc_errmsg
     public :: swpc_version, swpc_device_count
-    public :: swpc3d_create, swpc3d_destroy, swpc3d_set_medium, swpc3d_set_surface, swpc3d_set_pml
+    public :: swpc3d_create, swpc3d_destroy, swpc3d_set_medium, swpc3d_set_density, swpc3d_set_surface, swpc3d_set_pml
     public :: swpc3d_set_sources, swpc3d_set_bodyforce, swpc3d_set_receivers
     public :: swpc3d_update_stress, swpc3d_update_vel, swpc3d_stressglut, swpc3d_bodyforce
     public :: swpc3d_store_receivers, swpc3d_add_snapshot, swpc3d_snapshots, swpc3d_get_snapshot
@@ -78,6 +78,13 @@ module swpc_hip
             type(c_ptr), value :: ctx
             real(c_float), intent(in) :: bx(*), by(*), bz(*), lam(*), mu(*), taup(*), taus(*)
             real(c_float), intent(in) :: muyz(*), muxz(*), muxy(*)
+        end function
+
+        !! optional density over the memory box (interior buoyancy formed in-kernel)
+        integer(c_int) function swpc3d_set_density(ctx, rho) bind(C, name='swpc3d_set_density')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            real(c_float), intent(in) :: rho(*)
         end function
 
         !! kfs_top, kfs_bot, kob_top, kob_bot, kob over (i,j) of the memory box

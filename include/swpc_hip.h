@@ -32,7 +32,8 @@ extern "C" {
 enum swpc_precision {
     SWPC_FP32 = 0,        /* state, memory variables and arithmetic in FP32 (reference built with MP = SP) */
     SWPC_FP64 = 1,        /* the same scheme in FP64 */
-    SWPC_FP32_S16 = 2     /* FP32 arithmetic and V/S; memory variables and medium stored in 16 bit with per-array scale */
+    SWPC_FP32_S16 = 2,    /* FP32 arithmetic and V/S; memory variables and medium stored in 16 bit with per-array scale */
+    SWPC_FP32_R16 = 3     /* FP32 arithmetic, V/S and medium; memory variables (FP16, scaled) and ADE arrays (BF16) in 16 bit */
 };
 
 enum swpc_error {
@@ -107,6 +108,13 @@ int swpc3d_destroy(swpc3d_ctx *ctx);
 int swpc3d_set_medium(swpc3d_ctx *ctx, const float *bx, const float *by, const float *bz,
                       const float *lam, const float *mu, const float *taup, const float *taus,
                       const float *muyz, const float *muxz, const float *muxy);
+
+/* Density rho over the memory box (the reference's real(SP) rho,
+   m_global.f90:39).  Optional: with it the FP32 interior velocity kernel
+   forms bx, by, bz = 2/(rho + rho') itself (m_medium.f90:442-444) from
+   one density stream instead of reading three buoyancy arrays; results are
+   identical.  Ignored in SWPC_FP32_S16 mode. */
+int swpc3d_set_density(swpc3d_ctx *ctx, const float *rho);
 
 /* Order-switch maps kfs_top/bot, kob_top/bot and kob over the (i,j)
    memory box (m_medium.f90:342-397; used by m_kernel.f90:103-104 and
@@ -206,7 +214,8 @@ int swpc3d_memory(swpc3d_ctx *ctx, long long *bytes);
      [10]    4-cell-per-thread PML kernels (0/1, default 0);
      [11]    4-wide stress form: 0 normal + shear launches, 1 one pass with
              staged loads, 2 one pass (default 1 for FP32, 0 for 16-bit
-             storage); [12] its waves hint (0/3/4);
+             storage); [12] tile rows of the z-marching velocity kernel
+             (1/2/4/8, default 2);
      [13]    PML kernels on a second stream, concurrent with the interior
              (0/1, default 0);
      [14]    4-wide velocity z-marching chunk (0 = one plane per workgroup;

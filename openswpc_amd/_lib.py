@@ -12,7 +12,7 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libswpc_hip.so"
 
-SWPC_FP32, SWPC_FP64, SWPC_FP32_S16 = 0, 1, 2
+SWPC_FP32, SWPC_FP64, SWPC_FP32_S16, SWPC_FP32_R16 = 0, 1, 2, 3
 
 P = C.c_void_p
 I = C.c_int
@@ -42,6 +42,7 @@ _PROTOS = {
     "swpc3d_create": (I, [C.POINTER(Desc3), C.POINTER(P)]),
     "swpc3d_destroy": (I, [P]),
     "swpc3d_set_medium": (I, [P] + [PF] * 10),
+    "swpc3d_set_density": (I, [P, PF]),
     "swpc3d_set_surface": (I, [P] + [PI_] * 5),
     "swpc3d_set_pml": (I, [P] + [PF] * 6),
     "swpc3d_set_sources": (I, [P, I, PI_, PI_, PI_, PD, I, PD]),

@@ -46,7 +46,7 @@ def tune(sim, it0=1, steps=6, verbose=False):
     table = []
     best_s = best_v = None
     for form, sz in STRESS4:
-        v = BASE + [form, 0, 0, 8, sz]
+        v = BASE + [form, 2, 0, 8, sz]
         sim.set_tuning(*v)
         t = _time(sim, it, steps, 0)
         it += steps
@@ -57,7 +57,7 @@ def tune(sim, it0=1, steps=6, verbose=False):
             best_s = ((form, sz), t)
     form, sz = best_s[0]
     for vz in VEL4:
-        sim.set_tuning(*(BASE + [form, 0, 0, vz, sz]))
+        sim.set_tuning(*(BASE + [form, 2, 0, vz, sz]))
         t = _time(sim, it, steps, 2)
         it += steps
         table.append((("vel4", vz), t))
@@ -65,7 +65,7 @@ def tune(sim, it0=1, steps=6, verbose=False):
             print(f"vel zchunk={vz}: {t * 1e3:.1f} us", flush=True)
         if best_v is None or t < best_v[1]:
             best_v = (vz, t)
-    best = BASE + [form, 0, 0, best_v[0], sz]
+    best = BASE + [form, 2, 0, best_v[0], sz]
     sim.set_tuning(*best)
     return best, table, it
 

@@ -51,6 +51,8 @@ struct K3 {
     T *R[18];   // Rxx(m) Ryy(m) Rzz(m) Ryz(m) Rxz(m) Rxy(m), m = 0..2
     T *A[18];   // ADE arrays (order of enum swpc3d_field)
     const float *med[10];   // bx by bz lam mu taup taus muyz muxz muxy (the reference's real(SP) arrays)
+    const float *rho;       // density (FP32 box) when set: the 4-wide velocity kernel forms bx, by, bz itself
+    int m16;                // medium stored as 16-bit residuals (SWPC_FP32_S16); else FP32 arrays
     // reduced storage (SWPC_FP32_S16): FP32 arithmetic, V and S in FP32
     const __half *hmed[10];     // medium: FP16 residual about a per-plane FP32 base
     const float2 *mbs[10];      // per-plane (base, scale), indexed k + KO
@@ -83,6 +85,7 @@ template <bool S16, typename T>
 __device__ __forceinline__ T medv(const K3<T> &a, int q, int c, int k)
 {
     if constexpr (S16) {
+        if (!a.m16) return (T)a.med[q][c];
         const float2 bs = a.mbs[q][k + KO];
         float v = bs.x;
         if (bs.y != 0.0f) v += bs.y * __half2float(a.hmed[q][c]);
@@ -389,6 +392,7 @@ template <bool S16>
 __device__ __forceinline__ f4v med4(const K3<float> &a, int q, int c, int k)
 {
     if constexpr (S16) {
+        if (!a.m16) return ld4nt(a.med[q] + c);
         const float2 bs = a.mbs[q][k + KO];
         f4v v = {bs.x, bs.x, bs.x, bs.x};
         if (bs.y != 0.0f) v += bs.y * h4(a.hmed[q] + c);
@@ -624,14 +628,23 @@ __global__ void __launch_bounds__(64 * BY) k3x_shear(const K3<float> a)
     st4m(a.S[5] + c, sxy_, msk);
 }
 
+// 4-point harmonic mean of averaged_medium (m_medium.f90:446-459) in the
+// reference's single-precision operation order, without contraction, so
+// that it equals the host-computed muxz/muxy/muyz bit for bit.
+__device__ __forceinline__ float harm4s(float a, float b, float c, float d)
+{
+#pragma clang fp contract(off)
+    const float epsl = 1.1920928955078125e-07f;  // epsilon(1.0)
+    return 4 * a * b * c * d / (a * b * c + a * b * d + a * c * d + b * c * d + epsl);
+}
+
 // Normal and shear stresses in one pass (m_kernel.f90:142-326): the
 // velocity differences of both parts are formed first from one set of V
 // loads, then the normal and the shear memory variables are streamed in
 // turn.  BAR = 1 keeps the compiler from hoisting the shear loads above the
 // normal stores, which bounds the registers held at once.
-template <int BY, bool S16, int BAR, int WPE>
-__global__ void __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8)))
-k3x_stress(const K3<float> a)
+template <int BY, bool S16, int BAR>
+__global__ void __launch_bounds__(64 * BY) k3x_stress(const K3<float> a)
 {
     int i0, j, k;
     unsigned msk;
@@ -675,7 +688,11 @@ k3x_stress(const K3<float> a)
     const float dt = a.dt;
     const f4v taus = med4<S16>(a, MTAUS, c, k);
     {
-        const f4v lam = med4<S16>(a, MLAM, c, k), mu = med4<S16>(a, MMU, c, k), taup = med4<S16>(a, MTAUP, c, k);
+        // FP32: mu with cached loads, its neighbours feed the shear means below
+        const f4v lam = med4<S16>(a, MLAM, c, k), taup = med4<S16>(a, MTAUP, c, k);
+        f4v mu;
+        if (S16 && a.m16) mu = med4<S16>(a, MMU, c, k);
+        else mu = ld4(a.med[MMU] + c);
         f4v R[9];
 #pragma unroll
         for (int q = 0; q < 9; q++) R[q] = ldR4<S16>(a, q, c);
@@ -717,7 +734,28 @@ k3x_stress(const K3<float> a)
     }
     if constexpr (BAR != 0) __builtin_amdgcn_sched_barrier(0);
     {
-        const f4v myz = med4<S16>(a, MMUYZ, c, k), mxz = med4<S16>(a, MMUXZ, c, k), mxy = med4<S16>(a, MMUXY, c, k);
+        f4v myz, mxz, mxy;
+        if (S16 && a.m16) {
+            myz = med4<S16>(a, MMUYZ, c, k);
+            mxz = med4<S16>(a, MMUXZ, c, k);
+            mxy = med4<S16>(a, MMUXY, c, k);
+        } else {
+            // interior cells: muxz, muxy, muyz from mu and its +1 neighbours
+            // (cached loads), instead of three more streamed arrays
+            const float *__restrict__ M = a.med[MMU];
+            const f4v m0 = ld4(M + c), mk = ld4(M + c + sxy), mj = ld4(M + c + px), mkj = ld4(M + c + sxy + px);
+            const float mi[5] = {m0.x, m0.y, m0.z, m0.w, M[c + 4]};
+            const float mki[5] = {mk.x, mk.y, mk.z, mk.w, M[c + sxy + 4]};
+            const float mji[5] = {mj.x, mj.y, mj.z, mj.w, M[c + px + 4]};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const float nnn = mi[q], pnn = mk[q], npn = mi[q + 1], ppn = mki[q + 1];
+                const float nnp = mj[q], npp = mji[q + 1], pnp = mkj[q];
+                mxz[q] = harm4s(nnn, pnn, npn, ppn);
+                mxy[q] = harm4s(nnn, npn, nnp, npp);
+                myz[q] = harm4s(nnn, pnn, nnp, pnp);
+            }
+        }
         f4v R[9];
 #pragma unroll
         for (int q = 0; q < 9; q++) R[q] = ldR4<S16>(a, 9 + q, c);
@@ -965,10 +1003,16 @@ __global__ void __launch_bounds__(64 * BY) k3x_vel(const K3<float> a)
 // register window (one new plane per step), so each of their planes is
 // read once per chunk instead of four times; y- and x-neighbours are read
 // as in k3x_vel.  Workgroups: 256-cell x tiles x BY rows x z-chunks.
-template <int BY, bool S16>
+template <int BY, bool S16, bool RHO>
 __global__ void __launch_bounds__(64 * BY) k3x_velz(const K3<float> a)
 {
-    const unsigned b = blockIdx.x;
+    unsigned b = blockIdx.x;
+    if (a.xcd) {
+        // each XCD walks one contiguous run of tiles, so the y-neighbour
+        // rows of a tile were fetched into the same L2
+        b = (b & 7u) * (a.nblk_pad >> 3) + (b >> 3);
+        if (b >= a.nblk) return;
+    }
     const unsigned xt = b % (unsigned)a.ntx;
     const unsigned t = b / (unsigned)a.ntx;
     const unsigned yt = t % (unsigned)a.nty;
@@ -998,6 +1042,11 @@ __global__ void __launch_bounds__(64 * BY) k3x_velz(const K3<float> a)
     f4v zz_m1 = ld4(Szz + c - sxy), zz_0 = ld4(Szz + c), zz_p1 = ld4(Szz + c + sxy);
     f4v xz_m2 = ld4(Sxz + c - 2 * sxy), xz_m1 = ld4(Sxz + c - sxy), xz_0 = ld4(Sxz + c);
     f4v yz_m2 = ld4(Syz + c - 2 * sxy), yz_m1 = ld4(Syz + c - sxy), yz_0 = ld4(Syz + c);
+    // RHO: buoyancies 2/(rho + rho') of averaged_medium (m_medium.f90:442-444)
+    // from a density window (planes k, k+1) instead of three streamed arrays
+    const float *__restrict__ Rh = a.rho;
+    f4v r_0;
+    if constexpr (RHO) r_0 = ld4(Rh + c);
     const float dt = a.dt;
     for (int k = kb; k <= ke; k++, c += sxy) {
         const f4v zz_p2 = ld4(Szz + c + 2 * sxy), xz_p1 = ld4(Sxz + c + sxy), yz_p1 = ld4(Syz + c + sxy);
@@ -1008,7 +1057,22 @@ __global__ void __launch_bounds__(64 * BY) k3x_velz(const K3<float> a)
         const float2 sxyl = ld2(Sxy + c - 2), sxyr = ld2(Sxy + c + 4);
         const float2 sxzl = ld2(Sxz + c - 2), sxzr = ld2(Sxz + c + 4);
         const f4v syzym1 = ld4(Syz + c - px), syzym2 = ld4(Syz + c - 2 * px), syzyp1 = ld4(Syz + c + px);
-        const f4v bx = med4<S16>(a, MBX, c, k), by = med4<S16>(a, MBY, c, k), bz = med4<S16>(a, MBZ, c, k);
+        f4v bx, by, bz, r_p1;
+        if constexpr (RHO) {
+            r_p1 = ld4(Rh + c + sxy);
+            const f4v rj = ld4(Rh + c + px);
+            const float ri[5] = {r_0.x, r_0.y, r_0.z, r_0.w, Rh[c + 4]};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                bx[q] = 2.0f / (ri[q] + ri[q + 1]);
+                by[q] = 2.0f / (r_0[q] + rj[q]);
+                bz[q] = 2.0f / (r_0[q] + r_p1[q]);
+            }
+        } else {
+            bx = med4<S16>(a, MBX, c, k);
+            by = med4<S16>(a, MBY, c, k);
+            bz = med4<S16>(a, MBZ, c, k);
+        }
         f4v vx = ld4nt(a.V[0] + c), vy = ld4nt(a.V[1] + c), vz = ld4nt(a.V[2] + c);
         // x rows i0-2 .. i0+5
         const float xx[8] = {sxxl.x, sxxl.y, sxx.x, sxx.y, sxx.z, sxx.w, sxxr.x, sxxr.y};
@@ -1041,6 +1105,7 @@ __global__ void __launch_bounds__(64 * BY) k3x_velz(const K3<float> a)
         zz_m1 = zz_0; zz_0 = zz_p1; zz_p1 = zz_p2;
         xz_m2 = xz_m1; xz_m1 = xz_0; xz_0 = xz_p1;
         yz_m2 = yz_m1; yz_m1 = yz_0; yz_0 = yz_p1;
+        if constexpr (RHO) r_0 = r_p1;
     }
 }
 
@@ -1125,9 +1190,23 @@ __global__ void __launch_bounds__(256) k3_stress_pml(const K3<T> a)
     const T ayvx = ly ? ldA<S16>(a, 1, c) : (T)0, azvx = lz ? ldA<S16>(a, 2, c) : (T)0;
     const T axvy = lx ? ldA<S16>(a, 3, c) : (T)0, azvy = lz ? ldA<S16>(a, 5, c) : (T)0;
     const T axvz = lx ? ldA<S16>(a, 6, c) : (T)0, ayvz = ly ? ldA<S16>(a, 7, c) : (T)0;
-    a.S[3][c] = a.S[3][c] + medv<S16>(a, MMUYZ, c, k) * (gye1 * dyVz + gze1 * dzVy + gye2 * ayvz + gze2 * azvy) * dt;
-    a.S[4][c] = a.S[4][c] + medv<S16>(a, MMUXZ, c, k) * (gxe1 * dxVz + gze1 * dzVx + gxe2 * axvz + gze2 * azvx) * dt;
-    a.S[5][c] = a.S[5][c] + medv<S16>(a, MMUXY, c, k) * (gxe1 * dxVy + gye1 * dyVx + gxe2 * axvy + gye2 * ayvx) * dt;
+    T myz, mxz, mxy;
+    if (S16 && a.m16) {
+        myz = medv<S16>(a, MMUYZ, c, k);
+        mxz = medv<S16>(a, MMUXZ, c, k);
+        mxy = medv<S16>(a, MMUXY, c, k);
+    } else {
+        // the averaged moduli of averaged_medium from mu and its +1 neighbours
+        const float *__restrict__ M = a.med[MMU];
+        const float nnn = M[c], pnn = M[c + sxy], npn = M[c + 1], ppn = M[c + sxy + 1];
+        const float nnp = M[c + px], npp = M[c + px + 1], pnp = M[c + sxy + px];
+        mxz = (T)harm4s(nnn, pnn, npn, ppn);
+        mxy = (T)harm4s(nnn, npn, nnp, npp);
+        myz = (T)harm4s(nnn, pnn, nnp, pnp);
+    }
+    a.S[3][c] = a.S[3][c] + myz * (gye1 * dyVz + gze1 * dzVy + gye2 * ayvz + gze2 * azvy) * dt;
+    a.S[4][c] = a.S[4][c] + mxz * (gxe1 * dxVz + gze1 * dzVx + gxe2 * axvz + gze2 * azvx) * dt;
+    a.S[5][c] = a.S[5][c] + mxy * (gxe1 * dxVy + gye1 * dyVx + gxe2 * axvy + gye2 * ayvx) * dt;
     if (ly) stA<S16>(a, 1, c, gye3 * ayvx + gye4 * dyVx * dt);
     if (lz) stA<S16>(a, 2, c, gze3 * azvx + gze4 * dzVx * dt);
     if (lx) stA<S16>(a, 3, c, gxe3 * axvy + gxe4 * dxVy * dt);
@@ -1178,12 +1257,25 @@ __global__ void __launch_bounds__(256) k3_vel_pml(const K3<T> a)
     const T aysyy = ly ? ldA<S16>(a, 13, c) : (T)0, azsyz = lz ? ldA<S16>(a, 14, c) : (T)0;
     const T axsxz = lx ? ldA<S16>(a, 15, c) : (T)0, aysyz = ly ? ldA<S16>(a, 16, c) : (T)0;
     const T azszz = lz ? ldA<S16>(a, 17, c) : (T)0;
-    a.V[0][c] = a.V[0][c] + medv<S16>(a, MBX, c, k) * (gxe1 * dxSxx + gyc1 * dySxy + gzc1 * dzSxz
-                                                      + gxe2 * axsxx + gyc2 * aysxy + gzc2 * azsxz) * dt;
-    a.V[1][c] = a.V[1][c] + medv<S16>(a, MBY, c, k) * (gxc1 * dxSxy + gye1 * dySyy + gzc1 * dzSyz
-                                                      + gxc2 * axsxy + gye2 * aysyy + gzc2 * azsyz) * dt;
-    a.V[2][c] = a.V[2][c] + medv<S16>(a, MBZ, c, k) * (gxc1 * dxSxz + gyc1 * dySyz + gze1 * dzSzz
-                                                      + gxc2 * axsxz + gyc2 * aysyz + gze2 * azszz) * dt;
+    T bx, by, bz;
+    if (a.rho) {
+        // buoyancies 2/(rho + rho') of averaged_medium from the density
+        const float *__restrict__ Rh = a.rho;
+        const float r0 = Rh[c];
+        bx = (T)(2.0f / (r0 + Rh[c + 1]));
+        by = (T)(2.0f / (r0 + Rh[c + px]));
+        bz = (T)(2.0f / (r0 + Rh[c + sxy]));
+    } else {
+        bx = medv<S16>(a, MBX, c, k);
+        by = medv<S16>(a, MBY, c, k);
+        bz = medv<S16>(a, MBZ, c, k);
+    }
+    a.V[0][c] = a.V[0][c] + bx * (gxe1 * dxSxx + gyc1 * dySxy + gzc1 * dzSxz
+                                  + gxe2 * axsxx + gyc2 * aysxy + gzc2 * azsxz) * dt;
+    a.V[1][c] = a.V[1][c] + by * (gxc1 * dxSxy + gye1 * dySyy + gzc1 * dzSyz
+                                  + gxc2 * axsxy + gye2 * aysyy + gzc2 * azsyz) * dt;
+    a.V[2][c] = a.V[2][c] + bz * (gxc1 * dxSxz + gyc1 * dySyz + gze1 * dzSzz
+                                  + gxc2 * axsxz + gyc2 * aysyz + gze2 * azszz) * dt;
     if (lx) stA<S16>(a, 9, c, gxe3 * axsxx + gxe4 * dxSxx * dt);
     if (ly) stA<S16>(a, 10, c, gyc3 * aysxy + gyc4 * dySxy * dt);
     if (lz) stA<S16>(a, 11, c, gzc3 * azsxz + gzc4 * dzSxz * dt);
@@ -1631,7 +1723,9 @@ struct swpc3d_ctx {
     swpc::DBuf V[3], S[6], R[18], A[18];
     swpc::DBuf med[10];     // bx by bz lam mu taup taus muyz muxz muxy
     swpc::DBuf mbs[10];     // reduced storage: per-plane (base, scale)
-    bool s16 = false;       // SWPC_FP32_S16
+    swpc::DBuf rho;         // optional density (swpc3d_set_density)
+    bool s16 = false;       // memory variables and ADE arrays in 16 bit (SWPC_FP32_S16, SWPC_FP32_R16)
+    bool m16 = false;       // medium in 16 bit too (SWPC_FP32_S16)
     float r_scale = 1.0f;   // memory-variable scale of the reduced storage
     swpc::DBuf surf, kob, gx, gy, gz, gxv;
     swpc::DBuf src, m6, sdrop, bsrc, f3, fdt;
@@ -1647,7 +1741,7 @@ struct swpc3d_ctx {
     int s_by = 4, s_wpe = 3, s_split = 0, v_by = 4, v_wpe = 5, xcd = 0, s_kc = 32, v_kc = 0;
     int x4 = 1, x4_by = 2;  // 4-wide FP32 / reduced-storage interior kernels
     int x4_form = 0;        // 4-wide stress: 0 normal + shear launches, 1 one pass (staged), 2 one pass (free)
-    int x4_wpe = 0;         // waves-per-SIMD hint of the one-pass kernel (0, 3, 4)
+    int x4_vby = 2;         // tile rows of the z-marching velocity kernel (1, 2, 4, 8)
     int x4_vz = 0;          // 4-wide velocity: 0 one plane per workgroup row, > 0 z-marching chunk length
     int x4_sz = 0;          // 4-wide stress: > 0 one-pass z-marching kernel with this chunk length
     int pml4 = 0;           // 4-wide PML kernels (fewer waves per SIMD; off by default)
@@ -1682,6 +1776,8 @@ static K3<T> make_args(swpc3d_ctx *c)
     for (int q = 0; q < 18; q++) a.R[q] = (T *)c->R[q].p;
     for (int q = 0; q < 18; q++) a.A[q] = (T *)c->A[q].p;
     for (int q = 0; q < 10; q++) a.med[q] = (const float *)c->med[q].p;
+    a.rho = c->m16 ? nullptr : (const float *)c->rho.p;
+    a.m16 = c->m16 ? 1 : 0;
     if (c->s16) {
         for (int q = 0; q < 10; q++) {
             a.hmed[q] = (const __half *)c->med[q].p;
@@ -1824,11 +1920,21 @@ static unsigned x4_tiles(const swpc3d_ctx *c, K3<float> &a, int by)
     return (unsigned)a.ntx * (unsigned)a.nty * (unsigned)nk;
 }
 
-template <int BAR, int WPE>
+template <int BY, int BAR>
 static void go_x4_fused(const K3<float> &a, unsigned nb, bool s16, hipStream_t s)
 {
-    if (s16) hipLaunchKernelGGL((k3x_stress<2, true, BAR, WPE>), dim3(nb), dim3(64, 2), 0, s, a);
-    else hipLaunchKernelGGL((k3x_stress<2, false, BAR, WPE>), dim3(nb), dim3(64, 2), 0, s, a);
+    if (s16) hipLaunchKernelGGL((k3x_stress<BY, true, BAR>), dim3(nb), dim3(64, BY), 0, s, a);
+    else hipLaunchKernelGGL((k3x_stress<BY, false, BAR>), dim3(nb), dim3(64, BY), 0, s, a);
+}
+
+template <int BY>
+static void go_x4_velz(const K3<float> &a, unsigned nb, bool s16, hipStream_t s)
+{
+    // V and S are FP32 in every 4-wide mode: with a density the kernel
+    // reads no medium array at all
+    if (a.rho) hipLaunchKernelGGL((k3x_velz<BY, false, true>), dim3(nb), dim3(64, BY), 0, s, a);
+    else if (s16) hipLaunchKernelGGL((k3x_velz<BY, true, false>), dim3(nb), dim3(64, BY), 0, s, a);
+    else hipLaunchKernelGGL((k3x_velz<BY, false, false>), dim3(nb), dim3(64, BY), 0, s, a);
 }
 
 template <int BY>
@@ -1865,14 +1971,14 @@ static bool x4_stress(const swpc3d_ctx *c, K3<float> a, hipStream_t s)
         return true;
     }
     if (c->x4_form != 0) {
-        // one-pass normal + shear kernel (64 x 2 tiles)
-        const unsigned nb = x4_tiles(c, a, 2);
+        // one-pass normal + shear kernel (64 x by tiles)
+        const unsigned nb = x4_tiles(c, a, c->x4_by);
         if (nb == 0) return true;
         const bool bar = c->x4_form == 1;
-        switch (c->x4_wpe) {
-        case 0: bar ? go_x4_fused<1, 0>(a, nb, c->s16, s) : go_x4_fused<0, 0>(a, nb, c->s16, s); return true;
-        case 3: bar ? go_x4_fused<1, 3>(a, nb, c->s16, s) : go_x4_fused<0, 3>(a, nb, c->s16, s); return true;
-        case 4: bar ? go_x4_fused<1, 4>(a, nb, c->s16, s) : go_x4_fused<0, 4>(a, nb, c->s16, s); return true;
+        switch (c->x4_by) {
+        case 1: bar ? go_x4_fused<1, 1>(a, nb, c->s16, s) : go_x4_fused<1, 0>(a, nb, c->s16, s); return true;
+        case 2: bar ? go_x4_fused<2, 1>(a, nb, c->s16, s) : go_x4_fused<2, 0>(a, nb, c->s16, s); return true;
+        case 4: bar ? go_x4_fused<4, 1>(a, nb, c->s16, s) : go_x4_fused<4, 0>(a, nb, c->s16, s); return true;
         default: return false;
         }
     }
@@ -1889,16 +1995,23 @@ static bool x4_stress(const swpc3d_ctx *c, K3<float> a, hipStream_t s)
 static bool x4_vel(const swpc3d_ctx *c, K3<float> a, hipStream_t s)
 {
     if (c->x4_vz > 0) {
-        // z-marching 4-wide velocity: 64 x 2 tiles, chunks of x4_vz planes
-        x4_tiles(c, a, 2);
+        // z-marching 4-wide velocity: 64 x vby tiles, chunks of x4_vz planes
+        const int vby = c->x4_vby;
+        x4_tiles(c, a, vby);
         const int nk = c->d.nz - c->d.na;
         a.kc = c->x4_vz;
         a.nkc = (nk + a.kc - 1) / a.kc;
         if (a.ntx <= 0 || a.nty <= 0 || nk <= 0) return true;
-        const unsigned nb = (unsigned)a.ntx * (unsigned)a.nty * (unsigned)a.nkc;
-        if (c->s16) hipLaunchKernelGGL((k3x_velz<2, true>), dim3(nb), dim3(64, 2), 0, s, a);
-        else hipLaunchKernelGGL((k3x_velz<2, false>), dim3(nb), dim3(64, 2), 0, s, a);
-        return true;
+        a.nblk = (unsigned)a.ntx * (unsigned)a.nty * (unsigned)a.nkc;
+        a.nblk_pad = (a.nblk + 7u) & ~7u;
+        const unsigned nb = a.xcd ? a.nblk_pad : a.nblk;
+        switch (vby) {
+        case 1: go_x4_velz<1>(a, nb, c->s16, s); return true;
+        case 2: go_x4_velz<2>(a, nb, c->s16, s); return true;
+        case 4: go_x4_velz<4>(a, nb, c->s16, s); return true;
+        case 8: go_x4_velz<8>(a, nb, c->s16, s); return true;
+        default: return false;
+        }
     }
     const unsigned nb = x4_tiles(c, a, c->x4_by);
     if (nb == 0) return true;
@@ -2219,7 +2332,8 @@ int swpc3d_create(const swpc3d_desc *desc, swpc3d_ctx **out)
     if (d.na < 0 || 2 * d.na >= d.nx || 2 * d.na >= d.ny || d.na >= d.nz)
         return fail(SWPC_EINVAL, "absorber thickness na=%d does not fit the grid", d.na);
     if (d.nm != NM3) return fail(SWPC_EINVAL, "nm=%d: the kernels are built for NM = 3", d.nm);
-    if (d.precision != SWPC_FP32 && d.precision != SWPC_FP64 && d.precision != SWPC_FP32_S16)
+    if (d.precision != SWPC_FP32 && d.precision != SWPC_FP64 && d.precision != SWPC_FP32_S16 &&
+        d.precision != SWPC_FP32_R16)
         return fail(SWPC_EINVAL, "precision mode %d not supported by this build", d.precision);
     if (!(d.dx > 0 && d.dy > 0 && d.dz > 0 && d.dt > 0)) return fail(SWPC_EINVAL, "non-positive dx/dy/dz/dt");
     for (int m = 0; m < d.nm; m++)
@@ -2233,11 +2347,13 @@ int swpc3d_create(const swpc3d_desc *desc, swpc3d_ctx **out)
     c->d = d;
     c->L = L;
     c->esz = d.precision == SWPC_FP64 ? 8 : 4;
-    c->s16 = d.precision == SWPC_FP32_S16;
-    // measured defaults at 512 x 512 x 256 on MI355X: one-pass stress for
-    // FP32 (the two-launch form for 16-bit storage), z-marching velocity
-    c->x4_form = c->s16 ? 0 : 1;
-    c->x4_vz = c->s16 ? 16 : 8;
+    c->s16 = d.precision == SWPC_FP32_S16 || d.precision == SWPC_FP32_R16;
+    c->m16 = d.precision == SWPC_FP32_S16;
+    // measured defaults at 512 x 512 x 256 on MI355X: one-pass stress (the
+    // two-launch form for the 16-bit medium), z-marching velocity
+    c->x4_form = c->m16 ? 0 : 1;
+    c->x4_vz = c->m16 ? 16 : 8;
+    c->x4_vby = c->m16 ? 2 : 4;
     SWPC_HIP(hipStreamCreate(&c->stream));
     SWPC_HIP(hipStreamCreate(&c->side));
     SWPC_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
@@ -2249,8 +2365,8 @@ int swpc3d_create(const swpc3d_desc *desc, swpc3d_ctx **out)
     const size_t vb16 = (size_t)L.vol * 2;
     for (auto &b : c->R) if ((r = dalloc(b, c->s16 ? vb16 : vb, c->bytes))) return r;
     for (auto &b : c->A) if ((r = dalloc(b, c->s16 ? vb16 : vb, c->bytes))) return r;
-    for (auto &b : c->med) if ((r = dalloc(b, c->s16 ? vb16 : (size_t)L.vol * 4, c->bytes))) return r;
-    if (c->s16)
+    for (auto &b : c->med) if ((r = dalloc(b, c->m16 ? vb16 : (size_t)L.vol * 4, c->bytes))) return r;
+    if (c->m16)
         for (auto &b : c->mbs) if ((r = dalloc(b, (size_t)L.pz * 8, c->bytes))) return r;
     if ((r = dalloc(c->surf, (size_t)d.nx * d.ny * 16, c->bytes))) return r;
     if ((r = dalloc(c->kob, (size_t)d.nx * d.ny * 4, c->bytes))) return r;
@@ -2307,6 +2423,7 @@ int swpc3d_destroy(swpc3d_ctx *c)
     for (auto &b : c->A) dfree(b);
     for (auto &b : c->med) dfree(b);
     for (auto &b : c->mbs) dfree(b);
+    dfree(c->rho);
     DBuf *rest[] = {&c->surf, &c->kob, &c->gx, &c->gy, &c->gz, &c->gxv, &c->src, &c->m6, &c->sdrop, &c->bsrc, &c->f3,
                     &c->fdt, &c->st, &c->uacc, &c->wvel, &c->wdisp, &c->vmax};
     for (auto b : rest) dfree(*b);
@@ -2338,7 +2455,7 @@ int swpc3d_set_medium(swpc3d_ctx *c, const float *bx, const float *by, const flo
     std::vector<float> h((size_t)c->L.vol, 0.0f);
     for (int q = 0; q < 10; q++) {
         box_to_dev(c->L, in[q], h.data());
-        if (!c->s16) {
+        if (!c->m16) {
             SWPC_HIP(hipMemcpy(c->med[q].p, h.data(), h.size() * 4, hipMemcpyHostToDevice));
             continue;
         }
@@ -2370,6 +2487,19 @@ int swpc3d_set_medium(swpc3d_ctx *c, const float *bx, const float *by, const flo
         SWPC_HIP(hipMemcpy(c->mbs[q].p, bs.data(), bs.size() * 8, hipMemcpyHostToDevice));
     }
     c->have_medium = true;
+    return SWPC_OK;
+}
+
+int swpc3d_set_density(swpc3d_ctx *c, const float *rho)
+{
+    clear_error();
+    if (!c || !rho) return fail(SWPC_EINVAL, "null argument");
+    if (c->m16) return SWPC_OK;  // the 16-bit medium kernels stream their own buoyancy arrays
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    std::vector<float> h((size_t)c->L.vol, 0.0f);
+    box_to_dev(c->L, rho, h.data());
+    if (int r = dalloc(c->rho, h.size() * 4, c->bytes)) return r;
+    SWPC_HIP(hipMemcpy(c->rho.p, h.data(), h.size() * 4, hipMemcpyHostToDevice));
     return SWPC_OK;
 }
 
@@ -2771,16 +2901,17 @@ int swpc3d_set_tuning(swpc3d_ctx *c, const int *p, int n)
     clear_error();
     if (!c || (n > 0 && !p)) return fail(SWPC_EINVAL, "null argument");
     int v[16] = {c->s_by, c->s_wpe, c->s_split, c->v_by, c->v_wpe, c->xcd, c->s_kc, c->v_kc, c->x4, c->x4_by, c->pml4,
-                 c->x4_form, c->x4_wpe, c->conc, c->x4_vz, c->x4_sz};
+                 c->x4_form, c->x4_vby, c->conc, c->x4_vz, c->x4_sz};
     for (int q = 0; q < n && q < 16; q++) v[q] = p[q];
     if (v[14] < 0 || v[15] < 0) return fail(SWPC_EINVAL, "negative z-chunk");
     if (v[6] < 0 || v[7] < 0) return fail(SWPC_EINVAL, "negative z-chunk");
     if (v[9] != 1 && v[9] != 2 && v[9] != 4) return fail(SWPC_EINVAL, "4-wide tile rows must be 1, 2 or 4");
     if (v[11] < 0 || v[11] > 2) return fail(SWPC_EINVAL, "4-wide stress form must be 0, 1 or 2");
-    if (v[12] != 0 && v[12] != 3 && v[12] != 4) return fail(SWPC_EINVAL, "waves hint must be 0, 3 or 4");
+    if (v[12] != 1 && v[12] != 2 && v[12] != 4 && v[12] != 8)
+        return fail(SWPC_EINVAL, "z-marching velocity tile rows must be 1, 2, 4 or 8");
     c->s_by = v[0]; c->s_wpe = v[1]; c->s_split = v[2] != 0; c->v_by = v[3]; c->v_wpe = v[4];
     c->xcd = v[5] != 0; c->s_kc = v[6]; c->v_kc = v[7]; c->x4 = v[8] != 0; c->x4_by = v[9]; c->pml4 = v[10] != 0;
-    c->x4_form = v[11]; c->x4_wpe = v[12]; c->conc = v[13] != 0; c->x4_vz = v[14]; c->x4_sz = v[15];
+    c->x4_form = v[11]; c->x4_vby = v[12]; c->conc = v[13] != 0; c->x4_vz = v[14]; c->x4_sz = v[15];
     return SWPC_OK;
 }
 

@@ -28,7 +28,8 @@ FIELDS2 = ["Vx", "Vz", "Sxx", "Szz", "Sxz"] + \
     [f"R{c}{m}" for c in ("xx", "zz", "xz") for m in (1, 2, 3)] + \
     ["axVx", "azVx", "axVz", "azVz", "axSxx", "azSxz", "axSxz", "azSzz"]
 
-PRECISION = {"fp32": _lib.SWPC_FP32, "fp64": _lib.SWPC_FP64, "fp32_s16": _lib.SWPC_FP32_S16}
+PRECISION = {"fp32": _lib.SWPC_FP32, "fp64": _lib.SWPC_FP64, "fp32_s16": _lib.SWPC_FP32_S16,
+             "fp32_r16": _lib.SWPC_FP32_R16}
 
 
 def _i32(a):
@@ -64,6 +65,8 @@ class Swpc3D:
         m = medium
         self._keep = [_f32(a) for a in (m.bx, m.by, m.bz, m.lam, m.mu, m.taup, m.taus, m.muyz, m.muxz, m.muxy)]
         check(L.swpc3d_set_medium(h, *[ptr(a, C.c_float) for a in self._keep]), "set_medium")
+        rho = _f32(m.rho)
+        check(L.swpc3d_set_density(h, ptr(rho, C.c_float)), "set_density")
         s = [_i32(a) for a in (m.kfs_top, m.kfs_bot, m.kob_top, m.kob_bot, m.kob)]
         check(L.swpc3d_set_surface(h, *[ptr(a, C.c_int) for a in s]), "set_surface")
         self._keep = None

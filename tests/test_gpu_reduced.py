@@ -1,7 +1,8 @@
-"""Reduced-storage form (FP32 arithmetic; memory variables in FP16 with a
-per-array power-of-two scale, ADE arrays in BF16, medium as an FP32
-per-plane base plus an FP16 residual) against the FP64 run of the same
-scheme.  Bar: seismogram relative L2 misfit <= 1e-3.
+"""Reduced-storage forms against the FP64 run of the same scheme: FP32
+arithmetic; memory variables in FP16 with a per-array power-of-two scale and
+ADE arrays in BF16; the medium either as an FP32 per-plane base plus an FP16
+residual (fp32_s16) or in FP32 (fp32_r16).  Bar: seismogram relative L2
+misfit <= 1e-3.
 """
 import numpy as np
 import pytest
@@ -26,14 +27,15 @@ def run(case, precision, nt):
         return vel, disp, sim.field("Vz")
 
 
+@pytest.mark.parametrize("form", ["fp32_s16", "fp32_r16"])
 @pytest.mark.parametrize("rmed", [None, (0.05, 2.0, 0.5)])
-def test_reduced_storage_misfit_vs_fp64(rmed):
+def test_reduced_storage_misfit_vs_fp64(rmed, form):
     nt = 400
     case = problem.case3d(128, 120, 80, 0.25, 0.01, nt, na=12, src_depth=5.0, nst_side=4, zbeg=-2.0,
                           rmed=rmed, rmed_tile=(64, 64, 64))
-    print(f"reduced vs fp64: 128x120x80 nt={nt} rmed={rmed}", flush=True)
+    print(f"reduced ({form}) vs fp64: 128x120x80 nt={nt} rmed={rmed}", flush=True)
     v64, d64, vz64 = run(case, "fp64", nt)
-    v16, d16, vz16 = run(case, "fp32_s16", nt)
+    v16, d16, vz16 = run(case, form, nt)
     v32, d32, _ = run(case, "fp32", nt)
     e16 = rel(v16, v64)
     e32 = rel(v32, v64)
