@@ -15,7 +15,8 @@ import math
 import numpy as np
 
 from . import fdtool, medium as md, pml, rmedia
-from .source import PointSource, Station, stress_glut2, stress_glut3, station_cells2, station_cells3
+from .source import (BodyForce, PointSource, Station, body_force2, body_force3, station_cells2, station_cells3,
+                     stress_glut2, stress_glut3)
 
 F32 = np.float32
 
@@ -42,6 +43,7 @@ class Case3D:
     nt: int
     ntdec_w: int
     mech: tuple
+    bforce: object = None     # body-force table (bf_mode) instead of the stress glut
 
 
 @dataclass
@@ -55,6 +57,7 @@ class Case2D:
     nt: int
     ntdec_w: int
     mech: tuple
+    bforce: object = None
 
 
 def mechanism(seed):
@@ -65,9 +68,12 @@ def mechanism(seed):
 
 def case3d(nx, ny, nz, dx, dt, nt, na=20, layers=LHM_EXAMPLE, seed=1, rmed=None, src_depth=10.0,
            tr=1.0, t0=0.1, mo=1e15, nst_side=8, st_spacing=None, ntdec_w=1, rmed_tile=(128, 128, 64),
-           benchmark=False, zbeg=None):
+           benchmark=False, zbeg=None, bodyforce=False):
     """3-D case: layered (+ random) medium, double couple at src_depth below
     the grid centre, an nst_side x nst_side receiver grid on the free surface.
+    bodyforce=True replaces the double couple by a single force of the same
+    position and time function with a direction drawn from the seed (the
+    reference's bf_mode).
 
     rmed = (epsilon, a_km, kappa) adds a von Karman perturbation generated
     on a periodic tile and mapped by rmed_tile periodicity.
@@ -84,7 +90,12 @@ def case3d(nx, ny, nz, dx, dt, nt, na=20, layers=LHM_EXAMPLE, seed=1, rmed=None,
     strike, dip, rake = mechanism(seed)
     src = PointSource(0.0, 0.0, src_depth, t0, tr, mo, fdtool.sdr2moment(strike, dip, rake))
     glut = stress_glut3(g, [src], nt)
-    prof = pml.pml3(g, glut.fcut)
+    bforce = None
+    if bodyforce:
+        f = np.random.default_rng(seed + 1000).standard_normal(3)
+        bforce = body_force3(g, [BodyForce(0.0, 0.0, src_depth, t0, tr, tuple(float(v) for v in f))], nt)
+        glut = None
+    prof = pml.pml3(g, (bforce or glut).fcut)
     if st_spacing is None:
         st_spacing = (nx - 2 * na) * dx / (nst_side + 1)
     sts = []
@@ -94,11 +105,11 @@ def case3d(nx, ny, nz, dx, dt, nt, na=20, layers=LHM_EXAMPLE, seed=1, rmed=None,
             y = (b_ - (nst_side - 1) / 2) * st_spacing
             sts.append(Station(x + 0.3 * dx, y + 0.3 * dx, 0.0, f"S{a_:02d}{b_:02d}", "fsb"))
     cells, kept = station_cells3(g, med.kfs, med.kob, sts)
-    return Case3D(g, med, prof, glut, cells, kept, nt, ntdec_w, (strike, dip, rake))
+    return Case3D(g, med, prof, glut, cells, kept, nt, ntdec_w, (strike, dip, rake), bforce)
 
 
 def case2d(nx, nz, dx, dt, nt, na=20, layers=LHM_TWO, seed=1, rmed=None, src_depth=10.0, tr=1.0, t0=0.1,
-           mo=1e15, nst=16, st_spacing=2.0, ntdec_w=1, benchmark=False, zbeg=None):
+           mo=1e15, nst=16, st_spacing=2.0, ntdec_w=1, benchmark=False, zbeg=None, bodyforce=False):
     """P-SV case: layered (+ random) medium, double couple (its x-z part) at
     src_depth, nst receivers on the free surface at st_spacing km."""
     g = md.Grid2D(nx, nz, dx, dx, dt, na=na, zbeg=zbeg)
@@ -112,10 +123,15 @@ def case2d(nx, nz, dx, dt, nt, na=20, layers=LHM_TWO, seed=1, rmed=None, src_dep
     strike, dip, rake = mechanism(seed)
     src = PointSource(0.0, 0.0, src_depth, t0, tr, mo, fdtool.sdr2moment(strike, dip, rake))
     glut = stress_glut2(g, [src], nt)
-    prof = pml.pml2(g, glut.fcut)
+    bforce = None
+    if bodyforce:
+        f = np.random.default_rng(seed + 1000).standard_normal(3)
+        bforce = body_force2(g, [BodyForce(0.0, 0.0, src_depth, t0, tr, tuple(float(v) for v in f))], nt)
+        glut = None
+    prof = pml.pml2(g, (bforce or glut).fcut)
     sts = [Station((n - (nst - 1) / 2) * st_spacing + 0.3 * dx, 0.0, 0.0, f"P{n:02d}", "fsb") for n in range(nst)]
     cells, kept = station_cells2(g, med.kfs, med.kob, sts)
-    return Case2D(g, med, prof, glut, cells, kept, nt, ntdec_w, (strike, dip, rake))
+    return Case2D(g, med, prof, glut, cells, kept, nt, ntdec_w, (strike, dip, rake), bforce)
 
 
 def stability3(g, vmax):

@@ -15,7 +15,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import check, lib, ptr
+from ._lib import SwpcError, check, lib, ptr
 
 UC3 = 1e-15   # src/swpc_3d/m_global.f90:29
 UC2 = 1e-12   # src/swpc_psv/m_global.f90:29
@@ -42,6 +42,50 @@ def _f32(a):
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class NumericalDivergence(SwpcError):
+    """Raised by run_monitored when max|V| * UC exceeds 1e5 (the check of
+    report__progress, src/swpc_3d/m_report.f90:141-150) or is not finite."""
+
+
+DIVERGENCE_TOL = 1e5
+
+
+def _run_monitored(sim, it0, nsteps, ntdec_r, log):
+    """report__progress (m_report.f90:120-185) around the time loop: at every
+    it with mod(it, ntdec_r) = 0, before that step's updates, max|V| at the
+    ocean-bottom/surface layer is checked for divergence and reported as
+    physical velocities (times UC*M0), with the time per step and the ETA."""
+    import math
+    import time
+    t0 = time.perf_counter()
+    it = it0
+    end = it0 + nsteps
+    while it < end:
+        nxt = (it // ntdec_r + 1) * ntdec_r if it % ntdec_r else it
+        if nxt > it:
+            m = min(nxt, end) - it
+            sim.run(it, m)
+            it += m
+            continue
+        vm = sim.kernel__vmax().astype(np.float64)
+        amp = float(vm.max()) * sim.UC
+        if not math.isfinite(amp) or not np.all(np.isfinite(vm)) or amp > DIVERGENCE_TOL:
+            raise NumericalDivergence(f"numerical divergence detected at it={it} with max amp = {amp:.3e} "
+                                      "[(m/s)/moment]")
+        if log is not None:
+            el = time.perf_counter() - t0
+            done = it - it0
+            per = el / done if done else 0.0
+            eta = per * (end - it)
+            h, r = divmod(int(eta), 3600)
+            va = vm * sim.UC * sim.M0
+            log(f"  it={it:07d},{per:6.3f} s/loop, eta {h:03d}:{r // 60:02d}:{r % 60:02d}, ("
+                + " ".join(f"{v:9.2E}" for v in va) + " )")
+        m = min(ntdec_r, end - it)
+        sim.run(it, m)
+        it += m
 
 
 class Swpc3D:
@@ -73,6 +117,7 @@ class Swpc3D:
         gp = [_f32(a) for a in pml]
         check(L.swpc3d_set_pml(h, *[ptr(a, C.c_float) for a in gp]), "set_pml")
         self.M0 = 1.0
+        self.UC = UC3
         if glut is not None and len(glut.idx):
             idx = _i32(glut.idx)
             i, j, k = _i32(idx[:, 0]), _i32(idx[:, 1]), _i32(idx[:, 2])
@@ -89,6 +134,7 @@ class Swpc3D:
                                          ptr(_f64(bodyforce.m), C.c_double), tab.shape[0], ptr(tab, C.c_double)),
                   "set_bodyforce")
             self.M0 = bodyforce.M0
+            self.UC = UC3 * 1e3   # bf_mode unit conversion (m_source.f90:190-195)
         self.nst = 0
         self.ntw = 0
         self.ntdec_w = ntdec_w
@@ -121,6 +167,11 @@ class Swpc3D:
         check(lib().swpc3d_vmax(self.h, ptr(out, C.c_float)), "vmax")
         return out
 
+    def run_monitored(self, it0, nsteps, ntdec_r=100, log=None):
+        """run() with the reference's progress report and divergence check
+        every ntdec_r steps; raises NumericalDivergence."""
+        _run_monitored(self, it0, nsteps, ntdec_r, log)
+
     # -- whole steps ------------------------------------------------------
     def step(self, it):
         check(lib().swpc3d_step(self.h, it), "step")
@@ -140,7 +191,7 @@ class Swpc3D:
         if self.nst:
             check(lib().swpc3d_get_receivers(self.h, ptr(vel, C.c_float), ptr(disp, C.c_double)), "get_receivers")
         if scaled:
-            s = self.M0 * UC3 * 1e9
+            s = self.M0 * self.UC * 1e9
             return (vel * np.float32(s)).astype(np.float32), disp * s
         return vel, disp
 
@@ -230,6 +281,7 @@ class SwpcPSV:
         gp = [_f32(a) for a in pml]
         check(L.swpc2d_set_pml(h, *[ptr(a, C.c_float) for a in gp]), "set_pml")
         self.M0 = 1.0
+        self.UC = UC2
         if glut is not None and len(glut.idx):
             idx = _i32(glut.idx)
             tab = _f64(glut.table)
@@ -244,6 +296,7 @@ class SwpcPSV:
                                          ptr(_f64(bodyforce.m), C.c_double), tab.shape[0], ptr(tab, C.c_double)),
                   "set_bodyforce")
             self.M0 = bodyforce.M0
+            self.UC = UC2 * 1e3   # bf_mode unit conversion (src/swpc_psv/m_source.f90:157-160)
         self.nst = 0
         self.ntw = 0
         if stations is not None and len(stations):
@@ -274,6 +327,12 @@ class SwpcPSV:
         check(lib().swpc2d_vmax(self.h, ptr(out, C.c_float)), "vmax")
         return out
 
+    def run_monitored(self, it0, nsteps, ntdec_r=100, log=None):
+        """run() with the reference's progress report and divergence check
+        every ntdec_r steps (src/swpc_psv/m_report.f90); raises
+        NumericalDivergence."""
+        _run_monitored(self, it0, nsteps, ntdec_r, log)
+
     def step(self, it):
         check(lib().swpc2d_step(self.h, it), "step")
 
@@ -289,7 +348,7 @@ class SwpcPSV:
         if self.nst:
             check(lib().swpc2d_get_receivers(self.h, ptr(vel, C.c_float), ptr(disp, C.c_double)), "get_receivers")
         if scaled:
-            s = self.M0 * UC2 * 1e9
+            s = self.M0 * self.UC * 1e9
             return (vel * np.float32(s)).astype(np.float32), disp * s
         return vel, disp
 

@@ -168,6 +168,22 @@ def body_force3(g, forces, nt, stftype="kupper"):
     return SourceTable(np.array(idx, np.int32), f3, tab, M0, fcut)
 
 
+def body_force2(g, forces, nt, stftype="kupper"):
+    """P-SV body-force table (src/swpc_psv/m_source.f90:616-654): f2 =
+    (fx, fz) normalised by M0 = |f| (m_source.f90:157-158); table[it-1, n] =
+    momentrate(tbeg+it dt) dt/(dx dz)/2; the kernel divides by rho(k,i)."""
+    M0 = math.sqrt(sum(float(F32(f.f3[0])) ** 2 + float(F32(f.f3[2])) ** 2 for f in forces))
+    idx = [(fdtool.x2i(f.x, g.xbeg, g.dx), fdtool.x2i(f.z, g.zbeg, g.dz)) for f in forces]
+    dt_dxz = float(F32(g.dt)) / (float(g.dx) * float(g.dz))
+    tab = np.zeros((nt, len(forces)))
+    for n, f in enumerate(forces):
+        for it in range(1, nt + 1):
+            tab[it - 1, n] = float(fdtool.momentrate(_t_vel(g.tbeg, it, g.dt), stftype, f.t0, f.tr)) * dt_dxz / 2
+    f2 = np.array([[float(f.f3[0]) / M0, float(f.f3[2]) / M0] for f in forces])
+    fcut = max(1.0 / float(F32(f.tr)) for f in forces)
+    return SourceTable(np.array(idx, np.int32), f2, tab, M0, fcut)
+
+
 # ---------------------------------------------------------------------------
 # stations
 # ---------------------------------------------------------------------------

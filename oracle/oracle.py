@@ -126,6 +126,12 @@ class Oracle3D:
             self.ksrc = np.ascontiguousarray(gl.idx[:, 2], np.int32)
             self.m6 = np.ascontiguousarray(gl.m, np.float64)
             self.tab = np.ascontiguousarray(gl.table, np.float64)
+        bf = getattr(case, "bforce", None)
+        self.nbf = len(bf.idx) if bf is not None else 0
+        if self.nbf:
+            self.bf_idx = [np.ascontiguousarray(bf.idx[:, q], np.int32) for q in range(3)]
+            self.bf_f = np.ascontiguousarray(bf.m, np.float64)
+            self.bf_tab = np.ascontiguousarray(bf.table, np.float64)
         st = case.stations
         self.nst = len(st)
         if self.nst:
@@ -194,6 +200,14 @@ class Oracle3D:
             if due:
                 sn["frames"].append(out.reshape(sn["nvar"], n))
 
+    def bodyforce(self, it):
+        if self.nbf:
+            row = np.ascontiguousarray(self.bf_tab[it - 1])
+            PD = C.POINTER(C.c_double)
+            i, j, k = self.bf_idx
+            self.L.o3d_bodyforce(self._ps(), self.nbf, i.ctypes.data_as(_PI), j.ctypes.data_as(_PI),
+                                 k.ctypes.data_as(_PI), self.bf_f.ctypes.data_as(PD), row.ctypes.data_as(PD))
+
     def step(self, it):
         """main.f90:125-144 order (receivers and snapshots sampled before
         the updates)."""
@@ -202,6 +216,7 @@ class Oracle3D:
         self.update_stress()
         self.stressglut(it)
         self.update_vel()
+        self.bodyforce(it)
 
     def run(self, it0, n):
         for it in range(it0, it0 + n):
@@ -273,6 +288,12 @@ class Oracle2D:
             self.ksrc = np.ascontiguousarray(gl.idx[:, 1], np.int32)
             self.m3 = np.ascontiguousarray(gl.m, np.float64)
             self.tab = np.ascontiguousarray(gl.table, np.float64)
+        bf = getattr(case, "bforce", None)
+        self.nbf = len(bf.idx) if bf is not None else 0
+        if self.nbf:
+            self.bf_idx = [np.ascontiguousarray(bf.idx[:, q], np.int32) for q in range(2)]
+            self.bf_f = np.ascontiguousarray(bf.m, np.float64)
+            self.bf_tab = np.ascontiguousarray(bf.table, np.float64)
         st = case.stations
         self.nst = len(st)
         if self.nst:
@@ -336,13 +357,24 @@ class Oracle2D:
             if due:
                 sn["frames"].append(out.reshape(2, n))
 
+    def bodyforce(self, it):
+        if self.nbf:
+            row = np.ascontiguousarray(self.bf_tab[it - 1])
+            PD = C.POINTER(C.c_double)
+            i, k = self.bf_idx
+            self.L.o2d_bodyforce(self._ps(), self.nbf, i.ctypes.data_as(_PI), k.ctypes.data_as(_PI),
+                                 self.bf_f.ctypes.data_as(PD), row.ctypes.data_as(PD))
+
     def step(self, it):
-        """src/swpc_psv/main.f90:95-114 order."""
+        """src/swpc_psv/main.f90:95-114 order; the body force (added after the
+        velocity update there, before the absorber) commutes with the
+        absorber's disjoint-cell update."""
         self.snapshots(it)
         self.receivers(it)
         self.update_stress()
         self.stressglut(it)
         self.update_vel()
+        self.bodyforce(it)
 
     def run(self, it0, n):
         for it in range(it0, it0 + n):

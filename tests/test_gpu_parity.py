@@ -179,3 +179,58 @@ def test_3d_undamped_ade_skip_is_exact():
         assert np.abs(a[0]).max() > 0
         for x, y in zip(a, b):
             assert np.array_equal(x, y), q
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_3d_bodyforce(bits):
+    """bf_mode: a single force injected into V after the velocity update
+    (m_source.f90:835-876), traces and fields against the oracle."""
+    nt = 60
+    case = problem.case3d(64, 56, 48, 0.5, 0.04, nt, na=8, src_depth=5.0, nst_side=3, zbeg=-2.0, bodyforce=True)
+    say(f"3-D body force fp{bits}")
+    ref = Oracle3D(case, bits)
+    ref.run(1, nt)
+    with Swpc3D(case.grid, case.medium, case.pml, None, case.stations, 1, nt,
+                precision="fp32" if bits == 32 else "fp64", bodyforce=case.bforce) as sim:
+        sim.run(1, nt)
+        vel, disp = sim.receivers()
+        tol = TOL32 if bits == 32 else TOL64
+        assert np.abs(ref.vel).max() > 0
+        assert rel(vel, ref.vel) <= tol
+        for f in ("Vx", "Vz", "Sxy"):
+            assert rel(sim.field(f), ref.field(f)) <= tol, f
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_psv_bodyforce(bits):
+    nt = 80
+    case = problem.case2d(160, 80, 0.25, 0.02, nt, na=10, src_depth=5.0, zbeg=-2.0, bodyforce=True)
+    say(f"P-SV body force fp{bits}")
+    ref = Oracle2D(case, bits)
+    ref.run(1, nt)
+    with SwpcPSV(case.grid, case.medium, case.pml, None, case.stations, 1, nt,
+                 precision="fp32" if bits == 32 else "fp64", bodyforce=case.bforce) as sim:
+        sim.run(1, nt)
+        vel, _ = sim.receivers()
+        tol = TOL32 if bits == 32 else TOL64
+        assert np.abs(ref.vel).max() > 0
+        assert rel(vel, ref.vel) <= tol
+        assert rel(sim.field("Vz"), ref.field("Vz")) <= tol
+
+
+def test_divergence_detected():
+    """A time step far above the stability limit: run_monitored stops with
+    NumericalDivergence at a report step (m_report.f90:141-150) instead of
+    running on with overflowing fields; a stable run passes its checks."""
+    from openswpc_amd.solver import NumericalDivergence
+    say("divergence check")
+    case = problem.case3d(64, 56, 48, 0.5, 0.08, 400, na=8, src_depth=5.0, nst_side=2, zbeg=-2.0)
+    assert problem.stability3(case.grid, case.medium.vmax) > 1.0
+    with Swpc3D(case.grid, case.medium, case.pml, case.glut, case.stations, 1, 400) as sim:
+        with pytest.raises(NumericalDivergence):
+            sim.run_monitored(1, 400, ntdec_r=20)
+    ok = problem.case3d(64, 56, 48, 0.5, 0.02, 100, na=8, src_depth=5.0, nst_side=2, zbeg=-2.0)
+    lines = []
+    with Swpc3D(ok.grid, ok.medium, ok.pml, ok.glut, ok.stations, 1, 100) as sim:
+        sim.run_monitored(1, 100, ntdec_r=25, log=lines.append)
+    assert len(lines) == 4

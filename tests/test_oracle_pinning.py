@@ -352,3 +352,32 @@ def test_psv_plane_wave_speed():
     assert abs(peak - expected) <= 1.5
     # no up-going wave of comparable size
     assert prof[: int(z0)].max() < 0.05 * prof.max()
+
+
+def test_bodyforce_momentum_balance():
+    """A single force F f(t) inside a homogeneous whole space adds momentum
+    F * sum_it f(t_it) dt, and the interior update conserves it (the
+    staggered divergence telescopes) until waves reach the absorber: the
+    total sum Vx / bx * dV must equal fx times the discrete STF integral.
+    Pins the injection nodes, the factor 1/2 per node and the sign (a wrong
+    one changes the total or breaks its conservation)."""
+    from openswpc_amd.source import BodyForce, body_force3
+    n, nt = 41, 22
+    dx, dt = 0.5, 0.03
+    beg = -(n / 2) * dx
+    g = md.Grid3D(n, n, n, dx, dx, dx, dt, na=6, xbeg=beg, ybeg=beg, zbeg=beg)
+    rho, lam, mu, qp, qs = md.layered3(g, [md.Layer(-1000.0, 2.7, 6.0, 3.5, 1e30, 1e30)])
+    m = md.medium_setup3(g, rho, lam, mu, qp, qs, benchmark=True)
+    f = (0.6, -0.3, 0.8)
+    bf = body_force3(g, [BodyForce(0.0, 0.0, 0.0, 0.0, 0.6, f)], nt)
+    case = problem.Case3D(g, m, pml.pml3(g, bf.fcut), None, np.zeros((0, 3), np.int32), [], nt, 1, (0, 0, 0), bf)
+    o = Oracle3D(case, 64)
+    o.run(1, nt)
+    dV = g.dx * g.dy * g.dz
+    stf_sum = float(bf.table[:, 0].sum()) * 2 * dV   # = sum_it f(t_it) dt
+    assert abs(stf_sum - 1.0) < 1e-3                   # the Kupper pulse is complete
+    for name, b, comp in (("Vx", "bx", 0), ("Vy", "by", 1), ("Vz", "bz", 2)):
+        p = float((o.f[name] / np.where(getattr(m, b) > 0, getattr(m, b), 1.0)).sum()) * dV
+        assert math.isclose(p, bf.m[0][comp] * stf_sum, rel_tol=1e-7, abs_tol=1e-12), (name, p)
+    # the wavefield has not reached the absorber
+    assert np.abs(o.f["Vx"][:, : g.na + 4, :]).max() < 1e-6 * np.abs(o.f["Vx"]).max()
