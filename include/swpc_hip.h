@@ -220,7 +220,11 @@ int swpc3d_memory(swpc3d_ctx *ctx, long long *bytes);
              (0/1, default 0);
      [14]    4-wide velocity z-marching chunk (0 = one plane per workgroup;
              default 8 for FP32, 16 for 16-bit storage);
-     [15]    4-wide one-pass z-marching stress chunk (0 = off, default). */
+     [15]    4-wide one-pass z-marching stress chunk (0 = off, default);
+     [16]    steps per captured HIP graph in swpc3d_run (0 = launch step
+             by step; default 16).  The step number lives in device memory,
+             so one captured sequence replays for any it; setters drop the
+             graph, and profiling runs step by step. */
 int swpc3d_set_tuning(swpc3d_ctx *ctx, const int *p, int n);
 
 /* ------------------------------------------------------------------ */
@@ -270,6 +274,10 @@ int swpc2d_add_snapshot(swpc2d_ctx *ctx, int kind, int npts, const int *ik, int 
 int swpc2d_snapshots(swpc2d_ctx *ctx, int it);
 int swpc2d_get_snapshot(swpc2d_ctx *ctx, int id, int frame, float *out);
 int swpc2d_step(swpc2d_ctx *ctx, int it);
+/* Steps per captured HIP graph in swpc2d_run (0 launches step by step;
+   default 16).  No counterpart in the reference: the per-step work of
+   small P-SV grids is shorter than its launch overhead. */
+int swpc2d_set_graph(swpc2d_ctx *ctx, int steps);
 int swpc2d_run(swpc2d_ctx *ctx, int it0, int nsteps);
 /* vel[itw][n][2], disp[itw][n][2] (x, z-up) */
 int swpc2d_get_receivers(swpc2d_ctx *ctx, float *vel, double *disp);

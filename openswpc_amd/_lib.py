@@ -83,6 +83,7 @@ _PROTOS = {
     "swpc2d_add_snapshot": (I, [P, I, I, PI_, I, I, C.c_float, C.c_float, C.c_float, PI_]),
     "swpc2d_snapshots": (I, [P, I]),
     "swpc2d_get_snapshot": (I, [P, I, I, PF]),
+    "swpc2d_set_graph": (I, [P, I]),
     "swpc2d_step": (I, [P, I]),
     "swpc2d_run": (I, [P, I, I]),
     "swpc2d_get_receivers": (I, [P, PF, PD]),

@@ -228,9 +228,12 @@ __global__ void __launch_bounds__(256) k2_vel_pml(const K2<T> a)
 
 template <typename T>
 __global__ void k2_stressglut(const K2<T> a, int nsrc, const int *__restrict__ src, const double *__restrict__ m3,
-                              const double *__restrict__ sdrop)
+                              const double *__restrict__ sdrop_all, int nt, const int *__restrict__ dit)
 {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int it = *dit;
+    if (it < 1 || it > nt) return;
+    const double *__restrict__ sdrop = sdrop_all + (size_t)(it - 1) * nsrc;
     const int px = a.px;
     for (int n = 0; n < nsrc; n++) {
         const int c = at2(px, src[2 * n], src[2 * n + 1]);
@@ -247,9 +250,12 @@ __global__ void k2_stressglut(const K2<T> a, int nsrc, const int *__restrict__ s
 
 template <typename T>
 __global__ void k2_bodyforce(const K2<T> a, int nsrc, const int *__restrict__ src, const double *__restrict__ f2,
-                             const double *__restrict__ fdt)
+                             const double *__restrict__ fdt_all, int nt, const int *__restrict__ dit)
 {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int it = *dit;
+    if (it < 1 || it > nt) return;
+    const double *__restrict__ fdt = fdt_all + (size_t)(it - 1) * nsrc;
     const int px = a.px;
     for (int n = 0; n < nsrc; n++) {
         const int c = at2(px, src[2 * n], src[2 * n + 1]);
@@ -264,10 +270,16 @@ __global__ void k2_bodyforce(const K2<T> a, int nsrc, const int *__restrict__ sr
 
 template <typename T>
 __global__ void k2_receivers(const K2<T> a, int nst, const int *__restrict__ st, double *__restrict__ uacc,
-                             float *__restrict__ vel_out, double *__restrict__ disp_out, int store)
+                             float *__restrict__ wvel, double *__restrict__ wdisp, int ntdec_w, int ntw,
+                             const int *__restrict__ dit)
 {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= nst) return;
+    const int it = *dit;
+    const int itw = (it - 1) / ntdec_w;
+    const int store = ((it - 1) % ntdec_w) == 0 && itw < ntw;
+    float *__restrict__ vel_out = wvel + (size_t)(store ? itw : 0) * nst * 2;
+    double *__restrict__ disp_out = wdisp + (size_t)(store ? itw : 0) * nst * 2;
     const int px = a.px;
     const int c = at2(px, st[2 * n], st[2 * n + 1]);
     const T vx = a.Vx[c] + a.Vx[c - 1];
@@ -289,6 +301,17 @@ __global__ void k2_fill_u32(unsigned *p, int n, unsigned v)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = v;
+}
+
+// device-side step number (see the 3-D solver): set per run, advanced per step
+__global__ void k2_step_set(int *dit, int it)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *dit = it;
+}
+
+__global__ void k2_step_advance(int *dit)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *dit = *dit + 1;
 }
 
 template <typename T>
@@ -318,11 +341,16 @@ __global__ void k2_vmax(const K2<T> a, const int *__restrict__ kob, unsigned *__
 // output out[n*npts + p]; the reference's operation order, no contraction.
 template <typename T>
 __global__ void k2_snap(const K2<T> a, int kind, int npts, const int2 *__restrict__ pts, float uc, float m0, float dt,
-                        float *__restrict__ acc, float *__restrict__ out, int store)
+                        float *__restrict__ acc, float *__restrict__ frames, int ntdec, int nbuf,
+                        const int *__restrict__ dit)
 {
 #pragma clang fp contract(off)
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npts) return;
+    const int it = *dit;
+    const int store = ((it - 1) % ntdec) == 0;
+    if (!store && kind != 2) return;
+    float *__restrict__ out = frames + (size_t)(((it - 1) / ntdec) % nbuf) * 2 * npts;
     const int2 q = pts[p];
     const int px = a.px;
     const int c = at2(px, q.x, q.y);
@@ -371,6 +399,10 @@ struct swpc2d_ctx {
     bool have_medium = false, have_surface = false, have_pml = false;
     swpc::Boxes2 boxes;
     int ti0 = 1, ntx = 0, ntz = 0;
+    int graph_k = 16;       // steps per captured step graph (0 = launch step by step)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    swpc::DBuf dit;         // device step number
     struct Snap {
         int kind, npts, ntdec, nbuf;
         float uc, m0, dt;
@@ -464,84 +496,138 @@ static int l2_vel(swpc2d_ctx *c)
 }
 
 template <typename T>
-static int l2_glut(swpc2d_ctx *c, int it)
+static int l2_glut(swpc2d_ctx *c)
 {
     if (c->nsrc == 0) return SWPC_OK;
-    if (it < 1 || it > c->nt_src) return fail(SWPC_EINVAL, "stressglut: it=%d outside 1..%d", it, c->nt_src);
     K2<T> a = make_args2<T>(c);
     hipLaunchKernelGGL((k2_stressglut<T>), dim3(1), dim3(64), 0, c->stream, a, c->nsrc, (const int *)c->src.p,
-                       (const double *)c->m3.p, (const double *)c->sdrop.p + (size_t)(it - 1) * c->nsrc);
+                       (const double *)c->m3.p, (const double *)c->sdrop.p, c->nt_src, (const int *)c->dit.p);
     SWPC_CHECK_LAUNCH();
     return SWPC_OK;
 }
 
 template <typename T>
-static int l2_bf(swpc2d_ctx *c, int it)
+static int l2_bf(swpc2d_ctx *c)
 {
     if (c->nbf == 0) return SWPC_OK;
-    if (it < 1 || it > c->nt_bf) return fail(SWPC_EINVAL, "bodyforce: it=%d outside 1..%d", it, c->nt_bf);
     K2<T> a = make_args2<T>(c);
     hipLaunchKernelGGL((k2_bodyforce<T>), dim3(1), dim3(64), 0, c->stream, a, c->nbf, (const int *)c->bsrc.p,
-                       (const double *)c->f2.p, (const double *)c->fdt.p + (size_t)(it - 1) * c->nbf);
+                       (const double *)c->f2.p, (const double *)c->fdt.p, c->nt_bf, (const int *)c->dit.p);
     SWPC_CHECK_LAUNCH();
     return SWPC_OK;
 }
 
 template <typename T>
-static int l2_recv(swpc2d_ctx *c, int it)
+static int l2_recv(swpc2d_ctx *c)
 {
     if (c->nst == 0) return SWPC_OK;
-    int store = ((it - 1) % c->ntdec_w) == 0;
-    int itw = (it - 1) / c->ntdec_w;
-    if (store && itw >= c->ntw) store = 0;
     K2<T> a = make_args2<T>(c);
-    size_t off = (size_t)(store ? itw : 0) * c->nst * 2;
     hipLaunchKernelGGL((k2_receivers<T>), dim3((c->nst + 63) / 64), dim3(64), 0, c->stream, a, c->nst,
-                       (const int *)c->st.p, (double *)c->uacc.p, (float *)c->wvel.p + off,
-                       (double *)c->wdisp.p + off, store);
+                       (const int *)c->st.p, (double *)c->uacc.p, (float *)c->wvel.p, (double *)c->wdisp.p,
+                       c->ntdec_w, c->ntw, (const int *)c->dit.p);
     SWPC_CHECK_LAUNCH();
     return SWPC_OK;
 }
 
 template <typename T>
-static int l2_snap(swpc2d_ctx *c, int it)
+static int l2_snap(swpc2d_ctx *c)
 {
     if (c->snaps.empty()) return SWPC_OK;
     K2<T> a = make_args2<T>(c);
     for (auto &sn : c->snaps) {
-        const bool due = ((it - 1) % sn.ntdec) == 0;
-        if (!due && sn.kind != 2) continue;
-        const long long frame = (it - 1) / sn.ntdec;
-        float *out = (float *)sn.frames.p + (size_t)(frame % sn.nbuf) * 2 * sn.npts;
         hipLaunchKernelGGL((k2_snap<T>), dim3((sn.npts + 255) / 256), dim3(256), 0, c->stream, a, sn.kind, sn.npts,
-                           (const int2 *)sn.pts.p, sn.uc, sn.m0, sn.dt, (float *)sn.acc.p, out, due ? 1 : 0);
+                           (const int2 *)sn.pts.p, sn.uc, sn.m0, sn.dt, (float *)sn.acc.p, (float *)sn.frames.p,
+                           sn.ntdec, sn.nbuf, (const int *)c->dit.p);
         SWPC_CHECK_LAUNCH();
-        if (due) sn.last = std::max(sn.last, frame);
     }
+    return SWPC_OK;
+}
+
+static void note_snapshots2(swpc2d_ctx *c, int it0, int it1)
+{
+    for (auto &sn : c->snaps) {
+        const long long last_due = (long long)(it1 - 1) / sn.ntdec * sn.ntdec + 1;
+        if (last_due >= it0) sn.last = std::max(sn.last, (last_due - 1) / sn.ntdec);
+    }
+}
+
+static int set_step2(swpc2d_ctx *c, int it)
+{
+    hipLaunchKernelGGL(k2_step_set, dim3(1), dim3(64), 0, c->stream, (int *)c->dit.p, it);
+    SWPC_CHECK_LAUNCH();
     return SWPC_OK;
 }
 
 // P-SV step order (src/swpc_psv/main.f90:95-114): snapshots, receivers, stress,
 // absorber, stress glut, velocity, body force, absorber.  The additive
-// body force commutes with the absorber update.
-static int step2(swpc2d_ctx *c, int it)
+// body force commutes with the absorber update.  The step number is the
+// device counter, advanced at the end.
+template <typename T>
+static int step2_dev(swpc2d_ctx *c)
 {
     int r;
-    if (fp64(c)) {
-        if ((r = l2_snap<double>(c, it))) return r;
-        if ((r = l2_recv<double>(c, it))) return r;
-        if ((r = l2_stress<double>(c))) return r;
-        if ((r = l2_glut<double>(c, it))) return r;
-        if ((r = l2_vel<double>(c))) return r;
-        if ((r = l2_bf<double>(c, it))) return r;
-    } else {
-        if ((r = l2_snap<float>(c, it))) return r;
-        if ((r = l2_recv<float>(c, it))) return r;
-        if ((r = l2_stress<float>(c))) return r;
-        if ((r = l2_glut<float>(c, it))) return r;
-        if ((r = l2_vel<float>(c))) return r;
-        if ((r = l2_bf<float>(c, it))) return r;
+    if ((r = l2_snap<T>(c))) return r;
+    if ((r = l2_recv<T>(c))) return r;
+    if ((r = l2_stress<T>(c))) return r;
+    if ((r = l2_glut<T>(c))) return r;
+    if ((r = l2_vel<T>(c))) return r;
+    if ((r = l2_bf<T>(c))) return r;
+    hipLaunchKernelGGL(k2_step_advance, dim3(1), dim3(64), 0, c->stream, (int *)c->dit.p);
+    SWPC_CHECK_LAUNCH();
+    return SWPC_OK;
+}
+
+static int step2_any(swpc2d_ctx *c) { return fp64(c) ? step2_dev<double>(c) : step2_dev<float>(c); }
+
+static void graph_drop2(swpc2d_ctx *c)
+{
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+}
+
+static int graph_ready2(swpc2d_ctx *c)
+{
+    if (c->gexec) return SWPC_OK;
+    SWPC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int r = SWPC_OK;
+    for (int q = 0; q < c->graph_k && r == SWPC_OK; q++) r = step2_any(c);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (r != SWPC_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
     }
+    if (e != hipSuccess || !g) return fail(SWPC_EHIP, "step graph capture failed: %s", hipGetErrorString(e));
+    hipGraphExec_t ge = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(SWPC_EHIP, "step graph instantiation failed: %s", hipGetErrorString(ei));
+    }
+    c->graph = g;
+    c->gexec = ge;
+    return SWPC_OK;
+}
+
+static int run_steps2(swpc2d_ctx *c, int it0, int n)
+{
+    if (n <= 0) return SWPC_OK;
+    if (c->nsrc > 0 && it0 + n - 1 > c->nt_src)
+        return fail(SWPC_EINVAL, "stressglut: steps %d..%d outside 1..%d", it0, it0 + n - 1, c->nt_src);
+    if (c->nbf > 0 && it0 + n - 1 > c->nt_bf)
+        return fail(SWPC_EINVAL, "bodyforce: steps %d..%d outside 1..%d", it0, it0 + n - 1, c->nt_bf);
+    int r;
+    if ((r = set_step2(c, it0))) return r;
+    int done = 0;
+    if (c->graph_k > 0 && n >= c->graph_k) {
+        if ((r = graph_ready2(c))) return r;
+        for (; done + c->graph_k <= n; done += c->graph_k) SWPC_HIP(hipGraphLaunch(c->gexec, c->stream));
+    }
+    for (; done < n; done++)
+        if ((r = step2_any(c))) return r;
+    note_snapshots2(c, it0, it0 + n - 1);
     return SWPC_OK;
 }
 
@@ -627,6 +713,7 @@ int swpc2d_create(const swpc2d_desc *desc, swpc2d_ctx **out)
     if ((r = dalloc(c->surf, (size_t)d.nx * 16, c->bytes))) return r;
     if ((r = dalloc(c->kob, (size_t)d.nx * 4, c->bytes))) return r;
     if ((r = dalloc(c->vmax, 64, c->bytes))) return r;
+    if ((r = dalloc(c->dit, 64, c->bytes))) return r;
     const int ilo = d.na + 1, ihi = d.nx - d.na;
     c->ti0 = 1 + ((ilo - 1) / 64) * 64;
     c->ntx = ihi >= ilo ? (ihi - c->ti0) / 64 + 1 : 0;
@@ -660,8 +747,9 @@ int swpc2d_destroy(swpc2d_ctx *c)
     for (auto &b : c->R) dfree(b);
     for (auto &b : c->A) dfree(b);
     for (auto &b : c->med) dfree(b);
+    graph_drop2(c);
     DBuf *rest[] = {&c->surf, &c->kob, &c->gx, &c->gz, &c->src, &c->m3, &c->sdrop, &c->bsrc, &c->f2,
-                    &c->fdt, &c->st, &c->uacc, &c->wvel, &c->wdisp, &c->vmax};
+                    &c->fdt, &c->st, &c->uacc, &c->wvel, &c->wdisp, &c->vmax, &c->dit};
     for (auto b : rest) dfree(*b);
     for (auto &sn : c->snaps) {
         dfree(sn.pts);
@@ -677,6 +765,7 @@ int swpc2d_set_medium(swpc2d_ctx *c, const float *rho, const float *lam, const f
                       const float *taus)
 {
     clear_error();
+    if (c) graph_drop2(c);  // the captured steps would launch stale arguments
     if (!c || !rho || !lam || !mu || !taup || !taus) return fail(SWPC_EINVAL, "null argument");
     const float *in[5] = {rho, lam, mu, taup, taus};
     SWPC_HIP(hipStreamSynchronize(c->stream));
@@ -693,6 +782,7 @@ int swpc2d_set_surface(swpc2d_ctx *c, const int *kfs_top, const int *kfs_bot, co
                        const int *kob)
 {
     clear_error();
+    if (c) graph_drop2(c);  // the captured steps would launch stale arguments
     if (!c || !kfs_top || !kfs_bot || !kob_top || !kob_bot || !kob) return fail(SWPC_EINVAL, "null argument");
     const int nx = c->d.nx;
     std::vector<int> s4((size_t)nx * 4), kb(nx);
@@ -715,6 +805,7 @@ int swpc2d_set_surface(swpc2d_ctx *c, const int *kfs_top, const int *kfs_bot, co
 int swpc2d_set_pml(swpc2d_ctx *c, const float *gxc, const float *gxe, const float *gzc, const float *gze)
 {
     clear_error();
+    if (c) graph_drop2(c);  // the captured steps would launch stale arguments
     if (!c || !gxc || !gxe || !gzc || !gze) return fail(SWPC_EINVAL, "null argument");
     SWPC_HIP(hipStreamSynchronize(c->stream));
     int r;
@@ -740,6 +831,7 @@ int swpc2d_set_sources(swpc2d_ctx *c, int nsrc, const int *isrc, const int *ksrc
                        const double *sdrop)
 {
     clear_error();
+    if (c) graph_drop2(c);  // the captured steps would launch stale arguments
     if (!c || nsrc < 0 || nt < 0) return fail(SWPC_EINVAL, "bad argument");
     if (nsrc > 0 && (!isrc || !ksrc || !m3 || !sdrop || nt < 1)) return fail(SWPC_EINVAL, "null source array");
     std::vector<int> pack;
@@ -763,6 +855,7 @@ int swpc2d_set_bodyforce(swpc2d_ctx *c, int nsrc, const int *isrc, const int *ks
                          const double *fdt)
 {
     clear_error();
+    if (c) graph_drop2(c);  // the captured steps would launch stale arguments
     if (!c || nsrc < 0 || nt < 0) return fail(SWPC_EINVAL, "bad argument");
     if (nsrc > 0 && (!isrc || !ksrc || !f2 || !fdt || nt < 1)) return fail(SWPC_EINVAL, "null source array");
     std::vector<int> pack;
@@ -785,6 +878,7 @@ int swpc2d_set_bodyforce(swpc2d_ctx *c, int nsrc, const int *isrc, const int *ks
 int swpc2d_set_receivers(swpc2d_ctx *c, int nst, const int *ist, const int *kst, int ntdec_w, int ntw)
 {
     clear_error();
+    if (c) graph_drop2(c);  // the captured steps would launch stale arguments
     if (!c || nst < 0 || ntdec_w < 1 || ntw < 0) return fail(SWPC_EINVAL, "bad argument");
     if (nst > 0 && (!ist || !kst)) return fail(SWPC_EINVAL, "null station array");
     std::vector<int> pack((size_t)2 * nst);
@@ -825,14 +919,18 @@ int swpc2d_stressglut(swpc2d_ctx *c, int it)
 {
     clear_error();
     if (int r = ready2(c)) return r;
-    return fp64(c) ? l2_glut<double>(c, it) : l2_glut<float>(c, it);
+    if (c->nsrc > 0 && (it < 1 || it > c->nt_src)) return fail(SWPC_EINVAL, "stressglut: it=%d outside 1..%d", it, c->nt_src);
+    if (int r = set_step2(c, it)) return r;
+    return fp64(c) ? l2_glut<double>(c) : l2_glut<float>(c);
 }
 
 int swpc2d_bodyforce(swpc2d_ctx *c, int it)
 {
     clear_error();
     if (int r = ready2(c)) return r;
-    return fp64(c) ? l2_bf<double>(c, it) : l2_bf<float>(c, it);
+    if (c->nbf > 0 && (it < 1 || it > c->nt_bf)) return fail(SWPC_EINVAL, "bodyforce: it=%d outside 1..%d", it, c->nt_bf);
+    if (int r = set_step2(c, it)) return r;
+    return fp64(c) ? l2_bf<double>(c) : l2_bf<float>(c);
 }
 
 int swpc2d_store_receivers(swpc2d_ctx *c, int it)
@@ -840,7 +938,8 @@ int swpc2d_store_receivers(swpc2d_ctx *c, int it)
     clear_error();
     if (int r = ready2(c)) return r;
     if (it < 1) return fail(SWPC_EINVAL, "it must be >= 1");
-    return fp64(c) ? l2_recv<double>(c, it) : l2_recv<float>(c, it);
+    if (int r = set_step2(c, it)) return r;
+    return fp64(c) ? l2_recv<double>(c) : l2_recv<float>(c);
 }
 
 int swpc2d_add_snapshot(swpc2d_ctx *c, int kind, int npts, const int *ik, int ntdec_s, int nbuf, float uc, float m0,
@@ -873,6 +972,7 @@ int swpc2d_add_snapshot(swpc2d_ctx *c, int kind, int npts, const int *ik, int nt
     SWPC_HIP(hipMemcpy(sn.pts.p, p2.data(), p2.size() * 4, hipMemcpyHostToDevice));
     c->snaps.push_back(sn);
     *id = (int)c->snaps.size() - 1;
+    graph_drop2(c);
     return SWPC_OK;
 }
 
@@ -881,7 +981,10 @@ int swpc2d_snapshots(swpc2d_ctx *c, int it)
     clear_error();
     if (int r = ready2(c)) return r;
     if (it < 1) return fail(SWPC_EINVAL, "it must be >= 1");
-    return fp64(c) ? l2_snap<double>(c, it) : l2_snap<float>(c, it);
+    if (int r = set_step2(c, it)) return r;
+    if (int r = fp64(c) ? l2_snap<double>(c) : l2_snap<float>(c)) return r;
+    note_snapshots2(c, it, it);
+    return SWPC_OK;
 }
 
 int swpc2d_get_snapshot(swpc2d_ctx *c, int id, int frame, float *out)
@@ -904,7 +1007,7 @@ int swpc2d_step(swpc2d_ctx *c, int it)
     clear_error();
     if (int r = ready2(c)) return r;
     if (it < 1) return fail(SWPC_EINVAL, "it must be >= 1");
-    return step2(c, it);
+    return run_steps2(c, it, 1);
 }
 
 int swpc2d_run(swpc2d_ctx *c, int it0, int nsteps)
@@ -912,8 +1015,16 @@ int swpc2d_run(swpc2d_ctx *c, int it0, int nsteps)
     clear_error();
     if (int r = ready2(c)) return r;
     if (it0 < 1 || nsteps < 0) return fail(SWPC_EINVAL, "bad step range");
-    for (int it = it0; it < it0 + nsteps; it++)
-        if (int r = step2(c, it)) return r;
+    return run_steps2(c, it0, nsteps);
+}
+
+int swpc2d_set_graph(swpc2d_ctx *c, int steps)
+{
+    clear_error();
+    if (!c) return fail(SWPC_EINVAL, "null context");
+    if (steps < 0 || steps > 1024) return fail(SWPC_EINVAL, "steps per graph must be in 0..1024");
+    graph_drop2(c);
+    c->graph_k = steps;
     return SWPC_OK;
 }
 
@@ -951,6 +1062,7 @@ int swpc2d_get_field(swpc2d_ctx *c, int f, double *out)
 int swpc2d_set_field(swpc2d_ctx *c, int f, const double *in)
 {
     clear_error();
+    if (c) graph_drop2(c);  // the captured steps would launch stale arguments
     if (!c || !in) return fail(SWPC_EINVAL, "null argument");
     DBuf *b = field_buf2(c, f);
     if (!b) return fail(SWPC_EINVAL, "unknown field %d", f);

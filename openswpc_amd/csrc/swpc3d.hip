@@ -1544,9 +1544,12 @@ __global__ void __launch_bounds__(256) k3x_vel_pml(const K3<float> a, const Boxe
 // (m_source.f90:796 `loop seq`): several sources may share a cell.
 template <typename T>
 __global__ void k3_stressglut(const K3<T> a, int nsrc, const int *__restrict__ src, const double *__restrict__ m6,
-                              const double *__restrict__ sdrop)
+                              const double *__restrict__ sdrop_all, int nt, const int *__restrict__ dit)
 {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int it = *dit;
+    if (it < 1 || it > nt) return;
+    const double *__restrict__ sdrop = sdrop_all + (size_t)(it - 1) * nsrc;
     const int px = a.px, sxy = a.sxy;
     for (int n = 0; n < nsrc; n++) {
         const int ii = src[3 * n], jj = src[3 * n + 1], kk = src[3 * n + 2];
@@ -1574,9 +1577,12 @@ __global__ void k3_stressglut(const K3<T> a, int nsrc, const int *__restrict__ s
 
 template <typename T, bool S16>
 __global__ void k3_bodyforce(const K3<T> a, int nsrc, const int *__restrict__ src, const double *__restrict__ f3,
-                             const double *__restrict__ fdt)
+                             const double *__restrict__ fdt_all, int nt, const int *__restrict__ dit)
 {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int it = *dit;
+    if (it < 1 || it > nt) return;
+    const double *__restrict__ fdt = fdt_all + (size_t)(it - 1) * nsrc;
     const int px = a.px, sxy = a.sxy;
     for (int n = 0; n < nsrc; n++) {
         const int ii = src[3 * n], jj = src[3 * n + 1], kk = src[3 * n + 2];
@@ -1593,13 +1599,21 @@ __global__ void k3_bodyforce(const K3<T> a, int nsrc, const int *__restrict__ sr
 }
 
 // wav__store (m_wav.f90:436-447, 524-535): disp accumulates every step in
-// FP64; when store != 0 the sample itw is written.
+// FP64; at it with mod(it-1, ntdec_w) = 0 the sample itw = (it-1)/ntdec_w
+// is written (while itw < ntw).  The step number comes from the device
+// counter dit, so a captured step replays unchanged.
 template <typename T>
 __global__ void k3_receivers(const K3<T> a, int nst, const int *__restrict__ st, double *__restrict__ uacc,
-                             float *__restrict__ vel_out, double *__restrict__ disp_out, int store)
+                             float *__restrict__ wvel, double *__restrict__ wdisp, int ntdec_w, int ntw,
+                             const int *__restrict__ dit)
 {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= nst) return;
+    const int it = *dit;
+    const int itw = (it - 1) / ntdec_w;
+    const int store = ((it - 1) % ntdec_w) == 0 && itw < ntw;
+    float *__restrict__ vel_out = wvel + (size_t)(store ? itw : 0) * nst * 3;
+    double *__restrict__ disp_out = wdisp + (size_t)(store ? itw : 0) * nst * 3;
     const int px = a.px, sxy = a.sxy;
     const int c = at3(px, sxy, st[3 * n], st[3 * n + 1], st[3 * n + 2]);
     const T vx = a.V[0][c] + a.V[0][c - 1];
@@ -1626,6 +1640,20 @@ __global__ void k_fill_u32(unsigned *p, int n, unsigned v)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = v;
+}
+
+// Device-side step number of the time loop: set once per run, advanced by
+// one at the end of every step, read by the kernels whose work depends on
+// it (sources, receivers, snapshots).  A captured step graph therefore
+// replays unchanged for any it.
+__global__ void k_step_set(int *dit, int it)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *dit = it;
+}
+
+__global__ void k_step_advance(int *dit)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *dit = *dit + 1;
 }
 
 template <typename T>
@@ -1664,11 +1692,18 @@ __global__ void k3_vmax(const K3<T> a, const int *__restrict__ kob, unsigned *__
 // Evaluated in the reference's order without contraction.
 template <typename T, bool S16>
 __global__ void k3_snap(const K3<T> a, int kind, int npts, const int4 *__restrict__ pts, float uc, float m0, float dt,
-                        float *__restrict__ acc, float *__restrict__ out, int store)
+                        float *__restrict__ acc, float *__restrict__ frames, int ntdec, int nbuf,
+                        const int *__restrict__ dit)
 {
 #pragma clang fp contract(off)
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npts) return;
+    // frame (it-1)/ntdec at mod(it-1, ntdec) = 0, kept in slot frame % nbuf
+    const int it = *dit;
+    const int store = ((it - 1) % ntdec) == 0;
+    if (!store && kind != 2) return;
+    const int nvar = kind == 0 ? 4 : 3;
+    float *__restrict__ out = frames + (size_t)(((it - 1) / ntdec) % nbuf) * nvar * npts;
     const int4 q = pts[p];
     const int px = a.px, sxy = a.sxy;
     const int c = at3(px, sxy, q.x, q.y, q.z);
@@ -1750,6 +1785,10 @@ struct swpc3d_ctx {
     // interior kernels of the same half-step (they write disjoint cells and
     // read the same inputs); events fork and join the two streams.
     int conc = 0;
+    int graph_k = 16;       // steps per captured step graph (0 = launch step by step)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    swpc::DBuf dit;         // device step number (int)
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     swpc::Prof prof;
@@ -2149,45 +2188,39 @@ static int launch_vel(swpc3d_ctx *c)
 }
 
 template <typename T>
-static int launch_glut(swpc3d_ctx *c, int it)
+static int launch_glut(swpc3d_ctx *c)
 {
     if (c->nsrc == 0) return SWPC_OK;
-    if (it < 1 || it > c->nt_src) return fail(SWPC_EINVAL, "stressglut: it=%d outside 1..%d", it, c->nt_src);
     K3<T> a = make_args<T>(c);
     hipLaunchKernelGGL((k3_stressglut<T>), dim3(1), dim3(64), 0, c->stream, a, c->nsrc, (const int *)c->src.p,
-                       (const double *)c->m6.p, (const double *)c->sdrop.p + (size_t)(it - 1) * c->nsrc);
+                       (const double *)c->m6.p, (const double *)c->sdrop.p, c->nt_src, (const int *)c->dit.p);
     SWPC_CHECK_LAUNCH();
     return SWPC_OK;
 }
 
 template <typename T>
-static int launch_bf(swpc3d_ctx *c, int it)
+static int launch_bf(swpc3d_ctx *c)
 {
     if (c->nbf == 0) return SWPC_OK;
-    if (it < 1 || it > c->nt_bf) return fail(SWPC_EINVAL, "bodyforce: it=%d outside 1..%d", it, c->nt_bf);
     K3<T> a = make_args<T>(c);
     if (c->s16)
         hipLaunchKernelGGL((k3_bodyforce<T, true>), dim3(1), dim3(64), 0, c->stream, a, c->nbf, (const int *)c->bsrc.p,
-                           (const double *)c->f3.p, (const double *)c->fdt.p + (size_t)(it - 1) * c->nbf);
+                           (const double *)c->f3.p, (const double *)c->fdt.p, c->nt_bf, (const int *)c->dit.p);
     else
         hipLaunchKernelGGL((k3_bodyforce<T, false>), dim3(1), dim3(64), 0, c->stream, a, c->nbf, (const int *)c->bsrc.p,
-                           (const double *)c->f3.p, (const double *)c->fdt.p + (size_t)(it - 1) * c->nbf);
+                           (const double *)c->f3.p, (const double *)c->fdt.p, c->nt_bf, (const int *)c->dit.p);
     SWPC_CHECK_LAUNCH();
     return SWPC_OK;
 }
 
 template <typename T>
-static int launch_recv(swpc3d_ctx *c, int it)
+static int launch_recv(swpc3d_ctx *c)
 {
     if (c->nst == 0) return SWPC_OK;
-    int store = ((it - 1) % c->ntdec_w) == 0;
-    int itw = (it - 1) / c->ntdec_w;
-    if (store && itw >= c->ntw) store = 0;
     K3<T> a = make_args<T>(c);
-    size_t off = (size_t)(store ? itw : 0) * c->nst * 3;
     hipLaunchKernelGGL((k3_receivers<T>), dim3((c->nst + 63) / 64), dim3(64), 0, c->stream, a, c->nst,
-                       (const int *)c->st.p, (double *)c->uacc.p, (float *)c->wvel.p + off,
-                       (double *)c->wdisp.p + off, store);
+                       (const int *)c->st.p, (double *)c->uacc.p, (float *)c->wvel.p, (double *)c->wdisp.p,
+                       c->ntdec_w, c->ntw, (const int *)c->dit.p);
     SWPC_CHECK_LAUNCH();
     return SWPC_OK;
 }
@@ -2208,25 +2241,37 @@ static int launch_vmax(swpc3d_ctx *c)
 }
 
 template <typename T>
-static int launch_snap(swpc3d_ctx *c, int it)
+static int launch_snap(swpc3d_ctx *c)
 {
     if (c->snaps.empty()) return SWPC_OK;
     K3<T> a = make_args<T>(c);
     for (auto &sn : c->snaps) {
-        const bool due = ((it - 1) % sn.ntdec) == 0;
-        if (!due && sn.kind != 2) continue;
-        const long long frame = (it - 1) / sn.ntdec;
-        const int nvar = sn.kind == 0 ? 4 : 3;
-        float *out = (float *)sn.frames.p + (size_t)(frame % sn.nbuf) * nvar * sn.npts;
         if (c->s16)
             hipLaunchKernelGGL((k3_snap<T, true>), dim3((sn.npts + 255) / 256), dim3(256), 0, c->stream, a, sn.kind,
-                               sn.npts, (const int4 *)sn.pts.p, sn.uc, sn.m0, sn.dt, (float *)sn.acc.p, out, due ? 1 : 0);
+                               sn.npts, (const int4 *)sn.pts.p, sn.uc, sn.m0, sn.dt, (float *)sn.acc.p,
+                               (float *)sn.frames.p, sn.ntdec, sn.nbuf, (const int *)c->dit.p);
         else
             hipLaunchKernelGGL((k3_snap<T, false>), dim3((sn.npts + 255) / 256), dim3(256), 0, c->stream, a, sn.kind,
-                               sn.npts, (const int4 *)sn.pts.p, sn.uc, sn.m0, sn.dt, (float *)sn.acc.p, out, due ? 1 : 0);
+                               sn.npts, (const int4 *)sn.pts.p, sn.uc, sn.m0, sn.dt, (float *)sn.acc.p,
+                               (float *)sn.frames.p, sn.ntdec, sn.nbuf, (const int *)c->dit.p);
         SWPC_CHECK_LAUNCH();
-        if (due) sn.last = std::max(sn.last, frame);
     }
+    return SWPC_OK;
+}
+
+// host record of the snapshot frames that steps it0 .. it1 produce
+static void note_snapshots(swpc3d_ctx *c, int it0, int it1)
+{
+    for (auto &sn : c->snaps) {
+        const long long last_due = (long long)(it1 - 1) / sn.ntdec * sn.ntdec + 1;  // largest due it <= it1
+        if (last_due >= it0) sn.last = std::max(sn.last, (last_due - 1) / sn.ntdec);
+    }
+}
+
+static int set_step(swpc3d_ctx *c, int it)
+{
+    hipLaunchKernelGGL(k_step_set, dim3(1), dim3(64), 0, c->stream, (int *)c->dit.p, it);
+    SWPC_CHECK_LAUNCH();
     return SWPC_OK;
 }
 
@@ -2240,24 +2285,80 @@ static int ready(const swpc3d_ctx *c)
     return SWPC_OK;
 }
 
-static int step(swpc3d_ctx *c, int it)
+// One time step at the device step number (main.f90:125-144 order), then
+// the counter advances.
+template <typename T>
+static int step_dev(swpc3d_ctx *c)
 {
     int r;
-    if (fp64(c)) {
-        if ((r = launch_recv<double>(c, it))) return r;
-        if ((r = launch_snap<double>(c, it))) return r;
-        if ((r = launch_stress<double>(c))) return r;
-        if ((r = launch_glut<double>(c, it))) return r;
-        if ((r = launch_vel<double>(c))) return r;
-        if ((r = launch_bf<double>(c, it))) return r;
-    } else {
-        if ((r = launch_recv<float>(c, it))) return r;
-        if ((r = launch_snap<float>(c, it))) return r;
-        if ((r = launch_stress<float>(c))) return r;
-        if ((r = launch_glut<float>(c, it))) return r;
-        if ((r = launch_vel<float>(c))) return r;
-        if ((r = launch_bf<float>(c, it))) return r;
+    if ((r = launch_recv<T>(c))) return r;
+    if ((r = launch_snap<T>(c))) return r;
+    if ((r = launch_stress<T>(c))) return r;
+    if ((r = launch_glut<T>(c))) return r;
+    if ((r = launch_vel<T>(c))) return r;
+    if ((r = launch_bf<T>(c))) return r;
+    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, c->stream, (int *)c->dit.p);
+    SWPC_CHECK_LAUNCH();
+    return SWPC_OK;
+}
+
+static int step_dev_any(swpc3d_ctx *c) { return fp64(c) ? step_dev<double>(c) : step_dev<float>(c); }
+
+static void graph_drop(swpc3d_ctx *c)
+{
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+}
+
+// Capture graph_k steps (kernels and counter advances) once; the graph is
+// dropped whenever a setter changes what the steps launch.
+static int graph_ready(swpc3d_ctx *c)
+{
+    if (c->gexec) return SWPC_OK;
+    SWPC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int r = SWPC_OK;
+    for (int q = 0; q < c->graph_k && r == SWPC_OK; q++) r = step_dev_any(c);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (r != SWPC_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
     }
+    if (e != hipSuccess || !g) return fail(SWPC_EHIP, "step graph capture failed: %s", hipGetErrorString(e));
+    hipGraphExec_t ge = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(SWPC_EHIP, "step graph instantiation failed: %s", hipGetErrorString(ei));
+    }
+    c->graph = g;
+    c->gexec = ge;
+    return SWPC_OK;
+}
+
+// Steps it0 .. it0+n-1: replays of the captured graph where they fit, one
+// step at a time otherwise (and always while profiling, whose events are
+// per launch, or with the PML on a second stream).
+static int run_steps(swpc3d_ctx *c, int it0, int n)
+{
+    if (n <= 0) return SWPC_OK;
+    if (c->nsrc > 0 && it0 + n - 1 > c->nt_src)
+        return fail(SWPC_EINVAL, "stressglut: steps %d..%d outside 1..%d", it0, it0 + n - 1, c->nt_src);
+    if (c->nbf > 0 && it0 + n - 1 > c->nt_bf)
+        return fail(SWPC_EINVAL, "bodyforce: steps %d..%d outside 1..%d", it0, it0 + n - 1, c->nt_bf);
+    int r;
+    if ((r = set_step(c, it0))) return r;
+    int done = 0;
+    const bool graphs = c->graph_k > 0 && !c->prof.on && !c->conc;
+    if (graphs && n >= c->graph_k) {
+        if ((r = graph_ready(c))) return r;
+        for (; done + c->graph_k <= n; done += c->graph_k) SWPC_HIP(hipGraphLaunch(c->gexec, c->stream));
+    }
+    for (; done < n; done++)
+        if ((r = step_dev_any(c))) return r;
+    note_snapshots(c, it0, it0 + n - 1);
     return SWPC_OK;
 }
 
@@ -2371,6 +2472,7 @@ int swpc3d_create(const swpc3d_desc *desc, swpc3d_ctx **out)
     if ((r = dalloc(c->surf, (size_t)d.nx * d.ny * 16, c->bytes))) return r;
     if ((r = dalloc(c->kob, (size_t)d.nx * d.ny * 4, c->bytes))) return r;
     if ((r = dalloc(c->vmax, 64, c->bytes))) return r;
+    if ((r = dalloc(c->dit, 64, c->bytes))) return r;
 
     // interior tiles: 64 wide, aligned to i = 1 + 64 t (128-byte rows)
     c->ti0 = 1 + (d.na / 64) * 64;
@@ -2433,6 +2535,8 @@ int swpc3d_destroy(swpc3d_ctx *c)
         dfree(sn.frames);
     }
     c->prof.reset();
+    graph_drop(c);
+    dfree(c->dit);
     if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
@@ -2447,6 +2551,7 @@ int swpc3d_set_medium(swpc3d_ctx *c, const float *bx, const float *by, const flo
                       const float *muxy)
 {
     clear_error();
+    if (c) graph_drop(c);  // the captured steps would launch stale arguments
     if (!c) return fail(SWPC_EINVAL, "null context");
     const float *in[10] = {bx, by, bz, lam, mu, taup, taus, muyz, muxz, muxy};
     for (int q = 0; q < 10; q++)
@@ -2493,6 +2598,7 @@ int swpc3d_set_medium(swpc3d_ctx *c, const float *bx, const float *by, const flo
 int swpc3d_set_density(swpc3d_ctx *c, const float *rho)
 {
     clear_error();
+    if (c) graph_drop(c);  // the captured steps would launch stale arguments
     if (!c || !rho) return fail(SWPC_EINVAL, "null argument");
     if (c->m16) return SWPC_OK;  // the 16-bit medium kernels stream their own buoyancy arrays
     SWPC_HIP(hipStreamSynchronize(c->stream));
@@ -2507,6 +2613,7 @@ int swpc3d_set_surface(swpc3d_ctx *c, const int *kfs_top, const int *kfs_bot, co
                        const int *kob)
 {
     clear_error();
+    if (c) graph_drop(c);  // the captured steps would launch stale arguments
     if (!c || !kfs_top || !kfs_bot || !kob_top || !kob_bot || !kob) return fail(SWPC_EINVAL, "null argument");
     const int nx = c->d.nx, ny = c->d.ny, nib = nx + 6;
     std::vector<int> s4((size_t)nx * ny * 4), kb((size_t)nx * ny);
@@ -2532,6 +2639,7 @@ int swpc3d_set_pml(swpc3d_ctx *c, const float *gxc, const float *gxe, const floa
                    const float *gzc, const float *gze)
 {
     clear_error();
+    if (c) graph_drop(c);  // the captured steps would launch stale arguments
     if (!c || !gxc || !gxe || !gyc || !gye || !gzc || !gze) return fail(SWPC_EINVAL, "null argument");
     SWPC_HIP(hipStreamSynchronize(c->stream));
     int r;
@@ -2575,6 +2683,7 @@ int swpc3d_set_sources(swpc3d_ctx *c, int nsrc, const int *isrc, const int *jsrc
                        int nt, const double *sdrop)
 {
     clear_error();
+    if (c) graph_drop(c);  // the captured steps would launch stale arguments
     if (!c || nsrc < 0 || nt < 0) return fail(SWPC_EINVAL, "bad argument");
     if (nsrc > 0 && (!isrc || !jsrc || !ksrc || !m6 || !sdrop || nt < 1)) return fail(SWPC_EINVAL, "null source array");
     std::vector<int> pack;
@@ -2613,6 +2722,7 @@ int swpc3d_set_bodyforce(swpc3d_ctx *c, int nsrc, const int *isrc, const int *js
                          int nt, const double *fdt)
 {
     clear_error();
+    if (c) graph_drop(c);  // the captured steps would launch stale arguments
     if (!c || nsrc < 0 || nt < 0) return fail(SWPC_EINVAL, "bad argument");
     if (nsrc > 0 && (!isrc || !jsrc || !ksrc || !f3 || !fdt || nt < 1)) return fail(SWPC_EINVAL, "null source array");
     std::vector<int> pack;
@@ -2635,6 +2745,7 @@ int swpc3d_set_bodyforce(swpc3d_ctx *c, int nsrc, const int *isrc, const int *js
 int swpc3d_set_receivers(swpc3d_ctx *c, int nst, const int *ist, const int *jst, const int *kst, int ntdec_w, int ntw)
 {
     clear_error();
+    if (c) graph_drop(c);  // the captured steps would launch stale arguments
     if (!c || nst < 0 || ntdec_w < 1 || ntw < 0) return fail(SWPC_EINVAL, "bad argument");
     if (nst > 0 && (!ist || !jst || !kst)) return fail(SWPC_EINVAL, "null station array");
     std::vector<int> pack((size_t)3 * nst);
@@ -2676,14 +2787,18 @@ int swpc3d_stressglut(swpc3d_ctx *c, int it)
 {
     clear_error();
     if (int r = ready(c)) return r;
-    return fp64(c) ? launch_glut<double>(c, it) : launch_glut<float>(c, it);
+    if (c->nsrc > 0 && (it < 1 || it > c->nt_src)) return fail(SWPC_EINVAL, "stressglut: it=%d outside 1..%d", it, c->nt_src);
+    if (int r = set_step(c, it)) return r;
+    return fp64(c) ? launch_glut<double>(c) : launch_glut<float>(c);
 }
 
 int swpc3d_bodyforce(swpc3d_ctx *c, int it)
 {
     clear_error();
     if (int r = ready(c)) return r;
-    return fp64(c) ? launch_bf<double>(c, it) : launch_bf<float>(c, it);
+    if (c->nbf > 0 && (it < 1 || it > c->nt_bf)) return fail(SWPC_EINVAL, "bodyforce: it=%d outside 1..%d", it, c->nt_bf);
+    if (int r = set_step(c, it)) return r;
+    return fp64(c) ? launch_bf<double>(c) : launch_bf<float>(c);
 }
 
 int swpc3d_store_receivers(swpc3d_ctx *c, int it)
@@ -2691,7 +2806,8 @@ int swpc3d_store_receivers(swpc3d_ctx *c, int it)
     clear_error();
     if (int r = ready(c)) return r;
     if (it < 1) return fail(SWPC_EINVAL, "it must be >= 1");
-    return fp64(c) ? launch_recv<double>(c, it) : launch_recv<float>(c, it);
+    if (int r = set_step(c, it)) return r;
+    return fp64(c) ? launch_recv<double>(c) : launch_recv<float>(c);
 }
 
 int swpc3d_add_snapshot(swpc3d_ctx *c, int kind, int npts, const int *ijk, int ntdec_s, int nbuf, float uc, float m0,
@@ -2725,6 +2841,7 @@ int swpc3d_add_snapshot(swpc3d_ctx *c, int kind, int npts, const int *ijk, int n
     SWPC_HIP(hipMemcpy(sn.pts.p, p4.data(), p4.size() * 4, hipMemcpyHostToDevice));
     c->snaps.push_back(sn);
     *id = (int)c->snaps.size() - 1;
+    graph_drop(c);
     return SWPC_OK;
 }
 
@@ -2733,7 +2850,10 @@ int swpc3d_snapshots(swpc3d_ctx *c, int it)
     clear_error();
     if (int r = ready(c)) return r;
     if (it < 1) return fail(SWPC_EINVAL, "it must be >= 1");
-    return fp64(c) ? launch_snap<double>(c, it) : launch_snap<float>(c, it);
+    if (int r = set_step(c, it)) return r;
+    if (int r = fp64(c) ? launch_snap<double>(c) : launch_snap<float>(c)) return r;
+    note_snapshots(c, it, it);
+    return SWPC_OK;
 }
 
 int swpc3d_get_snapshot(swpc3d_ctx *c, int id, int frame, float *out)
@@ -2757,7 +2877,7 @@ int swpc3d_step(swpc3d_ctx *c, int it)
     clear_error();
     if (int r = ready(c)) return r;
     if (it < 1) return fail(SWPC_EINVAL, "it must be >= 1");
-    return step(c, it);
+    return run_steps(c, it, 1);
 }
 
 int swpc3d_run(swpc3d_ctx *c, int it0, int nsteps)
@@ -2765,9 +2885,7 @@ int swpc3d_run(swpc3d_ctx *c, int it0, int nsteps)
     clear_error();
     if (int r = ready(c)) return r;
     if (it0 < 1 || nsteps < 0) return fail(SWPC_EINVAL, "bad step range");
-    for (int it = it0; it < it0 + nsteps; it++)
-        if (int r = step(c, it)) return r;
-    return SWPC_OK;
+    return run_steps(c, it0, nsteps);
 }
 
 int swpc3d_get_receivers(swpc3d_ctx *c, float *vel, double *disp)
@@ -2820,6 +2938,7 @@ int swpc3d_get_field(swpc3d_ctx *c, int f, double *out)
 int swpc3d_set_field(swpc3d_ctx *c, int f, const double *in)
 {
     clear_error();
+    if (c) graph_drop(c);  // the captured steps would launch stale arguments
     if (!c || !in) return fail(SWPC_EINVAL, "null argument");
     DBuf *b = field_buf(c, f);
     if (!b) return fail(SWPC_EINVAL, "unknown field %d", f);
@@ -2899,10 +3018,12 @@ int swpc3d_profile_read(swpc3d_ctx *c, double *ms5, long long *count5)
 int swpc3d_set_tuning(swpc3d_ctx *c, const int *p, int n)
 {
     clear_error();
+    if (c) graph_drop(c);  // the captured steps would launch stale arguments
     if (!c || (n > 0 && !p)) return fail(SWPC_EINVAL, "null argument");
-    int v[16] = {c->s_by, c->s_wpe, c->s_split, c->v_by, c->v_wpe, c->xcd, c->s_kc, c->v_kc, c->x4, c->x4_by, c->pml4,
-                 c->x4_form, c->x4_vby, c->conc, c->x4_vz, c->x4_sz};
-    for (int q = 0; q < n && q < 16; q++) v[q] = p[q];
+    int v[17] = {c->s_by, c->s_wpe, c->s_split, c->v_by, c->v_wpe, c->xcd, c->s_kc, c->v_kc, c->x4, c->x4_by, c->pml4,
+                 c->x4_form, c->x4_vby, c->conc, c->x4_vz, c->x4_sz, c->graph_k};
+    for (int q = 0; q < n && q < 17; q++) v[q] = p[q];
+    if (v[16] < 0 || v[16] > 1024) return fail(SWPC_EINVAL, "steps per graph must be in 0..1024");
     if (v[14] < 0 || v[15] < 0) return fail(SWPC_EINVAL, "negative z-chunk");
     if (v[6] < 0 || v[7] < 0) return fail(SWPC_EINVAL, "negative z-chunk");
     if (v[9] != 1 && v[9] != 2 && v[9] != 4) return fail(SWPC_EINVAL, "4-wide tile rows must be 1, 2 or 4");
@@ -2912,6 +3033,7 @@ int swpc3d_set_tuning(swpc3d_ctx *c, const int *p, int n)
     c->s_by = v[0]; c->s_wpe = v[1]; c->s_split = v[2] != 0; c->v_by = v[3]; c->v_wpe = v[4];
     c->xcd = v[5] != 0; c->s_kc = v[6]; c->v_kc = v[7]; c->x4 = v[8] != 0; c->x4_by = v[9]; c->pml4 = v[10] != 0;
     c->x4_form = v[11]; c->x4_vby = v[12]; c->conc = v[13] != 0; c->x4_vz = v[14]; c->x4_sz = v[15];
+    c->graph_k = v[16];
     return SWPC_OK;
 }
 

@@ -376,6 +376,10 @@ class SwpcPSV:
         assert a.shape == self.g.shape
         check(lib().swpc2d_set_field(self.h, FIELDS2.index(name), ptr(a, C.c_double)), "set_field")
 
+    def set_graph(self, steps):
+        """Steps per captured HIP graph in run() (0: step by step)."""
+        check(lib().swpc2d_set_graph(self.h, int(steps)), "set_graph")
+
     def close(self):
         if getattr(self, "h", None):
             lib().swpc2d_destroy(self.h)
