@@ -37,6 +37,7 @@ module swpc_hip
     public :: swpc3d_set_sources, swpc3d_set_bodyforce, swpc3d_set_receivers
     public :: swpc3d_update_stress, swpc3d_update_vel, swpc3d_stressglut, swpc3d_bodyforce
     public :: swpc3d_store_receivers, swpc3d_add_snapshot, swpc3d_snapshots, swpc3d_get_snapshot
+    public :: swpc3d_snapshot_peaks, swpc3d_finish_snapshots, swpc3d_get_peaks
     public :: swpc3d_step, swpc3d_run, swpc3d_get_receivers, swpc3d_vmax, swpc3d_sync
     public :: swpc2d_create, swpc2d_destroy, swpc2d_set_medium, swpc2d_set_surface, swpc2d_set_pml
     public :: swpc2d_set_sources, swpc2d_set_receivers, swpc2d_update_stress, swpc2d_update_vel
@@ -179,6 +180,27 @@ module swpc_hip
             import :: c_int, c_ptr, c_float
             type(c_ptr), value :: ctx
             integer(c_int), value :: id, frame
+            real(c_float), intent(out) :: out(*)
+        end function
+
+        !! max-V/H/A of a 'v' or 'u' stream (output__put_maxval, m_snap.f90:2291-2344)
+        integer(c_int) function swpc3d_snapshot_peaks(ctx, id) bind(C, name='swpc3d_snapshot_peaks')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: id
+        end function
+
+        integer(c_int) function swpc3d_finish_snapshots(ctx, it) bind(C, name='swpc3d_finish_snapshots')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it
+        end function
+
+        !! out(npts, 3)
+        integer(c_int) function swpc3d_get_peaks(ctx, id, out) bind(C, name='swpc3d_get_peaks')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: id
             real(c_float), intent(out) :: out(*)
         end function
 

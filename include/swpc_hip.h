@@ -174,6 +174,19 @@ int swpc3d_snapshots(swpc3d_ctx *ctx, int it);
    (synchronises); fails if the frame is not yet written or has been
    overwritten in the ring. */
 int swpc3d_get_snapshot(swpc3d_ctx *ctx, int id, int frame, float *out);
+/* Keep the running maximum amplitudes of a 'v' or 'u' stream from the
+   next step on: the max_fs_v / max_ob_v / max_fs_u / max_ob_u arrays of
+   wbuf_fs_v and its siblings (m_snap.f90:1690-1703), written to the
+   netCDF file as max-V, max-H, max-A by output__put_maxval
+   (m_snap.f90:2291-2344). */
+int swpc3d_snapshot_peaks(swpc3d_ctx *ctx, int id);
+/* The closing update of snap__closefiles in netCDF mode
+   (m_snap.f90:2215-2231, wbuf_*(nt+1)): advances the 'u' running sums
+   and the maxima at step number it, stores no frame. */
+int swpc3d_finish_snapshots(swpc3d_ctx *ctx, int it);
+/* Copy the 3 x npts maxima (max-V, max-H, max-A, variable-major) of
+   stream id to out (synchronises). */
+int swpc3d_get_peaks(swpc3d_ctx *ctx, int id, float *out);
 /* One time step it in the order of main.f90:125-144: receivers,
    snapshots, stress (interior + PML), stress glut, velocity (interior +
    PML), body force.  The halo exchange is absent: ghost cells stay in

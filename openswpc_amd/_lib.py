@@ -56,6 +56,9 @@ _PROTOS = {
     "swpc3d_add_snapshot": (I, [P, I, I, PI_, I, I, C.c_float, C.c_float, C.c_float, PI_]),
     "swpc3d_snapshots": (I, [P, I]),
     "swpc3d_get_snapshot": (I, [P, I, I, PF]),
+    "swpc3d_snapshot_peaks": (I, [P, I]),
+    "swpc3d_finish_snapshots": (I, [P, I]),
+    "swpc3d_get_peaks": (I, [P, I, PF]),
     "swpc3d_step": (I, [P, I]),
     "swpc3d_run": (I, [P, I, I]),
     "swpc3d_get_receivers": (I, [P, PF, PD]),

@@ -1689,21 +1689,37 @@ __global__ void k3_vmax(const K3<T> a, const int *__restrict__ kob, unsigned *__
 // UC*M0*1e-3 (m_snap.f90:1007-1027); kind 1 ('v'): V*UC*M0 (1469-1472);
 // kind 2 ('u'): running sum of V*UC*M0*dt, advanced at every call
 // (2046-2048).  Output planes are variable-major: out[n*npts + p].
+// With peak != nullptr a 'v' or 'u' stream also keeps, at every call, the
+// running maxima of wbuf_fs_v/wbuf_ob_v/wbuf_*_u (m_snap.f90:1690-1703):
+// peak[p] = max |Z|, peak[npts+p] = max sqrt(X^2+Y^2) and
+// peak[2*npts+p] = sqrt of the sum of their squares.  A closing call
+// (snap__closefiles' wbuf_*(nt+1), m_snap.f90:2215-2231) advances the
+// running sums and maxima but stores no frame.
 // Evaluated in the reference's order without contraction.
+__device__ __forceinline__ void snap_peak(float *__restrict__ peak, int p, int npts, float x, float y, float z)
+{
+#pragma clang fp contract(off)
+    const float pv = fmaxf(peak[p], fabsf(z));
+    const float ph = fmaxf(peak[npts + p], sqrtf(x * x + y * y));
+    peak[p] = pv;
+    peak[npts + p] = ph;
+    peak[2 * npts + p] = sqrtf(pv * pv + ph * ph);
+}
+
 template <typename T, bool S16>
 __global__ void k3_snap(const K3<T> a, int kind, int npts, const int4 *__restrict__ pts, float uc, float m0, float dt,
-                        float *__restrict__ acc, float *__restrict__ frames, int ntdec, int nbuf,
-                        const int *__restrict__ dit)
+                        float *__restrict__ acc, float *__restrict__ frames, float *__restrict__ peak, int ntdec,
+                        int nbuf, int closing, const int *__restrict__ dit)
 {
 #pragma clang fp contract(off)
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npts) return;
     // frame (it-1)/ntdec at mod(it-1, ntdec) = 0, kept in slot frame % nbuf
     const int it = *dit;
-    const int store = ((it - 1) % ntdec) == 0;
-    if (!store && kind != 2) return;
+    const int store = !closing && ((it - 1) % ntdec) == 0;
+    if (!store && kind != 2 && !(peak && kind == 1)) return;
     const int nvar = kind == 0 ? 4 : 3;
-    float *__restrict__ out = frames + (size_t)(((it - 1) / ntdec) % nbuf) * nvar * npts;
+    float *__restrict__ out = frames + (size_t)(store ? ((it - 1) / ntdec) % nbuf : 0) * nvar * npts;
     const int4 q = pts[p];
     const int px = a.px, sxy = a.sxy;
     const int c = at3(px, sxy, q.x, q.y, q.z);
@@ -1726,17 +1742,25 @@ __global__ void k3_snap(const K3<T> a, int kind, int npts, const int4 *__restric
         out[2 * npts + p] = r2 * uc * m0 * 1e-3f;
         out[3 * npts + p] = r3 * uc * m0 * 1e-3f;
     } else if (kind == 1) {
-        out[p] = (float)(vx[c] * (T)uc * (T)m0);
-        out[npts + p] = (float)(vy[c] * (T)uc * (T)m0);
-        out[2 * npts + p] = (float)(vz[c] * (T)uc * (T)m0);
+        const float bx = (float)(vx[c] * (T)uc * (T)m0);
+        const float by = (float)(vy[c] * (T)uc * (T)m0);
+        const float bz = (float)(vz[c] * (T)uc * (T)m0);
+        if (store) {
+            out[p] = bx;
+            out[npts + p] = by;
+            out[2 * npts + p] = bz;
+        }
+        if (peak) snap_peak(peak, p, npts, bx, by, bz);
     } else {
         const T *v[3] = {vx, vy, vz};
+        float u[3];
 #pragma unroll
         for (int n = 0; n < 3; n++) {
-            const float u = (float)((T)acc[n * npts + p] + v[n][c] * (T)uc * (T)m0 * (T)dt);
-            acc[n * npts + p] = u;
-            if (store) out[n * npts + p] = u;
+            u[n] = (float)((T)acc[n * npts + p] + v[n][c] * (T)uc * (T)m0 * (T)dt);
+            acc[n * npts + p] = u[n];
+            if (store) out[n * npts + p] = u[n];
         }
+        if (peak) snap_peak(peak, p, npts, u[0], u[1], u[2]);
     }
 }
 
@@ -1795,7 +1819,7 @@ struct swpc3d_ctx {
     struct Snap {
         int kind, npts, ntdec, nbuf;
         float uc, m0, dt;
-        swpc::DBuf pts, acc, frames;
+        swpc::DBuf pts, acc, frames, peak;
         long long last = -1;  // latest frame index written
     };
     std::vector<Snap> snaps;
@@ -2241,19 +2265,22 @@ static int launch_vmax(swpc3d_ctx *c)
 }
 
 template <typename T>
-static int launch_snap(swpc3d_ctx *c)
+static int launch_snap(swpc3d_ctx *c, int closing = 0)
 {
     if (c->snaps.empty()) return SWPC_OK;
     K3<T> a = make_args<T>(c);
     for (auto &sn : c->snaps) {
+        if (closing && sn.kind == SWPC_SNAP_PS) continue;
         if (c->s16)
             hipLaunchKernelGGL((k3_snap<T, true>), dim3((sn.npts + 255) / 256), dim3(256), 0, c->stream, a, sn.kind,
                                sn.npts, (const int4 *)sn.pts.p, sn.uc, sn.m0, sn.dt, (float *)sn.acc.p,
-                               (float *)sn.frames.p, sn.ntdec, sn.nbuf, (const int *)c->dit.p);
+                               (float *)sn.frames.p, (float *)sn.peak.p, sn.ntdec, sn.nbuf, closing,
+                               (const int *)c->dit.p);
         else
             hipLaunchKernelGGL((k3_snap<T, false>), dim3((sn.npts + 255) / 256), dim3(256), 0, c->stream, a, sn.kind,
                                sn.npts, (const int4 *)sn.pts.p, sn.uc, sn.m0, sn.dt, (float *)sn.acc.p,
-                               (float *)sn.frames.p, sn.ntdec, sn.nbuf, (const int *)c->dit.p);
+                               (float *)sn.frames.p, (float *)sn.peak.p, sn.ntdec, sn.nbuf, closing,
+                               (const int *)c->dit.p);
         SWPC_CHECK_LAUNCH();
     }
     return SWPC_OK;
@@ -2533,6 +2560,7 @@ int swpc3d_destroy(swpc3d_ctx *c)
         dfree(sn.pts);
         dfree(sn.acc);
         dfree(sn.frames);
+        dfree(sn.peak);
     }
     c->prof.reset();
     graph_drop(c);
@@ -2853,6 +2881,42 @@ int swpc3d_snapshots(swpc3d_ctx *c, int it)
     if (int r = set_step(c, it)) return r;
     if (int r = fp64(c) ? launch_snap<double>(c) : launch_snap<float>(c)) return r;
     note_snapshots(c, it, it);
+    return SWPC_OK;
+}
+
+int swpc3d_snapshot_peaks(swpc3d_ctx *c, int id)
+{
+    clear_error();
+    if (!c) return fail(SWPC_EINVAL, "null context");
+    if (id < 0 || id >= (int)c->snaps.size()) return fail(SWPC_EINVAL, "unknown snapshot id %d", id);
+    auto &sn = c->snaps[id];
+    if (sn.kind == SWPC_SNAP_PS)
+        return fail(SWPC_EINVAL, "peak amplitudes are kept for velocity and displacement snapshots only");
+    if (sn.peak.p) return SWPC_OK;
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    if (int r = dalloc(sn.peak, (size_t)3 * sn.npts * 4, c->bytes)) return r;
+    graph_drop(c);
+    return SWPC_OK;
+}
+
+int swpc3d_finish_snapshots(swpc3d_ctx *c, int it)
+{
+    clear_error();
+    if (int r = ready(c)) return r;
+    if (it < 1) return fail(SWPC_EINVAL, "it must be >= 1");
+    if (int r = set_step(c, it)) return r;
+    return fp64(c) ? launch_snap<double>(c, 1) : launch_snap<float>(c, 1);
+}
+
+int swpc3d_get_peaks(swpc3d_ctx *c, int id, float *out)
+{
+    clear_error();
+    if (!c || !out) return fail(SWPC_EINVAL, "null argument");
+    if (id < 0 || id >= (int)c->snaps.size()) return fail(SWPC_EINVAL, "unknown snapshot id %d", id);
+    const auto &sn = c->snaps[id];
+    if (!sn.peak.p) return fail(SWPC_ESTATE, "snapshot %d keeps no peak amplitudes (swpc3d_snapshot_peaks)", id);
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    SWPC_HIP(hipMemcpy(out, sn.peak.p, (size_t)3 * sn.npts * 4, hipMemcpyDeviceToHost));
     return SWPC_OK;
 }
 

@@ -99,6 +99,15 @@ def ikupper(t, ts, tr):
     return f32(1.0)
 
 
+def moment_magnitude(m0):
+    """m_fdtool.f90:281-293: Mw = (log10 M0 - 9.1) * 2/3 in real(SP), -12345
+    for M0 below epsilon."""
+    m0 = _f(m0)
+    if m0 < _f(np.finfo(np.float32).eps):
+        return _f(-12345.0)
+    return _f(_f(_f(np.log10(m0)) - _f(9.1)) * _f(2.0) / _f(3.0))
+
+
 def seismic_moment(mw):
     """m_fdtool.f90:296-304: M0 = 10^(1.5 Mw + 9.05)."""
     return _f(10.0 ** float(_f(1.5) * _f(mw) + _f(9.05)))

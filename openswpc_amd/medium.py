@@ -157,10 +157,12 @@ def _vcut(layers, vcut):
     return L
 
 
-def _seawater_vel(z):
-    # m_seawater.f90:34-47 with the default (non-Munk) epsilon
+def _seawater_vel(z, munk=False):
+    # seawater__vel (src/shared/m_seawater.f90:29-47): 1.5 km/s, or the Munk
+    # profile with epsilon 0.00737 when munk_profile is set
+    eps = 0.00737 if munk else 0.0
     zb = 2 * (float(z) * 1000.0 - 1300.0) / 1300.0
-    return F32(1.5 * (1.0 + 0.0 * (zb - 1.0 + math.exp(-zb))))
+    return F32(1.5 * (1.0 + eps * (zb - 1.0 + math.exp(-zb))))
 
 
 def _layer_index(zc, depths):
@@ -171,7 +173,7 @@ def _layer_index(zc, depths):
     return idx
 
 
-def vmodel_lhm(zc, layers, vcut=0.0, xi=None, vmax_clip=None, rhomin=1.0):
+def vmodel_lhm(zc, layers, vcut=0.0, xi=None, vmax_clip=None, rhomin=1.0, munk=False):
     """1-D layered model over the k axis of the memory box.
 
     Returns per-k arrays (rho, lam, mu, qp, qs) when ``xi`` is None; with a
@@ -225,13 +227,91 @@ def vmodel_lhm(zc, layers, vcut=0.0, xi=None, vmax_clip=None, rhomin=1.0):
     qs = np.where(air, F32(10.0), qs).astype(F32)
     if np.any(ocean):
         zz = np.broadcast_to(zc, ocean.shape)
-        vw = np.vectorize(_seawater_vel, otypes=[F32])(zz)
+        vw = np.vectorize(lambda z: _seawater_vel(z, munk), otypes=[F32])(zz)
         rho = np.where(ocean, F32(1.0), rho).astype(F32)
         mu = np.where(ocean, F32(0.0), mu).astype(F32)
         lam = np.where(ocean, vw * vw, lam).astype(F32)
         qp = np.where(ocean, F32(1e6), qp).astype(F32)
         qs = np.where(ocean, F32(1e6), qs).astype(F32)
     return rho, lam, mu, qp, qs
+
+
+def vmodel_uni(zc, shape, vp0=5.0, vs0=None, rho0=2.7, qp0=1e6, qs0=1e6, topo0=0.0, xi=None, vmin=0.0,
+               vmax_clip=None, rhomin=1.0, munk=False):
+    """vmodel_uni / vmodel_uni_rmed (src/swpc_3d/m_vmodel_uni.f90:22-146,
+    m_vmodel_uni_rmed.f90:22-175): a half-space below z = topo0 (strictly
+    deeper cells), ocean between 0 and topo0, air above; with a random field
+    xi the solid takes rho(1+0.8 xi), v(1+xi) and the vcheck clamp.  vs0
+    defaults to vp0/sqrt(3) in real(SP).  Returns full arrays of ``shape``
+    (the k axis last)."""
+    vp0 = F32(vp0)
+    vs0 = F32(vp0 / F32(math.sqrt(3.0))) if vs0 is None else F32(vs0)
+    rho0, qp0, qs0, topo0 = F32(rho0), F32(qp0), F32(qs0), F32(topo0)
+    z = np.broadcast_to(np.asarray(zc, F32), shape)
+    solid = z > topo0
+    ocean = ~solid & (z > 0)
+    if xi is None:
+        rho_s = np.full(shape, rho0, F32)
+        vp_s = np.full(shape, vp0, F32)
+        vs_s = np.full(shape, vs0, F32)
+    else:
+        xi = np.asarray(xi, F32)
+        rho_s = ((F32(1.0) + F32(0.8) * xi) * rho0).astype(F32)
+        vp_s = ((F32(1.0) + xi) * vp0).astype(F32)
+        vs_s = ((F32(1.0) + xi) * vs0).astype(F32)
+        vp_s, vs_s, rho_s, _ = fdtool.vcheck(vp_s, vs_s, rho_s, xi, F32(vmin),
+                                             F32(vmax_clip if vmax_clip is not None else 1e30), rhomin)
+    rho = np.where(solid, rho_s, np.where(ocean, F32(1.0), F32(0.001))).astype(F32)
+    mu = np.where(solid, rho_s * vs_s * vs_s, F32(0.0)).astype(F32)
+    vw = np.vectorize(lambda zz: _seawater_vel(zz, munk), otypes=[F32])(np.asarray(zc, F32))
+    vw = np.broadcast_to(vw, shape)
+    lam = np.where(solid, rho_s * (vp_s * vp_s - F32(2) * vs_s * vs_s),
+                   np.where(ocean, F32(1.0) * (vw * vw), F32(0.0))).astype(F32)
+    qp = np.where(solid, qp0, np.where(ocean, F32(1e6), F32(10.0))).astype(F32)
+    qs = np.where(solid, qs0, np.where(ocean, F32(1e6), F32(10.0))).astype(F32)
+    return rho, lam, mu, qp, qs
+
+
+def read_lhm_rmed(path):
+    """Layer file of vmodel_lhm_rmed (m_vmodel_lhm_rmed.f90:101-110):
+    depth rho vp vs qp qs rmed-file per line -> (layers, file names)."""
+    layers, files = [], []
+    with open(path) as f:
+        for line in f:
+            t = line.strip()
+            if not t or t.startswith("#"):
+                continue
+            w = t.split()
+            layers.append(Layer(*[float(x) for x in w[:6]]))
+            files.append(w[6] if len(w) > 6 else "")
+    return layers, files
+
+
+def layer_xi(zc, layers, vcut, fields):
+    """The random field each cell of vmodel_lhm_rmed perturbs with: the
+    field of the deepest layer whose top lies at or above the cell
+    (m_vmodel_lhm_rmed.f90:176-196).  ``fields[l]`` is a full array or None
+    (no file: zero)."""
+    L = _vcut(layers, vcut)
+    li = _layer_index(np.asarray(zc, F32), [l.depth for l in L])
+    shape = next(f.shape for f in fields if f is not None)
+    out = np.zeros(shape, F32)
+    for l, f in enumerate(fields):
+        if f is not None:
+            sel = np.broadcast_to(li == l, shape)
+            out = np.where(sel, f, out)
+    return out
+
+
+def rmed_tile2(vol, i0, i1, k0, k1):
+    """Periodic tiling of a 2-D random medium vol[nxc, nzc] onto
+    [i0,i1] x [k0,k1] (src/shared/m_rdrmed.f90:24-70); returns (ni, nk)."""
+    nxc, nzc = vol.shape
+    ii = np.arange(i0, i1 + 1) % nxc
+    ii = np.where(ii <= 0, ii + nxc, ii) - 1
+    kk = np.arange(k0, k1 + 1) % nzc
+    kk = np.where(kk <= 0, kk + nzc, kk) - 1
+    return np.ascontiguousarray(vol[ii][:, kk])
 
 
 def rmed_tile(vol, i0, i1, j0, j1, k0, k1):
@@ -466,26 +546,34 @@ def medium_setup2(g: Grid2D, rho, lam, mu, qp, qs, fq_min=0.05, fq_max=5.0, fq_r
 # ---------------------------------------------------------------------------
 # model builders
 # ---------------------------------------------------------------------------
-def layered3(g: Grid3D, layers, vcut=0.0, xi=None, rhomin=1.0):
+def vmax_clip3(g: Grid3D):
+    """vmax of the random-media clamp: 6/7 dh / dt with dh evaluated in
+    double precision and stored in real(SP) (m_vmodel_lhm_rmed.f90:92-94)."""
+    dh = F32(1.0 / math.sqrt(1.0 / g.dx ** 2 + 1.0 / g.dy ** 2 + 1.0 / g.dz ** 2))
+    return F32(F32(6.0) / F32(7.0)) * dh / F32(g.dt)
+
+
+def vmax_clip2(g: Grid2D):
+    dh = F32(1.0 / math.sqrt(1.0 / g.dx ** 2 + 1.0 / g.dz ** 2))
+    return F32(F32(6.0) / F32(7.0)) * dh / F32(g.dt)
+
+
+def layered3(g: Grid3D, layers, vcut=0.0, xi=None, rhomin=1.0, munk=False):
     """rho, lam, mu, qp, qs over the memory box for a 1-D layered model,
     optionally perturbed by a random field xi of the box's shape."""
     zc = g.zc
     if xi is None:
-        cols = vmodel_lhm(zc, layers, vcut)
+        cols = vmodel_lhm(zc, layers, vcut, munk=munk)
         return tuple(np.ascontiguousarray(np.broadcast_to(c, g.shape)).astype(F32) for c in cols)
-    dh = F32(1.0) / F32(math.sqrt(1.0 / g.dx ** 2 + 1.0 / g.dy ** 2 + 1.0 / g.dz ** 2))
-    vmax = F32(6.0 / 7.0) * dh / F32(g.dt)
-    return vmodel_lhm(zc, layers, vcut, xi=xi, vmax_clip=vmax, rhomin=rhomin)
+    return vmodel_lhm(zc, layers, vcut, xi=xi, vmax_clip=vmax_clip3(g), rhomin=rhomin, munk=munk)
 
 
-def layered2(g: Grid2D, layers, vcut=0.0, xi=None, rhomin=1.0):
+def layered2(g: Grid2D, layers, vcut=0.0, xi=None, rhomin=1.0, munk=False):
     zc = g.zc
     if xi is None:
-        cols = vmodel_lhm(zc, layers, vcut)
+        cols = vmodel_lhm(zc, layers, vcut, munk=munk)
         return tuple(np.ascontiguousarray(np.broadcast_to(c, g.shape)).astype(F32) for c in cols)
-    dh = F32(1.0) / F32(math.sqrt(1.0 / g.dx ** 2 + 1.0 / g.dz ** 2))
-    vmax = F32(6.0 / 7.0) * dh / F32(g.dt)
-    return vmodel_lhm(zc, layers, vcut, xi=xi, vmax_clip=vmax, rhomin=rhomin)
+    return vmodel_lhm(zc, layers, vcut, xi=xi, vmax_clip=vmax_clip2(g), rhomin=rhomin, munk=munk)
 
 
 def uniform(shape, zc, vp=5.0, vs=None, rho=2.7, qp=1e6, qs=1e6, topo=0.0):

@@ -233,7 +233,7 @@ COMPONENTS_PSV = {"v": [("Vx", 90.0, 0.0), ("Vz", 0.0, 0.0)],
 
 def write_wav(odir, meta: WavMeta, stations, vel=None, disp=None, dim="3d", stlonlat=None):
     """Export receiver traces as SAC files (``wav__write`` with
-    wav_format = 'sac').  ``vel``/``disp`` have shape (ntw, ncomp, nst) as
+    wav_format = 'sac').  ``vel``/``disp`` have shape (ntw, nst, ncomp) as
     returned by the solvers' ``receivers()``; ``stations`` are objects with
     ``x, y, z, name`` (y ignored for P-SV).  Returns the written paths."""
     comps = COMPONENTS_3D if dim == "3d" else COMPONENTS_PSV
@@ -260,6 +260,95 @@ def write_wav(odir, meta: WavMeta, stations, vel=None, disp=None, dim="3d", stlo
                 write_sac(p, h, arr[:, s, c])
                 out.append(p)
     return out
+
+
+def wav_traces(meta: WavMeta, stations, vel=None, disp=None, dim="3d", stlonlat=None):
+    """(header, samples) of every exported trace, in the reference's order:
+    velocity components of all stations, then displacement (``sh_vel(j,i)``
+    with the component fastest, m_wav.f90:680-700, 710-716)."""
+    comps = COMPONENTS_3D if dim == "3d" else COMPONENTS_PSV
+    out = []
+    for kind, arr, idep in (("v", vel, IDEP_VEL), ("u", disp, IDEP_DISP)):
+        if arr is None:
+            continue
+        arr = np.asarray(arr, F32)
+        if arr.shape[:2] != (meta.ntw, len(stations)):
+            raise ValueError(f"trace array shape {arr.shape} does not match ntw={meta.ntw}, nst={len(stations)}")
+        for s, st in enumerate(stations):
+            lo, la = (stlonlat[s] if stlonlat is not None else (UNDEF_F, UNDEF_F))
+            h0 = station_header(meta, st.name, st.x, getattr(st, "y", meta.sy0) if dim == "3d" else meta.sy0,
+                                st.z, lo, la, dim)
+            for c, (name, inc, az) in enumerate(comps[kind]):
+                h = h0.copy()
+                h.kcmpnm = name
+                h.cmpinc = inc
+                h.cmpaz = 0.0 if az is None else az + meta.phi
+                h.idep = idep
+                out.append((kind, st.name.strip(), name, h, arr[:, s, c]))
+    return out
+
+
+def write_csf(odir, meta: WavMeta, stations, vel=None, disp=None, dim="3d", stlonlat=None, rank=0):
+    """wav_format = 'csf' (export_wav__csf, m_wav.f90:798-811, and wcsf_s,
+    src/shared/m_sac.f90:585-647): one file per product,
+    ``<odir>/wav/<title>__<rank:05d>__.csf`` holding 'CSFD', int32 ntrace,
+    int32 npts, then header + samples per trace (component fastest).  Both
+    products name the same file; the reference then asks on standard input
+    whether to overwrite it.  Here the first product written (velocity when
+    it is on) is kept and a second one is refused with a warning, the
+    reference's answer 'n'."""
+    import warnings
+    wdir = os.path.join(odir, "wav")
+    os.makedirs(wdir, exist_ok=True)
+    path = os.path.join(wdir, f"{meta.title.strip()}__{rank:05d}__.csf")
+    done = False
+    for arr in (vel, disp):
+        if arr is None:
+            continue
+        if done:
+            warnings.warn(f"wcsf: file {path} exists; not overwritten")
+            continue
+        done = True
+        tr = wav_traces(meta, stations, arr if arr is vel else None, arr if arr is disp else None, dim, stlonlat)
+        with open(path, "wb") as fo:
+            fo.write(b"CSFD" + np.array([len(tr), meta.ntw], np.int32).tobytes())
+            for _, _, _, h, d in tr:
+                fo.write(h.pack())
+                fo.write(np.asarray(d, F32)[:meta.ntw].tobytes())
+    return path
+
+
+def write_tar(odir, meta: WavMeta, stations, vel=None, disp=None, dim="3d", stlonlat=None, per="station", rank=0):
+    """wav_format = 'tar_st' (one archive per station,
+    ``<title>.<dim>.<stnm>.sac.tar``) or 'tar_node' (one archive per rank,
+    ``<title>.<dim>.<rank:06d>.sac.tar``) of SAC members named
+    ``<title>.<stnm>.<dim>.<cmp>.sac`` (m_wav.f90:717-786).  The archives
+    are USTAR with mode 0644, owner 0 and mtime = exedate."""
+    import io
+    import tarfile
+    wdir = os.path.join(odir, "wav")
+    os.makedirs(wdir, exist_ok=True)
+    tr = wav_traces(meta, stations, vel, disp, dim, stlonlat)
+    title = meta.title.strip()
+    groups = {}
+    for kind, stnm, cmp, h, d in tr:
+        key = stnm if per == "station" else f"{rank:06d}"
+        groups.setdefault(key, []).append((stnm, cmp, h, d))
+    order = [st.name.strip() for st in stations] if per == "station" else [f"{rank:06d}"]
+    paths = []
+    for key in order:
+        if key not in groups:
+            continue
+        path = os.path.join(wdir, f"{title}.{dim}.{key}.sac.tar")
+        with tarfile.open(path, "w", format=tarfile.USTAR_FORMAT) as tf:
+            for stnm, cmp, h, d in groups[key]:
+                body = h.pack() + np.asarray(d, F32)[:int(h.npts)].tobytes()
+                ti = tarfile.TarInfo(f"{title}.{stnm}.{dim}.{cmp}.sac")
+                ti.size, ti.mode, ti.uid, ti.gid = len(body), 0o644, 0, 0
+                ti.mtime = int(meta.exedate or 0)
+                tf.addfile(ti, io.BytesIO(body))
+        paths.append(path)
+    return paths
 
 
 if sys.byteorder != "little":  # pragma: no cover

@@ -134,7 +134,8 @@ class Swpc3D:
                                          ptr(_f64(bodyforce.m), C.c_double), tab.shape[0], ptr(tab, C.c_double)),
                   "set_bodyforce")
             self.M0 = bodyforce.M0
-            self.UC = UC3 * 1e3   # bf_mode unit conversion (m_source.f90:190-195)
+            # bf_mode unit conversion UC = UC * 10**3 in real(SP) (m_source.f90:190-195)
+            self.UC = float(np.float32(np.float32(UC3) * np.float32(1000)))
         self.nst = 0
         self.ntw = 0
         self.ntdec_w = ntdec_w
@@ -207,6 +208,19 @@ class Swpc3D:
     def get_snapshot(self, sid, frame, nvar, npts):
         out = np.zeros((nvar, npts), np.float32)
         check(lib().swpc3d_get_snapshot(self.h, int(sid), int(frame), ptr(out, C.c_float)), "get_snapshot")
+        return out
+
+    def snapshot_peaks(self, sid):
+        """Keep max-V/H/A of a 'v' or 'u' stream from the next step on."""
+        check(lib().swpc3d_snapshot_peaks(self.h, int(sid)), "snapshot_peaks")
+
+    def finish_snapshots(self, it):
+        """Closing update at step number it (snap__closefiles in netCDF mode)."""
+        check(lib().swpc3d_finish_snapshots(self.h, int(it)), "finish_snapshots")
+
+    def get_peaks(self, sid, npts):
+        out = np.zeros((3, npts), np.float32)
+        check(lib().swpc3d_get_peaks(self.h, int(sid), ptr(out, C.c_float)), "get_peaks")
         return out
 
     def field(self, name):
@@ -296,7 +310,8 @@ class SwpcPSV:
                                          ptr(_f64(bodyforce.m), C.c_double), tab.shape[0], ptr(tab, C.c_double)),
                   "set_bodyforce")
             self.M0 = bodyforce.M0
-            self.UC = UC2 * 1e3   # bf_mode unit conversion (src/swpc_psv/m_source.f90:157-160)
+            # bf_mode unit conversion in real(SP) (src/swpc_psv/m_source.f90:157-160)
+            self.UC = float(np.float32(np.float32(UC2) * np.float32(1000)))
         self.nst = 0
         self.ntw = 0
         if stations is not None and len(stations):

@@ -38,32 +38,109 @@ class BodyForce:
     f3: tuple          # (fx, fy, fz)
 
 
-def read_source_file(path, fmt="xym0ij", phi=0.0):
-    """Parse a source file in the xym0ij / xym0dc / xymwij / xymwdc formats
-    (m_source.f90:511-560).  Mechanisms given as strike/dip/rake use the
-    strike measured from the grid azimuth phi."""
-    out = []
+def _records(path):
+    """Non-comment, non-blank lines split list-directed (blanks or commas)."""
     with open(path) as f:
         for line in f:
-            s = line.strip()
-            if not s or s.startswith("#"):
+            t = line.strip()
+            if not t or t.startswith("#"):
                 continue
-            w = [float(v) for v in s.replace(",", " ").split()]
-            x, y, z, t0, tr = w[:5]
-            if fmt == "xym0ij":
-                mo, m6 = w[5], tuple(F32(v) for v in w[6:12])
-            elif fmt == "xym0dc":
-                mo = w[5]
-                m6 = _dc(*w[6:9], phi)
-            elif fmt == "xymwij":
-                mo, m6 = float(fdtool.seismic_moment(_mw(w[5]))), tuple(F32(v) for v in w[6:12])
-            elif fmt == "xymwdc":
-                mo = float(fdtool.seismic_moment(_mw(w[5])))
-                m6 = _dc(*w[6:9], phi)
+            yield t.replace(",", " ").split()
+
+
+def _lonlat(lon, lat):
+    if not (-360 <= lon <= 360 and -90 <= lat <= 90):
+        raise ValueError(f"longitude/latitude ({lon}, {lat}) out of range")
+
+
+def read_source_file(path, fmt="xym0ij", phi=0.0, clon=0.0, clat=0.0, mu_at=None, with_info=False):
+    """Parse a moment-tensor source file (source__grid_moment,
+    m_source.f90:467-699) in any of its formats: xym0ij, xym0dc, llm0ij,
+    llm0dc, xymwij, xymwdc, llmwij, llmwdc, xydsdc, lldsdc, psmeca.
+
+    Strike/dip/rake mechanisms use the strike measured from the grid
+    azimuth phi; 'll' forms are mapped to the model frame about
+    (clon, clat) with phi; the 'ds' forms take the moment as
+    1e9 mu D S with mu the rigidity at the source cell, which ``mu_at(x, y, z)``
+    supplies.  With with_info=True also returns the hypocentre record of
+    the first source (evlo, evla, evdp, sx0, sy0, m6, otim)."""
+    from . import geomap
+    out = []
+    info = None
+    for w in _records(path):
+        v = [float(x) for x in w[:12]] if fmt != "psmeca" else None
+        if fmt in ("xym0ij", "xym0dc", "xymwij", "xymwdc", "xydsdc"):
+            x, y = F32(v[0]), F32(v[1])
+        elif fmt in ("llm0ij", "llm0dc", "llmwij", "llmwdc", "lldsdc"):
+            _lonlat(v[0], v[1])
+            x, y = geomap.g2c(v[0], v[1], clon, clat, phi)
+        elif fmt == "psmeca":
+            pass
+        else:
+            raise ValueError(f"unsupported stf_format {fmt!r}")
+        if fmt == "psmeca":
+            lon, lat, z = float(w[0]), float(w[1]), float(w[2])
+            mzz, mxx, myy, mxz, myz, mxy = (F32(float(t)) for t in w[3:9])
+            iex = int(w[9])
+            myz, mxy = -myz, -mxy
+            x, y = geomap.g2c(lon, lat, clon, clat, phi)
+            # norm in dyn cm, then N m; rise time 2 x the empirical half duration
+            s1 = mxx * mxx + myy * myy + mzz * mzz
+            s2 = mxz * mxz + myz * myz + mxy * mxy
+            m0tmp = float(np.sqrt(F32(s1 + F32(2) * s2)) / np.sqrt(F32(2.0)))
+            mo = F32(m0tmp * float(F32(10.0 ** iex)))
+            tr = F32(float(F32(F32(2) * F32(1.05)) * F32(1e-8)) * float(mo) ** (1.0 / 3.0))
+            mo = F32(mo * F32(1e-7))
+            m6 = tuple(F32(float(c) / m0tmp) for c in (mxx, myy, mzz, myz, mxz, mxy))
+            s0 = PointSource(float(x), float(y), z, 0.0, float(tr), float(mo), m6)
+        else:
+            z, t0, tr, a = v[2], v[3], v[4], v[5]
+            if fmt.endswith("ij"):
+                m6 = tuple(F32(c) for c in v[6:12])
             else:
-                raise ValueError(f"unsupported stf_format {fmt!r}")
-            out.append(PointSource(x, y, z, t0, tr, float(F32(mo)), m6))
-    return out
+                m6 = _dc(*(v[7:10] if fmt.endswith("dsdc") else v[6:9]), phi)
+            if fmt[2:4] == "m0":
+                mo = F32(a)
+            elif fmt[2:4] == "mw":
+                mo = fdtool.seismic_moment(_mw(a))
+            else:   # D [m] and S [km^2] times the cell rigidity
+                if mu_at is None:
+                    raise ValueError(f"stf_format {fmt} needs the medium (mu_at)")
+                D, S = v[5], v[6]
+                mo = F32(float(F32(F32(1e9) * F32(mu_at(float(x), float(y), z)))) * D * S)
+            s0 = PointSource(float(x), float(y), z, t0, tr, float(mo), m6)
+        out.append(s0)
+        if info is None:
+            from . import geomap as gm
+            lo, la = gm.c2g(s0.x, s0.y, clon, clat, phi)
+            info = {"evlo": float(lo), "evla": float(la), "evdp": float(F32(s0.z)), "sx0": s0.x, "sy0": s0.y,
+                    "m6": tuple(float(c) for c in s0.m6), "otim": float(F32(s0.t0))}
+    return (out, info) if with_info else out
+
+
+def read_bodyforce_file(path, fmt="xym0ij", phi=0.0, clon=0.0, clat=0.0, with_info=False):
+    """Body-force source file (source__grid_bodyforce, m_source.f90:701-773):
+    x y z tstart trise fx fy fz, or lon lat in place of x y when the format
+    starts with 'll'."""
+    from . import geomap
+    out = []
+    info = None
+    for w in _records(path):
+        v = [float(t) for t in w[:8]]
+        if fmt[:2] == "xy":
+            x, y = F32(v[0]), F32(v[1])
+        elif fmt[:2] == "ll":
+            _lonlat(v[0], v[1])
+            x, y = geomap.g2c(v[0], v[1], clon, clat, phi)
+        else:
+            raise ValueError(f"unsupported body-force stf_format {fmt!r}")
+        b = BodyForce(float(x), float(y), v[2], v[3], v[4], (float(F32(v[5])), float(F32(v[6])), float(F32(v[7]))))
+        out.append(b)
+        if info is None:
+            lo, la = geomap.c2g(b.x, b.y, clon, clat, phi)
+            info = {"evlo": float(lo), "evla": float(la), "evdp": float(F32(b.z)), "sx0": b.x, "sy0": b.y,
+                    "f3": b.f3, "otim": float(F32(b.t0))}
+    return (out, info) if with_info else out
 
 
 def _mw(mw):
@@ -196,17 +273,28 @@ class Station:
     zsw: str = "dep"
 
 
-def read_station_file(path):
-    """Station list in the 'xy' format: x y z name zsw (m_wav.f90:176-180)."""
-    out = []
-    with open(path) as f:
-        for line in f:
-            s = line.strip()
-            if not s or s.startswith("#"):
-                continue
-            w = s.split()
-            out.append(Station(float(w[0]), float(w[1]), float(w[2]), w[3], w[4] if len(w) > 4 else "dep"))
-    return out
+def read_station_file(path, st_format="xy", clon=0.0, clat=0.0, phi=0.0, with_lonlat=False):
+    """Station list (set_stinfo, m_wav.f90:144-190): 'xy' lines hold
+    x y z name zsw, 'll' lines lon lat z name zsw; the other coordinate pair
+    follows from the model frame about (clon, clat) rotated by phi.  With
+    with_lonlat=True also returns the (lon, lat) of every station."""
+    from . import geomap
+    out, ll = [], []
+    for w in _records(path):
+        a, b, z = float(w[0]), float(w[1]), float(w[2])
+        name, zsw = w[3][:8], (w[4] if len(w) > 4 else "dep")[:3]
+        if st_format == "xy":
+            x, y = float(F32(a)), float(F32(b))
+            lo, la = geomap.c2g(x, y, clon, clat, phi)
+        elif st_format == "ll":
+            _lonlat(a, b)
+            lo, la = F32(a), F32(b)
+            x, y = (float(t) for t in geomap.g2c(a, b, clon, clat, phi))
+        else:
+            raise ValueError(f"unknown st_format: {st_format}")
+        out.append(Station(x, y, float(F32(z)), name, zsw))
+        ll.append((float(lo), float(la)))
+    return (out, ll) if with_lonlat else out
 
 
 def ntw_of(nt, ntdec_w):

@@ -185,18 +185,39 @@ class Oracle3D:
         if not hasattr(self, "snaps"):
             self.snaps = []
         self.snaps.append({"kind": kind, "ijk": ijk, "ntdec": ntdec_s, "uc": np.float32(uc), "m0": np.float32(m0),
-                           "acc": np.zeros(3 * n, np.float32), "nvar": nvar, "frames": []})
+                           "acc": np.zeros(3 * n, np.float32), "nvar": nvar, "frames": [], "peak": None})
         return len(self.snaps) - 1
 
-    def snapshots(self, it):
+    def snapshot_peaks(self, sid):
+        """Keep the running maxima of a 'v' or 'u' stream at every call:
+        max |Z|, max sqrt(X^2+Y^2) and the root of their squares, in SP
+        (wbuf_fs_v and siblings, src/swpc_3d/m_snap.f90:1690-1703)."""
+        sn = self.snaps[sid]
+        assert sn["kind"] in (1, 2)
+        sn["peak"] = np.zeros((3, len(sn["ijk"])), np.float32)
+
+    @staticmethod
+    def _peak(pk, b):
+        pk[0] = np.maximum(pk[0], np.abs(b[2]))
+        pk[1] = np.maximum(pk[1], np.sqrt(b[0] * b[0] + b[1] * b[1]))
+        pk[2] = np.sqrt(pk[0] * pk[0] + pk[1] * pk[1])
+
+    def snapshots(self, it, closing=False):
+        """snap__write(it); closing=True is the wbuf_*(nt+1) update of
+        snap__closefiles (m_snap.f90:2215-2231): sums and maxima advance,
+        no frame is kept."""
         for sn in getattr(self, "snaps", []):
-            due = (it - 1) % sn["ntdec"] == 0
-            if not due and sn["kind"] != 2:
+            due = (it - 1) % sn["ntdec"] == 0 and not closing
+            if not due and sn["kind"] != 2 and sn["peak"] is None:
+                continue
+            if closing and sn["kind"] == 0:
                 continue
             n = len(sn["ijk"])
             out = np.zeros(sn["nvar"] * n, np.float32)
             self.L.o3d_snapshot(self._ps(), sn["kind"], n, sn["ijk"].ctypes.data_as(_PI), C.c_float(sn["uc"]),
                                 C.c_float(sn["m0"]), sn["acc"].ctypes.data_as(_PF), out.ctypes.data_as(_PF))
+            if sn["peak"] is not None:
+                self._peak(sn["peak"], out.reshape(sn["nvar"], n))
             if due:
                 sn["frames"].append(out.reshape(sn["nvar"], n))
 

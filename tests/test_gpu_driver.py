@@ -1,0 +1,118 @@
+"""The input.inf-driven host on the GPU.
+
+* A small 3-D input run end to end: the SAC files it writes hold the
+  oracle's receiver traces of the same setup (times M0 UC 1e9), to the
+  FP32 bar (relative L2 <= 1e-5); the netCDF snapshot files exist with
+  their max-V/H/A fields.
+* The reference's own example (example/input.inf: 384^3, 1000 steps, an
+  explosion under three ocean-bottom stations, netCDF snapshots) run
+  unchanged: the per-50-step maxima of |V| on the surface layer that
+  report__progress prints agree with the numbers the reference printed
+  (example/example.out:17-36, three significant figures, from an earlier
+  version of the reference with mixed FP64/FP32 arithmetic).  All 60
+  printed values agree digit for digit; the bar allows one rounding flip
+  in the third digit (0.6 %).
+"""
+import os
+import re
+import shutil
+
+import numpy as np
+import pytest
+
+from openswpc_amd import driver, problem, sac, snapshot
+from openswpc_amd.inifile import InputFile
+from oracle.oracle import Oracle3D
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+EX = os.path.join(os.path.dirname(__file__), "golden", "example")
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_small_input_end_to_end(tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
+    (tmp_path / "src.dat").write_text("1.0 -0.5 4.0 0.1 0.6 1e15 30 70 -40\n")
+    (tmp_path / "st.xy").write_text("0.0 0.0 0.0 S1 fsb\n3.0 2.0 0.0 S2 fsb\n-4.0 1.0 2.0 S3 dep\n")
+    shutil.copy(os.path.join(EX, "lhm.dat"), tmp_path / "lhm.dat")
+    txt = """
+title = 'small'
+odir = './out'
+ntdec_r = 20
+nx = 64
+ny = 56
+nz = 48
+nt = 60
+dx = 0.25
+dy = 0.25
+dz = 0.25
+dt = 0.015
+zbeg = -1.0
+na = 8
+vmodel_type = 'lhm'
+fn_lhm = 'lhm.dat'
+vcut = 1.5
+stf_format = 'xym0dc'
+fn_stf = 'src.dat'
+sw_wav_v = .true.
+sw_wav_u = .true.
+ntdec_w = 2
+fn_stloc = 'st.xy'
+snp_format = 'netcdf'
+fs_v%sw = .true.
+xz_ps%sw = .true.
+ntdec_s = 10
+"""
+    print("driver: small 3-D input 64x56x48, 60 steps", flush=True)
+    lines = []
+    S = driver.run(InputFile(text=txt), log=lines.append, exedate=1700000000)
+    assert sum("it=" in ln for ln in lines) == 3
+    case = problem.Case3D(S.grid, S.medium, S.prof, S.glut, S.cells, S.stations, S.nt, S.ntdec_w, None)
+    ref = Oracle3D(case, 32)
+    ref.run(1, S.nt)
+    scale = F32(S.glut.M0 * 1e-15 * 1e9)
+    for s, st in enumerate(S.stations):
+        for c, cmp in enumerate(("Vx", "Vy", "Vz")):
+            h, d = sac.read_sac(f"out/wav/small.3d.{st.name}.{cmp}.sac")
+            assert h.npts == (S.nt - 1) // 2 + 1
+            assert rel(d, ref.vel[:, s, c] * scale) <= 1e-5, (st.name, cmp)
+        h, d = sac.read_sac(f"out/wav/small.3d.{st.name}.Uz.sac")
+        assert rel(d, ref.disp[:, s, 2] * float(scale)) <= 1e-5
+    g, v = snapshot.read_nc_snp("out/small.3d.fs.v.nc")
+    assert v["max-V"].shape == (64, 56) and v["max-V"].max() > 0
+    assert v["Vz"].shape[0] == (S.nt - 1) // 10 + 1
+
+
+def _progress(lines):
+    out = {}
+    for ln in lines:
+        m = re.search(r"it=(\d+),.*\(\s*([-\d.E+]+)\s+([-\d.E+]+)\s+([-\d.E+]+)\s*\)", ln)
+        if m:
+            out[int(m.group(1))] = np.array([float(m.group(k)) for k in (2, 3, 4)])
+    return out
+
+
+def test_reference_example_run(tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
+    (tmp_path / "example").mkdir()
+    for f in ("lhm.dat", "source.dat", "stloc.xy"):
+        shutil.copy(os.path.join(EX, f), tmp_path / "example" / f)
+    shutil.copy(os.path.join(EX, "input.inf"), tmp_path / "input.inf")
+    print("driver: reference example 384^3, 1000 steps", flush=True)
+    lines = []
+    S = driver.run("input.inf", log=lines.append)
+    got = _progress(lines)
+    want = _progress(open(os.path.join(EX, "example_progress.txt")).read().splitlines())
+    assert sorted(got) == sorted(want) == list(range(50, 1001, 50))
+    dev = max(float(np.max(np.abs(got[i] / want[i] - 1))) for i in want)
+    same = sum(int(np.sum(got[i] == want[i])) for i in want)
+    print(f"example: {same}/60 printed maxima equal to example.out, largest relative deviation {dev:.4f}",
+          flush=True)
+    assert dev <= 0.006
+    for p in ("out/swpc.3d.ob.v.nc", "out/swpc.3d.xz.ps.nc", "out/wav/swpc.3d.st01.Vz.sac"):
+        assert os.path.exists(p), p

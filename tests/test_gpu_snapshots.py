@@ -110,3 +110,41 @@ def test_psv_snapshot_frames(bits):
             for f in range(len(frames) - 4, len(frames)):
                 e = rel(sim.get_snapshot(sid, f, 2, len(ik)), frames[f])
                 assert e <= tol, (k, f, e)
+
+
+def test_3d_netcdf_snapshots_and_peaks(tmp_path):
+    """netCDF mode: frames, and the max-V/H/A fields kept on the GPU at
+    every step (plus the closing update), against the oracle."""
+    nt, ntdec = 30, 4
+    case = problem.case3d(48, 44, 40, 0.5, 0.04, nt, na=6, src_depth=4.0, nst_side=2, zbeg=-2.0)
+    g = case.grid
+    sg = snapshot.snap_grid3(g, 2, 2, 2)
+    print("3-D netCDF snapshots with peaks: 48x44x40", flush=True)
+    ref = Oracle3D(case, 32)
+    m0 = F32(case.glut.M0)
+    rid = {}
+    for plane, kind in (("fs", "v"), ("ob", "u"), ("xz", "ps")):
+        _, _, ijk = snapshot.cells3(sg, plane, case.medium.kfs, case.medium.kob)
+        rid[(plane, kind)] = ref.add_snapshot(snapshot.KIND[kind], ijk, ntdec, snapshot.UC3, m0)
+        if kind != "ps":
+            ref.snapshot_peaks(rid[(plane, kind)])
+    ref.run(1, nt)
+    ref.snapshots(nt + 1, closing=True)
+    with Swpc3D(g, case.medium, case.pml, case.glut, case.stations, 1, nt, precision="fp32") as sim:
+        ss = snapshot.SnapshotSet(sim, str(tmp_path), "n", ntdec_s=ntdec, nbuf=3, fmt="netcdf")
+        paths = {pk: ss.add3(sg, *pk, case.medium) for pk in rid}
+        ss.run(1, nt)
+        ss.close(nt)
+    for pk, path in paths.items():
+        _, v = snapshot.read_nc_snp(path)
+        names = snapshot.VNAMES3[pk[1]][0]
+        frames = ref.snaps[rid[pk]]["frames"]
+        for f, w in enumerate(frames):
+            r = (f * ntdec + 1) // ntdec
+            got = np.stack([v[nm][r].T.ravel() for nm in names])
+            assert rel(got, w) <= 1e-5, (pk, f)
+        if pk[1] != "ps":
+            pk_got = np.stack([v[nm].T.ravel() for nm in ("max-V", "max-H", "max-A")])
+            want = ref.snaps[rid[pk]]["peak"]
+            assert np.abs(want).max() > 0
+            assert rel(pk_got, want) <= 1e-5, pk
