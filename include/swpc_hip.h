@@ -241,6 +241,64 @@ int swpc3d_memory(swpc3d_ctx *ctx, long long *bytes);
 int swpc3d_set_tuning(swpc3d_ctx *ctx, const int *p, int n);
 
 /* ------------------------------------------------------------------ */
+/* Learned absorbing boundary (3-D)                                    */
+/* ------------------------------------------------------------------ */
+/* No counterpart in the reference: a thin absorbing boundary that takes
+   the place of the ADE-PML of absorb_p__update_stress/vel
+   (m_absorb_p.f90:126-311, 313-505) on a grid created with na = 0.  The
+   interior update then covers every cell, and before each half-step the
+   ghost cells beyond the x-, x+, y-, y+ and bottom faces are written by a
+   small MLP per face and half-step (ten nets, 2*face + phase, phase 0
+   before the stress update, 1 before the velocity update) from the six
+   outermost cells of each column and the net's own previous ghost values;
+   columns whose outermost cell is air (lam + 2 mu = 0) keep zero ghosts.
+   Each net has swpc_lab_nparam() FP32 parameters: W1[32][61] b1[32]
+   W2[32][32] b2[32] W3[5][32] b3[5] Ws[5][61]; the time step stores and
+   uses them rounded to BF16 (nearest even), with FP32 arithmetic. */
+int swpc_lab_nparam(void);
+/* Switch the learned boundary on with params[10][swpc_lab_nparam()], or
+   off with params = NULL.  Needs na = 0, SWPC_FP32 or SWPC_FP32_R16, and
+   set_medium + set_density first. */
+int swpc3d_set_lab(swpc3d_ctx *ctx, const float *params);
+
+/* Training (no counterpart in the reference).  A trainer holds up to cap
+   samples per net on the device.  Samples come from wide-PML runs: the
+   domain of nxs x nys x nzs cells at offset (oi, oj, 0) of a wider grid
+   stands for a grid with a learned boundary, and the wide run's true
+   fields in its ghost cells are the targets.  swpc_lab_begin_run starts
+   such a run (quota[10] more samples per net, NULL: up to capacity);
+   swpc3d_lab_gather must then be called at every step before
+   swpc3d_update_stress (phase 0) and before swpc3d_update_vel (phase 1).
+   A column enters with probability keep (hashed from seed and the column)
+   once its scale N reaches thr times the largest N seen so far in the
+   run.  A sample is 72 floats: features[61] | targets[5] | column, N,
+   rho*vp | pad. */
+typedef struct swpc_lab_trainer swpc_lab_trainer;
+int swpc_lab_trainer_create(int cap, swpc_lab_trainer **out);
+int swpc_lab_trainer_destroy(swpc_lab_trainer *tr);
+int swpc_lab_begin_run(swpc_lab_trainer *tr, swpc3d_ctx *wide, int oi, int oj, int nxs, int nys, int nzs,
+                       const int *quota);
+int swpc3d_lab_gather(swpc3d_ctx *wide, swpc_lab_trainer *tr, int phase, float keep, unsigned seed, float thr);
+/* counts[10]: samples held per net. */
+int swpc_lab_count(swpc_lab_trainer *tr, int *counts);
+/* Append n host samples to net `net`; copy samples first..first+n-1 out. */
+int swpc_lab_add(swpc_lab_trainer *tr, int net, int n, const float *rec);
+int swpc_lab_samples(swpc_lab_trainer *tr, int net, int first, int n, float *rec);
+/* Adam (0.9, 0.999, 1e-8) on net `net` for `steps` batches of `batch`
+   samples (a multiple of 128) drawn at random from the store, half of them
+   with the sign of every field feature and target flipped, and Gaussian
+   noise of standard deviation `noise` added to the non-zero
+   previous-ghost features (robustness of the closed loop); the learning
+   rate follows a cosine from lr to lr/100.  params (FP32) in and out;
+   loss[steps] (or NULL) receives each batch's mean square error. */
+int swpc_lab_train(swpc_lab_trainer *tr, int net, float *params, int steps, int batch, float lr, float noise,
+                   unsigned seed, float *loss);
+/* Mean square error over the first nout outputs of n host samples and its
+   gradient grad[swpc_lab_nparam()], computed by the training kernels. */
+int swpc_lab_grad(swpc_lab_trainer *tr, const float *params, int nout, int n, const float *rec, double *loss,
+                  float *grad);
+
+/* ------------------------------------------------------------------ */
 /* P-SV (swpc_psv)                                                     */
 /* ------------------------------------------------------------------ */
 

@@ -70,6 +70,17 @@ _PROTOS = {
     "swpc3d_profile_read": (I, [P, PD, PLL]),
     "swpc3d_memory": (I, [P, PLL]),
     "swpc3d_set_tuning": (I, [P, PI_, I]),
+    "swpc_lab_nparam": (I, []),
+    "swpc3d_set_lab": (I, [P, PF]),
+    "swpc_lab_trainer_create": (I, [I, C.POINTER(P)]),
+    "swpc_lab_trainer_destroy": (I, [P]),
+    "swpc_lab_begin_run": (I, [P, P, I, I, I, I, I, PI_]),
+    "swpc3d_lab_gather": (I, [P, P, I, C.c_float, C.c_uint, C.c_float]),
+    "swpc_lab_count": (I, [P, PI_]),
+    "swpc_lab_add": (I, [P, I, I, PF]),
+    "swpc_lab_samples": (I, [P, I, I, I, PF]),
+    "swpc_lab_train": (I, [P, I, PF, I, I, C.c_float, C.c_float, C.c_uint, PF]),
+    "swpc_lab_grad": (I, [P, PF, I, I, PF, PD, PF]),
     "swpc2d_create": (I, [C.POINTER(Desc2), C.POINTER(P)]),
     "swpc2d_destroy": (I, [P]),
     "swpc2d_set_medium": (I, [P] + [PF] * 5),

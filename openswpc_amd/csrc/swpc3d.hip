@@ -20,6 +20,7 @@
 // bottom slab) with one thread per cell.  There is no MFMA: the stencil
 // is a gather, not a contraction, and is bound by HBM bandwidth.
 #include "swpc_common.h"
+#include "swpc_lab.h"
 
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
@@ -1783,6 +1784,10 @@ struct swpc3d_ctx {
     swpc::DBuf med[10];     // bx by bz lam mu taup taus muyz muxz muxy
     swpc::DBuf mbs[10];     // reduced storage: per-plane (base, scale)
     swpc::DBuf rho;         // optional density (swpc3d_set_density)
+    bool lab = false;       // learned absorbing boundary on (swpc3d_set_lab)
+    swpc::DBuf lab_w;       // its ten nets in BF16
+    swpc::DBuf lab_colmed;  // per-column medium features
+    swpc::LabDomain lab_dom;
     bool s16 = false;       // memory variables and ADE arrays in 16 bit (SWPC_FP32_S16, SWPC_FP32_R16)
     bool m16 = false;       // medium in 16 bit too (SWPC_FP32_S16)
     float r_scale = 1.0f;   // memory-variable scale of the reduced storage
@@ -2106,9 +2111,42 @@ static int join_side(swpc3d_ctx *c, hipStream_t ps)
     return SWPC_OK;
 }
 
+// What the learned-boundary kernels need of the context (FP32 fields).
+static LabGrid lab_grid(const swpc3d_ctx *c)
+{
+    LabGrid g;
+    g.px = c->L.px;
+    g.sxy = c->L.sxy;
+    for (int q = 0; q < 3; q++) g.F[q] = (float *)c->V[q].p;
+    for (int q = 0; q < 6; q++) g.F[3 + q] = (float *)c->S[q].p;
+    g.rho = (const float *)c->rho.p;
+    g.lam = (const float *)c->med[MLAM].p;
+    g.mu = (const float *)c->med[MMU].p;
+    g.dt = (float)c->d.dt;
+    g.h[0] = (float)c->d.dx; g.h[1] = (float)c->d.dy; g.h[2] = (float)c->d.dz;
+    return g;
+}
+
+// Ghost fill of the learned boundary before a half-step (phase 0: the
+// velocity ghosts the stress update reads; 1: the stress ghosts), timed
+// under the absorber class of that half-step.
+static int launch_lab(swpc3d_ctx *c, int phase)
+{
+    if (!c->lab) return SWPC_OK;
+    hipEvent_t e0;
+    const int cls = phase == 0 ? 1 : 3;
+    if (int r = prof_begin(c->prof, cls, c->stream, &e0)) return r;
+    if (int r = lab_fill(lab_grid(c), c->lab_dom, c->lab_w.p, (const float4 *)c->lab_colmed.p, phase, c->stream))
+        return r;
+    return prof_end(c->prof, cls, c->stream, e0);
+}
+
 template <typename T>
 static int launch_stress(swpc3d_ctx *c)
 {
+    if constexpr (sizeof(T) == 4) {
+        if (int r = launch_lab(c, 0)) return r;
+    }
     K3<T> a = make_args<T>(c);
     const bool f64 = sizeof(T) == 8;
     const int by = f64 ? 4 : c->s_by, wpe = f64 ? 0 : c->s_wpe;
@@ -2168,6 +2206,9 @@ static int launch_stress(swpc3d_ctx *c)
 template <typename T>
 static int launch_vel(swpc3d_ctx *c)
 {
+    if constexpr (sizeof(T) == 4) {
+        if (int r = launch_lab(c, 1)) return r;
+    }
     K3<T> a = make_args<T>(c);
     const bool f64 = sizeof(T) == 8;
     const int by = f64 ? 4 : c->v_by, wpe = f64 ? 0 : c->v_wpe;
@@ -2553,6 +2594,8 @@ int swpc3d_destroy(swpc3d_ctx *c)
     for (auto &b : c->med) dfree(b);
     for (auto &b : c->mbs) dfree(b);
     dfree(c->rho);
+    dfree(c->lab_w);
+    dfree(c->lab_colmed);
     DBuf *rest[] = {&c->surf, &c->kob, &c->gx, &c->gy, &c->gz, &c->gxv, &c->src, &c->m6, &c->sdrop, &c->bsrc, &c->f3,
                     &c->fdt, &c->st, &c->uacc, &c->wvel, &c->wdisp, &c->vmax};
     for (auto b : rest) dfree(*b);
@@ -2635,6 +2678,72 @@ int swpc3d_set_density(swpc3d_ctx *c, const float *rho)
     if (int r = dalloc(c->rho, h.size() * 4, c->bytes)) return r;
     SWPC_HIP(hipMemcpy(c->rho.p, h.data(), h.size() * 4, hipMemcpyHostToDevice));
     return SWPC_OK;
+}
+
+// FP32 -> BF16, round to nearest even (the rounding the tests apply too)
+static uint16_t bf16_rne(float f)
+{
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+static int lab_check(const swpc3d_ctx *c, bool need_na0)
+{
+    if (!c) return fail(SWPC_EINVAL, "null context");
+    if (c->d.precision == SWPC_FP64 || c->m16)
+        return fail(SWPC_EINVAL, "the learned boundary runs with FP32 fields and an FP32 medium (SWPC_FP32 or SWPC_FP32_R16)");
+    if (need_na0 && c->d.na != 0) return fail(SWPC_EINVAL, "the learned boundary replaces the PML: create the context with na = 0");
+    if (!c->have_medium || !c->rho.p) return fail(SWPC_ESTATE, "set_medium and set_density must precede the learned boundary");
+    return SWPC_OK;
+}
+
+int swpc3d_set_lab(swpc3d_ctx *c, const float *params)
+{
+    clear_error();
+    if (c) graph_drop(c);  // the captured steps would launch stale arguments
+    if (c && !params) {
+        c->lab = false;
+        return SWPC_OK;
+    }
+    if (int r = lab_check(c, true)) return r;
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    std::vector<uint16_t> w((size_t)LAB_NNET * LAB_NP);
+    for (size_t q = 0; q < w.size(); q++) {
+        if (!std::isfinite(params[q])) return fail(SWPC_EINVAL, "learned-boundary parameter %zu is not finite", q);
+        w[q] = bf16_rne(params[q]);
+    }
+    if (!c->lab_w.p)
+        if (int r = dalloc(c->lab_w, w.size() * 2, c->bytes)) return r;
+    SWPC_HIP(hipMemcpy(c->lab_w.p, w.data(), w.size() * 2, hipMemcpyHostToDevice));
+    c->lab_dom = lab_domain(0, 0, c->d.nx, c->d.ny, c->d.nz);
+    if (int r = lab_colmed(lab_grid(c), c->lab_dom, c->lab_colmed, c->bytes, c->stream)) return r;
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    c->lab = true;
+    return SWPC_OK;
+}
+
+int swpc_lab_begin_run(swpc_lab_trainer *tr, swpc3d_ctx *c, int oi, int oj, int nxs, int nys, int nzs, const int *quota)
+{
+    clear_error();
+    if (!tr) return fail(SWPC_EINVAL, "null trainer");
+    if (int r = lab_check(c, false)) return r;
+    if (oi < 2 || oj < 2 || nxs < LAB_NS || nys < LAB_NS || nzs < LAB_NS || oi + nxs + 2 > c->d.nx ||
+        oj + nys + 2 > c->d.ny || nzs + 2 > c->d.nz)
+        return fail(SWPC_EINVAL, "gather domain %dx%dx%d at (%d,%d) does not fit inside the grid with its ghosts", nxs,
+                    nys, nzs, oi, oj);
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    return lab_begin(tr, lab_grid(c), lab_domain(oi, oj, nxs, nys, nzs), quota, c->stream);
+}
+
+int swpc3d_lab_gather(swpc3d_ctx *c, swpc_lab_trainer *tr, int phase, float keep, unsigned seed, float thr)
+{
+    clear_error();
+    if (!tr || (phase != 0 && phase != 1)) return fail(SWPC_EINVAL, "bad argument");
+    if (int r = lab_check(c, false)) return r;
+    return lab_gather(tr, lab_grid(c), phase, keep, seed, thr, c->stream);
 }
 
 int swpc3d_set_surface(swpc3d_ctx *c, const int *kfs_top, const int *kfs_bot, const int *kob_top, const int *kob_bot,

@@ -1,0 +1,812 @@
+// Learned absorbing boundary for MI355X: the ghost-fill networks of the
+// per-face MLP boundary (inference inside the time step), the gather of
+// training samples from wide-PML runs, and the training kernels (forward,
+// backward, Adam) that fit the networks on the GPU.
+//
+// The definition of the boundary (features, network, ghost cells) is
+// stated in swpc_lab.h.  Everything here is FP32 arithmetic; the nets used
+// in the time step are stored in BF16.
+//
+// Kernels
+//   k_lab_colmed  per-column medium features (vs/vp, vp*dt/h, rho*vp)
+//   k_lab_fill    ghost fill of one phase: one thread per face column, the
+//                 face's net staged in LDS (broadcast reads)
+//   k_lab_gather  training samples of one phase from a wide run's fields
+//   k_lab_fwdbwd  per-workgroup forward + backward of 128 samples; the
+//                 weight gradients are formed as LDS-resident outer-product
+//                 sums and written as one partial per workgroup
+//   k_lab_adam    deterministic reduction of the partials + Adam update
+// None of this is a dense contraction big enough for MFMA to matter: the
+// ghost fill is bound by the strided strip loads (about 240 B per column
+// against 7 kFLOP), the training by launch latency.
+#include "swpc_lab.h"
+
+#include <hip/hip_bf16.h>
+
+#include <algorithm>
+#include <cmath>
+
+namespace swpc {
+
+// ghost targets per net (2*face + phase): field id (0..8 = Vx Vy Vz Sxx Syy
+// Szz Syz Sxz Sxy) and ghost depth (1 = the cell next to the face).  These
+// are exactly the ghost cells that the 4th-order stress (phase 0) and
+// velocity (phase 1) stencils read beyond each face.
+__constant__ int c_tgt[LAB_NNET][LAB_NOUT][2] = {
+    {{0, 2}, {0, 1}, {1, 1}, {2, 1}, {0, 0}},   // x- V
+    {{3, 1}, {8, 2}, {8, 1}, {7, 2}, {7, 1}},   // x- S
+    {{0, 1}, {1, 1}, {1, 2}, {2, 1}, {2, 2}},   // x+ V
+    {{3, 1}, {3, 2}, {8, 1}, {7, 1}, {0, 0}},   // x+ S
+    {{1, 2}, {1, 1}, {0, 1}, {2, 1}, {0, 0}},   // y- V
+    {{4, 1}, {8, 2}, {8, 1}, {6, 2}, {6, 1}},   // y- S
+    {{1, 1}, {0, 1}, {0, 2}, {2, 1}, {2, 2}},   // y+ V
+    {{4, 1}, {4, 2}, {8, 1}, {6, 1}, {0, 0}},   // y+ S
+    {{2, 1}, {0, 1}, {0, 2}, {1, 1}, {1, 2}},   // z+ V
+    {{5, 1}, {5, 2}, {7, 1}, {6, 1}, {0, 0}},   // z+ S
+};
+static const int h_nout[LAB_NNET] = {4, 5, 5, 4, 4, 5, 5, 4, 5, 4};
+__constant__ int c_nout[LAB_NNET] = {4, 5, 5, 4, 4, 5, 5, 4, 5, 4};
+
+LabDomain lab_domain(int oi, int oj, int nx, int ny, int nz)
+{
+    LabDomain d;
+    d.oi = oi; d.oj = oj; d.nx = nx; d.ny = ny; d.nz = nz;
+    d.ncol[0] = d.ncol[1] = ny * nz;
+    d.ncol[2] = d.ncol[3] = nx * nz;
+    d.ncol[4] = nx * ny;
+    d.col0[0] = 0;
+    for (int f = 0; f < 5; f++) d.col0[f + 1] = d.col0[f] + d.ncol[f];
+    return d;
+}
+
+struct LabDev {
+    int px, sxy;
+    float *F[9];
+    const float *rho, *lam, *mu;
+    float dt, h[3];
+    int oi, oj, nx, ny, nz;
+    int ncol[5];
+    long long col0[6];
+};
+
+static LabDev make_dev(const LabGrid &g, const LabDomain &d)
+{
+    LabDev a;
+    a.px = g.px; a.sxy = g.sxy;
+    for (int q = 0; q < 9; q++) a.F[q] = g.F[q];
+    a.rho = g.rho; a.lam = g.lam; a.mu = g.mu;
+    a.dt = g.dt;
+    for (int q = 0; q < 3; q++) a.h[q] = g.h[q];
+    a.oi = d.oi; a.oj = d.oj; a.nx = d.nx; a.ny = d.ny; a.nz = d.nz;
+    for (int f = 0; f < 5; f++) a.ncol[f] = d.ncol[f];
+    for (int f = 0; f < 6; f++) a.col0[f] = d.col0[f];
+    return a;
+}
+
+// Address of strip cell d = 1 of column col of face f, and the address
+// step from cell d to d+1 (inward); ghost depth h sits at d = 1 - h.
+__device__ __forceinline__ void lab_col(const LabDev &a, int f, int col, int &base, int &step)
+{
+    int i, j, k;
+    if (f < 2) {
+        j = col % a.ny + 1; k = col / a.ny + 1;
+        i = f == 0 ? 1 : a.nx;
+        step = f == 0 ? 1 : -1;
+    } else if (f < 4) {
+        i = col % a.nx + 1; k = col / a.nx + 1;
+        j = f == 2 ? 1 : a.ny;
+        step = f == 2 ? a.px : -a.px;
+    } else {
+        i = col % a.nx + 1; j = col / a.nx + 1;
+        k = a.nz;
+        step = -a.sxy;
+    }
+    base = at3(a.px, a.sxy, i + a.oi, j + a.oj, k);
+}
+
+__device__ __forceinline__ int lab_face(const LabDev &a, long long g, int &col)
+{
+    int f = 0;
+#pragma unroll
+    for (int q = 1; q < 5; q++) f += g >= a.col0[q] ? 1 : 0;
+    col = (int)(g - a.col0[f]);
+    return f;
+}
+
+__global__ void __launch_bounds__(256) k_lab_colmed(const LabDev a, float4 *__restrict__ out)
+{
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.col0[5]) return;
+    int col;
+    const int f = lab_face(a, g, col);
+    int base, step;
+    lab_col(a, f, col, base, step);
+    const float rho = a.rho[base], lam = a.lam[base], mu = a.mu[base];
+    const float l2m = lam + 2.0f * mu;
+    const float h = a.h[f < 2 ? 0 : (f < 4 ? 1 : 2)];
+    float4 m = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+    if (l2m > 0.0f) {
+        const float vp = sqrtf(l2m / rho);
+        const float vs = sqrtf(fmaxf(mu, 0.0f) / rho);
+        m.x = vs / vp;
+        m.y = vp * a.dt / h;
+        m.z = rho * vp;
+    }
+    out[g] = m;
+}
+
+// Features of one column into x[LAB_NIN]; prev holds the previous ghost
+// values (stresses already divided by rho*vp).  Returns N (0: no wave).
+__device__ __forceinline__ float lab_features(const LabDev &a, int base, int step, float4 m, const float *prev,
+                                              float *x)
+{
+    const float inv_imp = 1.0f / m.z;
+    float amax = 0.0f;
+#pragma unroll
+    for (int fi = 0; fi < 9; fi++) {
+        const float *F = a.F[fi];
+#pragma unroll
+        for (int d = 0; d < LAB_NS; d++) {
+            float v = F[base + d * step];
+            if (fi >= 3) v = v * inv_imp;
+            x[fi * LAB_NS + d] = v;
+            amax = fmaxf(amax, fabsf(v));
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < LAB_NG; q++) {
+        x[LAB_NF + q] = prev[q];
+        amax = fmaxf(amax, fabsf(prev[q]));
+    }
+    // N = sqrt(mean square), formed on values scaled by their largest
+    // magnitude so that the far-field precursors (|v| ~ 1e-25) do not
+    // underflow; columns below LAB_TINY count as empty (N = 0), which keeps
+    // 1/amax finite and denormals (flushed or not) out of the result
+    float N = 0.0f;
+    if (amax > LAB_TINY) {
+        const float ia = 1.0f / amax;
+        float s2 = 0.0f;
+#pragma unroll
+        for (int q = 0; q < LAB_NSC; q++) {
+            const float u = x[q] * ia;
+            s2 = fmaf(u, u, s2);
+        }
+        N = amax * sqrtf(s2 * (1.0f / (float)LAB_NSC));
+        const float inv = 1.0f / N;
+#pragma unroll
+        for (int q = 0; q < LAB_NSC; q++) x[q] *= inv;
+    }
+    x[LAB_NSC] = m.x;
+    x[LAB_NSC + 1] = m.y;
+    return N;
+}
+
+// y = Ws x + W3 tanh(W2 tanh(W1 x + b1) + b2) + b3 with the net in LDS.
+__device__ __forceinline__ void lab_forward(const float *__restrict__ W, const float *x, float *y)
+{
+    float h1[LAB_NH], h2[LAB_NH];
+#pragma unroll 4
+    for (int h = 0; h < LAB_NH; h++) {
+        float z = W[LAB_B1 + h];
+#pragma unroll
+        for (int q = 0; q < LAB_NIN; q++) z = fmaf(W[LAB_W1 + h * LAB_NIN + q], x[q], z);
+        h1[h] = tanhf(z);
+    }
+#pragma unroll 4
+    for (int h = 0; h < LAB_NH; h++) {
+        float z = W[LAB_B2 + h];
+#pragma unroll
+        for (int g = 0; g < LAB_NH; g++) z = fmaf(W[LAB_W2 + h * LAB_NH + g], h1[g], z);
+        h2[h] = tanhf(z);
+    }
+#pragma unroll
+    for (int o = 0; o < LAB_NOUT; o++) {
+        float z = W[LAB_B3 + o];
+#pragma unroll
+        for (int h = 0; h < LAB_NH; h++) z = fmaf(W[LAB_W3 + o * LAB_NH + h], h2[h], z);
+#pragma unroll
+        for (int q = 0; q < LAB_NIN; q++) z = fmaf(W[LAB_WS + o * LAB_NIN + q], x[q], z);
+        y[o] = z;
+    }
+}
+
+struct LabFillArgs {
+    LabDev a;
+    const __hip_bfloat16 *w;   // ten nets
+    const float4 *colmed;
+    int phase;
+    int blk0[6];               // first workgroup of each face
+};
+
+// One thread per column; each workgroup serves one face, whose net is
+// widened from BF16 into LDS.
+__global__ void __launch_bounds__(256) k_lab_fill(const LabFillArgs p)
+{
+    __shared__ float W[LAB_NP];
+    int f = 0;
+#pragma unroll
+    for (int q = 1; q < 5; q++) f += (int)blockIdx.x >= p.blk0[q] ? 1 : 0;
+    const int net = 2 * f + p.phase;
+    const __hip_bfloat16 *wn = p.w + (size_t)net * LAB_NP;
+    for (int q = threadIdx.x; q < LAB_NP; q += blockDim.x) W[q] = __bfloat162float(wn[q]);
+    __syncthreads();
+    const LabDev &a = p.a;
+    const int col = ((int)blockIdx.x - p.blk0[f]) * blockDim.x + threadIdx.x;
+    if (col >= a.ncol[f]) return;
+    int base, step;
+    lab_col(a, f, col, base, step);
+    const float4 m = p.colmed[a.col0[f] + col];
+    const int nout = c_nout[net];
+    int gaddr[LAB_NOUT];
+    float prev[LAB_NG];
+#pragma unroll
+    for (int q = 0; q < LAB_NOUT; q++) {
+        const int fi = c_tgt[net][q][0], h = c_tgt[net][q][1];
+        gaddr[q] = base - h * step;
+        float v = q < nout ? a.F[fi][gaddr[q]] : 0.0f;
+        prev[q] = fi >= 3 ? v / m.z : v;
+    }
+    float x[LAB_NIN], y[LAB_NOUT];
+    const float N = lab_features(a, base, step, m, prev, x);
+    if (N > 0.0f && m.y > 0.0f) {  // air columns (lam + 2 mu = 0) keep zero ghosts
+        lab_forward(W, x, y);
+    } else {
+#pragma unroll
+        for (int q = 0; q < LAB_NOUT; q++) y[q] = 0.0f;
+    }
+#pragma unroll
+    for (int q = 0; q < LAB_NOUT; q++) {
+        if (q >= nout) break;
+        const int fi = c_tgt[net][q][0];
+        float v = y[q] * N;
+        if (fi >= 3) v = v * m.z;
+        a.F[fi][gaddr[q]] = v;
+    }
+}
+
+int lab_colmed(const LabGrid &g, const LabDomain &dom, DBuf &out, long long &bytes, hipStream_t s)
+{
+    const long long n = dom.col0[5];
+    if (int r = dalloc(out, (size_t)n * sizeof(float4), bytes)) return r;
+    if (n == 0) return SWPC_OK;
+    const LabDev a = make_dev(g, dom);
+    hipLaunchKernelGGL(k_lab_colmed, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, (float4 *)out.p);
+    SWPC_CHECK_LAUNCH();
+    return SWPC_OK;
+}
+
+int lab_fill(const LabGrid &g, const LabDomain &dom, const void *w, const float4 *colmed, int phase, hipStream_t s)
+{
+    LabFillArgs p;
+    p.a = make_dev(g, dom);
+    p.w = (const __hip_bfloat16 *)w;
+    p.colmed = colmed;
+    p.phase = phase;
+    int b = 0;
+    for (int f = 0; f < 5; f++) {
+        p.blk0[f] = b;
+        b += (dom.ncol[f] + 255) / 256;
+    }
+    p.blk0[5] = b;
+    if (b == 0) return SWPC_OK;
+    hipLaunchKernelGGL(k_lab_fill, dim3(b), dim3(256), 0, s, p);
+    SWPC_CHECK_LAUNCH();
+    return SWPC_OK;
+}
+
+__host__ __device__ __forceinline__ unsigned lab_hash(unsigned x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+}  // namespace swpc
+
+// ---------------------------------------------------------------------------
+// trainer
+// ---------------------------------------------------------------------------
+struct swpc_lab_trainer {
+    int cap = 0;                 // samples per net
+    long long bytes = 0;
+    hipStream_t stream = nullptr;
+    swpc::DBuf samples;          // [10][cap][LAB_REC]
+    swpc::DBuf count;            // int[10], may run past the limit until commit
+    swpc::DBuf runmax;           // float[10] (as uint for atomicMax)
+    swpc::DBuf prev;             // gather domain: previous ghost values [10][prev_cols][5]
+    long long prev_cols = 0;
+    swpc::LabDomain dom;         // gather domain inside the wide grid
+    swpc::DBuf colmed;           // its column medium
+    int limit[swpc::LAB_NNET] = {0};
+    swpc::DBuf P, M, V, partial, ploss, loss, X;   // optimiser state
+};
+
+extern "C" int swpc_lab_count(swpc_lab_trainer *tr, int *counts);
+
+namespace swpc {
+
+struct LabGatherArgs {
+    LabDev a;
+    const float4 *colmed;
+    float *samples;
+    int *count;
+    unsigned *runmax;
+    float *prev;                 // [10][prev_cols][5]
+    long long prev_cols;
+    int cap;
+    int limit[5];
+    int phase;
+    float keep, thr;
+    unsigned seed;
+};
+
+__global__ void __launch_bounds__(256) k_lab_gather(const LabGatherArgs p)
+{
+    const LabDev &a = p.a;
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.col0[5]) return;
+    int col;
+    const int f = lab_face(a, g, col);
+    const int net = 2 * f + p.phase;
+    int base, step;
+    lab_col(a, f, col, base, step);
+    const float4 m = p.colmed[g];
+    const int nout = c_nout[net];
+    float *pv = p.prev + ((size_t)net * p.prev_cols + col) * LAB_NG;
+    float prev[LAB_NG], t[LAB_NOUT];
+#pragma unroll
+    for (int q = 0; q < LAB_NOUT; q++) {
+        prev[q] = pv[q];
+        const int fi = c_tgt[net][q][0], h = c_tgt[net][q][1];
+        float v = q < nout ? a.F[fi][base - h * step] : 0.0f;
+        t[q] = fi >= 3 ? v / m.z : v;
+    }
+    float x[LAB_NIN];
+    const float N = lab_features(a, base, step, m, prev, x);
+#pragma unroll
+    for (int q = 0; q < LAB_NOUT; q++) pv[q] = t[q];
+    if (!(N > 0.0f) || !(m.y > 0.0f)) return;
+    const float rmax = __uint_as_float(p.runmax[net]);
+    atomicMax(p.runmax + net, __float_as_uint(N));
+    if (N < p.thr * rmax) return;
+    const unsigned u = lab_hash(p.seed ^ lab_hash((unsigned)g * 2u + (unsigned)p.phase + 0x9e3779b9u));
+    if ((float)(u >> 8) * (1.0f / 16777216.0f) >= p.keep) return;
+    const int idx = atomicAdd(p.count + net, 1);
+    if (idx >= p.limit[f] || idx >= p.cap) return;
+    float *o = p.samples + ((size_t)net * p.cap + idx) * LAB_REC;
+    const float inv = 1.0f / N;
+#pragma unroll
+    for (int q = 0; q < LAB_NIN; q++) o[q] = x[q];
+#pragma unroll
+    for (int q = 0; q < LAB_NOUT; q++) o[LAB_T0 + q] = t[q] * inv;
+    o[LAB_T0 + 5] = (float)col;
+    o[LAB_T0 + 6] = N;
+    o[LAB_T0 + 7] = m.z;
+#pragma unroll
+    for (int q = LAB_T0 + 8; q < LAB_REC; q++) o[q] = 0.0f;
+}
+
+int lab_begin(swpc_lab_trainer *tr, const LabGrid &g, const LabDomain &dom, const int *quota, hipStream_t s)
+{
+    int c[LAB_NNET];
+    if (int r = swpc_lab_count(tr, c)) return r;
+    long long maxcol = 0;
+    for (int f = 0; f < 5; f++) maxcol = std::max(maxcol, (long long)dom.ncol[f]);
+    if (int r = dalloc(tr->prev, (size_t)LAB_NNET * maxcol * LAB_NG * sizeof(float), tr->bytes)) return r;
+    tr->prev_cols = maxcol;
+    tr->dom = dom;
+    if (int r = lab_colmed(g, dom, tr->colmed, tr->bytes, s)) return r;
+    SWPC_HIP(hipMemsetAsync(tr->runmax.p, 0, LAB_NNET * sizeof(unsigned), s));
+    for (int n = 0; n < LAB_NNET; n++) tr->limit[n] = std::min(tr->cap, c[n] + std::max(0, quota ? quota[n] : tr->cap));
+    SWPC_HIP(hipStreamSynchronize(s));
+    return SWPC_OK;
+}
+
+int lab_gather(swpc_lab_trainer *tr, const LabGrid &g, int phase, float keep, unsigned seed, float thr, hipStream_t s)
+{
+    const LabDomain &dom = tr->dom;
+    const long long n = dom.col0[5];
+    if (!tr->prev.p) return fail(SWPC_ESTATE, "swpc_lab_begin_run must precede swpc3d_lab_gather");
+    LabGatherArgs p;
+    p.a = make_dev(g, dom);
+    p.colmed = (const float4 *)tr->colmed.p;
+    p.samples = (float *)tr->samples.p;
+    p.count = (int *)tr->count.p;
+    p.runmax = (unsigned *)tr->runmax.p;
+    p.prev = (float *)tr->prev.p;
+    p.prev_cols = tr->prev_cols;
+    p.cap = tr->cap;
+    for (int f = 0; f < 5; f++) p.limit[f] = tr->limit[2 * f + phase];
+    p.phase = phase;
+    p.keep = keep;
+    p.thr = thr;
+    p.seed = seed;
+    if (n == 0) return SWPC_OK;
+    hipLaunchKernelGGL(k_lab_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p);
+    SWPC_CHECK_LAUNCH();
+    return SWPC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// training
+// ---------------------------------------------------------------------------
+constexpr int TB = 128;                 // samples (threads) per workgroup
+constexpr int XP = LAB_NIN + 1;         // padded LDS rows (odd strides)
+constexpr int HP = LAB_NH + 1;
+
+struct FwdBwdArgs {
+    const float *P;        // FP32 parameters of one net
+    const float *samples;  // records of this net
+    int count;             // records available
+    int nout;
+    int mode;              // 0: random records with random sign; 1: records 0..count-1 as given
+    unsigned seed;
+    float noise;           // mode 0: N(0, noise^2) added to the non-zero previous-ghost features
+    float scale;           // 2 / (batch * nout)
+    float *partial;        // [nblk][LAB_NP]
+    float *ploss;          // [nblk]
+};
+
+// Forward and backward of TB samples per workgroup: one thread per sample
+// keeps its activations in registers and parks x, h1, h2 and the deltas in
+// LDS; the weight gradients are then summed over the workgroup's samples
+// from LDS, each thread owning a set of parameters.  The loss is the mean
+// square error over the batch and the net's outputs; with mode 0 a random
+// half of the samples enter with the sign of every field feature and
+// target flipped (the wave equation is odd in the fields).
+__global__ void __launch_bounds__(TB) k_lab_fwdbwd(const FwdBwdArgs p)
+{
+    __shared__ float W[LAB_NP];
+    __shared__ float sX[TB][XP];
+    __shared__ float sH1[TB][HP], sD1[TB][HP], sH2[TB][HP], sD2[TB][HP];
+    __shared__ float sD3[TB][LAB_NOUT];
+    __shared__ float red[TB];
+    const int tid = threadIdx.x;
+    for (int q = tid; q < LAB_NP; q += TB) W[q] = p.P[q];
+    __syncthreads();
+    const int b = blockIdx.x * TB + tid;
+    int idx;
+    float sg = 1.0f;
+    if (p.mode == 0) {
+        const unsigned u = lab_hash(p.seed ^ lab_hash((unsigned)b + 0x632be5abu));
+        idx = p.count > 0 ? (int)(u % (unsigned)p.count) : -1;
+        sg = (lab_hash(u) & 1u) ? -1.0f : 1.0f;
+    } else {
+        idx = b < p.count ? b : -1;
+    }
+    float x[LAB_NIN], t[LAB_NOUT];
+    if (idx >= 0) {
+        const float *r = p.samples + (size_t)idx * LAB_REC;
+#pragma unroll
+        for (int q = 0; q < LAB_NIN; q++) x[q] = q < LAB_NSC ? sg * r[q] : r[q];
+#pragma unroll
+        for (int q = 0; q < LAB_NOUT; q++) t[q] = sg * r[LAB_T0 + q];
+        if (p.mode == 0 && p.noise > 0.0f) {
+            // Box-Muller pairs from hashed uniforms: the recurrent ghost
+            // inputs are trained against perturbations of their own
+            // previous outputs, which keeps the closed loop stable
+#pragma unroll
+            for (int q = 0; q < LAB_NG; q += 2) {
+                const unsigned h1 = lab_hash(p.seed ^ lab_hash((unsigned)b * 8u + (unsigned)q + 0x51ed27u));
+                const unsigned h2 = lab_hash(h1 + 0x2545f491u);
+                const float u1 = ((float)(h1 >> 8) + 0.5f) * (1.0f / 16777216.0f);
+                const float u2 = (float)(h2 >> 8) * (1.0f / 16777216.0f);
+                const float rad = sqrtf(-2.0f * logf(u1));
+                const float g0 = rad * cosf(6.2831853f * u2), g1 = rad * sinf(6.2831853f * u2);
+                if (x[LAB_NF + q] != 0.0f) x[LAB_NF + q] += p.noise * g0;
+                if (q + 1 < LAB_NG && x[LAB_NF + q + 1] != 0.0f) x[LAB_NF + q + 1] += p.noise * g1;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < LAB_NIN; q++) x[q] = 0.0f;
+#pragma unroll
+        for (int q = 0; q < LAB_NOUT; q++) t[q] = 0.0f;
+    }
+    float h1[LAB_NH], h2[LAB_NH], y[LAB_NOUT];
+    for (int h = 0; h < LAB_NH; h++) {
+        float z = W[LAB_B1 + h];
+#pragma unroll
+        for (int q = 0; q < LAB_NIN; q++) z = fmaf(W[LAB_W1 + h * LAB_NIN + q], x[q], z);
+        h1[h] = tanhf(z);
+    }
+    for (int h = 0; h < LAB_NH; h++) {
+        float z = W[LAB_B2 + h];
+#pragma unroll
+        for (int g = 0; g < LAB_NH; g++) z = fmaf(W[LAB_W2 + h * LAB_NH + g], h1[g], z);
+        h2[h] = tanhf(z);
+    }
+    float l = 0.0f;
+    float d3[LAB_NOUT];
+#pragma unroll
+    for (int o = 0; o < LAB_NOUT; o++) {
+        float z = W[LAB_B3 + o];
+#pragma unroll
+        for (int h = 0; h < LAB_NH; h++) z = fmaf(W[LAB_W3 + o * LAB_NH + h], h2[h], z);
+#pragma unroll
+        for (int q = 0; q < LAB_NIN; q++) z = fmaf(W[LAB_WS + o * LAB_NIN + q], x[q], z);
+        y[o] = z;
+        const float e = (idx >= 0 && o < p.nout) ? y[o] - t[o] : 0.0f;
+        l = fmaf(e, e, l);
+        d3[o] = p.scale * e;
+    }
+    float d2[LAB_NH];
+    for (int h = 0; h < LAB_NH; h++) {
+        float s = 0.0f;
+#pragma unroll
+        for (int o = 0; o < LAB_NOUT; o++) s = fmaf(W[LAB_W3 + o * LAB_NH + h], d3[o], s);
+        d2[h] = s * (1.0f - h2[h] * h2[h]);
+    }
+    for (int g = 0; g < LAB_NH; g++) {
+        float s = 0.0f;
+#pragma unroll
+        for (int h = 0; h < LAB_NH; h++) s = fmaf(W[LAB_W2 + h * LAB_NH + g], d2[h], s);
+        sD1[tid][g] = s * (1.0f - h1[g] * h1[g]);
+    }
+#pragma unroll
+    for (int q = 0; q < LAB_NIN; q++) sX[tid][q] = x[q];
+#pragma unroll
+    for (int h = 0; h < LAB_NH; h++) {
+        sH1[tid][h] = h1[h];
+        sH2[tid][h] = h2[h];
+        sD2[tid][h] = d2[h];
+    }
+#pragma unroll
+    for (int o = 0; o < LAB_NOUT; o++) sD3[tid][o] = d3[o];
+    red[tid] = l;
+    __syncthreads();
+    float *out = p.partial + (size_t)blockIdx.x * LAB_NP;
+    for (int q = tid; q < LAB_NP; q += TB) {
+        float s = 0.0f;
+        if (q < LAB_B1) {
+            const int h = q / LAB_NIN, f = q % LAB_NIN;
+            for (int n = 0; n < TB; n++) s = fmaf(sD1[n][h], sX[n][f], s);
+        } else if (q < LAB_W2) {
+            const int h = q - LAB_B1;
+            for (int n = 0; n < TB; n++) s += sD1[n][h];
+        } else if (q < LAB_B2) {
+            const int h = (q - LAB_W2) / LAB_NH, g = (q - LAB_W2) % LAB_NH;
+            for (int n = 0; n < TB; n++) s = fmaf(sD2[n][h], sH1[n][g], s);
+        } else if (q < LAB_W3) {
+            const int h = q - LAB_B2;
+            for (int n = 0; n < TB; n++) s += sD2[n][h];
+        } else if (q < LAB_B3) {
+            const int o = (q - LAB_W3) / LAB_NH, h = (q - LAB_W3) % LAB_NH;
+            for (int n = 0; n < TB; n++) s = fmaf(sD3[n][o], sH2[n][h], s);
+        } else if (q < LAB_WS) {
+            const int o = q - LAB_B3;
+            for (int n = 0; n < TB; n++) s += sD3[n][o];
+        } else {
+            const int o = (q - LAB_WS) / LAB_NIN, f = (q - LAB_WS) % LAB_NIN;
+            for (int n = 0; n < TB; n++) s = fmaf(sD3[n][o], sX[n][f], s);
+        }
+        out[q] = s;
+    }
+    if (tid == 0) {
+        float s = 0.0f;
+        for (int n = 0; n < TB; n++) s += red[n];
+        p.ploss[blockIdx.x] = s;
+    }
+}
+
+// Sum of the partials in workgroup order (deterministic), then Adam with
+// bias correction; thread 0 of workgroup 0 also records the step's loss.
+__global__ void __launch_bounds__(256) k_lab_adam(float *__restrict__ P, float *__restrict__ M, float *__restrict__ V,
+                                                  const float *__restrict__ partial, const float *__restrict__ ploss,
+                                                  int nblk, float lr, float b1, float b2, float eps, float c1, float c2,
+                                                  float lnorm, float *__restrict__ loss_out, float *__restrict__ grad)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q == 0 && loss_out) {
+        float s = 0.0f;
+        for (int n = 0; n < nblk; n++) s += ploss[n];
+        *loss_out = s * lnorm;
+    }
+    if (q >= LAB_NP) return;
+    float g = 0.0f;
+    for (int n = 0; n < nblk; n++) g += partial[(size_t)n * LAB_NP + q];
+    if (grad) {
+        grad[q] = g;
+        return;
+    }
+    const float m = b1 * M[q] + (1.0f - b1) * g;
+    const float v = b2 * V[q] + (1.0f - b2) * g * g;
+    M[q] = m;
+    V[q] = v;
+    P[q] -= lr * (m * c1) / (sqrtf(v * c2) + eps);
+}
+
+static int tr_ensure(swpc_lab_trainer *tr, int nblk)
+{
+    if (!tr->P.p) {
+        if (int r = dalloc(tr->P, LAB_NP * sizeof(float), tr->bytes)) return r;
+        if (int r = dalloc(tr->M, LAB_NP * sizeof(float), tr->bytes)) return r;
+        if (int r = dalloc(tr->V, LAB_NP * sizeof(float), tr->bytes)) return r;
+    }
+    if (tr->partial.bytes < (size_t)nblk * LAB_NP * sizeof(float)) {
+        if (int r = dalloc(tr->partial, (size_t)nblk * LAB_NP * sizeof(float), tr->bytes)) return r;
+        if (int r = dalloc(tr->ploss, (size_t)nblk * sizeof(float), tr->bytes)) return r;
+    }
+    return SWPC_OK;
+}
+
+}  // namespace swpc
+
+using namespace swpc;
+
+extern "C" {
+
+int swpc_lab_nparam(void) { return LAB_NP; }
+
+int swpc_lab_trainer_create(int cap, swpc_lab_trainer **out)
+{
+    clear_error();
+    if (!out || cap <= 0) return fail(SWPC_EINVAL, "bad trainer capacity");
+    *out = nullptr;
+    auto *tr = new swpc_lab_trainer();
+    tr->cap = cap;
+    int r;
+    if (hipStreamCreate(&tr->stream) != hipSuccess) r = fail(SWPC_EHIP, "hipStreamCreate failed");
+    else if ((r = dalloc(tr->samples, (size_t)LAB_NNET * cap * LAB_REC * sizeof(float), tr->bytes)) == 0 &&
+             (r = dalloc(tr->count, LAB_NNET * sizeof(int), tr->bytes)) == 0)
+        r = dalloc(tr->runmax, LAB_NNET * sizeof(unsigned), tr->bytes);
+    if (r) {
+        swpc_lab_trainer_destroy(tr);
+        return r;
+    }
+    *out = tr;
+    return SWPC_OK;
+}
+
+int swpc_lab_trainer_destroy(swpc_lab_trainer *tr)
+{
+    if (!tr) return SWPC_OK;
+    if (tr->stream) (void)hipStreamSynchronize(tr->stream);
+    for (DBuf *b : {&tr->samples, &tr->count, &tr->runmax, &tr->prev, &tr->P, &tr->M, &tr->V, &tr->partial,
+                    &tr->ploss, &tr->loss, &tr->X})
+        dfree(*b);
+    if (tr->stream) (void)hipStreamDestroy(tr->stream);
+    delete tr;
+    return SWPC_OK;
+}
+
+// Clamp the sample counts to what was stored and copy them to counts[10].
+int swpc_lab_count(swpc_lab_trainer *tr, int *counts)
+{
+    clear_error();
+    if (!tr || !counts) return fail(SWPC_EINVAL, "null argument");
+    SWPC_HIP(hipDeviceSynchronize());
+    int c[LAB_NNET];
+    SWPC_HIP(hipMemcpy(c, tr->count.p, sizeof c, hipMemcpyDeviceToHost));
+    for (int n = 0; n < LAB_NNET; n++) c[n] = std::min(c[n], std::min(tr->cap, tr->limit[n]));
+    SWPC_HIP(hipMemcpy(tr->count.p, c, sizeof c, hipMemcpyHostToDevice));
+    for (int n = 0; n < LAB_NNET; n++) counts[n] = c[n];
+    return SWPC_OK;
+}
+
+int swpc_lab_add(swpc_lab_trainer *tr, int net, int n, const float *rec)
+{
+    clear_error();
+    if (!tr || !rec || net < 0 || net >= LAB_NNET || n < 0) return fail(SWPC_EINVAL, "bad argument");
+    int c[LAB_NNET];
+    if (int r = swpc_lab_count(tr, c)) return r;
+    if (c[net] + n > tr->cap) return fail(SWPC_EINVAL, "trainer capacity %d exceeded", tr->cap);
+    float *dst = (float *)tr->samples.p + ((size_t)net * tr->cap + c[net]) * LAB_REC;
+    SWPC_HIP(hipMemcpy(dst, rec, (size_t)n * LAB_REC * sizeof(float), hipMemcpyHostToDevice));
+    c[net] += n;
+    tr->limit[net] = std::max(tr->limit[net], c[net]);
+    SWPC_HIP(hipMemcpy(tr->count.p, c, sizeof c, hipMemcpyHostToDevice));
+    return SWPC_OK;
+}
+
+int swpc_lab_samples(swpc_lab_trainer *tr, int net, int first, int n, float *rec)
+{
+    clear_error();
+    if (!tr || !rec || net < 0 || net >= LAB_NNET || first < 0 || n < 0 || first + n > tr->cap)
+        return fail(SWPC_EINVAL, "bad argument");
+    SWPC_HIP(hipDeviceSynchronize());
+    const float *src = (const float *)tr->samples.p + ((size_t)net * tr->cap + first) * LAB_REC;
+    SWPC_HIP(hipMemcpy(rec, src, (size_t)n * LAB_REC * sizeof(float), hipMemcpyDeviceToHost));
+    return SWPC_OK;
+}
+
+// Adam on net `net` of the sample store: params (FP32, LAB_NP) in and out;
+// the learning rate follows a cosine from lr to lr/100 over `steps`;
+// noise perturbs the previous-ghost features; loss[steps] (optional)
+// receives the batch loss before each update.
+int swpc_lab_train(swpc_lab_trainer *tr, int net, float *params, int steps, int batch, float lr, float noise,
+                   unsigned seed, float *loss)
+{
+    clear_error();
+    if (!tr || !params || net < 0 || net >= LAB_NNET || steps < 0 || batch < TB || batch % TB)
+        return fail(SWPC_EINVAL, "bad argument (batch must be a positive multiple of %d)", TB);
+    int c[LAB_NNET];
+    if (int r = swpc_lab_count(tr, c)) return r;
+    if (c[net] == 0) return fail(SWPC_ESTATE, "no samples for net %d", net);
+    const int nblk = batch / TB;
+    if (int r = tr_ensure(tr, nblk)) return r;
+    if (int r = dalloc(tr->loss, (size_t)std::max(steps, 1) * sizeof(float), tr->bytes)) return r;
+    hipStream_t s = tr->stream;
+    SWPC_HIP(hipMemcpyAsync(tr->P.p, params, LAB_NP * sizeof(float), hipMemcpyHostToDevice, s));
+    SWPC_HIP(hipMemsetAsync(tr->M.p, 0, LAB_NP * sizeof(float), s));
+    SWPC_HIP(hipMemsetAsync(tr->V.p, 0, LAB_NP * sizeof(float), s));
+    FwdBwdArgs a;
+    a.P = (const float *)tr->P.p;
+    a.samples = (const float *)tr->samples.p + (size_t)net * tr->cap * LAB_REC;
+    a.count = c[net];
+    a.nout = h_nout[net];
+    a.mode = 0;
+    a.noise = noise;
+    a.scale = 2.0f / ((float)batch * (float)a.nout);
+    a.partial = (float *)tr->partial.p;
+    a.ploss = (float *)tr->ploss.p;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    double p1 = 1.0, p2 = 1.0;
+    for (int t = 0; t < steps; t++) {
+        a.seed = lab_hash(seed * 0x9e3779b1u + (unsigned)t * 0x85ebca77u + 1u);
+        hipLaunchKernelGGL(k_lab_fwdbwd, dim3(nblk), dim3(TB), 0, s, a);
+        SWPC_CHECK_LAUNCH();
+        p1 *= b1;
+        p2 *= b2;
+        const double frac = steps > 1 ? (double)t / (double)(steps - 1) : 0.0;
+        const float lrt = (float)(lr * (0.01 + 0.99 * 0.5 * (1.0 + cos(M_PI * frac))));
+        hipLaunchKernelGGL(k_lab_adam, dim3((LAB_NP + 255) / 256), dim3(256), 0, s, (float *)tr->P.p, (float *)tr->M.p,
+                           (float *)tr->V.p, (const float *)tr->partial.p, (const float *)tr->ploss.p, nblk, lrt, b1,
+                           b2, eps, (float)(1.0 / (1.0 - p1)), (float)(1.0 / (1.0 - p2)),
+                           1.0f / ((float)batch * (float)a.nout), (float *)tr->loss.p + t, (float *)nullptr);
+        SWPC_CHECK_LAUNCH();
+    }
+    SWPC_HIP(hipMemcpyAsync(params, tr->P.p, LAB_NP * sizeof(float), hipMemcpyDeviceToHost, s));
+    if (loss && steps > 0)
+        SWPC_HIP(hipMemcpyAsync(loss, tr->loss.p, (size_t)steps * sizeof(float), hipMemcpyDeviceToHost, s));
+    SWPC_HIP(hipStreamSynchronize(s));
+    return SWPC_OK;
+}
+
+// Loss and gradient of one net over n host records (no sign flips): the
+// check of the training kernels.  nout selects the outputs in the loss.
+int swpc_lab_grad(swpc_lab_trainer *tr, const float *params, int nout, int n, const float *rec, double *loss,
+                  float *grad)
+{
+    clear_error();
+    if (!tr || !params || !rec || !grad || n <= 0 || nout < 1 || nout > LAB_NOUT) return fail(SWPC_EINVAL, "bad argument");
+    const int nblk = (n + TB - 1) / TB;
+    if (int r = tr_ensure(tr, nblk)) return r;
+    if (tr->X.bytes < (size_t)n * LAB_REC * sizeof(float))
+        if (int r = dalloc(tr->X, (size_t)n * LAB_REC * sizeof(float), tr->bytes)) return r;
+    if (!tr->loss.p)
+        if (int r = dalloc(tr->loss, sizeof(float), tr->bytes)) return r;
+    hipStream_t s = tr->stream;
+    SWPC_HIP(hipMemcpyAsync(tr->P.p, params, LAB_NP * sizeof(float), hipMemcpyHostToDevice, s));
+    SWPC_HIP(hipMemcpyAsync(tr->X.p, rec, (size_t)n * LAB_REC * sizeof(float), hipMemcpyHostToDevice, s));
+    FwdBwdArgs a;
+    a.P = (const float *)tr->P.p;
+    a.samples = (const float *)tr->X.p;
+    a.count = n;
+    a.nout = nout;
+    a.mode = 1;
+    a.seed = 0;
+    a.noise = 0.0f;
+    a.scale = 2.0f / ((float)n * (float)nout);
+    a.partial = (float *)tr->partial.p;
+    a.ploss = (float *)tr->ploss.p;
+    hipLaunchKernelGGL(k_lab_fwdbwd, dim3(nblk), dim3(TB), 0, s, a);
+    SWPC_CHECK_LAUNCH();
+    // M doubles as the gradient output buffer here
+    hipLaunchKernelGGL(k_lab_adam, dim3((LAB_NP + 255) / 256), dim3(256), 0, s, (float *)tr->P.p, (float *)nullptr,
+                       (float *)nullptr, (const float *)tr->partial.p, (const float *)tr->ploss.p, nblk, 0.0f, 0.0f,
+                       0.0f, 0.0f, 0.0f, 0.0f, 1.0f / ((float)n * (float)nout), (float *)tr->loss.p,
+                       (float *)tr->M.p);
+    SWPC_CHECK_LAUNCH();
+    float l;
+    SWPC_HIP(hipMemcpyAsync(grad, tr->M.p, LAB_NP * sizeof(float), hipMemcpyDeviceToHost, s));
+    SWPC_HIP(hipMemcpyAsync(&l, tr->loss.p, sizeof(float), hipMemcpyDeviceToHost, s));
+    SWPC_HIP(hipStreamSynchronize(s));
+    if (loss) *loss = l;
+    return SWPC_OK;
+}
+
+}  // extern "C"

@@ -252,6 +252,16 @@ class Swpc3D:
         check(lib().swpc3d_memory(self.h, C.byref(b)), "memory")
         return b.value
 
+    def set_learned_boundary(self, params):
+        """Learned absorbing boundary in place of the PML (grid with na = 0):
+        params[10][swpc_lab_nparam()] FP32 (openswpc_amd.lab), used in BF16;
+        None switches it off."""
+        if params is None:
+            check(lib().swpc3d_set_lab(self.h, None), "set_lab")
+            return
+        p = _f32(params)
+        check(lib().swpc3d_set_lab(self.h, ptr(p, C.c_float)), "set_lab")
+
     def close(self):
         if getattr(self, "h", None):
             lib().swpc3d_destroy(self.h)

@@ -265,6 +265,52 @@ class Oracle3D:
         self.L.o3d_vmax(self._ps(), out.ctypes.data_as(_PF))
         return out
 
+    # -- learned absorbing boundary (o3d_lab_*) ---------------------------
+    def set_lab(self, params):
+        """Learned boundary on (params[10][NP] as given, already rounded as
+        the time step uses them) or off (None); needs na = 0."""
+        if params is None:
+            self.lab = None
+            return
+        assert self.case.grid.na == 0
+        self.lab = np.ascontiguousarray(params, np.float32)
+        self.rho = np.ascontiguousarray(self.case.medium.rho, np.float32)
+
+    def lab_fill(self, phase):
+        self.L.o3d_lab_fill(self._ps(), phase, self.lab.ctypes.data_as(_PF), self.rho.ctypes.data_as(_PF))
+
+    def lab_gather_begin(self, oi, oj, nxs, nys, nzs):
+        """Training gather for the inner box (oi, oj, nxs, nys, nzs); the
+        previous-ghost state of every net starts at zero."""
+        self.rho = np.ascontiguousarray(self.case.medium.rho, np.float32)
+        self.gdom = (oi, oj, nxs, nys, nzs)
+        ext = [nys * nzs, nys * nzs, nxs * nzs, nxs * nzs, nxs * nys]
+        self.gprev = {(f, ph): np.zeros((ext[f], 5), np.float32) for f in range(5) for ph in (0, 1)}
+
+    def lab_gather(self, phase):
+        """Records (column order) of the five faces of one phase: {face: (n, 72)}."""
+        oi, oj, nxs, nys, nzs = self.gdom
+        out = {}
+        for f in range(5):
+            pv = self.gprev[(f, phase)]
+            buf = np.zeros((len(pv), 72), np.float32)
+            n = self.L.o3d_lab_gather(self._ps(), phase, f, self.rho.ctypes.data_as(_PF), oi, oj, nxs, nys, nzs,
+                                      pv.ctypes.data_as(_PF), buf.ctypes.data_as(_PF))
+            out[f] = buf[:n]
+        return out
+
+    def lab_step(self, it):
+        """step() of a learned-boundary grid: ghost fill before each
+        half-step (the HIP step order)."""
+        self.receivers(it)
+        self.snapshots(it)
+        self.lab_fill(0)
+        self.update_stress()
+        self.stressglut(it)
+        self.lab_fill(1)
+        self.update_vel()
+        self.bodyforce(it)
+
 
 class Oracle2D:
     def __init__(self, case, bits=32):

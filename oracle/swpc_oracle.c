@@ -797,3 +797,252 @@ void oracle_zener(int nm, const float *ts, float dt, double *c1, double *c2, dou
     for (int m = 0; m < nm; m++) { c1[m] = z.c1[m]; c2[m] = z.c2[m]; d1[m] = z.d1[m]; }
     *d2 = z.d2;
 }
+
+/* ------------------------------------------------------------------ */
+/* Learned absorbing boundary                                          */
+/* ------------------------------------------------------------------ */
+/* This project's own absorbing boundary (the reference has none like it;
+   it takes the place of absorb_p__update_* on a grid without PML, na = 0),
+   restated here independently of the HIP kernels as their checker.
+   Before each half-step the ghost cells beyond every absorbing face (x-,
+   x+, y-, y+ and the bottom z+) are written with values that a small MLP
+   per face and half-step predicts; the interior update then runs over the
+   whole grid.  Net 2*face + phase: phase 0 fills the velocity ghosts read
+   by the stress update, phase 1 the stress ghosts read by the velocity
+   update.
+
+   Column features (FP32): the nine fields Vx Vy Vz Sxx Syy Szz Syz Sxz
+   Sxy at strip cells d = 1..6 (d = 1 outermost), index fi*6 + d-1, with
+   the stresses divided by the impedance rho*vp of cell d = 1; then the
+   net's previous ghost values (stresses / rho*vp, zero-padded to 5); N =
+   sqrt(sum of the squares of these 59 / 59), formed on the values scaled
+   by their largest magnitude (0 when that is below 1e-30); they are
+   divided by N; then
+   vs/vp and vp*dt/h (h the spacing normal to the face).  Net: y = Ws x +
+   W3 tanh(W2 tanh(W1 x + b1) + b2) + b3; ghost value q = y_q * N (* rho*vp
+   for a stress).  N = 0 and air columns (lam + 2 mu = 0 at d = 1) write
+   zeros. */
+#define LAB_NS 6
+#define LAB_NF 54
+#define LAB_NG 5
+#define LAB_NSC 59
+#define LAB_NIN 61
+#define LAB_NH 32
+#define LAB_NOUT 5
+#define LAB_W1 0
+#define LAB_B1 (LAB_W1 + LAB_NH * LAB_NIN)
+#define LAB_W2 (LAB_B1 + LAB_NH)
+#define LAB_B2 (LAB_W2 + LAB_NH * LAB_NH)
+#define LAB_W3 (LAB_B2 + LAB_NH)
+#define LAB_B3 (LAB_W3 + LAB_NOUT * LAB_NH)
+#define LAB_WS (LAB_B3 + LAB_NOUT)
+#define LAB_NP (LAB_WS + LAB_NOUT * LAB_NIN)
+#define LAB_REC 72
+
+/* ghost targets per net: (field id 0..8 = Vx Vy Vz Sxx Syy Szz Syz Sxz
+   Sxy, ghost depth h); the sets are the ghost cells that the 4th-order
+   stencils of o3d_update_stress (phase 0) and o3d_update_vel (phase 1)
+   read beyond each face. */
+static const int lab_tgt[10][LAB_NOUT][2] = {
+    {{0, 2}, {0, 1}, {1, 1}, {2, 1}, {0, 0}},   /* x- V */
+    {{3, 1}, {8, 2}, {8, 1}, {7, 2}, {7, 1}},   /* x- S */
+    {{0, 1}, {1, 1}, {1, 2}, {2, 1}, {2, 2}},   /* x+ V */
+    {{3, 1}, {3, 2}, {8, 1}, {7, 1}, {0, 0}},   /* x+ S */
+    {{1, 2}, {1, 1}, {0, 1}, {2, 1}, {0, 0}},   /* y- V */
+    {{4, 1}, {8, 2}, {8, 1}, {6, 2}, {6, 1}},   /* y- S */
+    {{1, 1}, {0, 1}, {0, 2}, {2, 1}, {2, 2}},   /* y+ V */
+    {{4, 1}, {4, 2}, {8, 1}, {6, 1}, {0, 0}},   /* y+ S */
+    {{2, 1}, {0, 1}, {0, 2}, {1, 1}, {1, 2}},   /* z+ V */
+    {{5, 1}, {5, 2}, {7, 1}, {6, 1}, {0, 0}},   /* z+ S */
+};
+static const int lab_nout_tab[10] = {4, 5, 5, 4, 4, 5, 5, 4, 5, 4};
+
+int oracle_lab_nparam(void) { return LAB_NP; }
+int oracle_lab_nout(int net) { return lab_nout_tab[net]; }
+void oracle_lab_target(int net, int q, int *field, int *h) { *field = lab_tgt[net][q][0]; *h = lab_tgt[net][q][1]; }
+
+static REAL *lab_field(o3d_t *s, int f)
+{
+    REAL *t[9] = {s->Vx, s->Vy, s->Vz, s->Sxx, s->Syy, s->Szz, s->Syz, s->Sxz, s->Sxy};
+    return t[f];
+}
+
+/* Columns of face f over a domain of nxs x nys x nzs cells: (a, b)
+   extents, a fastest in the column number (a-1) + nA*(b-1). */
+static void lab_extent(int f, int nxs, int nys, int nzs, int *nA, int *nB)
+{
+    if (f < 2) { *nA = nys; *nB = nzs; }
+    else if (f < 4) { *nA = nxs; *nB = nzs; }
+    else { *nA = nxs; *nB = nys; }
+}
+
+/* Domain cell (i, j, k) of column (a, b) at depth d: d >= 1 is strip cell
+   d (d = 1 outermost), d = 1 - h the ghost cell at depth h. */
+static void lab_cell(int f, int a, int b, int d, int nxs, int nys, int nzs, int *i, int *j, int *k)
+{
+    int n = f < 2 ? nxs : (f < 4 ? nys : nzs);
+    int lo = f == 0 || f == 2;   /* x- and y-; x+, y+ and the bottom z+ count inward from n */
+    int p = lo ? d : n + 1 - d;
+    if (f < 2) { *i = p; *j = a; *k = b; }
+    else if (f < 4) { *i = a; *j = p; *k = b; }
+    else { *i = a; *j = b; *k = p; }
+}
+
+/* Medium of the column (cell d = 1): vs/vp, vp*dt/h, rho*vp. */
+static void lab_medium(o3d_t *s, const float *rho, int f, size_t c0, float *m0, float *m1, float *im)
+{
+    float l2m = s->lam[c0] + 2.0f * s->mu[c0];
+    float h = (float)(f < 2 ? s->dx : (f < 4 ? s->dy : s->dz));
+    *m0 = 0.0f; *m1 = 0.0f; *im = 1.0f;
+    if (l2m > 0.0f) {
+        float vp = sqrtf(l2m / rho[c0]);
+        float vs = sqrtf(fmaxf(s->mu[c0], 0.0f) / rho[c0]);
+        *im = rho[c0] * vp;
+        *m0 = vs / vp;
+        *m1 = vp * s->dt / h;
+    }
+}
+
+/* Features of column (a, b) of face f of the domain at offset (oi, oj, 0)
+   with previous ghost values prev[5]; returns N. */
+static float lab_features(o3d_t *s, const float *rho, int f, int a, int b, int oi, int oj, int nxs, int nys, int nzs,
+                          const float *prev, float *x, float *imp)
+{
+    int i, j, k;
+    lab_cell(f, a, b, 1, nxs, nys, nzs, &i, &j, &k);
+    float m0, m1, im;
+    lab_medium(s, rho, f, I3(s, k, i + oi, j + oj), &m0, &m1, &im);
+    float amax = 0.0f;
+    for (int fi = 0; fi < 9; fi++) {
+        REAL *F = lab_field(s, fi);
+        for (int d = 1; d <= LAB_NS; d++) {
+            lab_cell(f, a, b, d, nxs, nys, nzs, &i, &j, &k);
+            float v = (float)F[I3(s, k, i + oi, j + oj)];
+            if (fi >= 3) v = v / im;
+            x[fi * LAB_NS + d - 1] = v;
+            amax = fmaxf(amax, fabsf(v));
+        }
+    }
+    for (int q = 0; q < LAB_NG; q++) {
+        x[LAB_NF + q] = prev[q];
+        amax = fmaxf(amax, fabsf(prev[q]));
+    }
+    /* N = sqrt(mean square) on values scaled by the largest magnitude;
+       columns whose largest magnitude is below 1e-30 count as empty */
+    float n = 0.0f;
+    if (amax > 1e-30f) {
+        float ia = 1.0f / amax, s2 = 0.0f;
+        for (int q = 0; q < LAB_NSC; q++) {
+            float u = x[q] * ia;
+            s2 += u * u;
+        }
+        n = amax * sqrtf(s2 / (float)LAB_NSC);
+        for (int q = 0; q < LAB_NSC; q++) x[q] = x[q] / n;
+    }
+    x[LAB_NSC] = m0;
+    x[LAB_NSC + 1] = m1;
+    *imp = im;
+    return n;
+}
+
+/* One net, FP32, sequential sums in index order. */
+void oracle_lab_forward(const float *p, const float *x, float *y)
+{
+    float h1[LAB_NH], h2[LAB_NH];
+    for (int h = 0; h < LAB_NH; h++) {
+        float z = p[LAB_B1 + h];
+        for (int q = 0; q < LAB_NIN; q++) z += p[LAB_W1 + h * LAB_NIN + q] * x[q];
+        h1[h] = tanhf(z);
+    }
+    for (int h = 0; h < LAB_NH; h++) {
+        float z = p[LAB_B2 + h];
+        for (int g = 0; g < LAB_NH; g++) z += p[LAB_W2 + h * LAB_NH + g] * h1[g];
+        h2[h] = tanhf(z);
+    }
+    for (int o = 0; o < LAB_NOUT; o++) {
+        float z = p[LAB_B3 + o];
+        for (int h = 0; h < LAB_NH; h++) z += p[LAB_W3 + o * LAB_NH + h] * h2[h];
+        for (int q = 0; q < LAB_NIN; q++) z += p[LAB_WS + o * LAB_NIN + q] * x[q];
+        y[o] = z;
+    }
+}
+
+/* Ghost fill of one phase over the whole grid: params holds the ten nets,
+   LAB_NP floats each (net 2*face + phase); the previous ghost values are
+   read back from the ghost cells before they are overwritten. */
+void o3d_lab_fill(o3d_t *s, int phase, const float *params, const float *rho)
+{
+    for (int f = 0; f < 5; f++) {
+        int net = 2 * f + phase, nA, nB, nout = lab_nout_tab[net];
+        lab_extent(f, s->nx, s->ny, s->nz, &nA, &nB);
+#pragma omp parallel for schedule(static)
+        for (int b = 1; b <= nB; b++)
+            for (int a = 1; a <= nA; a++) {
+                float x[LAB_NIN], y[LAB_NOUT], prev[LAB_NG], im, m0, m1;
+                size_t gc[LAB_NOUT];
+                int i, j, k;
+                lab_cell(f, a, b, 1, s->nx, s->ny, s->nz, &i, &j, &k);
+                lab_medium(s, rho, f, I3(s, k, i, j), &m0, &m1, &im);
+                for (int q = 0; q < LAB_NG; q++) {
+                    prev[q] = 0.0f;
+                    if (q >= nout) continue;
+                    int fi = lab_tgt[net][q][0], h = lab_tgt[net][q][1];
+                    lab_cell(f, a, b, 1 - h, s->nx, s->ny, s->nz, &i, &j, &k);
+                    gc[q] = I3(s, k, i, j);
+                    float v = (float)lab_field(s, fi)[gc[q]];
+                    prev[q] = fi >= 3 ? v / im : v;
+                }
+                float N = lab_features(s, rho, f, a, b, 0, 0, s->nx, s->ny, s->nz, prev, x, &im);
+                int live = N > 0.0f && m1 > 0.0f;   /* air columns keep zero ghosts */
+                if (live) oracle_lab_forward(params + (size_t)net * LAB_NP, x, y);
+                for (int q = 0; q < nout; q++) {
+                    int fi = lab_tgt[net][q][0];
+                    float v = live ? y[q] * N : 0.0f;
+                    if (fi >= 3) v = v * im;
+                    lab_field(s, fi)[gc[q]] = (REAL)v;
+                }
+            }
+    }
+}
+
+/* Training samples of face f, phase `phase`, for the domain of nxs x nys x
+   nzs cells at offset (oi, oj, 0) of this (wider) grid.  prev[nA*nB][5]
+   holds each column's ghost values of the previous call (in: features;
+   out: this step's values).  Every column with N > 0 gives one LAB_REC
+   record: x[61], the ghost values / N (stresses / rho*vp) in [61, 66),
+   then the column number, N and rho*vp; air columns give none.  Returns
+   the number written. */
+int o3d_lab_gather(o3d_t *s, int phase, int f, const float *rho, int oi, int oj, int nxs, int nys, int nzs,
+                   float *prev, float *out)
+{
+    int net = 2 * f + phase, nA, nB, n = 0, nout = lab_nout_tab[net];
+    lab_extent(f, nxs, nys, nzs, &nA, &nB);
+    for (int b = 1; b <= nB; b++)
+        for (int a = 1; a <= nA; a++) {
+            float x[LAB_NIN], t[LAB_NOUT], im, m0, m1;
+            float *pv = prev + (size_t)((a - 1) + nA * (b - 1)) * LAB_NG;
+            int i, j, k;
+            lab_cell(f, a, b, 1, nxs, nys, nzs, &i, &j, &k);
+            lab_medium(s, rho, f, I3(s, k, i + oi, j + oj), &m0, &m1, &im);
+            for (int q = 0; q < LAB_NOUT; q++) {
+                t[q] = 0.0f;
+                if (q >= nout) continue;
+                int fi = lab_tgt[net][q][0], h = lab_tgt[net][q][1];
+                lab_cell(f, a, b, 1 - h, nxs, nys, nzs, &i, &j, &k);
+                float v = (float)lab_field(s, fi)[I3(s, k, i + oi, j + oj)];
+                t[q] = fi >= 3 ? v / im : v;
+            }
+            float N = lab_features(s, rho, f, a, b, oi, oj, nxs, nys, nzs, pv, x, &im);
+            for (int q = 0; q < LAB_NG; q++) pv[q] = t[q];
+            if (!(N > 0.0f) || !(m1 > 0.0f)) continue;
+            float *o = out + (size_t)n * LAB_REC;
+            for (int q = 0; q < LAB_REC; q++) o[q] = 0.0f;
+            for (int q = 0; q < LAB_NIN; q++) o[q] = x[q];
+            for (int q = 0; q < LAB_NOUT; q++) o[LAB_NIN + q] = t[q] / N;
+            o[LAB_NIN + 5] = (float)((a - 1) + nA * (b - 1));
+            o[LAB_NIN + 6] = N;
+            o[LAB_NIN + 7] = im;
+            n++;
+        }
+    return n;
+}
