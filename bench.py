@@ -103,14 +103,37 @@ def main():
 
     chosen = None
     if not args.no_chosen:
+        from openswpc_amd import lab
         red = timed_run(case, CHOSEN, args.warmup, args.steps)
         ms_red = red["wall"] / args.steps * 1e3
-        mis = misfit(case, args.misfit_steps or nt)
-        chosen = {"form": "reduced storage (fp32_r16): FP32 arithmetic, V, S and medium; memory variables in FP16 "
-                          "with a power-of-two scale, PML ADE arrays in BF16; 20-cell ADE-PML",
-                  "ms_per_step": ms_red, "value": ncell * args.steps / red["wall"],
-                  "speedup_vs_plain": ms_step / ms_red, **mis}
-        log(f"chosen: {ms_red:.3f} ms/step, speedup {ms_step / ms_red:.2f}x; misfit {mis}")
+        weights = lab.load_weights()[0] if lab.WEIGHTS.exists() else None
+        icase = lab.interior_case(case) if weights is not None else None
+        labr = timed_run(icase, CHOSEN, args.warmup, args.steps, lab=weights) if weights is not None else None
+        lab32 = timed_run(icase, "fp32", args.warmup, args.steps, lab=weights) if weights is not None else None
+        mis = misfit(case, args.misfit_steps or nt, icase, weights)
+        reduced = {"form": "reduced storage (fp32_r16): FP32 arithmetic, V, S and medium; memory variables in FP16 "
+                           "with a power-of-two scale, PML ADE arrays in BF16; 20-cell ADE-PML",
+                   "ms_per_step": ms_red, "value": ncell * args.steps / red["wall"],
+                   "speedup_vs_plain": ms_step / ms_red, "misfit_vs_fp64": mis["r16"]}
+        log(f"reduced storage: {ms_red:.3f} ms/step, speedup {ms_step / ms_red:.2f}x, misfit {mis['r16']:.2e}")
+        chosen = {"misfit_steps": mis["steps"], "plain_misfit_vs_fp64": mis["fp32"], "reduced_storage": reduced}
+        if labr is not None:
+            gi = icase.grid
+            ms_lab = labr["wall"] / args.steps * 1e3
+            ms_lab32 = lab32["wall"] / args.steps * 1e3
+            chosen.update({
+                "form": f"learned absorbing boundary (per-face MLP ghost fill, BF16 weights, FP32 arithmetic) on the "
+                        f"{gi.nx}x{gi.ny}x{gi.nz} interior of the plain grid (same physical region, receivers and "
+                        f"source, no absorber cells) + reduced storage (fp32_r16)",
+                "ms_per_step": ms_lab, "cells": gi.nx * gi.ny * gi.nz,
+                "value": ncell * args.steps / labr["wall"],
+                "speedup_vs_plain": ms_step / ms_lab,
+                "misfit_vs_fp64": mis["lab_r16"],
+                "lab_fp32_ms_per_step": ms_lab32, "lab_fp32_misfit_vs_fp64": mis["lab_fp32"],
+                "lab_fill_us_per_step": (labr["ms"][1] / max(labr["cnt"][1], 1) + labr["ms"][3] / max(labr["cnt"][3], 1)) * 1e3,
+            })
+            log(f"chosen (learned boundary + r16): {ms_lab:.3f} ms/step, speedup {ms_step / ms_lab:.2f}x, "
+                f"misfit {mis['lab_r16']:.2e} (learned boundary fp32: {ms_lab32:.3f} ms/step, misfit {mis['lab_fp32']:.2e})")
 
     cpu = None
     if not args.no_cpu:
@@ -159,14 +182,17 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def timed_run(case, precision, warmup, steps):
+def timed_run(case, precision, warmup, steps, lab=None):
     """Build the context, run warmup steps, then time `steps` steps with the
-    per-kernel HIP events enabled; inputs are resident in HBM throughout."""
+    per-kernel HIP events enabled; inputs are resident in HBM throughout.
+    lab: learned-boundary nets for a grid without PML."""
     from openswpc_amd.solver import Swpc3D
     g = case.grid
     nt = warmup + steps
     sim = Swpc3D(g, case.medium, case.pml, case.glut, case.stations, case.ntdec_w, nt, precision=precision)
-    log(f"{precision}: device memory {sim.memory() / 2**30:.2f} GiB")
+    if lab is not None:
+        sim.set_learned_boundary(lab)
+    log(f"{precision}{' + learned boundary' if lab is not None else ''}: device memory {sim.memory() / 2**30:.2f} GiB")
     sim.run(1, warmup)
     sim.sync()
     sim.profile(True)
@@ -183,23 +209,30 @@ def timed_run(case, precision, warmup, steps):
     return {"wall": wall, "ms": ms, "cnt": cnt}
 
 
-def misfit(case, steps):
-    """Receiver-trace relative L2 misfit of the plain FP32 and the reduced
-    form against the FP64 run of the same scheme, over the first `steps`
-    steps of the benchmark case."""
+def misfit(case, steps, icase=None, weights=None):
+    """Receiver-trace relative L2 misfit against the FP64 run of the plain
+    scheme (20-cell PML) over the first `steps` steps of the benchmark case:
+    plain FP32, the reduced form, and the learned-boundary forms on the
+    interior grid (same receivers at the same physical positions)."""
     from openswpc_amd.solver import Swpc3D
-    g = case.grid
+    runs = [("fp64", case, "fp64", None), ("fp32", case, "fp32", None), ("r16", case, CHOSEN, None)]
+    if weights is not None:
+        runs += [("lab_r16", icase, CHOSEN, weights), ("lab_fp32", icase, "fp32", weights)]
     tr = {}
-    for prec in ("fp64", "fp32", CHOSEN):
-        sim = Swpc3D(g, case.medium, case.pml, case.glut, case.stations, case.ntdec_w, steps, precision=prec)
+    for key, c, prec, w in runs:
+        sim = Swpc3D(c.grid, c.medium, c.pml, c.glut, c.stations, c.ntdec_w, steps, precision=prec)
+        if w is not None:
+            sim.set_learned_boundary(w)
         sim.run(1, steps)
-        tr[prec] = sim.receivers()[0].astype(np.float64)
+        tr[key] = sim.receivers()[0].astype(np.float64)
         sim.close()
     ref = tr["fp64"]
     n = np.linalg.norm(ref)
-    return {"misfit_steps": steps,
-            "misfit_vs_fp64": float(np.linalg.norm(tr[CHOSEN] - ref) / n),
-            "plain_misfit_vs_fp64": float(np.linalg.norm(tr["fp32"] - ref) / n)}
+    out = {"steps": steps}
+    for key in tr:
+        if key != "fp64":
+            out[key] = float(np.linalg.norm(tr[key] - ref) / n)
+    return out
 
 
 def traffic_from_profile():

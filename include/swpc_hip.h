@@ -357,6 +357,26 @@ int swpc2d_set_field(swpc2d_ctx *ctx, int field, const double *in);
 int swpc2d_vmax(swpc2d_ctx *ctx, float *out2);
 int swpc2d_sync(swpc2d_ctx *ctx);
 
+/* ------------------------------------------------------------------ */
+/* Batches of independent runs                                         */
+/* ------------------------------------------------------------------ */
+/* No counterpart in the reference (one run per MPI job there): many 3-D
+   and P-SV contexts of uneven size on one GPU advanced together.
+   swpc_batch_run(b, it0, n, k) advances every member through steps
+   it0..it0+n-1 (each in its own step order, as swpc3d_run/swpc2d_run
+   would); k steps of all members are captured once into one HIP graph,
+   one branch per member, and replayed with one launch per k steps (k = 0:
+   step by step).  A member's setters invalidate the captured graph, which
+   is then recaptured.  Members must outlive the batch; a member may not
+   profile or use the concurrent PML stream. */
+typedef struct swpc_batch swpc_batch;
+int swpc_batch_create(swpc_batch **out);
+int swpc_batch_destroy(swpc_batch *b);
+int swpc_batch_add3d(swpc_batch *b, swpc3d_ctx *ctx);
+int swpc_batch_add2d(swpc_batch *b, swpc2d_ctx *ctx);
+int swpc_batch_run(swpc_batch *b, int it0, int nsteps, int graph_k);
+int swpc_batch_sync(swpc_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -81,6 +81,12 @@ _PROTOS = {
     "swpc_lab_samples": (I, [P, I, I, I, PF]),
     "swpc_lab_train": (I, [P, I, PF, I, I, C.c_float, C.c_float, C.c_uint, PF]),
     "swpc_lab_grad": (I, [P, PF, I, I, PF, PD, PF]),
+    "swpc_batch_create": (I, [C.POINTER(P)]),
+    "swpc_batch_destroy": (I, [P]),
+    "swpc_batch_add3d": (I, [P, P]),
+    "swpc_batch_add2d": (I, [P, P]),
+    "swpc_batch_run": (I, [P, I, I, I]),
+    "swpc_batch_sync": (I, [P]),
     "swpc2d_create": (I, [C.POINTER(Desc2), C.POINTER(P)]),
     "swpc2d_destroy": (I, [P]),
     "swpc2d_set_medium": (I, [P] + [PF] * 5),

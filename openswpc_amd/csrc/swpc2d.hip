@@ -11,6 +11,7 @@
 // One thread per cell in an x-fastest (k, i) layout: a P-SV grid has too
 // few columns to stream z per thread and still fill 256 CUs.
 #include "swpc_common.h"
+#include "swpc_batch.h"
 
 #include <algorithm>
 #include <memory>
@@ -401,6 +402,7 @@ struct swpc2d_ctx {
     int ti0 = 1, ntx = 0, ntz = 0;
     int graph_k = 16;       // steps per captured step graph (0 = launch step by step)
     hipGraph_t graph = nullptr;
+    unsigned gen = 0;       // bumped by every graph drop (batches recapture on change)
     hipGraphExec_t gexec = nullptr;
     swpc::DBuf dit;         // device step number
     struct Snap {
@@ -581,6 +583,7 @@ static int step2_any(swpc2d_ctx *c) { return fp64(c) ? step2_dev<double>(c) : st
 
 static void graph_drop2(swpc2d_ctx *c)
 {
+    c->gen++;
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     c->gexec = nullptr;
@@ -630,6 +633,29 @@ static int run_steps2(swpc2d_ctx *c, int it0, int n)
     note_snapshots2(c, it0, it0 + n - 1);
     return SWPC_OK;
 }
+
+// Hooks of swpc_batch.hip.
+int batch_member2(swpc2d_ctx *c, BatchMember &m)
+{
+    if (int r = ready2(c)) return r;
+    m.stream = c->stream;
+    m.dit = (int *)c->dit.p;
+    m.gen = c->gen;
+    return SWPC_OK;
+}
+
+int batch_check2(swpc2d_ctx *c, int it0, int n)
+{
+    if (c->nsrc > 0 && it0 + n - 1 > c->nt_src)
+        return fail(SWPC_EINVAL, "stressglut: steps %d..%d outside 1..%d", it0, it0 + n - 1, c->nt_src);
+    if (c->nbf > 0 && it0 + n - 1 > c->nt_bf)
+        return fail(SWPC_EINVAL, "bodyforce: steps %d..%d outside 1..%d", it0, it0 + n - 1, c->nt_bf);
+    return SWPC_OK;
+}
+
+int batch_step2(swpc2d_ctx *c) { return step2_any(c); }
+
+void batch_done2(swpc2d_ctx *c, int it0, int it1) { note_snapshots2(c, it0, it1); }
 
 template <typename S, typename D>
 static void box2_to_dev(const Lay2 &L, const S *src, D *dst)

@@ -21,6 +21,7 @@
 // is a gather, not a contraction, and is bound by HBM bandwidth.
 #include "swpc_common.h"
 #include "swpc_lab.h"
+#include "swpc_batch.h"
 
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
@@ -1817,6 +1818,7 @@ struct swpc3d_ctx {
     int graph_k = 16;       // steps per captured step graph (0 = launch step by step)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    unsigned gen = 0;       // bumped by every graph drop (batches recapture on change)
     swpc::DBuf dit;         // device step number (int)
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -2374,6 +2376,7 @@ static int step_dev_any(swpc3d_ctx *c) { return fp64(c) ? step_dev<double>(c) : 
 
 static void graph_drop(swpc3d_ctx *c)
 {
+    c->gen++;
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     c->gexec = nullptr;
@@ -2429,6 +2432,30 @@ static int run_steps(swpc3d_ctx *c, int it0, int n)
     note_snapshots(c, it0, it0 + n - 1);
     return SWPC_OK;
 }
+
+// Hooks of swpc_batch.hip.
+int batch_member3(swpc3d_ctx *c, BatchMember &m)
+{
+    if (int r = ready(c)) return r;
+    if (c->prof.on || c->conc) return fail(SWPC_ESTATE, "a batch member may not profile or run its PML on a second stream");
+    m.stream = c->stream;
+    m.dit = (int *)c->dit.p;
+    m.gen = c->gen;
+    return SWPC_OK;
+}
+
+int batch_check3(swpc3d_ctx *c, int it0, int n)
+{
+    if (c->nsrc > 0 && it0 + n - 1 > c->nt_src)
+        return fail(SWPC_EINVAL, "stressglut: steps %d..%d outside 1..%d", it0, it0 + n - 1, c->nt_src);
+    if (c->nbf > 0 && it0 + n - 1 > c->nt_bf)
+        return fail(SWPC_EINVAL, "bodyforce: steps %d..%d outside 1..%d", it0, it0 + n - 1, c->nt_bf);
+    return SWPC_OK;
+}
+
+int batch_step3(swpc3d_ctx *c) { return step_dev_any(c); }
+
+void batch_done3(swpc3d_ctx *c, int it0, int it1) { note_snapshots(c, it0, it1); }
 
 // Fortran (k,i,j) memory box -> device layout (row-blocked transposition).
 template <typename S, typename D>

@@ -1,0 +1,200 @@
+// Batches of independent runs on one GPU: many 3-D and P-SV contexts of
+// uneven size advanced together, graph_k steps of every member captured
+// once into one HIP graph (one branch per member, joined at the end) and
+// replayed with one launch per graph_k steps.  Each member keeps its own
+// stream, step counter and state, so results equal the members' own runs;
+// the graph removes the per-kernel launch cost of the many small grids
+// and lets the runtime overlap the branches.
+#include "swpc_batch.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace swpc {
+
+__global__ void k_batch_set_steps(int *const *__restrict__ dits, int n, int it)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) *dits[q] = it;
+}
+
+}  // namespace swpc
+
+struct swpc_batch {
+    struct M {
+        swpc3d_ctx *c3 = nullptr;
+        swpc2d_ctx *c2 = nullptr;
+        swpc::BatchMember m;
+        hipEvent_t ev = nullptr;
+        unsigned gen_captured = 0;
+    };
+    std::vector<M> mem;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_k = 0;             // steps in the captured graph
+    swpc::DBuf dits;             // device array of the members' step counters
+    long long bytes = 0;
+};
+
+namespace swpc {
+
+static void batch_drop(swpc_batch *b)
+{
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    b->graph_k = 0;
+}
+
+static int member_info(swpc_batch::M &x)
+{
+    return x.c3 ? batch_member3(x.c3, x.m) : batch_member2(x.c2, x.m);
+}
+
+static int member_step(swpc_batch::M &x) { return x.c3 ? batch_step3(x.c3) : batch_step2(x.c2); }
+
+// Capture k steps of every member: the batch stream forks to each member's
+// stream (an event each), the member enqueues its k steps there, and the
+// batch stream joins them all.
+static int batch_capture(swpc_batch *b, int k)
+{
+    batch_drop(b);
+    SWPC_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int r = SWPC_OK;
+    hipError_t e = hipEventRecord(b->ev_fork, b->stream);
+    for (auto &x : b->mem) {
+        if (r != SWPC_OK || e != hipSuccess) break;
+        e = hipStreamWaitEvent(x.m.stream, b->ev_fork, 0);
+        for (int q = 0; q < k && r == SWPC_OK && e == hipSuccess; q++) r = member_step(x);
+        if (r == SWPC_OK && e == hipSuccess) e = hipEventRecord(x.ev, x.m.stream);
+        if (r == SWPC_OK && e == hipSuccess) e = hipStreamWaitEvent(b->stream, x.ev, 0);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(b->stream, &g);
+    if (r != SWPC_OK || e != hipSuccess || ee != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        if (r != SWPC_OK) return r;
+        return fail(SWPC_EHIP, "batch graph capture failed: %s", hipGetErrorString(e != hipSuccess ? e : ee));
+    }
+    hipGraphExec_t ge = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(SWPC_EHIP, "batch graph instantiation failed: %s", hipGetErrorString(ei));
+    }
+    b->graph = g;
+    b->gexec = ge;
+    b->graph_k = k;
+    for (auto &x : b->mem) x.gen_captured = x.m.gen;
+    return SWPC_OK;
+}
+
+}  // namespace swpc
+
+using namespace swpc;
+
+extern "C" {
+
+int swpc_batch_create(swpc_batch **out)
+{
+    clear_error();
+    if (!out) return fail(SWPC_EINVAL, "null argument");
+    *out = nullptr;
+    auto *b = new swpc_batch();
+    if (hipStreamCreate(&b->stream) != hipSuccess || hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess) {
+        swpc_batch_destroy(b);
+        return fail(SWPC_EHIP, "batch stream/event creation failed");
+    }
+    *out = b;
+    return SWPC_OK;
+}
+
+int swpc_batch_destroy(swpc_batch *b)
+{
+    if (!b) return SWPC_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_drop(b);
+    for (auto &x : b->mem)
+        if (x.ev) (void)hipEventDestroy(x.ev);
+    dfree(b->dits);
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return SWPC_OK;
+}
+
+static int batch_add(swpc_batch *b, swpc3d_ctx *c3, swpc2d_ctx *c2)
+{
+    clear_error();
+    if (!b || (!c3 && !c2)) return fail(SWPC_EINVAL, "null argument");
+    for (auto &x : b->mem)
+        if ((c3 && x.c3 == c3) || (c2 && x.c2 == c2)) return fail(SWPC_EINVAL, "context already in the batch");
+    swpc_batch::M x;
+    x.c3 = c3;
+    x.c2 = c2;
+    if (int r = member_info(x)) return r;
+    SWPC_HIP(hipEventCreateWithFlags(&x.ev, hipEventDisableTiming));
+    b->mem.push_back(x);
+    batch_drop(b);
+    std::vector<int *> d;
+    for (auto &y : b->mem) d.push_back(y.m.dit);
+    if (int r = dalloc(b->dits, d.size() * sizeof(int *), b->bytes)) return r;
+    SWPC_HIP(hipMemcpy(b->dits.p, d.data(), d.size() * sizeof(int *), hipMemcpyHostToDevice));
+    return SWPC_OK;
+}
+
+int swpc_batch_add3d(swpc_batch *b, swpc3d_ctx *c) { return batch_add(b, c, nullptr); }
+int swpc_batch_add2d(swpc_batch *b, swpc2d_ctx *c) { return batch_add(b, nullptr, c); }
+
+int swpc_batch_run(swpc_batch *b, int it0, int nsteps, int graph_k)
+{
+    clear_error();
+    if (!b || nsteps < 0 || graph_k < 0) return fail(SWPC_EINVAL, "bad argument");
+    if (b->mem.empty() || nsteps == 0) return SWPC_OK;
+    bool stale = false;
+    for (auto &x : b->mem) {
+        if (int r = member_info(x)) return r;
+        if (int r = x.c3 ? batch_check3(x.c3, it0, nsteps) : batch_check2(x.c2, it0, nsteps)) return r;
+        stale |= x.m.gen != x.gen_captured;
+    }
+    // order after every member's earlier work, then set all step counters
+    for (auto &x : b->mem) {
+        SWPC_HIP(hipEventRecord(x.ev, x.m.stream));
+        SWPC_HIP(hipStreamWaitEvent(b->stream, x.ev, 0));
+    }
+    const int n = (int)b->mem.size();
+    hipLaunchKernelGGL(k_batch_set_steps, dim3((n + 63) / 64), dim3(64), 0, b->stream, (int *const *)b->dits.p, n, it0);
+    SWPC_CHECK_LAUNCH();
+    int done = 0;
+    if (graph_k > 0 && nsteps >= graph_k) {
+        if (!b->gexec || stale || b->graph_k != graph_k)
+            if (int r = batch_capture(b, graph_k)) return r;
+        for (; done + graph_k <= nsteps; done += graph_k) SWPC_HIP(hipGraphLaunch(b->gexec, b->stream));
+    }
+    // the remaining steps go member by member on their own streams
+    SWPC_HIP(hipEventRecord(b->ev_join, b->stream));
+    for (auto &x : b->mem) {
+        SWPC_HIP(hipStreamWaitEvent(x.m.stream, b->ev_join, 0));
+        for (int q = done; q < nsteps; q++)
+            if (int r = member_step(x)) return r;
+        if (x.c3) batch_done3(x.c3, it0, it0 + nsteps - 1);
+        else batch_done2(x.c2, it0, it0 + nsteps - 1);
+    }
+    return SWPC_OK;
+}
+
+int swpc_batch_sync(swpc_batch *b)
+{
+    clear_error();
+    if (!b) return fail(SWPC_EINVAL, "null batch");
+    SWPC_HIP(hipStreamSynchronize(b->stream));
+    for (auto &x : b->mem) SWPC_HIP(hipStreamSynchronize(x.m.stream));
+    return SWPC_OK;
+}
+
+}  // extern "C"

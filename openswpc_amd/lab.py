@@ -273,22 +273,34 @@ def train_boundary(nruns=200, nxs=64, nys=64, nzs=48, dx=0.25, dt=0.01, nt=500, 
     a random double couple), gather their ghost targets, and fit the ten
     nets.  Returns (params[10][NP], report)."""
     import time
+    from concurrent.futures import ThreadPoolExecutor
     t0 = time.time()
     tr = Trainer(cap)
     quota = cap // nruns
     rng = np.random.default_rng(seed)
+    specs = []
     for r in range(nruns):
-        s = int(seed) * 100_003 + r
         if r % 2 == 0:
             layers, zbeg = problem.LHM_EXAMPLE, float(rng.uniform(-0.75, 50.0))
         else:
             layers, zbeg = random_layers(rng), -0.75
-        zbeg = float(np.float32(round(zbeg / dx) * dx))
-        lc = wide_case(nxs, nys, nzs, dx, dt, nt, s, zbeg=zbeg, layers=layers)
-        gather_run(tr, lc, quota, keep, thr, s)
-        if log and (r % 20 == 19 or r == nruns - 1):
-            log(f"lab: {r + 1}/{nruns} wide runs gathered, samples per net {tr.counts().tolist()}, "
-                f"{time.time() - t0:.0f} s")
+        specs.append((int(seed) * 100_003 + r, layers, float(np.float32(round(zbeg / dx) * dx))))
+    # the host builds the next cases (FFT random media, medium__setup)
+    # while the GPU runs the current one
+    with ThreadPoolExecutor(max_workers=4) as pool:
+        futs = [pool.submit(wide_case, nxs, nys, nzs, dx, dt, nt, s, zbeg, 0.05, 2.0, layers)
+                for s, layers, zbeg in specs[:4]]
+        for r in range(nruns):
+            lc = futs[r].result()
+            futs[r] = None
+            if r + 4 < nruns:
+                s, layers, zbeg = specs[r + 4]
+                futs.append(pool.submit(wide_case, nxs, nys, nzs, dx, dt, nt, s, zbeg, 0.05, 2.0, layers))
+            gather_run(tr, lc, quota, keep, thr, specs[r][0])
+            del lc
+            if log and (r % 20 == 19 or r == nruns - 1):
+                log(f"lab: {r + 1}/{nruns} wide runs gathered, samples per net {tr.counts().tolist()}, "
+                    f"{time.time() - t0:.0f} s")
     counts = tr.counts()
     P = init_params(seed)
     final = []
@@ -410,3 +422,31 @@ def _main(argv=None):
 
 if __name__ == "__main__":
     _main()
+
+
+def interior_case(case: problem.Case3D):
+    """A case with a PML of na cells restated as the learned-boundary grid
+    that covers the same physical region: its interior box (nx - 2na) x
+    (ny - 2na) x (nz - na) with na = 0, the same assembled medium, the
+    source cells shifted by the absorber width and the receivers at the
+    same physical positions."""
+    from .source import SourceTable
+    g0, m0 = case.grid, case.medium
+    na = g0.na
+    g = md.Grid3D(g0.nx - 2 * na, g0.ny - 2 * na, g0.nz - na, g0.dx, g0.dy, g0.dz, g0.dt, na=0,
+                  xbeg=float(np.float32(g0.xbeg + na * np.float32(g0.dx))),
+                  ybeg=float(np.float32(g0.ybeg + na * np.float32(g0.dy))), zbeg=g0.zbeg)
+    sj, si, sk = slice(na, na + g.ny + 6), slice(na, na + g.nx + 6), slice(0, g.nz + 6)
+    kw = {n: np.ascontiguousarray(getattr(m0, n)[sj, si, sk])
+          for n in ("rho", "lam", "mu", "taup", "taus", "bx", "by", "bz", "muyz", "muxz", "muxy")}
+    kw.update({n: np.ascontiguousarray(getattr(m0, n)[sj, si])
+               for n in ("kfs", "kob", "kfs_top", "kfs_bot", "kob_top", "kob_bot")})
+    med = md.Medium3D(**kw, ts=m0.ts, zeta=m0.zeta, vmin=m0.vmin, vmax=m0.vmax)
+    gl = case.glut
+    glut = SourceTable(np.ascontiguousarray(gl.idx - np.array([na, na, 0], np.int32)), gl.m, gl.table, gl.M0,
+                       gl.fcut)
+    cells, kept = station_cells3(g, med.kfs, med.kob, case.station_list)
+    if len(kept) != len(case.station_list):
+        raise ValueError("a receiver falls outside the interior box")
+    prof = tuple(np.zeros((n, 4), np.float32) for n in (g.nx, g.nx, g.ny, g.ny, g.nz, g.nz))
+    return problem.Case3D(g, med, prof, glut, cells, kept, case.nt, case.ntdec_w, case.mech)

@@ -68,7 +68,7 @@ def mechanism(seed):
 
 def case3d(nx, ny, nz, dx, dt, nt, na=20, layers=LHM_EXAMPLE, seed=1, rmed=None, src_depth=10.0,
            tr=1.0, t0=0.1, mo=1e15, nst_side=8, st_spacing=None, ntdec_w=1, rmed_tile=(128, 128, 64),
-           benchmark=False, zbeg=None, bodyforce=False):
+           benchmark=False, zbeg=None, bodyforce=False, src_xy=(0.0, 0.0)):
     """3-D case: layered (+ random) medium, double couple at src_depth below
     the grid centre, an nst_side x nst_side receiver grid on the free surface.
     bodyforce=True replaces the double couple by a single force of the same
@@ -88,7 +88,7 @@ def case3d(nx, ny, nz, dx, dt, nt, na=20, layers=LHM_EXAMPLE, seed=1, rmed=None,
     rho, lam, mu, qp, qs = md.layered3(g, layers, xi=xi)
     med = md.medium_setup3(g, rho, lam, mu, qp, qs, benchmark=benchmark)
     strike, dip, rake = mechanism(seed)
-    src = PointSource(0.0, 0.0, src_depth, t0, tr, mo, fdtool.sdr2moment(strike, dip, rake))
+    src = PointSource(float(src_xy[0]), float(src_xy[1]), src_depth, t0, tr, mo, fdtool.sdr2moment(strike, dip, rake))
     glut = stress_glut3(g, [src], nt)
     bforce = None
     if bodyforce:
@@ -109,7 +109,7 @@ def case3d(nx, ny, nz, dx, dt, nt, na=20, layers=LHM_EXAMPLE, seed=1, rmed=None,
 
 
 def case2d(nx, nz, dx, dt, nt, na=20, layers=LHM_TWO, seed=1, rmed=None, src_depth=10.0, tr=1.0, t0=0.1,
-           mo=1e15, nst=16, st_spacing=2.0, ntdec_w=1, benchmark=False, zbeg=None, bodyforce=False):
+           mo=1e15, nst=16, st_spacing=2.0, ntdec_w=1, benchmark=False, zbeg=None, bodyforce=False, src_x=0.0):
     """P-SV case: layered (+ random) medium, double couple (its x-z part) at
     src_depth, nst receivers on the free surface at st_spacing km."""
     g = md.Grid2D(nx, nz, dx, dx, dt, na=na, zbeg=zbeg)
@@ -121,7 +121,7 @@ def case2d(nx, nz, dx, dt, nt, na=20, layers=LHM_TWO, seed=1, rmed=None, src_dep
     rho, lam, mu, qp, qs = md.layered2(g, layers, xi=xi)
     med = md.medium_setup2(g, rho, lam, mu, qp, qs, benchmark=benchmark)
     strike, dip, rake = mechanism(seed)
-    src = PointSource(0.0, 0.0, src_depth, t0, tr, mo, fdtool.sdr2moment(strike, dip, rake))
+    src = PointSource(float(src_x), 0.0, src_depth, t0, tr, mo, fdtool.sdr2moment(strike, dip, rake))
     glut = stress_glut2(g, [src], nt)
     bforce = None
     if bodyforce:
