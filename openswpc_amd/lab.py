@@ -9,9 +9,12 @@ small MLP per face and half-step from the six outermost cells of each
 column and the net's own previous ghost values (the definition is stated
 in openswpc_amd/csrc/swpc_lab.h and include/swpc_hip.h).
 
-Training data come from the solver itself: wide-PML runs (60 cells) whose
-inner box stands for a grid with a learned boundary; the wide run's true
-fields in that box's ghost cells are the targets.  ``Trainer`` holds the
+Training data come from the solver itself: reference runs on a grid 110
+cells wider at each absorbing face, with reflecting edges far enough away
+that no echo of them re-enters the inner box within the run (an
+echo-free stand-in for the unbounded medium); the inner box stands for a
+grid with a learned boundary, and the reference run's fields in that
+box's ghost cells are the targets.  ``Trainer`` holds the
 samples on the GPU and fits the nets with the library's own forward,
 backward and Adam kernels.
 """
@@ -33,8 +36,8 @@ NOUT = (4, 5, 5, 4, 4, 5, 5, 4, 5, 4)
 NIN, NH, NO = 61, 32, 5
 NSC = 59                        # features scaled by 1/N
 REC = 72                        # floats per training sample
-WIDE_NA = 60                    # PML of the training runs
-OFFSET = 62                     # inner box offset: its ghosts sit outside the wide PML
+REF_NA = 0                      # reference runs: no absorber, the edges are far away
+OFFSET = 110                    # inner box offset inside the reference grid (cells)
 WEIGHTS = Path(__file__).resolve().parent / "data" / "lab_weights.npz"
 
 
@@ -157,28 +160,35 @@ class Trainer:
 
 
 # ---------------------------------------------------------------------------
-# cases: wide-PML runs and the learned-boundary grid inside them
+# cases: reference runs and the learned-boundary grid inside them
 # ---------------------------------------------------------------------------
 @dataclass
 class LabCase:
-    wide: problem.Case3D       # the training / reference run (60-cell PML)
+    wide: problem.Case3D       # the reference run (training targets, truth of the evaluation)
     src: PointSource
     nxs: int
     nys: int
     nzs: int
     raw: tuple = None          # (rho, lam, mu, qp, qs) of the wide grid before medium__setup
+    off: int = OFFSET          # inner box offset inside the wide grid
 
 
-def wide_case(nxs, nys, nzs, dx, dt, nt, seed, zbeg=-0.75, eps=0.05, a=2.0, layers=problem.LHM_EXAMPLE):
-    """A run on a grid of (nxs + 2*OFFSET) x (nys + 2*OFFSET) x (nzs +
-    OFFSET) cells with a 60-cell PML: the layered medium of ``layers`` with
-    a von Karman perturbation (eps, a km, kappa 0.5) and a double couple of
-    random position, depth and mechanism inside the inner nxs x nys x nzs
-    box, all drawn from the seed."""
-    g = md.Grid3D(nxs + 2 * OFFSET, nys + 2 * OFFSET, nzs + OFFSET, dx, dx, dx, dt, na=WIDE_NA, zbeg=zbeg)
-    tile = tuple(min(128, n) for n in (g.nx, g.ny, g.nz))
-    vol = rmedia.generate(tile, dx, a, eps, 0.5, seed=seed)
-    xi = md.rmed_tile(vol, -2, g.nx + 3, -2, g.ny + 3, -2, g.nz + 3)
+def wide_case(nxs, nys, nzs, dx, dt, nt, seed, zbeg=-0.75, eps=0.05, a=2.0, layers=problem.LHM_EXAMPLE,
+              off=OFFSET, na=REF_NA):
+    """The reference run of one training or evaluation case: a grid of
+    (nxs + 2*off) x (nys + 2*off) x (nzs + off) cells (absorber of na cells,
+    none by default) holding the inner nxs x nys x nzs box at (off, off, 0);
+    the layered medium of ``layers`` with a von Karman perturbation (eps, a
+    km, kappa 0.5) and a double couple of random position, depth and
+    mechanism inside the inner box, all drawn from the seed.  With na = 0
+    the run stands for the unbounded medium while no echo of its edges can
+    reach the inner box: 2*off cells of travel, i.e. nt*vmax*dt/dx < 2*off."""
+    g = md.Grid3D(nxs + 2 * off, nys + 2 * off, nzs + off, dx, dx, dx, dt, na=na, zbeg=zbeg)
+    xi = None
+    if eps > 0:
+        tile = tuple(min(128, n) for n in (g.nx, g.ny, g.nz))
+        vol = rmedia.generate(tile, dx, a, eps, 0.5, seed=seed)
+        xi = md.rmed_tile(vol, -2, g.nx + 3, -2, g.ny + 3, -2, g.nz + 3)
     rho, lam, mu, qp, qs = md.layered3(g, layers, xi=xi)
     med = md.medium_setup3(g, rho, lam, mu, qp, qs)
     rng = np.random.default_rng(seed + 7919)
@@ -188,9 +198,12 @@ def wide_case(nxs, nys, nzs, dx, dt, nt, seed, zbeg=-0.75, eps=0.05, a=2.0, laye
     z = float(rng.uniform(max(1.0, zbeg + 8 * dx), zbeg + (nzs - 8) * dx))
     src = PointSource(x, y, z, 0.1, 1.0, 1e15, fdtool.sdr2moment(*mech))
     glut = stress_glut3(g, [src], nt)
-    prof = pml.pml3(g, glut.fcut)
+    if na > 0:
+        prof = pml.pml3(g, glut.fcut)
+    else:
+        prof = tuple(np.zeros((n, 4), np.float32) for n in (g.nx, g.nx, g.ny, g.ny, g.nz, g.nz))
     case = problem.Case3D(g, med, prof, glut, np.zeros((0, 3), np.int32), [], nt, 1, mech)
-    return LabCase(case, src, nxs, nys, nzs, (rho, lam, mu, qp, qs))
+    return LabCase(case, src, nxs, nys, nzs, (rho, lam, mu, qp, qs), off)
 
 
 def _stations(lc: LabCase, nst_side):
@@ -211,7 +224,7 @@ def inner_case(lc: LabCase, na=0, nst_side=0, grow=0):
     the absorber as the reference does (m_medium.f90:124-190)."""
     wg, wm = lc.wide.grid, lc.wide.medium
     nx, ny, nz = lc.nxs + 2 * grow, lc.nys + 2 * grow, lc.nzs + grow
-    o = OFFSET - grow
+    o = lc.off - grow
     g = md.Grid3D(nx, ny, nz, wg.dx, wg.dy, wg.dz, wg.dt, na=na, zbeg=wg.zbeg)
     sj, si, sk = slice(o, o + ny + 6), slice(o, o + nx + 6), slice(0, nz + 6)
     if na > 0 and lc.raw is not None:
@@ -234,12 +247,12 @@ def inner_case(lc: LabCase, na=0, nst_side=0, grow=0):
 
 
 def gather_run(trainer, lc: LabCase, quota=None, keep=1.0, thr=1e-3, seed=0, precision="fp32"):
-    """One wide-PML run on the GPU with the inner box's ghost targets and
+    """One reference run on the GPU with the inner box's ghost targets and
     features gathered at every half-step."""
     from .solver import Swpc3D
     c = lc.wide
     with Swpc3D(c.grid, c.medium, c.pml, c.glut, None, 1, c.nt, precision=precision) as sim:
-        trainer.begin_run(sim, OFFSET, OFFSET, lc.nxs, lc.nys, lc.nzs, quota)
+        trainer.begin_run(sim, lc.off, lc.off, lc.nxs, lc.nys, lc.nzs, quota)
         for it in range(1, c.nt + 1):
             s = (seed * 1_000_003 + it) & 0xFFFFFFFF
             trainer.gather(sim, 0, keep, s, thr)
@@ -267,7 +280,7 @@ def random_layers(rng):
 
 def train_boundary(nruns=200, nxs=64, nys=64, nzs=48, dx=0.25, dt=0.01, nt=500, cap=2_000_000, steps=6000,
                    batch=8192, lr=3e-3, noise=0.1, keep=0.012, thr=1e-3, seed=1, log=print):
-    """Generate nruns wide-PML runs on the GPU (even runs: the example's
+    """Generate nruns reference runs on the GPU (even runs: the example's
     layered model at a random depth offset; odd runs: a random layered
     model under a free surface; all with a 5 % von Karman perturbation and
     a random double couple), gather their ghost targets, and fit the ten
@@ -287,7 +300,7 @@ def train_boundary(nruns=200, nxs=64, nys=64, nzs=48, dx=0.25, dt=0.01, nt=500, 
         specs.append((int(seed) * 100_003 + r, layers, float(np.float32(round(zbeg / dx) * dx))))
     # the host builds the next cases (FFT random media, medium__setup)
     # while the GPU runs the current one
-    with ThreadPoolExecutor(max_workers=4) as pool:
+    with ThreadPoolExecutor(max_workers=6) as pool:
         futs = [pool.submit(wide_case, nxs, nys, nzs, dx, dt, nt, s, zbeg, 0.05, 2.0, layers)
                 for s, layers, zbeg in specs[:4]]
         for r in range(nruns):
@@ -319,10 +332,10 @@ def _inner(a, nxs, nys, nzs, off=0):
     return a[off + 3:off + 3 + nys, off + 3:off + 3 + nxs, 3:3 + nzs]
 
 
-def evaluate_boundary(params, seed=101, nxs=64, nys=64, nzs=48, dx=0.25, dt=0.01, nt=600, every=20, zbeg=-0.75,
-                      layers=problem.LHM_EXAMPLE, margin=6, extra=(20,)):
+def evaluate_boundary(params, seed=101, nxs=64, nys=64, nzs=48, dx=0.25, dt=0.01, nt=500, every=20, zbeg=-0.75,
+                      layers=problem.LHM_EXAMPLE, margin=6, extra=(20,), eps=0.05):
     """One held-out case run several ways on the GPU, measured against the
-    wide-PML run (the truth) over one region U: the inner box minus
+    reference run (the truth) over one region U: the inner box minus
     ``margin`` cells at each absorbing face.
       lab      the inner box with the learned boundary (na = 0)
       zero     the same grid with zero ghosts (the boundary without a net)
@@ -336,7 +349,7 @@ def evaluate_boundary(params, seed=101, nxs=64, nys=64, nzs=48, dx=0.25, dt=0.01
     ms_per_step."""
     import time
     from .solver import Swpc3D
-    lc = wide_case(nxs, nys, nzs, dx, dt, nt, seed, zbeg=zbeg, layers=layers)
+    lc = wide_case(nxs, nys, nzs, dx, dt, nt, seed, zbeg=zbeg, layers=layers, eps=eps)
     wst, _ = station_cells3(lc.wide.grid, lc.wide.medium.kfs, lc.wide.medium.kob, _stations(lc, 4))
     runs = {"truth": (lc.wide, None, 0, wst)}
     c = inner_case(lc, 0, nst_side=4)
@@ -353,7 +366,7 @@ def evaluate_boundary(params, seed=101, nxs=64, nys=64, nzs=48, dx=0.25, dt=0.01
         sims[k] = Swpc3D(cs.grid, cs.medium, cs.pml, cs.glut, st, 1, nt)
         if p is not None:
             sims[k].set_learned_boundary(p)
-        grow[k] = OFFSET if k == "truth" else gr
+        grow[k] = lc.off if k == "truth" else gr
     def region(k, f):
         o = grow[k]
         a = sims[k].field(f)
@@ -408,7 +421,7 @@ def _main(argv=None):
     e = sub.add_parser("eval")
     e.add_argument("--weights", default=str(WEIGHTS))
     e.add_argument("--seed", type=int, default=101)
-    e.add_argument("--nt", type=int, default=600)
+    e.add_argument("--nt", type=int, default=500)
     a = ap.parse_args(argv)
     if a.cmd == "train":
         P, rep = train_boundary(nruns=a.runs, steps=a.steps, nt=a.nt, noise=a.noise, seed=a.seed)

@@ -68,7 +68,9 @@ def test_grad_matches_fp64_oracle():
 
 
 def _small_wide(seed=5, nt=60):
-    return lab.wide_case(24, 20, 18, 0.25, 0.01, nt, seed, zbeg=-0.75)
+    # a small offset keeps the field copies of the gather check cheap; the
+    # reference grid then echoes into the inner box, which parity does not mind
+    return lab.wide_case(24, 20, 18, 0.25, 0.01, nt, seed, zbeg=-0.75, off=12)
 
 
 def test_gather_matches_oracle():
@@ -81,11 +83,11 @@ def test_gather_matches_oracle():
     lc = _small_wide(nt=nt)
     c = lc.wide
     o = Oracle3D(c, 32)
-    o.lab_gather_begin(lab.OFFSET, lab.OFFSET, lc.nxs, lc.nys, lc.nzs)
+    o.lab_gather_begin(lc.off, lc.off, lc.nxs, lc.nys, lc.nzs)
     tr = lab.Trainer(100_000)
     nrec = 0
     with Swpc3D(c.grid, c.medium, c.pml, c.glut, None, 1, nt) as sim:
-        tr.begin_run(sim, lab.OFFSET, lab.OFFSET, lc.nxs, lc.nys, lc.nzs)
+        tr.begin_run(sim, lc.off, lc.off, lc.nxs, lc.nys, lc.nzs)
         for it in range(1, nt + 1):
             for ph in (0, 1):
                 for n in F:
@@ -170,13 +172,13 @@ def test_training_lowers_loss():
 
 def test_trained_boundary_beats_no_net():
     """The committed nets on a held-out case (seed not used in training):
-    stable over 600 steps and far closer to the wide-PML truth than the
+    stable over 500 steps and far closer to the echo-free reference run than the
     same grid with zero ghosts; the 6- and 20-cell PML figures are
     reported beside it."""
     if not lab.WEIGHTS.exists():
         pytest.skip("no trained weights committed")
     P, meta = lab.load_weights()
-    res = lab.evaluate_boundary(P, seed=424242, nt=600)
+    res = lab.evaluate_boundary(P, seed=424242, nt=500)
     for k, v in res.items():
         print(k, {a: (round(b, 4) if isinstance(b, float) else b) for a, b in v.items()})
     assert np.isfinite(res["lab"]["rms_rel"])
