@@ -55,6 +55,9 @@ def main():
     ap.add_argument("--no-chosen", action="store_true")
     ap.add_argument("--misfit-steps", type=int, default=0,
                     help="steps of the misfit runs (0: the full warmup + timed length)")
+    ap.add_argument("--config", choices=("c2", "c3"), default="c2",
+                    help="c2: the 512x512x256 bench line (default); c3: the 1024x1024x512 reduced-storage "
+                         "run against an FP64 run of the same scheme (one JSON line of its own)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -65,6 +68,8 @@ def main():
     from openswpc_amd import problem
     from openswpc_amd.solver import Swpc3D
 
+    if args.config == "c3":
+        return run_c3(args)
     nt = args.warmup + args.steps
     t0 = time.time()
     case = problem.case3d(args.nx, args.ny, args.nz, 0.25, 0.01, nt, na=20, seed=args.seed,
@@ -233,6 +238,54 @@ def misfit(case, steps, icase=None, weights=None):
         if key != "fp64":
             out[key] = float(np.linalg.norm(tr[key] - ref) / n)
     return out
+
+
+def run_c3(args, nt=4000, plain_steps=300, chunk=250):
+    """1024 x 1024 x 512 cells, dx = 0.125 km, dt = 5 ms, 4000 steps: the
+    reduced-storage form (fp32_r16) timed over the whole run, its receiver
+    traces against an FP64 run of the same scheme (20-cell PML), and the
+    plain FP32 form timed over a short window for its step time."""
+    from openswpc_amd import problem
+    from openswpc_amd.solver import Swpc3D
+    t0 = time.time()
+    case = problem.case3d(1024, 1024, 512, 0.125, 0.005, nt, na=20, seed=args.seed, rmed=(0.03, 1.0, 0.5),
+                          src_depth=10.0, nst_side=8, rmed_tile=(128, 128, 128), log=log)
+    g = case.grid
+    ncell = g.nx * g.ny * g.nz
+    log(f"c3 setup {time.time() - t0:.0f} s: vmax {case.medium.vmax:.2f} km/s, "
+        f"courant {problem.stability3(g, case.medium.vmax):.3f}")
+    out = {"config": {"workload": "swpc_3d 1024x1024x512 dx=0.125km dt=5ms na=20 NM=3, layered + 3% von Karman "
+                                  "(a=1 km), 1 double couple, 64 receivers", "nt": nt}}
+    traces = {}
+    for prec in (CHOSEN, "fp64"):
+        sim = Swpc3D(g, case.medium, case.pml, case.glut, case.stations, case.ntdec_w, nt, precision=prec)
+        log(f"c3 {prec}: device memory {sim.memory() / 2**30:.1f} GiB")
+        sim.sync()
+        t1 = time.perf_counter()
+        for it in range(1, nt + 1, chunk):
+            sim.run(it, min(chunk, nt - it + 1))
+            sim.sync()
+            log(f"c3 {prec}: {min(it + chunk - 1, nt)}/{nt} steps, {time.perf_counter() - t1:.1f} s")
+        wall = time.perf_counter() - t1
+        traces[prec] = sim.receivers()[0].astype(np.float64)
+        out[prec] = {"ms_per_step": wall / nt * 1e3, "cell_updates_per_s": ncell * nt / wall,
+                     "device_GiB": sim.memory() / 2**30}
+        sim.close()
+    sim = Swpc3D(g, case.medium, case.pml, case.glut, case.stations, case.ntdec_w, nt, precision="fp32")
+    sim.run(1, 20)
+    sim.sync()
+    t1 = time.perf_counter()
+    sim.run(21, plain_steps)
+    sim.sync()
+    wall = time.perf_counter() - t1
+    out["fp32"] = {"ms_per_step": wall / plain_steps * 1e3, "cell_updates_per_s": ncell * plain_steps / wall,
+                   "device_GiB": sim.memory() / 2**30, "timed_steps": plain_steps}
+    sim.close()
+    ref = traces["fp64"]
+    out["misfit_r16_vs_fp64"] = float(np.linalg.norm(traces[CHOSEN] - ref) / np.linalg.norm(ref))
+    out["speedup_r16_vs_fp32"] = out["fp32"]["ms_per_step"] / out[CHOSEN]["ms_per_step"]
+    log(f"c3: {out}")
+    print(json.dumps(out), flush=True)
 
 
 def traffic_from_profile():

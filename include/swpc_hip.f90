@@ -43,6 +43,9 @@ module swpc_hip
     public :: swpc2d_set_sources, swpc2d_set_receivers, swpc2d_update_stress, swpc2d_update_vel
     public :: swpc2d_stressglut, swpc2d_store_receivers, swpc2d_step, swpc2d_run, swpc2d_get_receivers
     public :: swpc2d_vmax, swpc2d_sync
+    public :: swpc_lab_nparam, swpc3d_set_lab
+    public :: swpc_batch_create, swpc_batch_destroy, swpc_batch_add3d, swpc_batch_add2d, swpc_batch_run
+    public :: swpc_batch_sync
 
     interface
 
@@ -334,6 +337,53 @@ module swpc_hip
         integer(c_int) function swpc2d_sync(ctx) bind(C, name='swpc2d_sync')
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx
+        end function
+
+        ! ------------------------------------------ learned absorbing boundary
+        ! (no counterpart in the reference: replaces absorb__update_* on a
+        ! grid created with na = 0; params(nparam, 10) as in swpc_hip.h)
+
+        integer(c_int) function swpc_lab_nparam() bind(C, name='swpc_lab_nparam')
+            import :: c_int
+        end function
+
+        integer(c_int) function swpc3d_set_lab(ctx, params) bind(C, name='swpc3d_set_lab')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            real(c_float), intent(in) :: params(*)
+        end function
+
+        ! ---------------------------------------- batches of independent runs
+
+        integer(c_int) function swpc_batch_create(b) bind(C, name='swpc_batch_create')
+            import :: c_int, c_ptr
+            type(c_ptr), intent(out) :: b
+        end function
+
+        integer(c_int) function swpc_batch_destroy(b) bind(C, name='swpc_batch_destroy')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: b
+        end function
+
+        integer(c_int) function swpc_batch_add3d(b, ctx) bind(C, name='swpc_batch_add3d')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: b, ctx
+        end function
+
+        integer(c_int) function swpc_batch_add2d(b, ctx) bind(C, name='swpc_batch_add2d')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: b, ctx
+        end function
+
+        integer(c_int) function swpc_batch_run(b, it0, nsteps, graph_k) bind(C, name='swpc_batch_run')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: b
+            integer(c_int), value :: it0, nsteps, graph_k
+        end function
+
+        integer(c_int) function swpc_batch_sync(b) bind(C, name='swpc_batch_sync')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: b
         end function
 
     end interface

@@ -1786,7 +1786,7 @@ struct swpc3d_ctx {
     swpc::DBuf mbs[10];     // reduced storage: per-plane (base, scale)
     swpc::DBuf rho;         // optional density (swpc3d_set_density)
     bool lab = false;       // learned absorbing boundary on (swpc3d_set_lab)
-    swpc::DBuf lab_w;       // its ten nets in BF16
+    swpc::DBuf lab_w;       // its ten nets: BF16 values (nearest even) widened to FP32
     swpc::DBuf lab_colmed;  // per-column medium features
     swpc::LabDomain lab_dom;
     bool s16 = false;       // memory variables and ADE arrays in 16 bit (SWPC_FP32_S16, SWPC_FP32_R16)
@@ -2737,14 +2737,17 @@ int swpc3d_set_lab(swpc3d_ctx *c, const float *params)
     }
     if (int r = lab_check(c, true)) return r;
     SWPC_HIP(hipStreamSynchronize(c->stream));
-    std::vector<uint16_t> w((size_t)LAB_NNET * LAB_NP);
+    // the nets are rounded to BF16 and kept widened to FP32, the form the
+    // fill kernel reads through the scalar cache
+    std::vector<float> w((size_t)LAB_NNET * LAB_NP);
     for (size_t q = 0; q < w.size(); q++) {
         if (!std::isfinite(params[q])) return fail(SWPC_EINVAL, "learned-boundary parameter %zu is not finite", q);
-        w[q] = bf16_rne(params[q]);
+        const uint32_t b = (uint32_t)bf16_rne(params[q]) << 16;
+        memcpy(&w[q], &b, 4);
     }
     if (!c->lab_w.p)
-        if (int r = dalloc(c->lab_w, w.size() * 2, c->bytes)) return r;
-    SWPC_HIP(hipMemcpy(c->lab_w.p, w.data(), w.size() * 2, hipMemcpyHostToDevice));
+        if (int r = dalloc(c->lab_w, w.size() * 4, c->bytes)) return r;
+    SWPC_HIP(hipMemcpy(c->lab_w.p, w.data(), w.size() * 4, hipMemcpyHostToDevice));
     c->lab_dom = lab_domain(0, 0, c->d.nx, c->d.ny, c->d.nz);
     if (int r = lab_colmed(lab_grid(c), c->lab_dom, c->lab_colmed, c->bytes, c->stream)) return r;
     SWPC_HIP(hipStreamSynchronize(c->stream));

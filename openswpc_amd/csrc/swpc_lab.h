@@ -70,7 +70,7 @@ struct LabGrid {
 
 // Column medium (vs/vp, vp*dt/h, rho*vp) of every column of dom.
 int lab_colmed(const LabGrid &g, const LabDomain &dom, DBuf &out, long long &bytes, hipStream_t s);
-// Ghost fill of one phase; w holds the ten nets in BF16.
+// Ghost fill of one phase; w holds the ten nets (BF16 values widened to FP32).
 int lab_fill(const LabGrid &g, const LabDomain &dom, const void *w, const float4 *colmed, int phase, hipStream_t s);
 
 }  // namespace swpc

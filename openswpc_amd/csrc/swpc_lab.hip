@@ -10,7 +10,8 @@
 // Kernels
 //   k_lab_colmed  per-column medium features (vs/vp, vp*dt/h, rho*vp)
 //   k_lab_fill    ghost fill of one phase: one thread per face column, the
-//                 face's net staged in LDS (broadcast reads)
+//                 face's net read through the scalar cache (uniform per
+//                 workgroup) into SGPR operands of the FMAs
 //   k_lab_gather  training samples of one phase from a wide run's fields
 //   k_lab_fwdbwd  per-workgroup forward + backward of 128 samples; the
 //                 weight gradients are formed as LDS-resident outer-product
@@ -181,7 +182,8 @@ __device__ __forceinline__ float lab_features(const LabDev &a, int base, int ste
     return N;
 }
 
-// y = Ws x + W3 tanh(W2 tanh(W1 x + b1) + b2) + b3 with the net in LDS.
+// y = Ws x + W3 tanh(W2 tanh(W1 x + b1) + b2) + b3 with the net in LDS
+// (tanhf of the device library).
 __device__ __forceinline__ void lab_forward(const float *__restrict__ W, const float *x, float *y)
 {
     float h1[LAB_NH], h2[LAB_NH];
@@ -212,24 +214,62 @@ __device__ __forceinline__ void lab_forward(const float *__restrict__ W, const f
 
 struct LabFillArgs {
     LabDev a;
-    const __hip_bfloat16 *w;   // ten nets
+    const float *w;            // ten nets, BF16 values widened to FP32
     const float4 *colmed;
     int phase;
     int blk0[6];               // first workgroup of each face
 };
 
-// One thread per column; each workgroup serves one face, whose net is
-// widened from BF16 into LDS.
+// tanh(z) = 1 - 2/(1 + e^(2z)) with the hardware exponential and a fast
+// reciprocal: absolute error ~1e-7, which the ghost values inherit as
+// ~1e-7 relative; saturates cleanly (e^(2z) = inf gives 1, 0 gives -1).
+__device__ __forceinline__ float lab_tanh(float z)
+{
+    const float t = __expf(2.0f * z);
+    return 1.0f - __fdividef(2.0f, 1.0f + t);
+}
+
+// The forward pass of one column with the net read from global memory at
+// wave-uniform addresses: the compiler serves the weights through the
+// scalar data cache (s_load into SGPRs) and feeds them to the FMAs as
+// scalar operands, so no LDS or vector-memory traffic per weight.
+__device__ __forceinline__ void lab_forward_s(const float *__restrict__ W, const float *x, float *y)
+{
+    float h1[LAB_NH], h2[LAB_NH];
+#pragma unroll
+    for (int h = 0; h < LAB_NH; h++) {
+        float z = W[LAB_B1 + h];
+#pragma unroll
+        for (int q = 0; q < LAB_NIN; q++) z = fmaf(W[LAB_W1 + h * LAB_NIN + q], x[q], z);
+        h1[h] = lab_tanh(z);
+    }
+#pragma unroll
+    for (int h = 0; h < LAB_NH; h++) {
+        float z = W[LAB_B2 + h];
+#pragma unroll
+        for (int g = 0; g < LAB_NH; g++) z = fmaf(W[LAB_W2 + h * LAB_NH + g], h1[g], z);
+        h2[h] = lab_tanh(z);
+    }
+#pragma unroll
+    for (int o = 0; o < LAB_NOUT; o++) {
+        float z = W[LAB_B3 + o];
+#pragma unroll
+        for (int h = 0; h < LAB_NH; h++) z = fmaf(W[LAB_W3 + o * LAB_NH + h], h2[h], z);
+#pragma unroll
+        for (int q = 0; q < LAB_NIN; q++) z = fmaf(W[LAB_WS + o * LAB_NIN + q], x[q], z);
+        y[o] = z;
+    }
+}
+
+// One thread per column; each workgroup serves one face (its net is
+// uniform across the workgroup).
 __global__ void __launch_bounds__(256) k_lab_fill(const LabFillArgs p)
 {
-    __shared__ float W[LAB_NP];
     int f = 0;
 #pragma unroll
     for (int q = 1; q < 5; q++) f += (int)blockIdx.x >= p.blk0[q] ? 1 : 0;
     const int net = 2 * f + p.phase;
-    const __hip_bfloat16 *wn = p.w + (size_t)net * LAB_NP;
-    for (int q = threadIdx.x; q < LAB_NP; q += blockDim.x) W[q] = __bfloat162float(wn[q]);
-    __syncthreads();
+    const float *__restrict__ W = p.w + (size_t)net * LAB_NP;
     const LabDev &a = p.a;
     const int col = ((int)blockIdx.x - p.blk0[f]) * blockDim.x + threadIdx.x;
     if (col >= a.ncol[f]) return;
@@ -249,7 +289,7 @@ __global__ void __launch_bounds__(256) k_lab_fill(const LabFillArgs p)
     float x[LAB_NIN], y[LAB_NOUT];
     const float N = lab_features(a, base, step, m, prev, x);
     if (N > 0.0f && m.y > 0.0f) {  // air columns (lam + 2 mu = 0) keep zero ghosts
-        lab_forward(W, x, y);
+        lab_forward_s(W, x, y);
     } else {
 #pragma unroll
         for (int q = 0; q < LAB_NOUT; q++) y[q] = 0.0f;
@@ -279,7 +319,7 @@ int lab_fill(const LabGrid &g, const LabDomain &dom, const void *w, const float4
 {
     LabFillArgs p;
     p.a = make_dev(g, dom);
-    p.w = (const __hip_bfloat16 *)w;
+    p.w = (const float *)w;
     p.colmed = colmed;
     p.phase = phase;
     int b = 0;

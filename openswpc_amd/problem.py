@@ -68,7 +68,7 @@ def mechanism(seed):
 
 def case3d(nx, ny, nz, dx, dt, nt, na=20, layers=LHM_EXAMPLE, seed=1, rmed=None, src_depth=10.0,
            tr=1.0, t0=0.1, mo=1e15, nst_side=8, st_spacing=None, ntdec_w=1, rmed_tile=(128, 128, 64),
-           benchmark=False, zbeg=None, bodyforce=False, src_xy=(0.0, 0.0)):
+           benchmark=False, zbeg=None, bodyforce=False, src_xy=(0.0, 0.0), log=None):
     """3-D case: layered (+ random) medium, double couple at src_depth below
     the grid centre, an nst_side x nst_side receiver grid on the free surface.
     bodyforce=True replaces the double couple by a single force of the same
@@ -76,8 +76,12 @@ def case3d(nx, ny, nz, dx, dt, nt, na=20, layers=LHM_EXAMPLE, seed=1, rmed=None,
     reference's bf_mode).
 
     rmed = (epsilon, a_km, kappa) adds a von Karman perturbation generated
-    on a periodic tile and mapped by rmed_tile periodicity.
+    on a periodic tile and mapped by rmed_tile periodicity.  log (optional)
+    receives a line after each setup stage.
     """
+    import time
+    t0 = time.time()
+    note = (lambda m: log(f"case3d: {m} ({time.time() - t0:.0f} s)")) if log else (lambda m: None)
     g = md.Grid3D(nx, ny, nz, dx, dx, dx, dt, na=na, zbeg=zbeg)
     xi = None
     if rmed is not None:
@@ -85,8 +89,12 @@ def case3d(nx, ny, nz, dx, dt, nt, na=20, layers=LHM_EXAMPLE, seed=1, rmed=None,
         tile = tuple(min(t, n) for t, n in zip(rmed_tile, (nx, ny, nz)))
         vol = rmedia.generate(tile, dx, a, eps, kappa, seed=seed)
         xi = md.rmed_tile(vol, -2, nx + 3, -2, ny + 3, -2, nz + 3)
+        note("random medium")
     rho, lam, mu, qp, qs = md.layered3(g, layers, xi=xi)
+    xi = None
+    note("layered model")
     med = md.medium_setup3(g, rho, lam, mu, qp, qs, benchmark=benchmark)
+    note("medium__setup")
     strike, dip, rake = mechanism(seed)
     src = PointSource(float(src_xy[0]), float(src_xy[1]), src_depth, t0, tr, mo, fdtool.sdr2moment(strike, dip, rake))
     glut = stress_glut3(g, [src], nt)
