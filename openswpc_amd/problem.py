@@ -80,8 +80,8 @@ def case3d(nx, ny, nz, dx, dt, nt, na=20, layers=LHM_EXAMPLE, seed=1, rmed=None,
     receives a line after each setup stage.
     """
     import time
-    t0 = time.time()
-    note = (lambda m: log(f"case3d: {m} ({time.time() - t0:.0f} s)")) if log else (lambda m: None)
+    clock0 = time.time()
+    note = (lambda m: log(f"case3d: {m} ({time.time() - clock0:.0f} s)")) if log else (lambda m: None)
     g = md.Grid3D(nx, ny, nz, dx, dx, dx, dt, na=na, zbeg=zbeg)
     xi = None
     if rmed is not None:

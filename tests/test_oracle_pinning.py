@@ -381,3 +381,16 @@ def test_bodyforce_momentum_balance():
         assert math.isclose(p, bf.m[0][comp] * stf_sum, rel_tol=1e-7, abs_tol=1e-12), (name, p)
     # the wavefield has not reached the absorber
     assert np.abs(o.f["Vx"][:, : g.na + 4, :]).max() < 1e-6 * np.abs(o.f["Vx"]).max()
+
+
+def test_case_builders_fire_their_source():
+    """The synthetic cases inject their source: each table entry is
+    (mo/M0) momentrate(t) dt/(dx dy dz) and the Kupper moment rate
+    integrates to one, so sum(table) dx dy dz = 1 once the rise is over."""
+    import numpy as np
+    from openswpc_amd import problem
+    c = problem.case3d(64, 56, 48, 0.25, 0.01, 200, na=8, seed=1, nst_side=2, src_depth=3.0)
+    g = c.grid
+    assert np.isclose(c.glut.table.sum() * g.dx * g.dy * g.dz, 1.0, rtol=1e-3)
+    c2 = problem.case2d(96, 64, 0.1, 0.005, 300, na=10, seed=1, nst=2, src_depth=2.0)
+    assert np.isclose(c2.glut.table.sum() * c2.grid.dx * c2.grid.dz, 1.0, rtol=1e-3)
