@@ -253,8 +253,10 @@ int swpc3d_set_tuning(swpc3d_ctx *ctx, const int *p, int n);
    outermost cells of each column and the net's own previous ghost values;
    columns whose outermost cell is air (lam + 2 mu = 0) keep zero ghosts.
    Each net has swpc_lab_nparam() FP32 parameters: W1[32][61] b1[32]
-   W2[32][32] b2[32] W3[5][32] b3[5] Ws[5][61]; the time step stores and
-   uses them rounded to BF16 (nearest even), with FP32 arithmetic. */
+   W2[32][32] b2[32] W3[5][32] b3[5] Ws[5][61]; the time step uses them
+   rounded to BF16 (nearest even).  Its two hidden layers run on the
+   matrix cores with BF16 inputs (features, layer-1 activations) and FP32
+   accumulation; the output layer Ws x + W3 h2 + b3 is FP32. */
 int swpc_lab_nparam(void);
 /* Switch the learned boundary on with params[10][swpc_lab_nparam()], or
    off with params = NULL.  Needs na = 0, SWPC_FP32 or SWPC_FP32_R16, and

@@ -9,17 +9,17 @@
 //
 // Kernels
 //   k_lab_colmed  per-column medium features (vs/vp, vp*dt/h, rho*vp)
-//   k_lab_fill    ghost fill of one phase: one thread per face column, the
-//                 face's net read through the scalar cache (uniform per
-//                 workgroup) into SGPR operands of the FMAs
+//   k_lab_fill    ghost fill of one phase, one launch per face: one lane
+//                 per column; the hidden layers on the matrix cores
+//                 (32x32x16 BF16 MFMA, FP32 accumulation) over each
+//                 wave's 64 columns, the output layer in FP32
 //   k_lab_gather  training samples of one phase from a wide run's fields
 //   k_lab_fwdbwd  per-workgroup forward + backward of 128 samples; the
 //                 weight gradients are formed as LDS-resident outer-product
 //                 sums and written as one partial per workgroup
 //   k_lab_adam    deterministic reduction of the partials + Adam update
-// None of this is a dense contraction big enough for MFMA to matter: the
-// ghost fill is bound by the strided strip loads (about 240 B per column
-// against 7 kFLOP), the training by launch latency.
+// The training kernels stay on the vector units: they are bound by launch
+// latency, not arithmetic.
 #include "swpc_lab.h"
 
 #include <hip/hip_bf16.h>
@@ -217,90 +217,158 @@ struct LabFillArgs {
     const float *w;            // ten nets, BF16 values widened to FP32
     const float4 *colmed;
     int phase;
-    int blk0[6];               // first workgroup of each face
+    int face;                  // one launch per face
 };
 
 // tanh(z) = 1 - 2/(1 + e^(2z)) with the hardware exponential and a fast
-// reciprocal: absolute error ~1e-7, which the ghost values inherit as
-// ~1e-7 relative; saturates cleanly (e^(2z) = inf gives 1, 0 gives -1).
+// reciprocal: absolute error ~1e-7; saturates cleanly (e^(2z) = inf gives
+// 1, 0 gives -1).
 __device__ __forceinline__ float lab_tanh(float z)
 {
     const float t = __expf(2.0f * z);
     return 1.0f - __fdividef(2.0f, 1.0f + t);
 }
 
-// The forward pass of one column with the net read from global memory at
-// wave-uniform addresses: the compiler serves the weights through the
-// scalar data cache (s_load into SGPRs) and feeds them to the FMAs as
-// scalar operands, so no LDS or vector-memory traffic per weight.
-__device__ __forceinline__ void lab_forward_s(const float *__restrict__ W, const float *x, float *y)
+// BF16 of an FP32 value, nearest even (finite inputs)
+__device__ __forceinline__ __bf16 lab_bf16(float f)
 {
-    float h1[LAB_NH], h2[LAB_NH];
-#pragma unroll
-    for (int h = 0; h < LAB_NH; h++) {
-        float z = W[LAB_B1 + h];
-#pragma unroll
-        for (int q = 0; q < LAB_NIN; q++) z = fmaf(W[LAB_W1 + h * LAB_NIN + q], x[q], z);
-        h1[h] = lab_tanh(z);
+    unsigned u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    const unsigned short h = (unsigned short)(u >> 16);
+    __bf16 r;
+    __builtin_memcpy(&r, &h, 2);
+    return r;
+}
+
+typedef __bf16 lab_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float lab_f32x16 __attribute__((ext_vector_type(16)));
+
+// LDS row strides (elements): 8 extra BF16 per row spread the 16-byte
+// operand reads of one MFMA over the banks.
+constexpr int LX = 72;     // features, BF16 [64 columns][72]
+constexpr int LH = 40;     // hidden layer 1, BF16 [64][40]
+constexpr int LH2 = 33;    // hidden layer 2, FP32 [64][33] (in the feature rows' space)
+constexpr int LW1 = 72;    // W1 as BF16 [32][72]
+constexpr int LW2 = 40;    // W2 as BF16 [32][40]
+constexpr int FILL_WAVES = 4;
+constexpr int WAVE_LDS = 64 * LX * 2 + 64 * LH * 2;   // bytes per wave
+
+// The ghost fill on the matrix cores.  Each wave owns 64 columns (one per
+// lane).  A lane forms its column's 61 features in FP32; the two hidden
+// layers then run as 32x32x16 BF16 MFMAs with FP32 accumulation over the
+// wave's 64 columns (features and layer-1 activations rounded to BF16 on
+// the way in, as the weights are), and the output layer Ws x + W3 h2 + b3
+// stays in FP32 per lane, with Ws and W3 read at wave-uniform addresses
+// through the scalar cache.  MFMA operand layouts (gfx950,
+// v_mfma_f32_32x32x16_bf16): lane l holds A[l % 32][8 (l / 32) + 0..7] and
+// B[8 (l / 32) + 0..7][l % 32]; the accumulator's register r holds
+// D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
+__global__ void __launch_bounds__(64 * FILL_WAVES) k_lab_fill(const LabFillArgs p)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    __bf16 *sW1 = (__bf16 *)lds;                       // [32][LW1]
+    __bf16 *sW2 = sW1 + 32 * LW1;                      // [32][LW2]
+    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+    unsigned char *wl = lds + (32 * LW1 + 32 * LW2) * 2 + wave * WAVE_LDS;
+    __bf16 *sX = (__bf16 *)wl;                         // [64][LX]
+    float *sH2 = (float *)wl;                          // [64][LH2], reuses the feature rows after layer 1
+    __bf16 *sH1 = (__bf16 *)(wl + 64 * LX * 2);        // [64][LH]
+
+    const int f = p.face, net = 2 * f + p.phase;
+    const float *__restrict__ W = p.w + (size_t)net * LAB_NP;
+    for (int q = threadIdx.x; q < 32 * LW1; q += blockDim.x) {
+        const int n = q / LW1, k = q % LW1;
+        sW1[q] = lab_bf16(k < LAB_NIN ? W[LAB_W1 + n * LAB_NIN + k] : 0.0f);
     }
-#pragma unroll
-    for (int h = 0; h < LAB_NH; h++) {
-        float z = W[LAB_B2 + h];
-#pragma unroll
-        for (int g = 0; g < LAB_NH; g++) z = fmaf(W[LAB_W2 + h * LAB_NH + g], h1[g], z);
-        h2[h] = lab_tanh(z);
+    for (int q = threadIdx.x; q < 32 * LW2; q += blockDim.x) {
+        const int n = q / LW2, k = q % LW2;
+        sW2[q] = lab_bf16(k < LAB_NH ? W[LAB_W2 + n * LAB_NH + k] : 0.0f);
     }
+
+    const LabDev &a = p.a;
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool have = col < a.ncol[f];
+    int base = 0, step = 0;
+    float4 m = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+    const int nout = c_nout[net];
+    int gaddr[LAB_NOUT];
+    float prev[LAB_NG], x[LAB_NIN];
+    float N = 0.0f;
+    if (have) {
+        lab_col(a, f, col, base, step);
+        m = p.colmed[a.col0[f] + col];
 #pragma unroll
-    for (int o = 0; o < LAB_NOUT; o++) {
+        for (int q = 0; q < LAB_NOUT; q++) {
+            const int fi = c_tgt[net][q][0], h = c_tgt[net][q][1];
+            gaddr[q] = base - h * step;
+            const float v = q < nout ? a.F[fi][gaddr[q]] : 0.0f;
+            prev[q] = fi >= 3 ? v / m.z : v;
+        }
+        N = lab_features(a, base, step, m, prev, x);
+    } else {
+#pragma unroll
+        for (int q = 0; q < LAB_NIN; q++) x[q] = 0.0f;
+    }
+    const bool live = have && N > 0.0f && m.y > 0.0f;   // air columns keep zero ghosts
+#pragma unroll
+    for (int q = 0; q < LX; q++) sX[l * LX + q] = lab_bf16(q < LAB_NIN && live ? x[q] : 0.0f);
+    __syncthreads();
+
+    // hidden layer 1: H1 = tanh(X W1^T + b1), two 32-row tiles of the wave
+    const int ln = l & 31, lk = 8 * (l >> 5);
+    const float b1 = W[LAB_B1 + ln], b2 = W[LAB_B2 + ln];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        lab_f32x16 acc = {};
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) {
+            const lab_bf16x8 av = *(const lab_bf16x8 *)(sX + (32 * t + ln) * LX + 16 * kc + lk);
+            const lab_bf16x8 bv = *(const lab_bf16x8 *)(sW1 + ln * LW1 + 16 * kc + lk);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int row = 32 * t + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
+            sH1[row * LH + ln] = lab_bf16(lab_tanh(acc[r] + b1));
+        }
+    }
+    __syncthreads();   // layer 1 done: the feature rows are free for H2
+
+    // hidden layer 2: H2 = tanh(H1 W2^T + b2), kept in FP32
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        lab_f32x16 acc = {};
+#pragma unroll
+        for (int kc = 0; kc < 2; kc++) {
+            const lab_bf16x8 av = *(const lab_bf16x8 *)(sH1 + (32 * t + ln) * LH + 16 * kc + lk);
+            const lab_bf16x8 bv = *(const lab_bf16x8 *)(sW2 + ln * LW2 + 16 * kc + lk);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int row = 32 * t + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
+            sH2[row * LH2 + ln] = lab_tanh(acc[r] + b2);
+        }
+    }
+    __syncthreads();
+
+    if (!have) return;
+    // output layer in FP32: y = b3 + W3 h2 + Ws x
+    float h2[LAB_NH];
+#pragma unroll
+    for (int h = 0; h < LAB_NH; h++) h2[h] = sH2[l * LH2 + h];
+    // one output at a time keeps its 93 weights within the SGPR budget
+#pragma unroll 1
+    for (int o = 0; o < nout; o++) {
         float z = W[LAB_B3 + o];
 #pragma unroll
         for (int h = 0; h < LAB_NH; h++) z = fmaf(W[LAB_W3 + o * LAB_NH + h], h2[h], z);
 #pragma unroll
         for (int q = 0; q < LAB_NIN; q++) z = fmaf(W[LAB_WS + o * LAB_NIN + q], x[q], z);
-        y[o] = z;
-    }
-}
-
-// One thread per column; each workgroup serves one face (its net is
-// uniform across the workgroup).
-__global__ void __launch_bounds__(256) k_lab_fill(const LabFillArgs p)
-{
-    int f = 0;
-#pragma unroll
-    for (int q = 1; q < 5; q++) f += (int)blockIdx.x >= p.blk0[q] ? 1 : 0;
-    const int net = 2 * f + p.phase;
-    const float *__restrict__ W = p.w + (size_t)net * LAB_NP;
-    const LabDev &a = p.a;
-    const int col = ((int)blockIdx.x - p.blk0[f]) * blockDim.x + threadIdx.x;
-    if (col >= a.ncol[f]) return;
-    int base, step;
-    lab_col(a, f, col, base, step);
-    const float4 m = p.colmed[a.col0[f] + col];
-    const int nout = c_nout[net];
-    int gaddr[LAB_NOUT];
-    float prev[LAB_NG];
-#pragma unroll
-    for (int q = 0; q < LAB_NOUT; q++) {
-        const int fi = c_tgt[net][q][0], h = c_tgt[net][q][1];
-        gaddr[q] = base - h * step;
-        float v = q < nout ? a.F[fi][gaddr[q]] : 0.0f;
-        prev[q] = fi >= 3 ? v / m.z : v;
-    }
-    float x[LAB_NIN], y[LAB_NOUT];
-    const float N = lab_features(a, base, step, m, prev, x);
-    if (N > 0.0f && m.y > 0.0f) {  // air columns (lam + 2 mu = 0) keep zero ghosts
-        lab_forward_s(W, x, y);
-    } else {
-#pragma unroll
-        for (int q = 0; q < LAB_NOUT; q++) y[q] = 0.0f;
-    }
-#pragma unroll
-    for (int q = 0; q < LAB_NOUT; q++) {
-        if (q >= nout) break;
-        const int fi = c_tgt[net][q][0];
-        float v = y[q] * N;
+        const int fi = c_tgt[net][o][0];
+        float v = live ? z * N : 0.0f;
         if (fi >= 3) v = v * m.z;
-        a.F[fi][gaddr[q]] = v;
+        a.F[fi][gaddr[o]] = v;
     }
 }
 
@@ -322,15 +390,14 @@ int lab_fill(const LabGrid &g, const LabDomain &dom, const void *w, const float4
     p.w = (const float *)w;
     p.colmed = colmed;
     p.phase = phase;
-    int b = 0;
+    const size_t lds = (size_t)(32 * LW1 + 32 * LW2) * 2 + (size_t)FILL_WAVES * WAVE_LDS;
     for (int f = 0; f < 5; f++) {
-        p.blk0[f] = b;
-        b += (dom.ncol[f] + 255) / 256;
+        const int nb = (dom.ncol[f] + 64 * FILL_WAVES - 1) / (64 * FILL_WAVES);
+        if (nb == 0) continue;
+        p.face = f;
+        hipLaunchKernelGGL(k_lab_fill, dim3(nb), dim3(64 * FILL_WAVES), lds, s, p);
+        SWPC_CHECK_LAUNCH();
     }
-    p.blk0[5] = b;
-    if (b == 0) return SWPC_OK;
-    hipLaunchKernelGGL(k_lab_fill, dim3(b), dim3(256), 0, s, p);
-    SWPC_CHECK_LAUNCH();
     return SWPC_OK;
 }
 

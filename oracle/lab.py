@@ -41,11 +41,23 @@ def unpack(p):
             p[o["Ws"]:o["n"]].reshape(NO, NIN))
 
 
-def forward(p, X):
-    """y[n, 5] in FP64 for features X[n, 61]."""
+def bf16(a):
+    """Nearest-even BF16 of FP32 values, as FP64."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64).reshape(np.shape(a))
+
+
+def forward(p, X, hidden_bf16=False):
+    """y[n, 5] in FP64 for features X[n, 61].  hidden_bf16 rounds the
+    inputs of both hidden layers to BF16 (the time step's matrix-core
+    form; the training loss and gradient below are the plain form)."""
     W1, b1, W2, b2, W3, b3, Ws = (np.asarray(a, np.float64) for a in unpack(np.asarray(p, np.float64)))
     X = np.asarray(X, np.float64)
-    h1 = np.tanh(X @ W1.T + b1)
+    Xh = bf16(X) if hidden_bf16 else X
+    h1 = np.tanh(Xh @ W1.T + b1)
+    if hidden_bf16:
+        h1 = bf16(h1)
     h2 = np.tanh(h1 @ W2.T + b2)
     return h2 @ W3.T + b3 + X @ Ws.T
 

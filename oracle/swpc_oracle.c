@@ -945,19 +945,34 @@ static float lab_features(o3d_t *s, const float *rho, int f, int a, int b, int o
     return n;
 }
 
-/* One net, FP32, sequential sums in index order. */
+/* BF16 of an FP32 value, nearest even (finite inputs). */
+static float lab_bf16(float f)
+{
+    unsigned u;
+    memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+/* One net, FP32, sequential sums in index order.  The two hidden layers
+   take their inputs rounded to BF16 (the features for layer 1, tanh of
+   layer 1 for layer 2), as the matrix cores of the time step do; the
+   output layer uses the FP32 features and FP32 layer-2 activations. */
 void oracle_lab_forward(const float *p, const float *x, float *y)
 {
-    float h1[LAB_NH], h2[LAB_NH];
+    float h1[LAB_NH], h2[LAB_NH], xb[LAB_NIN];
+    for (int q = 0; q < LAB_NIN; q++) xb[q] = lab_bf16(x[q]);
     for (int h = 0; h < LAB_NH; h++) {
-        float z = p[LAB_B1 + h];
-        for (int q = 0; q < LAB_NIN; q++) z += p[LAB_W1 + h * LAB_NIN + q] * x[q];
-        h1[h] = tanhf(z);
+        float z = 0.0f;
+        for (int q = 0; q < LAB_NIN; q++) z += p[LAB_W1 + h * LAB_NIN + q] * xb[q];
+        h1[h] = lab_bf16(tanhf(z + p[LAB_B1 + h]));
     }
     for (int h = 0; h < LAB_NH; h++) {
-        float z = p[LAB_B2 + h];
+        float z = 0.0f;
         for (int g = 0; g < LAB_NH; g++) z += p[LAB_W2 + h * LAB_NH + g] * h1[g];
-        h2[h] = tanhf(z);
+        h2[h] = tanhf(z + p[LAB_B2 + h]);
     }
     for (int o = 0; o < LAB_NOUT; o++) {
         float z = p[LAB_B3 + o];

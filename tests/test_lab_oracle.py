@@ -63,16 +63,22 @@ def test_gradient_matches_finite_differences():
 
 
 def test_c_forward_matches_fp64():
+    """The C forward (BF16 inputs to the hidden layers) against the FP64
+    one with the same roundings; and the rounding is what separates it from
+    the plain FP64 net (by about 2^-9 of the hidden path)."""
     rng = np.random.default_rng(1)
-    p = (rng.standard_normal(olab.NP) * 0.2).astype(np.float32)
+    p = olab.bf16(rng.standard_normal(olab.NP) * 0.2).astype(np.float32)
     L = olib(32)
+    far = 0.0
     for _ in range(20):
         x = rng.standard_normal(olab.NIN).astype(np.float32)
         y = np.zeros(5, np.float32)
         L.oracle_lab_forward(p.ctypes.data_as(C.POINTER(C.c_float)), x.ctypes.data_as(C.POINTER(C.c_float)),
                              y.ctypes.data_as(C.POINTER(C.c_float)))
-        ref = olab.forward(p, x[None])[0]
+        ref = olab.forward(p, x[None], hidden_bf16=True)[0]
         assert np.allclose(y, ref, rtol=1e-5, atol=1e-5)
+        far = max(far, float(np.abs(olab.forward(p, x[None])[0] - ref).max()))
+    assert 1e-4 < far < 0.1
 
 
 def _target(net, q):
