@@ -237,7 +237,14 @@ int swpc3d_memory(swpc3d_ctx *ctx, long long *bytes);
      [16]    steps per captured HIP graph in swpc3d_run (0 = launch step
              by step; default 16).  The step number lives in device memory,
              so one captured sequence replays for any it; setters drop the
-             graph, and profiling runs step by step. */
+             graph, and profiling runs step by step;
+     [17]    reduced storage: stochastic rounding of the 16-bit memory
+             variable and ADE stores (0/1, default 0), hashed from cell,
+             array and value (deterministic);
+     [18]    reduced storage: log2 shift of the memory-variable scale
+             (-24..24, default 0; negative stores larger values, away from
+             the FP16 subnormals, at the cost of headroom before the
+             +-65504 clamp).  Stored values are rescaled with it. */
 int swpc3d_set_tuning(swpc3d_ctx *ctx, const int *p, int n);
 
 /* ------------------------------------------------------------------ */

@@ -60,6 +60,7 @@ struct K3 {
     const float2 *mbs[10];      // per-plane (base, scale), indexed k + KO
     __half *hR[18];             // memory variables stored as R * r_inv
     float r_scale, r_inv;
+    int sr;                     // 16-bit stores round stochastically (else to nearest)
     __hip_bfloat16 *hA[18];     // ADE arrays of the PML
     const int4 *surf;   // (kfs_top, kfs_bot, kob_top, kob_bot) per column (i,j) in [1,nx]x[1,ny]
     const T *gx, *gy, *gz;  // profiles [8][n]: c1..c4 at centres then e1..e4 at edges
@@ -97,6 +98,38 @@ __device__ __forceinline__ T medv(const K3<T> &a, int q, int c, int k)
     }
 }
 
+// Stochastic rounding of the 16-bit stores: the dropped low bits of the
+// FP32 value are compared with a hashed threshold, so a store rounds up
+// with probability equal to the dropped fraction.  Small increments of a
+// slowly varying accumulator (memory variables with c1 close to 1, ADE
+// arrays) then survive on average instead of being rounded away every
+// step.  The hash depends on the cell, the array and the value, so runs
+// stay deterministic and graph replays identical.
+__device__ __forceinline__ unsigned sr_hash(unsigned c, unsigned q, unsigned vb)
+{
+    unsigned x = c * 0x9E3779B1u ^ (q << 24) ^ vb;
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// FP32 -> FP16 (finite, within +-65504 after scaling), stochastic
+__device__ __forceinline__ __half f2h_sr(float v, unsigned c, unsigned q)
+{
+    const unsigned vb = __float_as_uint(v);
+    return __float2half_rz(__uint_as_float(vb + (sr_hash(c, q, vb) & 0x1FFFu)));
+}
+
+// FP32 -> BF16 bits, stochastic
+__device__ __forceinline__ unsigned short f2bf_sr(float v, unsigned c, unsigned q)
+{
+    const unsigned vb = __float_as_uint(v);
+    return (unsigned short)((vb + (sr_hash(c, q, vb) & 0xFFFFu)) >> 16);
+}
+
 template <bool S16, typename T>
 __device__ __forceinline__ T ldR(const K3<T> &a, int q, int c)
 {
@@ -107,8 +140,12 @@ __device__ __forceinline__ T ldR(const K3<T> &a, int q, int c)
 template <bool S16, typename T>
 __device__ __forceinline__ void stR(const K3<T> &a, int q, int c, T v)
 {
-    if constexpr (S16) a.hR[q][c] = __float2half_rn(fminf(fmaxf((float)v * a.r_inv, -65504.0f), 65504.0f));
-    else a.R[q][c] = v;
+    if constexpr (S16) {
+        const float w = fminf(fmaxf((float)v * a.r_inv, -65504.0f), 65504.0f);
+        a.hR[q][c] = a.sr ? f2h_sr(w, (unsigned)c, (unsigned)q) : __float2half_rn(w);
+    } else {
+        a.R[q][c] = v;
+    }
 }
 
 template <bool S16, typename T>
@@ -121,8 +158,16 @@ __device__ __forceinline__ T ldA(const K3<T> &a, int q, int c)
 template <bool S16, typename T>
 __device__ __forceinline__ void stA(const K3<T> &a, int q, int c, T v)
 {
-    if constexpr (S16) a.hA[q][c] = __float2bfloat16((float)v);
-    else a.A[q][c] = v;
+    if constexpr (S16) {
+        if (a.sr) {
+            const unsigned short b = f2bf_sr((float)v, (unsigned)c, 32u + (unsigned)q);
+            a.hA[q][c] = *(const __hip_bfloat16 *)&b;
+        } else {
+            a.hA[q][c] = __float2bfloat16((float)v);
+        }
+    } else {
+        a.A[q][c] = v;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -434,7 +479,18 @@ __device__ __forceinline__ void stR4(const K3<float> &a, int q, int c, f4v v, un
         v.y = fminf(fmaxf(v.y, -lim), lim);
         v.z = fminf(fmaxf(v.z, -lim), lim);
         v.w = fminf(fmaxf(v.w, -lim), lim);
-        if (m == 15u) {
+        if (a.sr) {
+            __half h[4] = {f2h_sr(v.x, (unsigned)c, (unsigned)q), f2h_sr(v.y, (unsigned)c + 1u, (unsigned)q),
+                           f2h_sr(v.z, (unsigned)c + 2u, (unsigned)q), f2h_sr(v.w, (unsigned)c + 3u, (unsigned)q)};
+            if (m == 15u) {
+                h4v hv;
+                for (int e = 0; e < 4; e++) hv[e] = *(const _Float16 *)&h[e];
+                __builtin_nontemporal_store(hv, (h4v *)(a.hR[q] + c));
+            } else {
+                for (int e = 0; e < 4; e++)
+                    if (m & (1u << e)) a.hR[q][c + e] = h[e];
+            }
+        } else if (m == 15u) {
             sth4(a.hR[q] + c, v);
         } else {
             if (m & 1u) a.hR[q][c] = __float2half_rn(v.x);
@@ -1336,10 +1392,17 @@ __device__ __forceinline__ void stA4(const K3<float> &a, int q, int c, f4v v, un
 {
     if constexpr (S16) {
         u4v r;
-        r.x = bf16_rne(v.x);
-        r.y = bf16_rne(v.y);
-        r.z = bf16_rne(v.z);
-        r.w = bf16_rne(v.w);
+        if (a.sr) {
+            r.x = f2bf_sr(v.x, (unsigned)c, 32u + (unsigned)q);
+            r.y = f2bf_sr(v.y, (unsigned)c + 1u, 32u + (unsigned)q);
+            r.z = f2bf_sr(v.z, (unsigned)c + 2u, 32u + (unsigned)q);
+            r.w = f2bf_sr(v.w, (unsigned)c + 3u, 32u + (unsigned)q);
+        } else {
+            r.x = bf16_rne(v.x);
+            r.y = bf16_rne(v.y);
+            r.z = bf16_rne(v.z);
+            r.w = bf16_rne(v.w);
+        }
         unsigned short *p = (unsigned short *)(a.hA[q] + c);
         if (m == 15u) {
             __builtin_nontemporal_store(r, (u4v *)p);
@@ -1792,6 +1855,8 @@ struct swpc3d_ctx {
     bool s16 = false;       // memory variables and ADE arrays in 16 bit (SWPC_FP32_S16, SWPC_FP32_R16)
     bool m16 = false;       // medium in 16 bit too (SWPC_FP32_S16)
     float r_scale = 1.0f;   // memory-variable scale of the reduced storage
+    float r_scale0 = 1.0f;  // ... as derived from the sources, before r_shift
+    int r_shift = 0;        // log2 adjustment of r_scale (tuning entry 18)
     swpc::DBuf surf, kob, gx, gy, gz, gxv;
     swpc::DBuf src, m6, sdrop, bsrc, f3, fdt;
     swpc::DBuf st, uacc, wvel, wdisp, vmax;
@@ -1816,6 +1881,7 @@ struct swpc3d_ctx {
     // read the same inputs); events fork and join the two streams.
     int conc = 0;
     int graph_k = 16;       // steps per captured step graph (0 = launch step by step)
+    int sr = 0;             // stochastic rounding of the 16-bit stores (reduced storage)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     unsigned gen = 0;       // bumped by every graph drop (batches recapture on change)
@@ -1857,6 +1923,7 @@ static K3<T> make_args(swpc3d_ctx *c)
         for (int q = 0; q < 18; q++) a.hA[q] = (__hip_bfloat16 *)c->A[q].p;
         a.r_scale = c->r_scale;
         a.r_inv = 1.0f / c->r_scale;
+        a.sr = c->sr;
     }
     a.surf = (const int4 *)c->surf.p;
     a.gx = (const T *)c->gx.p; a.gy = (const T *)c->gy.p; a.gz = (const T *)c->gz.p;
@@ -2868,10 +2935,13 @@ int swpc3d_set_sources(swpc3d_ctx *c, int nsrc, const int *isrc, const int *jsrc
     c->nsrc = nsrc;
     c->nt_src = nt;
     if (c->s16 && nsrc > 0) {
-        // The stress at any cell stays below the total stress glut injected
-        // at the source, sum_t |sdrop| max|m|; memory variables are smaller
-        // still (of order tau w S).  Map 32x that bound to 2^14 in FP16,
-        // as a power of two so that the scaling itself is exact.
+        // B = sum_t |sdrop| max|m| is the total stress glut injected; the
+        // memory variables (of order tau w S) stay far below it.  B maps to
+        // 2^17, so values up to B/2 fit the FP16 range and the far field,
+        // many decades below B, stays clear of the FP16 subnormals, which
+        // would otherwise cost it its relative precision (a mapping of 32 B
+        // to 2^14 put the coda there and gave 8e-3 trace misfit at
+        // dx = 125 m).  A power of two, so the scaling itself is exact.
         double bound = 0, mmax = 0;
         for (int it = 0; it < nt; it++) {
             double mx = 0;
@@ -2879,8 +2949,9 @@ int swpc3d_set_sources(swpc3d_ctx *c, int nsrc, const int *isrc, const int *jsrc
             bound += mx;
         }
         for (int q = 0; q < 6 * nsrc; q++) mmax = std::max(mmax, std::fabs(m6[q]));
-        bound *= mmax * 32.0;
-        if (bound > 0) c->r_scale = (float)std::ldexp(1.0, (int)std::ceil(std::log2(bound / 16384.0)));
+        bound *= mmax;
+        if (bound > 0) c->r_scale0 = (float)std::ldexp(1.0, (int)std::ceil(std::log2(bound / 131072.0)));
+        c->r_scale = std::ldexp(c->r_scale0, c->r_shift);
     }
     return SWPC_OK;
 }
@@ -3218,14 +3289,35 @@ int swpc3d_profile_read(swpc3d_ctx *c, double *ms5, long long *count5)
     return prof_read(c->prof, ms5, count5);
 }
 
+__global__ void k_rescale_h(__half *p, long long n, float f)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) p[q] = __float2half_rn(fminf(fmaxf(__half2float(p[q]) * f, -65504.0f), 65504.0f));
+}
+
+// Multiply the stored 16-bit memory variables by f (a power of two).
+static int rescale_r(swpc3d_ctx *c, float f)
+{
+    if (!c->s16 || f == 1.0f) return SWPC_OK;
+    for (int q = 0; q < 18; q++) {
+        const long long n = (long long)(c->R[q].bytes / 2);
+        if (n == 0) continue;
+        hipLaunchKernelGGL(k_rescale_h, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, (__half *)c->R[q].p,
+                           n, f);
+        SWPC_CHECK_LAUNCH();
+    }
+    return SWPC_OK;
+}
+
 int swpc3d_set_tuning(swpc3d_ctx *c, const int *p, int n)
 {
     clear_error();
     if (c) graph_drop(c);  // the captured steps would launch stale arguments
     if (!c || (n > 0 && !p)) return fail(SWPC_EINVAL, "null argument");
-    int v[17] = {c->s_by, c->s_wpe, c->s_split, c->v_by, c->v_wpe, c->xcd, c->s_kc, c->v_kc, c->x4, c->x4_by, c->pml4,
-                 c->x4_form, c->x4_vby, c->conc, c->x4_vz, c->x4_sz, c->graph_k};
-    for (int q = 0; q < n && q < 17; q++) v[q] = p[q];
+    int v[19] = {c->s_by, c->s_wpe, c->s_split, c->v_by, c->v_wpe, c->xcd, c->s_kc, c->v_kc, c->x4, c->x4_by, c->pml4,
+                 c->x4_form, c->x4_vby, c->conc, c->x4_vz, c->x4_sz, c->graph_k, c->sr, c->r_shift};
+    for (int q = 0; q < n && q < 19; q++) v[q] = p[q];
+    if (v[18] < -24 || v[18] > 24) return fail(SWPC_EINVAL, "memory-variable scale shift must be in -24..24");
     if (v[16] < 0 || v[16] > 1024) return fail(SWPC_EINVAL, "steps per graph must be in 0..1024");
     if (v[14] < 0 || v[15] < 0) return fail(SWPC_EINVAL, "negative z-chunk");
     if (v[6] < 0 || v[7] < 0) return fail(SWPC_EINVAL, "negative z-chunk");
@@ -3237,6 +3329,13 @@ int swpc3d_set_tuning(swpc3d_ctx *c, const int *p, int n)
     c->xcd = v[5] != 0; c->s_kc = v[6]; c->v_kc = v[7]; c->x4 = v[8] != 0; c->x4_by = v[9]; c->pml4 = v[10] != 0;
     c->x4_form = v[11]; c->x4_vby = v[12]; c->conc = v[13] != 0; c->x4_vz = v[14]; c->x4_sz = v[15];
     c->graph_k = v[16];
+    c->sr = v[17] != 0;
+    if (v[18] != c->r_shift) {
+        // the stored memory variables are rescaled with the new scale
+        if (int r = rescale_r(c, std::ldexp(1.0f, c->r_shift - v[18]))) return r;
+        c->r_shift = v[18];
+        c->r_scale = std::ldexp(c->r_scale0, c->r_shift);
+    }
     return SWPC_OK;
 }
 
