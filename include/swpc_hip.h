@@ -266,8 +266,8 @@ int swpc3d_set_tuning(swpc3d_ctx *ctx, const int *p, int n);
    accumulation; the output layer Ws x + W3 h2 + b3 is FP32. */
 int swpc_lab_nparam(void);
 /* Switch the learned boundary on with params[10][swpc_lab_nparam()], or
-   off with params = NULL.  Needs na = 0, SWPC_FP32 or SWPC_FP32_R16, and
-   set_medium + set_density first. */
+   off with params = NULL.  Needs na = 0, an FP32 medium (any precision
+   but SWPC_FP32_S16) and set_medium + set_density first. */
 int swpc3d_set_lab(swpc3d_ctx *ctx, const float *params);
 
 /* Training (no counterpart in the reference).  A trainer holds up to cap

@@ -2180,14 +2180,15 @@ static int join_side(swpc3d_ctx *c, hipStream_t ps)
     return SWPC_OK;
 }
 
-// What the learned-boundary kernels need of the context (FP32 fields).
+// What the learned-boundary kernels need of the context.
 static LabGrid lab_grid(const swpc3d_ctx *c)
 {
     LabGrid g;
     g.px = c->L.px;
     g.sxy = c->L.sxy;
-    for (int q = 0; q < 3; q++) g.F[q] = (float *)c->V[q].p;
-    for (int q = 0; q < 6; q++) g.F[3 + q] = (float *)c->S[q].p;
+    for (int q = 0; q < 3; q++) g.F[q] = c->V[q].p;
+    for (int q = 0; q < 6; q++) g.F[3 + q] = c->S[q].p;
+    g.f64 = c->d.precision == SWPC_FP64 ? 1 : 0;
     g.rho = (const float *)c->rho.p;
     g.lam = (const float *)c->med[MLAM].p;
     g.mu = (const float *)c->med[MMU].p;
@@ -2213,9 +2214,7 @@ static int launch_lab(swpc3d_ctx *c, int phase)
 template <typename T>
 static int launch_stress(swpc3d_ctx *c)
 {
-    if constexpr (sizeof(T) == 4) {
-        if (int r = launch_lab(c, 0)) return r;
-    }
+    if (int r = launch_lab(c, 0)) return r;
     K3<T> a = make_args<T>(c);
     const bool f64 = sizeof(T) == 8;
     const int by = f64 ? 4 : c->s_by, wpe = f64 ? 0 : c->s_wpe;
@@ -2275,9 +2274,7 @@ static int launch_stress(swpc3d_ctx *c)
 template <typename T>
 static int launch_vel(swpc3d_ctx *c)
 {
-    if constexpr (sizeof(T) == 4) {
-        if (int r = launch_lab(c, 1)) return r;
-    }
+    if (int r = launch_lab(c, 1)) return r;
     K3<T> a = make_args<T>(c);
     const bool f64 = sizeof(T) == 8;
     const int by = f64 ? 4 : c->v_by, wpe = f64 ? 0 : c->v_wpe;
@@ -2787,8 +2784,7 @@ static uint16_t bf16_rne(float f)
 static int lab_check(const swpc3d_ctx *c, bool need_na0)
 {
     if (!c) return fail(SWPC_EINVAL, "null context");
-    if (c->d.precision == SWPC_FP64 || c->m16)
-        return fail(SWPC_EINVAL, "the learned boundary runs with FP32 fields and an FP32 medium (SWPC_FP32 or SWPC_FP32_R16)");
+    if (c->m16) return fail(SWPC_EINVAL, "the learned boundary needs an FP32 medium (not SWPC_FP32_S16)");
     if (need_na0 && c->d.na != 0) return fail(SWPC_EINVAL, "the learned boundary replaces the PML: create the context with na = 0");
     if (!c->have_medium || !c->rho.p) return fail(SWPC_ESTATE, "set_medium and set_density must precede the learned boundary");
     return SWPC_OK;

@@ -60,10 +60,11 @@ struct LabDomain {
 
 LabDomain lab_domain(int oi, int oj, int nx, int ny, int nz);
 
-// What the lab kernels need of a 3-D context (FP32 fields).
+// What the lab kernels need of a 3-D context.
 struct LabGrid {
     int px, sxy;
-    float *F[9];            // Vx Vy Vz Sxx Syy Szz Syz Sxz Sxy
+    void *F[9];             // Vx Vy Vz Sxx Syy Szz Syz Sxz Sxy (FP32, or FP64 with f64)
+    int f64 = 0;
     const float *rho, *lam, *mu;
     float dt, h[3];
 };

@@ -62,7 +62,8 @@ LabDomain lab_domain(int oi, int oj, int nx, int ny, int nz)
 
 struct LabDev {
     int px, sxy;
-    float *F[9];
+    void *F[9];             // FP32 or FP64 fields (f64)
+    int f64;
     const float *rho, *lam, *mu;
     float dt, h[3];
     int oi, oj, nx, ny, nz;
@@ -75,6 +76,7 @@ static LabDev make_dev(const LabGrid &g, const LabDomain &d)
     LabDev a;
     a.px = g.px; a.sxy = g.sxy;
     for (int q = 0; q < 9; q++) a.F[q] = g.F[q];
+    a.f64 = g.f64;
     a.rho = g.rho; a.lam = g.lam; a.mu = g.mu;
     a.dt = g.dt;
     for (int q = 0; q < 3; q++) a.h[q] = g.h[q];
@@ -82,6 +84,19 @@ static LabDev make_dev(const LabGrid &g, const LabDomain &d)
     for (int f = 0; f < 5; f++) a.ncol[f] = d.ncol[f];
     for (int f = 0; f < 6; f++) a.col0[f] = d.col0[f];
     return a;
+}
+
+// Field value at an element, FP32 or FP64 state (T).
+template <typename T>
+__device__ __forceinline__ float lab_ld(const LabDev &a, int fi, int e)
+{
+    return (float)((const T *)a.F[fi])[e];
+}
+
+template <typename T>
+__device__ __forceinline__ void lab_st(const LabDev &a, int fi, int e, float v)
+{
+    ((T *)a.F[fi])[e] = (T)v;
 }
 
 // Address of strip cell d = 1 of column col of face f, and the address
@@ -138,6 +153,7 @@ __global__ void __launch_bounds__(256) k_lab_colmed(const LabDev a, float4 *__re
 
 // Features of one column into x[LAB_NIN]; prev holds the previous ghost
 // values (stresses already divided by rho*vp).  Returns N (0: no wave).
+template <typename T>
 __device__ __forceinline__ float lab_features(const LabDev &a, int base, int step, float4 m, const float *prev,
                                               float *x)
 {
@@ -145,10 +161,9 @@ __device__ __forceinline__ float lab_features(const LabDev &a, int base, int ste
     float amax = 0.0f;
 #pragma unroll
     for (int fi = 0; fi < 9; fi++) {
-        const float *F = a.F[fi];
 #pragma unroll
         for (int d = 0; d < LAB_NS; d++) {
-            float v = F[base + d * step];
+            float v = lab_ld<T>(a, fi, base + d * step);
             if (fi >= 3) v = v * inv_imp;
             x[fi * LAB_NS + d] = v;
             amax = fmaxf(amax, fabsf(v));
@@ -263,6 +278,7 @@ constexpr int WAVE_LDS = 64 * LX * 2 + 64 * LH * 2;   // bytes per wave
 // v_mfma_f32_32x32x16_bf16): lane l holds A[l % 32][8 (l / 32) + 0..7] and
 // B[8 (l / 32) + 0..7][l % 32]; the accumulator's register r holds
 // D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
+template <typename T>
 __global__ void __launch_bounds__(64 * FILL_WAVES) k_lab_fill(const LabFillArgs p)
 {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -301,10 +317,10 @@ __global__ void __launch_bounds__(64 * FILL_WAVES) k_lab_fill(const LabFillArgs 
         for (int q = 0; q < LAB_NOUT; q++) {
             const int fi = c_tgt[net][q][0], h = c_tgt[net][q][1];
             gaddr[q] = base - h * step;
-            const float v = q < nout ? a.F[fi][gaddr[q]] : 0.0f;
+            const float v = q < nout ? lab_ld<T>(a, fi, gaddr[q]) : 0.0f;
             prev[q] = fi >= 3 ? v / m.z : v;
         }
-        N = lab_features(a, base, step, m, prev, x);
+        N = lab_features<T>(a, base, step, m, prev, x);
     } else {
 #pragma unroll
         for (int q = 0; q < LAB_NIN; q++) x[q] = 0.0f;
@@ -368,7 +384,7 @@ __global__ void __launch_bounds__(64 * FILL_WAVES) k_lab_fill(const LabFillArgs 
         const int fi = c_tgt[net][o][0];
         float v = live ? z * N : 0.0f;
         if (fi >= 3) v = v * m.z;
-        a.F[fi][gaddr[o]] = v;
+        lab_st<T>(a, fi, gaddr[o], v);
     }
 }
 
@@ -395,7 +411,8 @@ int lab_fill(const LabGrid &g, const LabDomain &dom, const void *w, const float4
         const int nb = (dom.ncol[f] + 64 * FILL_WAVES - 1) / (64 * FILL_WAVES);
         if (nb == 0) continue;
         p.face = f;
-        hipLaunchKernelGGL(k_lab_fill, dim3(nb), dim3(64 * FILL_WAVES), lds, s, p);
+        if (g.f64) hipLaunchKernelGGL(k_lab_fill<double>, dim3(nb), dim3(64 * FILL_WAVES), lds, s, p);
+        else hipLaunchKernelGGL(k_lab_fill<float>, dim3(nb), dim3(64 * FILL_WAVES), lds, s, p);
         SWPC_CHECK_LAUNCH();
     }
     return SWPC_OK;
@@ -450,6 +467,7 @@ struct LabGatherArgs {
     unsigned seed;
 };
 
+template <typename T>
 __global__ void __launch_bounds__(256) k_lab_gather(const LabGatherArgs p)
 {
     const LabDev &a = p.a;
@@ -468,11 +486,11 @@ __global__ void __launch_bounds__(256) k_lab_gather(const LabGatherArgs p)
     for (int q = 0; q < LAB_NOUT; q++) {
         prev[q] = pv[q];
         const int fi = c_tgt[net][q][0], h = c_tgt[net][q][1];
-        float v = q < nout ? a.F[fi][base - h * step] : 0.0f;
+        float v = q < nout ? lab_ld<T>(a, fi, base - h * step) : 0.0f;
         t[q] = fi >= 3 ? v / m.z : v;
     }
     float x[LAB_NIN];
-    const float N = lab_features(a, base, step, m, prev, x);
+    const float N = lab_features<T>(a, base, step, m, prev, x);
 #pragma unroll
     for (int q = 0; q < LAB_NOUT; q++) pv[q] = t[q];
     if (!(N > 0.0f) || !(m.y > 0.0f)) return;
@@ -532,7 +550,8 @@ int lab_gather(swpc_lab_trainer *tr, const LabGrid &g, int phase, float keep, un
     p.thr = thr;
     p.seed = seed;
     if (n == 0) return SWPC_OK;
-    hipLaunchKernelGGL(k_lab_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p);
+    if (g.f64) hipLaunchKernelGGL(k_lab_gather<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p);
+    else hipLaunchKernelGGL(k_lab_gather<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p);
     SWPC_CHECK_LAUNCH();
     return SWPC_OK;
 }
