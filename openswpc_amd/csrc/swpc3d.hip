@@ -3227,7 +3227,7 @@ int swpc3d_set_field(swpc3d_ctx *c, int f, const double *in)
         std::vector<uint16_t> raw(n);
         for (size_t e = 0; e < n; e++) {
             if (f < SWPC3D_ADE0) {
-                __half_raw hr = __half_raw(__float2half(h[e] / c->r_scale));
+                __half_raw hr = __half_raw(__float2half(std::fmin(std::fmax(h[e] / c->r_scale, -65504.0), 65504.0)));
                 raw[e] = hr.x;
             } else {
                 // round to nearest even, as __float2bfloat16

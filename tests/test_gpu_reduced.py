@@ -50,9 +50,11 @@ def test_reduced_storage_misfit_vs_fp64(rmed, form):
 def test_reduced_storage_set_get_roundtrip():
     """Memory variables and ADE arrays survive a set/get through their
     16-bit storage within the format's rounding."""
-    case = problem.case3d(48, 44, 40, 0.5, 0.04, 4, na=6, src_depth=4.0, nst_side=2, zbeg=-2.0)
+    # the memory-variable scale follows the source's whole moment: give it
+    # the full rise (1 s) so that values of the size set here are in range
+    case = problem.case3d(48, 44, 40, 0.5, 0.04, 40, na=6, src_depth=4.0, nst_side=2, zbeg=-2.0)
     rng = np.random.default_rng(5)
-    with Swpc3D(case.grid, case.medium, case.pml, case.glut, case.stations, 1, 4, precision="fp32_s16") as sim:
+    with Swpc3D(case.grid, case.medium, case.pml, case.glut, case.stations, 1, 40, precision="fp32_s16") as sim:
         a = rng.standard_normal(case.grid.shape) * 1e-3
         sim.set_field("Rxx2", a)
         b = sim.field("Rxx2")
