@@ -43,7 +43,7 @@ module swpc_hip
     public :: swpc2d_set_sources, swpc2d_set_receivers, swpc2d_update_stress, swpc2d_update_vel
     public :: swpc2d_stressglut, swpc2d_store_receivers, swpc2d_step, swpc2d_run, swpc2d_get_receivers
     public :: swpc2d_vmax, swpc2d_sync
-    public :: swpc_lab_nparam, swpc3d_set_lab
+    public :: swpc_lab_nparam, swpc3d_set_lab, swpc3d_set_lab_sponge
     public :: swpc_batch_create, swpc_batch_destroy, swpc_batch_add3d, swpc_batch_add2d, swpc_batch_run
     public :: swpc_batch_sync
 
@@ -351,6 +351,12 @@ module swpc_hip
             import :: c_int, c_ptr, c_float
             type(c_ptr), value :: ctx
             real(c_float), intent(in) :: params(*)
+        end function
+
+        integer(c_int) function swpc3d_set_lab_sponge(ctx, sponge) bind(C, name='swpc3d_set_lab_sponge')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            real(c_float), value :: sponge
         end function
 
         ! ---------------------------------------- batches of independent runs

@@ -263,12 +263,22 @@ int swpc3d_set_tuning(swpc3d_ctx *ctx, const int *p, int n);
    W2[32][32] b2[32] W3[5][32] b3[5] Ws[5][61]; the time step uses them
    rounded to BF16 (nearest even).  Its two hidden layers run on the
    matrix cores with BF16 inputs (features, layer-1 activations) and FP32
-   accumulation; the output layer Ws x + W3 h2 + b3 is FP32. */
+   accumulation; the output layer Ws x + W3 h2 + b3 is FP32.  After the
+   ghosts of a phase are written, a strip sponge multiplies that phase's
+   own fields (velocities at phase 0, stresses at phase 1) at strip cell
+   d = 1..6 (d = 1 outermost) by 1 - s ((7 - d)/6)^2, the factor formed in
+   FP64 and rounded to FP32.  The nets alone are not a stable closed loop
+   over thousands of steps (a mode confined to the strip grows); the
+   sponge, default s = 0.02, removes it (the 64 x 64 x 48 test grid grows
+   without bound at s <= 0.005 and decays at s >= 0.01). */
 int swpc_lab_nparam(void);
 /* Switch the learned boundary on with params[10][swpc_lab_nparam()], or
    off with params = NULL.  Needs na = 0, an FP32 medium (any precision
    but SWPC_FP32_S16) and set_medium + set_density first. */
 int swpc3d_set_lab(swpc3d_ctx *ctx, const float *params);
+/* Strip sponge strength s of the learned boundary (0 <= s < 1; 0 = none;
+   default 0.02). */
+int swpc3d_set_lab_sponge(swpc3d_ctx *ctx, float sponge);
 
 /* Training (no counterpart in the reference).  A trainer holds up to cap
    samples per net on the device.  Samples come from wide-PML runs: the

@@ -72,6 +72,7 @@ _PROTOS = {
     "swpc3d_set_tuning": (I, [P, PI_, I]),
     "swpc_lab_nparam": (I, []),
     "swpc3d_set_lab": (I, [P, PF]),
+    "swpc3d_set_lab_sponge": (I, [P, C.c_float]),
     "swpc_lab_trainer_create": (I, [I, C.POINTER(P)]),
     "swpc_lab_trainer_destroy": (I, [P]),
     "swpc_lab_begin_run": (I, [P, P, I, I, I, I, I, PI_]),

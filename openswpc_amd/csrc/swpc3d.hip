@@ -1852,6 +1852,7 @@ struct swpc3d_ctx {
     swpc::DBuf lab_w;       // its ten nets: BF16 values (nearest even) widened to FP32
     swpc::DBuf lab_colmed;  // per-column medium features
     swpc::LabDomain lab_dom;
+    float lab_sponge = swpc::LAB_SPONGE;  // strip sponge strength (swpc3d_set_lab_sponge)
     bool s16 = false;       // memory variables and ADE arrays in 16 bit (SWPC_FP32_S16, SWPC_FP32_R16)
     bool m16 = false;       // medium in 16 bit too (SWPC_FP32_S16)
     float r_scale = 1.0f;   // memory-variable scale of the reduced storage
@@ -2206,7 +2207,8 @@ static int launch_lab(swpc3d_ctx *c, int phase)
     hipEvent_t e0;
     const int cls = phase == 0 ? 1 : 3;
     if (int r = prof_begin(c->prof, cls, c->stream, &e0)) return r;
-    if (int r = lab_fill(lab_grid(c), c->lab_dom, c->lab_w.p, (const float4 *)c->lab_colmed.p, phase, c->stream))
+    if (int r = lab_fill(lab_grid(c), c->lab_dom, c->lab_w.p, (const float4 *)c->lab_colmed.p, phase, c->lab_sponge,
+                         c->stream))
         return r;
     return prof_end(c->prof, cls, c->stream, e0);
 }
@@ -2815,6 +2817,16 @@ int swpc3d_set_lab(swpc3d_ctx *c, const float *params)
     if (int r = lab_colmed(lab_grid(c), c->lab_dom, c->lab_colmed, c->bytes, c->stream)) return r;
     SWPC_HIP(hipStreamSynchronize(c->stream));
     c->lab = true;
+    return SWPC_OK;
+}
+
+int swpc3d_set_lab_sponge(swpc3d_ctx *c, float sponge)
+{
+    clear_error();
+    if (c) graph_drop(c);  // the captured steps would launch stale arguments
+    if (!c) return fail(SWPC_EINVAL, "null context");
+    if (!(sponge >= 0.0f && sponge < 1.0f)) return fail(SWPC_EINVAL, "sponge must be in [0, 1)");
+    c->lab_sponge = sponge;
     return SWPC_OK;
 }
 

@@ -43,6 +43,7 @@ constexpr int LAB_B3 = LAB_W3 + LAB_NOUT * LAB_NH;
 constexpr int LAB_WS = LAB_B3 + LAB_NOUT;
 constexpr int LAB_NP = LAB_WS + LAB_NOUT * LAB_NIN;   // parameters per net
 constexpr int LAB_NNET = 10;
+constexpr float LAB_SPONGE = 0.02f;  // default strip sponge strength
 constexpr float LAB_TINY = 1e-30f;  // columns whose largest |feature| is below this count as empty
 // Training sample record (floats): x[0, 61) | targets / N (/ rho*vp for a
 // stress) [61, 66) | column id, N, rho*vp [66, 69) | zero pad to 72.
@@ -71,8 +72,14 @@ struct LabGrid {
 
 // Column medium (vs/vp, vp*dt/h, rho*vp) of every column of dom.
 int lab_colmed(const LabGrid &g, const LabDomain &dom, DBuf &out, long long &bytes, hipStream_t s);
-// Ghost fill of one phase; w holds the ten nets (BF16 values widened to FP32).
-int lab_fill(const LabGrid &g, const LabDomain &dom, const void *w, const float4 *colmed, int phase, hipStream_t s);
+// Ghost fill of one phase; w holds the ten nets (BF16 values widened to
+// FP32).  With sponge > 0 the phase's own fields (V at phase 0, S at
+// phase 1) at strip cell d = 1..6 of every column are then multiplied by
+// 1 - sponge ((7 - d)/6)^2 (lab_sponge_factors, formed in FP64 and
+// rounded to FP32).
+int lab_fill(const LabGrid &g, const LabDomain &dom, const void *w, const float4 *colmed, int phase, float sponge,
+             hipStream_t s);
+void lab_sponge_factors(float sponge, float *sp);
 
 }  // namespace swpc
 

@@ -233,6 +233,8 @@ struct LabFillArgs {
     const float4 *colmed;
     int phase;
     int face;                  // one launch per face
+    float sp[LAB_NS];          // strip sponge factor of cell d = 1..6 (all 1: no sponge)
+    int sponge;                // any factor below 1
 };
 
 // tanh(z) = 1 - 2/(1 + e^(2z)) with the hardware exponential and a fast
@@ -386,6 +388,17 @@ __global__ void __launch_bounds__(64 * FILL_WAVES) k_lab_fill(const LabFillArgs 
         if (fi >= 3) v = v * m.z;
         lab_st<T>(a, fi, gaddr[o], v);
     }
+    // strip sponge on the fields this phase's ghosts belong to (their
+    // values were read into the features above; the column owns its cells)
+    if (p.sponge) {
+        const int f0 = p.phase == 0 ? 0 : 3, f1 = p.phase == 0 ? 3 : 9;
+#pragma unroll 1
+        for (int fi = f0; fi < f1; fi++) {
+            T *F = (T *)a.F[fi];
+#pragma unroll
+            for (int d = 0; d < LAB_NS; d++) F[base + d * step] *= (T)p.sp[d];
+        }
+    }
 }
 
 int lab_colmed(const LabGrid &g, const LabDomain &dom, DBuf &out, long long &bytes, hipStream_t s)
@@ -399,9 +412,20 @@ int lab_colmed(const LabGrid &g, const LabDomain &dom, DBuf &out, long long &byt
     return SWPC_OK;
 }
 
-int lab_fill(const LabGrid &g, const LabDomain &dom, const void *w, const float4 *colmed, int phase, hipStream_t s)
+void lab_sponge_factors(float sponge, float *sp)
+{
+    for (int d = 0; d < LAB_NS; d++) {
+        const double r = (double)(LAB_NS - d) / LAB_NS;
+        sp[d] = (float)(1.0 - (double)sponge * r * r);
+    }
+}
+
+int lab_fill(const LabGrid &g, const LabDomain &dom, const void *w, const float4 *colmed, int phase, float sponge,
+             hipStream_t s)
 {
     LabFillArgs p;
+    lab_sponge_factors(sponge, p.sp);
+    p.sponge = sponge > 0.0f;
     p.a = make_dev(g, dom);
     p.w = (const float *)w;
     p.colmed = colmed;

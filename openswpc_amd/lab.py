@@ -38,6 +38,7 @@ NSC = 59                        # features scaled by 1/N
 REC = 72                        # floats per training sample
 REF_NA = 0                      # reference runs: no absorber, the edges are far away
 OFFSET = 110                    # inner box offset inside the reference grid (cells)
+SPONGE = 0.02                   # default strip sponge strength of the time step
 WEIGHTS = Path(__file__).resolve().parent / "data" / "lab_weights.npz"
 
 
@@ -333,11 +334,13 @@ def _inner(a, nxs, nys, nzs, off=0):
 
 
 def evaluate_boundary(params, seed=101, nxs=64, nys=64, nzs=48, dx=0.25, dt=0.01, nt=500, every=20, zbeg=-0.75,
-                      layers=problem.LHM_EXAMPLE, margin=6, extra=(20,), eps=0.05):
+                      layers=problem.LHM_EXAMPLE, margin=6, extra=(20,), eps=0.05, sponges=(SPONGE,)):
     """One held-out case run several ways on the GPU, measured against the
     reference run (the truth) over one region U: the inner box minus
     ``margin`` cells at each absorbing face.
-      lab      the inner box with the learned boundary (na = 0)
+      lab      the inner box with the learned boundary (na = 0) and the
+               default strip sponge; for each other s in sponges also
+               lab_s<s>, the same with sponge strength s
       zero     the same grid with zero ghosts (the boundary without a net)
       pml6     the same grid with a 6-cell ADE-PML (its interior is U)
       pml<n>   for n in extra: the box grown so that an n-cell PML leaves
@@ -352,8 +355,9 @@ def evaluate_boundary(params, seed=101, nxs=64, nys=64, nzs=48, dx=0.25, dt=0.01
     lc = wide_case(nxs, nys, nzs, dx, dt, nt, seed, zbeg=zbeg, layers=layers, eps=eps)
     wst, _ = station_cells3(lc.wide.grid, lc.wide.medium.kfs, lc.wide.medium.kob, _stations(lc, 4))
     runs = {"truth": (lc.wide, None, 0, wst)}
-    c = inner_case(lc, 0, nst_side=4)
-    runs["lab"] = (c, params, 0, c.stations)
+    for sp in sponges:
+        c = inner_case(lc, 0, nst_side=4)
+        runs["lab" if sp == SPONGE else f"lab_s{sp:g}"] = (c, (params, sp), 0, c.stations)
     c = inner_case(lc, 0, nst_side=4)
     runs["zero"] = (c, None, 0, c.stations)
     c = inner_case(lc, margin, nst_side=4)
@@ -365,7 +369,8 @@ def evaluate_boundary(params, seed=101, nxs=64, nys=64, nzs=48, dx=0.25, dt=0.01
     for k, (cs, p, gr, st) in runs.items():
         sims[k] = Swpc3D(cs.grid, cs.medium, cs.pml, cs.glut, st, 1, nt)
         if p is not None:
-            sims[k].set_learned_boundary(p)
+            sims[k].set_learned_boundary(p[0])
+            sims[k].set_learned_boundary_sponge(p[1])
         grow[k] = lc.off if k == "truth" else gr
     def region(k, f):
         o = grow[k]

@@ -262,6 +262,10 @@ class Swpc3D:
         p = _f32(params)
         check(lib().swpc3d_set_lab(self.h, ptr(p, C.c_float)), "set_lab")
 
+    def set_learned_boundary_sponge(self, sponge):
+        """Strip sponge strength of the learned boundary (0 = none, default 0.02)."""
+        check(lib().swpc3d_set_lab_sponge(self.h, float(sponge)), "swpc3d_set_lab_sponge")
+
     def close(self):
         if getattr(self, "h", None):
             lib().swpc3d_destroy(self.h)

@@ -266,18 +266,21 @@ class Oracle3D:
         return out
 
     # -- learned absorbing boundary (o3d_lab_*) ---------------------------
-    def set_lab(self, params):
+    def set_lab(self, params, sponge=0.02):
         """Learned boundary on (params[10][NP] as given, already rounded as
-        the time step uses them) or off (None); needs na = 0."""
+        the time step uses them; strip sponge strength `sponge`) or off
+        (None); needs na = 0."""
         if params is None:
             self.lab = None
             return
         assert self.case.grid.na == 0
         self.lab = np.ascontiguousarray(params, np.float32)
+        self.lab_sponge = float(sponge)
         self.rho = np.ascontiguousarray(self.case.medium.rho, np.float32)
 
     def lab_fill(self, phase):
-        self.L.o3d_lab_fill(self._ps(), phase, self.lab.ctypes.data_as(_PF), self.rho.ctypes.data_as(_PF))
+        self.L.o3d_lab_fill(self._ps(), phase, self.lab.ctypes.data_as(_PF), self.rho.ctypes.data_as(_PF),
+                            C.c_float(self.lab_sponge))
 
     def lab_gather_begin(self, oi, oj, nxs, nys, nzs):
         """Training gather for the inner box (oi, oj, nxs, nys, nzs); the

@@ -985,8 +985,14 @@ void oracle_lab_forward(const float *p, const float *x, float *y)
 /* Ghost fill of one phase over the whole grid: params holds the ten nets,
    LAB_NP floats each (net 2*face + phase); the previous ghost values are
    read back from the ghost cells before they are overwritten. */
-void o3d_lab_fill(o3d_t *s, int phase, const float *params, const float *rho)
+void o3d_lab_fill(o3d_t *s, int phase, const float *params, const float *rho, float sponge)
 {
+    /* strip sponge factor of cell d = 1..6, formed in double, rounded to float */
+    float sp[LAB_NS];
+    for (int d = 1; d <= LAB_NS; d++) {
+        double r = (double)(LAB_NS + 1 - d) / LAB_NS;
+        sp[d - 1] = (float)(1.0 - (double)sponge * r * r);
+    }
     for (int f = 0; f < 5; f++) {
         int net = 2 * f + phase, nA, nB, nout = lab_nout_tab[net];
         lab_extent(f, s->nx, s->ny, s->nz, &nA, &nB);
@@ -1016,6 +1022,13 @@ void o3d_lab_fill(o3d_t *s, int phase, const float *params, const float *rho)
                     if (fi >= 3) v = v * im;
                     lab_field(s, fi)[gc[q]] = (REAL)v;
                 }
+                /* the sponge on this phase's own fields of the column's strip */
+                if (sponge > 0.0f)
+                    for (int fi = phase == 0 ? 0 : 3; fi < (phase == 0 ? 3 : 9); fi++)
+                        for (int d = 1; d <= LAB_NS; d++) {
+                            lab_cell(f, a, b, d, s->nx, s->ny, s->nz, &i, &j, &k);
+                            lab_field(s, fi)[I3(s, k, i, j)] *= (REAL)sp[d - 1];
+                        }
             }
     }
 }

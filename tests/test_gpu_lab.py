@@ -118,22 +118,26 @@ def test_gather_matches_oracle():
     assert nrec > 1000
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp32_r16", "fp64"])
-def test_lab_step_matches_oracle(precision):
+@pytest.mark.parametrize("precision,sponge", [("fp32", None), ("fp32_r16", None), ("fp64", None), ("fp32", 0.0),
+                                              ("fp32", 0.3)])
+def test_lab_step_matches_oracle(precision, sponge):
     """A learned-boundary run (na = 0) with bounded demo nets: the GPU's
     fields and traces against the oracle's learned-boundary step (FP32,
-    or FP64 fields for the FP64 context; the nets run in FP32 either way)."""
+    or FP64 fields for the FP64 context; the nets run in FP32 either way),
+    with the default strip sponge (None), none, or a strong one."""
     from oracle.oracle import Oracle3D
     nt = 120
     lc = _small_wide(seed=6, nt=nt)
     c = lab.inner_case(lc, 0, nst_side=3)
     P = _demo_params(1)
     o = Oracle3D(c, 64 if precision == "fp64" else 32)
-    o.set_lab(P)
+    o.set_lab(P, sponge=lab.SPONGE if sponge is None else sponge)
     for it in range(1, nt + 1):
         o.lab_step(it)
     with Swpc3D(c.grid, c.medium, c.pml, c.glut, c.stations, 1, nt, precision=precision) as sim:
         sim.set_learned_boundary(P)
+        if sponge is not None:
+            sim.set_learned_boundary_sponge(sponge)
         sim.run(1, nt)
         vel, _ = sim.receivers()
         fields = {n: sim.field(n) for n in ("Vx", "Vz", "Sxx", "Sxz")}
@@ -176,11 +180,37 @@ def test_training_lowers_loss():
     tr.close()
 
 
+def test_trained_boundary_stable_over_long_runs():
+    """The committed nets with the default strip sponge on a 64 x 64 x 48
+    grid for 3000 steps (the source is done after ~200): the velocity
+    energy of the grid decays, window after window, once the direct waves
+    have left (without the sponge a mode confined to the strip grows
+    ~1.5 % per step from step ~1200 on)."""
+    if not lab.WEIGHTS.exists():
+        pytest.skip("no trained weights committed")
+    P, _ = lab.load_weights()
+    nt = 3000
+    lc = lab.wide_case(64, 64, 48, 0.25, 0.01, nt, 4242, off=8)
+    c = lab.inner_case(lc, 0)
+    e = []
+    with Swpc3D(c.grid, c.medium, c.pml, c.glut, None, 1, nt) as sim:
+        sim.set_learned_boundary(P)
+        for it in range(1, nt + 1, 400):
+            n = min(400, nt + 1 - it)
+            sim.run(it, n)
+            e.append(float(sum((sim.field(f)[3:-3, 3:-3, 3:-3].astype(np.float64) ** 2).sum()
+                               for f in ("Vx", "Vy", "Vz"))))
+    print("energy per 400 steps:", " ".join(f"{v:.2e}" for v in e))
+    assert np.isfinite(e).all()
+    assert all(b < a for a, b in zip(e[2:], e[3:])), e
+    assert e[-1] < 1e-2 * e[0]
+
+
 def test_trained_boundary_beats_no_net():
-    """The committed nets on a held-out case (seed not used in training):
-    stable over 500 steps and far closer to the echo-free reference run than the
-    same grid with zero ghosts; the 6- and 20-cell PML figures are
-    reported beside it."""
+    """The committed nets (default strip sponge) on a held-out case (seed
+    not used in training): stable over 500 steps and far closer to the
+    echo-free reference run than the same grid with zero ghosts; the 6-
+    and 20-cell PML figures are reported beside it."""
     if not lab.WEIGHTS.exists():
         pytest.skip("no trained weights committed")
     P, meta = lab.load_weights()

@@ -123,7 +123,7 @@ def test_fill_copy_nets_pin_the_layout(phase):
             _, _, _, _, _, _, Ws = olab.unpack(P[net])
             Ws[q, fsrc * 6 + d - 1] = 1.0
             picks[(net, q)] = (fi, h, fsrc, d)
-    o.set_lab(P)
+    o.set_lab(P, sponge=0.0)
     before = {n: o.f[n].copy() for n in F}
     o.lab_fill(phase)
     g = case.grid
@@ -153,7 +153,7 @@ def test_fill_previous_ghost_recurrence():
         _, _, _, _, _, _, Ws = olab.unpack(P[net])
         for q in range(lab.NOUT[net]):
             Ws[q, 54 + q] = 1.0
-    o.set_lab(P)
+    o.set_lab(P, sponge=0.0)
     air = ~((case.medium.lam + 2 * case.medium.mu) > 0)
     for phase in (0, 1):
         before = {n: o.f[n].copy() for n in F}
@@ -166,6 +166,40 @@ def test_fill_previous_ghost_recurrence():
                 got = o.f[F[fi]][_sl(o, face, 1 - h)]
                 want = np.where(solid, before[F[fi]][_sl(o, face, 1 - h)], 0.0)
                 assert np.allclose(got, want, rtol=3e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("phase", [0, 1])
+def test_fill_sponge_profile(phase):
+    """With all-zero nets the fill writes zero ghosts and the sponge alone
+    acts: every interior cell of the phase's own fields (V at 0, S at 1)
+    is multiplied, once per absorbing face whose strip holds it (depth d =
+    1..6 from that face), by the FP32 factor 1 - s ((7 - d)/6)^2; the other
+    phase's fields and the cells deeper than 6 are untouched."""
+    s = 0.3
+    case = _tiny_case(nx=16, ny=15, nz=14)
+    o = Oracle3D(case, 64)
+    _random_fields(o, 20 + phase)
+    o.set_lab(np.zeros((10, olab.NP), np.float32), sponge=s)
+    before = {n: o.f[n].copy() for n in F}
+    o.lab_fill(phase)
+    g = case.grid
+    i = np.arange(1, g.nx + 1)[None, :, None]
+    j = np.arange(1, g.ny + 1)[:, None, None]
+    k = np.arange(1, g.nz + 1)[None, None, :]
+    fac = np.ones((g.ny, g.nx, g.nz))
+    for depth in (i, g.nx + 1 - i, j, g.ny + 1 - j, g.nz + 1 - k):
+        r = np.float32(1.0 - s * ((7 - depth) / 6.0) ** 2).astype(np.float64)
+        fac = fac * np.where(depth <= 6, r, 1.0)
+    assert (fac < 1).sum() > 0 and (fac == 1).sum() > 0
+    inner = (slice(3, g.ny + 3), slice(3, g.nx + 3), slice(3, g.nz + 3))
+    for n in range(9):
+        got = o.f[F[n]][inner].astype(np.float64)
+        was = before[F[n]][inner].astype(np.float64)
+        own = (n < 3) == (phase == 0)
+        if own:
+            assert np.allclose(got, was * fac, rtol=1e-6, atol=0), F[n]
+        else:
+            assert (got == was).all(), F[n]
 
 
 def test_gather_records_match_fill_features_and_ghosts():
