@@ -31,6 +31,7 @@ import numpy as np  # noqa: E402
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 # algorithmic bytes per cell-update (compulsory traffic, FP32, NM = 3)
 B_INT_STRESS = 58 * 4   # V 3r, medium 7r, R 18r/18w, S 6r/6w
+B_INT_STRESS_NEED = 55 * 4   # what k3x_stress streams: muyz, muxz, muxy come from mu and its neighbours
 B_INT_VEL = 15 * 4      # S 6r, b 3r, V 3r/3w
 B_PML_STRESS = 38 * 4
 B_PML_VEL = 33 * 4
@@ -100,6 +101,7 @@ def main():
     avg_stress = ms[0] / max(cnt[0], 1) * 1e-3
     alg_stress = B_INT_STRESS * n_int
     achieved = alg_stress / avg_stress
+    need_stress = B_INT_STRESS_NEED - 4 * plane_constant_arrays(case)
     step_alg = (B_INT_STRESS + B_INT_VEL) * n_int + (B_PML_STRESS + B_PML_VEL) * n_pml
     kern = {name: {"avg_us": ms[q] / max(cnt[q], 1) * 1e3, "launches": int(cnt[q])}
             for q, name in enumerate(["stress_int", "stress_pml", "vel_int", "vel_pml", "other"])}
@@ -176,6 +178,9 @@ def main():
             "frac": achieved / HBM_PEAK,
             "traffic": traffic_from_profile(),
             "alg_bytes_per_launch": alg_stress,
+            "alg_bytes_per_cell": B_INT_STRESS,
+            "kernel_streams_bytes_per_cell": need_stress,
+            "kernel_streams_GBs": need_stress * n_int / avg_stress / 1e9,
             "avg_launch_us": avg_stress * 1e6,
             "measured_copy_GBs": copy_bw,
         },
@@ -189,6 +194,21 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def plane_constant_arrays(case):
+    """Mean number, over the interior planes, of the stress update's
+    medium arrays (lam, taup, taus) that the library reads from its
+    per-plane table: a plane qualifies when every cell i, j in [1, n] holds
+    the same value, bit for bit (swpc3d_set_medium).  Each such array is
+    4 bytes per cell the kernel does not stream."""
+    g, m = case.grid, case.medium
+    tot = 0.0
+    for a in (m.lam, m.taup, m.taus):
+        u = np.ascontiguousarray(a[3:3 + g.ny, 3:3 + g.nx, :], np.float32).view(np.uint32)
+        flags = [bool((u[:, :, k + 2] == u[0, 0, k + 2]).all()) for k in range(1, g.nz - g.na + 1)]
+        tot += float(np.mean(flags))
+    return tot
 
 
 def timed_run(case, precision, warmup, steps, lab=None):

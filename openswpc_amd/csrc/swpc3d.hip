@@ -58,6 +58,10 @@ struct K3 {
     // reduced storage (SWPC_FP32_S16): FP32 arithmetic, V and S in FP32
     const __half *hmed[10];     // medium: FP16 residual about a per-plane FP32 base
     const float2 *mbs[10];      // per-plane (base, scale), indexed k + KO
+    // FP32 medium: per-plane value of an array that is constant over the
+    // plane's cells i, j in [1, n] (NaN where the plane varies), indexed
+    // k + KO; null when no plane of the array is constant
+    const float *cpl[10];
     __half *hR[18];             // memory variables stored as R * r_inv
     float r_scale, r_inv;
     int sr;                     // 16-bit stores round stochastically (else to nearest)
@@ -439,14 +443,18 @@ template <bool S16>
 __device__ __forceinline__ f4v med4(const K3<float> &a, int q, int c, int k)
 {
     if constexpr (S16) {
-        if (!a.m16) return ld4nt(a.med[q] + c);
-        const float2 bs = a.mbs[q][k + KO];
-        f4v v = {bs.x, bs.x, bs.x, bs.x};
-        if (bs.y != 0.0f) v += bs.y * h4(a.hmed[q] + c);
-        return v;
-    } else {
-        return ld4nt(a.med[q] + c);
+        if (a.m16) {
+            const float2 bs = a.mbs[q][k + KO];
+            f4v v = {bs.x, bs.x, bs.x, bs.x};
+            if (bs.y != 0.0f) v += bs.y * h4(a.hmed[q] + c);
+            return v;
+        }
     }
+    if (a.cpl[q]) {   // a plane-constant array costs no stream
+        const float v = a.cpl[q][k + KO];
+        if (v == v) return f4v{v, v, v, v};
+    }
+    return ld4nt(a.med[q] + c);
 }
 
 template <bool S16>
@@ -1847,9 +1855,11 @@ struct swpc3d_ctx {
     swpc::DBuf V[3], S[6], R[18], A[18];
     swpc::DBuf med[10];     // bx by bz lam mu taup taus muyz muxz muxy
     swpc::DBuf mbs[10];     // reduced storage: per-plane (base, scale)
+    swpc::DBuf cpl[10];     // FP32 medium: per-plane constant values (K3::cpl)
+    bool use_cpl[10] = {};
     swpc::DBuf rho;         // optional density (swpc3d_set_density)
     bool lab = false;       // learned absorbing boundary on (swpc3d_set_lab)
-    swpc::DBuf lab_w;       // its ten nets: BF16 values (nearest even) widened to FP32
+    swpc::DBuf lab_w;       // its ten nets as the fill kernel's LDS images (BF16 values, nearest even)
     swpc::DBuf lab_colmed;  // per-column medium features
     swpc::LabDomain lab_dom;
     float lab_sponge = swpc::LAB_SPONGE;  // strip sponge strength (swpc3d_set_lab_sponge)
@@ -1913,6 +1923,7 @@ static K3<T> make_args(swpc3d_ctx *c)
     for (int q = 0; q < 18; q++) a.R[q] = (T *)c->R[q].p;
     for (int q = 0; q < 18; q++) a.A[q] = (T *)c->A[q].p;
     for (int q = 0; q < 10; q++) a.med[q] = (const float *)c->med[q].p;
+    for (int q = 0; q < 10; q++) a.cpl[q] = !c->m16 && c->use_cpl[q] ? (const float *)c->cpl[q].p : nullptr;
     a.rho = c->m16 ? nullptr : (const float *)c->rho.p;
     a.m16 = c->m16 ? 1 : 0;
     if (c->s16) {
@@ -2686,6 +2697,7 @@ int swpc3d_destroy(swpc3d_ctx *c)
     for (auto &b : c->A) dfree(b);
     for (auto &b : c->med) dfree(b);
     for (auto &b : c->mbs) dfree(b);
+    for (auto &b : c->cpl) dfree(b);
     dfree(c->rho);
     dfree(c->lab_w);
     dfree(c->lab_colmed);
@@ -2726,6 +2738,29 @@ int swpc3d_set_medium(swpc3d_ctx *c, const float *bx, const float *by, const flo
         box_to_dev(c->L, in[q], h.data());
         if (!c->m16) {
             SWPC_HIP(hipMemcpy(c->med[q].p, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+            // planes whose cells i, j in [1, n] hold one value (bit for bit)
+            // are read from a per-plane table instead of the array
+            const Lay3 &L = c->L;
+            std::vector<float> pc((size_t)L.pz, std::nanf(""));
+            bool any = false;
+            for (int k = 1; k <= L.nz; k++) {
+                const float v0 = h[at3(L.px, L.sxy, 1, 1, k)];
+                bool same = v0 == v0;
+                for (int j = 1; j <= L.ny && same; j++) {
+                    const float *row = h.data() + at3(L.px, L.sxy, 1, j, k);
+                    for (int i = 0; i < L.nx && same; i++) same = memcmp(row + i, &v0, 4) == 0;
+                }
+                if (same) {
+                    pc[k + KO] = v0;
+                    any = true;
+                }
+            }
+            c->use_cpl[q] = any;
+            if (any) {
+                if (!c->cpl[q].p)
+                    if (int r = dalloc(c->cpl[q], pc.size() * 4, c->bytes)) return r;
+                SWPC_HIP(hipMemcpy(c->cpl[q].p, pc.data(), pc.size() * 4, hipMemcpyHostToDevice));
+            }
             continue;
         }
         // per-plane base (mean over the cells the kernels read, i, j in
@@ -2802,17 +2837,19 @@ int swpc3d_set_lab(swpc3d_ctx *c, const float *params)
     }
     if (int r = lab_check(c, true)) return r;
     SWPC_HIP(hipStreamSynchronize(c->stream));
-    // the nets are rounded to BF16 and kept widened to FP32, the form the
-    // fill kernel reads through the scalar cache
+    // the nets are rounded to BF16 and laid out as the fill kernel's LDS
+    // images (lab_fill_image)
     std::vector<float> w((size_t)LAB_NNET * LAB_NP);
     for (size_t q = 0; q < w.size(); q++) {
         if (!std::isfinite(params[q])) return fail(SWPC_EINVAL, "learned-boundary parameter %zu is not finite", q);
         const uint32_t b = (uint32_t)bf16_rne(params[q]) << 16;
         memcpy(&w[q], &b, 4);
     }
+    std::vector<unsigned char> img((size_t)LAB_NNET * LAB_IMG_BYTES);
+    lab_fill_image(w.data(), img.data());
     if (!c->lab_w.p)
-        if (int r = dalloc(c->lab_w, w.size() * 4, c->bytes)) return r;
-    SWPC_HIP(hipMemcpy(c->lab_w.p, w.data(), w.size() * 4, hipMemcpyHostToDevice));
+        if (int r = dalloc(c->lab_w, img.size(), c->bytes)) return r;
+    SWPC_HIP(hipMemcpy(c->lab_w.p, img.data(), img.size(), hipMemcpyHostToDevice));
     c->lab_dom = lab_domain(0, 0, c->d.nx, c->d.ny, c->d.nz);
     if (int r = lab_colmed(lab_grid(c), c->lab_dom, c->lab_colmed, c->bytes, c->stream)) return r;
     SWPC_HIP(hipStreamSynchronize(c->stream));

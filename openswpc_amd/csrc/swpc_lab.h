@@ -72,12 +72,17 @@ struct LabGrid {
 
 // Column medium (vs/vp, vp*dt/h, rho*vp) of every column of dom.
 int lab_colmed(const LabGrid &g, const LabDomain &dom, DBuf &out, long long &bytes, hipStream_t s);
-// Ghost fill of one phase; w holds the ten nets (BF16 values widened to
-// FP32).  With sponge > 0 the phase's own fields (V at phase 0, S at
+// Bytes of one net's image: its weights in the fill kernel's LDS form
+// (W1 and W2 in BF16, zero-padded rows; the output layer and the hidden
+// biases in FP32).
+constexpr int LAB_IMG_BYTES = 9344;
+// The ten nets' images from w[10][LAB_NP] (values already rounded to BF16).
+void lab_fill_image(const float *w, unsigned char *img);
+// Ghost fill of one phase; img holds the ten nets' images.  With sponge > 0 the phase's own fields (V at phase 0, S at
 // phase 1) at strip cell d = 1..6 of every column are then multiplied by
 // 1 - sponge ((7 - d)/6)^2 (lab_sponge_factors, formed in FP64 and
 // rounded to FP32).
-int lab_fill(const LabGrid &g, const LabDomain &dom, const void *w, const float4 *colmed, int phase, float sponge,
+int lab_fill(const LabGrid &g, const LabDomain &dom, const void *img, const float4 *colmed, int phase, float sponge,
              hipStream_t s);
 void lab_sponge_factors(float sponge, float *sp);
 

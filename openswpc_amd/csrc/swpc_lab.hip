@@ -26,6 +26,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <cstring>
 
 namespace swpc {
 
@@ -151,19 +153,81 @@ __global__ void __launch_bounds__(256) k_lab_colmed(const LabDev a, float4 *__re
     out[g] = m;
 }
 
-// Features of one column into x[LAB_NIN]; prev holds the previous ghost
-// values (stresses already divided by rho*vp).  Returns N (0: no wave).
+// The six strip values (cells d = 1..6) of one field of a column.  The
+// x faces' columns lie along a row, one row per lane: with FP32 state and
+// a 16-byte or 8-byte aligned first cell, one 16-byte and one 8-byte
+// access cover the six cells (two cache-line requests per lane instead of
+// six).  The alignment is the same for every column of a face (rows start
+// at multiples of 32 elements).
 template <typename T>
-__device__ __forceinline__ float lab_features(const LabDev &a, int base, int step, float4 m, const float *prev,
-                                              float *x)
+__device__ __forceinline__ void lab_strip_load(const T *__restrict__ F, int base, int step, bool xrow, T *v)
+{
+    if constexpr (sizeof(T) == 4) {
+        const int lo = step > 0 ? base : base - (LAB_NS - 1);
+        if (xrow && (lo & 1) == 0) {
+            float w[LAB_NS];
+            if ((lo & 3) == 0) {
+                const float4 A = *(const float4 *)(F + lo);
+                const float2 B = *(const float2 *)(F + lo + 4);
+                w[0] = A.x; w[1] = A.y; w[2] = A.z; w[3] = A.w; w[4] = B.x; w[5] = B.y;
+            } else {
+                const float2 A = *(const float2 *)(F + lo);
+                const float4 B = *(const float4 *)(F + lo + 2);
+                w[0] = A.x; w[1] = A.y; w[2] = B.x; w[3] = B.y; w[4] = B.z; w[5] = B.w;
+            }
+#pragma unroll
+            for (int d = 0; d < LAB_NS; d++) v[d] = step > 0 ? w[d] : w[LAB_NS - 1 - d];
+            return;
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < LAB_NS; d++) v[d] = F[base + d * step];
+}
+
+// Store v[d] * sp[d] back to the six strip cells (the sponge).
+template <typename T>
+__device__ __forceinline__ void lab_strip_store(T *__restrict__ F, int base, int step, bool xrow, const float *sp,
+                                                const T *v)
+{
+    T u[LAB_NS];
+#pragma unroll
+    for (int d = 0; d < LAB_NS; d++) u[d] = v[d] * (T)sp[d];
+    if constexpr (sizeof(T) == 4) {
+        const int lo = step > 0 ? base : base - (LAB_NS - 1);
+        if (xrow && (lo & 1) == 0) {
+            float w[LAB_NS];
+#pragma unroll
+            for (int d = 0; d < LAB_NS; d++) w[d] = step > 0 ? u[d] : u[LAB_NS - 1 - d];
+            if ((lo & 3) == 0) {
+                *(float4 *)(F + lo) = make_float4(w[0], w[1], w[2], w[3]);
+                *(float2 *)(F + lo + 4) = make_float2(w[4], w[5]);
+            } else {
+                *(float2 *)(F + lo) = make_float2(w[0], w[1]);
+                *(float4 *)(F + lo + 2) = make_float4(w[2], w[3], w[4], w[5]);
+            }
+            return;
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < LAB_NS; d++) F[base + d * step] = u[d];
+}
+
+// Features of one column into x[LAB_NIN]; prev holds the previous ghost
+// values (stresses already divided by rho*vp).  raw[LAB_NF] receives the
+// strip values as stored (field-major).  Returns N (0: no wave).
+template <typename T>
+__device__ __forceinline__ float lab_features(const LabDev &a, int base, int step, bool xrow, float4 m,
+                                              const float *prev, float *x, T *raw)
 {
     const float inv_imp = 1.0f / m.z;
     float amax = 0.0f;
 #pragma unroll
+    for (int fi = 0; fi < 9; fi++) lab_strip_load<T>((const T *)a.F[fi], base, step, xrow, raw + fi * LAB_NS);
+#pragma unroll
     for (int fi = 0; fi < 9; fi++) {
 #pragma unroll
         for (int d = 0; d < LAB_NS; d++) {
-            float v = lab_ld<T>(a, fi, base + d * step);
+            float v = (float)raw[fi * LAB_NS + d];
             if (fi >= 3) v = v * inv_imp;
             x[fi * LAB_NS + d] = v;
             amax = fmaxf(amax, fabsf(v));
@@ -229,10 +293,11 @@ __device__ __forceinline__ void lab_forward(const float *__restrict__ W, const f
 
 struct LabFillArgs {
     LabDev a;
-    const float *w;            // ten nets, BF16 values widened to FP32
+    const uint4 *img;          // ten nets in their LDS form (lab_fill_image)
     const float4 *colmed;
     int phase;
-    int face;                  // one launch per face
+    int face[2];               // blocks [0, nb0) fill face[0], the rest face[1]
+    int nb0;
     float sp[LAB_NS];          // strip sponge factor of cell d = 1..6 (all 1: no sponge)
     int sponge;                // any factor below 1
 };
@@ -267,16 +332,23 @@ constexpr int LH = 40;     // hidden layer 1, BF16 [64][40]
 constexpr int LH2 = 33;    // hidden layer 2, FP32 [64][33] (in the feature rows' space)
 constexpr int LW1 = 72;    // W1 as BF16 [32][72]
 constexpr int LW2 = 40;    // W2 as BF16 [32][40]
+constexpr int LO = 96;     // output layer as FP32 [5][96]: W3 row, Ws row, b3
 constexpr int FILL_WAVES = 4;
 constexpr int WAVE_LDS = 64 * LX * 2 + 64 * LH * 2;   // bytes per wave
+constexpr int IMG_W2 = 32 * LW1 * 2, IMG_O = IMG_W2 + 32 * LW2 * 2;      // byte offsets in a net's image
+constexpr int IMG_B1 = IMG_O + LAB_NOUT * LO * 4, IMG_B2 = IMG_B1 + 32 * 4;
+constexpr int FILL_LDS0 = IMG_B2 + 32 * 4;   // one net's image: bytes before the waves' rows
+static_assert(FILL_LDS0 == LAB_IMG_BYTES && FILL_LDS0 % 16 == 0, "net image layout");
 
 // The ghost fill on the matrix cores.  Each wave owns 64 columns (one per
 // lane).  A lane forms its column's 61 features in FP32; the two hidden
 // layers then run as 32x32x16 BF16 MFMAs with FP32 accumulation over the
 // wave's 64 columns (features and layer-1 activations rounded to BF16 on
 // the way in, as the weights are), and the output layer Ws x + W3 h2 + b3
-// stays in FP32 per lane, with Ws and W3 read at wave-uniform addresses
-// through the scalar cache.  MFMA operand layouts (gfx950,
+// stays in FP32 per lane, its weights read from LDS at wave-uniform
+// addresses.  The strip loads of all nine fields go out together, and the
+// sponge stores follow from the loaded values, so a column costs one
+// memory round trip.  Opposite faces (disjoint strips) share a launch.  MFMA operand layouts (gfx950,
 // v_mfma_f32_32x32x16_bf16): lane l holds A[l % 32][8 (l / 32) + 0..7] and
 // B[8 (l / 32) + 0..7][l % 32]; the accumulator's register r holds
 // D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
@@ -284,28 +356,35 @@ template <typename T>
 __global__ void __launch_bounds__(64 * FILL_WAVES) k_lab_fill(const LabFillArgs p)
 {
     extern __shared__ __align__(16) unsigned char lds[];
-    __bf16 *sW1 = (__bf16 *)lds;                       // [32][LW1]
-    __bf16 *sW2 = sW1 + 32 * LW1;                      // [32][LW2]
+    const __bf16 *sW1 = (const __bf16 *)lds;                 // [32][LW1]
+    const __bf16 *sW2 = (const __bf16 *)(lds + IMG_W2);      // [32][LW2]
+    const float *sB1 = (const float *)(lds + IMG_B1), *sB2 = (const float *)(lds + IMG_B2);
     const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
-    unsigned char *wl = lds + (32 * LW1 + 32 * LW2) * 2 + wave * WAVE_LDS;
+    const float *sO = (const float *)(lds + IMG_O);          // [LAB_NOUT][LO]: W3 row, Ws row, b3
+    unsigned char *wl = lds + FILL_LDS0 + wave * WAVE_LDS;
     __bf16 *sX = (__bf16 *)wl;                         // [64][LX]
     float *sH2 = (float *)wl;                          // [64][LH2], reuses the feature rows after layer 1
     __bf16 *sH1 = (__bf16 *)(wl + 64 * LX * 2);        // [64][LH]
 
-    const int f = p.face, net = 2 * f + p.phase;
-    const float *__restrict__ W = p.w + (size_t)net * LAB_NP;
-    for (int q = threadIdx.x; q < 32 * LW1; q += blockDim.x) {
-        const int n = q / LW1, k = q % LW1;
-        sW1[q] = lab_bf16(k < LAB_NIN ? W[LAB_W1 + n * LAB_NIN + k] : 0.0f);
-    }
-    for (int q = threadIdx.x; q < 32 * LW2; q += blockDim.x) {
-        const int n = q / LW2, k = q % LW2;
-        sW2[q] = lab_bf16(k < LAB_NH ? W[LAB_W2 + n * LAB_NH + k] : 0.0f);
+    const bool second = (int)blockIdx.x >= p.nb0;
+    const int f = p.face[second ? 1 : 0], net = 2 * f + p.phase;
+    // the net's LDS image, 16 bytes per lane per round: loaded first,
+    // written to LDS after the column's loads have been issued
+    constexpr int NV = FILL_LDS0 / 16, NT = 64 * FILL_WAVES, NR = (NV + NT - 1) / NT;
+    uint4 imgr[NR];
+    {
+        const uint4 *__restrict__ src = p.img + (size_t)net * NV;
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const int q = threadIdx.x + r * NT;
+            imgr[r] = q < NV ? src[q] : make_uint4(0, 0, 0, 0);
+        }
     }
 
     const LabDev &a = p.a;
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const int col = ((int)blockIdx.x - (second ? p.nb0 : 0)) * blockDim.x + threadIdx.x;
     const bool have = col < a.ncol[f];
+    const bool xrow = f < 2;
     int base = 0, step = 0;
     float4 m = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
     const int nout = c_nout[net];
@@ -322,10 +401,24 @@ __global__ void __launch_bounds__(64 * FILL_WAVES) k_lab_fill(const LabFillArgs 
             const float v = q < nout ? lab_ld<T>(a, fi, gaddr[q]) : 0.0f;
             prev[q] = fi >= 3 ? v / m.z : v;
         }
-        N = lab_features<T>(a, base, step, m, prev, x);
+        T raw[LAB_NF];
+        N = lab_features<T>(a, base, step, xrow, m, prev, x, raw);
+        // strip sponge on this phase's own fields, from the values read
+        // above (the column owns its strip cells within the launch)
+        if (p.sponge) {
+            const int f0 = p.phase == 0 ? 0 : 3, f1 = p.phase == 0 ? 3 : 9;
+#pragma unroll
+            for (int fi = 0; fi < 9; fi++)
+                if (fi >= f0 && fi < f1) lab_strip_store<T>((T *)a.F[fi], base, step, xrow, p.sp, raw + fi * LAB_NS);
+        }
     } else {
 #pragma unroll
         for (int q = 0; q < LAB_NIN; q++) x[q] = 0.0f;
+    }
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const int q = threadIdx.x + r * NT;
+        if (q < NV) ((uint4 *)lds)[q] = imgr[r];
     }
     const bool live = have && N > 0.0f && m.y > 0.0f;   // air columns keep zero ghosts
 #pragma unroll
@@ -334,7 +427,7 @@ __global__ void __launch_bounds__(64 * FILL_WAVES) k_lab_fill(const LabFillArgs 
 
     // hidden layer 1: H1 = tanh(X W1^T + b1), two 32-row tiles of the wave
     const int ln = l & 31, lk = 8 * (l >> 5);
-    const float b1 = W[LAB_B1 + ln], b2 = W[LAB_B2 + ln];
+    const float b1 = sB1[ln], b2 = sB2[ln];
 #pragma unroll
     for (int t = 0; t < 2; t++) {
         lab_f32x16 acc = {};
@@ -375,29 +468,18 @@ __global__ void __launch_bounds__(64 * FILL_WAVES) k_lab_fill(const LabFillArgs 
     float h2[LAB_NH];
 #pragma unroll
     for (int h = 0; h < LAB_NH; h++) h2[h] = sH2[l * LH2 + h];
-    // one output at a time keeps its 93 weights within the SGPR budget
 #pragma unroll 1
     for (int o = 0; o < nout; o++) {
-        float z = W[LAB_B3 + o];
+        const float *wo = sO + o * LO;
+        float z = wo[LAB_NH + LAB_NIN];
 #pragma unroll
-        for (int h = 0; h < LAB_NH; h++) z = fmaf(W[LAB_W3 + o * LAB_NH + h], h2[h], z);
+        for (int h = 0; h < LAB_NH; h++) z = fmaf(wo[h], h2[h], z);
 #pragma unroll
-        for (int q = 0; q < LAB_NIN; q++) z = fmaf(W[LAB_WS + o * LAB_NIN + q], x[q], z);
+        for (int q = 0; q < LAB_NIN; q++) z = fmaf(wo[LAB_NH + q], x[q], z);
         const int fi = c_tgt[net][o][0];
         float v = live ? z * N : 0.0f;
         if (fi >= 3) v = v * m.z;
         lab_st<T>(a, fi, gaddr[o], v);
-    }
-    // strip sponge on the fields this phase's ghosts belong to (their
-    // values were read into the features above; the column owns its cells)
-    if (p.sponge) {
-        const int f0 = p.phase == 0 ? 0 : 3, f1 = p.phase == 0 ? 3 : 9;
-#pragma unroll 1
-        for (int fi = f0; fi < f1; fi++) {
-            T *F = (T *)a.F[fi];
-#pragma unroll
-            for (int d = 0; d < LAB_NS; d++) F[base + d * step] *= (T)p.sp[d];
-        }
     }
 }
 
@@ -420,23 +502,62 @@ void lab_sponge_factors(float sponge, float *sp)
     }
 }
 
-int lab_fill(const LabGrid &g, const LabDomain &dom, const void *w, const float4 *colmed, int phase, float sponge,
+void lab_fill_image(const float *w, unsigned char *img)
+{
+    for (int n = 0; n < LAB_NNET; n++) {
+        const float *W = w + (size_t)n * LAB_NP;
+        unsigned char *o = img + (size_t)n * LAB_IMG_BYTES;
+        memset(o, 0, LAB_IMG_BYTES);
+        auto bf = [](float v) {   // the upper half of an FP32 value already rounded to BF16
+            uint32_t u;
+            memcpy(&u, &v, 4);
+            return (uint16_t)(u >> 16);
+        };
+        uint16_t *w1 = (uint16_t *)o, *w2 = (uint16_t *)(o + IMG_W2);
+        for (int h = 0; h < 32; h++) {
+            for (int k = 0; k < LAB_NIN; k++) w1[h * LW1 + k] = bf(W[LAB_W1 + h * LAB_NIN + k]);
+            for (int k = 0; k < LAB_NH; k++) w2[h * LW2 + k] = bf(W[LAB_W2 + h * LAB_NH + k]);
+        }
+        float *ow = (float *)(o + IMG_O);
+        for (int q = 0; q < LAB_NOUT; q++) {
+            for (int h = 0; h < LAB_NH; h++) ow[q * LO + h] = W[LAB_W3 + q * LAB_NH + h];
+            for (int k = 0; k < LAB_NIN; k++) ow[q * LO + LAB_NH + k] = W[LAB_WS + q * LAB_NIN + k];
+            ow[q * LO + LAB_NH + LAB_NIN] = W[LAB_B3 + q];
+        }
+        float *b1 = (float *)(o + IMG_B1), *b2 = (float *)(o + IMG_B2);
+        for (int h = 0; h < 32; h++) {
+            b1[h] = W[LAB_B1 + h];
+            b2[h] = W[LAB_B2 + h];
+        }
+    }
+}
+
+int lab_fill(const LabGrid &g, const LabDomain &dom, const void *img, const float4 *colmed, int phase, float sponge,
              hipStream_t s)
 {
     LabFillArgs p;
     lab_sponge_factors(sponge, p.sp);
     p.sponge = sponge > 0.0f;
     p.a = make_dev(g, dom);
-    p.w = (const float *)w;
+    p.img = (const uint4 *)img;
     p.colmed = colmed;
     p.phase = phase;
-    const size_t lds = (size_t)(32 * LW1 + 32 * LW2) * 2 + (size_t)FILL_WAVES * WAVE_LDS;
-    for (int f = 0; f < 5; f++) {
-        const int nb = (dom.ncol[f] + 64 * FILL_WAVES - 1) / (64 * FILL_WAVES);
+    const size_t lds = (size_t)FILL_LDS0 + (size_t)FILL_WAVES * WAVE_LDS;
+    const int cols = 64 * FILL_WAVES;
+    // faces in order x-, x+, y-, y+, bottom; opposite faces share a launch
+    // when their strips are disjoint (the sponge of one does not touch the
+    // other's features)
+    int f = 0;
+    while (f < 5) {
+        const bool pair = f < 4 && (f < 2 ? dom.nx : dom.ny) >= 2 * LAB_NS;
+        p.face[0] = f;
+        p.face[1] = pair ? f + 1 : f;
+        p.nb0 = (dom.ncol[f] + cols - 1) / cols;
+        const int nb = p.nb0 + (pair ? (dom.ncol[f + 1] + cols - 1) / cols : 0);
+        f += pair ? 2 : 1;
         if (nb == 0) continue;
-        p.face = f;
-        if (g.f64) hipLaunchKernelGGL(k_lab_fill<double>, dim3(nb), dim3(64 * FILL_WAVES), lds, s, p);
-        else hipLaunchKernelGGL(k_lab_fill<float>, dim3(nb), dim3(64 * FILL_WAVES), lds, s, p);
+        if (g.f64) hipLaunchKernelGGL(k_lab_fill<double>, dim3(nb), dim3(cols), lds, s, p);
+        else hipLaunchKernelGGL(k_lab_fill<float>, dim3(nb), dim3(cols), lds, s, p);
         SWPC_CHECK_LAUNCH();
     }
     return SWPC_OK;
@@ -514,7 +635,8 @@ __global__ void __launch_bounds__(256) k_lab_gather(const LabGatherArgs p)
         t[q] = fi >= 3 ? v / m.z : v;
     }
     float x[LAB_NIN];
-    const float N = lab_features<T>(a, base, step, m, prev, x);
+    T raw[LAB_NF];
+    const float N = lab_features<T>(a, base, step, false, m, prev, x, raw);
 #pragma unroll
     for (int q = 0; q < LAB_NOUT; q++) pv[q] = t[q];
     if (!(N > 0.0f) || !(m.y > 0.0f)) return;
