@@ -279,17 +279,23 @@ def run_c3(args, nt=4000, plain_steps=300, chunk=250):
     from openswpc_amd.solver import Swpc3D
     t0 = time.time()
     case = problem.case3d(1024, 1024, 512, 0.125, 0.005, nt, na=20, seed=args.seed, rmed=(0.03, 1.0, 0.5),
-                          src_depth=10.0, nst_side=8, rmed_tile=(128, 128, 128), log=log)
+                          src_depth=10.0, nst_side=8, rmed_tile=(128, 128, 128), log=log, device_medium=True)
     g = case.grid
     ncell = g.nx * g.ny * g.nz
-    log(f"c3 setup {time.time() - t0:.0f} s: vmax {case.medium.vmax:.2f} km/s, "
-        f"courant {problem.stability3(g, case.medium.vmax):.3f}")
+    host_setup = time.time() - t0
+    log(f"c3 host setup (velocity model) {host_setup:.0f} s; medium__setup runs on the GPU")
     out = {"config": {"workload": "swpc_3d 1024x1024x512 dx=0.125km dt=5ms na=20 NM=3, layered + 3% von Karman "
-                                  "(a=1 km), 1 double couple, 64 receivers", "nt": nt}}
+                                  "(a=1 km), 1 double couple, 64 receivers", "nt": nt},
+           "setup": {"host_velocity_model_s": host_setup}}
     traces = {}
     for prec in (CHOSEN, "fp64"):
-        sim = Swpc3D(g, case.medium, case.pml, case.glut, case.stations, case.ntdec_w, nt, precision=prec)
-        log(f"c3 {prec}: device memory {sim.memory() / 2**30:.1f} GiB")
+        t1 = time.perf_counter()
+        sim = problem.open_case3d(case, prec)
+        sim.sync()
+        out["setup"][f"device_setup_{prec}_s"] = time.perf_counter() - t1
+        log(f"c3 {prec}: context with the medium set up on the GPU in {time.perf_counter() - t1:.1f} s, "
+            f"vmax {case.medium.vmax:.2f} km/s, courant {problem.stability3(g, case.medium.vmax):.3f}, "
+            f"device memory {sim.memory() / 2**30:.1f} GiB")
         sim.sync()
         t1 = time.perf_counter()
         for it in range(1, nt + 1, chunk):
@@ -301,7 +307,7 @@ def run_c3(args, nt=4000, plain_steps=300, chunk=250):
         out[prec] = {"ms_per_step": wall / nt * 1e3, "cell_updates_per_s": ncell * nt / wall,
                      "device_GiB": sim.memory() / 2**30}
         sim.close()
-    sim = Swpc3D(g, case.medium, case.pml, case.glut, case.stations, case.ntdec_w, nt, precision="fp32")
+    sim = problem.open_case3d(case, "fp32")
     sim.run(1, 20)
     sim.sync()
     t1 = time.perf_counter()

@@ -34,6 +34,7 @@ module swpc_hip
     public :: swpc_errmsg
     public :: swpc_version, swpc_device_count
     public :: swpc3d_create, swpc3d_destroy, swpc3d_set_medium, swpc3d_set_density, swpc3d_set_surface, swpc3d_set_pml
+    public :: swpc3d_setup_medium, swpc3d_get_medium
     public :: swpc3d_set_sources, swpc3d_set_bodyforce, swpc3d_set_receivers
     public :: swpc3d_update_stress, swpc3d_update_vel, swpc3d_stressglut, swpc3d_bodyforce
     public :: swpc3d_store_receivers, swpc3d_add_snapshot, swpc3d_snapshots, swpc3d_get_snapshot
@@ -89,6 +90,25 @@ module swpc_hip
             import :: c_int, c_ptr, c_float
             type(c_ptr), value :: ctx
             real(c_float), intent(in) :: rho(*)
+        end function
+
+        !! medium__setup on the GPU from the velocity model's rho, lam, mu, qp, qs
+        integer(c_int) function swpc3d_setup_medium(ctx, rho, lam, mu, qp, qs, zeta3, cc_re, cc_im, kfs, kob, vminmax) &
+            bind(C, name='swpc3d_setup_medium')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            real(c_float), intent(in) :: rho(*), lam(*), mu(*), qp(*), qs(*)
+            real(c_float), value :: zeta3, cc_re, cc_im
+            integer(c_int), intent(out) :: kfs(*), kob(*)
+            real(c_float), intent(out) :: vminmax(2)
+        end function
+
+        !! one medium array (0..9 set_medium order, 10 rho) back in the reference's order
+        integer(c_int) function swpc3d_get_medium(ctx, which, out) bind(C, name='swpc3d_get_medium')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: which
+            real(c_float), intent(out) :: out(*)
         end function
 
         !! kfs_top, kfs_bot, kob_top, kob_bot, kob over (i,j) of the memory box

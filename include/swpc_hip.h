@@ -115,6 +115,27 @@ int swpc3d_set_medium(swpc3d_ctx *ctx, const float *bx, const float *by, const f
    one density stream instead of reading three buoyancy arrays; results are
    identical.  Ignored in SWPC_FP32_S16 mode. */
 int swpc3d_set_density(swpc3d_ctx *ctx, const float *rho);
+/* medium__setup (m_medium.f90:36-467) on the GPU, in place of the host
+   assembly followed by set_medium, set_density and set_surface.  In: the
+   velocity model's memory-box arrays rho, lam, mu, qp, qs (reference
+   (k,i,j) order, as vmodel_* leaves them).  The library applies the
+   absorber homogenisation (124-190), taup = zeta3/qp and taus = zeta3/qs
+   with zeta3 = NM*zeta (the FP32 product; 0 in benchmark mode), the
+   unrelaxed moduli at fq_ref through cc = (cc_re, cc_im), the mean of
+   i w ts/(1 - i w ts) (relaxed_medium, 234-274), surface detection and
+   its 6 x 6 window extremes (342-397), buoyancy and the harmonic shear
+   means (432-465), and velocity_minmax (399-430).  Out (each optional):
+   kfs and kob over the (ny+6) x (nx+6) columns of the memory box (box
+   index (j+2, i+2); the receivers are placed from them) and vminmax[2] =
+   (vmin, vmax).  Results equal the host assembly's except the
+   relaxation factor, formed in FP64 and rounded once (FP32 ulps).  Not
+   for SWPC_FP32_S16. */
+int swpc3d_setup_medium(swpc3d_ctx *ctx, const float *rho, const float *lam, const float *mu, const float *qp,
+                        const float *qs, float zeta3, float cc_re, float cc_im, int *kfs, int *kob, float *vminmax);
+/* Read back one medium array over the memory box in the reference's
+   (k,i,j) order: which = 0..9 for bx by bz lam mu taup taus muyz muxz
+   muxy (the set_medium order), 10 for rho.  Not for SWPC_FP32_S16. */
+int swpc3d_get_medium(swpc3d_ctx *ctx, int which, float *out);
 
 /* Order-switch maps kfs_top/bot, kob_top/bot and kob over the (i,j)
    memory box (m_medium.f90:342-397; used by m_kernel.f90:103-104 and

@@ -43,6 +43,8 @@ _PROTOS = {
     "swpc3d_destroy": (I, [P]),
     "swpc3d_set_medium": (I, [P] + [PF] * 10),
     "swpc3d_set_density": (I, [P, PF]),
+    "swpc3d_setup_medium": (I, [P, PF, PF, PF, PF, PF, C.c_float, C.c_float, C.c_float, PI_, PI_, PF]),
+    "swpc3d_get_medium": (I, [P, I, PF]),
     "swpc3d_set_surface": (I, [P] + [PI_] * 5),
     "swpc3d_set_pml": (I, [P] + [PF] * 6),
     "swpc3d_set_sources": (I, [P, I, PI_, PI_, PI_, PD, I, PD]),

@@ -22,6 +22,7 @@
 #include "swpc_common.h"
 #include "swpc_lab.h"
 #include "swpc_batch.h"
+#include "swpc_medium.h"
 
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
@@ -2805,6 +2806,97 @@ int swpc3d_set_density(swpc3d_ctx *c, const float *rho)
     box_to_dev(c->L, rho, h.data());
     if (int r = dalloc(c->rho, h.size() * 4, c->bytes)) return r;
     SWPC_HIP(hipMemcpy(c->rho.p, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+    return SWPC_OK;
+}
+
+// Scratch device buffers of one call, freed on every return path.
+struct DScratch {
+    DBuf b[4];
+    long long bytes = 0;
+    ~DScratch()
+    {
+        for (auto &x : b) dfree(x);
+    }
+};
+
+int swpc3d_setup_medium(swpc3d_ctx *c, const float *rho, const float *lam, const float *mu, const float *qp,
+                        const float *qs, float zeta3, float cc_re, float cc_im, int *kfs, int *kob, float *vminmax)
+{
+    clear_error();
+    if (c) graph_drop(c);  // the captured steps would launch stale arguments
+    if (!c || !rho || !lam || !mu || !qp || !qs) return fail(SWPC_EINVAL, "null argument");
+    if (c->m16) return fail(SWPC_EINVAL, "the device medium setup needs an FP32 medium (not SWPC_FP32_S16)");
+    const Lay3 &L = c->L;
+    if (c->d.na < 0 || 2 * c->d.na >= std::min(L.nx, L.ny) || c->d.na >= L.nz)
+        return fail(SWPC_EINVAL, "absorber of %d cells does not fit the grid", c->d.na);
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    if (!c->rho.p)
+        if (int r = dalloc(c->rho, (size_t)L.vol * 4, c->bytes)) return r;
+    MedSetup m;
+    m.nx = L.nx; m.ny = L.ny; m.nz = L.nz; m.na = c->d.na; m.px = L.px; m.sxy = L.sxy;
+    m.rho = (float *)c->rho.p;
+    m.bx = (float *)c->med[MBX].p; m.by = (float *)c->med[MBY].p; m.bz = (float *)c->med[MBZ].p;
+    m.lam = (float *)c->med[MLAM].p; m.mu = (float *)c->med[MMU].p;
+    m.taup = (float *)c->med[MTAUP].p; m.taus = (float *)c->med[MTAUS].p;
+    m.muyz = (float *)c->med[MMUYZ].p; m.muxz = (float *)c->med[MMUXZ].p; m.muxy = (float *)c->med[MMUXY].p;
+    m.zeta3 = zeta3; m.cc_re = cc_re; m.cc_im = cc_im;
+
+    DScratch t;
+    const size_t nbox = (size_t)(L.nx + 6) * (L.ny + 6) * (L.nz + 6), nmap = (size_t)(L.nx + 6) * (L.ny + 6);
+    if (int r = dalloc(t.b[0], nbox * 4, t.bytes)) return r;
+    const float *in[5] = {rho, lam, mu, qp, qs};
+    float *out[5] = {m.rho, m.lam, m.mu, m.taup, m.taus};
+    for (int q = 0; q < 5; q++) {
+        SWPC_HIP(hipMemcpy(t.b[0].p, in[q], nbox * 4, hipMemcpyHostToDevice));
+        if (int r = med_ingest(m, (const float *)t.b[0].p, out[q], c->stream)) return r;
+        SWPC_HIP(hipStreamSynchronize(c->stream));   // the next upload reuses the staging buffer
+    }
+    dfree(t.b[0]);
+    if (int r = med_relax(m, c->stream)) return r;
+    if (int r = dalloc(t.b[1], nmap * 4, t.bytes)) return r;
+    if (int r = dalloc(t.b[2], nmap * 4, t.bytes)) return r;
+    int *dkfs = (int *)t.b[1].p, *dkob = (int *)t.b[2].p;
+    if (int r = med_surface(m, dkfs, dkob, (int4 *)c->surf.p, (int *)c->kob.p, c->stream)) return r;
+    if (int r = med_average(m, c->stream)) return r;
+    float mm[2];
+    if (int r = med_minmax(m, dkfs, mm, c->stream)) return r;
+    if (vminmax) {
+        vminmax[0] = mm[0];
+        vminmax[1] = mm[1];
+    }
+    if (kfs) SWPC_HIP(hipMemcpy(kfs, dkfs, nmap * 4, hipMemcpyDeviceToHost));
+    if (kob) SWPC_HIP(hipMemcpy(kob, dkob, nmap * 4, hipMemcpyDeviceToHost));
+    // planes on which an array is constant (K3::cpl)
+    if (int r = dalloc(t.b[3], (size_t)L.pz * 4, t.bytes)) return r;
+    std::vector<int> fl((size_t)L.pz);
+    for (int q = 0; q < 10; q++) {
+        if (!c->cpl[q].p)
+            if (int r = dalloc(c->cpl[q], (size_t)L.pz * 4, c->bytes)) return r;
+        if (int r = med_planes(m, (const float *)c->med[q].p, (int *)t.b[3].p, (float *)c->cpl[q].p, L.pz, c->stream))
+            return r;
+        SWPC_HIP(hipMemcpyAsync(fl.data(), t.b[3].p, fl.size() * 4, hipMemcpyDeviceToHost, c->stream));
+        SWPC_HIP(hipStreamSynchronize(c->stream));
+        bool any = false;
+        for (int v : fl) any |= v != 0;
+        c->use_cpl[q] = any;
+    }
+    c->have_medium = true;
+    c->have_surface = true;
+    return SWPC_OK;
+}
+
+int swpc3d_get_medium(swpc3d_ctx *c, int which, float *out)
+{
+    clear_error();
+    if (!c || !out) return fail(SWPC_EINVAL, "null argument");
+    if (which < 0 || which > 10) return fail(SWPC_EINVAL, "medium array %d: expected 0..9 (bx..muxy) or 10 (rho)", which);
+    if (c->m16) return fail(SWPC_EINVAL, "the medium is stored in 16 bit (SWPC_FP32_S16)");
+    const DBuf &b = which == 10 ? c->rho : c->med[which];
+    if (!b.p) return fail(SWPC_ESTATE, "medium array %d is not set", which);
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    std::vector<float> h((size_t)c->L.vol);
+    SWPC_HIP(hipMemcpy(h.data(), b.p, h.size() * 4, hipMemcpyDeviceToHost));
+    dev_to_box(c->L, h.data(), out);
     return SWPC_OK;
 }
 

@@ -456,6 +456,36 @@ def medium_setup3(g: Grid3D, rho, lam, mu, qp, qs, fq_min=0.05, fq_max=5.0, fq_r
                     kfs, kob, kfs_top, kfs_bot, kob_top, kob_bot, ts, float(zeta), vmin, vmax)
 
 
+@dataclass
+class RawMedium3D:
+    """The velocity model's memory-box arrays before medium__setup, for the
+    device setup (``Swpc3D`` calls swpc3d_setup_medium with them): the
+    absorber homogenisation, Q, relaxation, surface detection and averaged
+    coefficients are then formed on the GPU.  ``kfs``, ``kob``, ``vmin``
+    and ``vmax`` are filled in by the context that ran the setup."""
+    rho: np.ndarray
+    lam: np.ndarray
+    mu: np.ndarray
+    qp: np.ndarray
+    qs: np.ndarray
+    fq_min: float = 0.05
+    fq_max: float = 5.0
+    fq_ref: float = 1.0
+    benchmark: bool = False
+    kfs: np.ndarray = None
+    kob: np.ndarray = None
+    vmin: float = None
+    vmax: float = None
+
+    def constants(self):
+        """(ts, zeta3 = NM*zeta as FP32, cc) as medium_setup3 forms them."""
+        ts = fdtool.visco_set_relaxtime(NM, self.fq_min, self.fq_max)
+        zeta = F32(0.0) if self.benchmark else fdtool.visco_constq_zeta(NM, self.fq_min, self.fq_max, ts)
+        zeta3 = F32(F32(NM) * F32(zeta))
+        cc = fdtool.relaxed_cc(NM, ts, self.fq_ref)
+        return ts, zeta3, cc
+
+
 def _minmax(lam, mu, rho, kfs):
     """velocity_minmax over k >= kfs+1 of each column."""
     nk = lam.shape[-1]

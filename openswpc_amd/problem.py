@@ -68,7 +68,7 @@ def mechanism(seed):
 
 def case3d(nx, ny, nz, dx, dt, nt, na=20, layers=LHM_EXAMPLE, seed=1, rmed=None, src_depth=10.0,
            tr=1.0, t0=0.1, mo=1e15, nst_side=8, st_spacing=None, ntdec_w=1, rmed_tile=(128, 128, 64),
-           benchmark=False, zbeg=None, bodyforce=False, src_xy=(0.0, 0.0), log=None):
+           benchmark=False, zbeg=None, bodyforce=False, src_xy=(0.0, 0.0), log=None, device_medium=False):
     """3-D case: layered (+ random) medium, double couple at src_depth below
     the grid centre, an nst_side x nst_side receiver grid on the free surface.
     bodyforce=True replaces the double couple by a single force of the same
@@ -78,6 +78,11 @@ def case3d(nx, ny, nz, dx, dt, nt, na=20, layers=LHM_EXAMPLE, seed=1, rmed=None,
     rmed = (epsilon, a_km, kappa) adds a von Karman perturbation generated
     on a periodic tile and mapped by rmed_tile periodicity.  log (optional)
     receives a line after each setup stage.
+
+    device_medium=True leaves medium__setup to the GPU: ``medium`` is then
+    a RawMedium3D, and the receivers (``stations`` None, ``station_list``
+    the Station list) are placed by ``open_case3d`` from the kfs/kob maps
+    the device setup returns.
     """
     import time
     clock0 = time.time()
@@ -93,8 +98,11 @@ def case3d(nx, ny, nz, dx, dt, nt, na=20, layers=LHM_EXAMPLE, seed=1, rmed=None,
     rho, lam, mu, qp, qs = md.layered3(g, layers, xi=xi)
     xi = None
     note("layered model")
-    med = md.medium_setup3(g, rho, lam, mu, qp, qs, benchmark=benchmark)
-    note("medium__setup")
+    if device_medium:
+        med = md.RawMedium3D(rho, lam, mu, qp, qs, benchmark=benchmark)
+    else:
+        med = md.medium_setup3(g, rho, lam, mu, qp, qs, benchmark=benchmark)
+        note("medium__setup")
     strike, dip, rake = mechanism(seed)
     src = PointSource(float(src_xy[0]), float(src_xy[1]), src_depth, t0, tr, mo, fdtool.sdr2moment(strike, dip, rake))
     glut = stress_glut3(g, [src], nt)
@@ -112,8 +120,27 @@ def case3d(nx, ny, nz, dx, dt, nt, na=20, layers=LHM_EXAMPLE, seed=1, rmed=None,
             x = (a_ - (nst_side - 1) / 2) * st_spacing
             y = (b_ - (nst_side - 1) / 2) * st_spacing
             sts.append(Station(x + 0.3 * dx, y + 0.3 * dx, 0.0, f"S{a_:02d}{b_:02d}", "fsb"))
+    if device_medium:
+        return Case3D(g, med, prof, glut, None, sts, nt, ntdec_w, (strike, dip, rake), bforce)
     cells, kept = station_cells3(g, med.kfs, med.kob, sts)
     return Case3D(g, med, prof, glut, cells, kept, nt, ntdec_w, (strike, dip, rake), bforce)
+
+
+def open_case3d(case, precision="fp32"):
+    """A Swpc3D for a case; with a RawMedium3D (device_medium) the medium is
+    set up on the GPU and the receivers are placed from its kfs/kob maps
+    (case.stations and case.station_list are updated)."""
+    from .solver import Swpc3D
+    raw = isinstance(case.medium, md.RawMedium3D)
+    sim = Swpc3D(case.grid, case.medium, case.pml, case.glut, None if raw else case.stations, case.ntdec_w, case.nt,
+                 precision=precision, bodyforce=case.bforce)
+    if raw:
+        if case.stations is None:
+            case.stations, case.station_list = station_cells3(case.grid, case.medium.kfs, case.medium.kob,
+                                                              case.station_list)
+        if len(case.stations):
+            sim.set_receivers(case.stations)
+    return sim
 
 
 def case2d(nx, nz, dx, dt, nt, na=20, layers=LHM_TWO, seed=1, rmed=None, src_depth=10.0, tr=1.0, t0=0.1,

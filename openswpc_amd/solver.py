@@ -14,7 +14,7 @@ import ctypes as C
 
 import numpy as np
 
-from . import _lib
+from . import _lib, medium as md
 from ._lib import SwpcError, check, lib, ptr
 
 UC3 = 1e-15   # src/swpc_3d/m_global.f90:29
@@ -101,19 +101,36 @@ class Swpc3D:
         d.nx, d.ny, d.nz, d.na, d.nm = grid.nx, grid.ny, grid.nz, grid.na, 3
         d.precision = PRECISION[precision]
         d.dx, d.dy, d.dz, d.dt = grid.dx, grid.dy, grid.dz, grid.dt
+        raw = isinstance(medium, md.RawMedium3D)
+        if raw:
+            ts, zeta3, cc = medium.constants()
+        else:
+            ts = medium.ts
         for m in range(3):
-            d.ts[m] = float(medium.ts[m])
+            d.ts[m] = float(ts[m])
         h = C.c_void_p()
         check(L.swpc3d_create(C.byref(d), C.byref(h)), "swpc3d_create")
         self.h = h
         m = medium
-        self._keep = [_f32(a) for a in (m.bx, m.by, m.bz, m.lam, m.mu, m.taup, m.taus, m.muyz, m.muxz, m.muxy)]
-        check(L.swpc3d_set_medium(h, *[ptr(a, C.c_float) for a in self._keep]), "set_medium")
-        rho = _f32(m.rho)
-        check(L.swpc3d_set_density(h, ptr(rho, C.c_float)), "set_density")
-        s = [_i32(a) for a in (m.kfs_top, m.kfs_bot, m.kob_top, m.kob_bot, m.kob)]
-        check(L.swpc3d_set_surface(h, *[ptr(a, C.c_int) for a in s]), "set_surface")
-        self._keep = None
+        if raw:
+            # medium__setup on the GPU; kfs/kob and vmin/vmax come back
+            arrs = [_f32(a) for a in (m.rho, m.lam, m.mu, m.qp, m.qs)]
+            kfs = np.zeros((grid.ny + 6, grid.nx + 6), np.int32)
+            kob = np.zeros_like(kfs)
+            mm = np.zeros(2, np.float32)
+            check(L.swpc3d_setup_medium(h, *[ptr(a, C.c_float) for a in arrs], float(zeta3), float(cc.real),
+                                        float(cc.imag), ptr(kfs, C.c_int), ptr(kob, C.c_int), ptr(mm, C.c_float)),
+                  "setup_medium")
+            m.kfs, m.kob, m.vmin, m.vmax = kfs, kob, float(mm[0]), float(mm[1])
+            arrs = None
+        else:
+            self._keep = [_f32(a) for a in (m.bx, m.by, m.bz, m.lam, m.mu, m.taup, m.taus, m.muyz, m.muxz, m.muxy)]
+            check(L.swpc3d_set_medium(h, *[ptr(a, C.c_float) for a in self._keep]), "set_medium")
+            rho = _f32(m.rho)
+            check(L.swpc3d_set_density(h, ptr(rho, C.c_float)), "set_density")
+            s = [_i32(a) for a in (m.kfs_top, m.kfs_bot, m.kob_top, m.kob_bot, m.kob)]
+            check(L.swpc3d_set_surface(h, *[ptr(a, C.c_int) for a in s]), "set_surface")
+            self._keep = None
         gp = [_f32(a) for a in pml]
         check(L.swpc3d_set_pml(h, *[ptr(a, C.c_float) for a in gp]), "set_pml")
         self.M0 = 1.0
@@ -140,12 +157,25 @@ class Swpc3D:
         self.ntw = 0
         self.ntdec_w = ntdec_w
         if stations is not None and len(stations):
-            st = _i32(stations)
-            from .source import ntw_of
-            self.nst = len(st)
-            self.ntw = ntw_of(nt, ntdec_w)
-            check(L.swpc3d_set_receivers(h, self.nst, ptr(_i32(st[:, 0]), C.c_int), ptr(_i32(st[:, 1]), C.c_int),
-                                         ptr(_i32(st[:, 2]), C.c_int), ntdec_w, self.ntw), "set_receivers")
+            self.set_receivers(stations)
+
+    def set_receivers(self, stations):
+        """Receiver cells (n, 3) of (i, j, k), sampled every ntdec_w steps."""
+        L = lib()
+        st = _i32(stations)
+        from .source import ntw_of
+        self.nst = len(st)
+        self.ntw = ntw_of(self.nt, self.ntdec_w)
+        check(L.swpc3d_set_receivers(self.h, self.nst, ptr(_i32(st[:, 0]), C.c_int), ptr(_i32(st[:, 1]), C.c_int),
+                                     ptr(_i32(st[:, 2]), C.c_int), self.ntdec_w, self.ntw), "set_receivers")
+
+    def medium_array(self, name):
+        """One medium array read back from the device in the memory-box
+        (j, i, k) shape: bx by bz lam mu taup taus muyz muxz muxy or rho."""
+        names = ("bx", "by", "bz", "lam", "mu", "taup", "taus", "muyz", "muxz", "muxy", "rho")
+        out = np.zeros((self.g.ny + 6, self.g.nx + 6, self.g.nz + 6), np.float32)
+        check(lib().swpc3d_get_medium(self.h, names.index(name), ptr(out, C.c_float)), "get_medium")
+        return out
 
     # -- reference routine names ------------------------------------------
     def kernel__update_stress(self):
