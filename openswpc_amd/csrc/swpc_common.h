@@ -91,8 +91,9 @@ __host__ __device__ inline int at2(int px, int i, int k) { return (k + KO) * px 
 
 // Device buffer owned by a context.
 struct DBuf {
-    void *p = nullptr;
+    void *p = nullptr;       // the buffer
     size_t bytes = 0;
+    void *base = nullptr;    // the allocation (p may start past it)
 };
 
 int dalloc(DBuf &b, size_t bytes, long long &total);
