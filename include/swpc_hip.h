@@ -71,6 +71,13 @@ enum swpc2d_field {
     SWPC2D_NFIELD = 22
 };
 
+enum swpcsh_field {
+    SWPCSH_VY = 0, SWPCSH_SYZ, SWPCSH_SXY,
+    SWPCSH_R0 = 3,        /* Ryz x m = 1..3, then Rxy x m = 1..3 */
+    SWPCSH_ADE0 = 9,      /* axVy azVy axSxy azSyz */
+    SWPCSH_NFIELD = 13
+};
+
 const char *swpc_last_error(void);
 int swpc_version(void);
 /* Number of HIP devices visible (0 when no GPU); never fails. */
@@ -396,6 +403,54 @@ int swpc2d_get_field(swpc2d_ctx *ctx, int field, double *out);
 int swpc2d_set_field(swpc2d_ctx *ctx, int field, const double *in);
 int swpc2d_vmax(swpc2d_ctx *ctx, float *out2);
 int swpc2d_sync(swpc2d_ctx *ctx);
+
+/* ------------------------------------------------------------------ */
+/* SH (swpc_sh)                                                        */
+/* ------------------------------------------------------------------ */
+/* The 2-D SH core (src/swpc_sh): Vy, Syz, Sxy with NM = 3 memory
+   variables each, the ADE-CFS PML, stress glut and body force, receivers.
+   Same descriptor, layouts and conventions as the P-SV context.  Step
+   order of src/swpc_sh/main.f90:97-114.  The SH PML velocity update of
+   the reference indexes the z damping profile by i (m_absorb_p.f90:
+   181-182), which reads past the profile when nx > nz; this core takes it
+   at k. */
+typedef struct swpcsh_ctx swpcsh_ctx;
+
+int swpcsh_create(const swpc2d_desc *desc, swpcsh_ctx **out);
+int swpcsh_destroy(swpcsh_ctx *ctx);
+/* rho, mu, taus over the (k,i) memory box (after medium__setup); the
+   kernels form 1/rho and the two-point harmonic mu_xy, mu_yz themselves
+   (src/swpc_sh/m_kernel.f90:127, 206-209). */
+int swpcsh_set_medium(swpcsh_ctx *ctx, const float *rho, const float *mu, const float *taus);
+/* maps over i in [-2:nx+3] */
+int swpcsh_set_surface(swpcsh_ctx *ctx, const int *kfs_top, const int *kfs_bot,
+                       const int *kob_top, const int *kob_bot, const int *kob);
+/* profiles g(1..4) per cell (pa = 2 in the SH damping profile,
+   src/swpc_sh/m_absorb_p.f90:393-425) */
+int swpcsh_set_pml(swpcsh_ctx *ctx, const float *gxc, const float *gxe, const float *gzc, const float *gze);
+/* m2 = (myz, mxy); sdrop[it-1][n] = mo/M0 * momentrate(tbeg+(it-1/2)dt) * dt/(dx dz)
+   (m_source.f90:554-594: each component split over two nodes) */
+int swpcsh_set_sources(swpcsh_ctx *ctx, int nsrc, const int *isrc, const int *ksrc,
+                       const double *m2, int nt, const double *sdrop);
+/* fy/M0; fdt[it-1][n] = momentrate(tbeg+it dt) * dt/(dx dz) (m_source.f90:596-629) */
+int swpcsh_set_bodyforce(swpcsh_ctx *ctx, int nsrc, const int *isrc, const int *ksrc,
+                         const double *fy, int nt, const double *fdt);
+int swpcsh_set_receivers(swpcsh_ctx *ctx, int nst, const int *ist, const int *kst, int ntdec_w, int ntw);
+int swpcsh_update_stress(swpcsh_ctx *ctx);
+int swpcsh_update_vel(swpcsh_ctx *ctx);
+int swpcsh_stressglut(swpcsh_ctx *ctx, int it);
+int swpcsh_bodyforce(swpcsh_ctx *ctx, int it);
+int swpcsh_store_receivers(swpcsh_ctx *ctx, int it);
+int swpcsh_step(swpcsh_ctx *ctx, int it);
+int swpcsh_set_graph(swpcsh_ctx *ctx, int steps);
+int swpcsh_run(swpcsh_ctx *ctx, int it0, int nsteps);
+/* vel[itw][n] = Vy (x M0 UC 1e9 gives the reference's output), disp[itw][n] = sum Vy dt */
+int swpcsh_get_receivers(swpcsh_ctx *ctx, float *vel, double *disp);
+int swpcsh_get_field(swpcsh_ctx *ctx, int field, double *out);
+int swpcsh_set_field(swpcsh_ctx *ctx, int field, const double *in);
+/* kernel__vmax (m_kernel.f90:225-240): max |Vy(kob+1, i)| */
+int swpcsh_vmax(swpcsh_ctx *ctx, float *out1);
+int swpcsh_sync(swpcsh_ctx *ctx);
 
 /* ------------------------------------------------------------------ */
 /* Batches of independent runs                                         */

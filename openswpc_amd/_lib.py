@@ -90,6 +90,27 @@ _PROTOS = {
     "swpc_batch_add2d": (I, [P, P]),
     "swpc_batch_run": (I, [P, I, I, I]),
     "swpc_batch_sync": (I, [P]),
+    "swpcsh_create": (I, [C.POINTER(Desc2), C.POINTER(P)]),
+    "swpcsh_destroy": (I, [P]),
+    "swpcsh_set_medium": (I, [P, PF, PF, PF]),
+    "swpcsh_set_surface": (I, [P, PI_, PI_, PI_, PI_, PI_]),
+    "swpcsh_set_pml": (I, [P, PF, PF, PF, PF]),
+    "swpcsh_set_sources": (I, [P, I, PI_, PI_, PD, I, PD]),
+    "swpcsh_set_bodyforce": (I, [P, I, PI_, PI_, PD, I, PD]),
+    "swpcsh_set_receivers": (I, [P, I, PI_, PI_, I, I]),
+    "swpcsh_update_stress": (I, [P]),
+    "swpcsh_update_vel": (I, [P]),
+    "swpcsh_stressglut": (I, [P, I]),
+    "swpcsh_bodyforce": (I, [P, I]),
+    "swpcsh_store_receivers": (I, [P, I]),
+    "swpcsh_step": (I, [P, I]),
+    "swpcsh_set_graph": (I, [P, I]),
+    "swpcsh_run": (I, [P, I, I]),
+    "swpcsh_get_receivers": (I, [P, PF, PD]),
+    "swpcsh_get_field": (I, [P, I, PD]),
+    "swpcsh_set_field": (I, [P, I, PD]),
+    "swpcsh_vmax": (I, [P, PF]),
+    "swpcsh_sync": (I, [P]),
     "swpc2d_create": (I, [C.POINTER(Desc2), C.POINTER(P)]),
     "swpc2d_destroy": (I, [P]),
     "swpc2d_set_medium": (I, [P] + [PF] * 5),

@@ -15,8 +15,8 @@ import math
 import numpy as np
 
 from . import fdtool, medium as md, pml, rmedia
-from .source import (BodyForce, PointSource, Station, body_force2, body_force3, station_cells2, station_cells3,
-                     stress_glut2, stress_glut3)
+from .source import (BodyForce, PointSource, Station, body_force2, body_force3, body_force_sh, station_cells2,
+                     station_cells3, stress_glut2, stress_glut3, stress_glut_sh)
 
 F32 = np.float32
 
@@ -165,6 +165,34 @@ def case2d(nx, nz, dx, dt, nt, na=20, layers=LHM_TWO, seed=1, rmed=None, src_dep
         glut = None
     prof = pml.pml2(g, (bforce or glut).fcut)
     sts = [Station((n - (nst - 1) / 2) * st_spacing + 0.3 * dx, 0.0, 0.0, f"P{n:02d}", "fsb") for n in range(nst)]
+    cells, kept = station_cells2(g, med.kfs, med.kob, sts)
+    return Case2D(g, med, prof, glut, cells, kept, nt, ntdec_w, (strike, dip, rake), bforce)
+
+
+def case_sh(nx, nz, dx, dt, nt, na=20, layers=LHM_TWO, seed=1, rmed=None, src_depth=10.0, tr=1.0, t0=0.1,
+            mo=1e15, nst=16, st_spacing=2.0, ntdec_w=1, benchmark=False, zbeg=None, bodyforce=False, src_x=0.0):
+    """SH case on the x-z plane (swpc_sh): the P-SV medium recipe (the SH
+    medium__setup is the P-SV one, src/swpc_sh/m_medium.f90), a double
+    couple whose (myz, mxy) components drive the SH stress glut, nst
+    receivers on the free surface; bodyforce=True uses a single force fy
+    of the same time function instead."""
+    g = md.Grid2D(nx, nz, dx, dx, dt, na=na, zbeg=zbeg)
+    xi = None
+    if rmed is not None:
+        eps, a, kappa = rmed
+        xi = rmedia.generate((nx + 6, nz + 6), dx, a, eps, kappa, seed=seed)
+    rho, lam, mu, qp, qs = md.layered2(g, layers, xi=xi)
+    med = md.medium_setup2(g, rho, lam, mu, qp, qs, benchmark=benchmark)
+    strike, dip, rake = mechanism(seed)
+    src = PointSource(float(src_x), 0.0, src_depth, t0, tr, mo, fdtool.sdr2moment(strike, dip, rake))
+    glut = stress_glut_sh(g, [src], nt)
+    bforce = None
+    if bodyforce:
+        fy = float(np.random.default_rng(seed + 1000).standard_normal())
+        bforce = body_force_sh(g, [BodyForce(float(src_x), 0.0, src_depth, t0, tr, (0.0, fy, 0.0))], nt)
+        glut = None
+    prof = pml.pml_sh(g, (bforce or glut).fcut)
+    sts = [Station((n - (nst - 1) / 2) * st_spacing + 0.3 * dx, 0.0, 0.0, f"H{n:02d}", "fsb") for n in range(nst)]
     cells, kept = station_cells2(g, med.kfs, med.kob, sts)
     return Case2D(g, med, prof, glut, cells, kept, nt, ntdec_w, (strike, dip, rake), bforce)
 

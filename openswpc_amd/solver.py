@@ -455,3 +455,130 @@ class SwpcPSV:
 
     def __exit__(self, *a):
         self.close()
+
+
+FIELDS_SH = ["Vy", "Syz", "Sxy", "Ryz1", "Ryz2", "Ryz3", "Rxy1", "Rxy2", "Rxy3", "axVy", "azVy", "axSxy", "azSyz"]
+
+
+class SwpcSH:
+    """One 2-D SH simulation resident on the GPU (swpc_sh)."""
+
+    def __init__(self, grid, medium, pml, glut=None, stations=None, ntdec_w=1, nt=None,
+                 precision="fp32", bodyforce=None):
+        self.g = grid
+        self.nt = nt
+        L = lib()
+        d = _lib.Desc2()
+        d.nx, d.nz, d.na, d.nm = grid.nx, grid.nz, grid.na, 3
+        d.precision = PRECISION[precision]
+        d.dx, d.dz, d.dt = grid.dx, grid.dz, grid.dt
+        for m in range(3):
+            d.ts[m] = float(medium.ts[m])
+        h = C.c_void_p()
+        check(L.swpcsh_create(C.byref(d), C.byref(h)), "swpcsh_create")
+        self.h = h
+        m = medium
+        arrs = [_f32(a) for a in (m.rho, m.mu, m.taus)]
+        check(L.swpcsh_set_medium(h, *[ptr(a, C.c_float) for a in arrs]), "set_medium")
+        s = [_i32(a) for a in (m.kfs_top, m.kfs_bot, m.kob_top, m.kob_bot, m.kob)]
+        check(L.swpcsh_set_surface(h, *[ptr(a, C.c_int) for a in s]), "set_surface")
+        gp = [_f32(a) for a in pml]
+        check(L.swpcsh_set_pml(h, *[ptr(a, C.c_float) for a in gp]), "set_pml")
+        self.M0 = 1.0
+        self.UC = UC2
+        if glut is not None and len(glut.idx):
+            idx = _i32(glut.idx)
+            tab = _f64(glut.table)
+            check(L.swpcsh_set_sources(h, len(idx), ptr(_i32(idx[:, 0]), C.c_int), ptr(_i32(idx[:, 1]), C.c_int),
+                                       ptr(_f64(glut.m), C.c_double), tab.shape[0], ptr(tab, C.c_double)),
+                  "set_sources")
+            self.M0 = glut.M0
+        if bodyforce is not None and len(bodyforce.idx):
+            idx = _i32(bodyforce.idx)
+            tab = _f64(bodyforce.table)
+            check(L.swpcsh_set_bodyforce(h, len(idx), ptr(_i32(idx[:, 0]), C.c_int), ptr(_i32(idx[:, 1]), C.c_int),
+                                         ptr(_f64(bodyforce.m), C.c_double), tab.shape[0], ptr(tab, C.c_double)),
+                  "set_bodyforce")
+            self.M0 = bodyforce.M0
+            # bf_mode unit conversion in real(SP) (src/swpc_sh/m_source.f90:160)
+            self.UC = float(np.float32(np.float32(UC2) * np.float32(1000)))
+        self.nst = 0
+        self.ntw = 0
+        if stations is not None and len(stations):
+            st = _i32(stations)
+            from .source import ntw_of
+            self.nst = len(st)
+            self.ntw = ntw_of(nt, ntdec_w)
+            check(L.swpcsh_set_receivers(h, self.nst, ptr(_i32(st[:, 0]), C.c_int), ptr(_i32(st[:, 1]), C.c_int),
+                                         ntdec_w, self.ntw), "set_receivers")
+
+    def kernel__update_stress(self):
+        check(lib().swpcsh_update_stress(self.h), "update_stress")
+
+    def kernel__update_vel(self):
+        check(lib().swpcsh_update_vel(self.h), "update_vel")
+
+    def source__stressglut(self, it):
+        check(lib().swpcsh_stressglut(self.h, it), "stressglut")
+
+    def source__bodyforce(self, it):
+        check(lib().swpcsh_bodyforce(self.h, it), "bodyforce")
+
+    def wav__store(self, it):
+        check(lib().swpcsh_store_receivers(self.h, it), "store_receivers")
+
+    def kernel__vmax(self):
+        out = np.zeros(1, np.float32)
+        check(lib().swpcsh_vmax(self.h, ptr(out, C.c_float)), "vmax")
+        return out
+
+    def step(self, it):
+        check(lib().swpcsh_step(self.h, it), "step")
+
+    def run(self, it0, nsteps):
+        check(lib().swpcsh_run(self.h, it0, nsteps), "run")
+
+    def set_graph(self, steps):
+        """Steps per captured HIP graph in run() (0: step by step)."""
+        check(lib().swpcsh_set_graph(self.h, int(steps)), "set_graph")
+
+    def sync(self):
+        check(lib().swpcsh_sync(self.h), "sync")
+
+    def receivers(self, scaled=False):
+        """vel[itw][n] (Vy), disp[itw][n]; scaled: x M0 UC 1e9 as wav__store writes them."""
+        vel = np.zeros((self.ntw, self.nst), np.float32)
+        disp = np.zeros((self.ntw, self.nst), np.float64)
+        if self.nst:
+            check(lib().swpcsh_get_receivers(self.h, ptr(vel, C.c_float), ptr(disp, C.c_double)), "get_receivers")
+        if scaled:
+            s = self.M0 * self.UC * 1e9
+            return (vel * np.float32(s)).astype(np.float32), disp * s
+        return vel, disp
+
+    def field(self, name):
+        out = np.zeros(self.g.shape, np.float64)
+        check(lib().swpcsh_get_field(self.h, FIELDS_SH.index(name), ptr(out, C.c_double)), "get_field")
+        return out
+
+    def set_field(self, name, a):
+        a = _f64(a)
+        assert a.shape == self.g.shape
+        check(lib().swpcsh_set_field(self.h, FIELDS_SH.index(name), ptr(a, C.c_double)), "set_field")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().swpcsh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -230,6 +230,35 @@ def stress_glut2(g, sources, nt, stftype="kupper"):
     return SourceTable(np.array(idx, np.int32), m3, tab, M0, fcut)
 
 
+def stress_glut_sh(g, sources, nt, stftype="kupper"):
+    """SH stress glut (src/swpc_sh/m_source.f90:554-594): m = (myz, mxy),
+    the 4th and 6th moment-tensor components; table[it-1, n] =
+    mo/M0 * momentrate(tbeg+(it-1/2)dt) * dt/(dx dz); the source y
+    coordinate is ignored."""
+    if not sources:
+        return SourceTable(np.zeros((0, 2), np.int32), np.zeros((0, 2)), np.zeros((nt, 0)), 1.0, 0.0)
+    t = stress_glut2(g, sources, nt, stftype)
+    m2 = np.array([(s.m6[3], s.m6[5]) for s in sources], np.float64)
+    return SourceTable(t.idx, m2, t.table, t.M0, t.fcut)
+
+
+def body_force_sh(g, forces, nt, stftype="kupper"):
+    """SH body force (src/swpc_sh/m_source.f90:596-629): fy normalised by
+    M0 = sqrt(sum(fy)**2) = |sum fy| as the reference writes it (line 158);
+    table[it-1, n] = momentrate(tbeg+it dt) * dt/(dx dz), no halving (one
+    node); the kernel divides by rho(k, i)."""
+    M0 = math.sqrt(float(F32(sum(float(F32(f.f3[1])) for f in forces))) ** 2)
+    idx = [(fdtool.x2i(f.x, g.xbeg, g.dx), fdtool.x2i(f.z, g.zbeg, g.dz)) for f in forces]
+    dt_dxz = float(F32(g.dt)) / (float(g.dx) * float(g.dz))
+    tab = np.zeros((nt, len(forces)))
+    for n, f in enumerate(forces):
+        for it in range(1, nt + 1):
+            tab[it - 1, n] = float(fdtool.momentrate(_t_vel(g.tbeg, it, g.dt), stftype, f.t0, f.tr)) * dt_dxz
+    fy = np.array([float(F32(f.f3[1])) / M0 for f in forces])
+    fcut = max(1.0 / float(F32(f.tr)) for f in forces)
+    return SourceTable(np.array(idx, np.int32), fy, tab, M0, fcut)
+
+
 def body_force3(g, forces, nt, stftype="kupper"):
     """Body-force table (m_source.f90:835-876): f3 normalised by
     M0 = |f| (m_source.f90:212-216); table[it-1, n] = momentrate(tbeg+it dt) dt/(dx dy dz)/2."""

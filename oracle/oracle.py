@@ -471,3 +471,153 @@ class Oracle2D:
         out = np.zeros(2, np.float32)
         self.L.o2d_vmax(self._ps(), out.ctypes.data_as(_PF))
         return out
+
+
+FIELDS_SH = ["Vy", "Syz", "Sxy"]
+R_SH = ["Ryz", "Rxy"]
+ADE_SH = ["axVy", "azVy", "axSxy", "azSyz"]
+
+
+def _osh_struct(real):
+    PR = C.POINTER(real)
+
+    class OSH(C.Structure):
+        _fields_ = [("nx", C.c_int), ("nz", C.c_int), ("na", C.c_int), ("nm", C.c_int),
+                    ("dx", C.c_double), ("dz", C.c_double), ("dt", C.c_float)] + \
+                   [(n, PR) for n in FIELDS_SH] + [(n, PR) for n in R_SH] + \
+                   [(n, _PF) for n in ("rho", "mu", "taus")] + \
+                   [(n, _PI) for n in ("kfs_top", "kfs_bot", "kob_top", "kob_bot", "kob")] + \
+                   [(n, _PF) for n in ("gxc", "gxe", "gzc", "gze")] + \
+                   [("ade", PR * 4), ("ts", _PF)]
+    return OSH
+
+
+class OracleSH:
+    """The SH restatement (osh_* in swpc_oracle.c, src/swpc_sh) on a
+    problem.CaseSH: fields over the (i, k) memory box (nx+6, nz+6)."""
+
+    def __init__(self, case, bits=32):
+        self.case = case
+        self.bits = bits
+        self.L = olib(bits)
+        real = C.c_float if bits == 32 else C.c_double
+        self.dtype = np.float32 if bits == 32 else np.float64
+        g, m = case.grid, case.medium
+        shp = g.shape
+        self.f = {n: np.zeros(shp, self.dtype) for n in FIELDS_SH}
+        self.R = {n: np.zeros(shp + (3,), self.dtype) for n in R_SH}
+        self.A = {n: np.zeros(shp, self.dtype) for n in ADE_SH}
+        self.med = {n: np.ascontiguousarray(getattr(m, n), np.float32) for n in ("rho", "mu", "taus")}
+        self.surf = {n: np.ascontiguousarray(getattr(m, n), np.int32)
+                     for n in ("kfs_top", "kfs_bot", "kob_top", "kob_bot", "kob")}
+        self.prof = [np.ascontiguousarray(a, np.float32) for a in case.pml]
+        self.ts = np.ascontiguousarray(m.ts, np.float32)
+        s = _osh_struct(real)()
+        s.nx, s.nz, s.na, s.nm = g.nx, g.nz, g.na, 3
+        s.dx, s.dz, s.dt = g.dx, g.dz, g.dt
+        PR = C.POINTER(real)
+        for n in FIELDS_SH:
+            setattr(s, n, self.f[n].ctypes.data_as(PR))
+        for n in R_SH:
+            setattr(s, n, self.R[n].ctypes.data_as(PR))
+        for n, a in self.med.items():
+            setattr(s, n, a.ctypes.data_as(_PF))
+        for n, a in self.surf.items():
+            setattr(s, n, a.ctypes.data_as(_PI))
+        for n, a in zip(("gxc", "gxe", "gzc", "gze"), self.prof):
+            setattr(s, n, a.ctypes.data_as(_PF))
+        for q, n in enumerate(ADE_SH):
+            s.ade[q] = self.A[n].ctypes.data_as(PR)
+        s.ts = self.ts.ctypes.data_as(_PF)
+        self.s = s
+        PD = C.POINTER(C.c_double)
+        self.L.osh_vmax.restype = C.c_float
+        gl = case.glut
+        self.nsrc = len(gl.idx) if gl is not None else 0
+        if self.nsrc:
+            self.src = [np.ascontiguousarray(gl.idx[:, q], np.int32) for q in range(2)]
+            self.m2 = np.ascontiguousarray(gl.m, np.float64)
+            self.tab = np.ascontiguousarray(gl.table, np.float64)
+        bf = case.bforce
+        self.nbf = len(bf.idx) if bf is not None else 0
+        if self.nbf:
+            self.bf_idx = [np.ascontiguousarray(bf.idx[:, q], np.int32) for q in range(2)]
+            self.bf_f = np.ascontiguousarray(bf.m, np.float64)
+            self.bf_tab = np.ascontiguousarray(bf.table, np.float64)
+        st = case.stations
+        self.nst = len(st)
+        if self.nst:
+            from openswpc_amd.source import ntw_of
+            self.ist = np.ascontiguousarray(st[:, 0], np.int32)
+            self.kst = np.ascontiguousarray(st[:, 1], np.int32)
+            self.ntw = ntw_of(case.nt, case.ntdec_w)
+            self.vel = np.zeros((self.ntw, self.nst), np.float32)
+            self.disp = np.zeros((self.ntw, self.nst), np.float64)
+            self.uacc = np.zeros(self.nst, np.float64)
+        self._PD = PD
+
+    def _ps(self):
+        return C.byref(self.s)
+
+    def receivers(self, it):
+        if not self.nst:
+            return
+        dec = self.case.ntdec_w
+        vel = np.zeros(self.nst, np.float32)
+        self.L.osh_receivers(self._ps(), self.nst, self.ist.ctypes.data_as(_PI), self.kst.ctypes.data_as(_PI),
+                             vel.ctypes.data_as(_PF), self.uacc.ctypes.data_as(self._PD))
+        if (it - 1) % dec == 0 and (it - 1) // dec < self.ntw:
+            self.vel[(it - 1) // dec] = vel
+            self.disp[(it - 1) // dec] = self.uacc
+
+    def update_stress(self):
+        self.L.osh_update_stress(self._ps())
+        self.L.osh_absorb_stress(self._ps())
+
+    def update_vel(self):
+        self.L.osh_update_vel(self._ps())
+        self.L.osh_absorb_vel(self._ps())
+
+    def stressglut(self, it):
+        if self.nsrc:
+            row = np.ascontiguousarray(self.tab[it - 1])
+            i, k = self.src
+            self.L.osh_stressglut(self._ps(), self.nsrc, i.ctypes.data_as(_PI), k.ctypes.data_as(_PI),
+                                  self.m2.ctypes.data_as(self._PD), row.ctypes.data_as(self._PD))
+
+    def bodyforce(self, it):
+        if self.nbf:
+            row = np.ascontiguousarray(self.bf_tab[it - 1])
+            i, k = self.bf_idx
+            self.L.osh_bodyforce(self._ps(), self.nbf, i.ctypes.data_as(_PI), k.ctypes.data_as(_PI),
+                                 self.bf_f.ctypes.data_as(self._PD), row.ctypes.data_as(self._PD))
+
+    def step(self, it):
+        """src/swpc_sh/main.f90:97-114 order."""
+        self.receivers(it)
+        self.update_stress()
+        self.stressglut(it)
+        self.update_vel()
+        self.bodyforce(it)
+
+    def run(self, it0, n):
+        for it in range(it0, it0 + n):
+            self.step(it)
+
+    def field(self, name):
+        if name in self.f:
+            return self.f[name]
+        if name in self.A:
+            return self.A[name]
+        return self.R[name[:3]][..., int(name[3:]) - 1]
+
+    def set_field(self, name, a):
+        if name in self.f:
+            self.f[name][...] = a
+        elif name in self.A:
+            self.A[name][...] = a
+        else:
+            self.R[name[:3]][..., int(name[3:]) - 1] = a
+
+    def vmax(self):
+        return float(self.L.osh_vmax(self._ps()))

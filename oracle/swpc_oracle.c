@@ -776,6 +776,182 @@ void o2d_step(o2d_t *s, int nsrc, const int *isrc, const int *ksrc, const double
 #undef B
 
 /* ------------------------------------------------------------------ */
+/* SH (src/swpc_sh)                                                    */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int nx, nz, na, nm;
+    double dx, dz;
+    float dt;
+    REAL *Vy, *Syz, *Sxy;
+    REAL *Ryz, *Rxy;                       /* (m, box) */
+    const float *rho, *mu, *taus;
+    const int *kfs_top, *kfs_bot, *kob_top, *kob_bot, *kob;   /* over i in the memory box */
+    const float *gxc, *gxe, *gzc, *gze;
+    REAL *ade[4];                          /* axVy azVy axSxy azSyz */
+    const float *ts;
+} osh_t;
+
+/* 2 a b / (a + b + epsilon(1.0)) in real(SP) (src/swpc_sh/m_kernel.f90:206-209) */
+static inline float harm2(float a, float b)
+{
+    const float epsl = 1.1920928955078125e-07f;
+    return 2 * a * b / (a + b + epsl);
+}
+
+#define BS(F, dk, di) s->F[J2(s, k + (dk), i + (di))]
+
+/* src/swpc_sh/m_kernel.f90:88-139 */
+void osh_update_vel(osh_t *s)
+{
+    const int na = s->na, nx = s->nx, nz = s->nz;
+    const coef_t cx = mkcoef(s->dx), cz = mkcoef(s->dz);
+    const REAL dt = (REAL)s->dt;
+#pragma omp parallel for schedule(static)
+    for (int i = na + 1; i <= nx - na; i++)
+        for (int k = 1; k <= nz - na; k++) {
+            int isign = order_sign(k, s->kfs_top[J1(i)], s->kfs_bot[J1(i)], s->kob_top[J1(i)], s->kob_bot[J1(i)]);
+            REAL e40x = cx.rc40 + isign * cx.rd40, e41x = cx.rc41 + isign * cx.rd41;
+            REAL e40z = cz.rc40 + isign * cz.rd40, e41z = cz.rc41 + isign * cz.rd41;
+            REAL dzSyz = (BS(Syz, 0, 0) - BS(Syz, -1, 0)) * e40z - (BS(Syz, 1, 0) - BS(Syz, -2, 0)) * e41z;
+            REAL dxSxy = (BS(Sxy, 0, 0) - BS(Sxy, 0, -1)) * e40x - (BS(Sxy, 0, 1) - BS(Sxy, 0, -2)) * e41x;
+            size_t c = J2(s, k, i);
+            float by = 1.0f / s->rho[c];
+            s->Vy[c] = s->Vy[c] + by * (dxSxy + dzSyz) * dt;
+        }
+}
+
+/* src/swpc_sh/m_kernel.f90:141-223 */
+void osh_update_stress(osh_t *s)
+{
+    const int na = s->na, nx = s->nx, nz = s->nz, nm = s->nm;
+    const coef_t cx = mkcoef(s->dx), cz = mkcoef(s->dz);
+    const zener_t z = mkzener(nm, s->ts, s->dt);
+    const REAL dt = (REAL)s->dt;
+#pragma omp parallel for schedule(static)
+    for (int i = na + 1; i <= nx - na; i++)
+        for (int k = 1; k <= nz - na; k++) {
+            int isign = order_sign(k, s->kfs_top[J1(i)], s->kfs_bot[J1(i)], s->kob_top[J1(i)], s->kob_bot[J1(i)]);
+            REAL e40x = cx.rc40 + isign * cx.rd40, e41x = cx.rc41 + isign * cx.rd41;
+            REAL e40z = cz.rc40 + isign * cz.rd40, e41z = cz.rc41 + isign * cz.rd41;
+            REAL dxVy = (BS(Vy, 0, 1) - BS(Vy, 0, 0)) * e40x - (BS(Vy, 0, 2) - BS(Vy, 0, -1)) * e41x;
+            REAL dzVy = (BS(Vy, 1, 0) - BS(Vy, 0, 0)) * e40z - (BS(Vy, 2, 0) - BS(Vy, -1, 0)) * e41z;
+            size_t c = J2(s, k, i);
+            float nnn = s->mu[c], pnn = s->mu[J2(s, k + 1, i)], npn = s->mu[J2(s, k, i + 1)];
+            REAL mu_xy = harm2(nnn, npn), mu_yz = harm2(nnn, pnn);
+            REAL taus1 = s->taus[c];
+            REAL f_Ryz = mu_yz * taus1 * dzVy;
+            REAL f_Rxy = mu_xy * taus1 * dxVy;
+            REAL Ryz_n = 0, Rxy_n = 0;
+            for (int m = 0; m < nm; m++) {
+                size_t r = (size_t)m + (size_t)nm * c;
+                s->Ryz[r] = z.c1[m] * s->Ryz[r] - z.c2[m] * f_Ryz * dt;
+                s->Rxy[r] = z.c1[m] * s->Rxy[r] - z.c2[m] * f_Rxy * dt;
+                Ryz_n += z.d1[m] * s->Ryz[r];
+                Rxy_n += z.d1[m] * s->Rxy[r];
+            }
+            REAL taus_plus1 = 1 + taus1 * (1 + z.d2);
+            s->Syz[c] = s->Syz[c] + (mu_yz * taus_plus1 * dzVy + Ryz_n) * dt;
+            s->Sxy[c] = s->Sxy[c] + (mu_xy * taus_plus1 * dxVy + Rxy_n) * dt;
+        }
+}
+
+static inline int kbeg_ash(const osh_t *s, int i)
+{
+    if (i <= s->na || s->nx - s->na + 1 <= i) return 1;
+    return s->nz - s->na + 1;
+}
+
+/* src/swpc_sh/m_absorb_p.f90:174-187.  The reference takes gzc(1:2, i) in
+   the velocity update, reading past the profile when nx > nz; the z
+   profile is taken at k here (its evident intent; gzc(3:4, k) in the same
+   loop). */
+void osh_absorb_vel(osh_t *s)
+{
+    const REAL r20x = (REAL)(float)(1.0 / s->dx), r20z = (REAL)(float)(1.0 / s->dz);
+    const REAL dt = (REAL)s->dt;
+    REAL *axSxy = s->ade[2], *azSyz = s->ade[3];
+#pragma omp parallel for schedule(dynamic)
+    for (int i = 1; i <= s->nx; i++)
+        for (int k = kbeg_ash(s, i); k <= s->nz; k++) {
+            REAL dzSyz = (BS(Syz, 0, 0) - BS(Syz, -1, 0)) * r20z;
+            REAL dxSxy = (BS(Sxy, 0, 0) - BS(Sxy, 0, -1)) * r20x;
+            size_t c = J2(s, k, i);
+            float by = 1.0f / s->rho[c];
+            s->Vy[c] = s->Vy[c] + by * (G(s->gxc, 1, i) * dxSxy + G(s->gzc, 1, k) * dzSyz
+                                       + G(s->gxc, 2, i) * axSxy[c] + G(s->gzc, 2, k) * azSyz[c]) * dt;
+            axSxy[c] = G(s->gxc, 3, i) * axSxy[c] + G(s->gxc, 4, i) * dxSxy * dt;
+            azSyz[c] = G(s->gzc, 3, k) * azSyz[c] + G(s->gzc, 4, k) * dzSyz * dt;
+        }
+}
+
+/* src/swpc_sh/m_absorb_p.f90:262-275: ADE values advanced before the
+   stresses, which use the advanced values */
+void osh_absorb_stress(osh_t *s)
+{
+    const REAL r20x = (REAL)(float)(1.0 / s->dx), r20z = (REAL)(float)(1.0 / s->dz);
+    const REAL dt = (REAL)s->dt;
+    REAL *axVy = s->ade[0], *azVy = s->ade[1];
+#pragma omp parallel for schedule(dynamic)
+    for (int i = 1; i <= s->nx; i++)
+        for (int k = kbeg_ash(s, i); k <= s->nz; k++) {
+            REAL dxVy = (BS(Vy, 0, 1) - BS(Vy, 0, 0)) * r20x;
+            REAL dzVy = (BS(Vy, 1, 0) - BS(Vy, 0, 0)) * r20z;
+            size_t c = J2(s, k, i);
+            float nnn = s->mu[c], pnn = s->mu[J2(s, k + 1, i)], npn = s->mu[J2(s, k, i + 1)];
+            REAL muxy = harm2(nnn, npn), muyz = harm2(nnn, pnn);
+            axVy[c] = G(s->gxe, 3, i) * axVy[c] + G(s->gxe, 4, i) * dxVy * dt;
+            azVy[c] = G(s->gze, 3, k) * azVy[c] + G(s->gze, 4, k) * dzVy * dt;
+            s->Syz[c] = s->Syz[c] + muyz * (G(s->gze, 1, k) * dzVy + G(s->gze, 2, k) * azVy[c]) * dt;
+            s->Sxy[c] = s->Sxy[c] + muxy * (G(s->gxe, 1, i) * dxVy + G(s->gxe, 2, i) * axVy[c]) * dt;
+        }
+}
+
+/* src/swpc_sh/m_source.f90:575-588; m2 = (myz, mxy). */
+void osh_stressglut(osh_t *s, int nsrc, const int *isrc, const int *ksrc, const double *m2, const double *sdrop)
+{
+    for (int n = 0; n < nsrc; n++) {
+        int ii = isrc[n], kk = ksrc[n];
+        REAL sd = (REAL)sdrop[n];
+        REAL myz = (REAL)m2[2 * n], mxy = (REAL)m2[2 * n + 1];
+        s->Sxy[J2(s, kk, ii)] -= mxy * sd / 2;
+        s->Sxy[J2(s, kk, ii - 1)] -= mxy * sd / 2;
+        s->Syz[J2(s, kk, ii)] -= myz * sd / 2;
+        s->Syz[J2(s, kk - 1, ii)] -= myz * sd / 2;
+    }
+}
+
+/* src/swpc_sh/m_source.f90:614-621; fdt = stime*dt/(dx dz). */
+void osh_bodyforce(osh_t *s, int nsrc, const int *isrc, const int *ksrc, const double *fy, const double *fdt)
+{
+    for (int n = 0; n < nsrc; n++) {
+        size_t c = J2(s, ksrc[n], isrc[n]);
+        s->Vy[c] += (REAL)fy[n] / (REAL)s->rho[c] * (REAL)fdt[n];
+    }
+}
+
+/* src/swpc_sh/m_wav.f90:163-200 (v and u): Vy at the station cell; u sums Vy dt */
+void osh_receivers(osh_t *s, int nst, const int *ist, const int *kst, float *vel, double *disp)
+{
+    for (int n = 0; n < nst; n++) {
+        REAL vy = s->Vy[J2(s, kst[n], ist[n])];
+        if (disp) disp[n] += (double)vy * (double)s->dt;
+        if (vel) vel[n] = (float)vy;
+    }
+}
+
+/* src/swpc_sh/m_kernel.f90:225-240 */
+float osh_vmax(osh_t *s)
+{
+    float m = 0;
+    for (int i = s->na + 1; i <= s->nx - s->na; i++) {
+        float a = fabsf((float)s->Vy[J2(s, s->kob[J1(i)] + 1, i)]);
+        if (a > m) m = a;
+    }
+    return m;
+}
+#undef BS
+
+/* ------------------------------------------------------------------ */
 /* helpers                                                             */
 /* ------------------------------------------------------------------ */
 

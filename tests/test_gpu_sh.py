@@ -1,0 +1,76 @@
+"""The SH core (swpcsh_*) against the SH oracle on the same inputs: FP32
+kernels against the FP32 oracle (the reference built with MP = SP), FP64
+against FP64; traces, fields and memory variables, the body-force mode,
+vmax, and step graphs against step-by-step launches."""
+import numpy as np
+import pytest
+
+from openswpc_amd import problem
+from openswpc_amd.solver import SwpcSH
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(nt=240, bodyforce=False, nx=203, nz=117, seed=4):
+    return problem.case_sh(nx, nz, 0.1, 0.005, nt, na=20, seed=seed, rmed=(0.05, 2.0, 0.5), src_depth=4.0, nst=9,
+                           zbeg=-1.0, bodyforce=bodyforce, src_x=0.7)
+
+
+def _open(c, precision):
+    return SwpcSH(c.grid, c.medium, c.pml, c.glut, c.stations, c.ntdec_w, c.nt, precision=precision,
+                  bodyforce=c.bforce)
+
+
+@pytest.mark.parametrize("precision,bodyforce", [("fp32", False), ("fp64", False), ("fp32", True)])
+def test_sh_matches_oracle(precision, bodyforce):
+    from oracle.oracle import OracleSH
+    c = _case(bodyforce=bodyforce)
+    bits = 64 if precision == "fp64" else 32
+    o = OracleSH(c, bits)
+    o.run(1, c.nt)
+    with _open(c, precision) as sim:
+        sim.run(1, c.nt)
+        vel, disp = sim.receivers()
+        fields = {n: sim.field(n) for n in ("Vy", "Syz", "Sxy", "Ryz1", "Rxy3", "axVy", "azSyz")}
+        vmax = float(sim.kernel__vmax()[0])
+    tol = 1e-11 if bits == 64 else 1e-5
+    e = np.linalg.norm(vel - o.vel) / np.linalg.norm(o.vel)
+    ed = np.linalg.norm(disp - o.disp) / np.linalg.norm(o.disp)
+    print(f"{precision} bf={bodyforce}: traces {e:.2e}, displacement {ed:.2e}")
+    assert np.linalg.norm(o.vel) > 0
+    assert e <= tol and ed <= tol
+    for n, a in fields.items():
+        r = o.field(n).astype(np.float64)
+        en = np.linalg.norm(a - r) / max(np.linalg.norm(r), 1e-300)
+        print(f"  {n}: {en:.2e}")
+        # memory variables are small differences of O(1) terms
+        assert en <= (tol if n[0] != "R" else 10 * tol), n
+    assert vmax == pytest.approx(o.vmax(), rel=1e-5 if bits == 32 else 1e-12)
+
+
+def test_sh_graph_replay_equals_step_by_step():
+    c = _case(nt=100, nx=150, nz=90)
+    out = []
+    for k in (16, 0):
+        with _open(c, "fp32") as sim:
+            sim.set_graph(k)
+            sim.run(1, 40)
+            sim.run(41, 60)
+            out.append((sim.receivers()[0], sim.field("Vy")))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_sh_step_functions_equal_run():
+    """The reference routine order driven call by call equals swpcsh_run."""
+    c = _case(nt=60, nx=120, nz=80)
+    with _open(c, "fp32") as a, _open(c, "fp32") as b:
+        a.run(1, c.nt)
+        for it in range(1, c.nt + 1):
+            b.wav__store(it)
+            b.kernel__update_stress()
+            b.source__stressglut(it)
+            b.kernel__update_vel()
+            b.source__bodyforce(it)
+        assert np.array_equal(a.field("Sxy"), b.field("Sxy"))
+        assert np.array_equal(a.receivers()[0], b.receivers()[0])
