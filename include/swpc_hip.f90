@@ -44,6 +44,10 @@ module swpc_hip
     public :: swpc2d_set_sources, swpc2d_set_receivers, swpc2d_update_stress, swpc2d_update_vel
     public :: swpc2d_stressglut, swpc2d_store_receivers, swpc2d_step, swpc2d_run, swpc2d_get_receivers
     public :: swpc2d_vmax, swpc2d_sync
+    public :: swpcsh_create, swpcsh_destroy, swpcsh_set_medium, swpcsh_set_surface, swpcsh_set_pml
+    public :: swpcsh_set_sources, swpcsh_set_receivers, swpcsh_update_stress, swpcsh_update_vel
+    public :: swpcsh_stressglut, swpcsh_store_receivers, swpcsh_step, swpcsh_run, swpcsh_get_receivers
+    public :: swpcsh_vmax, swpcsh_sync
     public :: swpc_lab_nparam, swpc3d_set_lab, swpc3d_set_lab_sponge
     public :: swpc_batch_create, swpc_batch_destroy, swpc_batch_add3d, swpc_batch_add2d, swpc_batch_run
     public :: swpc_batch_sync
@@ -355,6 +359,107 @@ module swpc_hip
         end function
 
         integer(c_int) function swpc2d_sync(ctx) bind(C, name='swpc2d_sync')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+
+        ! ---------------------------------------- SH (swpc_sh)
+
+        integer(c_int) function swpcsh_create(desc, ctx) bind(C, name='swpcsh_create')
+            import :: c_int, c_ptr, swpc2d_desc
+            type(swpc2d_desc), intent(in) :: desc
+            type(c_ptr), intent(out) :: ctx
+        end function
+
+        integer(c_int) function swpcsh_destroy(ctx) bind(C, name='swpcsh_destroy')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+
+        integer(c_int) function swpcsh_set_medium(ctx, rho, mu, taus) bind(C, name='swpcsh_set_medium')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            real(c_float), intent(in) :: rho(*), mu(*), taus(*)
+        end function
+
+        integer(c_int) function swpcsh_set_surface(ctx, kfs_top, kfs_bot, kob_top, kob_bot, kob) &
+            bind(C, name='swpcsh_set_surface')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), intent(in) :: kfs_top(*), kfs_bot(*), kob_top(*), kob_bot(*), kob(*)
+        end function
+
+        integer(c_int) function swpcsh_set_pml(ctx, gxc, gxe, gzc, gze) bind(C, name='swpcsh_set_pml')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            real(c_float), intent(in) :: gxc(*), gxe(*), gzc(*), gze(*)
+        end function
+
+        integer(c_int) function swpcsh_set_sources(ctx, nsrc, isrc, ksrc, m2, nt, sdrop) &
+            bind(C, name='swpcsh_set_sources')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nsrc, nt
+            integer(c_int), intent(in) :: isrc(*), ksrc(*)
+            real(c_double), intent(in) :: m2(*), sdrop(*)
+        end function
+
+        integer(c_int) function swpcsh_set_receivers(ctx, nst, ist, kst, ntdec_w, ntw) &
+            bind(C, name='swpcsh_set_receivers')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nst, ntdec_w, ntw
+            integer(c_int), intent(in) :: ist(*), kst(*)
+        end function
+
+        integer(c_int) function swpcsh_update_stress(ctx) bind(C, name='swpcsh_update_stress')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+
+        integer(c_int) function swpcsh_update_vel(ctx) bind(C, name='swpcsh_update_vel')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+
+        integer(c_int) function swpcsh_stressglut(ctx, it) bind(C, name='swpcsh_stressglut')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it
+        end function
+
+        integer(c_int) function swpcsh_store_receivers(ctx, it) bind(C, name='swpcsh_store_receivers')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it
+        end function
+
+        integer(c_int) function swpcsh_step(ctx, it) bind(C, name='swpcsh_step')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it
+        end function
+
+        integer(c_int) function swpcsh_run(ctx, it0, nsteps) bind(C, name='swpcsh_run')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it0, nsteps
+        end function
+
+        integer(c_int) function swpcsh_get_receivers(ctx, vel, disp) bind(C, name='swpcsh_get_receivers')
+            import :: c_int, c_ptr, c_float, c_double
+            type(c_ptr), value :: ctx
+            real(c_float), intent(out) :: vel(*)
+            real(c_double), intent(out) :: disp(*)
+        end function
+
+        integer(c_int) function swpcsh_vmax(ctx, out1) bind(C, name='swpcsh_vmax')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            real(c_float), intent(out) :: out1(1)
+        end function
+
+        integer(c_int) function swpcsh_sync(ctx) bind(C, name='swpcsh_sync')
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx
         end function

@@ -41,8 +41,9 @@ import numpy as np
 
 from . import cdf, fdtool, medium as md, pml, problem, sac, snapshot
 from .inifile import InputFile
-from .source import (BodyForce, PointSource, body_force2, body_force3, read_bodyforce_file, read_source_file,
-                     read_station_file, station_cells2, station_cells3, stress_glut2, stress_glut3)
+from .source import (BodyForce, PointSource, body_force2, body_force3, body_force_sh, read_bodyforce_file,
+                     read_source_file, read_station_file, station_cells2, station_cells3, stress_glut2, stress_glut3,
+                     stress_glut_sh)
 
 F32 = np.float32
 SNAP3 = [(p, k) for k in ("ps", "v", "u") for p in ("yz", "xz", "xy", "fs", "ob")]
@@ -272,6 +273,10 @@ def sources(ini, g, dim, bench, med, bddep0, clon, clat, phi, nt):
         if bf:
             return None, body_force3(g, src, nt, stftype), info
         return stress_glut3(g, src, nt, stftype), None, info
+    if dim == "sh":
+        if bf:
+            return None, body_force_sh(g, src, nt, stftype), info
+        return stress_glut_sh(g, src, nt, stftype), None, info
     if bf:
         return None, body_force2(g, src, nt, stftype), info
     return stress_glut2(g, src, nt, stftype), None, info
@@ -283,7 +288,7 @@ def sources(ini, g, dim, bench, med, bddep0, clon, clat, phi, nt):
 def setup(inp, dim="3d"):
     ini = _ini(inp)
     _unsupported(ini, dim)
-    title = ini.get("title", "swpc3d" if dim == "3d" else "swpc_psv")
+    title = ini.get("title", {"3d": "swpc3d", "psv": "swpc_psv", "sh": "swpc_sh"}[dim])
     nt = ini.get("nt", 1000)
     odir = ini.get("odir", "./out")
     g, bench, clon, clat, phi = _grid(ini, dim)
@@ -297,7 +302,7 @@ def setup(inp, dim="3d"):
         med = md.medium_setup2(g, rho, lam, mu, qp, qs, fq_min, fq_max, fq_ref, benchmark=bench)
     glut, bforce, info = sources(ini, g, dim, bench, med, bddep0, clon, clat, phi, nt)
     tab = glut if glut is not None else bforce
-    prof = pml.pml3(g, tab.fcut) if dim == "3d" else pml.pml2(g, tab.fcut)
+    prof = pml.pml3(g, tab.fcut) if dim == "3d" else (pml.pml_sh(g, tab.fcut) if dim == "sh" else pml.pml2(g, tab.fcut))
     # report__setup: stability and wavelength conditions (m_report.f90:71-104)
     fmax = 2.0 * tab.fcut
     c = problem.stability3(g, med.vmax) if dim == "3d" else \
@@ -336,6 +341,8 @@ def setup(inp, dim="3d"):
         snaps = [(p, k) for (p, k) in SNAP3 if ini.get(f"{p}_{k}%sw", False)]
     else:
         snaps = [("xz", k) for k in ("ps", "v", "u") if ini.get(f"xz_{k}%sw", False)]
+        if dim == "sh" and snaps:
+            raise InputError("SH snapshots (xz_*%sw) are not supported by this host")
     wav = {"v": sw_v, "u": sw_u, "format": ini.get("wav_format", "sac"), "prg": ini.get("ntdec_w_prg", 0)}
     if wav["format"] not in ("sac", "csf", "tar_st", "tar_node"):
         raise InputError(f"unknown wav_format {wav['format']!r}")
@@ -363,7 +370,9 @@ def write_waveforms(S, sim, meta):
         return []
     vel, disp = sim.receivers(scaled=True)
     meta.ntw = sim.ntw
-    comp = [0, 1, 2] if S.dim == "3d" else [0, 1]
+    if S.dim == "sh":   # one component (Vy, Uy)
+        vel, disp = vel[:, :, None], disp[:, :, None]
+    comp = {"3d": [0, 1, 2], "psv": [0, 1], "sh": [0]}[S.dim]
     v = vel[:, :, comp] if S.wav["v"] else None
     u = disp[:, :, comp].astype(F32) if S.wav["u"] else None
     fmt = S.wav["format"]
@@ -378,7 +387,7 @@ def write_waveforms(S, sim, meta):
 def run(inp, dim="3d", precision="fp32", log=None, exedate=None):
     """Set up and run the simulation of an input.inf; returns the Setup with
     report["wall"] and report["paths"] filled in."""
-    from .solver import NumericalDivergence, Swpc3D, SwpcPSV, DIVERGENCE_TOL
+    from .solver import NumericalDivergence, Swpc3D, SwpcPSV, SwpcSH, DIVERGENCE_TOL
     log = log if log is not None else (lambda s: print(s, file=sys.stderr, flush=True))
     S = setup(inp, dim)
     g = S.grid
@@ -391,7 +400,7 @@ def run(inp, dim="3d", precision="fp32", log=None, exedate=None):
     log(f"  Minimum velocity        : {rp['vmin']:15.3f}  [km/s]")
     log(f"  Maximum velocity        : {rp['vmax']:15.3f}  [km/s]")
     log(f"  Maximum frequency       : {rp['fmax']:15.3f}  [Hz]")
-    Sim = Swpc3D if dim == "3d" else SwpcPSV
+    Sim = {"3d": Swpc3D, "psv": SwpcPSV, "sh": SwpcSH}[dim]
     cells = S.cells if len(S.cells) else None
     with Sim(g, S.medium, S.prof, S.glut, cells, S.ntdec_w, S.nt, precision=precision, bodyforce=S.bforce) as sim:
         meta = _wav_meta(S, exedate, sim.M0)
@@ -449,7 +458,7 @@ def run(inp, dim="3d", precision="fp32", log=None, exedate=None):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("-i", default="./in/input.inf", help="parameter file (input.inf)")
-    ap.add_argument("-d", "--dim", default="3d", choices=["3d", "psv"])
+    ap.add_argument("-d", "--dim", default="3d", choices=["3d", "psv", "sh"])
     ap.add_argument("-p", "--precision", default="fp32", choices=["fp32", "fp64", "fp32_s16", "fp32_r16"])
     a = ap.parse_args(argv)
     try:

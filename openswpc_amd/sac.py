@@ -199,6 +199,8 @@ def station_header(meta: WavMeta, stnm, xst, yst, zst, stlo=UNDEF_F, stla=UNDEF_
             h.user0, h.user1, h.user2 = meta.f3
         elif meta.m6 is not None:
             h.user0, h.user1, h.user2, h.user3, h.user4, h.user5 = meta.m6
+    elif dim == "sh":
+        pass   # the SH header leaves user0..5 undefined (src/swpc_sh/m_source.f90:458-463, 542-544)
     else:  # P-SV keeps the in-plane components only
         if meta.f3 is not None:
             h.user0, h.user2 = meta.f3[0], meta.f3[2]
@@ -229,6 +231,12 @@ COMPONENTS_3D = {"v": [("Vx", 90.0, 0.0), ("Vy", 90.0, 90.0), ("Vz", 90.0, None)
                  "u": [("Ux", 90.0, 0.0), ("Uy", 90.0, 90.0), ("Uz", 90.0, None)]}
 COMPONENTS_PSV = {"v": [("Vx", 90.0, 0.0), ("Vz", 0.0, 0.0)],
                   "u": [("Ux", 90.0, 0.0), ("Uz", 0.0, 0.0)]}
+# SH (src/swpc_sh/m_wav.f90:564-573)
+COMPONENTS_SH = {"v": [("Vy", 90.0, 90.0)], "u": [("Uy", 90.0, 90.0)]}
+
+
+def _components(dim):
+    return {"3d": COMPONENTS_3D, "psv": COMPONENTS_PSV, "sh": COMPONENTS_SH}[dim]
 
 
 def write_wav(odir, meta: WavMeta, stations, vel=None, disp=None, dim="3d", stlonlat=None):
@@ -236,7 +244,7 @@ def write_wav(odir, meta: WavMeta, stations, vel=None, disp=None, dim="3d", stlo
     wav_format = 'sac').  ``vel``/``disp`` have shape (ntw, nst, ncomp) as
     returned by the solvers' ``receivers()``; ``stations`` are objects with
     ``x, y, z, name`` (y ignored for P-SV).  Returns the written paths."""
-    comps = COMPONENTS_3D if dim == "3d" else COMPONENTS_PSV
+    comps = _components(dim)
     wdir = os.path.join(odir, "wav")
     os.makedirs(wdir, exist_ok=True)
     out = []
@@ -266,7 +274,7 @@ def wav_traces(meta: WavMeta, stations, vel=None, disp=None, dim="3d", stlonlat=
     """(header, samples) of every exported trace, in the reference's order:
     velocity components of all stations, then displacement (``sh_vel(j,i)``
     with the component fastest, m_wav.f90:680-700, 710-716)."""
-    comps = COMPONENTS_3D if dim == "3d" else COMPONENTS_PSV
+    comps = _components(dim)
     out = []
     for kind, arr, idep in (("v", vel, IDEP_VEL), ("u", disp, IDEP_DISP)):
         if arr is None:

@@ -88,6 +88,53 @@ ntdec_s = 10
     assert v["Vz"].shape[0] == (S.nt - 1) // 10 + 1
 
 
+def test_small_sh_input_end_to_end(tmp_path, monkeypatch):
+    """swpc_sh.x -i input.inf: the SH host writes Vy/Uy SAC files holding
+    the SH oracle's traces of the same setup (x M0 UC 1e9, UC = 1e-12)."""
+    from oracle.oracle import OracleSH
+    monkeypatch.chdir(tmp_path)
+    (tmp_path / "src.dat").write_text("0.7 0.0 4.0 0.1 0.6 1e15 30 70 -40\n")
+    (tmp_path / "st.xy").write_text("0.0 0.0 0.0 H1 fsb\n3.0 0.0 0.0 H2 fsb\n-4.0 0.0 2.0 H3 dep\n")
+    shutil.copy(os.path.join(EX, "lhm.dat"), tmp_path / "lhm.dat")
+    txt = """
+title = 'smallsh'
+odir = './out'
+ntdec_r = 20
+nx = 160
+nz = 100
+nt = 80
+dx = 0.1
+dz = 0.1
+dt = 0.008
+zbeg = -1.0
+na = 10
+vmodel_type = 'lhm'
+fn_lhm = 'lhm.dat'
+vcut = 1.5
+stf_format = 'xym0dc'
+fn_stf = 'src.dat'
+sw_wav_v = .true.
+sw_wav_u = .true.
+ntdec_w = 2
+fn_stloc = 'st.xy'
+"""
+    print("driver: small SH input 160x100, 80 steps", flush=True)
+    lines = []
+    S = driver.run(InputFile(text=txt), dim="sh", log=lines.append, exedate=1700000000)
+    assert sum("it=" in ln for ln in lines) == 4
+    case = problem.Case2D(S.grid, S.medium, S.prof, S.glut, S.cells, S.stations, S.nt, S.ntdec_w, None)
+    ref = OracleSH(case, 32)
+    ref.run(1, S.nt)
+    scale = F32(S.glut.M0 * 1e-12 * 1e9)
+    assert np.abs(ref.vel).max() > 0
+    for s, st in enumerate(S.stations):
+        h, d = sac.read_sac(f"out/wav/smallsh.sh.{st.name}.Vy.sac")
+        assert h.npts == (S.nt - 1) // 2 + 1 and h.kcmpnm.strip() == "Vy"
+        assert rel(d, ref.vel[:, s] * scale) <= 1e-5, st.name
+        h, d = sac.read_sac(f"out/wav/smallsh.sh.{st.name}.Uy.sac")
+        assert rel(d, ref.disp[:, s] * float(scale)) <= 1e-5, st.name
+
+
 def _progress(lines):
     out = {}
     for ln in lines:
