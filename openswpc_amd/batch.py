@@ -88,16 +88,18 @@ def mixed_cases(seed=1, n2d=24, n3d=24, nt=2000, side2=(256, 1024), side3=(96, 2
             depth = float(rng.uniform(2.0, min(20.0, (nz - na - 8) * dx - 0.75)))
             specs.append(("3d", dict(nx=nx, ny=ny, nz=nz, dx=dx, dt=0.01, nt=nt, na=na, seed=s,
                                      rmed=(eps, 2.0, 0.5), src_depth=depth, nst_side=2, zbeg=-0.75,
-                                     rmed_tile=(64, 64, 64),
+                                     rmed_tile=(64, 64, 64), device_medium=True,
                                      src_xy=(float(rng.uniform(-hx, hx)), float(rng.uniform(-hy, hy))))))
     for kind, kw in specs:
         yield kind, (problem.case2d(**kw) if kind == "2d" else problem.case3d(**kw))
 
 
 def open_case(kind, c, precision="fp32"):
-    from .solver import Swpc3D, SwpcPSV
-    cls = Swpc3D if kind == "3d" else SwpcPSV
-    return cls(c.grid, c.medium, c.pml, c.glut, c.stations, c.ntdec_w, c.nt, precision=precision)
+    """A context for one case; 3-D cases set their medium up on the GPU."""
+    from .solver import SwpcPSV
+    if kind == "3d":
+        return problem.open_case3d(c, precision)
+    return SwpcPSV(c.grid, c.medium, c.pml, c.glut, c.stations, c.ntdec_w, c.nt, precision=precision)
 
 
 def size_class(g):

@@ -175,7 +175,7 @@ class LabCase:
 
 
 def wide_case(nxs, nys, nzs, dx, dt, nt, seed, zbeg=-0.75, eps=0.05, a=2.0, layers=problem.LHM_EXAMPLE,
-              off=OFFSET, na=REF_NA):
+              off=OFFSET, na=REF_NA, device_medium=False):
     """The reference run of one training or evaluation case: a grid of
     (nxs + 2*off) x (nys + 2*off) x (nzs + off) cells (absorber of na cells,
     none by default) holding the inner nxs x nys x nzs box at (off, off, 0);
@@ -183,7 +183,9 @@ def wide_case(nxs, nys, nzs, dx, dt, nt, seed, zbeg=-0.75, eps=0.05, a=2.0, laye
     km, kappa 0.5) and a double couple of random position, depth and
     mechanism inside the inner box, all drawn from the seed.  With na = 0
     the run stands for the unbounded medium while no echo of its edges can
-    reach the inner box: 2*off cells of travel, i.e. nt*vmax*dt/dx < 2*off."""
+    reach the inner box: 2*off cells of travel, i.e. nt*vmax*dt/dx < 2*off.
+    device_medium=True leaves medium__setup to the GPU (the training runs
+    need no host copy of the assembled medium; inner_case does)."""
     g = md.Grid3D(nxs + 2 * off, nys + 2 * off, nzs + off, dx, dx, dx, dt, na=na, zbeg=zbeg)
     xi = None
     if eps > 0:
@@ -191,7 +193,10 @@ def wide_case(nxs, nys, nzs, dx, dt, nt, seed, zbeg=-0.75, eps=0.05, a=2.0, laye
         vol = rmedia.generate(tile, dx, a, eps, 0.5, seed=seed)
         xi = md.rmed_tile(vol, -2, g.nx + 3, -2, g.ny + 3, -2, g.nz + 3)
     rho, lam, mu, qp, qs = md.layered3(g, layers, xi=xi)
-    med = md.medium_setup3(g, rho, lam, mu, qp, qs)
+    if device_medium:
+        med = md.RawMedium3D(rho, lam, mu, qp, qs)
+    else:
+        med = md.medium_setup3(g, rho, lam, mu, qp, qs)
     rng = np.random.default_rng(seed + 7919)
     mech = (float(rng.uniform(0, 360)), float(rng.uniform(0, 90)), float(rng.uniform(-180, 180)))
     hx, hy = (nxs / 2 - 8) * dx, (nys / 2 - 8) * dx
@@ -299,17 +304,17 @@ def train_boundary(nruns=200, nxs=64, nys=64, nzs=48, dx=0.25, dt=0.01, nt=500, 
         else:
             layers, zbeg = random_layers(rng), -0.75
         specs.append((int(seed) * 100_003 + r, layers, float(np.float32(round(zbeg / dx) * dx))))
-    # the host builds the next cases (FFT random media, medium__setup)
-    # while the GPU runs the current one
+    # the host builds the next cases (FFT random media, layered model)
+    # while the GPU runs the current one; medium__setup runs on the GPU
     with ThreadPoolExecutor(max_workers=6) as pool:
-        futs = [pool.submit(wide_case, nxs, nys, nzs, dx, dt, nt, s, zbeg, 0.05, 2.0, layers)
+        futs = [pool.submit(wide_case, nxs, nys, nzs, dx, dt, nt, s, zbeg, 0.05, 2.0, layers, OFFSET, REF_NA, True)
                 for s, layers, zbeg in specs[:4]]
         for r in range(nruns):
             lc = futs[r].result()
             futs[r] = None
             if r + 4 < nruns:
                 s, layers, zbeg = specs[r + 4]
-                futs.append(pool.submit(wide_case, nxs, nys, nzs, dx, dt, nt, s, zbeg, 0.05, 2.0, layers))
+                futs.append(pool.submit(wide_case, nxs, nys, nzs, dx, dt, nt, s, zbeg, 0.05, 2.0, layers, OFFSET, REF_NA, True))
             gather_run(tr, lc, quota, keep, thr, specs[r][0])
             del lc
             if log and (r % 20 == 19 or r == nruns - 1):
