@@ -9,7 +9,7 @@
 //
 // Kernels
 //   k_lab_colmed  per-column medium features (vs/vp, vp*dt/h, rho*vp)
-//   k_lab_fill    ghost fill of one phase, one launch per face: one lane
+//   k_lab_fill_f32 / k_lab_fill_f64   ghost fill of one phase: one lane
 //                 per column; the hidden layers on the matrix cores
 //                 (32x32x16 BF16 MFMA, FP32 accumulation) over each
 //                 wave's 64 columns, the output layer in FP32
@@ -334,7 +334,19 @@ constexpr int LW1 = 72;    // W1 as BF16 [32][72]
 constexpr int LW2 = 40;    // W2 as BF16 [32][40]
 constexpr int LO = 96;     // output layer as FP32 [5][96]: W3 row, Ws row, b3
 constexpr int FILL_WAVES = 4;
-constexpr int WAVE_LDS = 64 * LX * 2 + 64 * LH * 2;   // bytes per wave
+// Per-wave LDS region, shared by three arrays that are each written only
+// after the rows they overwrite have been consumed by the same wave:
+//   features X  BF16 [64][LX] at 0 (9216 B), read by layer 1;
+//   layer 1 H1  BF16 [64][LH] at 0 (5120 B): tile t writes its rows after
+//               its MFMAs have read X rows 32t..32t+31, and rows 32..63 of
+//               X start at byte 4608, past H1's tile-0 rows;
+//   layer 2 H2  FP32 rows of LH2: tile 0's rows at H2_T0 = 5120 (past all
+//               of H1), tile 1's at 0 (H1 rows 0..31 are dead by then; tile
+//               1 reads its own H1 rows before it writes).
+// 9344 B per wave instead of 14336 lets three 4-wave blocks share a CU.
+constexpr int H2_T0 = 64 * LH * 2;
+constexpr int WAVE_LDS = H2_T0 + 32 * LH2 * 4;   // 9344 bytes per wave
+static_assert(WAVE_LDS >= 64 * LX * 2 && WAVE_LDS % 16 == 0, "fill LDS region");
 constexpr int IMG_W2 = 32 * LW1 * 2, IMG_O = IMG_W2 + 32 * LW2 * 2;      // byte offsets in a net's image
 constexpr int IMG_B1 = IMG_O + LAB_NOUT * LO * 4, IMG_B2 = IMG_B1 + 32 * 4;
 constexpr int FILL_LDS0 = IMG_B2 + 32 * 4;   // one net's image: bytes before the waves' rows
@@ -353,7 +365,7 @@ static_assert(FILL_LDS0 == LAB_IMG_BYTES && FILL_LDS0 % 16 == 0, "net image layo
 // B[8 (l / 32) + 0..7][l % 32]; the accumulator's register r holds
 // D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
 template <typename T>
-__global__ void __launch_bounds__(64 * FILL_WAVES) k_lab_fill(const LabFillArgs p)
+__device__ __forceinline__ void lab_fill_body(const LabFillArgs &p)
 {
     extern __shared__ __align__(16) unsigned char lds[];
     const __bf16 *sW1 = (const __bf16 *)lds;                 // [32][LW1]
@@ -363,8 +375,8 @@ __global__ void __launch_bounds__(64 * FILL_WAVES) k_lab_fill(const LabFillArgs 
     const float *sO = (const float *)(lds + IMG_O);          // [LAB_NOUT][LO]: W3 row, Ws row, b3
     unsigned char *wl = lds + FILL_LDS0 + wave * WAVE_LDS;
     __bf16 *sX = (__bf16 *)wl;                         // [64][LX]
-    float *sH2 = (float *)wl;                          // [64][LH2], reuses the feature rows after layer 1
-    __bf16 *sH1 = (__bf16 *)(wl + 64 * LX * 2);        // [64][LH]
+    __bf16 *sH1 = (__bf16 *)wl;                        // [64][LH], over X (see WAVE_LDS)
+    float *sH2t[2] = {(float *)(wl + H2_T0), (float *)wl};   // rows 0..31, rows 32..63
 
     const bool second = (int)blockIdx.x >= p.nb0;
     const int f = p.face[second ? 1 : 0], net = 2 * f + p.phase;
@@ -458,7 +470,7 @@ __global__ void __launch_bounds__(64 * FILL_WAVES) k_lab_fill(const LabFillArgs 
 #pragma unroll
         for (int r = 0; r < 16; r++) {
             const int row = 32 * t + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
-            sH2[row * LH2 + ln] = lab_tanh(acc[r] + b2);
+            sH2t[t][(row - 32 * t) * LH2 + ln] = lab_tanh(acc[r] + b2);
         }
     }
     __syncthreads();
@@ -467,7 +479,7 @@ __global__ void __launch_bounds__(64 * FILL_WAVES) k_lab_fill(const LabFillArgs 
     // output layer in FP32: y = b3 + W3 h2 + Ws x
     float h2[LAB_NH];
 #pragma unroll
-    for (int h = 0; h < LAB_NH; h++) h2[h] = sH2[l * LH2 + h];
+    for (int h = 0; h < LAB_NH; h++) h2[h] = sH2t[l >> 5][(l & 31) * LH2 + h];
 #pragma unroll 1
     for (int o = 0; o < nout; o++) {
         const float *wo = sO + o * LO;
@@ -481,6 +493,19 @@ __global__ void __launch_bounds__(64 * FILL_WAVES) k_lab_fill(const LabFillArgs 
         if (fi >= 3) v = v * m.z;
         lab_st<T>(a, fi, gaddr[o], v);
     }
+}
+
+// FP32 state: 168 VGPRs and three 4-wave blocks per CU (the LDS of
+// lab_fill_body fits three); FP64 state: two, without spills.
+__global__ void __launch_bounds__(64 * FILL_WAVES) __attribute__((amdgpu_waves_per_eu(3, 3)))
+k_lab_fill_f32(const LabFillArgs p)
+{
+    lab_fill_body<float>(p);
+}
+
+__global__ void __launch_bounds__(64 * FILL_WAVES) __attribute__((amdgpu_waves_per_eu(2, 2))) k_lab_fill_f64(const LabFillArgs p)
+{
+    lab_fill_body<double>(p);
 }
 
 int lab_colmed(const LabGrid &g, const LabDomain &dom, DBuf &out, long long &bytes, hipStream_t s)
@@ -556,8 +581,8 @@ int lab_fill(const LabGrid &g, const LabDomain &dom, const void *img, const floa
         const int nb = p.nb0 + (pair ? (dom.ncol[f + 1] + cols - 1) / cols : 0);
         f += pair ? 2 : 1;
         if (nb == 0) continue;
-        if (g.f64) hipLaunchKernelGGL(k_lab_fill<double>, dim3(nb), dim3(cols), lds, s, p);
-        else hipLaunchKernelGGL(k_lab_fill<float>, dim3(nb), dim3(cols), lds, s, p);
+        if (g.f64) hipLaunchKernelGGL(k_lab_fill_f64, dim3(nb), dim3(cols), lds, s, p);
+        else hipLaunchKernelGGL(k_lab_fill_f32, dim3(nb), dim3(cols), lds, s, p);
         SWPC_CHECK_LAUNCH();
     }
     return SWPC_OK;

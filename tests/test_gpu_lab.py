@@ -118,16 +118,19 @@ def test_gather_matches_oracle():
     assert nrec > 1000
 
 
-@pytest.mark.parametrize("precision,sponge", [("fp32", None), ("fp32_r16", None), ("fp64", None), ("fp32", 0.0),
-                                              ("fp32", 0.3)])
-def test_lab_step_matches_oracle(precision, sponge):
+@pytest.mark.parametrize("precision,sponge,nxs", [("fp32", None, 24), ("fp32_r16", None, 24), ("fp64", None, 24),
+                                                  ("fp32", 0.0, 24), ("fp32", 0.3, 24), ("fp32", None, 25)])
+def test_lab_step_matches_oracle(precision, sponge, nxs):
     """A learned-boundary run (na = 0) with bounded demo nets: the GPU's
     fields and traces against the oracle's learned-boundary step (FP32,
     or FP64 fields for the FP64 context; the nets run in FP32 either way),
-    with the default strip sponge (None), none, or a strong one."""
+    with the default strip sponge (None), none, or a strong one.  With
+    nxs = 24 the x faces' strips are read and written with 16- and 8-byte
+    accesses (x-: 16-byte aligned, x+: 8-byte aligned); nxs = 25 makes the
+    x+ strip odd-aligned, which takes the per-cell path."""
     from oracle.oracle import Oracle3D
     nt = 120
-    lc = _small_wide(seed=6, nt=nt)
+    lc = lab.wide_case(nxs, 20, 18, 0.25, 0.01, nt, 6, zbeg=-0.75, off=12)
     c = lab.inner_case(lc, 0, nst_side=3)
     P = _demo_params(1)
     o = Oracle3D(c, 64 if precision == "fp64" else 32)
