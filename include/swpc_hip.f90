@@ -32,7 +32,7 @@ module swpc_hip
     end type swpc2d_desc
 
     public :: swpc_errmsg
-    public :: swpc_version, swpc_device_count
+    public :: swpc_version, swpc_device_count, swpc_rmedia_generate
     public :: swpc3d_create, swpc3d_destroy, swpc3d_set_medium, swpc3d_set_density, swpc3d_set_surface, swpc3d_set_pml
     public :: swpc3d_setup_medium, swpc3d_get_medium
     public :: swpc3d_set_sources, swpc3d_set_bodyforce, swpc3d_set_receivers
@@ -65,6 +65,19 @@ module swpc_hip
 
         integer(c_int) function swpc_device_count() bind(C, name='swpc_device_count')
             import :: c_int
+        end function
+
+        ! shape, d, a in C order (last axis fastest): pass them reversed
+        ! from Fortran's (nx, ny, nz) so that x stays the fastest axis
+        integer(c_int) function swpc_rmedia_generate(nd, shape, d, a, epsil, kappa, ptype, noise, out) &
+                bind(C, name='swpc_rmedia_generate')
+            import :: c_int, c_double, c_float
+            integer(c_int), value :: nd, ptype
+            integer(c_int), intent(in) :: shape(*)
+            real(c_double), intent(in) :: d(*), a(*)
+            real(c_double), value :: epsil, kappa
+            real(c_float), intent(in) :: noise(*)
+            real(c_float) :: out(*)
         end function
 
         ! ---------------------------------------------------------------- 3-D

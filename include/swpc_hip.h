@@ -88,6 +88,22 @@ int swpc_device_count(void);
    ceiling the stencil kernels are reported against. */
 int swpc_membw(long long bytes, int iters, double *gbs);
 
+/* Random-media synthesis: replaces rmedia__3dgen_ani / rmedia__2dgen_ani
+   (src/shared/m_rmedia.f90:40-173; its PSDFs m_rmedia.f90:548-588 in 2-D,
+   682-732 in 3-D).
+   nd = 2 or 3; shape[nd] with the LAST axis fastest (C order); d[nd] grid
+   widths and a[nd] correlation lengths (km); rms epsil; kappa (von Karman
+   only); ptype 0 von Karman, 1 Gaussian, 2 exponential.  noise[N] is the
+   uniform noise the realisation is made of (the caller's random stream, as
+   the reference draws it inside the routine); out[N] receives the field:
+   FFT(noise) whitened to unit modulus, times sqrt(PSDF(k) N prod(d)), DC
+   zeroed, inverse FFT / N, demeaned and scaled to rms epsil.  Transforms in
+   FP32 (hipFFT), PSDF and mean/rms in FP64.  Host buffers; N < 2^31;
+   allocates its own device memory and stream and synchronises before
+   returning. */
+int swpc_rmedia_generate(int nd, const int *shape, const double *d, const double *a, double epsil, double kappa,
+                         int ptype, const float *noise, float *out);
+
 /* ------------------------------------------------------------------ */
 /* 3-D (swpc_3d)                                                       */
 /* ------------------------------------------------------------------ */

@@ -39,6 +39,7 @@ _PROTOS = {
     "swpc_version": (I, []),
     "swpc_device_count": (I, []),
     "swpc_membw": (I, [C.c_longlong, I, PD]),
+    "swpc_rmedia_generate": (I, [I, PI_, PD, PD, D, D, I, PF, PF]),
     "swpc3d_create": (I, [C.POINTER(Desc3), C.POINTER(P)]),
     "swpc3d_destroy": (I, [P]),
     "swpc3d_set_medium": (I, [P] + [PF] * 10),

@@ -52,7 +52,7 @@ def build_library(verbose: bool = False) -> Path:
             print(out)
         objs.append(obj)
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", str(tmp), *map(str, objs)])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", str(tmp), *map(str, objs), "-lhipfft"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -79,8 +79,8 @@ def case3d(nx, ny, nz, dx, dt, nt, na=20, layers=LHM_EXAMPLE, seed=1, rmed=None,
     on a periodic tile and mapped by rmed_tile periodicity.  log (optional)
     receives a line after each setup stage.
 
-    device_medium=True leaves medium__setup to the GPU: ``medium`` is then
-    a RawMedium3D, and the receivers (``stations`` None, ``station_list``
+    device_medium=True leaves the random-medium synthesis and medium__setup
+    to the GPU: ``medium`` is then a RawMedium3D, and the receivers (``stations`` None, ``station_list``
     the Station list) are placed by ``open_case3d`` from the kfs/kob maps
     the device setup returns.
     """
@@ -92,7 +92,7 @@ def case3d(nx, ny, nz, dx, dt, nt, na=20, layers=LHM_EXAMPLE, seed=1, rmed=None,
     if rmed is not None:
         eps, a, kappa = rmed
         tile = tuple(min(t, n) for t, n in zip(rmed_tile, (nx, ny, nz)))
-        vol = rmedia.generate(tile, dx, a, eps, kappa, seed=seed)
+        vol = rmedia.generate(tile, dx, a, eps, kappa, seed=seed, device=device_medium)
         xi = md.rmed_tile(vol, -2, nx + 3, -2, ny + 3, -2, nz + 3)
         note("random medium")
     rho, lam, mu, qp, qs = md.layered3(g, layers, xi=xi)

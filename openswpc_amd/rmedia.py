@@ -34,15 +34,24 @@ def psdf_exp(m2a2, a3, epsil, ndim=3):
     return 2 * math.pi * epsil ** 2 * a3 / (1 + m2a2) ** 1.5
 
 
-def generate(shape, d, a, epsil, kappa=0.5, ptype="vonkarman", seed=0):
+PTYPES = ("vonkarman", "gauss", "exp")
+
+
+def generate(shape, d, a, epsil, kappa=0.5, ptype="vonkarman", seed=0, device=False):
     """Periodic random field of the given shape (nx, ny[, nz]) with grid
-    widths d and correlation lengths a (per axis), rms epsil; float32."""
+    widths d and correlation lengths a (per axis), rms epsil; float32.
+
+    ``device=True`` runs the transforms, the filter and the scaling on the
+    GPU (swpc_rmedia_generate: hipFFT in FP32, the PSDF and the reductions
+    in FP64) on the same noise; it raises if the HIP library is absent."""
     shape = tuple(int(n) for n in shape)
     nd = len(shape)
     d = tuple(float(x) for x in (d if np.iterable(d) else [d] * nd))
     a = tuple(float(x) for x in (a if np.iterable(a) else [a] * nd))
     rng = np.random.default_rng(seed)
     noise = rng.random(shape)
+    if device:
+        return _generate_device(shape, d, a, epsil, kappa, ptype, noise)
     B = np.fft.rfftn(noise)
     mag = np.abs(B)
     mag[mag == 0] = 1.0
@@ -70,3 +79,20 @@ def generate(shape, d, a, epsil, kappa=0.5, ptype="vonkarman", seed=0):
     rms = math.sqrt(float(np.mean(f * f)))
     f *= epsil / rms
     return f.astype(np.float32)
+
+
+def _generate_device(shape, d, a, epsil, kappa, ptype, noise):
+    import ctypes as C
+
+    from ._lib import PD, PF, PI_, check, lib, ptr
+    if ptype not in PTYPES:
+        raise ValueError(ptype)
+    nd = len(shape)
+    sh = np.asarray(shape, np.int32)
+    dd, aa = np.asarray(d, np.float64), np.asarray(a, np.float64)
+    nz = np.ascontiguousarray(noise, np.float32)
+    out = np.empty(shape, np.float32)
+    check(lib().swpc_rmedia_generate(nd, ptr(sh, C.c_int), ptr(dd, C.c_double), ptr(aa, C.c_double), float(epsil),
+                                     float(kappa), PTYPES.index(ptype), ptr(nz, C.c_float), ptr(out, C.c_float)),
+          "swpc_rmedia_generate")
+    return out
