@@ -92,10 +92,17 @@ class Setup:
 # parameters
 # ---------------------------------------------------------------------------
 def _unsupported(ini, dim):
-    checks = [("pw_mode", False, "plane-wave mode (pw_mode)"), ("green_mode", False, "Green's function mode"),
+    checks = [("pw_mode", False, "plane-wave mode (pw_mode)"),
               ("stabilize_pml", False, "stabilize_pml"), ("earth_flattening", False, "earth_flattening"),
               ("sw_wav_stress", False, "stress waveforms (sw_wav_stress)"),
               ("sw_wav_strain", False, "strain waveforms (sw_wav_strain)")]
+    if ini.get("green_mode", False):
+        # wav__setup empties the station list in green_mode (m_wav.f90:66-73)
+        # before green__setup looks the pseudo-source station up in it
+        # (m_green.f90:159, m_wav.f90:643), so the reference stops at
+        # assert(src_somewhere) (m_green.f90:163) for every input
+        raise InputError("Green's function mode (green_mode): the reference stops in green__setup for every input "
+                         "(the station list is empty when the pseudo-source is looked up); not supported")
     for key, dflt, what in checks:
         if ini.get(key, dflt):
             raise InputError(f"{what} is not supported by this host")
