@@ -56,9 +56,10 @@ def main():
     ap.add_argument("--no-chosen", action="store_true")
     ap.add_argument("--misfit-steps", type=int, default=0,
                     help="steps of the misfit runs (0: the full warmup + timed length)")
-    ap.add_argument("--config", choices=("c2", "c3"), default="c2",
+    ap.add_argument("--config", choices=("c2", "c3", "sh"), default="c2",
                     help="c2: the 512x512x256 bench line (default); c3: the 1024x1024x512 reduced-storage "
-                         "run against an FP64 run of the same scheme (one JSON line of its own)")
+                         "run against an FP64 run of the same scheme (one JSON line of its own); sh: the "
+                         "8192x4096 SH (swpc_sh) step, FP32 and FP64 (one JSON line of its own)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -71,6 +72,8 @@ def main():
 
     if args.config == "c3":
         return run_c3(args)
+    if args.config == "sh":
+        return run_sh(args)
     nt = args.warmup + args.steps
     t0 = time.time()
     case = problem.case3d(args.nx, args.ny, args.nz, 0.25, 0.01, nt, na=20, seed=args.seed,
@@ -321,6 +324,47 @@ def run_c3(args, nt=4000, plain_steps=300, chunk=250):
     out["misfit_r16_vs_fp64"] = float(np.linalg.norm(traces[CHOSEN] - ref) / np.linalg.norm(ref))
     out["speedup_r16_vs_fp32"] = out["fp32"]["ms_per_step"] / out[CHOSEN]["ms_per_step"]
     log(f"c3: {out}")
+    print(json.dumps(out), flush=True)
+
+
+# Algorithmic HBM bytes per interior SH cell and step, FP32 words
+# (src/swpc_sh/m_kernel.f90): stress reads Vy, mu, taus, 6 memory variables,
+# Syz, Sxy and writes the 6 and the 2 stresses (19 words); velocity reads
+# Syz, Sxy, rho, Vy and writes Vy (5 words).
+SH_CELL_WORDS = 19 + 5
+
+
+def run_sh(args, nx=8192, nz=4096, chunk=100):
+    """The SH core's step time on an 8192 x 4096 grid (dx = 0.1 km,
+    dt = 5 ms, na = 20, NM = 3, layered medium, one double couple, 16
+    receivers) in FP32 and FP64, steps replayed from captured graphs."""
+    from openswpc_amd import problem
+    from openswpc_amd.solver import SwpcSH
+    log = lambda m: print(m, file=sys.stderr, flush=True)   # noqa: E731
+    steps = min(args.steps, 1000)
+    nt = args.warmup + steps
+    case = problem.case_sh(nx, nz, 0.1, 0.005, nt, na=20, seed=args.seed, src_depth=10.0, nst=16, zbeg=-1.0)
+    interior = (nx - 2 * case.grid.na) * (nz - case.grid.na)
+    out = {"config": {"workload": f"swpc_sh {nx}x{nz} dx=0.1km dt=5ms na=20 NM=3, layered, 1 double couple, "
+                                  "16 receivers", "timed_steps": steps}}
+    for prec in ("fp32", "fp64"):
+        with SwpcSH(case.grid, case.medium, case.pml, case.glut, case.stations, case.ntdec_w, nt,
+                    precision=prec) as sim:
+            sim.run(1, args.warmup)
+            sim.sync()
+            t1 = time.perf_counter()
+            for it in range(args.warmup + 1, nt + 1, chunk):
+                sim.run(it, min(chunk, nt + 1 - it))
+            sim.sync()
+            dt = (time.perf_counter() - t1) / steps
+            word = 4 if prec == "fp32" else 8
+            # FP64 fields; the medium stays FP32 (mu, taus, rho: 3 words)
+            nbytes = SH_CELL_WORDS * word - (3 * 4 if prec == "fp64" else 0)
+            out[prec] = {"ms_per_step": dt * 1e3, "cell_updates_per_s": nx * nz / dt,
+                         "interior_bytes_per_cell": nbytes,
+                         "interior_algorithmic_GBs": interior * nbytes / dt / 1e9,
+                         "hbm_peak_frac": interior * nbytes / dt / 8e12}
+            log(f"sh {prec}: {out[prec]}")
     print(json.dumps(out), flush=True)
 
 
