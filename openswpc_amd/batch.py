@@ -68,7 +68,20 @@ def mixed_cases(seed=1, n2d=24, n3d=24, nt=2000, side2=(256, 1024), side3=(96, 2
     layered model), each with a von Karman perturbation (eps, a = 2 km,
     kappa = 0.5), a double couple at a random position inside the
     interior and a random mechanism, and a few surface receivers.  Yields
-    ('2d'|'3d', case) in the seed's order, built lazily."""
+    ('2d'|'3d', case) in the seed's order, built lazily; the 3-D cases
+    leave medium__setup to the GPU (problem.case3d(device_medium=True))."""
+    for kind, kw in mixed_specs(seed, n2d, n3d, nt, side2, side3, na, eps):
+        yield kind, build_case(kind, kw)
+
+
+def build_case(kind, kw):
+    """One member of mixed_specs; kw may override device_medium (False:
+    the 3-D medium assembled on the host, as the oracle needs it)."""
+    return problem.case2d(**kw) if kind == "2d" else problem.case3d(**kw)
+
+
+def mixed_specs(seed=1, n2d=24, n3d=24, nt=2000, side2=(256, 1024), side3=(96, 256), na=20, eps=0.05):
+    """The (kind, case-builder arguments) list behind mixed_cases."""
     rng = np.random.default_rng(seed)
     specs = []
     for q in range(n2d + n3d):
@@ -90,8 +103,7 @@ def mixed_cases(seed=1, n2d=24, n3d=24, nt=2000, side2=(256, 1024), side3=(96, 2
                                      rmed=(eps, 2.0, 0.5), src_depth=depth, nst_side=2, zbeg=-0.75,
                                      rmed_tile=(64, 64, 64), device_medium=True,
                                      src_xy=(float(rng.uniform(-hx, hx)), float(rng.uniform(-hy, hy))))))
-    for kind, kw in specs:
-        yield kind, (problem.case2d(**kw) if kind == "2d" else problem.case3d(**kw))
+    return specs
 
 
 def open_case(kind, c, precision="fp32"):

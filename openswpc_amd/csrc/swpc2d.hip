@@ -18,6 +18,13 @@
 
 namespace swpc {
 
+// Every P-SV kernel rounds operation by operation in the reference's
+// expression order (no contraction into FMA), so that a run matches the
+// reference loops evaluated in IEEE single precision bit for bit.  The
+// P-SV step is bound by launch gaps and its state sits in the Infinity
+// Cache: the extra multiplies and adds cost nothing measurable.
+#pragma clang fp contract(off)
+
 constexpr int NM2 = 3;
 
 struct Box2 {
