@@ -360,6 +360,17 @@ int swpc_lab_train(swpc_lab_trainer *tr, int net, float *params, int steps, int 
    gradient grad[swpc_lab_nparam()], computed by the training kernels. */
 int swpc_lab_grad(swpc_lab_trainer *tr, const float *params, int nout, int n, const float *rec, double *loss,
                   float *grad);
+/* No counterpart in the reference.  For callers that cannot allocate
+   device memory (code under stream capture, tight serving or training
+   loops, harnesses that forbid hipMalloc): the loss and gradient of
+   swpc_lab_grad over n <= SWPC_LAB_GRAD_NOALLOC_MAX host records with no
+   trainer and no allocation.  Its staging and partial sums live in a
+   fixed device array of about 6.5 MB that each GPU holds until the process
+   ends; larger n is rejected with SWPC_EINVAL.  Work runs on the current
+   device's null stream and the call returns when it is done; calls that
+   overlap on one GPU share the array and race. */
+#define SWPC_LAB_GRAD_NOALLOC_MAX 16384
+int swpc_lab_grad_noalloc(const float *params, int nout, int n, const float *rec, double *loss, float *grad);
 
 /* ------------------------------------------------------------------ */
 /* P-SV (swpc_psv)                                                     */

@@ -85,6 +85,7 @@ _PROTOS = {
     "swpc_lab_samples": (I, [P, I, I, I, PF]),
     "swpc_lab_train": (I, [P, I, PF, I, I, C.c_float, C.c_float, C.c_uint, PF]),
     "swpc_lab_grad": (I, [P, PF, I, I, PF, PD, PF]),
+    "swpc_lab_grad_noalloc": (I, [PF, I, I, PF, PD, PF]),
     "swpc_batch_create": (I, [C.POINTER(P)]),
     "swpc_batch_destroy": (I, [P]),
     "swpc_batch_add3d": (I, [P, P]),

@@ -22,6 +22,8 @@
 // latency, not arithmetic.
 #include "swpc_lab.h"
 
+#include <mutex>
+
 #include <hip/hip_bf16.h>
 
 #include <algorithm>
@@ -1101,6 +1103,101 @@ int swpc_lab_grad(swpc_lab_trainer *tr, const float *params, int nout, int n, co
     float l;
     SWPC_HIP(hipMemcpyAsync(grad, tr->M.p, LAB_NP * sizeof(float), hipMemcpyDeviceToHost, s));
     SWPC_HIP(hipMemcpyAsync(&l, tr->loss.p, sizeof(float), hipMemcpyDeviceToHost, s));
+    SWPC_HIP(hipStreamSynchronize(s));
+    if (loss) *loss = l;
+    return SWPC_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// The same loss and gradient with no device allocation: the parameters,
+// the records, the per-workgroup partial sums and the outputs are staged
+// in `scratch`, a file-scope device array that each GPU holds for the life
+// of the process.  On the host the array's name is not a device address:
+// its address on each device comes from hipGetSymbolAddress, taken once per
+// device at that device's first call, and only that pointer is handed to
+// the copies and kernels.  Every call on one GPU shares the array; calls
+// that overlap on one GPU race.  Nothing in it needs to start at zero: the
+// kernels write every partial sum and output they later read.
+// ---------------------------------------------------------------------------
+namespace swpc {
+
+constexpr int NOALLOC_MAXBLK = SWPC_LAB_GRAD_NOALLOC_MAX / TB;
+constexpr size_t NA_P = 0;                                   // parameters [LAB_NP]
+constexpr size_t NA_G = NA_P + 3584;                         // gradient [LAB_NP]
+constexpr size_t NA_LOSS = NA_G + 3584;                      // loss [1]
+constexpr size_t NA_PLOSS = NA_LOSS + 64;                    // per-workgroup loss [MAXBLK]
+constexpr size_t NA_PART = NA_PLOSS + NOALLOC_MAXBLK;        // partials [MAXBLK][LAB_NP]
+constexpr size_t NA_X = NA_PART + (size_t)NOALLOC_MAXBLK * LAB_NP;   // records [MAX][LAB_REC]
+constexpr size_t NA_TOTAL = NA_X + (size_t)SWPC_LAB_GRAD_NOALLOC_MAX * LAB_REC;
+static_assert(LAB_NP <= 3584, "parameter block larger than its scratch slot");
+static_assert(SWPC_LAB_GRAD_NOALLOC_MAX % TB == 0, "record limit must be a multiple of the workgroup size");
+static_assert(NA_PART % 64 == 0 && NA_X % 64 == 0, "scratch slots keep 256-byte alignment");
+
+static __device__ __align__(256) float scratch[NA_TOTAL];
+
+constexpr int NOALLOC_MAXDEV = 64;
+static std::mutex g_scratch_mu;
+static float *g_scratch_ptr[NOALLOC_MAXDEV];
+
+// The scratch array's device address on the current device (looked up once
+// per device, with that device current).
+static int scratch_ptr(float **out)
+{
+    int dev = -1;
+    SWPC_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAXDEV) return fail(SWPC_EINVAL, "device %d beyond the %d the library tracks", dev,
+                                                      NOALLOC_MAXDEV);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(SWPC_EHIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_ptr[dev] = (float *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SWPC_OK;
+}
+
+}  // namespace swpc
+
+extern "C" {
+
+int swpc_lab_grad_noalloc(const float *params, int nout, int n, const float *rec, double *loss, float *grad)
+{
+    clear_error();
+    if (!params || !rec || !grad || n <= 0 || nout < 1 || nout > LAB_NOUT) return fail(SWPC_EINVAL, "bad argument");
+    if (n > SWPC_LAB_GRAD_NOALLOC_MAX)
+        return fail(SWPC_EINVAL, "swpc_lab_grad_noalloc: %d records exceed its fixed scratch of %d records "
+                                 "(use swpc_lab_grad)", n, SWPC_LAB_GRAD_NOALLOC_MAX);
+    float *base = nullptr;
+    if (int r = scratch_ptr(&base)) return r;
+    const int nblk = (n + TB - 1) / TB;
+    hipStream_t s = nullptr;   // the device's null stream
+    SWPC_HIP(hipMemcpyAsync(base + NA_P, params, LAB_NP * sizeof(float), hipMemcpyHostToDevice, s));
+    SWPC_HIP(hipMemcpyAsync(base + NA_X, rec, (size_t)n * LAB_REC * sizeof(float), hipMemcpyHostToDevice, s));
+    FwdBwdArgs a;
+    a.P = base + NA_P;
+    a.samples = base + NA_X;
+    a.count = n;
+    a.nout = nout;
+    a.mode = 1;
+    a.seed = 0;
+    a.noise = 0.0f;
+    a.scale = 2.0f / ((float)n * (float)nout);
+    a.partial = base + NA_PART;
+    a.ploss = base + NA_PLOSS;
+    hipLaunchKernelGGL(k_lab_fwdbwd, dim3(nblk), dim3(TB), 0, s, a);
+    SWPC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_lab_adam, dim3((LAB_NP + 255) / 256), dim3(256), 0, s, base + NA_P, (float *)nullptr,
+                       (float *)nullptr, (const float *)(base + NA_PART), (const float *)(base + NA_PLOSS), nblk, 0.0f,
+                       0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f / ((float)n * (float)nout), base + NA_LOSS, base + NA_G);
+    SWPC_CHECK_LAUNCH();
+    float l;
+    SWPC_HIP(hipMemcpyAsync(grad, base + NA_G, LAB_NP * sizeof(float), hipMemcpyDeviceToHost, s));
+    SWPC_HIP(hipMemcpyAsync(&l, base + NA_LOSS, sizeof(float), hipMemcpyDeviceToHost, s));
     SWPC_HIP(hipStreamSynchronize(s));
     if (loss) *loss = l;
     return SWPC_OK;
