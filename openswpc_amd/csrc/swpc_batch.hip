@@ -138,13 +138,30 @@ static int batch_add(swpc_batch *b, swpc3d_ctx *c3, swpc2d_ctx *c2)
     x.c3 = c3;
     x.c2 = c2;
     if (int r = member_info(x)) return r;
-    SWPC_HIP(hipEventCreateWithFlags(&x.ev, hipEventDisableTiming));
-    b->mem.push_back(x);
-    batch_drop(b);
+    // the step-counter table with the new member goes into a fresh buffer
+    // first; the batch takes the member only once the table is on the device
     std::vector<int *> d;
     for (auto &y : b->mem) d.push_back(y.m.dit);
-    if (int r = dalloc(b->dits, d.size() * sizeof(int *), b->bytes)) return r;
-    SWPC_HIP(hipMemcpy(b->dits.p, d.data(), d.size() * sizeof(int *), hipMemcpyHostToDevice));
+    d.push_back(x.m.dit);
+    DBuf nd;
+    long long nbytes = 0;
+    if (int r = dalloc(nd, d.size() * sizeof(int *), nbytes)) {
+        dfree(nd);
+        return r;
+    }
+    hipError_t e = hipMemcpy(nd.p, d.data(), d.size() * sizeof(int *), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&x.ev, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        dfree(nd);
+        return fail(SWPC_EHIP, "batch_add: %s", hipGetErrorString(e));
+    }
+    batch_drop(b);
+    b->bytes -= (long long)b->dits.bytes;
+    dfree(b->dits);
+    b->dits = nd;
+    nd = DBuf();
+    b->bytes += nbytes;
+    b->mem.push_back(x);
     return SWPC_OK;
 }
 
@@ -156,6 +173,7 @@ int swpc_batch_run(swpc_batch *b, int it0, int nsteps, int graph_k)
     clear_error();
     if (!b || nsteps < 0 || graph_k < 0) return fail(SWPC_EINVAL, "bad argument");
     if (b->mem.empty() || nsteps == 0) return SWPC_OK;
+    if (!b->dits.p) return fail(SWPC_EINVAL, "batch has no step-counter table (a failed add?)");
     bool stale = false;
     for (auto &x : b->mem) {
         if (int r = member_info(x)) return r;

@@ -17,6 +17,11 @@
 
 namespace swpc {
 
+// Every SH kernel rounds operation by operation in the reference's
+// expression order (no contraction into FMA), so that a run matches the
+// reference loops evaluated in IEEE single precision bit for bit.
+#pragma clang fp contract(off)
+
 constexpr int NMS = 3;
 
 struct BoxS {
@@ -170,10 +175,14 @@ __global__ void __launch_bounds__(256) ksh_vel_pml(const KS<T> a)
     const int px = a.px, nx = a.nx, nz = a.nz;
     const int c = at2(px, i, k);
     const T gxc1 = a.gx[0 * nx + i - 1], gxc2 = a.gx[1 * nx + i - 1], gxc3 = a.gx[2 * nx + i - 1], gxc4 = a.gx[3 * nx + i - 1];
-    const T gzc1 = a.gz[0 * nz + k - 1], gzc2 = a.gz[1 * nz + k - 1], gzc3 = a.gz[2 * nz + k - 1], gzc4 = a.gz[3 * nz + k - 1];
+    // src/swpc_sh/m_absorb_p.f90:174-187: the velocity update takes
+    // gzc(1:2, i), the ADE update gzc(3:4, k).  Where nx > nz the index i
+    // runs past the profile's bounds (no defined result); there the
+    // profile is taken at k.
+    const int kz = nx <= nz ? i : k;
+    const T gzc1 = a.gz[0 * nz + kz - 1], gzc2 = a.gz[1 * nz + kz - 1];
+    const T gzc3 = a.gz[2 * nz + k - 1], gzc4 = a.gz[3 * nz + k - 1];
     const T r20x = a.r2[0], r20z = a.r2[1], dt = a.dt;
-    // src/swpc_sh/m_absorb_p.f90:174-187, with the z profile taken at k
-    // (the reference indexes gzc by i there, past its bounds when nx > nz)
     const T dzSyz = (a.Syz[c] - a.Syz[c - px]) * r20z;
     const T dxSxy = (a.Sxy[c] - a.Sxy[c - 1]) * r20x;
     const float by = 1.0f / a.rho[c];

@@ -877,8 +877,11 @@ void osh_absorb_vel(osh_t *s)
             REAL dxSxy = (BS(Sxy, 0, 0) - BS(Sxy, 0, -1)) * r20x;
             size_t c = J2(s, k, i);
             float by = 1.0f / s->rho[c];
-            s->Vy[c] = s->Vy[c] + by * (G(s->gxc, 1, i) * dxSxy + G(s->gzc, 1, k) * dzSyz
-                                       + G(s->gxc, 2, i) * axSxy[c] + G(s->gzc, 2, k) * azSyz[c]) * dt;
+            /* src/swpc_sh/m_absorb_p.f90:180-181 take gzc(1:2, i); past the
+               profile's bounds (nx > nz) the profile is taken at k */
+            const int kz = s->nx <= s->nz ? i : k;
+            s->Vy[c] = s->Vy[c] + by * (G(s->gxc, 1, i) * dxSxy + G(s->gzc, 1, kz) * dzSyz
+                                       + G(s->gxc, 2, i) * axSxy[c] + G(s->gzc, 2, kz) * azSyz[c]) * dt;
             axSxy[c] = G(s->gxc, 3, i) * axSxy[c] + G(s->gxc, 4, i) * dxSxy * dt;
             azSyz[c] = G(s->gzc, 3, k) * azSyz[c] + G(s->gzc, 4, k) * dzSyz * dt;
         }

@@ -48,6 +48,30 @@ def test_sh_matches_oracle(precision, bodyforce):
     assert vmax == pytest.approx(o.vmax(), rel=1e-5 if bits == 32 else 1e-12)
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_sh_absorber_reference_z_index_where_defined(precision):
+    """nx <= nz: the absorber's velocity update takes the z profile at i, as
+    the reference does where that index stays inside the profile
+    (src/swpc_sh/m_absorb_p.f90:180-181); 500 steps, so the waves cross
+    the side and bottom absorbers."""
+    from oracle.oracle import OracleSH
+    c = _case(nt=500, nx=117, nz=203)
+    bits = 64 if precision == "fp64" else 32
+    o = OracleSH(c, bits)
+    o.run(1, c.nt)
+    with _open(c, precision) as sim:
+        sim.run(1, c.nt)
+        vel, _ = sim.receivers()
+        vy, az = sim.field("Vy"), sim.field("azSyz")
+    tol = 1e-11 if bits == 64 else 1e-5
+    e = np.linalg.norm(vel - o.vel) / np.linalg.norm(o.vel)
+    ev = np.linalg.norm(vy - o.field("Vy")) / np.linalg.norm(o.field("Vy"))
+    ea = np.linalg.norm(az - o.field("azSyz")) / np.linalg.norm(o.field("azSyz"))
+    print(f"{precision} 117x203: traces {e:.2e}, Vy {ev:.2e}, azSyz {ea:.2e}")
+    assert np.abs(o.field("azSyz")).max() > 0
+    assert e <= tol and ev <= tol and ea <= tol
+
+
 def test_sh_graph_replay_equals_step_by_step():
     c = _case(nt=100, nx=150, nz=90)
     out = []

@@ -133,3 +133,106 @@ def test_sh_case_runs_finite():
     o.run(1, 200)
     assert np.isfinite(o.vel).all() and np.abs(o.vel).max() > 0
     assert o.vmax() > 0
+
+
+def _sh_pml_case(nx, nz, na=5, dx=0.5, dt=0.04):
+    g = md.Grid2D(nx, nz, dx, dx, dt, na=na, zbeg=-dx * 3)
+    rho, lam, mu, qp, qs = md.layered2(g, [md.Layer(-1000.0, 2.7, 6.0, 3.5, 1e30, 1e30)])
+    med = md.medium_setup2(g, rho, lam, mu, qp, qs, benchmark=True)
+    return Case2D(g, med, pml.pml_sh(g, 1.0), stress_glut_sh(g, [], 2), np.zeros((0, 2), np.int32), [], 2, 1,
+                  (0, 0, 0), None)
+
+
+def _sh_box(g):
+    return np.arange(-2, g.nx + 4)[:, None], np.arange(-2, g.nz + 4)[None, :]
+
+
+def _sh_prof(g, prof, axis, q, at=None):
+    """Profile column q over the (i, k) box along `axis`, or, with at =
+    'i', the z profile indexed by i (broadcast along k)."""
+    n = g.nz if (axis == 1 or at == "i") else g.nx
+    v = np.full(n + 6, np.nan)
+    v[3:3 + n] = prof[:, q - 1].astype(np.float64)
+    if at == "i":
+        w = np.full(g.nx + 6, np.nan)
+        m = min(g.nx, g.nz)
+        w[3:3 + m] = v[3:3 + m]
+        return w.reshape(-1, 1)
+    return v.reshape((n + 6, 1) if axis == 0 else (1, n + 6))
+
+
+def _sh_mask(g):
+    i, k = _sh_box(g)
+    inside = (i >= 1) & (i <= g.nx) & (k >= 1) & (k <= g.nz)
+    kbeg = np.where((i <= g.na) | (i >= g.nx - g.na + 1), 1, g.nz - g.na + 1)
+    return inside & (k >= kbeg)
+
+
+@pytest.mark.parametrize("nx, nz", [(24, 30), (30, 24)])
+def test_sh_pml_velocity_closed_form(nx, nz):
+    """Linear Sxy, Syz: the ADE values follow a_n = g4 c dt (1 + ... +
+    g3^(n-1)) with gxc and gzc at k, and Vy advances by (g1 c + g2
+    a_(n-1)) dt / rho per direction.  The z term of the velocity takes
+    gzc(1:2, i) (src/swpc_sh/m_absorb_p.f90:180-181) where nx <= nz, the
+    profile's bounds; with nx > nz that index runs past them and the
+    profile at k is used."""
+    case = _sh_pml_case(nx, nz)
+    g = case.grid
+    o = OracleSH(case, 64)
+    o.med["rho"][...] = 2.5
+    I, K = _sh_box(g)
+    cxy, czy = 1.3, -0.8        # dSxy/dx, dSyz/dz (per km)
+    o.set_field("Sxy", np.broadcast_to((cxy * I + 0.4 * K) * g.dx, g.shape))
+    o.set_field("Syz", np.broadcast_to((-0.6 * I + czy * K) * g.dx, g.shape))
+    gxc, gxe, gzc, gze = case.pml
+    X = {q: _sh_prof(g, gxc, 0, q) for q in (1, 2, 3, 4)}
+    Z = {q: _sh_prof(g, gzc, 1, q) for q in (1, 2, 3, 4)}
+    Zv = {q: _sh_prof(g, gzc, 1, q, at="i") if nx <= nz else Z[q] for q in (1, 2)}
+    dt = float(np.float32(g.dt))
+    mask = _sh_mask(g)
+    ax = az = 0.0
+    v = np.zeros(g.shape)
+    for step in range(1, 5):
+        o.L.osh_absorb_vel(o._ps())
+        v = v + float(np.float32(1) / np.float32(2.5)) * (X[1] * cxy + Zv[1] * czy + X[2] * ax + Zv[2] * az) * dt
+        ax = X[4] * cxy * dt * sum(X[3] ** m for m in range(step))
+        az = Z[4] * czy * dt * sum(Z[3] ** m for m in range(step))
+        e = np.broadcast_to(v, g.shape)
+        assert np.allclose(o.f["Vy"][mask], e[mask], rtol=1e-11, atol=1e-13), step
+        assert np.all(o.f["Vy"][~mask] == 0)
+        assert np.allclose(o.A["axSxy"][mask], np.broadcast_to(ax, g.shape)[mask], rtol=1e-11, atol=1e-15)
+        assert np.allclose(o.A["azSyz"][mask], np.broadcast_to(az, g.shape)[mask], rtol=1e-11, atol=1e-15)
+
+
+def test_sh_pml_stress_closed_form():
+    """Linear Vy: the SH absorber advances its ADE values first and the
+    stresses use the advanced ones (src/swpc_sh/m_absorb_p.f90:262-275):
+    Sxy += mu_xy (gxe1 c + gxe2 a_n) dt with a_n = gxe4 c dt (1 + ... +
+    gxe3^(n-1)), Syz likewise with gze; mu is uniform, so both harmonic
+    means are 2 mu^2 / (2 mu + eps) in FP32."""
+    case = _sh_pml_case(24, 30)
+    g = case.grid
+    o = OracleSH(case, 64)
+    mu = 2.5
+    o.med["mu"][...] = mu
+    hm = float(np.float32(2 * mu * mu) / np.float32(2 * mu + np.float32(1.1920928955078125e-07)))
+    I, K = _sh_box(g)
+    cx, cz = 0.9, -1.7
+    o.set_field("Vy", np.broadcast_to((cx * I + cz * K) * g.dx, g.shape))
+    gxc, gxe, gzc, gze = case.pml
+    X = {q: _sh_prof(g, gxe, 0, q) for q in (1, 2, 3, 4)}
+    Z = {q: _sh_prof(g, gze, 1, q) for q in (1, 2, 3, 4)}
+    dt = float(np.float32(g.dt))
+    mask = _sh_mask(g)
+    sxy = np.zeros(g.shape)
+    syz = np.zeros(g.shape)
+    for step in range(1, 5):
+        o.L.osh_absorb_stress(o._ps())
+        ax = X[4] * cx * dt * sum(X[3] ** m for m in range(step))
+        az = Z[4] * cz * dt * sum(Z[3] ** m for m in range(step))
+        sxy = sxy + hm * (X[1] * cx + X[2] * ax) * dt
+        syz = syz + hm * (Z[1] * cz + Z[2] * az) * dt
+        for name, e in (("Sxy", sxy), ("Syz", syz)):
+            e = np.broadcast_to(e, g.shape)
+            assert np.allclose(o.f[name][mask], e[mask], rtol=1e-6, atol=1e-13), (name, step)
+            assert np.all(o.f[name][~mask] == 0)
