@@ -2661,12 +2661,15 @@ int swpc3d_create(const swpc3d_desc *desc, swpc3d_ctx **out)
         x.start = B.total;
         B.total += (unsigned)ni * (unsigned)nj * (unsigned)nk;
     };
+    // The y faces and the bottom take whole rows i in [1, nx] (their rows
+    // stream as aligned 128-byte lines); the x faces keep only the rows of
+    // the interior, na cells each.
     if (d.na > 0) {
-        add(1, d.na, 1, d.ny, 1, d.nz);
-        add(d.nx - d.na + 1, d.nx, 1, d.ny, 1, d.nz);
-        add(d.na + 1, d.nx - d.na, 1, d.na, 1, d.nz);
-        add(d.na + 1, d.nx - d.na, d.ny - d.na + 1, d.ny, 1, d.nz);
-        add(d.na + 1, d.nx - d.na, d.na + 1, d.ny - d.na, d.nz - d.na + 1, d.nz);
+        add(1, d.nx, 1, d.na, 1, d.nz);
+        add(1, d.nx, d.ny - d.na + 1, d.ny, 1, d.nz);
+        add(1, d.nx, d.na + 1, d.ny - d.na, d.nz - d.na + 1, d.nz);
+        add(1, d.na, d.na + 1, d.ny - d.na, 1, d.nz - d.na);
+        add(d.nx - d.na + 1, d.nx, d.na + 1, d.ny - d.na, 1, d.nz - d.na);
     }
     // the same boxes in groups of 4 x-cells for the 4-wide PML kernels
     Boxes4 &B4 = c->boxes4;
