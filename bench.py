@@ -51,7 +51,9 @@ def main():
     ap.add_argument("--ny", type=int, default=512)
     ap.add_argument("--nz", type=int, default=256)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--cpu-steps", type=int, default=60,
+                    help="oracle steps of the CPU baseline (after one untimed step); the GPU fields are compared "
+                         "with the oracle's after them, so the source (onset 0.1 s) must have started")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-chosen", action="store_true")
     ap.add_argument("--misfit-steps", type=int, default=600,
@@ -485,8 +487,17 @@ def cpu_baseline(case, steps):
     g = case.grid
     v = g.nx * g.ny * g.nz * steps / dt
     log(f"cpu baseline: {steps} steps in {dt:.2f} s on {nth} threads ({ncores} cores visible): {v / 1e6:.1f} M cell-updates/s")
+    # the same steps on the GPU: every field against the oracle's
+    from openswpc_amd.solver import Swpc3D
+    per = {}
+    with Swpc3D(g, case.medium, case.pml, case.glut, case.stations, case.ntdec_w, 1 + steps) as sim:
+        sim.run(1, 1 + steps)
+        for f in ("Vx", "Vy", "Vz", "Sxx", "Syy", "Szz", "Syz", "Sxz", "Sxy", "Rxx1", "Rxy3"):
+            per[f] = rel_misfit(sim.field(f), ref.field(f), f"field {f}")
+    log(f"GPU vs oracle after {1 + steps} steps: max field rel-L2 {max(per.values()):.2e}")
     return {"value": v, "unit": "cell-updates/s", "cores": int(nth), "kind": "port",
-            "sample": f"{steps} full steps (after 1 untimed) of the same {g.nx}x{g.ny}x{g.nz} case, FP32 oracle"}
+            "sample": f"{steps} full steps (after 1 untimed) of the same {g.nx}x{g.ny}x{g.nz} case, FP32 oracle",
+            "gpu_vs_oracle": {"steps": 1 + steps, "max_field_rel_l2": max(per.values()), "fields_rel_l2": per}}
 
 
 if __name__ == "__main__":

@@ -41,13 +41,15 @@ module swpc_hip
     public :: swpc3d_snapshot_peaks, swpc3d_finish_snapshots, swpc3d_get_peaks
     public :: swpc3d_step, swpc3d_run, swpc3d_get_receivers, swpc3d_vmax, swpc3d_sync
     public :: swpc2d_create, swpc2d_destroy, swpc2d_set_medium, swpc2d_set_surface, swpc2d_set_pml
-    public :: swpc2d_set_sources, swpc2d_set_receivers, swpc2d_update_stress, swpc2d_update_vel
-    public :: swpc2d_stressglut, swpc2d_store_receivers, swpc2d_step, swpc2d_run, swpc2d_get_receivers
-    public :: swpc2d_vmax, swpc2d_sync
+    public :: swpc2d_set_sources, swpc2d_set_bodyforce, swpc2d_set_receivers, swpc2d_update_stress
+    public :: swpc2d_update_vel, swpc2d_stressglut, swpc2d_bodyforce, swpc2d_store_receivers
+    public :: swpc2d_add_snapshot, swpc2d_snapshots, swpc2d_get_snapshot
+    public :: swpc2d_step, swpc2d_set_graph, swpc2d_run, swpc2d_get_receivers, swpc2d_vmax, swpc2d_sync
     public :: swpcsh_create, swpcsh_destroy, swpcsh_set_medium, swpcsh_set_surface, swpcsh_set_pml
-    public :: swpcsh_set_sources, swpcsh_set_receivers, swpcsh_update_stress, swpcsh_update_vel
-    public :: swpcsh_stressglut, swpcsh_store_receivers, swpcsh_step, swpcsh_run, swpcsh_get_receivers
-    public :: swpcsh_vmax, swpcsh_sync
+    public :: swpcsh_set_sources, swpcsh_set_bodyforce, swpcsh_set_receivers, swpcsh_update_stress
+    public :: swpcsh_update_vel, swpcsh_stressglut, swpcsh_bodyforce, swpcsh_store_receivers
+    public :: swpcsh_add_snapshot, swpcsh_snapshots, swpcsh_get_snapshot
+    public :: swpcsh_step, swpcsh_set_graph, swpcsh_run, swpcsh_get_receivers, swpcsh_vmax, swpcsh_sync
     public :: swpc_lab_nparam, swpc3d_set_lab, swpc3d_set_lab_sponge
     public :: swpc_batch_create, swpc_batch_destroy, swpc_batch_add3d, swpc_batch_add2d, swpc_batch_run
     public :: swpc_batch_sync
@@ -316,6 +318,16 @@ module swpc_hip
             real(c_double), intent(in) :: m3(*), sdrop(*)
         end function
 
+        !! f2(2, nsrc) = (fx, fz)/M0; fdt(nsrc, nt) (source__bodyforce, src/swpc_psv/m_source.f90:616-654)
+        integer(c_int) function swpc2d_set_bodyforce(ctx, nsrc, isrc, ksrc, f2, nt, fdt) &
+            bind(C, name='swpc2d_set_bodyforce')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nsrc, nt
+            integer(c_int), intent(in) :: isrc(*), ksrc(*)
+            real(c_double), intent(in) :: f2(*), fdt(*)
+        end function
+
         integer(c_int) function swpc2d_set_receivers(ctx, nst, ist, kst, ntdec_w, ntw) &
             bind(C, name='swpc2d_set_receivers')
             import :: c_int, c_ptr
@@ -340,10 +352,47 @@ module swpc_hip
             integer(c_int), value :: it
         end function
 
+        integer(c_int) function swpc2d_bodyforce(ctx, it) bind(C, name='swpc2d_bodyforce')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it
+        end function
+
         integer(c_int) function swpc2d_store_receivers(ctx, it) bind(C, name='swpc2d_store_receivers')
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx
             integer(c_int), value :: it
+        end function
+
+        !! ik(2, npts) 1-based cells of the xz section (snap__write, src/swpc_psv/m_snap.f90)
+        integer(c_int) function swpc2d_add_snapshot(ctx, kind, npts, ik, ntdec_s, nbuf, uc, m0, dt, id) &
+            bind(C, name='swpc2d_add_snapshot')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: kind, npts, ntdec_s, nbuf
+            integer(c_int), intent(in) :: ik(*)
+            real(c_float), value :: uc, m0, dt
+            integer(c_int), intent(out) :: id
+        end function
+
+        integer(c_int) function swpc2d_snapshots(ctx, it) bind(C, name='swpc2d_snapshots')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it
+        end function
+
+        !! frame 0, 1, ... of stream id: out(npts, 2)
+        integer(c_int) function swpc2d_get_snapshot(ctx, id, frame, out) bind(C, name='swpc2d_get_snapshot')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: id, frame
+            real(c_float), intent(out) :: out(*)
+        end function
+
+        integer(c_int) function swpc2d_set_graph(ctx, steps) bind(C, name='swpc2d_set_graph')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: steps
         end function
 
         integer(c_int) function swpc2d_step(ctx, it) bind(C, name='swpc2d_step')
@@ -417,6 +466,16 @@ module swpc_hip
             real(c_double), intent(in) :: m2(*), sdrop(*)
         end function
 
+        !! fy(nsrc) = fy/M0; fdt(nsrc, nt) (source__bodyforce, src/swpc_sh/m_source.f90:596-629)
+        integer(c_int) function swpcsh_set_bodyforce(ctx, nsrc, isrc, ksrc, fy, nt, fdt) &
+            bind(C, name='swpcsh_set_bodyforce')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nsrc, nt
+            integer(c_int), intent(in) :: isrc(*), ksrc(*)
+            real(c_double), intent(in) :: fy(*), fdt(*)
+        end function
+
         integer(c_int) function swpcsh_set_receivers(ctx, nst, ist, kst, ntdec_w, ntw) &
             bind(C, name='swpcsh_set_receivers')
             import :: c_int, c_ptr
@@ -441,10 +500,47 @@ module swpc_hip
             integer(c_int), value :: it
         end function
 
+        integer(c_int) function swpcsh_bodyforce(ctx, it) bind(C, name='swpcsh_bodyforce')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it
+        end function
+
         integer(c_int) function swpcsh_store_receivers(ctx, it) bind(C, name='swpcsh_store_receivers')
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx
             integer(c_int), value :: it
+        end function
+
+        !! ik(2, npts) 1-based cells; kind SWPC_SNAP_V (Vy) or SWPC_SNAP_U (Uy) (src/swpc_sh/m_snap.f90)
+        integer(c_int) function swpcsh_add_snapshot(ctx, kind, npts, ik, ntdec_s, nbuf, uc, m0, dt, id) &
+            bind(C, name='swpcsh_add_snapshot')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: kind, npts, ntdec_s, nbuf
+            integer(c_int), intent(in) :: ik(*)
+            real(c_float), value :: uc, m0, dt
+            integer(c_int), intent(out) :: id
+        end function
+
+        integer(c_int) function swpcsh_snapshots(ctx, it) bind(C, name='swpcsh_snapshots')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: it
+        end function
+
+        !! frame 0, 1, ... of stream id: out(npts)
+        integer(c_int) function swpcsh_get_snapshot(ctx, id, frame, out) bind(C, name='swpcsh_get_snapshot')
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: id, frame
+            real(c_float), intent(out) :: out(*)
+        end function
+
+        integer(c_int) function swpcsh_set_graph(ctx, steps) bind(C, name='swpcsh_set_graph')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: steps
         end function
 
         integer(c_int) function swpcsh_step(ctx, it) bind(C, name='swpcsh_step')

@@ -468,6 +468,15 @@ int swpcsh_update_vel(swpcsh_ctx *ctx);
 int swpcsh_stressglut(swpcsh_ctx *ctx, int it);
 int swpcsh_bodyforce(swpcsh_ctx *ctx, int it);
 int swpcsh_store_receivers(swpcsh_ctx *ctx, int it);
+/* SH snapshot streams (snap__write, src/swpc_sh/m_snap.f90:400-413, with
+   wbuf_xz_v and wbuf_xz_u, 415-520) over 1-based (i,k) cells ik[2*npts]:
+   kind SWPC_SNAP_V gives Vy x UC x M0, SWPC_SNAP_U the running sum of
+   Vy x UC x M0 x dt (advanced at every step); frames of 1 x npts floats.
+   Otherwise the conventions of swpc2d_add_snapshot. */
+int swpcsh_add_snapshot(swpcsh_ctx *ctx, int kind, int npts, const int *ik, int ntdec_s, int nbuf,
+                        float uc, float m0, float dt, int *id);
+int swpcsh_snapshots(swpcsh_ctx *ctx, int it);
+int swpcsh_get_snapshot(swpcsh_ctx *ctx, int id, int frame, float *out);
 int swpcsh_step(swpcsh_ctx *ctx, int it);
 int swpcsh_set_graph(swpcsh_ctx *ctx, int steps);
 int swpcsh_run(swpcsh_ctx *ctx, int it0, int nsteps);

@@ -105,6 +105,9 @@ _PROTOS = {
     "swpcsh_stressglut": (I, [P, I]),
     "swpcsh_bodyforce": (I, [P, I]),
     "swpcsh_store_receivers": (I, [P, I]),
+    "swpcsh_add_snapshot": (I, [P, I, I, PI_, I, I, C.c_float, C.c_float, C.c_float, PI_]),
+    "swpcsh_snapshots": (I, [P, I]),
+    "swpcsh_get_snapshot": (I, [P, I, I, PF]),
     "swpcsh_step": (I, [P, I]),
     "swpcsh_set_graph": (I, [P, I]),
     "swpcsh_run": (I, [P, I, I]),

@@ -59,25 +59,31 @@ def build_library(verbose: bool = False) -> Path:
 
 FLANG = os.environ.get("SWPC_FLANG", "/opt/rocm/lib/llvm/bin/flang")
 FDRIVE = PKG / "fhost" / "swpc3d_fdrive"
+FDRIVE2 = PKG / "fhost" / "swpc2d_fdrive"
 
 
 def build_fortran_host(verbose: bool = False):
-    """Fortran bindings + drop-in host; linked against libswpc_hip.so with
-    an rpath to the package directory.  Returns the executable, or None
-    where no Fortran compiler is installed."""
+    """Fortran bindings + the 3-D and P-SV drop-in hosts; linked against
+    libswpc_hip.so with an rpath to the package directory.  Returns the 3-D
+    executable, or None where no Fortran compiler is installed."""
     if not Path(FLANG).exists():
         return None
     mod = BUILD_DIR / "fmod"
     mod.mkdir(parents=True, exist_ok=True)
-    srcs = [ROOT / "include" / "swpc_hip.f90", PKG / "fhost" / "m_kernel_hip.f90", PKG / "fhost" / "swpc3d_fdrive.f90"]
-    objs = []
-    for src in srcs:
-        obj = mod / (src.stem + ".o")
-        _run([FLANG, "-O2", "-fPIC", "-module-dir", str(mod), "-I", str(mod), "-c", str(src), "-o", str(obj)])
-        objs.append(obj)
-    tmp = FDRIVE.with_name(FDRIVE.name + ".tmp")
-    _run([FLANG, "-o", str(tmp), *map(str, objs), f"-L{PKG}", "-lswpc_hip", "-Wl,-rpath,$ORIGIN/.."])
-    os.replace(tmp, FDRIVE)
+
+    def obj(src):
+        o = mod / (src.stem + ".o")
+        _run([FLANG, "-O2", "-fPIC", "-module-dir", str(mod), "-I", str(mod), "-c", str(src), "-o", str(o)])
+        return o
+
+    fh = PKG / "fhost"
+    binding = obj(ROOT / "include" / "swpc_hip.f90")
+    for exe, srcs in ((FDRIVE, ("m_kernel_hip.f90", "swpc3d_fdrive.f90")),
+                      (FDRIVE2, ("m_kernel_psv_hip.f90", "swpc2d_fdrive.f90"))):
+        objs = [binding] + [obj(fh / s) for s in srcs]
+        tmp = exe.with_name(exe.name + ".tmp")
+        _run([FLANG, "-o", str(tmp), *map(str, objs), f"-L{PKG}", "-lswpc_hip", "-Wl,-rpath,$ORIGIN/.."])
+        os.replace(tmp, exe)
     return FDRIVE
 
 

@@ -250,6 +250,32 @@ __global__ void ksh_receivers(const KS<T> a, int nst, const int *__restrict__ st
     }
 }
 
+// SH snapshot products at a list of (i,k) cells (src/swpc_sh/m_snap.f90):
+// kind 1 'v' follows wbuf_xz_v (Vy x UC x M0), kind 2 'u' wbuf_xz_u (the
+// real(SP) running sum of Vy x UC x M0 x dt, advanced at every call).  One
+// variable per frame, frames of npts floats in a ring of nbuf.
+template <typename T>
+__global__ void ksh_snap(const KS<T> a, int kind, int npts, const int2 *__restrict__ pts, float uc, float m0,
+                         float dt, float *__restrict__ acc, float *__restrict__ frames, int ntdec, int nbuf,
+                         const int *__restrict__ dit)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npts) return;
+    const int it = *dit;
+    const bool store = ((it - 1) % ntdec) == 0;
+    if (!store && kind != 2) return;
+    float *__restrict__ out = frames + (size_t)(((it - 1) / ntdec) % nbuf) * npts;
+    const int2 q = pts[p];
+    const T vy = a.Vy[at2(a.px, q.x, q.y)];
+    if (kind == 1) {
+        out[p] = (float)(vy * (T)uc * (T)m0);
+    } else {
+        const float uy = (float)((T)acc[p] + vy * (T)uc * (T)m0 * (T)dt);
+        acc[p] = uy;
+        if (store) out[p] = uy;
+    }
+}
+
 __global__ void ksh_step_set(int *dit, int it)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) *dit = it;
@@ -298,6 +324,13 @@ struct swpcsh_ctx {
     int graph_k = 16;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    struct Snap {
+        int kind, npts, ntdec, nbuf;
+        float uc, m0, dt;
+        swpc::DBuf pts, acc, frames;
+        long long last = -1;
+    };
+    std::vector<Snap> snaps;
 };
 
 namespace swpc {
@@ -417,6 +450,29 @@ static int lsh_recv(swpcsh_ctx *c)
     return SWPC_OK;
 }
 
+template <typename T>
+static int lsh_snap(swpcsh_ctx *c)
+{
+    if (c->snaps.empty()) return SWPC_OK;
+    const KS<T> a = make_args_sh<T>(c);
+    for (auto &sn : c->snaps) {
+        hipLaunchKernelGGL((ksh_snap<T>), dim3((sn.npts + 255) / 256), dim3(256), 0, c->stream, a, sn.kind,
+                           sn.npts, (const int2 *)sn.pts.p, sn.uc, sn.m0, sn.dt, (float *)sn.acc.p,
+                           (float *)sn.frames.p, sn.ntdec, sn.nbuf, (const int *)c->dit.p);
+        SWPC_CHECK_LAUNCH();
+    }
+    return SWPC_OK;
+}
+
+// the newest frame each stream holds after steps it0..it1
+static void note_snapshots_sh(swpcsh_ctx *c, int it0, int it1)
+{
+    for (auto &sn : c->snaps) {
+        const long long last_due = (long long)(it1 - 1) / sn.ntdec * sn.ntdec + 1;
+        if (last_due >= it0) sn.last = std::max(sn.last, (last_due - 1) / sn.ntdec);
+    }
+}
+
 static int set_step_sh(swpcsh_ctx *c, int it)
 {
     hipLaunchKernelGGL(ksh_step_set, dim3(1), dim3(64), 0, c->stream, (int *)c->dit.p, it);
@@ -424,13 +480,14 @@ static int set_step_sh(swpcsh_ctx *c, int it)
     return SWPC_OK;
 }
 
-// SH step order (src/swpc_sh/main.f90:97-114): receivers, stress,
-// absorber, stress glut, velocity, absorber, body force
+// SH step order (src/swpc_sh/main.f90:97-114): receivers, snapshots,
+// stress, absorber, stress glut, velocity, absorber, body force
 template <typename T>
 static int step_sh_dev(swpcsh_ctx *c)
 {
     int r;
     if ((r = lsh_recv<T>(c))) return r;
+    if ((r = lsh_snap<T>(c))) return r;
     if ((r = lsh_stress<T>(c))) return r;
     if ((r = lsh_glut<T>(c))) return r;
     if ((r = lsh_vel<T>(c))) return r;
@@ -616,6 +673,11 @@ int swpcsh_destroy(swpcsh_ctx *c)
     DBuf *rest[] = {&c->surf, &c->kob, &c->gx, &c->gz, &c->src, &c->m2, &c->sdrop, &c->bsrc, &c->fy,
                     &c->fdt, &c->st, &c->uacc, &c->wvel, &c->wdisp, &c->vmax, &c->dit};
     for (auto b : rest) dfree(*b);
+    for (auto &sn : c->snaps) {
+        dfree(sn.pts);
+        dfree(sn.acc);
+        dfree(sn.frames);
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SWPC_OK;
@@ -796,7 +858,9 @@ int swpcsh_step(swpcsh_ctx *c, int it)
     if (it < 1) return fail(SWPC_EINVAL, "it must be >= 1");
     if (int r = check_steps_sh(c, it, 1)) return r;
     if (int r = set_step_sh(c, it)) return r;
-    return step_sh_any(c);
+    if (int r = step_sh_any(c)) return r;
+    note_snapshots_sh(c, it, it);
+    return SWPC_OK;
 }
 
 int swpcsh_set_graph(swpcsh_ctx *c, int steps)
@@ -823,6 +887,72 @@ int swpcsh_run(swpcsh_ctx *c, int it0, int nsteps)
     }
     for (; done < nsteps; done++)
         if (int r = step_sh_any(c)) return r;
+    note_snapshots_sh(c, it0, it0 + nsteps - 1);
+    return SWPC_OK;
+}
+
+int swpcsh_add_snapshot(swpcsh_ctx *c, int kind, int npts, const int *ik, int ntdec_s, int nbuf, float uc, float m0,
+                        float dt, int *id)
+{
+    clear_error();
+    if (!c || !ik || !id) return fail(SWPC_EINVAL, "null argument");
+    if (kind != SWPC_SNAP_V && kind != SWPC_SNAP_U)
+        return fail(SWPC_EINVAL, "SH snapshot kind %d: only 'v' (%d) and 'u' (%d) exist", kind, SWPC_SNAP_V,
+                    SWPC_SNAP_U);
+    if (npts < 1 || ntdec_s < 1 || nbuf < 1) return fail(SWPC_EINVAL, "npts, ntdec_s and nbuf must be >= 1");
+    if ((long long)npts * nbuf >= (1LL << 31)) return fail(SWPC_EINVAL, "snapshot store too large");
+    std::vector<int> p2((size_t)2 * npts);
+    for (int n = 0; n < npts; n++) {
+        const int i = ik[2 * n], k = ik[2 * n + 1];
+        if (i < -1 || i > c->d.nx + 2 || k < -1 || k > c->d.nz + 2)
+            return fail(SWPC_EINVAL, "snapshot point %d at (%d,%d) outside the grid", n, i, k);
+        p2[2 * n] = i;
+        p2[2 * n + 1] = k;
+    }
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    swpcsh_ctx::Snap sn;
+    sn.kind = kind; sn.npts = npts; sn.ntdec = ntdec_s; sn.nbuf = nbuf; sn.uc = uc; sn.m0 = m0; sn.dt = dt;
+    int r;
+    if ((r = dalloc(sn.pts, p2.size() * 4, c->bytes))) return r;
+    if ((r = dalloc(sn.frames, (size_t)npts * nbuf * 4, c->bytes))) {
+        dfree(sn.pts);
+        return r;
+    }
+    if (kind == SWPC_SNAP_U && (r = dalloc(sn.acc, (size_t)npts * 4, c->bytes))) {
+        dfree(sn.pts);
+        dfree(sn.frames);
+        return r;
+    }
+    SWPC_HIP(hipMemcpy(sn.pts.p, p2.data(), p2.size() * 4, hipMemcpyHostToDevice));
+    c->snaps.push_back(sn);
+    *id = (int)c->snaps.size() - 1;
+    graph_drop_sh(c);
+    return SWPC_OK;
+}
+
+int swpcsh_snapshots(swpcsh_ctx *c, int it)
+{
+    clear_error();
+    if (int r = ready_sh(c)) return r;
+    if (it < 1) return fail(SWPC_EINVAL, "it must be >= 1");
+    if (int r = set_step_sh(c, it)) return r;
+    if (int r = fp64_sh(c) ? lsh_snap<double>(c) : lsh_snap<float>(c)) return r;
+    note_snapshots_sh(c, it, it);
+    return SWPC_OK;
+}
+
+int swpcsh_get_snapshot(swpcsh_ctx *c, int id, int frame, float *out)
+{
+    clear_error();
+    if (!c || !out) return fail(SWPC_EINVAL, "null argument");
+    if (id < 0 || id >= (int)c->snaps.size()) return fail(SWPC_EINVAL, "unknown snapshot id %d", id);
+    const auto &sn = c->snaps[id];
+    if (frame < 0 || frame > sn.last || frame <= sn.last - sn.nbuf)
+        return fail(SWPC_EINVAL, "snapshot %d frame %d not held (latest %lld, %d buffered)", id, frame, sn.last,
+                    sn.nbuf);
+    SWPC_HIP(hipStreamSynchronize(c->stream));
+    SWPC_HIP(hipMemcpy(out, (const float *)sn.frames.p + (size_t)(frame % sn.nbuf) * sn.npts, (size_t)sn.npts * 4,
+                       hipMemcpyDeviceToHost));
     return SWPC_OK;
 }
 

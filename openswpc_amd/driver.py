@@ -347,9 +347,9 @@ def setup(inp, dim="3d"):
         cfg["y0_xz"] = float(ini.get("y0_xz", F32(max(min(0.0, g.yend), g.ybeg)), "s"))
         snaps = [(p, k) for (p, k) in SNAP3 if ini.get(f"{p}_{k}%sw", False)]
     else:
-        snaps = [("xz", k) for k in ("ps", "v", "u") if ini.get(f"xz_{k}%sw", False)]
-        if dim == "sh" and snaps:
-            raise InputError("SH snapshots (xz_*%sw) are not supported by this host")
+        # swpc_sh reads only xz_v%sw and xz_u%sw (src/swpc_sh/m_snap.f90:82-83)
+        snaps = [("xz", k) for k in (("v", "u") if dim == "sh" else ("ps", "v", "u"))
+                 if ini.get(f"xz_{k}%sw", False)]
     wav = {"v": sw_v, "u": sw_u, "format": ini.get("wav_format", "sac"), "prg": ini.get("ntdec_w_prg", 0)}
     if wav["format"] not in ("sac", "csf", "tar_st", "tar_node"):
         raise InputError(f"unknown wav_format {wav['format']!r}")
@@ -426,7 +426,7 @@ def run(inp, dim="3d", precision="fp32", log=None, exedate=None):
             else:
                 sg = snapshot.snap_grid2(g, cfg["idec"], cfg["kdec"])
                 for _, k in S.snaps:
-                    ss.add2(sg, k, S.medium)
+                    (ss.add_sh if dim == "sh" else ss.add2)(sg, k, S.medium)
         ring = S.snap_cfg["ntdec_s"] * 8 if ss else S.nt
         prg = S.wav["prg"]
         t0 = time.perf_counter()

@@ -233,6 +233,7 @@ VNAMES3 = {"ps": (("div", "rot_x", "rot_y", "rot_z"), ("1/s",) * 4),
            "v": (("Vx", "Vy", "Vz"), ("m/s",) * 3), "u": (("Ux", "Uy", "Uz"), ("m",) * 3)}
 VNAMES2 = {"ps": (("divergence", "rotation"), ("1/s",) * 2),
            "v": (("Vx", "Vz"), ("m/s",) * 2), "u": (("Ux", "Uz"), ("m",) * 2)}
+VNAMES_SH = {"v": (("Vy",), ("m/s",)), "u": (("Uy",), ("m",))}
 PEAK_NAMES = (("max-V", "Maximum amplitude of the vertical component"),
               ("max-H", "Maximum amplitude of the horizontal components"),
               ("max-A", "Maximum amplitude of the vector motion"))
@@ -245,11 +246,12 @@ class NcSnapFile:
 
     def __init__(self, path, code, plane, snaptype, kind, xs1, xs2, medium, head, dt, ntdec_s,
                  lonlat=None, peaks=False):
-        psv = code == "SWPC_PSV"
+        # the P-SV and SH headers share one attribute order
+        psv = code in ("SWPC_PSV", "SWPC_SH ")
         horiz = plane in ("xy", "fs", "ob")
         n1, n2 = len(xs1), len(xs2)
         names = {"yz": ("y", "z"), "xz": ("x", "z")}.get(plane, ("x", "y"))
-        vn, vu = (VNAMES2 if psv else VNAMES3)[kind]
+        vn, vu = {"SWPC_PSV": VNAMES2, "SWPC_SH ": VNAMES_SH}.get(code, VNAMES3)[kind]
         self.path, self.nvar, self.n1, self.n2 = path, len(vn), n1, n2
         self.dt, self.ntdec_s, self.peaks = F32(dt), int(ntdec_s), peaks
         self.vnames = vn
@@ -382,7 +384,8 @@ class _Stream:
 @dataclass
 class SnapshotSet:
     """Snapshot streams of one solver, written under odir as native files
-    (``<odir>/<title>.3d.<sec>.<kind>.snp``, ``.psv.xz.<kind>.snp``) or,
+    (``<odir>/<title>.3d.<sec>.<kind>.snp``, ``.psv.xz.<kind>.snp``,
+    ``.sh.xz.<kind>.snp``) or,
     with fmt = 'netcdf', as classic netCDF files (``.nc``).
 
     ``run(it0, n)`` advances the solver in chunks that fit the device ring
@@ -476,6 +479,33 @@ class SnapshotSet:
                            self.clon, self.clat, self.phi, 0.0)
         path = os.path.join(self.odir, f"{self.title.strip()}.psv.xz.{kind}.snp")
         self.streams.append(_Stream(sid, "xz", kind, n1, n2, 2, SnapFile(path, hdr, med)))
+        return path
+
+    def add_sh(self, sg: SnapGrid2, kind, medium):
+        """An SH section (src/swpc_sh/m_snap.f90:73-143): 'v' (Vy) or 'u'
+        (Uy), one plane per frame; snaptype 'vy'/'uy', the header's three
+        trailing words -12345.0."""
+        if kind not in ("v", "u"):
+            raise ValueError(f"SH snapshots are 'v' or 'u', not {kind!r}")
+        g = self.sim.g
+        n1, n2, ik = cells2(sg)
+        sid = self.sim.add_snapshot(KIND[kind], ik, self.ntdec_s, self.nbuf, F32(getattr(self.sim, "UC", UC2)),
+                                    F32(self.sim.M0), F32(g.dt))
+        ii, kk = ik[:, 0] + 2, ik[:, 1] + 2
+        med = [np.asarray(a)[ii, kk].reshape(n2, n1).T for a in (medium.rho, medium.lam, medium.mu)]
+        snaptype = kind + "y"
+        if self.fmt == "netcdf":
+            ds1, ds2 = F32(sg.idec * float(g.dx)), F32(sg.kdec * float(g.dz))
+            path = os.path.join(self.odir, f"{self.title.strip()}.sh.xz.{kind}.nc")
+            f = NcSnapFile(path, "SWPC_SH ", "xz", snaptype, kind, sg.xsnp, sg.zsnp, med,
+                           self._head(g.na // sg.idec, g.na // sg.kdec, ds1, ds2), g.dt, self.ntdec_s)
+            self.streams.append(_Stream(sid, "xz", kind, n1, n2, 1, f))
+            return path
+        hdr = header_bytes("SWPC_SH ", self.title, self.exedate, "xz", snaptype, sg.xsnp, sg.zsnp, g.dt,
+                           self.ntdec_s, g.na // sg.idec, g.na // sg.kdec, 3, 1, self.clon, self.clat, self.phi,
+                           -12345.0)
+        path = os.path.join(self.odir, f"{self.title.strip()}.sh.xz.{kind}.snp")
+        self.streams.append(_Stream(sid, "xz", kind, n1, n2, 1, SnapFile(path, hdr, med)))
         return path
 
     def flush(self, it_done):

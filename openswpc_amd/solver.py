@@ -532,6 +532,24 @@ class SwpcSH:
         check(lib().swpcsh_vmax(self.h, ptr(out, C.c_float)), "vmax")
         return out
 
+    def add_snapshot(self, kind, ik, ntdec_s, nbuf, uc, m0, dt):
+        """Register an SH snapshot stream ('v' or 'u') over 1-based (i,k)
+        cells (src/swpc_sh/m_snap.f90); returns its id."""
+        ik = _i32(np.asarray(ik).reshape(-1, 2))
+        sid = C.c_int()
+        check(lib().swpcsh_add_snapshot(self.h, int(kind), len(ik), ptr(ik, C.c_int), int(ntdec_s), int(nbuf),
+                                        float(uc), float(m0), float(dt), C.byref(sid)), "add_snapshot")
+        return sid.value
+
+    def snapshots(self, it):
+        check(lib().swpcsh_snapshots(self.h, it), "snapshots")
+
+    def get_snapshot(self, sid, frame, nvar, npts):
+        assert nvar == 1
+        out = np.zeros((1, npts), np.float32)
+        check(lib().swpcsh_get_snapshot(self.h, int(sid), int(frame), ptr(out, C.c_float)), "get_snapshot")
+        return out
+
     def step(self, it):
         check(lib().swpcsh_step(self.h, it), "step")
 

@@ -592,9 +592,33 @@ class OracleSH:
             self.L.osh_bodyforce(self._ps(), self.nbf, i.ctypes.data_as(_PI), k.ctypes.data_as(_PI),
                                  self.bf_f.ctypes.data_as(self._PD), row.ctypes.data_as(self._PD))
 
+    def add_snapshot(self, kind, ik, ntdec_s, uc, m0):
+        """An SH snapshot stream ('v' = 1 or 'u' = 2) over 1-based (i,k)
+        cells (src/swpc_sh/m_snap.f90); frames in self.snaps[id]['frames']."""
+        assert kind in (1, 2)
+        ik = np.ascontiguousarray(ik, np.int32).reshape(-1, 2)
+        if not hasattr(self, "snaps"):
+            self.snaps = []
+        self.snaps.append({"kind": kind, "ik": ik, "ntdec": ntdec_s, "uc": np.float32(uc), "m0": np.float32(m0),
+                           "acc": np.zeros(len(ik), np.float32), "frames": []})
+        return len(self.snaps) - 1
+
+    def snapshots(self, it):
+        for sn in getattr(self, "snaps", []):
+            due = (it - 1) % sn["ntdec"] == 0
+            if not due and sn["kind"] != 2:
+                continue
+            n = len(sn["ik"])
+            out = np.zeros(n, np.float32)
+            self.L.osh_snapshot(self._ps(), sn["kind"], n, sn["ik"].ctypes.data_as(_PI), C.c_float(sn["uc"]),
+                                C.c_float(sn["m0"]), sn["acc"].ctypes.data_as(_PF), out.ctypes.data_as(_PF))
+            if due:
+                sn["frames"].append(out.reshape(1, n))
+
     def step(self, it):
         """src/swpc_sh/main.f90:97-114 order."""
         self.receivers(it)
+        self.snapshots(it)
         self.update_stress()
         self.stressglut(it)
         self.update_vel()

@@ -22,6 +22,10 @@
  *   P-SV source__stressglut      src/swpc_psv/m_source.f90:572-614
  *   P-SV source__bodyforce       src/swpc_psv/m_source.f90:616-654
  *   P-SV wav__store (v, u)       src/swpc_psv/m_wav.f90:137-315
+ *   P-SV snap__write products    src/swpc_psv/m_snap.f90:419-650
+ *   SH   kernel, absorber, source, receivers  src/swpc_sh/m_kernel.f90,
+ *        m_absorb_p.f90, m_source.f90, m_wav.f90 (cited per function)
+ *   SH   snap__write products    src/swpc_sh/m_snap.f90:415-520
  *   kernel__setup coefficients   src/swpc_3d/m_kernel.f90:30-73
  *   damping_profile              src/swpc_3d/m_absorb_p.f90:507-547
  *
@@ -939,6 +943,22 @@ void osh_receivers(osh_t *s, int nst, const int *ist, const int *kst, float *vel
         REAL vy = s->Vy[J2(s, kst[n], ist[n])];
         if (disp) disp[n] += (double)vy * (double)s->dt;
         if (vel) vel[n] = (float)vy;
+    }
+}
+
+/* snap__write products (src/swpc_sh/m_snap.f90:415-520): kind 1 'v' is
+   Vy x UC x M0 (wbuf_xz_v), kind 2 'u' advances the real(SP) sum
+   buf_u + Vy x UC x M0 x dt (wbuf_xz_u) and returns it. */
+void osh_snapshot(osh_t *s, int kind, int npts, const int *ik, float uc, float m0, float *acc, float *out)
+{
+    for (int p = 0; p < npts; p++) {
+        const REAL vy = s->Vy[J2(s, ik[2 * p + 1], ik[2 * p])];
+        if (kind == 1) {
+            out[p] = (float)(vy * (REAL)uc * (REAL)m0);
+        } else {
+            acc[p] = (float)((REAL)acc[p] + vy * (REAL)uc * (REAL)m0 * (REAL)s->dt);
+            out[p] = acc[p];
+        }
     }
 }
 

@@ -51,3 +51,83 @@ def test_fortran_host_matches_python_host(fdrive, tmp_path):
     ref = Oracle3D(case, 32)
     ref.run(1, nt)
     assert np.linalg.norm(vel - ref.vel) / np.linalg.norm(ref.vel) <= 1e-5
+
+
+# C entry points with no counterpart in the reference's Fortran program
+# (state inspection for tests, tuning and profiling, the learned-boundary
+# trainer, the copy probe): no Fortran interface is needed for them.
+NOT_BOUND = {"swpc2d_get_field", "swpc2d_set_field", "swpc3d_get_field", "swpc3d_set_field", "swpcsh_get_field",
+             "swpcsh_set_field", "swpc3d_lab_gather", "swpc3d_memory", "swpc3d_profile", "swpc3d_profile_read",
+             "swpc3d_set_tuning", "swpc_lab_add", "swpc_lab_begin_run", "swpc_lab_count", "swpc_lab_grad",
+             "swpc_lab_grad_noalloc", "swpc_lab_samples", "swpc_lab_train", "swpc_lab_trainer_create",
+             "swpc_lab_trainer_destroy", "swpc_membw"}
+
+
+def test_every_solver_entry_point_has_a_fortran_interface():
+    """include/swpc_hip.f90 binds every entry point of include/swpc_hip.h
+    that a reference-style Fortran host calls (3-D, P-SV and SH), with the
+    C name; the C names it binds all exist."""
+    import re
+    from pathlib import Path
+    inc = Path(__file__).resolve().parents[1] / "include"
+    h = (inc / "swpc_hip.h").read_text()
+    f = (inc / "swpc_hip.f90").read_text()
+    c_names = set(re.findall(r"^\s*(?:int|const char \*|double|float|void)\s+\*?(swpc\w+)\s*\(", h, re.M))
+    f_names = set(re.findall(r"bind\(C, name='(swpc\w+)'\)", f))
+    assert {"swpc2d_add_snapshot", "swpc2d_set_bodyforce", "swpcsh_bodyforce", "swpcsh_add_snapshot"} <= f_names
+    assert c_names - f_names == NOT_BOUND
+    assert f_names - c_names == {"swpc_last_error"} and "const char *swpc_last_error(void)" in h
+    # every bound function is also public in the module
+    pub = set(re.findall(r"\bswpc\w+", " ".join(re.findall(r"^\s*public ::(.*)$", f, re.M))))
+    assert (f_names - {"swpc_last_error"}) <= pub
+
+
+@pytest.fixture(scope="module")
+def fdrive2(fdrive):
+    from openswpc_amd import build as b
+    if not b.FDRIVE2.exists():
+        pytest.skip("P-SV Fortran host not built")
+    return b.FDRIVE2
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bodyforce", [False, True])
+def test_psv_fortran_host_matches_python_host(fdrive2, tmp_path, bodyforce):
+    """The P-SV drop-in (fhost/m_kernel_psv_hip.f90, the order of
+    src/swpc_psv/main.f90:95-114, one library call per reference routine)
+    against the Python host's swpc2d_run over the same library: receiver
+    traces, vmax and the 'ps', 'v' and 'u' snapshot frames bit for bit, and
+    the FP32 oracle within the kernels' bar."""
+    from openswpc_amd import snapshot
+    from openswpc_amd.solver import SwpcPSV
+    from oracle.oracle import Oracle2D
+    nt, ntdec = 60, 5
+    case = problem.case2d(150, 90, 0.25, 0.02, nt, na=10, src_depth=5.0, nst=6, zbeg=-2.0, bodyforce=bodyforce)
+    g = case.grid
+    print(f"Fortran P-SV host: 150x90, nt={nt}, body force {bodyforce}", flush=True)
+    sg = snapshot.snap_grid2(g, 3, 2)
+    _, _, ik = snapshot.cells2(sg)
+    src = case.bforce if bodyforce else case.glut
+    uc = np.float32(snapshot.UC2 * np.float32(1000)) if bodyforce else np.float32(snapshot.UC2)
+    snaps = [(k, ik, uc, np.float32(src.M0)) for k in range(3)]
+    ntw, nst = fcase.write_case2(tmp_path / "case.bin", case, snaps=snaps, ntdec_s=ntdec)
+    r = subprocess.run([str(fdrive2), str(tmp_path / "case.bin"), str(tmp_path / "out.bin")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    nfr = (nt - 1) // ntdec + 1
+    vel, disp, vmax, frames = fcase.read_result2(tmp_path / "out.bin", ntw, nst, [len(ik)] * 3, nfr)
+    with SwpcPSV(g, case.medium, case.pml, case.glut, case.stations, case.ntdec_w, nt,
+                 bodyforce=case.bforce) as sim:
+        ids = [sim.add_snapshot(k, ik, ntdec, nfr, uc, np.float32(src.M0), np.float32(g.dt)) for k in range(3)]
+        sim.run(1, nt)
+        v2, d2 = sim.receivers()
+        vm2 = sim.kernel__vmax()
+        fr2 = [np.array([sim.get_snapshot(s, f, 2, len(ik)) for f in range(nfr)]) for s in ids]
+    assert np.abs(v2).max() > 0
+    assert np.array_equal(vel, v2) and np.array_equal(disp, d2) and np.array_equal(vmax, vm2)
+    for k in range(3):
+        assert np.abs(fr2[k][-1]).max() > 0, k
+        assert np.array_equal(frames[k], fr2[k]), k
+    ref = Oracle2D(case, 32)
+    ref.run(1, nt)
+    assert np.linalg.norm(vel - ref.vel) / np.linalg.norm(ref.vel) <= 1e-5

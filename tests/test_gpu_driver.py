@@ -88,9 +88,12 @@ ntdec_s = 10
     assert v["Vz"].shape[0] == (S.nt - 1) // 10 + 1
 
 
-def test_small_sh_input_end_to_end(tmp_path, monkeypatch):
+@pytest.mark.parametrize("fmt", ["native", "netcdf"])
+def test_small_sh_input_end_to_end(tmp_path, monkeypatch, fmt):
     """swpc_sh.x -i input.inf: the SH host writes Vy/Uy SAC files holding
-    the SH oracle's traces of the same setup (x M0 UC 1e9, UC = 1e-12)."""
+    the SH oracle's traces of the same setup (x M0 UC 1e9, UC = 1e-12), and
+    xz_v / xz_u snapshot files (native or netCDF) holding the oracle's
+    snapshot frames."""
     from oracle.oracle import OracleSH
     monkeypatch.chdir(tmp_path)
     (tmp_path / "src.dat").write_text("0.7 0.0 4.0 0.1 0.6 1e15 30 70 -40\n")
@@ -117,14 +120,38 @@ sw_wav_v = .true.
 sw_wav_u = .true.
 ntdec_w = 2
 fn_stloc = 'st.xy'
-"""
-    print("driver: small SH input 160x100, 80 steps", flush=True)
+xz_v%sw = .true.
+xz_u%sw = .true.
+idec = 2
+kdec = 3
+ntdec_s = 10
+""" + f"snp_format = '{fmt}'\n"
+    print(f"driver: small SH input 160x100, 80 steps, {fmt} snapshots", flush=True)
     lines = []
     S = driver.run(InputFile(text=txt), dim="sh", log=lines.append, exedate=1700000000)
     assert sum("it=" in ln for ln in lines) == 4
     case = problem.Case2D(S.grid, S.medium, S.prof, S.glut, S.cells, S.stations, S.nt, S.ntdec_w, None)
     ref = OracleSH(case, 32)
+    sg = snapshot.snap_grid2(S.grid, 2, 3)
+    n1, n2, ik = snapshot.cells2(sg)
+    sids = {k: ref.add_snapshot(snapshot.KIND[k], ik, 10, snapshot.UC2, F32(S.glut.M0)) for k in ("v", "u")}
     ref.run(1, S.nt)
+    nfr = (S.nt - 1) // 10 + 1
+    for k in ("v", "u"):
+        want = np.array([fr[0].reshape(n2, n1).T for fr in ref.snaps[sids[k]]["frames"]])
+        assert len(want) == nfr and np.abs(want[-1]).max() > 0
+        if fmt == "native":
+            h, med, fr = snapshot.read_snp(f"out/smallsh.sh.xz.{k}.snp")
+            assert (h["codetype"], h["coordinate"], h["datatype"], h["nsnp"]) == ("SWPC_SH ", "xz", k + "y", 1)
+            assert (h["ns1"], h["ns2"]) == (n1, n2) and h["tail"] == [-12345.0] * 3
+            got = fr[:, 0]
+        else:
+            ga, var = snapshot.read_nc_snp(f"out/smallsh.sh.xz.{k}.nc")
+            assert ga["codetype"] == "SWPC_SH " and ga["datatype"] == k + "y" and int(ga["nsnp"]) == 1
+            # record r holds it0 = r*ntdec_s + 1 at index it0 // ntdec_s
+            got = var["Vy" if k == "v" else "Uy"][[(10 * f + 1) // 10 for f in range(nfr)]]
+        assert got.shape == want.shape
+        assert rel(got, want) <= 1e-5, k
     scale = F32(S.glut.M0 * 1e-12 * 1e9)
     assert np.abs(ref.vel).max() > 0
     for s, st in enumerate(S.stations):

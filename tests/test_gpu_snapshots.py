@@ -1,5 +1,5 @@
 """Snapshot streams (snap__write, src/swpc_3d/m_snap.f90; P-SV
-src/swpc_psv/m_snap.f90) on the GPU against the oracle, frame by frame,
+src/swpc_psv/m_snap.f90; SH src/swpc_sh/m_snap.f90) on the GPU against the oracle, frame by frame,
 through the C ABI, and the native files written from them.
 
 Bar: relative L2 per frame <= 1e-5 (FP32 kernels) and <= 1e-6 (FP64
@@ -109,6 +109,40 @@ def test_psv_snapshot_frames(bits):
                 sim.get_snapshot(sid, 0, 2, len(ik))
             for f in range(len(frames) - 4, len(frames)):
                 e = rel(sim.get_snapshot(sid, f, 2, len(ik)), frames[f])
+                assert e <= tol, (k, f, e)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_sh_snapshot_frames(bits):
+    """SH 'v' and 'u' streams (src/swpc_sh/m_snap.f90:415-520) against the
+    oracle's frames; the ring keeps the last nbuf frames; 'ps' does not
+    exist in SH and is refused."""
+    from openswpc_amd.solver import SwpcSH
+    from oracle.oracle import OracleSH
+    nt, ntdec = 60, 6
+    case = problem.case_sh(160, 80, 0.25, 0.02, nt, na=10, src_depth=5.0, zbeg=-2.0)
+    g = case.grid
+    sg = snapshot.snap_grid2(g, idec=3, kdec=2)
+    print(f"SH snapshots fp{bits}: 160x80, nt={nt}", flush=True)
+    ref = OracleSH(case, bits)
+    m0 = F32(case.glut.M0)
+    _, _, ik = snapshot.cells2(sg)
+    rids = [ref.add_snapshot(k, ik, ntdec, snapshot.UC2, m0) for k in (1, 2)]
+    ref.run(1, nt)
+    with SwpcSH(g, case.medium, case.pml, case.glut, case.stations, 1, nt,
+                precision="fp32" if bits == 32 else "fp64") as sim:
+        with pytest.raises(Exception):
+            sim.add_snapshot(0, ik, ntdec, 4, snapshot.UC2, m0, F32(g.dt))
+        ids = [sim.add_snapshot(k, ik, ntdec, 4, snapshot.UC2, m0, F32(g.dt)) for k in (1, 2)]
+        sim.run(1, nt)
+        tol = 1e-5 if bits == 32 else 1e-6
+        for k, (rid, sid) in enumerate(zip(rids, ids)):
+            frames = ref.snaps[rid]["frames"]
+            assert len(frames) == (nt - 1) // ntdec + 1 and np.abs(frames[-1]).max() > 0
+            with pytest.raises(Exception):
+                sim.get_snapshot(sid, 0, 1, len(ik))
+            for f in range(len(frames) - 4, len(frames)):
+                e = rel(sim.get_snapshot(sid, f, 1, len(ik)), frames[f])
                 assert e <= tol, (k, f, e)
 
 

@@ -152,3 +152,53 @@ def test_oracle_psv_ps(case2):
     for p in range(2):
         np.testing.assert_allclose(out[:, p], want, rtol=1e-6)
     assert np.all(out[:, 2] == 0)
+
+
+@pytest.fixture(scope="module")
+def case_sh():
+    return problem.case_sh(60, 40, 0.5, 0.04, 30, na=6, src_depth=4.0, nst=4, st_spacing=3.0, zbeg=-2.0)
+
+
+def test_sh_header_layout(tmp_path, case_sh):
+    """write_snp_header of swpc_sh (src/swpc_sh/m_snap.f90:188-232): code
+    'SWPC_SH ', coordinate 'xz', snaptype 'vy', nmed 3, nsnp 1 and the
+    three trailing words -12345.0."""
+    g = case_sh.grid
+    sg = snapshot.snap_grid2(g, 2, 3)
+    hb = snapshot.header_bytes("SWPC_SH ", "shrun", 7, "xz", "vy", sg.xsnp, sg.zsnp, g.dt, 5, g.na // 2, g.na // 3,
+                               3, 1, 0.0, 0.0, 0.0, -12345.0)
+    f = snapshot.SnapFile(tmp_path / "s.snp", hb, [np.ones((sg.nxs, sg.nzs), F32)] * 3)
+    f.append([np.full((sg.nxs, sg.nzs), 2, F32)])
+    f.close()
+    h, m, fr = snapshot.read_snp(tmp_path / "s.snp")
+    assert h["codetype"] == "SWPC_SH "
+    assert (h["coordinate"], h["datatype"], h["nmed"], h["nsnp"]) == ("xz", "vy", 3, 1)
+    assert (h["ns1"], h["ns2"]) == (sg.nxs, sg.nzs) and h["tail"] == [-12345.0] * 3
+    assert fr.shape == (1, 1, sg.nxs, sg.nzs) and np.all(fr == 2)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_oracle_sh_snapshots_against_receivers(case_sh, bits):
+    """The 'v' and 'u' snapshot products against the receiver traces,
+    which are written by another routine (wav__store, m_wav.f90): at a
+    station cell Vy x UC x M0 is the trace times UC M0, and the 'u' sum in
+    real(SP) follows the receiver's FP64 displacement sum x UC M0 to FP32
+    accumulation error.  ntdec_s = 3, so 'u' advances on the steps between
+    frames too."""
+    from oracle.oracle import OracleSH
+    c = case_sh
+    g = c.grid
+    ref = OracleSH(c, bits)
+    m0 = F32(c.glut.M0)
+    pts = np.asarray(c.stations, np.int32).reshape(-1, 2)
+    sv = ref.add_snapshot(1, pts, 3, snapshot.UC2, m0)
+    su = ref.add_snapshot(2, pts, 3, snapshot.UC2, m0)
+    ref.run(1, c.nt)
+    s = float(snapshot.UC2) * float(m0)
+    v, u = ref.snaps[sv]["frames"], ref.snaps[su]["frames"]
+    assert len(v) == len(u) == (c.nt - 1) // 3 + 1
+    assert np.abs(ref.vel).max() > 0
+    for f in range(len(v)):
+        it = 3 * f + 1
+        np.testing.assert_allclose(v[f][0], ref.vel[it - 1] * s, rtol=1e-6, atol=0)
+        np.testing.assert_allclose(u[f][0], ref.disp[it - 1] * s, rtol=2e-5, atol=1e-6 * np.abs(u[-1]).max())
