@@ -36,6 +36,7 @@ B_INT_VEL = 15 * 4      # S 6r, b 3r, V 3r/3w
 B_PML_STRESS = 38 * 4
 B_PML_VEL = 33 * 4
 CHOSEN = "fp32_r16"    # the reduced-storage form reported beside the plain one
+MISFIT_BAR = 1e-3      # receiver-trace misfit against the FP64 plain scheme that a faster form must meet
 
 
 def log(msg):
@@ -56,14 +57,19 @@ def main():
                          "with the oracle's after them, so the source (onset 0.1 s) must have started")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-chosen", action="store_true")
-    ap.add_argument("--misfit-steps", type=int, default=600,
+    ap.add_argument("--misfit-steps", type=int, default=2000,
                     help="steps of the misfit runs, independent of --steps: long enough for the P and S "
-                         "arrivals from the 10 km deep source to cross the surface receivers")
+                         "arrivals from the 10 km deep source to cross the surface receivers and for the "
+                         "absorbers' reflections to reach them (from about 8 s, 800 steps)")
     ap.add_argument("--no-c1", action="store_true", help="skip the P-SV 800x400 4000-step leg")
-    ap.add_argument("--config", choices=("c2", "c3", "sh"), default="c2",
+    ap.add_argument("--no-misfit", action="store_true",
+                    help="skip the misfit runs (profiling: the timed forms only, no accuracy and no choice)")
+    ap.add_argument("--config", choices=("c2", "c3", "c5", "sh"), default="c2",
                     help="c2: the 512x512x256 bench line (default); c3: the 1024x1024x512 reduced-storage "
                          "run against an FP64 run of the same scheme (one JSON line of its own); sh: the "
-                         "8192x4096 SH (swpc_sh) step, FP32 and FP64 (one JSON line of its own)")
+                         "8192x4096 SH (swpc_sh) step, FP32 and FP64 (one JSON line of its own); c5: the 20-cell "
+                         "PML and the learned boundary of the c2 workload against an echo-free reference on an "
+                         "extended grid (one JSON line of its own)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -78,6 +84,8 @@ def main():
         return run_c3(args)
     if args.config == "sh":
         return run_sh(args)
+    if args.config == "c5":
+        return run_c5(args)
     nt = args.warmup + args.steps
     t0 = time.time()
     # the source table covers both the timed run and the misfit window
@@ -125,18 +133,26 @@ def main():
         icase = lab.interior_case(case) if weights is not None else None
         labr = timed_run(icase, CHOSEN, args.warmup, args.steps, lab=weights) if weights is not None else None
         lab32 = timed_run(icase, "fp32", args.warmup, args.steps, lab=weights) if weights is not None else None
-        mis = misfit(case, args.misfit_steps, icase, weights)
+        if args.no_misfit:
+            # profiling runs: timings only, no accuracy and no choice
+            mis = {"steps": 0, "fp32": None, "r16": None, "lab_r16": None, "lab_fp64": None,
+                   "lab_r16_same_scheme": None, "lab_fp32_same_scheme": None}
+        else:
+            mis = misfit(case, args.misfit_steps, icase, weights)
         reduced = {"form": "reduced storage (fp32_r16): FP32 arithmetic, V, S and medium; memory variables in FP16 "
                            "with a power-of-two scale, PML ADE arrays in BF16; 20-cell ADE-PML",
                    "ms_per_step": ms_red, "value": ncell * args.steps / red["wall"],
                    "speedup_vs_plain": ms_step / ms_red, "misfit_vs_fp64": mis["r16"]}
-        log(f"reduced storage: {ms_red:.3f} ms/step, speedup {ms_step / ms_red:.2f}x, misfit {mis['r16']:.2e}")
-        chosen = {"misfit_steps": mis["steps"], "plain_misfit_vs_fp64": mis["fp32"], "reduced_storage": reduced}
+        log(f"reduced storage: {ms_red:.3f} ms/step, speedup {ms_step / ms_red:.2f}x, misfit {mis['r16']}")
+        chosen = {"misfit_steps": mis["steps"], "plain_misfit_vs_fp64": mis["fp32"], "reduced_storage": reduced,
+                  "selection": f"the fastest form whose receiver traces are within {MISFIT_BAR:g} (relative L2) of the "
+                               f"FP64 run of the plain scheme (20-cell PML) over the first {mis['steps']} steps"}
+        forms = [("reduced_storage", reduced)]
         if labr is not None:
             gi = icase.grid
             ms_lab = labr["wall"] / args.steps * 1e3
             ms_lab32 = lab32["wall"] / args.steps * 1e3
-            chosen.update({
+            learned = {
                 "form": f"learned absorbing boundary (per-face MLP ghost fill, BF16 weights, FP32 arithmetic) on the "
                         f"{gi.nx}x{gi.ny}x{gi.nz} interior of the plain grid (same physical region, receivers and "
                         f"source, no absorber cells) + reduced storage (fp32_r16)",
@@ -145,16 +161,29 @@ def main():
                 "value_note": "cell-updates/s over the cells this form updates (the 472x472x236 grid)",
                 "physical_region_equivalent_cell_updates_per_s": ncell * args.steps / labr["wall"],
                 "speedup_vs_plain": ms_step / ms_lab,
-                "misfit_vs_fp64": mis["lab_r16_same_scheme"],
-                "misfit_vs_fp64_note": "FP64 run of the same scheme (learned boundary, 472x472x236)",
-                "misfit_vs_fp64_plain_pml20": mis["lab_r16"],
+                "misfit_vs_fp64": mis["lab_r16"],
+                "misfit_vs_fp64_note": "FP64 run of the plain scheme (20-cell PML): the two boundaries' difference "
+                                       "once the reflections reach the receivers",
+                "misfit_vs_fp64_same_scheme": mis["lab_r16_same_scheme"],
                 "fp64_learned_vs_fp64_plain_pml20": mis["lab_fp64"],
-                "lab_fp32_ms_per_step": ms_lab32, "lab_fp32_misfit_vs_fp64": mis["lab_fp32_same_scheme"],
+                "lab_fp32_ms_per_step": ms_lab32, "lab_fp32_misfit_vs_fp64_same_scheme": mis["lab_fp32_same_scheme"],
                 "lab_fill_us_per_step": (labr["ms"][1] / max(labr["cnt"][1], 1) + labr["ms"][3] / max(labr["cnt"][3], 1)) * 1e3,
-            })
-            log(f"chosen (learned boundary + r16): {ms_lab:.3f} ms/step, speedup {ms_step / ms_lab:.2f}x, "
-                f"misfit {mis['lab_r16_same_scheme']:.2e} vs its FP64 run, {mis['lab_r16']:.2e} vs the FP64 20-cell-PML "
-                f"run (learned boundary fp32: {ms_lab32:.3f} ms/step)")
+            }
+            forms.append(("learned_boundary", learned))
+            chosen["learned_boundary"] = learned
+            log(f"learned boundary + r16: {ms_lab:.3f} ms/step, speedup {ms_step / ms_lab:.2f}x, "
+                f"misfit {mis['lab_r16']} vs the FP64 20-cell-PML run, {mis['lab_r16_same_scheme']} vs its "
+                f"own FP64 run (learned boundary fp32: {ms_lab32:.3f} ms/step)")
+        ok = [(f["ms_per_step"], k, f) for k, f in forms
+              if f["misfit_vs_fp64"] is not None and f["misfit_vs_fp64"] <= MISFIT_BAR]
+        if ok:
+            _, key, best = min(ok)
+            chosen.update({"chosen": key, "form": best["form"], "ms_per_step": best["ms_per_step"],
+                           "value": best["value"], "speedup_vs_plain": best["speedup_vs_plain"],
+                           "misfit_vs_fp64": best["misfit_vs_fp64"]})
+            log(f"chosen: {key}, {best['ms_per_step']:.3f} ms/step, {best['speedup_vs_plain']:.2f}x")
+        else:
+            chosen["chosen"] = None
 
     cpu = None
     if not args.no_cpu:
@@ -188,6 +217,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
             "traffic": traffic_from_profile(),
+            "traffic_source": "FETCH_SIZE x 2 + WRITE_SIZE per launch from the rocprofv3 counter runs committed in "
+                              "profiles/stress_int_512x512x256_traffic.json (a counter run of its own, not this one)",
             "alg_bytes_per_launch": alg_stress,
             "alg_bytes_per_cell": B_INT_STRESS,
             "kernel_streams_bytes_per_cell": need_stress,
@@ -411,6 +442,85 @@ def run_c3(args, nt=4000, plain_steps=300, chunk=250):
     out["misfit_r16_vs_fp64"] = float(np.linalg.norm(traces[CHOSEN] - ref) / np.linalg.norm(ref))
     out["speedup_r16_vs_fp32"] = out["fp32"]["ms_per_step"] / out[CHOSEN]["ms_per_step"]
     log(f"c3: {out}")
+    print(json.dumps(out), flush=True)
+
+
+def run_c5(args, nt=2000, ext=256, ext_z=200, chunk=250, early=700):
+    """The boundaries of the 512 x 512 x 256 workload against an echo-free
+    reference over nt steps: the same medium, source and receivers on a
+    grid extended by ext cells on each side in x and y and ext_z cells at
+    the bottom (1024 x 1024 x 456 by default), far enough that no wave
+    reflected by its own absorber reaches a receiver inside the window
+    (path length / vmax > nt dt).  Receiver traces of the 20-cell PML
+    (FP32, FP64) and of the learned boundary on the 472 x 472 x 236 interior
+    (reduced storage, FP32, FP64) against it: relative L2 misfit over the
+    whole window, over the first `early` steps (before the first boundary
+    reflection reaches the nearest receiver, which checks that the grids
+    agree) and over the rest (the reflections)."""
+    from openswpc_amd import lab, problem
+    from openswpc_amd.solver import Swpc3D
+    if not lab.WEIGHTS.exists():
+        raise RuntimeError("c5: no trained nets in the package")
+    weights = lab.load_weights()[0]
+    sp = (512 - 2 * 20) * 0.25 / 9   # the receiver spacing of the 512 grid's default 8 x 8 layout
+    kw = dict(na=20, seed=args.seed, rmed=(0.05, 2.0, 0.5), src_depth=10.0, nst_side=8, st_spacing=sp,
+              rmed_tile=(128, 128, 128))
+    t0 = time.time()
+    case = problem.case3d(512, 512, 256, 0.25, 0.01, nt, **kw)
+    icase = lab.interior_case(case)
+    big = problem.case3d(512 + 2 * ext, 512 + 2 * ext, 256 + ext_z, 0.25, 0.01, nt, device_medium=True, **kw)
+    gb = big.grid
+    vmax = case.medium.vmax
+    # shortest path source -> own absorber -> receiver, in km
+    xb = (gb.nx // 2 - gb.na) * gb.dx
+    zb = gb.zbeg + (gb.nz - gb.na) * gb.dz
+    rmax = max(abs(s.x) for s in case.station_list)
+    path = min(2 * xb - rmax, 2 * zb - 10.0)
+    log(f"c5 setup {time.time() - t0:.0f} s; reference grid {gb.nx}x{gb.ny}x{gb.nz}: its own reflections need "
+        f"{path / vmax:.1f} s > window {nt * 0.01:.1f} s")
+    if path / vmax <= nt * 0.01:
+        raise RuntimeError("c5: the reference grid is too small for the window")
+
+    def run(c, prec, w=None, raw=False):
+        t1 = time.perf_counter()
+        sim = problem.open_case3d(c, prec) if raw else Swpc3D(c.grid, c.medium, c.pml, c.glut, c.stations,
+                                                              c.ntdec_w, nt, precision=prec)
+        if w is not None:
+            sim.set_learned_boundary(w)
+        for it in range(1, nt + 1, chunk):
+            sim.run(it, min(chunk, nt - it + 1))
+            sim.sync()
+            log(f"c5 {c.grid.nx}x{c.grid.ny}x{c.grid.nz} {prec}{' + learned boundary' if w is not None else ''}: "
+                f"{min(it + chunk - 1, nt)}/{nt} steps, {time.perf_counter() - t1:.1f} s")
+        v = sim.receivers()[0].astype(np.float64)
+        sim.close()
+        if not np.all(np.isfinite(v)):
+            raise RuntimeError("c5: non-finite traces")
+        return v
+
+    truth = run(big, "fp32", raw=True)
+    names = [s.name for s in case.station_list]
+    if [s.name for s in big.station_list] != names or [s.name for s in icase.station_list] != names:
+        raise RuntimeError("c5: the receiver lists differ")
+    out = {"config": {"workload": f"swpc_3d 512x512x256 workload's boundaries vs an echo-free {gb.nx}x{gb.ny}x{gb.nz} "
+                                  f"reference, {nt} steps", "nt": nt, "early_steps": early,
+                      "reference_grid": [gb.nx, gb.ny, gb.nz],
+                      "reference_own_reflection_s": path / vmax},
+           "runs": {}}
+    forms = [("pml20_fp32", case, "fp32", None), ("pml20_fp64", case, "fp64", None),
+             ("learned_r16", icase, CHOSEN, weights), ("learned_fp32", icase, "fp32", weights),
+             ("learned_fp64", icase, "fp64", weights)]
+    for key, c, prec, w in forms:
+        v = run(c, prec, w)
+        r = {"misfit": rel_misfit(v, truth, f"{key} vs reference"),
+             "misfit_early": rel_misfit(v[:early], truth[:early], f"{key} early"),
+             "misfit_late": rel_misfit(v[early:], truth[early:], f"{key} late"),
+             "residual_energy_late_over_total": float(np.sum((v[early:] - truth[early:]) ** 2) / np.sum(truth ** 2))}
+        per = np.linalg.norm((v - truth).reshape(nt, -1, 3), axis=(0, 2)) / np.linalg.norm(truth.reshape(nt, -1, 3),
+                                                                                            axis=(0, 2))
+        r["worst_receiver"] = {"name": names[int(np.argmax(per))], "misfit": float(per.max())}
+        out["runs"][key] = r
+        log(f"c5 {key}: {r}")
     print(json.dumps(out), flush=True)
 
 

@@ -44,7 +44,7 @@ module swpc_hip
     public :: swpc2d_set_sources, swpc2d_set_bodyforce, swpc2d_set_receivers, swpc2d_update_stress
     public :: swpc2d_update_vel, swpc2d_stressglut, swpc2d_bodyforce, swpc2d_store_receivers
     public :: swpc2d_add_snapshot, swpc2d_snapshots, swpc2d_get_snapshot
-    public :: swpc2d_step, swpc2d_set_graph, swpc2d_run, swpc2d_get_receivers, swpc2d_vmax, swpc2d_sync
+    public :: swpc2d_step, swpc2d_set_graph, swpc2d_set_fusion, swpc2d_run, swpc2d_get_receivers, swpc2d_vmax, swpc2d_sync
     public :: swpcsh_create, swpcsh_destroy, swpcsh_set_medium, swpcsh_set_surface, swpcsh_set_pml
     public :: swpcsh_set_sources, swpcsh_set_bodyforce, swpcsh_set_receivers, swpcsh_update_stress
     public :: swpcsh_update_vel, swpcsh_stressglut, swpcsh_bodyforce, swpcsh_store_receivers
@@ -393,6 +393,12 @@ module swpc_hip
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx
             integer(c_int), value :: steps
+        end function
+
+        integer(c_int) function swpc2d_set_fusion(ctx, on) bind(C, name='swpc2d_set_fusion')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: on
         end function
 
         integer(c_int) function swpc2d_step(ctx, it) bind(C, name='swpc2d_step')

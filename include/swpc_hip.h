@@ -288,7 +288,12 @@ int swpc3d_memory(swpc3d_ctx *ctx, long long *bytes);
      [18]    reduced storage: log2 shift of the memory-variable scale
              (-24..24, default 0; negative stores larger values, away from
              the FP16 subnormals, at the cost of headroom before the
-             +-65504 clamp).  Stored values are rescaled with it. */
+             +-65504 clamp).  Stored values are rescaled with it;
+     [19]    absorber boxes of whole rows (the y faces and the bottom)
+             walked one 256-cell row segment per workgroup (0/1, default 1;
+             0 walks every box cell by cell);
+     [20]    PML workgroups ordered so that each XCD takes one contiguous
+             run of them (0/1, default 0). */
 int swpc3d_set_tuning(swpc3d_ctx *ctx, const int *p, int n);
 
 /* ------------------------------------------------------------------ */
@@ -423,6 +428,13 @@ int swpc2d_step(swpc2d_ctx *ctx, int it);
    default 16).  No counterpart in the reference: the per-step work of
    small P-SV grids is shorter than its launch overhead. */
 int swpc2d_set_graph(swpc2d_ctx *ctx, int steps);
+/* Steps of swpc2d_step/swpc2d_run in two fused kernels (on != 0, the
+   default) or in the separate kernels of the per-routine entry points
+   (0).  No counterpart in the reference; both give the same results bit
+   for bit.  The fused kernels also take the receivers, and the stress glut
+   and body force when there are at most 8 of them with their cells and
+   the cells' -x and -z neighbours inside [1, nx] x [1, nz]. */
+int swpc2d_set_fusion(swpc2d_ctx *ctx, int on);
 int swpc2d_run(swpc2d_ctx *ctx, int it0, int nsteps);
 /* vel[itw][n][2], disp[itw][n][2] (x, z-up) */
 int swpc2d_get_receivers(swpc2d_ctx *ctx, float *vel, double *disp);

@@ -133,6 +133,7 @@ _PROTOS = {
     "swpc2d_snapshots": (I, [P, I]),
     "swpc2d_get_snapshot": (I, [P, I, I, PF]),
     "swpc2d_set_graph": (I, [P, I]),
+    "swpc2d_set_fusion": (I, [P, I]),
     "swpc2d_step": (I, [P, I]),
     "swpc2d_run": (I, [P, I, I]),
     "swpc2d_get_receivers": (I, [P, PF, PD]),

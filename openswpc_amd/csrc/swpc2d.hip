@@ -65,11 +65,8 @@ __device__ inline float harm4(float nnn, float pnn, float npn, float ppn)
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k2_stress_int(const K2<T> a)
+__device__ __forceinline__ void stress_int_cell(const K2<T> &a, const int i, const int k)
 {
-    const int i = a.ti0 + (int)(blockIdx.x * 64 + threadIdx.x);
-    const int k = 1 + (int)(blockIdx.y * 4 + threadIdx.y);
-    if (i <= a.na || i > a.nx - a.na || k > a.nz - a.na) return;
     const int px = a.px;
     const int c = at2(px, i, k);
     const int4 fs = a.surf[i - 1];
@@ -123,12 +120,26 @@ __global__ void __launch_bounds__(256) k2_stress_int(const K2<T> a)
     a.Sxz[c] = a.Sxz[c] + (mu_xz * taus_plus1 * dxVz_dzVx + Rxz_n) * dt;
 }
 
+// interior cell of tile thread (bx, bz, tx, tz), or false outside the interior
 template <typename T>
-__global__ void __launch_bounds__(256) k2_vel_int(const K2<T> a)
+__device__ __forceinline__ bool int_cell2(const K2<T> &a, int bx, int bz, int tx, int tz, int &i, int &k)
 {
-    const int i = a.ti0 + (int)(blockIdx.x * 64 + threadIdx.x);
-    const int k = 1 + (int)(blockIdx.y * 4 + threadIdx.y);
-    if (i <= a.na || i > a.nx - a.na || k > a.nz - a.na) return;
+    i = a.ti0 + bx * 64 + tx;
+    k = 1 + bz * 4 + tz;
+    return !(i <= a.na || i > a.nx - a.na || k > a.nz - a.na);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k2_stress_int(const K2<T> a)
+{
+    int i, k;
+    if (int_cell2(a, (int)blockIdx.x, (int)blockIdx.y, (int)threadIdx.x, (int)threadIdx.y, i, k))
+        stress_int_cell(a, i, k);
+}
+
+template <typename T>
+__device__ __forceinline__ void vel_int_cell(const K2<T> &a, const int i, const int k)
+{
     const int px = a.px;
     const int c = at2(px, i, k);
     const int4 fs = a.surf[i - 1];
@@ -152,6 +163,14 @@ __global__ void __launch_bounds__(256) k2_vel_int(const K2<T> a)
     a.Vz[c] = a.Vz[c] + (T)bz * (dxSxz + dzSzz) * a.dt;
 }
 
+template <typename T>
+__global__ void __launch_bounds__(256) k2_vel_int(const K2<T> a)
+{
+    int i, k;
+    if (int_cell2(a, (int)blockIdx.x, (int)blockIdx.y, (int)threadIdx.x, (int)threadIdx.y, i, k))
+        vel_int_cell(a, i, k);
+}
+
 __device__ inline bool pml_cell2(const Boxes2 &B, unsigned id, int &i, int &k)
 {
     if (id >= B.total) return false;
@@ -167,10 +186,8 @@ __device__ inline bool pml_cell2(const Boxes2 &B, unsigned id, int &i, int &k)
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k2_stress_pml(const K2<T> a)
+__device__ __forceinline__ void stress_pml_cell2(const K2<T> &a, const int i, const int k)
 {
-    int i, k;
-    if (!pml_cell2(a.pml, blockIdx.x * 256u + threadIdx.x, i, k)) return;
     const int px = a.px, nx = a.nx, nz = a.nz;
     const int c = at2(px, i, k);
     const T *__restrict__ Vx = a.Vx;
@@ -205,10 +222,15 @@ __global__ void __launch_bounds__(256) k2_stress_pml(const K2<T> a)
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k2_vel_pml(const K2<T> a)
+__global__ void __launch_bounds__(256) k2_stress_pml(const K2<T> a)
 {
     int i, k;
-    if (!pml_cell2(a.pml, blockIdx.x * 256u + threadIdx.x, i, k)) return;
+    if (pml_cell2(a.pml, blockIdx.x * 256u + threadIdx.x, i, k)) stress_pml_cell2(a, i, k);
+}
+
+template <typename T>
+__device__ __forceinline__ void vel_pml_cell2(const K2<T> &a, const int i, const int k)
+{
     const int px = a.px, nx = a.nx, nz = a.nz;
     const int c = at2(px, i, k);
     const T gxc1 = a.gx[0 * nx + i - 1], gxc2 = a.gx[1 * nx + i - 1], gxc3 = a.gx[2 * nx + i - 1], gxc4 = a.gx[3 * nx + i - 1];
@@ -232,6 +254,13 @@ __global__ void __launch_bounds__(256) k2_vel_pml(const K2<T> a)
     a.A[5][c] = gzc3 * azsxz + gzc4 * dzSxz * dt;
     a.A[6][c] = gxc3 * axsxz + gxc4 * dxSxz * dt;
     a.A[7][c] = gze3 * azszz + gze4 * dzSzz * dt;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k2_vel_pml(const K2<T> a)
+{
+    int i, k;
+    if (pml_cell2(a.pml, blockIdx.x * 256u + threadIdx.x, i, k)) vel_pml_cell2(a, i, k);
 }
 
 template <typename T>
@@ -277,12 +306,11 @@ __global__ void k2_bodyforce(const K2<T> a, int nsrc, const int *__restrict__ sr
 }
 
 template <typename T>
-__global__ void k2_receivers(const K2<T> a, int nst, const int *__restrict__ st, double *__restrict__ uacc,
-                             float *__restrict__ wvel, double *__restrict__ wdisp, int ntdec_w, int ntw,
-                             const int *__restrict__ dit)
+__device__ __forceinline__ void receiver2(const K2<T> &a, int n, int nst, const int *__restrict__ st,
+                                          double *__restrict__ uacc, float *__restrict__ wvel,
+                                          double *__restrict__ wdisp, int ntdec_w, int ntw,
+                                          const int *__restrict__ dit)
 {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nst) return;
     const int it = *dit;
     const int itw = (it - 1) / ntdec_w;
     const int store = ((it - 1) % ntdec_w) == 0 && itw < ntw;
@@ -303,6 +331,131 @@ __global__ void k2_receivers(const K2<T> a, int nst, const int *__restrict__ st,
         disp_out[2 * n] = ux;
         disp_out[2 * n + 1] = uz;
     }
+}
+
+template <typename T>
+__global__ void k2_receivers(const K2<T> a, int nst, const int *__restrict__ st, double *__restrict__ uacc,
+                             float *__restrict__ wvel, double *__restrict__ wdisp, int ntdec_w, int ntw,
+                             const int *__restrict__ dit)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < nst) receiver2(a, n, nst, st, uacc, wvel, wdisp, ntdec_w, ntw, dit);
+}
+
+// ---------------------------------------------------------------------------
+// Fused step kernels.  A P-SV grid of a few hundred thousand cells is
+// updated in microseconds, so the step's seven launches cost more in gaps
+// and tails than in work.  Two launches do the same work: one workgroup
+// range per part, each part's workgroups uniform in what they run.
+//   k2_stress_step  receivers (they read V only), interior and absorber
+//                   stress, and the stress glut of each source cell by
+//                   the thread that updated that cell, after its update
+//   k2_vel_step     interior and absorber velocity, and the body force of
+//                   each force cell by the thread that updated that cell
+// The per-cell operations and their order are those of the separate
+// kernels (k2_stressglut applies its sources in order, and each cell's
+// share of a source after the cell's update), so both forms agree bit for
+// bit.  The host fuses only sources whose footprint lies in [1, nx] x
+// [1, nz] and at most MAXFUSE of them; otherwise it launches the separate
+// kernels.
+constexpr int MAXFUSE = 8;
+
+struct Src2 {
+    int n, nt;                      // sources (0: none fused), table steps
+    const int *__restrict__ ik;     // (i, k) per source
+    const double *__restrict__ m;   // m3 (stress glut) or f2 (body force) per source
+    const double *__restrict__ tab; // [it-1][n]
+};
+
+struct Recv2 {
+    int nst, ntdec_w, ntw;
+    const int *__restrict__ st;
+    double *__restrict__ uacc;
+    float *__restrict__ wvel;
+    double *__restrict__ wdisp;
+};
+
+// k2_stressglut's share for cell (i, k): Sxx, Szz and a quarter of Sxz at
+// the source cell, a quarter of Sxz at its -x, -z and -x-z neighbours
+template <typename T>
+__device__ __forceinline__ void glut_at2(const K2<T> &a, const Src2 &s, const int *__restrict__ dit, int i, int k)
+{
+    for (int n = 0; n < s.n; n++) {
+        const int di = s.ik[2 * n] - i, dk = s.ik[2 * n + 1] - k;
+        if ((unsigned)di > 1u || (unsigned)dk > 1u) continue;
+        const int it = *dit;
+        if (it < 1 || it > s.nt) return;
+        const int c = at2(a.px, i, k);
+        const T sd = (T)s.tab[(size_t)(it - 1) * s.n + n];
+        const T mxx = (T)s.m[3 * n], mzz = (T)s.m[3 * n + 1], mxz = (T)s.m[3 * n + 2];
+        if (di == 0 && dk == 0) {
+            a.Sxx[c] -= mxx * sd;
+            a.Szz[c] -= mzz * sd;
+        }
+        a.Sxz[c] -= mxz * sd / 4;
+    }
+}
+
+// k2_bodyforce's share for cell (i, k): Vx at the force cell and its -x
+// neighbour, Vz at the force cell and its -z neighbour, f = fdt / rho of
+// the force cell
+template <typename T>
+__device__ __forceinline__ void bf_at2(const K2<T> &a, const Src2 &s, const int *__restrict__ dit, int i, int k)
+{
+    for (int n = 0; n < s.n; n++) {
+        const int si = s.ik[2 * n], sk = s.ik[2 * n + 1];
+        const int di = si - i, dk = sk - k;
+        if ((unsigned)di > 1u || (unsigned)dk > 1u || (di == 1 && dk == 1)) continue;
+        const int it = *dit;
+        if (it < 1 || it > s.nt) return;
+        const int c = at2(a.px, i, k);
+        const T f = (T)s.tab[(size_t)(it - 1) * s.n + n] / (T)a.rho[at2(a.px, si, sk)];
+        const T fx = (T)s.m[2 * n], fz = (T)s.m[2 * n + 1];
+        if (dk == 0) a.Vx[c] += fx * f;
+        if (di == 0) a.Vz[c] += fz * f;
+    }
+}
+
+// workgroups: [0, nint) interior tiles (ntx per tile row), [nint, nint +
+// npml) absorber cells, then receivers
+template <typename T>
+__global__ void __launch_bounds__(256) k2_stress_step(const K2<T> a, int ntx, unsigned nint, unsigned npml, Src2 s,
+                                                       Recv2 r, const int *__restrict__ dit)
+{
+    const unsigned b = blockIdx.x;
+    int i, k;
+    if (b < nint) {
+        if (!int_cell2(a, (int)(b % (unsigned)ntx), (int)(b / (unsigned)ntx), (int)(threadIdx.x & 63),
+                       (int)(threadIdx.x >> 6), i, k))
+            return;
+        stress_int_cell(a, i, k);
+    } else if (b < nint + npml) {
+        if (!pml_cell2(a.pml, (b - nint) * 256u + threadIdx.x, i, k)) return;
+        stress_pml_cell2(a, i, k);
+    } else {
+        const int n = (int)((b - nint - npml) * 256u + threadIdx.x);
+        if (n < r.nst) receiver2(a, n, r.nst, r.st, r.uacc, r.wvel, r.wdisp, r.ntdec_w, r.ntw, dit);
+        return;
+    }
+    if (s.n) glut_at2(a, s, dit, i, k);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k2_vel_step(const K2<T> a, int ntx, unsigned nint, Src2 s,
+                                                    const int *__restrict__ dit)
+{
+    const unsigned b = blockIdx.x;
+    int i, k;
+    if (b < nint) {
+        if (!int_cell2(a, (int)(b % (unsigned)ntx), (int)(b / (unsigned)ntx), (int)(threadIdx.x & 63),
+                       (int)(threadIdx.x >> 6), i, k))
+            return;
+        vel_int_cell(a, i, k);
+    } else {
+        if (!pml_cell2(a.pml, (b - nint) * 256u + threadIdx.x, i, k)) return;
+        vel_pml_cell2(a, i, k);
+    }
+    if (s.n) bf_at2(a, s, dit, i, k);
 }
 
 __global__ void k2_fill_u32(unsigned *p, int n, unsigned v)
@@ -408,6 +561,8 @@ struct swpc2d_ctx {
     swpc::Boxes2 boxes;
     int ti0 = 1, ntx = 0, ntz = 0;
     int graph_k = 16;       // steps per captured step graph (0 = launch step by step)
+    bool fused = true;      // steps in the fused kernels (swpc2d_set_fusion)
+    bool fuse_src = true, fuse_bf = true;   // the stress glut / body force inside them
     hipGraph_t graph = nullptr;
     unsigned gen = 0;       // bumped by every graph drop (batches recapture on change)
     hipGraphExec_t gexec = nullptr;
@@ -572,7 +727,7 @@ static int set_step2(swpc2d_ctx *c, int it)
 // body force commutes with the absorber update.  The step number is the
 // device counter, advanced at the end.
 template <typename T>
-static int step2_dev(swpc2d_ctx *c)
+static int step2_split(swpc2d_ctx *c)
 {
     int r;
     if ((r = l2_snap<T>(c))) return r;
@@ -584,6 +739,60 @@ static int step2_dev(swpc2d_ctx *c)
     hipLaunchKernelGGL(k2_step_advance, dim3(1), dim3(64), 0, c->stream, (int *)c->dit.p);
     SWPC_CHECK_LAUNCH();
     return SWPC_OK;
+}
+
+// The same step in the fused kernels (k2_stress_step, k2_vel_step): the
+// snapshots, a stress glut or body force that is not fused, and the step
+// counter keep launches of their own.
+template <typename T>
+static int step2_fused(swpc2d_ctx *c)
+{
+    int r;
+    if ((r = l2_snap<T>(c))) return r;
+    const K2<T> a = make_args2<T>(c);
+    const unsigned nint = (unsigned)c->ntx * (unsigned)c->ntz;
+    const unsigned npml = (c->boxes.total + 255) / 256;
+    const int ntx = std::max(c->ntx, 1);
+    const int *dit = (const int *)c->dit.p;
+    Recv2 rv;
+    memset(&rv, 0, sizeof rv);
+    if (c->nst > 0) {
+        rv.nst = c->nst; rv.ntdec_w = c->ntdec_w; rv.ntw = c->ntw;
+        rv.st = (const int *)c->st.p; rv.uacc = (double *)c->uacc.p;
+        rv.wvel = (float *)c->wvel.p; rv.wdisp = (double *)c->wdisp.p;
+    }
+    Src2 sg, sb;
+    memset(&sg, 0, sizeof sg);
+    memset(&sb, 0, sizeof sb);
+    if (c->nsrc > 0 && c->fuse_src) {
+        sg.n = c->nsrc; sg.nt = c->nt_src;
+        sg.ik = (const int *)c->src.p; sg.m = (const double *)c->m3.p; sg.tab = (const double *)c->sdrop.p;
+    }
+    if (c->nbf > 0 && c->fuse_bf) {
+        sb.n = c->nbf; sb.nt = c->nt_bf;
+        sb.ik = (const int *)c->bsrc.p; sb.m = (const double *)c->f2.p; sb.tab = (const double *)c->fdt.p;
+    }
+    const unsigned nrec = ((unsigned)rv.nst + 255u) / 256u;
+    if (nint + npml + nrec > 0) {
+        hipLaunchKernelGGL((k2_stress_step<T>), dim3(nint + npml + nrec), dim3(256), 0, c->stream, a, ntx, nint, npml,
+                           sg, rv, dit);
+        SWPC_CHECK_LAUNCH();
+    }
+    if (!c->fuse_src && (r = l2_glut<T>(c))) return r;
+    if (nint + npml > 0) {
+        hipLaunchKernelGGL((k2_vel_step<T>), dim3(nint + npml), dim3(256), 0, c->stream, a, ntx, nint, sb, dit);
+        SWPC_CHECK_LAUNCH();
+    }
+    if (!c->fuse_bf && (r = l2_bf<T>(c))) return r;
+    hipLaunchKernelGGL(k2_step_advance, dim3(1), dim3(64), 0, c->stream, (int *)c->dit.p);
+    SWPC_CHECK_LAUNCH();
+    return SWPC_OK;
+}
+
+template <typename T>
+static int step2_dev(swpc2d_ctx *c)
+{
+    return c->fused ? step2_fused<T>(c) : step2_split<T>(c);
 }
 
 static int step2_any(swpc2d_ctx *c) { return fp64(c) ? step2_dev<double>(c) : step2_dev<float>(c); }
@@ -860,6 +1069,19 @@ static int pack_src2(swpc2d_ctx *c, int nsrc, const int *is, const int *ks, std:
     return SWPC_OK;
 }
 
+// The fused step kernels apply a source's share in the threads of the
+// updated cells [1, nx] x [1, nz]: a few sources whose footprint (the
+// cell and its -x, -z neighbours) lies there.
+static bool fusable2(const swpc2d_ctx *c, int nsrc, const std::vector<int> &pack)
+{
+    if (nsrc > MAXFUSE) return false;
+    for (int n = 0; n < nsrc; n++) {
+        const int i = pack[2 * n], k = pack[2 * n + 1];
+        if (i < 2 || i > c->d.nx || k < 2 || k > c->d.nz) return false;
+    }
+    return true;
+}
+
 int swpc2d_set_sources(swpc2d_ctx *c, int nsrc, const int *isrc, const int *ksrc, const double *m3, int nt,
                        const double *sdrop)
 {
@@ -881,6 +1103,7 @@ int swpc2d_set_sources(swpc2d_ctx *c, int nsrc, const int *isrc, const int *ksrc
     }
     c->nsrc = nsrc;
     c->nt_src = nt;
+    c->fuse_src = fusable2(c, nsrc, pack);
     return SWPC_OK;
 }
 
@@ -905,6 +1128,7 @@ int swpc2d_set_bodyforce(swpc2d_ctx *c, int nsrc, const int *isrc, const int *ks
     }
     c->nbf = nsrc;
     c->nt_bf = nt;
+    c->fuse_bf = fusable2(c, nsrc, pack);
     return SWPC_OK;
 }
 
@@ -1049,6 +1273,15 @@ int swpc2d_run(swpc2d_ctx *c, int it0, int nsteps)
     if (int r = ready2(c)) return r;
     if (it0 < 1 || nsteps < 0) return fail(SWPC_EINVAL, "bad step range");
     return run_steps2(c, it0, nsteps);
+}
+
+int swpc2d_set_fusion(swpc2d_ctx *c, int on)
+{
+    clear_error();
+    if (!c) return fail(SWPC_EINVAL, "null context");
+    graph_drop2(c);
+    c->fused = on != 0;
+    return SWPC_OK;
 }
 
 int swpc2d_set_graph(swpc2d_ctx *c, int steps)

@@ -1198,11 +1198,46 @@ __device__ inline bool pml_cell(const Boxes3 &B, unsigned id, int &i, int &j, in
     return true;
 }
 
-template <typename T, bool S16>
-__global__ void __launch_bounds__(256) k3_stress_pml(const K3<T> a)
+// The absorber boxes that span whole rows i in [1, nx] (the y faces and
+// the bottom), walked one 256-cell row segment per workgroup: j and k are
+// then uniform across the workgroup, and so are the y and z profiles.
+struct Rows3 {
+    int n, nseg;
+    int j0[5], nj[5], k0[5];
+    unsigned start[5];  // first row of each box
+    unsigned total;     // rows
+};
+
+// Workgroups go to the XCDs round robin.  With per != 0 (a grid of
+// 8 * per workgroups) XCD x takes the contiguous run of logical workgroups
+// [x * per, (x + 1) * per), so that the neighbour rows and planes a cell
+// reads were fetched into that XCD's L2 by its own neighbours; the
+// logical ids past the launch's count exit at once.
+__device__ __forceinline__ unsigned pml_block(unsigned per)
 {
-    int i, j, k;
-    if (!pml_cell(a.pml, blockIdx.x * 256u + threadIdx.x, i, j, k)) return;
+    const unsigned b = blockIdx.x;
+    return per ? (b & 7u) * per + (b >> 3) : b;
+}
+
+__device__ __forceinline__ bool pml_row(const Rows3 &R, unsigned per, int nx, int &i, int &j, int &k)
+{
+    const unsigned blk = pml_block(per);
+    const unsigned row = blk / (unsigned)R.nseg;
+    if (row >= R.total) return false;
+    i = 1 + (int)(blk - row * (unsigned)R.nseg) * 256 + (int)threadIdx.x;
+    int b = 0;
+#pragma unroll
+    for (int q = 1; q < 5; q++)
+        if (q < R.n && row >= R.start[q]) b = q;
+    const unsigned l = row - R.start[b];
+    j = R.j0[b] + (int)(l % (unsigned)R.nj[b]);
+    k = R.k0[b] + (int)(l / (unsigned)R.nj[b]);
+    return i <= nx;
+}
+
+template <typename T, bool S16>
+__device__ __forceinline__ void stress_pml_cell(const K3<T> &a, const int i, const int j, const int k)
+{
     const int px = a.px, sxy = a.sxy;
     const int c = at3(px, sxy, i, j, k);
     const T *__restrict__ Vx = a.V[0];
@@ -1283,10 +1318,22 @@ __global__ void __launch_bounds__(256) k3_stress_pml(const K3<T> a)
 }
 
 template <typename T, bool S16>
-__global__ void __launch_bounds__(256) k3_vel_pml(const K3<T> a)
+__global__ void __launch_bounds__(256) k3_stress_pml(const K3<T> a, unsigned per)
 {
     int i, j, k;
-    if (!pml_cell(a.pml, blockIdx.x * 256u + threadIdx.x, i, j, k)) return;
+    if (pml_cell(a.pml, pml_block(per) * 256u + threadIdx.x, i, j, k)) stress_pml_cell<T, S16>(a, i, j, k);
+}
+
+template <typename T, bool S16>
+__global__ void __launch_bounds__(256) k3_stress_pml_rows(const K3<T> a, const Rows3 R, unsigned per)
+{
+    int i, j, k;
+    if (pml_row(R, per, a.nx, i, j, k)) stress_pml_cell<T, S16>(a, i, j, k);
+}
+
+template <typename T, bool S16>
+__device__ __forceinline__ void vel_pml_cell(const K3<T> &a, const int i, const int j, const int k)
+{
     const int px = a.px, sxy = a.sxy;
     const int c = at3(px, sxy, i, j, k);
     const T *__restrict__ Sxx = a.S[0];
@@ -1352,6 +1399,20 @@ __global__ void __launch_bounds__(256) k3_vel_pml(const K3<T> a)
     if (lx) stA<S16>(a, 15, c, gxc3 * axsxz + gxc4 * dxSxz * dt);
     if (ly) stA<S16>(a, 16, c, gyc3 * aysyz + gyc4 * dySyz * dt);
     if (lz) stA<S16>(a, 17, c, gze3 * azszz + gze4 * dzSzz * dt);
+}
+
+template <typename T, bool S16>
+__global__ void __launch_bounds__(256) k3_vel_pml(const K3<T> a, unsigned per)
+{
+    int i, j, k;
+    if (pml_cell(a.pml, pml_block(per) * 256u + threadIdx.x, i, j, k)) vel_pml_cell<T, S16>(a, i, j, k);
+}
+
+template <typename T, bool S16>
+__global__ void __launch_bounds__(256) k3_vel_pml_rows(const K3<T> a, const Rows3 R, unsigned per)
+{
+    int i, j, k;
+    if (pml_row(R, per, a.nx, i, j, k)) vel_pml_cell<T, S16>(a, i, j, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -1876,6 +1937,8 @@ struct swpc3d_ctx {
     int nst = 0, ntdec_w = 1, ntw = 0;
     bool have_medium = false, have_surface = false, have_pml = false;
     swpc::Boxes3 boxes;
+    swpc::Boxes3 boxes_x;   // the boxes not walked by rows (the x faces)
+    swpc::Rows3 rows;       // the boxes of whole rows i in [1, nx]
     swpc::Boxes4 boxes4;
     int gxp = 0;
     int ti0 = 1;
@@ -1887,6 +1950,8 @@ struct swpc3d_ctx {
     int x4_vz = 0;          // 4-wide velocity: 0 one plane per workgroup row, > 0 z-marching chunk length
     int x4_sz = 0;          // 4-wide stress: > 0 one-pass z-marching kernel with this chunk length
     int pml4 = 0;           // 4-wide PML kernels (fewer waves per SIMD; off by default)
+    int pml_rows = 1;       // whole-row absorber boxes by row segments (tuning entry 19)
+    int pml_xcd = 0;        // PML workgroups XCD-contiguous (tuning entry 20)
     bool ade_sparse = true; // no ADE array has been set from outside since creation
     // The PML kernels run on a second stream, concurrently with the
     // interior kernels of the same half-step (they write disjoint cells and
@@ -1969,6 +2034,7 @@ static K3<T> make_args(swpc3d_ctx *c)
     a.ade_sparse = c->ade_sparse ? 1 : 0;
     return a;
 }
+
 
 // Interior tiling for a 64 x by tile and a z-chunk length kc (0 = chosen
 // so that the launch holds at least ~4096 workgroups, 16 per CU).
@@ -2225,6 +2291,14 @@ static int launch_lab(swpc3d_ctx *c, int phase)
     return prof_end(c->prof, cls, c->stream, e0);
 }
 
+// Workgroups of a PML launch of nblk logical workgroups: with xcd, 8 runs
+// of per = ceil(nblk / 8) (pml_block), else nblk in order (per = 0).
+static unsigned pml_launch(unsigned nblk, int xcd, unsigned &per)
+{
+    per = xcd ? (nblk + 7u) / 8u : 0u;
+    return xcd ? 8u * per : nblk;
+}
+
 template <typename T>
 static int launch_stress(swpc3d_ctx *c)
 {
@@ -2248,10 +2322,18 @@ static int launch_stress(swpc3d_ctx *c)
             }
         }
         if (!done) {
-            if (c->s16)
-                hipLaunchKernelGGL((k3_stress_pml<T, true>), dim3((c->boxes.total + 255) / 256), dim3(256), 0, ps, a);
-            else
-                hipLaunchKernelGGL((k3_stress_pml<T, false>), dim3((c->boxes.total + 255) / 256), dim3(256), 0, ps, a);
+            K3<T> b = a;
+            if (c->pml_rows && c->rows.total > 0) {
+                unsigned per, nb = pml_launch(c->rows.total * (unsigned)c->rows.nseg, c->pml_xcd, per);
+                if (c->s16) hipLaunchKernelGGL((k3_stress_pml_rows<T, true>), dim3(nb), dim3(256), 0, ps, a, c->rows, per);
+                else hipLaunchKernelGGL((k3_stress_pml_rows<T, false>), dim3(nb), dim3(256), 0, ps, a, c->rows, per);
+                b.pml = c->boxes_x;
+            }
+            if (b.pml.total > 0) {
+                unsigned per, nb = pml_launch((b.pml.total + 255) / 256, c->pml_xcd, per);
+                if (c->s16) hipLaunchKernelGGL((k3_stress_pml<T, true>), dim3(nb), dim3(256), 0, ps, b, per);
+                else hipLaunchKernelGGL((k3_stress_pml<T, false>), dim3(nb), dim3(256), 0, ps, b, per);
+            }
         }
         SWPC_CHECK_LAUNCH();
         if (int r = prof_end(c->prof, 1, ps, e0)) return r;
@@ -2308,10 +2390,18 @@ static int launch_vel(swpc3d_ctx *c)
             }
         }
         if (!done) {
-            if (c->s16)
-                hipLaunchKernelGGL((k3_vel_pml<T, true>), dim3((c->boxes.total + 255) / 256), dim3(256), 0, ps, a);
-            else
-                hipLaunchKernelGGL((k3_vel_pml<T, false>), dim3((c->boxes.total + 255) / 256), dim3(256), 0, ps, a);
+            K3<T> b = a;
+            if (c->pml_rows && c->rows.total > 0) {
+                unsigned per, nb = pml_launch(c->rows.total * (unsigned)c->rows.nseg, c->pml_xcd, per);
+                if (c->s16) hipLaunchKernelGGL((k3_vel_pml_rows<T, true>), dim3(nb), dim3(256), 0, ps, a, c->rows, per);
+                else hipLaunchKernelGGL((k3_vel_pml_rows<T, false>), dim3(nb), dim3(256), 0, ps, a, c->rows, per);
+                b.pml = c->boxes_x;
+            }
+            if (b.pml.total > 0) {
+                unsigned per, nb = pml_launch((b.pml.total + 255) / 256, c->pml_xcd, per);
+                if (c->s16) hipLaunchKernelGGL((k3_vel_pml<T, true>), dim3(nb), dim3(256), 0, ps, b, per);
+                else hipLaunchKernelGGL((k3_vel_pml<T, false>), dim3(nb), dim3(256), 0, ps, b, per);
+            }
         }
         SWPC_CHECK_LAUNCH();
         if (int r = prof_end(c->prof, 3, ps, e0)) return r;
@@ -2670,6 +2760,26 @@ int swpc3d_create(const swpc3d_desc *desc, swpc3d_ctx **out)
         add(1, d.nx, d.na + 1, d.ny - d.na, d.nz - d.na + 1, d.nz);
         add(1, d.na, d.na + 1, d.ny - d.na, 1, d.nz - d.na);
         add(d.nx - d.na + 1, d.nx, d.na + 1, d.ny - d.na, 1, d.nz - d.na);
+    }
+    // whole-row boxes as rows, the others as they are
+    Rows3 &RW = c->rows;
+    Boxes3 &BX = c->boxes_x;
+    memset(&RW, 0, sizeof RW);
+    memset(&BX, 0, sizeof BX);
+    RW.nseg = (d.nx + 255) / 256;
+    for (int q = 0; q < B.n; q++) {
+        const Box3 &x = B.b[q];
+        if (x.i0 == 1 && x.ni == d.nx) {
+            RW.j0[RW.n] = x.j0; RW.nj[RW.n] = x.nj; RW.k0[RW.n] = x.k0;
+            RW.start[RW.n] = RW.total;
+            RW.total += (unsigned)x.nj * (unsigned)x.nk;
+            RW.n++;
+        } else {
+            BX.b[BX.n] = x;
+            BX.b[BX.n].start = BX.total;
+            BX.total += (unsigned)x.ni * (unsigned)x.nj * (unsigned)x.nk;
+            BX.n++;
+        }
     }
     // the same boxes in groups of 4 x-cells for the 4-wide PML kernels
     Boxes4 &B4 = c->boxes4;
@@ -3454,9 +3564,10 @@ int swpc3d_set_tuning(swpc3d_ctx *c, const int *p, int n)
     clear_error();
     if (c) graph_drop(c);  // the captured steps would launch stale arguments
     if (!c || (n > 0 && !p)) return fail(SWPC_EINVAL, "null argument");
-    int v[19] = {c->s_by, c->s_wpe, c->s_split, c->v_by, c->v_wpe, c->xcd, c->s_kc, c->v_kc, c->x4, c->x4_by, c->pml4,
-                 c->x4_form, c->x4_vby, c->conc, c->x4_vz, c->x4_sz, c->graph_k, c->sr, c->r_shift};
-    for (int q = 0; q < n && q < 19; q++) v[q] = p[q];
+    int v[21] = {c->s_by, c->s_wpe, c->s_split, c->v_by, c->v_wpe, c->xcd, c->s_kc, c->v_kc, c->x4, c->x4_by, c->pml4,
+                 c->x4_form, c->x4_vby, c->conc, c->x4_vz, c->x4_sz, c->graph_k, c->sr, c->r_shift, c->pml_rows,
+                 c->pml_xcd};
+    for (int q = 0; q < n && q < 21; q++) v[q] = p[q];
     if (v[18] < -24 || v[18] > 24) return fail(SWPC_EINVAL, "memory-variable scale shift must be in -24..24");
     if (v[16] < 0 || v[16] > 1024) return fail(SWPC_EINVAL, "steps per graph must be in 0..1024");
     if (v[14] < 0 || v[15] < 0) return fail(SWPC_EINVAL, "negative z-chunk");
@@ -3470,6 +3581,8 @@ int swpc3d_set_tuning(swpc3d_ctx *c, const int *p, int n)
     c->x4_form = v[11]; c->x4_vby = v[12]; c->conc = v[13] != 0; c->x4_vz = v[14]; c->x4_sz = v[15];
     c->graph_k = v[16];
     c->sr = v[17] != 0;
+    c->pml_rows = v[19] != 0;
+    c->pml_xcd = v[20] != 0;
     if (v[18] != c->r_shift) {
         // the stored memory variables are rescaled with the new scale
         if (int r = rescale_r(c, std::ldexp(1.0f, c->r_shift - v[18]))) return r;

@@ -439,6 +439,11 @@ class SwpcPSV:
         """Steps per captured HIP graph in run() (0: step by step)."""
         check(lib().swpc2d_set_graph(self.h, int(steps)), "set_graph")
 
+    def set_fusion(self, on):
+        """Steps in the two fused kernels (True, the default) or in the
+        separate per-routine kernels (False); the results are the same."""
+        check(lib().swpc2d_set_fusion(self.h, int(bool(on))), "set_fusion")
+
     def close(self):
         if getattr(self, "h", None):
             lib().swpc2d_destroy(self.h)

@@ -234,3 +234,34 @@ def test_divergence_detected():
     with Swpc3D(ok.grid, ok.medium, ok.pml, ok.glut, ok.stations, 1, 100) as sim:
         sim.run_monitored(1, 100, ntdec_r=25, log=lines.append)
     assert len(lines) == 4
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("where", ["interior", "absorber", "edge", "bodyforce"])
+def test_psv_fused_step_equals_split(bits, where):
+    """swpc2d_run through the two fused step kernels against the separate
+    per-routine kernels, bit for bit (traces, displacement, every field):
+    a stress glut in the interior, one inside the absorber (applied by the
+    absorber threads), one on the first column (its footprint leaves the
+    updated cells, so the separate glut kernel applies it), and a body
+    force."""
+    nt = 80
+    g0 = problem.case2d(160, 80, 0.25, 0.02, nt, na=10, src_depth=5.0, zbeg=-2.0).grid
+    src_x = {"interior": 0.0, "absorber": g0.xbeg + 4.5 * g0.dx, "edge": g0.xbeg + 0.5 * g0.dx,
+             "bodyforce": 0.0}[where]
+    case = problem.case2d(160, 80, 0.25, 0.02, nt, na=10, src_depth=5.0, zbeg=-2.0, nst=8, st_spacing=2.0,
+                          src_x=src_x, bodyforce=where == "bodyforce")
+    src = case.bforce if where == "bodyforce" else case.glut
+    say(f"P-SV fused vs split fp{bits}: {where}, source cell {tuple(src.idx[0])}")
+    out = []
+    for fused in (True, False):
+        with SwpcPSV(case.grid, case.medium, case.pml, case.glut, case.stations, 1, nt,
+                     precision="fp32" if bits == 32 else "fp64", bodyforce=case.bforce) as sim:
+            sim.set_fusion(fused)
+            sim.run(1, nt)
+            vel, disp = sim.receivers()
+            out.append((vel, disp, {f: sim.field(f) for f in FIELDS2}))
+    assert np.abs(out[0][0]).max() > 0
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    for f in FIELDS2:
+        assert np.array_equal(out[0][2][f], out[1][2][f]), f
