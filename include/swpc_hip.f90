@@ -52,7 +52,7 @@ module swpc_hip
     public :: swpcsh_step, swpcsh_set_graph, swpcsh_run, swpcsh_get_receivers, swpcsh_vmax, swpcsh_sync
     public :: swpc_lab_nparam, swpc3d_set_lab, swpc3d_set_lab_sponge
     public :: swpc_batch_create, swpc_batch_destroy, swpc_batch_add3d, swpc_batch_add2d, swpc_batch_run
-    public :: swpc_batch_sync
+    public :: swpc_batch_sync, swpc_batch_set_grouping
 
     interface
 
@@ -625,6 +625,12 @@ module swpc_hip
             import :: c_int, c_ptr
             type(c_ptr), value :: b
             integer(c_int), value :: it0, nsteps, graph_k
+        end function
+
+        integer(c_int) function swpc_batch_set_grouping(b, on) bind(C, name='swpc_batch_set_grouping')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: b
+            integer(c_int), value :: on
         end function
 
         integer(c_int) function swpc_batch_sync(b) bind(C, name='swpc_batch_sync')

@@ -518,6 +518,11 @@ int swpc_batch_destroy(swpc_batch *b);
 int swpc_batch_add3d(swpc_batch *b, swpc3d_ctx *ctx);
 int swpc_batch_add2d(swpc_batch *b, swpc2d_ctx *ctx);
 int swpc_batch_run(swpc_batch *b, int it0, int nsteps, int graph_k);
+/* FP32 P-SV members with fused steps, no snapshot streams and their
+   sources inside the fused kernels advance as one group (on != 0, the
+   default): one launch per phase for all of them instead of one graph
+   branch each.  Results are the same bit for bit either way. */
+int swpc_batch_set_grouping(swpc_batch *b, int on);
 int swpc_batch_sync(swpc_batch *b);
 
 #ifdef __cplusplus

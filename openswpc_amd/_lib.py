@@ -92,6 +92,7 @@ _PROTOS = {
     "swpc_batch_add2d": (I, [P, P]),
     "swpc_batch_run": (I, [P, I, I, I]),
     "swpc_batch_sync": (I, [P]),
+    "swpc_batch_set_grouping": (I, [P, I]),
     "swpcsh_create": (I, [C.POINTER(Desc2), C.POINTER(P)]),
     "swpcsh_destroy": (I, [P]),
     "swpcsh_set_medium": (I, [P, PF, PF, PF]),

@@ -41,6 +41,11 @@ class Batch:
     def run(self, it0, nsteps, graph_k=16):
         check(lib().swpc_batch_run(self.h, it0, nsteps, graph_k), "swpc_batch_run")
 
+    def set_grouping(self, on):
+        """FP32 P-SV members in one group, one launch per phase for all of
+        them (True, the default), or one graph branch each (False)."""
+        check(lib().swpc_batch_set_grouping(self.h, int(bool(on))), "swpc_batch_set_grouping")
+
     def sync(self):
         check(lib().swpc_batch_sync(self.h), "swpc_batch_sync")
 
@@ -147,14 +152,33 @@ def cells(cases):
     return n
 
 
+def alg_bytes_per_step(cases):
+    """Algorithmic HBM bytes of one step of every case, FP32: P-SV 152 B per
+    interior and 136 B per absorber cell, 3-D 292 B and 284 B (the stress
+    and velocity updates' compulsory traffic, m_kernel.f90 and
+    m_absorb_p.f90 under the two-phase order)."""
+    n = 0
+    for kind, c in cases:
+        g = c.grid
+        if kind == "3d":
+            ni = g.n_interior_cells()
+            n += 292 * ni + 284 * (g.nx * g.ny * g.nz - ni)
+        else:
+            ni = max(g.nx - 2 * g.na, 0) * max(g.nz - g.na, 0)
+            n += 152 * ni + 136 * (g.nx * g.nz - ni)
+    return n
+
+
 def measure(cases, steps, graph_k=16, tuning=None, log=None):
-    """Time `steps` steps of every case three ways on fresh contexts:
-    'batch' (one graph per graph_k steps of all members), 'streams' (each
+    """Time `steps` steps of every case four ways on fresh contexts:
+    'batch' (one graph per graph_k steps of all members, the P-SV members
+    in one group, one launch per phase for all of them), 'branches' (the
+    same graph with every member a branch of its own), 'streams' (each
     member's own run, all enqueued before one wait: the members overlap on
     their own streams) and 'sequential' (one member after the other).
     Returns {mode: seconds} plus the total cells."""
     res = {}
-    for mode in ("batch", "streams", "sequential"):
+    for mode in ("batch", "branches", "streams", "sequential"):
         sims = []
         for kind, c in cases:
             s = open_case(kind, c)
@@ -162,8 +186,9 @@ def measure(cases, steps, graph_k=16, tuning=None, log=None):
                 s.set_tuning(*tuning[size_class(c.grid)])
             sims.append(s)
         warm = min(graph_k or 16, steps)
-        if mode == "batch":
+        if mode in ("batch", "branches"):
             b = Batch()
+            b.set_grouping(mode == "batch")
             for s in sims:
                 b.add(s)
             b.run(1, warm, graph_k)
@@ -208,20 +233,26 @@ def _main(argv=None):
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--graph-k", type=int, default=16)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--side2", type=int, nargs=2, default=(256, 1024), help="P-SV side range (cells)")
+    ap.add_argument("--side3", type=int, nargs=2, default=(96, 256), help="3-D side range (cells)")
     a = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     t0 = time.time()
-    cases = list(mixed_cases(a.seed, a.n2d, a.n3d, a.steps))
+    cases = list(mixed_cases(a.seed, a.n2d, a.n3d, a.steps, side2=tuple(a.side2), side3=tuple(a.side3)))
     log(f"batch: {len(cases)} cases built in {time.time() - t0:.1f} s")
     tuning = None if a.no_tune else tune_members(cases, log=log)
     res = measure(cases, a.steps, a.graph_k, tuning, log)
     n = res["cells"] * res["timed_steps"]
-    out = {"workload": f"{a.n2d} P-SV (256-1024 per side) + {a.n3d} 3-D (96-256 per side) runs, seed {a.seed}, "
-                       f"{a.steps} steps each",
+    ab = alg_bytes_per_step(cases) * res["timed_steps"]
+    out = {"workload": f"{a.n2d} P-SV ({a.side2[0]}-{a.side2[1]} per side) + {a.n3d} 3-D ({a.side3[0]}-{a.side3[1]} "
+                       f"per side) runs, seed {a.seed}, {a.steps} steps each",
            "cells": res["cells"], "timed_steps": res["timed_steps"], "graph_k": a.graph_k,
            "tuned_classes": len(tuning or {}),
-           **{f"{m}_cell_updates_per_s": n / res[m] for m in ("batch", "streams", "sequential")},
-           "speedup_vs_sequential": res["sequential"] / res["batch"]}
+           **{f"{m}_cell_updates_per_s": n / res[m] for m in ("batch", "branches", "streams", "sequential")},
+           "speedup_vs_sequential": res["sequential"] / res["batch"],
+           "speedup_vs_branches": res["branches"] / res["batch"],
+           "alg_bytes_per_step": alg_bytes_per_step(cases),
+           "batch_alg_GBs": ab / res["batch"] / 1e9, "batch_hbm_frac": ab / res["batch"] / 8e12}
     print(json.dumps(out))
 
 

@@ -419,10 +419,10 @@ __device__ __forceinline__ void bf_at2(const K2<T> &a, const Src2 &s, const int 
 // workgroups: [0, nint) interior tiles (ntx per tile row), [nint, nint +
 // npml) absorber cells, then receivers
 template <typename T>
-__global__ void __launch_bounds__(256) k2_stress_step(const K2<T> a, int ntx, unsigned nint, unsigned npml, Src2 s,
-                                                       Recv2 r, const int *__restrict__ dit)
+__device__ __forceinline__ void stress_step_block(const K2<T> &a, int ntx, unsigned nint, unsigned npml,
+                                                  const Src2 &s, const Recv2 &r, const int *__restrict__ dit,
+                                                  const unsigned b)
 {
-    const unsigned b = blockIdx.x;
     int i, k;
     if (b < nint) {
         if (!int_cell2(a, (int)(b % (unsigned)ntx), (int)(b / (unsigned)ntx), (int)(threadIdx.x & 63),
@@ -441,10 +441,9 @@ __global__ void __launch_bounds__(256) k2_stress_step(const K2<T> a, int ntx, un
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k2_vel_step(const K2<T> a, int ntx, unsigned nint, Src2 s,
-                                                    const int *__restrict__ dit)
+__device__ __forceinline__ void vel_step_block(const K2<T> &a, int ntx, unsigned nint, const Src2 &s,
+                                               const int *__restrict__ dit, const unsigned b)
 {
-    const unsigned b = blockIdx.x;
     int i, k;
     if (b < nint) {
         if (!int_cell2(a, (int)(b % (unsigned)ntx), (int)(b / (unsigned)ntx), (int)(threadIdx.x & 63),
@@ -456,6 +455,72 @@ __global__ void __launch_bounds__(256) k2_vel_step(const K2<T> a, int ntx, unsig
         vel_pml_cell2(a, i, k);
     }
     if (s.n) bf_at2(a, s, dit, i, k);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k2_stress_step(const K2<T> a, int ntx, unsigned nint, unsigned npml, Src2 s,
+                                                       Recv2 r, const int *__restrict__ dit)
+{
+    stress_step_block(a, ntx, nint, npml, s, r, dit, blockIdx.x);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k2_vel_step(const K2<T> a, int ntx, unsigned nint, Src2 s,
+                                                    const int *__restrict__ dit)
+{
+    vel_step_block(a, ntx, nint, s, dit, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Groups of P-SV runs in one launch per phase (batches): the workgroups of
+// every member's fused step kernel, end to end.  A workgroup finds its
+// member by bisection of the members' first workgroups and runs that
+// member's block of k2_stress_step / k2_vel_step, with the member's own
+// arguments, sources, receivers and step counter, so every member's cells
+// see exactly the operations of its own run.
+template <typename T>
+struct Member2 {
+    K2<T> a;
+    Src2 sg, sb;
+    Recv2 rv;
+    int *dit;
+    int ntx;
+    unsigned nint, npml;
+};
+
+__device__ __forceinline__ int member_of(const unsigned *__restrict__ first, int nm, unsigned b)
+{
+    int lo = 0, hi = nm - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k2_stress_group(const Member2<T> *__restrict__ mem,
+                                                        const unsigned *__restrict__ first, int nm)
+{
+    const int m = member_of(first, nm, blockIdx.x);
+    const Member2<T> &g = mem[m];
+    stress_step_block(g.a, g.ntx, g.nint, g.npml, g.sg, g.rv, g.dit, blockIdx.x - first[m]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k2_vel_group(const Member2<T> *__restrict__ mem,
+                                                     const unsigned *__restrict__ first, int nm)
+{
+    const int m = member_of(first, nm, blockIdx.x);
+    const Member2<T> &g = mem[m];
+    vel_step_block(g.a, g.ntx, g.nint, g.sb, g.dit, blockIdx.x - first[m]);
+}
+
+__global__ void k2_group_advance(int *const *__restrict__ dits, int n)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) *dits[q] = *dits[q] + 1;
 }
 
 __global__ void k2_fill_u32(unsigned *p, int n, unsigned v)
@@ -741,6 +806,34 @@ static int step2_split(swpc2d_ctx *c)
     return SWPC_OK;
 }
 
+// Arguments of the fused step kernels for one context.
+template <typename T>
+static void member2_args(swpc2d_ctx *c, Member2<T> &g)
+{
+    memset(&g, 0, sizeof g);
+    g.a = make_args2<T>(c);
+    g.nint = (unsigned)c->ntx * (unsigned)c->ntz;
+    g.npml = (c->boxes.total + 255) / 256;
+    g.ntx = std::max(c->ntx, 1);
+    g.dit = (int *)c->dit.p;
+    if (c->nst > 0) {
+        g.rv.nst = c->nst; g.rv.ntdec_w = c->ntdec_w; g.rv.ntw = c->ntw;
+        g.rv.st = (const int *)c->st.p; g.rv.uacc = (double *)c->uacc.p;
+        g.rv.wvel = (float *)c->wvel.p; g.rv.wdisp = (double *)c->wdisp.p;
+    }
+    if (c->nsrc > 0 && c->fuse_src) {
+        g.sg.n = c->nsrc; g.sg.nt = c->nt_src;
+        g.sg.ik = (const int *)c->src.p; g.sg.m = (const double *)c->m3.p; g.sg.tab = (const double *)c->sdrop.p;
+    }
+    if (c->nbf > 0 && c->fuse_bf) {
+        g.sb.n = c->nbf; g.sb.nt = c->nt_bf;
+        g.sb.ik = (const int *)c->bsrc.p; g.sb.m = (const double *)c->f2.p; g.sb.tab = (const double *)c->fdt.p;
+    }
+}
+
+template <typename T>
+static unsigned stress_blocks2(const Member2<T> &g) { return g.nint + g.npml + ((unsigned)g.rv.nst + 255u) / 256u; }
+
 // The same step in the fused kernels (k2_stress_step, k2_vel_step): the
 // snapshots, a stress glut or body force that is not fused, and the step
 // counter keep launches of their own.
@@ -749,38 +842,18 @@ static int step2_fused(swpc2d_ctx *c)
 {
     int r;
     if ((r = l2_snap<T>(c))) return r;
-    const K2<T> a = make_args2<T>(c);
-    const unsigned nint = (unsigned)c->ntx * (unsigned)c->ntz;
-    const unsigned npml = (c->boxes.total + 255) / 256;
-    const int ntx = std::max(c->ntx, 1);
-    const int *dit = (const int *)c->dit.p;
-    Recv2 rv;
-    memset(&rv, 0, sizeof rv);
-    if (c->nst > 0) {
-        rv.nst = c->nst; rv.ntdec_w = c->ntdec_w; rv.ntw = c->ntw;
-        rv.st = (const int *)c->st.p; rv.uacc = (double *)c->uacc.p;
-        rv.wvel = (float *)c->wvel.p; rv.wdisp = (double *)c->wdisp.p;
-    }
-    Src2 sg, sb;
-    memset(&sg, 0, sizeof sg);
-    memset(&sb, 0, sizeof sb);
-    if (c->nsrc > 0 && c->fuse_src) {
-        sg.n = c->nsrc; sg.nt = c->nt_src;
-        sg.ik = (const int *)c->src.p; sg.m = (const double *)c->m3.p; sg.tab = (const double *)c->sdrop.p;
-    }
-    if (c->nbf > 0 && c->fuse_bf) {
-        sb.n = c->nbf; sb.nt = c->nt_bf;
-        sb.ik = (const int *)c->bsrc.p; sb.m = (const double *)c->f2.p; sb.tab = (const double *)c->fdt.p;
-    }
-    const unsigned nrec = ((unsigned)rv.nst + 255u) / 256u;
-    if (nint + npml + nrec > 0) {
-        hipLaunchKernelGGL((k2_stress_step<T>), dim3(nint + npml + nrec), dim3(256), 0, c->stream, a, ntx, nint, npml,
-                           sg, rv, dit);
+    Member2<T> g;
+    member2_args(c, g);
+    const unsigned ns = stress_blocks2(g), nv = g.nint + g.npml;
+    if (ns > 0) {
+        hipLaunchKernelGGL((k2_stress_step<T>), dim3(ns), dim3(256), 0, c->stream, g.a, g.ntx, g.nint, g.npml, g.sg,
+                           g.rv, (const int *)g.dit);
         SWPC_CHECK_LAUNCH();
     }
     if (!c->fuse_src && (r = l2_glut<T>(c))) return r;
-    if (nint + npml > 0) {
-        hipLaunchKernelGGL((k2_vel_step<T>), dim3(nint + npml), dim3(256), 0, c->stream, a, ntx, nint, sb, dit);
+    if (nv > 0) {
+        hipLaunchKernelGGL((k2_vel_step<T>), dim3(nv), dim3(256), 0, c->stream, g.a, g.ntx, g.nint, g.sb,
+                           (const int *)g.dit);
         SWPC_CHECK_LAUNCH();
     }
     if (!c->fuse_bf && (r = l2_bf<T>(c))) return r;
@@ -872,6 +945,84 @@ int batch_check2(swpc2d_ctx *c, int it0, int n)
 int batch_step2(swpc2d_ctx *c) { return step2_any(c); }
 
 void batch_done2(swpc2d_ctx *c, int it0, int it1) { note_snapshots2(c, it0, it1); }
+
+struct Group2 {
+    DBuf mem, first_s, first_v, dits;
+    unsigned ns = 0, nv = 0;
+    int nm = 0;
+};
+
+bool group2_eligible(const swpc2d_ctx *c)
+{
+    return c && !fp64(c) && c->fused && c->snaps.empty() && (c->nsrc == 0 || c->fuse_src) &&
+           (c->nbf == 0 || c->fuse_bf) && ready2(c) == SWPC_OK;
+}
+
+void group2_free(Group2 *g)
+{
+    if (!g) return;
+    dfree(g->mem);
+    dfree(g->first_s);
+    dfree(g->first_v);
+    dfree(g->dits);
+    delete g;
+}
+
+int group2_build(const std::vector<swpc2d_ctx *> &cs, Group2 **out, long long &bytes)
+{
+    *out = nullptr;
+    if (cs.empty()) return SWPC_OK;
+    std::vector<Member2<float>> mem(cs.size());
+    std::vector<unsigned> fs(cs.size()), fv(cs.size());
+    std::vector<int *> dits(cs.size());
+    unsigned long long ns = 0, nv = 0;
+    for (size_t q = 0; q < cs.size(); q++) {
+        if (!group2_eligible(cs[q])) return fail(SWPC_EINVAL, "P-SV member %zu cannot join a group", q);
+        member2_args(cs[q], mem[q]);
+        fs[q] = (unsigned)ns;
+        fv[q] = (unsigned)nv;
+        ns += stress_blocks2(mem[q]);
+        nv += mem[q].nint + mem[q].npml;
+        dits[q] = mem[q].dit;
+    }
+    if (ns >= (1ULL << 31)) return fail(SWPC_EINVAL, "P-SV group too large for one launch");
+    auto *g = new Group2();
+    g->nm = (int)cs.size();
+    g->ns = (unsigned)ns;
+    g->nv = (unsigned)nv;
+    int r = dalloc(g->mem, mem.size() * sizeof(Member2<float>), bytes);
+    if (!r) r = dalloc(g->first_s, fs.size() * 4, bytes);
+    if (!r) r = dalloc(g->first_v, fv.size() * 4, bytes);
+    if (!r) r = dalloc(g->dits, dits.size() * sizeof(int *), bytes);
+    hipError_t e = hipSuccess;
+    if (!r) e = hipMemcpy(g->mem.p, mem.data(), mem.size() * sizeof(Member2<float>), hipMemcpyHostToDevice);
+    if (!r && e == hipSuccess) e = hipMemcpy(g->first_s.p, fs.data(), fs.size() * 4, hipMemcpyHostToDevice);
+    if (!r && e == hipSuccess) e = hipMemcpy(g->first_v.p, fv.data(), fv.size() * 4, hipMemcpyHostToDevice);
+    if (!r && e == hipSuccess) e = hipMemcpy(g->dits.p, dits.data(), dits.size() * sizeof(int *), hipMemcpyHostToDevice);
+    if (r || e != hipSuccess) {
+        group2_free(g);
+        return r ? r : fail(SWPC_EHIP, "P-SV group tables: %s", hipGetErrorString(e));
+    }
+    *out = g;
+    return SWPC_OK;
+}
+
+int group2_step(Group2 *g, hipStream_t s)
+{
+    if (g->ns > 0) {
+        hipLaunchKernelGGL((k2_stress_group<float>), dim3(g->ns), dim3(256), 0, s, (const Member2<float> *)g->mem.p,
+                           (const unsigned *)g->first_s.p, g->nm);
+        SWPC_CHECK_LAUNCH();
+    }
+    if (g->nv > 0) {
+        hipLaunchKernelGGL((k2_vel_group<float>), dim3(g->nv), dim3(256), 0, s, (const Member2<float> *)g->mem.p,
+                           (const unsigned *)g->first_v.p, g->nm);
+        SWPC_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(k2_group_advance, dim3((g->nm + 63) / 64), dim3(64), 0, s, (int *const *)g->dits.p, g->nm);
+    SWPC_CHECK_LAUNCH();
+    return SWPC_OK;
+}
 
 template <typename S, typename D>
 static void box2_to_dev(const Lay2 &L, const S *src, D *dst)

@@ -4,6 +4,8 @@
 #pragma once
 #include "swpc_common.h"
 
+#include <vector>
+
 namespace swpc {
 
 // What a batch needs of one member context.
@@ -22,5 +24,14 @@ int batch_member2(swpc2d_ctx *c, BatchMember &m);
 int batch_check2(swpc2d_ctx *c, int it0, int n);
 int batch_step2(swpc2d_ctx *c);
 void batch_done2(swpc2d_ctx *c, int it0, int it1);
+
+// A group of P-SV members stepped together, one launch per phase for all
+// of them (FP32, fused steps, no snapshots, sources inside the fused
+// kernels); results equal each member's own steps bit for bit.
+struct Group2;
+bool group2_eligible(const swpc2d_ctx *c);
+int group2_build(const std::vector<swpc2d_ctx *> &cs, Group2 **out, long long &bytes);
+int group2_step(Group2 *g, hipStream_t s);   // one step of every member, enqueued on s
+void group2_free(Group2 *g);
 
 }  // namespace swpc

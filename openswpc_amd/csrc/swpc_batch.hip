@@ -4,7 +4,10 @@
 // replayed with one launch per graph_k steps.  Each member keeps its own
 // stream, step counter and state, so results equal the members' own runs;
 // the graph removes the per-kernel launch cost of the many small grids
-// and lets the runtime overlap the branches.
+// and lets the runtime overlap the branches.  With grouping on (the
+// default) the FP32 P-SV members that qualify (group2_eligible) are not
+// branches: one group advances all of them with one launch per phase
+// (three per step), the workgroups of every member's step end to end.
 #include "swpc_batch.h"
 
 #include <algorithm>
@@ -36,6 +39,9 @@ struct swpc_batch {
     int graph_k = 0;             // steps in the captured graph
     swpc::DBuf dits;             // device array of the members' step counters
     long long bytes = 0;
+    bool grouping = true;        // P-SV members in one group (swpc_batch_set_grouping)
+    swpc::Group2 *group = nullptr;   // the group of the captured graph
+    long long group_bytes = 0;
 };
 
 namespace swpc {
@@ -47,6 +53,10 @@ static void batch_drop(swpc_batch *b)
     b->gexec = nullptr;
     b->graph = nullptr;
     b->graph_k = 0;
+    group2_free(b->group);
+    b->group = nullptr;
+    b->bytes -= b->group_bytes;
+    b->group_bytes = 0;
 }
 
 static int member_info(swpc_batch::M &x)
@@ -62,16 +72,39 @@ static int member_step(swpc_batch::M &x) { return x.c3 ? batch_step3(x.c3) : bat
 static int batch_capture(swpc_batch *b, int k)
 {
     batch_drop(b);
+    // the group: every qualifying P-SV member, stepped on the batch stream
+    std::vector<swpc2d_ctx *> gc;
+    std::vector<char> grouped(b->mem.size(), 0);
+    if (b->grouping)
+        for (size_t q = 0; q < b->mem.size(); q++)
+            if (b->mem[q].c2 && group2_eligible(b->mem[q].c2)) {
+                gc.push_back(b->mem[q].c2);
+                grouped[q] = 1;
+            }
+    if (gc.size() >= 2) {
+        long long gb = 0;
+        if (int r = group2_build(gc, &b->group, gb)) return r;
+        b->group_bytes = gb;
+        b->bytes += gb;
+    } else {
+        std::fill(grouped.begin(), grouped.end(), 0);
+    }
     SWPC_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     int r = SWPC_OK;
     hipError_t e = hipEventRecord(b->ev_fork, b->stream);
-    for (auto &x : b->mem) {
+    for (size_t q = 0; q < b->mem.size(); q++) {
+        auto &x = b->mem[q];
         if (r != SWPC_OK || e != hipSuccess) break;
+        if (grouped[q]) continue;
         e = hipStreamWaitEvent(x.m.stream, b->ev_fork, 0);
-        for (int q = 0; q < k && r == SWPC_OK && e == hipSuccess; q++) r = member_step(x);
+        for (int s = 0; s < k && r == SWPC_OK && e == hipSuccess; s++) r = member_step(x);
         if (r == SWPC_OK && e == hipSuccess) e = hipEventRecord(x.ev, x.m.stream);
-        if (r == SWPC_OK && e == hipSuccess) e = hipStreamWaitEvent(b->stream, x.ev, 0);
     }
+    // the group runs on the batch stream beside the branches, which join after it
+    if (b->group)
+        for (int s = 0; s < k && r == SWPC_OK && e == hipSuccess; s++) r = group2_step(b->group, b->stream);
+    for (size_t q = 0; q < b->mem.size() && r == SWPC_OK && e == hipSuccess; q++)
+        if (!grouped[q]) e = hipStreamWaitEvent(b->stream, b->mem[q].ev, 0);
     hipGraph_t g = nullptr;
     const hipError_t ee = hipStreamEndCapture(b->stream, &g);
     if (r != SWPC_OK || e != hipSuccess || ee != hipSuccess || !g) {
@@ -203,6 +236,15 @@ int swpc_batch_run(swpc_batch *b, int it0, int nsteps, int graph_k)
         if (x.c3) batch_done3(x.c3, it0, it0 + nsteps - 1);
         else batch_done2(x.c2, it0, it0 + nsteps - 1);
     }
+    return SWPC_OK;
+}
+
+int swpc_batch_set_grouping(swpc_batch *b, int on)
+{
+    clear_error();
+    if (!b) return fail(SWPC_EINVAL, "null batch");
+    batch_drop(b);
+    b->grouping = on != 0;
     return SWPC_OK;
 }
 

@@ -57,3 +57,60 @@ def test_batch_rejects_steps_beyond_the_source_table():
             b.add(sims[0])
     for s in sims:
         s.close()
+
+
+def test_psv_group_equals_branches_and_own_runs():
+    """The P-SV group (one launch per phase for every qualifying FP32 P-SV
+    member) against the same batch with every member a graph branch and
+    against each member's own run, bit for bit: five P-SV members, of which
+    one has a snapshot stream and one runs in FP64 (both stay branches),
+    one source on the first column (its stress glut stays a launch of its
+    own, so the member stays a branch too) and one body force, beside a
+    3-D member."""
+    from openswpc_amd import snapshot
+    from openswpc_amd.solver import SwpcPSV
+    nt = 70
+    cases = _cases(nt)
+    g0 = problem.case2d(110, 70, 0.1, 0.005, nt, na=10, src_depth=2.0).grid
+    extra = [("2d", problem.case2d(110, 70, 0.1, 0.005, nt, na=10, seed=15, nst=3, src_depth=2.0,
+                                   src_x=g0.xbeg + 0.5 * g0.dx)),
+             ("2d", problem.case2d(100, 72, 0.1, 0.005, nt, na=10, seed=16, nst=3, src_depth=2.5, bodyforce=True)),
+             ("2d", problem.case2d(90, 66, 0.1, 0.005, nt, na=10, seed=17, nst=3, src_depth=2.0))]
+    cases = cases + extra
+    prec = ["fp32"] * len(cases)
+    prec[-1] = "fp64"
+    snap_member = 1
+
+    def open_all():
+        sims = []
+        for q, (kind, c) in enumerate(cases):
+            if kind == "3d":
+                sims.append(batch.open_case(kind, c))
+            else:
+                sims.append(SwpcPSV(c.grid, c.medium, c.pml, c.glut, c.stations, c.ntdec_w, nt, precision=prec[q],
+                                    bodyforce=c.bforce))
+        c = cases[snap_member][1]
+        _, _, ik = snapshot.cells2(snapshot.snap_grid2(c.grid, 4, 4))
+        sid = sims[snap_member].add_snapshot(1, ik, 7, 16, snapshot.UC2, 1.0, np.float32(c.grid.dt))
+        return sims, sid, len(ik)
+
+    out = {}
+    for mode in ("own", "group", "branches"):
+        sims, sid, npts = open_all()
+        if mode == "own":
+            for s in sims:
+                s.run(1, nt)
+        else:
+            with batch.Batch() as b:
+                b.set_grouping(mode == "group")
+                for s in sims:
+                    b.add(s)
+                b.run(1, nt, 16)
+                b.sync()
+        out[mode] = [s.receivers()[0].copy() for s in sims] + [sims[snap_member].get_snapshot(sid, 9, 2, npts)]
+        for s in sims:
+            s.close()
+    for q, ref in enumerate(out["own"]):
+        assert np.isfinite(ref).all() and np.abs(ref).max() > 0, q
+        assert np.array_equal(out["group"][q], ref), ("group", q)
+        assert np.array_equal(out["branches"][q], ref), ("branches", q)
