@@ -293,7 +293,10 @@ int swpc3d_memory(swpc3d_ctx *ctx, long long *bytes);
              walked one 256-cell row segment per workgroup (0/1, default 1;
              0 walks every box cell by cell);
      [20]    PML workgroups ordered so that each XCD takes one contiguous
-             run of them (0/1, default 0). */
+             run of them (0/1, default 0);
+     [21]    z-marching velocity with the y-stencil rows of Syy, Sxy and
+             Syz staged in LDS per plane (0/1, default 0; tile rows 2, 4
+             or 8 from entry [12]). */
 int swpc3d_set_tuning(swpc3d_ctx *ctx, const int *p, int n);
 
 /* ------------------------------------------------------------------ */

@@ -1176,6 +1176,147 @@ __global__ void __launch_bounds__(64 * BY) k3x_velz(const K3<float> a)
     }
 }
 
+
+// Velocity, z-marching with the y-stencil rows staged in LDS: as k3x_velz,
+// but per plane the workgroup first copies rows j0-2 .. j0+BY+1 of Syy, Sxy
+// and Syz (256 + 8 cells each, x halo included) into LDS, and every thread
+// takes its y neighbours (and Sxy's x neighbours) from there instead of
+// from the L2.  NB = 2 double-buffers the planes (one barrier per plane),
+// NB = 1 reuses one buffer (two).  The arithmetic and its order are those
+// of k3x_velz, so both give the same bits.
+template <int BY, int NB, bool S16, bool RHO>
+__global__ void __launch_bounds__(64 * BY) k3x_velzl(const K3<float> a)
+{
+    constexpr int NR = BY + 4;        // staged rows j0-2 .. j0+BY+1
+    constexpr int RL = 264;           // cells i0b-4 .. i0b+259 per row
+    __shared__ __attribute__((aligned(16))) float L[NB][3][NR][RL];
+    const unsigned b = blockIdx.x;
+    const unsigned xt = b % (unsigned)a.ntx;
+    const unsigned t = b / (unsigned)a.ntx;
+    const unsigned yt = t % (unsigned)a.nty;
+    const unsigned zc = t / (unsigned)a.nty;
+    const int i0b = a.ti0 + (int)xt * 256;
+    const int j0 = a.na + 1 + (int)yt * BY;
+    const int tx = (int)threadIdx.x, ty = (int)threadIdx.y;
+    const int i0 = i0b + tx * 4;
+    const int j = j0 + ty;
+    unsigned msk = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        if (i0 + q > a.na && i0 + q <= a.nx - a.na) msk |= 1u << q;
+    // every thread takes part in the staging and the barriers
+    const bool act = j <= a.ny - a.na && msk != 0;
+    const int kb = 1 + (int)zc * a.kc;
+    const int ke = min(kb + a.kc - 1, a.nz - a.na);
+    if (kb > ke) return;
+    const int px = a.px, sxy = a.sxy;
+    const float *__restrict__ Sxx = a.S[0];
+    const float *__restrict__ Syy = a.S[1];
+    const float *__restrict__ Szz = a.S[2];
+    const float *__restrict__ Syz = a.S[3];
+    const float *__restrict__ Sxz = a.S[4];
+    const float *__restrict__ Sxy = a.S[5];
+    const float *__restrict__ Sy[3] = {Syy, Sxy, Syz};
+    int4 fs[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        fs[q] = a.surf[(min(j, a.ny) - 1) * a.nx + (min(max(i0 + q, 1), a.nx) - 1)];
+    int c = at3(px, sxy, i0, min(j, a.ny + 3), kb);
+    f4v zz_m1 = {0, 0, 0, 0}, zz_0 = zz_m1, zz_p1 = zz_m1, xz_m2 = zz_m1, xz_m1 = zz_m1, xz_0 = zz_m1;
+    f4v yz_m2 = zz_m1, yz_m1 = zz_m1, yz_0 = zz_m1, r_0 = zz_m1;
+    if (act) {
+        zz_m1 = ld4(Szz + c - sxy); zz_0 = ld4(Szz + c); zz_p1 = ld4(Szz + c + sxy);
+        xz_m2 = ld4(Sxz + c - 2 * sxy); xz_m1 = ld4(Sxz + c - sxy); xz_0 = ld4(Sxz + c);
+        yz_m2 = ld4(Syz + c - 2 * sxy); yz_m1 = ld4(Syz + c - sxy); yz_0 = ld4(Syz + c);
+        if constexpr (RHO) r_0 = ld4(a.rho + c);
+    }
+    const float *__restrict__ Rh = a.rho;
+    const float dt = a.dt;
+    // staging: NR rows x 66 float4 groups x 3 fields per plane; a group
+    // lies inside its row of the memory box or is filled with zeros (no
+    // active thread reads those)
+    constexpr int NG = 66, NSTAGE = 3 * NR * NG;
+    const int tid = ty * 64 + tx;
+    for (int k = kb; k <= ke; k++, c += sxy) {
+        const int buf = NB == 2 ? (k - kb) & 1 : 0;
+        if (NB == 1 && k > kb) __syncthreads();   // the previous plane's reads are done
+        for (int e = tid; e < NSTAGE; e += 64 * BY) {
+            const int f = e / (NR * NG), rem = e - f * (NR * NG);
+            const int r = rem / NG, g = rem - r * NG;
+            const int jj = j0 - 2 + r, ii = i0b - 4 + 4 * g;
+            f4v v = {0, 0, 0, 0};
+            if (jj >= -2 && jj <= a.ny + 3 && ii + 31 >= 0 && ii + 31 + 3 < px)
+                v = ld4(Sy[f] + at3(px, sxy, ii, jj, k));
+            *(f4v *)&L[buf][f][r][4 * g] = v;
+        }
+        __syncthreads();
+        if (!act) continue;
+        const float *Lyy = &L[buf][0][0][0], *Lxy = &L[buf][1][0][0], *Lyz = &L[buf][2][0][0];
+        const int o = 4 + 4 * tx;     // own float4 in a staged row
+        const f4v zz_p2 = ld4(Szz + c + 2 * sxy), xz_p1 = ld4(Sxz + c + sxy), yz_p1 = ld4(Syz + c + sxy);
+        const f4v sxx = ld4(Sxx + c);
+        const float2 sxxl = ld2(Sxx + c - 2), sxxr = ld2(Sxx + c + 4);
+        const f4v syy = *(const f4v *)&Lyy[(ty + 2) * RL + o], syyp1 = *(const f4v *)&Lyy[(ty + 3) * RL + o];
+        const f4v syyp2 = *(const f4v *)&Lyy[(ty + 4) * RL + o], syym1 = *(const f4v *)&Lyy[(ty + 1) * RL + o];
+        const f4v sxyv = *(const f4v *)&Lxy[(ty + 2) * RL + o], sxyym1 = *(const f4v *)&Lxy[(ty + 1) * RL + o];
+        const f4v sxyym2 = *(const f4v *)&Lxy[ty * RL + o], sxyyp1 = *(const f4v *)&Lxy[(ty + 3) * RL + o];
+        const float2 sxyl = *(const float2 *)&Lxy[(ty + 2) * RL + o - 2];
+        const float2 sxyr = *(const float2 *)&Lxy[(ty + 2) * RL + o + 4];
+        const float2 sxzl = ld2(Sxz + c - 2), sxzr = ld2(Sxz + c + 4);
+        const f4v syzym1 = *(const f4v *)&Lyz[(ty + 1) * RL + o], syzym2 = *(const f4v *)&Lyz[ty * RL + o];
+        const f4v syzyp1 = *(const f4v *)&Lyz[(ty + 3) * RL + o];
+        f4v bx, by, bz, r_p1;
+        if constexpr (RHO) {
+            r_p1 = ld4(Rh + c + sxy);
+            const f4v rj = ld4(Rh + c + px);
+            const float ri[5] = {r_0.x, r_0.y, r_0.z, r_0.w, Rh[c + 4]};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                bx[q] = 2.0f / (ri[q] + ri[q + 1]);
+                by[q] = 2.0f / (r_0[q] + rj[q]);
+                bz[q] = 2.0f / (r_0[q] + r_p1[q]);
+            }
+        } else {
+            bx = med4<S16>(a, MBX, c, k);
+            by = med4<S16>(a, MBY, c, k);
+            bz = med4<S16>(a, MBZ, c, k);
+        }
+        f4v vx = ld4nt(a.V[0] + c), vy = ld4nt(a.V[1] + c), vz = ld4nt(a.V[2] + c);
+        // x rows i0-2 .. i0+5
+        const float xx[8] = {sxxl.x, sxxl.y, sxx.x, sxx.y, sxx.z, sxx.w, sxxr.x, sxxr.y};
+        const float xy[8] = {sxyl.x, sxyl.y, sxyv.x, sxyv.y, sxyv.z, sxyv.w, sxyr.x, sxyr.y};
+        const float xz[8] = {sxzl.x, sxzl.y, xz_0.x, xz_0.y, xz_0.z, xz_0.w, sxzr.x, sxzr.y};
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int o1 = (k - fs[q].x) * (fs[q].y - k), o2 = (k - fs[q].z) * (fs[q].w - k);
+            const bool lo = max(o1, o2) >= 0;
+            const float e0x = lo ? a.e2[0][0] : a.e4[0][0], e1x = lo ? a.e2[0][1] : a.e4[0][1];
+            const float e0y = lo ? a.e2[1][0] : a.e4[1][0], e1y = lo ? a.e2[1][1] : a.e4[1][1];
+            const float e0z = lo ? a.e2[2][0] : a.e4[2][0], e1z = lo ? a.e2[2][1] : a.e4[2][1];
+            const int x = q + 2;  // index of cell i0+q in the x rows
+            const float d3Sx3 = (xx[x + 1] - xx[x]) * e0x - (xx[x + 2] - xx[x - 1]) * e1x
+                              + (sxyv[q] - sxyym1[q]) * e0y - (sxyyp1[q] - sxyym2[q]) * e1y
+                              + (xz_0[q] - xz_m1[q]) * e0z - (xz_p1[q] - xz_m2[q]) * e1z;
+            const float d3Sy3 = (xy[x] - xy[x - 1]) * e0x - (xy[x + 1] - xy[x - 2]) * e1x
+                              + (syyp1[q] - syy[q]) * e0y - (syyp2[q] - syym1[q]) * e1y
+                              + (yz_0[q] - yz_m1[q]) * e0z - (yz_p1[q] - yz_m2[q]) * e1z;
+            const float d3Sz3 = (xz[x] - xz[x - 1]) * e0x - (xz[x + 1] - xz[x - 2]) * e1x
+                              + (yz_0[q] - syzym1[q]) * e0y - (syzyp1[q] - syzym2[q]) * e1y
+                              + (zz_p1[q] - zz_0[q]) * e0z - (zz_p2[q] - zz_m1[q]) * e1z;
+            vx[q] = vx[q] + bx[q] * d3Sx3 * dt;
+            vy[q] = vy[q] + by[q] * d3Sy3 * dt;
+            vz[q] = vz[q] + bz[q] * d3Sz3 * dt;
+        }
+        st4m(a.V[0] + c, vx, msk);
+        st4m(a.V[1] + c, vy, msk);
+        st4m(a.V[2] + c, vz, msk);
+        zz_m1 = zz_0; zz_0 = zz_p1; zz_p1 = zz_p2;
+        xz_m2 = xz_m1; xz_m1 = xz_0; xz_0 = xz_p1;
+        yz_m2 = yz_m1; yz_m1 = yz_0; yz_0 = yz_p1;
+        if constexpr (RHO) r_0 = r_p1;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // PML (ADE-CFS, 2nd order, elastic)
 // ---------------------------------------------------------------------------
@@ -1952,6 +2093,7 @@ struct swpc3d_ctx {
     int pml4 = 0;           // 4-wide PML kernels (fewer waves per SIMD; off by default)
     int pml_rows = 1;       // whole-row absorber boxes by row segments (tuning entry 19)
     int pml_xcd = 0;        // PML workgroups XCD-contiguous (tuning entry 20)
+    int x4_vlds = 0;        // z-marching velocity with LDS-staged y rows (tuning entry 21)
     bool ade_sparse = true; // no ADE array has been set from outside since creation
     // The PML kernels run on a second stream, concurrently with the
     // interior kernels of the same half-step (they write disjoint cells and
@@ -2153,6 +2295,14 @@ static void go_x4_velz(const K3<float> &a, unsigned nb, bool s16, hipStream_t s)
     else hipLaunchKernelGGL((k3x_velz<BY, false, false>), dim3(nb), dim3(64, BY), 0, s, a);
 }
 
+template <int BY, int NB>
+static void go_x4_velzl(const K3<float> &a, unsigned nb, bool s16, hipStream_t s)
+{
+    if (a.rho) hipLaunchKernelGGL((k3x_velzl<BY, NB, false, true>), dim3(nb), dim3(64, BY), 0, s, a);
+    else if (s16) hipLaunchKernelGGL((k3x_velzl<BY, NB, true, false>), dim3(nb), dim3(64, BY), 0, s, a);
+    else hipLaunchKernelGGL((k3x_velzl<BY, NB, false, false>), dim3(nb), dim3(64, BY), 0, s, a);
+}
+
 template <int BY>
 static void go_x4_stress(const K3<float> &a, unsigned nb, bool s16, hipStream_t s)
 {
@@ -2221,6 +2371,15 @@ static bool x4_vel(const swpc3d_ctx *c, K3<float> a, hipStream_t s)
         a.nblk = (unsigned)a.ntx * (unsigned)a.nty * (unsigned)a.nkc;
         a.nblk_pad = (a.nblk + 7u) & ~7u;
         const unsigned nb = a.xcd ? a.nblk_pad : a.nblk;
+        if (c->x4_vlds) {
+            // the LDS-staged form walks the tiles in order (no XCD remap)
+            switch (vby) {
+            case 2: go_x4_velzl<2, 2>(a, a.nblk, c->s16, s); return true;
+            case 4: go_x4_velzl<4, 2>(a, a.nblk, c->s16, s); return true;
+            case 8: go_x4_velzl<8, 1>(a, a.nblk, c->s16, s); return true;
+            default: return false;
+            }
+        }
         switch (vby) {
         case 1: go_x4_velz<1>(a, nb, c->s16, s); return true;
         case 2: go_x4_velz<2>(a, nb, c->s16, s); return true;
@@ -3564,10 +3723,10 @@ int swpc3d_set_tuning(swpc3d_ctx *c, const int *p, int n)
     clear_error();
     if (c) graph_drop(c);  // the captured steps would launch stale arguments
     if (!c || (n > 0 && !p)) return fail(SWPC_EINVAL, "null argument");
-    int v[21] = {c->s_by, c->s_wpe, c->s_split, c->v_by, c->v_wpe, c->xcd, c->s_kc, c->v_kc, c->x4, c->x4_by, c->pml4,
+    int v[22] = {c->s_by, c->s_wpe, c->s_split, c->v_by, c->v_wpe, c->xcd, c->s_kc, c->v_kc, c->x4, c->x4_by, c->pml4,
                  c->x4_form, c->x4_vby, c->conc, c->x4_vz, c->x4_sz, c->graph_k, c->sr, c->r_shift, c->pml_rows,
-                 c->pml_xcd};
-    for (int q = 0; q < n && q < 21; q++) v[q] = p[q];
+                 c->pml_xcd, c->x4_vlds};
+    for (int q = 0; q < n && q < 22; q++) v[q] = p[q];
     if (v[18] < -24 || v[18] > 24) return fail(SWPC_EINVAL, "memory-variable scale shift must be in -24..24");
     if (v[16] < 0 || v[16] > 1024) return fail(SWPC_EINVAL, "steps per graph must be in 0..1024");
     if (v[14] < 0 || v[15] < 0) return fail(SWPC_EINVAL, "negative z-chunk");
@@ -3583,6 +3742,7 @@ int swpc3d_set_tuning(swpc3d_ctx *c, const int *p, int n)
     c->sr = v[17] != 0;
     c->pml_rows = v[19] != 0;
     c->pml_xcd = v[20] != 0;
+    c->x4_vlds = v[21] != 0;
     if (v[18] != c->r_shift) {
         // the stored memory variables are rescaled with the new scale
         if (int r = rescale_r(c, std::ldexp(1.0f, c->r_shift - v[18]))) return r;

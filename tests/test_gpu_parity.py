@@ -265,3 +265,28 @@ def test_psv_fused_step_equals_split(bits, where):
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
     for f in FIELDS2:
         assert np.array_equal(out[0][2][f], out[1][2][f]), f
+
+
+def test_3d_lds_staged_velocity_is_exact():
+    """The z-marching velocity update with its y-stencil rows staged in LDS
+    (tuning entry 21; slower, kept as a measured alternative) gives the
+    same bits as the default form on a ragged grid, for tile rows 2, 4
+    and 8, after stress and velocity updates over 40 steps."""
+    nt = 40
+    case = problem.case3d(71, 45, 39, 0.5, 0.04, nt, na=6, src_depth=4.0, nst_side=3, zbeg=-1.5,
+                          rmed=(0.05, 2.0, 0.5), rmed_tile=(32, 32, 32))
+    say("3-D LDS-staged velocity vs default: 71x45x39")
+    # the FP32 defaults of swpc3d_set_tuning entries 0..21
+    base = [4, 3, 0, 4, 5, 0, 32, 0, 1, 2, 0, 1, 4, 0, 8, 0, 16, 0, 0, 1, 0, 0]
+    out = []
+    for vby, lds in ((4, 0), (4, 1), (2, 1), (8, 1)):
+        v = list(base)
+        v[12], v[21] = vby, lds
+        with Swpc3D(case.grid, case.medium, case.pml, case.glut, case.stations, 1, nt) as sim:
+            sim.set_tuning(*v)
+            sim.run(1, nt)
+            out.append((sim.receivers()[0], sim.field("Vx"), sim.field("Vy"), sim.field("Vz")))
+    assert np.abs(out[0][0]).max() > 0
+    for o in out[1:]:
+        for x, y in zip(out[0], o):
+            assert np.array_equal(x, y)
