@@ -267,12 +267,14 @@ int swpc3d_memory(swpc3d_ctx *ctx, long long *bytes);
              hint (0/5), XCD-contiguous workgroup order (0/1), stress
              z-chunk, velocity z-chunk (0 = automatic);
      [8]     4-cell-per-thread kernels for FP32 and reduced storage (0/1,
-             default 1); [9] their tile rows (1/2/4);
+             default 1); [9] their tile rows (1/2/4; default 4 for FP32,
+             2 for 16-bit storage);
      [10]    4-cell-per-thread PML kernels (0/1, default 0);
      [11]    4-wide stress form: 0 normal + shear launches, 1 one pass with
-             staged loads, 2 one pass (default 1 for FP32, 0 for 16-bit
-             storage); [12] tile rows of the z-marching velocity kernel
-             (1/2/4/8, default 2);
+             staged loads, 2 one pass (default 2 for FP32, 1 for reduced
+             storage, 0 for the 16-bit medium); [12] tile rows of the
+             z-marching velocity kernel (1/2/4/8, default 4; 2 for the
+             16-bit medium);
      [13]    PML kernels on a second stream, concurrent with the interior
              (0/1, default 0);
      [14]    4-wide velocity z-marching chunk (0 = one plane per workgroup;

@@ -276,7 +276,7 @@ def test_3d_lds_staged_velocity_is_exact():
     case = problem.case3d(71, 45, 39, 0.5, 0.04, nt, na=6, src_depth=4.0, nst_side=3, zbeg=-1.5,
                           rmed=(0.05, 2.0, 0.5), rmed_tile=(32, 32, 32))
     say("3-D LDS-staged velocity vs default: 71x45x39")
-    # the FP32 defaults of swpc3d_set_tuning entries 0..21
+    # swpc3d_set_tuning entries 0..21 (256 x 2 staged stress, velocity tile rows as below)
     base = [4, 3, 0, 4, 5, 0, 32, 0, 1, 2, 0, 1, 4, 0, 8, 0, 16, 0, 0, 1, 0, 0]
     out = []
     for vby, lds in ((4, 0), (4, 1), (2, 1), (8, 1)):
