@@ -627,10 +627,10 @@ module swpc_hip
             integer(c_int), value :: it0, nsteps, graph_k
         end function
 
-        integer(c_int) function swpc_batch_set_grouping(b, on) bind(C, name='swpc_batch_set_grouping')
+        integer(c_int) function swpc_batch_set_grouping(b, mode) bind(C, name='swpc_batch_set_grouping')
             import :: c_int, c_ptr
             type(c_ptr), value :: b
-            integer(c_int), value :: on
+            integer(c_int), value :: mode
         end function
 
         integer(c_int) function swpc_batch_sync(b) bind(C, name='swpc_batch_sync')

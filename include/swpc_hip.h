@@ -523,11 +523,17 @@ int swpc_batch_destroy(swpc_batch *b);
 int swpc_batch_add3d(swpc_batch *b, swpc3d_ctx *ctx);
 int swpc_batch_add2d(swpc_batch *b, swpc2d_ctx *ctx);
 int swpc_batch_run(swpc_batch *b, int it0, int nsteps, int graph_k);
-/* FP32 P-SV members with fused steps, no snapshot streams and their
-   sources inside the fused kernels advance as one group (on != 0, the
-   default): one launch per phase for all of them instead of one graph
-   branch each.  Results are the same bit for bit either way. */
-int swpc_batch_set_grouping(swpc_batch *b, int on);
+/* Groups, one launch per phase for all their members instead of one graph
+   branch each (mode is a bit mask; results are the same bit for bit in
+   every mode):
+     bit 0 (default on): FP32 P-SV members with fused steps, no snapshot
+           streams and their sources inside the fused kernels;
+     bit 1 (default off): FP32 3-D members with a density, the one-pass
+           stress and z-marching velocity forms, no snapshots and no body
+           force (measured 0.96x the branches on the 48-run mix, whose 3-D
+           members fill the GPU on their own, and 1.03x on 64 small 3-D
+           grids). */
+int swpc_batch_set_grouping(swpc_batch *b, int mode);
 int swpc_batch_sync(swpc_batch *b);
 
 #ifdef __cplusplus

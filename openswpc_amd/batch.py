@@ -41,10 +41,11 @@ class Batch:
     def run(self, it0, nsteps, graph_k=16):
         check(lib().swpc_batch_run(self.h, it0, nsteps, graph_k), "swpc_batch_run")
 
-    def set_grouping(self, on):
-        """FP32 P-SV members in one group, one launch per phase for all of
-        them (True, the default), or one graph branch each (False)."""
-        check(lib().swpc_batch_set_grouping(self.h, int(bool(on))), "swpc_batch_set_grouping")
+    def set_grouping(self, psv=True, d3=False):
+        """Qualifying FP32 P-SV members (psv, the default) and FP32 3-D
+        members (d3) in groups, one launch per phase for all members of a
+        group, instead of one graph branch each; results are the same."""
+        check(lib().swpc_batch_set_grouping(self.h, int(bool(psv)) | (2 if d3 else 0)), "swpc_batch_set_grouping")
 
     def sync(self):
         check(lib().swpc_batch_sync(self.h), "swpc_batch_sync")
@@ -170,25 +171,28 @@ def alg_bytes_per_step(cases):
 
 
 def measure(cases, steps, graph_k=16, tuning=None, log=None):
-    """Time `steps` steps of every case four ways on fresh contexts:
+    """Time `steps` steps of every case five ways on fresh contexts:
     'batch' (one graph per graph_k steps of all members, the P-SV members
-    in one group, one launch per phase for all of them), 'branches' (the
+    in one group, one launch per phase for all of them), 'batch_3d_group'
+    (the same with the 3-D members in a group too, which uses launch shapes
+    of its own), 'branches' (the
     same graph with every member a branch of its own), 'streams' (each
     member's own run, all enqueued before one wait: the members overlap on
     their own streams) and 'sequential' (one member after the other).
     Returns {mode: seconds} plus the total cells."""
     res = {}
-    for mode in ("batch", "branches", "streams", "sequential"):
+    for mode in ("batch", "batch_3d_group", "branches", "streams", "sequential"):
         sims = []
         for kind, c in cases:
             s = open_case(kind, c)
-            if tuning and kind == "3d" and size_class(c.grid) in tuning:
+            # the 3-D group uses launch shapes of its own
+            if tuning and kind == "3d" and size_class(c.grid) in tuning and mode != "batch_3d_group":
                 s.set_tuning(*tuning[size_class(c.grid)])
             sims.append(s)
         warm = min(graph_k or 16, steps)
-        if mode in ("batch", "branches"):
+        if mode in ("batch", "batch_3d_group", "branches"):
             b = Batch()
-            b.set_grouping(mode == "batch")
+            b.set_grouping(psv=mode != "branches", d3=mode == "batch_3d_group")
             for s in sims:
                 b.add(s)
             b.run(1, warm, graph_k)
@@ -248,9 +252,11 @@ def _main(argv=None):
                        f"per side) runs, seed {a.seed}, {a.steps} steps each",
            "cells": res["cells"], "timed_steps": res["timed_steps"], "graph_k": a.graph_k,
            "tuned_classes": len(tuning or {}),
-           **{f"{m}_cell_updates_per_s": n / res[m] for m in ("batch", "branches", "streams", "sequential")},
+           **{f"{m}_cell_updates_per_s": n / res[m]
+              for m in ("batch", "batch_3d_group", "branches", "streams", "sequential")},
            "speedup_vs_sequential": res["sequential"] / res["batch"],
            "speedup_vs_branches": res["branches"] / res["batch"],
+           "batch_3d_group_vs_branches": res["branches"] / res["batch_3d_group"],
            "alg_bytes_per_step": alg_bytes_per_step(cases),
            "batch_alg_GBs": ab / res["batch"] / 1e9, "batch_hbm_frac": ab / res["batch"] / 8e12}
     print(json.dumps(out))

@@ -512,11 +512,10 @@ __device__ __forceinline__ void stR4(const K3<float> &a, int q, int c, f4v v, un
     }
 }
 
-// Tile decode of the 4-wide kernels: thread -> (i0, j, k) and the mask of
-// its 4 cells inside the interior box.
-__device__ __forceinline__ bool x4_cell(const K3<float> &a, int by, int &i0, int &j, int &k, unsigned &m)
+// Tile decode of the 4-wide kernels: (workgroup b, thread) -> (i0, j, k)
+// and the mask of its 4 cells inside the interior box.
+__device__ __forceinline__ bool x4_cell(const K3<float> &a, int by, unsigned b, int &i0, int &j, int &k, unsigned &m)
 {
-    const unsigned b = blockIdx.x;
     unsigned xt, yt, kk;
     if (a.xcd) {
         const unsigned nyc = ((unsigned)a.nty + 7u) >> 3;
@@ -570,7 +569,7 @@ __global__ void __launch_bounds__(64 * BY) k3x_normal(const K3<float> a)
 {
     int i0, j, k;
     unsigned msk;
-    if (!x4_cell(a, BY, i0, j, k, msk)) return;
+    if (!x4_cell(a, BY, blockIdx.x, i0, j, k, msk)) return;
     const int px = a.px, sxy = a.sxy;
     const int c = at3(px, sxy, i0, j, k);
     const float *__restrict__ Vx = a.V[0];
@@ -635,7 +634,7 @@ __global__ void __launch_bounds__(64 * BY) k3x_shear(const K3<float> a)
 {
     int i0, j, k;
     unsigned msk;
-    if (!x4_cell(a, BY, i0, j, k, msk)) return;
+    if (!x4_cell(a, BY, blockIdx.x, i0, j, k, msk)) return;
     const int px = a.px, sxy = a.sxy;
     const int c = at3(px, sxy, i0, j, k);
     const float *__restrict__ Vx = a.V[0];
@@ -711,11 +710,11 @@ __device__ __forceinline__ float harm4s(float a, float b, float c, float d)
 // turn.  BAR = 1 keeps the compiler from hoisting the shear loads above the
 // normal stores, which bounds the registers held at once.
 template <int BY, bool S16, int BAR>
-__global__ void __launch_bounds__(64 * BY) k3x_stress(const K3<float> a)
+__device__ __forceinline__ void stress4_blk(const K3<float> &a, unsigned blk)
 {
     int i0, j, k;
     unsigned msk;
-    if (!x4_cell(a, BY, i0, j, k, msk)) return;
+    if (!x4_cell(a, BY, blk, i0, j, k, msk)) return;
     const int px = a.px, sxy = a.sxy;
     const int c = at3(px, sxy, i0, j, k);
     const float *__restrict__ Vx = a.V[0];
@@ -857,6 +856,12 @@ __global__ void __launch_bounds__(64 * BY) k3x_stress(const K3<float> a)
         st4m(a.S[4] + c, sxz, msk);
         st4m(a.S[5] + c, sxy_, msk);
     }
+}
+
+template <int BY, bool S16, int BAR>
+__global__ void __launch_bounds__(64 * BY) k3x_stress(const K3<float> a)
+{
+    stress4_blk<BY, S16, BAR>(a, blockIdx.x);
 }
 
 // One-pass stress update, z-marching: each thread owns 4 x-cells of one
@@ -1017,7 +1022,7 @@ __global__ void __launch_bounds__(64 * BY) k3x_vel(const K3<float> a)
 {
     int i0, j, k;
     unsigned msk;
-    if (!x4_cell(a, BY, i0, j, k, msk)) return;
+    if (!x4_cell(a, BY, blockIdx.x, i0, j, k, msk)) return;
     const int px = a.px, sxy = a.sxy;
     const int c = at3(px, sxy, i0, j, k);
     const float *__restrict__ Sxx = a.S[0];
@@ -1071,9 +1076,8 @@ __global__ void __launch_bounds__(64 * BY) k3x_vel(const K3<float> a)
 // read once per chunk instead of four times; y- and x-neighbours are read
 // as in k3x_vel.  Workgroups: 256-cell x tiles x BY rows x z-chunks.
 template <int BY, bool S16, bool RHO>
-__global__ void __launch_bounds__(64 * BY) k3x_velz(const K3<float> a)
+__device__ __forceinline__ void velz_blk(const K3<float> &a, unsigned b)
 {
-    unsigned b = blockIdx.x;
     if (a.xcd) {
         // each XCD walks one contiguous run of tiles, so the y-neighbour
         // rows of a tile were fetched into the same L2
@@ -1174,6 +1178,12 @@ __global__ void __launch_bounds__(64 * BY) k3x_velz(const K3<float> a)
         yz_m2 = yz_m1; yz_m1 = yz_0; yz_0 = yz_p1;
         if constexpr (RHO) r_0 = r_p1;
     }
+}
+
+template <int BY, bool S16, bool RHO>
+__global__ void __launch_bounds__(64 * BY) k3x_velz(const K3<float> a)
+{
+    velz_blk<BY, S16, RHO>(a, blockIdx.x);
 }
 
 
@@ -1360,9 +1370,8 @@ __device__ __forceinline__ unsigned pml_block(unsigned per)
     return per ? (b & 7u) * per + (b >> 3) : b;
 }
 
-__device__ __forceinline__ bool pml_row(const Rows3 &R, unsigned per, int nx, int &i, int &j, int &k)
+__device__ __forceinline__ bool pml_row(const Rows3 &R, unsigned blk, int nx, int &i, int &j, int &k)
 {
-    const unsigned blk = pml_block(per);
     const unsigned row = blk / (unsigned)R.nseg;
     if (row >= R.total) return false;
     i = 1 + (int)(blk - row * (unsigned)R.nseg) * 256 + (int)threadIdx.x;
@@ -1469,7 +1478,7 @@ template <typename T, bool S16>
 __global__ void __launch_bounds__(256) k3_stress_pml_rows(const K3<T> a, const Rows3 R, unsigned per)
 {
     int i, j, k;
-    if (pml_row(R, per, a.nx, i, j, k)) stress_pml_cell<T, S16>(a, i, j, k);
+    if (pml_row(R, pml_block(per), a.nx, i, j, k)) stress_pml_cell<T, S16>(a, i, j, k);
 }
 
 template <typename T, bool S16>
@@ -1553,7 +1562,7 @@ template <typename T, bool S16>
 __global__ void __launch_bounds__(256) k3_vel_pml_rows(const K3<T> a, const Rows3 R, unsigned per)
 {
     int i, j, k;
-    if (pml_row(R, per, a.nx, i, j, k)) vel_pml_cell<T, S16>(a, i, j, k);
+    if (pml_row(R, pml_block(per), a.nx, i, j, k)) vel_pml_cell<T, S16>(a, i, j, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -1819,10 +1828,10 @@ __global__ void __launch_bounds__(256) k3x_vel_pml(const K3<float> a, const Boxe
 // Sources are applied in the reference's sequential order by one thread
 // (m_source.f90:796 `loop seq`): several sources may share a cell.
 template <typename T>
-__global__ void k3_stressglut(const K3<T> a, int nsrc, const int *__restrict__ src, const double *__restrict__ m6,
-                              const double *__restrict__ sdrop_all, int nt, const int *__restrict__ dit)
+__device__ __forceinline__ void stressglut_one(const K3<T> &a, int nsrc, const int *__restrict__ src,
+                                               const double *__restrict__ m6, const double *__restrict__ sdrop_all, int nt,
+                                               const int *__restrict__ dit)
 {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
     const int it = *dit;
     if (it < 1 || it > nt) return;
     const double *__restrict__ sdrop = sdrop_all + (size_t)(it - 1) * nsrc;
@@ -1849,6 +1858,14 @@ __global__ void k3_stressglut(const K3<T> a, int nsrc, const int *__restrict__ s
         a.S[3][c - px] -= myz * sd / 4;
         a.S[3][c - px - sxy] -= myz * sd / 4;
     }
+}
+
+template <typename T>
+__global__ void k3_stressglut(const K3<T> a, int nsrc, const int *__restrict__ src, const double *__restrict__ m6,
+                              const double *__restrict__ sdrop_all, int nt, const int *__restrict__ dit)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    stressglut_one<T>(a, nsrc, src, m6, sdrop_all, nt, dit);
 }
 
 template <typename T, bool S16>
@@ -1879,12 +1896,10 @@ __global__ void k3_bodyforce(const K3<T> a, int nsrc, const int *__restrict__ sr
 // is written (while itw < ntw).  The step number comes from the device
 // counter dit, so a captured step replays unchanged.
 template <typename T>
-__global__ void k3_receivers(const K3<T> a, int nst, const int *__restrict__ st, double *__restrict__ uacc,
-                             float *__restrict__ wvel, double *__restrict__ wdisp, int ntdec_w, int ntw,
-                             const int *__restrict__ dit)
+__device__ __forceinline__ void receiver_one(const K3<T> &a, int n, int nst, const int *__restrict__ st,
+                                             double *__restrict__ uacc, float *__restrict__ wvel,
+                                             double *__restrict__ wdisp, int ntdec_w, int ntw, const int *__restrict__ dit)
 {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nst) return;
     const int it = *dit;
     const int itw = (it - 1) / ntdec_w;
     const int store = ((it - 1) % ntdec_w) == 0 && itw < ntw;
@@ -1910,6 +1925,16 @@ __global__ void k3_receivers(const K3<T> a, int nst, const int *__restrict__ st,
         disp_out[3 * n + 1] = uy;
         disp_out[3 * n + 2] = uz;
     }
+}
+
+template <typename T>
+__global__ void k3_receivers(const K3<T> a, int nst, const int *__restrict__ st, double *__restrict__ uacc,
+                             float *__restrict__ wvel, double *__restrict__ wdisp, int ntdec_w, int ntw,
+                             const int *__restrict__ dit)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nst) return;
+    receiver_one<T>(a, n, nst, st, uacc, wvel, wdisp, ntdec_w, ntw, dit);
 }
 
 __global__ void k_fill_u32(unsigned *p, int n, unsigned v)
@@ -2783,6 +2808,224 @@ int batch_check3(swpc3d_ctx *c, int it0, int n)
 int batch_step3(swpc3d_ctx *c) { return step_dev_any(c); }
 
 void batch_done3(swpc3d_ctx *c, int it0, int it1) { note_snapshots(c, it0, it1); }
+
+// ---------------------------------------------------------------------------
+// Groups of 3-D runs in one launch per phase (batches)
+// ---------------------------------------------------------------------------
+// The workgroups of every member's kernels of one phase, end to end: a
+// workgroup finds its member by bisection of the members' first
+// workgroups and runs that member's block with the member's own arguments
+// (read from a device table), sources, receivers and step counter.  The
+// group uses its own launch shapes (interior stress over 256 x 4 tiles,
+// z-marching velocity over 256 x 4 tiles and 8-plane chunks, absorber
+// rows by 256-cell segments and the x faces by cells); each cell sees the
+// operations of the member's own run whatever its tuning, so results equal
+// the members' own steps bit for bit.
+constexpr int G3_SBY = 4, G3_VBY = 4, G3_VKC = 8;
+
+struct Member3 {
+    K3<float> s, v;          // stress- and velocity-phase arguments (group tilings; pml = the x faces)
+    Rows3 rows;              // the whole-row absorber boxes
+    unsigned nrow, nxc;      // absorber workgroups: row segments, then 256-cell blocks of the x faces
+    int nsrc, nt_src;
+    const int *src;
+    const double *m6, *sdrop;
+    int nst, ntdec_w, ntw;
+    const int *st;
+    double *uacc;
+    float *wvel;
+    double *wdisp;
+    int *dit;
+};
+
+__device__ __forceinline__ int member3_of(const unsigned *__restrict__ first, int nm, unsigned b)
+{
+    int lo = 0, hi = nm - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <bool VEL>
+__global__ void __launch_bounds__(256) k3_pml_group(const Member3 *__restrict__ mem, const unsigned *__restrict__ first,
+                                                    int nm)
+{
+    const int m = member3_of(first, nm, blockIdx.x);
+    const Member3 &g = mem[m];
+    const unsigned l = blockIdx.x - first[m];
+    const K3<float> &a = VEL ? g.v : g.s;
+    int i, j, k;
+    const bool on = l < g.nrow ? pml_row(g.rows, l, a.nx, i, j, k)
+                               : pml_cell(a.pml, (l - g.nrow) * 256u + threadIdx.x, i, j, k);
+    if (!on) return;
+    if constexpr (VEL) vel_pml_cell<float, false>(a, i, j, k);
+    else stress_pml_cell<float, false>(a, i, j, k);
+}
+
+__global__ void __launch_bounds__(64 * G3_SBY) k3x_stress_group(const Member3 *__restrict__ mem,
+                                                                const unsigned *__restrict__ first, int nm)
+{
+    const int m = member3_of(first, nm, blockIdx.x);
+    stress4_blk<G3_SBY, false, 0>(mem[m].s, blockIdx.x - first[m]);
+}
+
+__global__ void __launch_bounds__(64 * G3_VBY) k3x_velz_group(const Member3 *__restrict__ mem,
+                                                              const unsigned *__restrict__ first, int nm)
+{
+    const int m = member3_of(first, nm, blockIdx.x);
+    velz_blk<G3_VBY, false, true>(mem[m].v, blockIdx.x - first[m]);
+}
+
+// receivers of every member (before the step's updates): one workgroup each
+__global__ void __launch_bounds__(64) k3_recv_group(const Member3 *__restrict__ mem)
+{
+    const Member3 &g = mem[blockIdx.x];
+    for (int n = (int)threadIdx.x; n < g.nst; n += 64)
+        receiver_one<float>(g.s, n, g.nst, g.st, g.uacc, g.wvel, g.wdisp, g.ntdec_w, g.ntw, g.dit);
+}
+
+__global__ void __launch_bounds__(64) k3_glut_group(const Member3 *__restrict__ mem)
+{
+    const Member3 &g = mem[blockIdx.x];
+    if (threadIdx.x == 0 && g.nsrc > 0) stressglut_one<float>(g.s, g.nsrc, g.src, g.m6, g.sdrop, g.nt_src, g.dit);
+}
+
+__global__ void k3_group_advance(const Member3 *__restrict__ mem, int n)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) *mem[q].dit = *mem[q].dit + 1;
+}
+
+struct Group3 {
+    DBuf mem, first_p, first_s, first_v;
+    unsigned np = 0, ns = 0, nv = 0;
+    int nm = 0;
+};
+
+bool group3_eligible(const swpc3d_ctx *c)
+{
+    return c && ready(c) == SWPC_OK && !fp64(c) && !c->s16 && !c->m16 && c->rho.p && c->x4 &&
+           (c->x4_form == 1 || c->x4_form == 2) && c->x4_sz == 0 && c->x4_vz > 0 && !c->pml4 && !c->lab && !c->conc &&
+           !c->prof.on && c->snaps.empty() && c->nbf == 0;
+}
+
+// The member's table entry and its workgroup counts per phase (absorber,
+// interior stress, interior velocity).
+static void member3_args(swpc3d_ctx *c, Member3 &g, unsigned &np, unsigned &ns, unsigned &nv)
+{
+    memset(&g, 0, sizeof g);
+    const swpc3d_desc &d = c->d;
+    K3<float> a = make_args<float>(c);
+    a.pml = c->boxes_x;
+    const int nk = d.nz - d.na;
+    g.s = a;
+    (void)x4_tiles(c, g.s, G3_SBY);
+    g.s.xcd = 0;
+    ns = (g.s.ntx > 0 && g.s.nty > 0 && nk > 0) ? (unsigned)g.s.ntx * (unsigned)g.s.nty * (unsigned)nk : 0u;
+    g.v = a;
+    (void)x4_tiles(c, g.v, G3_VBY);
+    g.v.xcd = 0;
+    g.v.kc = G3_VKC;
+    g.v.nkc = nk > 0 ? (nk + G3_VKC - 1) / G3_VKC : 0;
+    g.v.nblk = (unsigned)std::max(g.v.ntx, 0) * (unsigned)std::max(g.v.nty, 0) * (unsigned)g.v.nkc;
+    g.v.nblk_pad = g.v.nblk;
+    nv = g.v.nblk;
+    g.rows = c->rows;
+    g.nrow = c->boxes.total > 0 ? c->rows.total * (unsigned)c->rows.nseg : 0u;
+    g.nxc = c->boxes.total > 0 ? (c->boxes_x.total + 255u) / 256u : 0u;
+    np = g.nrow + g.nxc;
+    g.nsrc = c->nsrc;
+    g.nt_src = c->nt_src;
+    g.src = (const int *)c->src.p;
+    g.m6 = (const double *)c->m6.p;
+    g.sdrop = (const double *)c->sdrop.p;
+    g.nst = c->nst;
+    g.ntdec_w = c->ntdec_w;
+    g.ntw = c->ntw;
+    g.st = (const int *)c->st.p;
+    g.uacc = (double *)c->uacc.p;
+    g.wvel = (float *)c->wvel.p;
+    g.wdisp = (double *)c->wdisp.p;
+    g.dit = (int *)c->dit.p;
+}
+
+void group3_free(Group3 *g)
+{
+    if (!g) return;
+    dfree(g->mem);
+    dfree(g->first_p);
+    dfree(g->first_s);
+    dfree(g->first_v);
+    delete g;
+}
+
+int group3_build(const std::vector<swpc3d_ctx *> &cs, Group3 **out, long long &bytes)
+{
+    *out = nullptr;
+    if (cs.empty()) return SWPC_OK;
+    std::vector<Member3> mem(cs.size());
+    std::vector<unsigned> fp(cs.size()), fs(cs.size()), fv(cs.size());
+    unsigned long long np = 0, ns = 0, nv = 0;
+    for (size_t q = 0; q < cs.size(); q++) {
+        if (!group3_eligible(cs[q])) return fail(SWPC_EINVAL, "3-D member %zu cannot join a group", q);
+        unsigned a, b, v;
+        member3_args(cs[q], mem[q], a, b, v);
+        fp[q] = (unsigned)np;
+        fs[q] = (unsigned)ns;
+        fv[q] = (unsigned)nv;
+        np += a;
+        ns += b;
+        nv += v;
+    }
+    if (np >= (1ULL << 31) || ns >= (1ULL << 31) || nv >= (1ULL << 31))
+        return fail(SWPC_EINVAL, "3-D group too large for one launch");
+    auto *g = new Group3();
+    g->nm = (int)cs.size();
+    g->np = (unsigned)np;
+    g->ns = (unsigned)ns;
+    g->nv = (unsigned)nv;
+    int r = dalloc(g->mem, mem.size() * sizeof(Member3), bytes);
+    if (!r) r = dalloc(g->first_p, fp.size() * 4, bytes);
+    if (!r) r = dalloc(g->first_s, fs.size() * 4, bytes);
+    if (!r) r = dalloc(g->first_v, fv.size() * 4, bytes);
+    hipError_t e = hipSuccess;
+    if (!r) e = hipMemcpy(g->mem.p, mem.data(), mem.size() * sizeof(Member3), hipMemcpyHostToDevice);
+    if (!r && e == hipSuccess) e = hipMemcpy(g->first_p.p, fp.data(), fp.size() * 4, hipMemcpyHostToDevice);
+    if (!r && e == hipSuccess) e = hipMemcpy(g->first_s.p, fs.data(), fs.size() * 4, hipMemcpyHostToDevice);
+    if (!r && e == hipSuccess) e = hipMemcpy(g->first_v.p, fv.data(), fv.size() * 4, hipMemcpyHostToDevice);
+    if (r || e != hipSuccess) {
+        group3_free(g);
+        return r ? r : fail(SWPC_EHIP, "3-D group tables: %s", hipGetErrorString(e));
+    }
+    *out = g;
+    return SWPC_OK;
+}
+
+// One step of every member, in the order of step_dev: receivers, absorber
+// and interior stress, stress glut, absorber and interior velocity, step
+// counters.
+int group3_step(Group3 *g, hipStream_t s)
+{
+    const Member3 *m = (const Member3 *)g->mem.p;
+    hipLaunchKernelGGL(k3_recv_group, dim3(g->nm), dim3(64), 0, s, m);
+    if (g->np > 0)
+        hipLaunchKernelGGL(k3_pml_group<false>, dim3(g->np), dim3(256), 0, s, m, (const unsigned *)g->first_p.p, g->nm);
+    if (g->ns > 0)
+        hipLaunchKernelGGL(k3x_stress_group, dim3(g->ns), dim3(64, G3_SBY), 0, s, m, (const unsigned *)g->first_s.p,
+                           g->nm);
+    hipLaunchKernelGGL(k3_glut_group, dim3(g->nm), dim3(64), 0, s, m);
+    if (g->np > 0)
+        hipLaunchKernelGGL(k3_pml_group<true>, dim3(g->np), dim3(256), 0, s, m, (const unsigned *)g->first_p.p, g->nm);
+    if (g->nv > 0)
+        hipLaunchKernelGGL(k3x_velz_group, dim3(g->nv), dim3(64, G3_VBY), 0, s, m, (const unsigned *)g->first_v.p,
+                           g->nm);
+    hipLaunchKernelGGL(k3_group_advance, dim3((g->nm + 63) / 64), dim3(64), 0, s, m, g->nm);
+    SWPC_CHECK_LAUNCH();
+    return SWPC_OK;
+}
 
 // Fortran (k,i,j) memory box -> device layout (row-blocked transposition).
 template <typename S, typename D>

@@ -34,4 +34,14 @@ int group2_build(const std::vector<swpc2d_ctx *> &cs, Group2 **out, long long &b
 int group2_step(Group2 *g, hipStream_t s);   // one step of every member, enqueued on s
 void group2_free(Group2 *g);
 
+// A group of 3-D members stepped together, one launch per phase for all of
+// them (FP32 with a density, the one-pass stress and z-marching velocity
+// forms, no snapshots, no body force); results equal each member's own
+// steps bit for bit.
+struct Group3;
+bool group3_eligible(const swpc3d_ctx *c);
+int group3_build(const std::vector<swpc3d_ctx *> &cs, Group3 **out, long long &bytes);
+int group3_step(Group3 *g, hipStream_t s);
+void group3_free(Group3 *g);
+
 }  // namespace swpc

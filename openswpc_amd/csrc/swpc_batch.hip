@@ -5,9 +5,13 @@
 // stream, step counter and state, so results equal the members' own runs;
 // the graph removes the per-kernel launch cost of the many small grids
 // and lets the runtime overlap the branches.  With grouping on (the
-// default) the FP32 P-SV members that qualify (group2_eligible) are not
-// branches: one group advances all of them with one launch per phase
-// (three per step), the workgroups of every member's step end to end.
+// default for P-SV) the members that qualify are not branches: one group
+// advances all FP32 P-SV members (group2_eligible) with one launch per
+// phase (three per step), and, when asked for, another all FP32 3-D
+// members (group3_eligible) with seven launches per step, the workgroups
+// of every member's kernels of a phase end to end.  The P-SV group runs on
+// the batch stream and the 3-D group on a stream of its own, beside the
+// branches.
 #include "swpc_batch.h"
 
 #include <algorithm>
@@ -39,8 +43,11 @@ struct swpc_batch {
     int graph_k = 0;             // steps in the captured graph
     swpc::DBuf dits;             // device array of the members' step counters
     long long bytes = 0;
-    bool grouping = true;        // P-SV members in one group (swpc_batch_set_grouping)
-    swpc::Group2 *group = nullptr;   // the group of the captured graph
+    int grouping = 1;            // groups: bit 0 P-SV, bit 1 3-D members (swpc_batch_set_grouping)
+    swpc::Group2 *group = nullptr;   // the P-SV group of the captured graph
+    std::vector<swpc::Group3 *> groups3;   // its 3-D group (none or one)
+    hipStream_t stream3 = nullptr;   // the 3-D group's stream
+    hipEvent_t ev3 = nullptr;
     long long group_bytes = 0;
 };
 
@@ -55,6 +62,8 @@ static void batch_drop(swpc_batch *b)
     b->graph_k = 0;
     group2_free(b->group);
     b->group = nullptr;
+    for (auto *g : b->groups3) group3_free(g);
+    b->groups3.clear();
     b->bytes -= b->group_bytes;
     b->group_bytes = 0;
 }
@@ -75,7 +84,7 @@ static int batch_capture(swpc_batch *b, int k)
     // the group: every qualifying P-SV member, stepped on the batch stream
     std::vector<swpc2d_ctx *> gc;
     std::vector<char> grouped(b->mem.size(), 0);
-    if (b->grouping)
+    if (b->grouping & 1)
         for (size_t q = 0; q < b->mem.size(); q++)
             if (b->mem[q].c2 && group2_eligible(b->mem[q].c2)) {
                 gc.push_back(b->mem[q].c2);
@@ -84,10 +93,30 @@ static int batch_capture(swpc_batch *b, int k)
     if (gc.size() >= 2) {
         long long gb = 0;
         if (int r = group2_build(gc, &b->group, gb)) return r;
-        b->group_bytes = gb;
+        b->group_bytes += gb;
         b->bytes += gb;
     } else {
-        std::fill(grouped.begin(), grouped.end(), 0);
+        for (size_t q = 0; q < b->mem.size(); q++)
+            if (b->mem[q].c2) grouped[q] = 0;
+    }
+    // the 3-D group: every qualifying 3-D member
+    if (b->grouping & 2) {
+        std::vector<swpc3d_ctx *> g3;
+        std::vector<size_t> idx;
+        for (size_t q = 0; q < b->mem.size(); q++)
+            if (b->mem[q].c3 && group3_eligible(b->mem[q].c3)) {
+                g3.push_back(b->mem[q].c3);
+                idx.push_back(q);
+            }
+        if (g3.size() >= 2) {
+            Group3 *g = nullptr;
+            long long gb = 0;
+            if (int r = group3_build(g3, &g, gb)) return r;
+            b->groups3.push_back(g);
+            b->group_bytes += gb;
+            b->bytes += gb;
+            for (size_t q : idx) grouped[q] = 1;
+        }
     }
     SWPC_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     int r = SWPC_OK;
@@ -100,9 +129,18 @@ static int batch_capture(swpc_batch *b, int k)
         for (int s = 0; s < k && r == SWPC_OK && e == hipSuccess; s++) r = member_step(x);
         if (r == SWPC_OK && e == hipSuccess) e = hipEventRecord(x.ev, x.m.stream);
     }
-    // the group runs on the batch stream beside the branches, which join after it
+    // the 3-D groups run on their own stream and the P-SV group on the batch
+    // stream, beside the branches; all join the batch stream at the end
+    if (!b->groups3.empty() && e == hipSuccess) {
+        e = hipStreamWaitEvent(b->stream3, b->ev_fork, 0);
+        for (int s = 0; s < k && r == SWPC_OK && e == hipSuccess; s++)
+            for (auto *g : b->groups3)
+                if (r == SWPC_OK) r = group3_step(g, b->stream3);
+        if (r == SWPC_OK && e == hipSuccess) e = hipEventRecord(b->ev3, b->stream3);
+    }
     if (b->group)
         for (int s = 0; s < k && r == SWPC_OK && e == hipSuccess; s++) r = group2_step(b->group, b->stream);
+    if (!b->groups3.empty() && r == SWPC_OK && e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->ev3, 0);
     for (size_t q = 0; q < b->mem.size() && r == SWPC_OK && e == hipSuccess; q++)
         if (!grouped[q]) e = hipStreamWaitEvent(b->stream, b->mem[q].ev, 0);
     hipGraph_t g = nullptr;
@@ -137,8 +175,10 @@ int swpc_batch_create(swpc_batch **out)
     if (!out) return fail(SWPC_EINVAL, "null argument");
     *out = nullptr;
     auto *b = new swpc_batch();
-    if (hipStreamCreate(&b->stream) != hipSuccess || hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess) {
+    if (hipStreamCreate(&b->stream) != hipSuccess || hipStreamCreate(&b->stream3) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev3, hipEventDisableTiming) != hipSuccess) {
         swpc_batch_destroy(b);
         return fail(SWPC_EHIP, "batch stream/event creation failed");
     }
@@ -156,6 +196,8 @@ int swpc_batch_destroy(swpc_batch *b)
     dfree(b->dits);
     if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
     if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+    if (b->ev3) (void)hipEventDestroy(b->ev3);
+    if (b->stream3) (void)hipStreamDestroy(b->stream3);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return SWPC_OK;
@@ -239,12 +281,13 @@ int swpc_batch_run(swpc_batch *b, int it0, int nsteps, int graph_k)
     return SWPC_OK;
 }
 
-int swpc_batch_set_grouping(swpc_batch *b, int on)
+int swpc_batch_set_grouping(swpc_batch *b, int mode)
 {
     clear_error();
     if (!b) return fail(SWPC_EINVAL, "null batch");
+    if (mode < 0 || mode > 3) return fail(SWPC_EINVAL, "grouping mode %d: a bit mask of 1 (P-SV) and 2 (3-D)", mode);
     batch_drop(b);
-    b->grouping = on != 0;
+    b->grouping = mode;
     return SWPC_OK;
 }
 

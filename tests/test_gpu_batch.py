@@ -114,3 +114,72 @@ def test_psv_group_equals_branches_and_own_runs():
         assert np.isfinite(ref).all() and np.abs(ref).max() > 0, q
         assert np.array_equal(out["group"][q], ref), ("group", q)
         assert np.array_equal(out["branches"][q], ref), ("branches", q)
+
+
+def test_3d_group_equals_branches_and_own_runs():
+    """The 3-D group (one launch per phase for every qualifying FP32 3-D
+    member) against the same batch with every member a graph branch and
+    against each member's own run, bit for bit, receivers and every field:
+    four FP32 3-D members of uneven size (a ragged one with a random medium
+    set up on the GPU, one 300 cells wide whose right x face lies past its
+    first interior tile, one retuned to other launch shapes, which the
+    group does not use), one FP64 member and one with a body force (both
+    stay branches), beside a P-SV member."""
+    from openswpc_amd.solver import FIELDS3, Swpc3D
+    nt = 70
+    base = _cases(nt)
+    cases = [base[0], base[1], base[2],
+             ("3d", problem.case3d(71, 45, 39, 0.25, 0.01, nt, na=6, seed=21, nst_side=2, src_depth=2.5, zbeg=-0.75,
+                                   rmed=(0.05, 2.0, 0.5), rmed_tile=(32, 32, 32), device_medium=True)),
+             ("3d", problem.case3d(300, 40, 36, 0.25, 0.01, nt, na=8, seed=22, nst_side=2, src_depth=3.0, zbeg=-0.75,
+                                   src_xy=(33.0, 0.3), st_spacing=4.0)),
+             ("3d", problem.case3d(44, 40, 34, 0.25, 0.01, nt, na=8, seed=23, nst_side=2, src_depth=3.0, zbeg=-0.75)),
+             ("3d", problem.case3d(46, 42, 34, 0.25, 0.01, nt, na=8, seed=24, nst_side=2, src_depth=3.0, zbeg=-0.75,
+                                   bodyforce=True))]
+    fp64_member, retuned = 5, 2
+
+    def open_all():
+        sims = []
+        for q, (kind, c) in enumerate(cases):
+            if q == fp64_member:
+                sims.append(Swpc3D(c.grid, c.medium, c.pml, c.glut, c.stations, c.ntdec_w, nt, precision="fp64"))
+            else:
+                sims.append(batch.open_case(kind, c))
+        # entries 0..15 of swpc3d_set_tuning: 256 x 2 staged stress, velocity tile rows 8, chunk 16
+        sims[retuned].set_tuning(4, 3, 0, 4, 5, 0, 32, 0, 1, 2, 0, 1, 8, 0, 16, 0)
+        return sims
+
+    fields = ("Vx", "Vy", "Vz", "Sxx", "Syz", "Sxy", "Rxx1", "Rxz3", "axVx", "ayVy", "azSzz", "axSxy")
+    assert set(fields) <= set(FIELDS3)
+    out = {}
+    for mode in ("own", "group", "branches"):
+        sims = open_all()
+        if mode == "own":
+            for s in sims:
+                s.run(1, nt)
+        else:
+            with batch.Batch() as b:
+                b.set_grouping(psv=mode == "group", d3=mode == "group")
+                for s in sims:
+                    b.add(s)
+                b.run(1, nt, 16)
+                b.sync()
+        res = []
+        for (kind, _), s in zip(cases, sims):
+            res.append(s.receivers()[0].copy())
+            if kind == "3d":
+                res += [s.field(f) for f in fields]
+        out[mode] = res
+        for s in sims:
+            s.close()
+    assert len(out["own"]) == len(out["group"]) == len(out["branches"])
+    for q, ref in enumerate(out["own"]):
+        assert np.isfinite(ref).all(), q
+        assert np.array_equal(out["group"][q], ref), ("group", q)
+        assert np.array_equal(out["branches"][q], ref), ("branches", q)
+    # every member moved (the wide member's source sits 2.5 km from its
+    # right absorber, far from its receivers: its fields carry the signal)
+    q = 0
+    for kind, _ in cases:
+        assert np.abs(out["own"][q + (1 if kind == "3d" else 0)]).max() > 0, q
+        q += 1 + (len(fields) if kind == "3d" else 0)
