@@ -139,8 +139,8 @@ def main():
                    "lab_r16_same_scheme": None, "lab_fp32_same_scheme": None}
         else:
             mis = misfit(case, args.misfit_steps, icase, weights)
-        reduced = {"form": "reduced storage (fp32_r16): FP32 arithmetic, V, S and medium; memory variables in FP16 "
-                           "with a power-of-two scale, PML ADE arrays in BF16; 20-cell ADE-PML",
+        reduced = {"form": "reduced storage (fp32_r16): FP32 arithmetic, V, S, medium and ADE arrays; memory "
+                           "variables in FP16 with a power-of-two scale; 20-cell ADE-PML",
                    "ms_per_step": ms_red, "value": ncell * args.steps / red["wall"],
                    "speedup_vs_plain": ms_step / ms_red, "misfit_vs_fp64": mis["r16"]}
         log(f"reduced storage: {ms_red:.3f} ms/step, speedup {ms_step / ms_red:.2f}x, misfit {mis['r16']}")

@@ -32,8 +32,8 @@ extern "C" {
 enum swpc_precision {
     SWPC_FP32 = 0,        /* state, memory variables and arithmetic in FP32 (reference built with MP = SP) */
     SWPC_FP64 = 1,        /* the same scheme in FP64 */
-    SWPC_FP32_S16 = 2,    /* FP32 arithmetic and V/S; memory variables and medium stored in 16 bit with per-array scale */
-    SWPC_FP32_R16 = 3     /* FP32 arithmetic, V/S and medium; memory variables (FP16, scaled) and ADE arrays (BF16) in 16 bit */
+    SWPC_FP32_S16 = 2,    /* FP32 arithmetic and V/S; memory variables, ADE arrays (BF16) and medium stored in 16 bit */
+    SWPC_FP32_R16 = 3     /* FP32 arithmetic, V/S, medium and ADE arrays; memory variables in 16 bit (FP16, scaled) */
 };
 
 enum swpc_error {
@@ -267,12 +267,11 @@ int swpc3d_memory(swpc3d_ctx *ctx, long long *bytes);
              hint (0/5), XCD-contiguous workgroup order (0/1), stress
              z-chunk, velocity z-chunk (0 = automatic);
      [8]     4-cell-per-thread kernels for FP32 and reduced storage (0/1,
-             default 1); [9] their tile rows (1/2/4; default 4 for FP32,
-             2 for 16-bit storage);
+             default 1); [9] their tile rows (1/2/4, default 2);
      [10]    4-cell-per-thread PML kernels (0/1, default 0);
      [11]    4-wide stress form: 0 normal + shear launches, 1 one pass with
-             staged loads, 2 one pass (default 2 for FP32, 1 for reduced
-             storage, 0 for the 16-bit medium); [12] tile rows of the
+             staged loads, 2 one pass (default 1; 0 for the 16-bit
+             medium); [12] tile rows of the
              z-marching velocity kernel (1/2/4/8, default 4; 2 for the
              16-bit medium);
      [13]    PML kernels on a second stream, concurrent with the interior

@@ -24,7 +24,7 @@ VEL = [(4, 0), (2, 0), (8, 0), (4, 5), (8, 5)]
 STRESS4 = [(0, 0), (1, 0), (2, 0), (1, 4), (1, 8), (1, 16)]
 VEL4 = [0, 8, 16, 32]
 # tuning vector defaults of the first eleven entries (see swpc3d_set_tuning)
-BASE = [4, 3, 0, 4, 5, 0, 32, 0, 1, 4, 0]
+BASE = [4, 3, 0, 4, 5, 0, 32, 0, 1, 2, 0]
 
 
 def _time(sim, it0, steps, cls):

@@ -2091,8 +2091,9 @@ struct swpc3d_ctx {
     swpc::DBuf lab_colmed;  // per-column medium features
     swpc::LabDomain lab_dom;
     float lab_sponge = swpc::LAB_SPONGE;  // strip sponge strength (swpc3d_set_lab_sponge)
-    bool s16 = false;       // memory variables and ADE arrays in 16 bit (SWPC_FP32_S16, SWPC_FP32_R16)
+    bool s16 = false;       // memory variables in 16 bit (SWPC_FP32_S16, SWPC_FP32_R16)
     bool m16 = false;       // medium in 16 bit too (SWPC_FP32_S16)
+    bool a16 = false;       // ADE arrays in BF16 (SWPC_FP32_S16)
     float r_scale = 1.0f;   // memory-variable scale of the reduced storage
     float r_scale0 = 1.0f;  // ... as derived from the sources, before r_shift
     int r_shift = 0;        // log2 adjustment of r_scale (tuning entry 18)
@@ -2500,7 +2501,7 @@ static int launch_stress(swpc3d_ctx *c)
         if constexpr (sizeof(T) == 4) {
             if (c->pml4) {
                 const unsigned nb = (c->boxes4.total + 255) / 256;
-                if (c->s16) hipLaunchKernelGGL((k3x_stress_pml<true>), dim3(nb), dim3(256), 0, ps, a, c->boxes4);
+                if (c->a16) hipLaunchKernelGGL((k3x_stress_pml<true>), dim3(nb), dim3(256), 0, ps, a, c->boxes4);
                 else hipLaunchKernelGGL((k3x_stress_pml<false>), dim3(nb), dim3(256), 0, ps, a, c->boxes4);
                 done = true;
             }
@@ -2509,13 +2510,13 @@ static int launch_stress(swpc3d_ctx *c)
             K3<T> b = a;
             if (c->pml_rows && c->rows.total > 0) {
                 unsigned per, nb = pml_launch(c->rows.total * (unsigned)c->rows.nseg, c->pml_xcd, per);
-                if (c->s16) hipLaunchKernelGGL((k3_stress_pml_rows<T, true>), dim3(nb), dim3(256), 0, ps, a, c->rows, per);
+                if (c->a16) hipLaunchKernelGGL((k3_stress_pml_rows<T, true>), dim3(nb), dim3(256), 0, ps, a, c->rows, per);
                 else hipLaunchKernelGGL((k3_stress_pml_rows<T, false>), dim3(nb), dim3(256), 0, ps, a, c->rows, per);
                 b.pml = c->boxes_x;
             }
             if (b.pml.total > 0) {
                 unsigned per, nb = pml_launch((b.pml.total + 255) / 256, c->pml_xcd, per);
-                if (c->s16) hipLaunchKernelGGL((k3_stress_pml<T, true>), dim3(nb), dim3(256), 0, ps, b, per);
+                if (c->a16) hipLaunchKernelGGL((k3_stress_pml<T, true>), dim3(nb), dim3(256), 0, ps, b, per);
                 else hipLaunchKernelGGL((k3_stress_pml<T, false>), dim3(nb), dim3(256), 0, ps, b, per);
             }
         }
@@ -2568,7 +2569,7 @@ static int launch_vel(swpc3d_ctx *c)
         if constexpr (sizeof(T) == 4) {
             if (c->pml4) {
                 const unsigned nb = (c->boxes4.total + 255) / 256;
-                if (c->s16) hipLaunchKernelGGL((k3x_vel_pml<true>), dim3(nb), dim3(256), 0, ps, a, c->boxes4);
+                if (c->a16) hipLaunchKernelGGL((k3x_vel_pml<true>), dim3(nb), dim3(256), 0, ps, a, c->boxes4);
                 else hipLaunchKernelGGL((k3x_vel_pml<false>), dim3(nb), dim3(256), 0, ps, a, c->boxes4);
                 done = true;
             }
@@ -2577,13 +2578,13 @@ static int launch_vel(swpc3d_ctx *c)
             K3<T> b = a;
             if (c->pml_rows && c->rows.total > 0) {
                 unsigned per, nb = pml_launch(c->rows.total * (unsigned)c->rows.nseg, c->pml_xcd, per);
-                if (c->s16) hipLaunchKernelGGL((k3_vel_pml_rows<T, true>), dim3(nb), dim3(256), 0, ps, a, c->rows, per);
+                if (c->a16) hipLaunchKernelGGL((k3_vel_pml_rows<T, true>), dim3(nb), dim3(256), 0, ps, a, c->rows, per);
                 else hipLaunchKernelGGL((k3_vel_pml_rows<T, false>), dim3(nb), dim3(256), 0, ps, a, c->rows, per);
                 b.pml = c->boxes_x;
             }
             if (b.pml.total > 0) {
                 unsigned per, nb = pml_launch((b.pml.total + 255) / 256, c->pml_xcd, per);
-                if (c->s16) hipLaunchKernelGGL((k3_vel_pml<T, true>), dim3(nb), dim3(256), 0, ps, b, per);
+                if (c->a16) hipLaunchKernelGGL((k3_vel_pml<T, true>), dim3(nb), dim3(256), 0, ps, b, per);
                 else hipLaunchKernelGGL((k3_vel_pml<T, false>), dim3(nb), dim3(256), 0, ps, b, per);
             }
         }
@@ -3114,13 +3115,16 @@ int swpc3d_create(const swpc3d_desc *desc, swpc3d_ctx **out)
     c->L = L;
     c->esz = d.precision == SWPC_FP64 ? 8 : 4;
     c->s16 = d.precision == SWPC_FP32_S16 || d.precision == SWPC_FP32_R16;
+    // ADE arrays in BF16 with the 16-bit medium only: the whole-row absorber
+    // kernels run slower on 2-byte ADE accesses (stress 236 against 205,
+    // velocity 191 against 170 us per step at 512 x 512 x 256) than the
+    // halved bytes save, so reduced storage keeps them in FP32
+    c->a16 = d.precision == SWPC_FP32_S16;
     c->m16 = d.precision == SWPC_FP32_S16;
-    // measured defaults at 512 x 512 x 256 on MI355X: one-pass stress (the
-    // two-launch form for the 16-bit medium; for FP32 the free form over
-    // 256 x 4 tiles, 1.5 % faster than the staged form over 256 x 2),
-    // z-marching velocity
-    c->x4_form = c->m16 ? 0 : (c->s16 ? 1 : 2);
-    c->x4_by = c->s16 ? 2 : 4;
+    // measured defaults at 512 x 512 x 256 on MI355X: one-pass stress over
+    // 256 x 2 tiles (the two-launch form for the 16-bit medium), z-marching
+    // velocity
+    c->x4_form = c->m16 ? 0 : 1;
     c->x4_vz = c->m16 ? 16 : 8;
     c->x4_vby = c->m16 ? 2 : 4;
     SWPC_HIP(hipStreamCreate(&c->stream));
@@ -3131,10 +3135,14 @@ int swpc3d_create(const swpc3d_desc *desc, swpc3d_ctx **out)
     int r;
     for (auto &b : c->V) if ((r = dalloc(b, vb, c->bytes))) return r;
     for (auto &b : c->S) if ((r = dalloc(b, vb, c->bytes))) return r;
-    const size_t vb16 = (size_t)L.vol * 2;
-    for (auto &b : c->R) if ((r = dalloc(b, c->s16 ? vb16 : vb, c->bytes))) return r;
-    for (auto &b : c->A) if ((r = dalloc(b, c->s16 ? vb16 : vb, c->bytes))) return r;
-    for (auto &b : c->med) if ((r = dalloc(b, c->m16 ? vb16 : (size_t)L.vol * 4, c->bytes))) return r;
+    // 16-bit arrays start 64 bytes into a line: cell i = 1 sits at element
+    // IO + 1 = 32, so its rows then begin on 128-byte lines as the FP32
+    // arrays' do, and a wave's 4-cell accesses cover whole lines
+    const size_t vb16 = (size_t)L.vol * 2, lead16 = 128 - (size_t)(IO + 1) * 2;
+    for (auto &b : c->R) if ((r = dalloc(b, c->s16 ? vb16 : vb, c->bytes, c->s16 ? lead16 : 0))) return r;
+    for (auto &b : c->A) if ((r = dalloc(b, c->a16 ? vb16 : vb, c->bytes, c->a16 ? lead16 : 0))) return r;
+    for (auto &b : c->med)
+        if ((r = dalloc(b, c->m16 ? vb16 : (size_t)L.vol * 4, c->bytes, c->m16 ? lead16 : 0))) return r;
     if (c->m16)
         for (auto &b : c->mbs) if ((r = dalloc(b, (size_t)L.pz * 8, c->bytes))) return r;
     if ((r = dalloc(c->surf, (size_t)d.nx * d.ny * 16, c->bytes))) return r;
@@ -3841,7 +3849,7 @@ int swpc3d_get_field(swpc3d_ctx *c, int f, double *out)
         std::vector<double> h(n);
         SWPC_HIP(hipMemcpy(h.data(), b->p, n * 8, hipMemcpyDeviceToHost));
         dev_to_box(c->L, h.data(), out);
-    } else if (c->s16 && f >= SWPC3D_R0) {
+    } else if (c->s16 && f >= SWPC3D_R0 && (f < SWPC3D_ADE0 || c->a16)) {
         std::vector<uint16_t> raw(n);
         SWPC_HIP(hipMemcpy(raw.data(), b->p, n * 2, hipMemcpyDeviceToHost));
         std::vector<float> h(n);
@@ -3880,7 +3888,7 @@ int swpc3d_set_field(swpc3d_ctx *c, int f, const double *in)
         std::vector<double> h(n, 0.0);
         box_to_dev(c->L, in, h.data());
         SWPC_HIP(hipMemcpy(b->p, h.data(), n * 8, hipMemcpyHostToDevice));
-    } else if (c->s16 && f >= SWPC3D_R0) {
+    } else if (c->s16 && f >= SWPC3D_R0 && (f < SWPC3D_ADE0 || c->a16)) {
         std::vector<float> h(n, 0.0f);
         box_to_dev(c->L, in, h.data());
         std::vector<uint16_t> raw(n);

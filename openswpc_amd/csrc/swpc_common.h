@@ -56,7 +56,8 @@ inline Lay3 make_lay3(int nx, int ny, int nz)
     Lay3 L;
     L.nx = nx; L.ny = ny; L.nz = nz;
     int need = IO + nx + 4;                // i = nx + 3 is the last element
-    L.px = (need + 31) / 32 * 32;
+    // whole 128-byte lines per row for FP32 and for 16-bit arrays
+    L.px = (need + 63) / 64 * 64;
     L.py = ny + 6;
     L.pz = nz + 6;
     L.sxy = L.px * L.py;
@@ -96,7 +97,9 @@ struct DBuf {
     void *base = nullptr;    // the allocation (p may start past it)
 };
 
-int dalloc(DBuf &b, size_t bytes, long long &total);
+// Device buffer of `bytes`, zeroed; b.p starts `lead` bytes past a
+// 128-byte line boundary (plus the stagger of large buffers).
+int dalloc(DBuf &b, size_t bytes, long long &total, size_t lead = 0);
 void dfree(DBuf &b);
 
 // Event-pair timing per kernel class.

@@ -28,11 +28,11 @@ void clear_error() { g_err.clear(); }
 constexpr size_t STAGGER = 2304;
 static unsigned g_stagger = 0;
 
-int dalloc(DBuf &b, size_t bytes, long long &total)
+int dalloc(DBuf &b, size_t bytes, long long &total, size_t lead)
 {
     dfree(b);
     if (bytes == 0) return SWPC_OK;
-    const size_t off = bytes >= ((size_t)1 << 24) ? (size_t)(g_stagger++ % 32) * STAGGER : 0;
+    const size_t off = (bytes >= ((size_t)1 << 24) ? (size_t)(g_stagger++ % 32) * STAGGER : 0) + lead;
     void *base = nullptr;
     hipError_t e = hipMalloc(&base, bytes + off);
     if (e != hipSuccess) {

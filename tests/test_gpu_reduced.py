@@ -1,8 +1,8 @@
 """Reduced-storage forms against the FP64 run of the same scheme: FP32
-arithmetic; memory variables in FP16 with a per-array power-of-two scale and
-ADE arrays in BF16; the medium either as an FP32 per-plane base plus an FP16
-residual (fp32_s16) or in FP32 (fp32_r16).  Bar: seismogram relative L2
-misfit <= 1e-3.
+arithmetic; memory variables in FP16 with a per-array power-of-two scale;
+the medium as an FP32 per-plane base plus an FP16 residual and the ADE
+arrays in BF16 (fp32_s16), or both in FP32 (fp32_r16).  Bar: seismogram
+relative L2 misfit <= 1e-3.
 """
 import numpy as np
 import pytest
