@@ -297,7 +297,11 @@ int swpc3d_memory(swpc3d_ctx *ctx, long long *bytes);
              run of them (0/1, default 0);
      [21]    z-marching velocity with the y-stencil rows of Syy, Sxy and
              Syz staged in LDS per plane (0/1, default 0; tile rows 2, 4
-             or 8 from entry [12]). */
+             or 8 from entry [12]);
+     [22]    the 4-wide one-pass stress and z-marching velocity kernels
+             take the x neighbours of a thread's four cells from the
+             adjacent lanes by wavefront shuffles instead of loads (0/1,
+             default 0). */
 int swpc3d_set_tuning(swpc3d_ctx *ctx, const int *p, int n);
 
 /* ------------------------------------------------------------------ */

@@ -563,6 +563,49 @@ __device__ __forceinline__ Ord order4(const K3<float> &a, int i, int j, int k)
     return o;
 }
 
+// x neighbours by wavefront shuffles: lane l holds cells i0 .. i0+3 and
+// lane l - 1 the four before them, so the cells i0-2, i0-1 (left) and
+// i0+4, i0+5 (right) a stencil needs come from the neighbour lanes'
+// registers.  A lane whose neighbour lane holds no interior cell (it has
+// returned) or lies outside the wave loads them instead (pass the loads
+// in ldl / ldr).
+struct XNb {
+    bool l, r;  // left / right neighbour lane active
+};
+
+__device__ __forceinline__ XNb x4_nb(const K3<float> &a, int i0)
+{
+    const unsigned lane = threadIdx.x;
+    XNb n;
+    n.l = lane > 0u && i0 - 1 > a.na;
+    n.r = lane < 63u && i0 + 4 <= a.nx - a.na;
+    return n;
+}
+
+// cells i0-2, i0-1 of array p (v holds i0 .. i0+3 of this lane)
+__device__ __forceinline__ float2 x4_left(const XNb &n, const f4v v, const float *p, int c)
+{
+    const float z = __shfl_up(v.z, 1), w = __shfl_up(v.w, 1);
+    if (n.l) return make_float2(z, w);
+    return ld2(p + c - 2);
+}
+
+// cells i0+4, i0+5
+__device__ __forceinline__ float2 x4_right(const XNb &n, const f4v v, const float *p, int c)
+{
+    const float x = __shfl_down(v.x, 1), y = __shfl_down(v.y, 1);
+    if (n.r) return make_float2(x, y);
+    return ld2(p + c + 4);
+}
+
+// cell i0+4
+__device__ __forceinline__ float x4_right1(const XNb &n, const f4v v, const float *p, int c)
+{
+    const float x = __shfl_down(v.x, 1);
+    if (n.r) return x;
+    return p[c + 4];
+}
+
 // normal stresses and their memory variables (m_kernel.f90:176-239)
 template <int BY, bool S16>
 __global__ void __launch_bounds__(64 * BY) k3x_normal(const K3<float> a)
@@ -709,7 +752,7 @@ __device__ __forceinline__ float harm4s(float a, float b, float c, float d)
 // loads, then the normal and the shear memory variables are streamed in
 // turn.  BAR = 1 keeps the compiler from hoisting the shear loads above the
 // normal stores, which bounds the registers held at once.
-template <int BY, bool S16, int BAR>
+template <int BY, bool S16, int BAR, bool XS = false>
 __device__ __forceinline__ void stress4_blk(const K3<float> &a, unsigned blk)
 {
     int i0, j, k;
@@ -720,19 +763,22 @@ __device__ __forceinline__ void stress4_blk(const K3<float> &a, unsigned blk)
     const float *__restrict__ Vx = a.V[0];
     const float *__restrict__ Vy = a.V[1];
     const float *__restrict__ Vz = a.V[2];
+    const XNb nb = XS ? x4_nb(a, i0) : XNb{false, false};
     f4v dxVx, dyVy, dzVz, dxVy_dyVx, dxVz_dzVx, dyVz_dzVy;
     {
         const f4v vx = ld4(Vx + c);
-        const float2 vxl = ld2(Vx + c - 2);
-        const float vxr = Vx[c + 4];
+        const float2 vxl = XS ? x4_left(nb, vx, Vx, c) : ld2(Vx + c - 2);
+        const float vxr = XS ? x4_right1(nb, vx, Vx, c) : Vx[c + 4];
         const f4v vxyp1 = ld4(Vx + c + px), vxyp2 = ld4(Vx + c + 2 * px), vxym1 = ld4(Vx + c - px);
         const f4v vxzp1 = ld4(Vx + c + sxy), vxzp2 = ld4(Vx + c + 2 * sxy), vxzm1 = ld4(Vx + c - sxy);
         const f4v vy = ld4(Vy + c), vym1 = ld4(Vy + c - px), vym2 = ld4(Vy + c - 2 * px), vyp1 = ld4(Vy + c + px);
         const f4v vyzp1 = ld4(Vy + c + sxy), vyzp2 = ld4(Vy + c + 2 * sxy), vyzm1 = ld4(Vy + c - sxy);
-        const float2 vyl = ld2(Vy + c - 2), vyr = ld2(Vy + c + 4);
+        const float2 vyl = XS ? x4_left(nb, vy, Vy, c) : ld2(Vy + c - 2);
+        const float2 vyr = XS ? x4_right(nb, vy, Vy, c) : ld2(Vy + c + 4);
         const f4v vz = ld4(Vz + c), vzm1 = ld4(Vz + c - sxy), vzm2 = ld4(Vz + c - 2 * sxy), vzp1 = ld4(Vz + c + sxy);
         const f4v vzyp1 = ld4(Vz + c + px), vzyp2 = ld4(Vz + c + 2 * px), vzym1 = ld4(Vz + c - px);
-        const float2 vzl = ld2(Vz + c - 2), vzr = ld2(Vz + c + 4);
+        const float2 vzl = XS ? x4_left(nb, vz, Vz, c) : ld2(Vz + c - 2);
+        const float2 vzr = XS ? x4_right(nb, vz, Vz, c) : ld2(Vz + c + 4);
         // x rows: Vx i0-2 .. i0+4, Vy and Vz i0-1 .. i0+5
         const float xs[7] = {vxl.x, vxl.y, vx.x, vx.y, vx.z, vx.w, vxr};
         const float ys[7] = {vyl.y, vy.x, vy.y, vy.z, vy.w, vyr.x, vyr.y};
@@ -810,9 +856,9 @@ __device__ __forceinline__ void stress4_blk(const K3<float> &a, unsigned blk)
             // (cached loads), instead of three more streamed arrays
             const float *__restrict__ M = a.med[MMU];
             const f4v m0 = ld4(M + c), mk = ld4(M + c + sxy), mj = ld4(M + c + px), mkj = ld4(M + c + sxy + px);
-            const float mi[5] = {m0.x, m0.y, m0.z, m0.w, M[c + 4]};
-            const float mki[5] = {mk.x, mk.y, mk.z, mk.w, M[c + sxy + 4]};
-            const float mji[5] = {mj.x, mj.y, mj.z, mj.w, M[c + px + 4]};
+            const float mi[5] = {m0.x, m0.y, m0.z, m0.w, XS ? x4_right1(nb, m0, M, c) : M[c + 4]};
+            const float mki[5] = {mk.x, mk.y, mk.z, mk.w, XS ? x4_right1(nb, mk, M + sxy, c) : M[c + sxy + 4]};
+            const float mji[5] = {mj.x, mj.y, mj.z, mj.w, XS ? x4_right1(nb, mj, M + px, c) : M[c + px + 4]};
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const float nnn = mi[q], pnn = mk[q], npn = mi[q + 1], ppn = mki[q + 1];
@@ -858,10 +904,10 @@ __device__ __forceinline__ void stress4_blk(const K3<float> &a, unsigned blk)
     }
 }
 
-template <int BY, bool S16, int BAR>
+template <int BY, bool S16, int BAR, bool XS>
 __global__ void __launch_bounds__(64 * BY) k3x_stress(const K3<float> a)
 {
-    stress4_blk<BY, S16, BAR>(a, blockIdx.x);
+    stress4_blk<BY, S16, BAR, XS>(a, blockIdx.x);
 }
 
 // One-pass stress update, z-marching: each thread owns 4 x-cells of one
@@ -1075,7 +1121,7 @@ __global__ void __launch_bounds__(64 * BY) k3x_vel(const K3<float> a)
 // register window (one new plane per step), so each of their planes is
 // read once per chunk instead of four times; y- and x-neighbours are read
 // as in k3x_vel.  Workgroups: 256-cell x tiles x BY rows x z-chunks.
-template <int BY, bool S16, bool RHO>
+template <int BY, bool S16, bool RHO, bool XS = false>
 __device__ __forceinline__ void velz_blk(const K3<float> &a, unsigned b)
 {
     if (a.xcd) {
@@ -1110,6 +1156,7 @@ __device__ __forceinline__ void velz_blk(const K3<float> &a, unsigned b)
 #pragma unroll
     for (int q = 0; q < 4; q++) fs[q] = a.surf[(j - 1) * a.nx + (min(max(i0 + q, 1), a.nx) - 1)];
     int c = at3(px, sxy, i0, j, kb);
+    const XNb nb = XS ? x4_nb(a, i0) : XNb{false, false};
     f4v zz_m1 = ld4(Szz + c - sxy), zz_0 = ld4(Szz + c), zz_p1 = ld4(Szz + c + sxy);
     f4v xz_m2 = ld4(Sxz + c - 2 * sxy), xz_m1 = ld4(Sxz + c - sxy), xz_0 = ld4(Sxz + c);
     f4v yz_m2 = ld4(Syz + c - 2 * sxy), yz_m1 = ld4(Syz + c - sxy), yz_0 = ld4(Syz + c);
@@ -1122,17 +1169,20 @@ __device__ __forceinline__ void velz_blk(const K3<float> &a, unsigned b)
     for (int k = kb; k <= ke; k++, c += sxy) {
         const f4v zz_p2 = ld4(Szz + c + 2 * sxy), xz_p1 = ld4(Sxz + c + sxy), yz_p1 = ld4(Syz + c + sxy);
         const f4v sxx = ld4(Sxx + c);
-        const float2 sxxl = ld2(Sxx + c - 2), sxxr = ld2(Sxx + c + 4);
+        const float2 sxxl = XS ? x4_left(nb, sxx, Sxx, c) : ld2(Sxx + c - 2);
+        const float2 sxxr = XS ? x4_right(nb, sxx, Sxx, c) : ld2(Sxx + c + 4);
         const f4v syy = ld4(Syy + c), syyp1 = ld4(Syy + c + px), syyp2 = ld4(Syy + c + 2 * px), syym1 = ld4(Syy + c - px);
         const f4v sxyv = ld4(Sxy + c), sxyym1 = ld4(Sxy + c - px), sxyym2 = ld4(Sxy + c - 2 * px), sxyyp1 = ld4(Sxy + c + px);
-        const float2 sxyl = ld2(Sxy + c - 2), sxyr = ld2(Sxy + c + 4);
-        const float2 sxzl = ld2(Sxz + c - 2), sxzr = ld2(Sxz + c + 4);
+        const float2 sxyl = XS ? x4_left(nb, sxyv, Sxy, c) : ld2(Sxy + c - 2);
+        const float2 sxyr = XS ? x4_right(nb, sxyv, Sxy, c) : ld2(Sxy + c + 4);
+        const float2 sxzl = XS ? x4_left(nb, xz_0, Sxz, c) : ld2(Sxz + c - 2);
+        const float2 sxzr = XS ? x4_right(nb, xz_0, Sxz, c) : ld2(Sxz + c + 4);
         const f4v syzym1 = ld4(Syz + c - px), syzym2 = ld4(Syz + c - 2 * px), syzyp1 = ld4(Syz + c + px);
         f4v bx, by, bz, r_p1;
         if constexpr (RHO) {
             r_p1 = ld4(Rh + c + sxy);
             const f4v rj = ld4(Rh + c + px);
-            const float ri[5] = {r_0.x, r_0.y, r_0.z, r_0.w, Rh[c + 4]};
+            const float ri[5] = {r_0.x, r_0.y, r_0.z, r_0.w, XS ? x4_right1(nb, r_0, Rh, c) : Rh[c + 4]};
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 bx[q] = 2.0f / (ri[q] + ri[q + 1]);
@@ -1180,10 +1230,10 @@ __device__ __forceinline__ void velz_blk(const K3<float> &a, unsigned b)
     }
 }
 
-template <int BY, bool S16, bool RHO>
+template <int BY, bool S16, bool RHO, bool XS>
 __global__ void __launch_bounds__(64 * BY) k3x_velz(const K3<float> a)
 {
-    velz_blk<BY, S16, RHO>(a, blockIdx.x);
+    velz_blk<BY, S16, RHO, XS>(a, blockIdx.x);
 }
 
 
@@ -2120,6 +2170,7 @@ struct swpc3d_ctx {
     int pml_rows = 1;       // whole-row absorber boxes by row segments (tuning entry 19)
     int pml_xcd = 0;        // PML workgroups XCD-contiguous (tuning entry 20)
     int x4_vlds = 0;        // z-marching velocity with LDS-staged y rows (tuning entry 21)
+    int xs = 0;             // x neighbours of the 4-wide kernels by wavefront shuffles (tuning entry 22)
     bool ade_sparse = true; // no ADE array has been set from outside since creation
     // The PML kernels run on a second stream, concurrently with the
     // interior kernels of the same half-step (they write disjoint cells and
@@ -2305,20 +2356,31 @@ static unsigned x4_tiles(const swpc3d_ctx *c, K3<float> &a, int by)
 }
 
 template <int BY, int BAR>
-static void go_x4_fused(const K3<float> &a, unsigned nb, bool s16, hipStream_t s)
+static void go_x4_fused(const K3<float> &a, unsigned nb, bool s16, bool xs, hipStream_t s)
 {
-    if (s16) hipLaunchKernelGGL((k3x_stress<BY, true, BAR>), dim3(nb), dim3(64, BY), 0, s, a);
-    else hipLaunchKernelGGL((k3x_stress<BY, false, BAR>), dim3(nb), dim3(64, BY), 0, s, a);
+    if (xs) {
+        if (s16) hipLaunchKernelGGL((k3x_stress<BY, true, BAR, true>), dim3(nb), dim3(64, BY), 0, s, a);
+        else hipLaunchKernelGGL((k3x_stress<BY, false, BAR, true>), dim3(nb), dim3(64, BY), 0, s, a);
+        return;
+    }
+    if (s16) hipLaunchKernelGGL((k3x_stress<BY, true, BAR, false>), dim3(nb), dim3(64, BY), 0, s, a);
+    else hipLaunchKernelGGL((k3x_stress<BY, false, BAR, false>), dim3(nb), dim3(64, BY), 0, s, a);
 }
 
 template <int BY>
-static void go_x4_velz(const K3<float> &a, unsigned nb, bool s16, hipStream_t s)
+static void go_x4_velz(const K3<float> &a, unsigned nb, bool s16, bool xs, hipStream_t s)
 {
     // V and S are FP32 in every 4-wide mode: with a density the kernel
     // reads no medium array at all
-    if (a.rho) hipLaunchKernelGGL((k3x_velz<BY, false, true>), dim3(nb), dim3(64, BY), 0, s, a);
-    else if (s16) hipLaunchKernelGGL((k3x_velz<BY, true, false>), dim3(nb), dim3(64, BY), 0, s, a);
-    else hipLaunchKernelGGL((k3x_velz<BY, false, false>), dim3(nb), dim3(64, BY), 0, s, a);
+    if (xs) {
+        if (a.rho) hipLaunchKernelGGL((k3x_velz<BY, false, true, true>), dim3(nb), dim3(64, BY), 0, s, a);
+        else if (s16) hipLaunchKernelGGL((k3x_velz<BY, true, false, true>), dim3(nb), dim3(64, BY), 0, s, a);
+        else hipLaunchKernelGGL((k3x_velz<BY, false, false, true>), dim3(nb), dim3(64, BY), 0, s, a);
+        return;
+    }
+    if (a.rho) hipLaunchKernelGGL((k3x_velz<BY, false, true, false>), dim3(nb), dim3(64, BY), 0, s, a);
+    else if (s16) hipLaunchKernelGGL((k3x_velz<BY, true, false, false>), dim3(nb), dim3(64, BY), 0, s, a);
+    else hipLaunchKernelGGL((k3x_velz<BY, false, false, false>), dim3(nb), dim3(64, BY), 0, s, a);
 }
 
 template <int BY, int NB>
@@ -2368,9 +2430,9 @@ static bool x4_stress(const swpc3d_ctx *c, K3<float> a, hipStream_t s)
         if (nb == 0) return true;
         const bool bar = c->x4_form == 1;
         switch (c->x4_by) {
-        case 1: bar ? go_x4_fused<1, 1>(a, nb, c->s16, s) : go_x4_fused<1, 0>(a, nb, c->s16, s); return true;
-        case 2: bar ? go_x4_fused<2, 1>(a, nb, c->s16, s) : go_x4_fused<2, 0>(a, nb, c->s16, s); return true;
-        case 4: bar ? go_x4_fused<4, 1>(a, nb, c->s16, s) : go_x4_fused<4, 0>(a, nb, c->s16, s); return true;
+        case 1: bar ? go_x4_fused<1, 1>(a, nb, c->s16, c->xs, s) : go_x4_fused<1, 0>(a, nb, c->s16, c->xs, s); return true;
+        case 2: bar ? go_x4_fused<2, 1>(a, nb, c->s16, c->xs, s) : go_x4_fused<2, 0>(a, nb, c->s16, c->xs, s); return true;
+        case 4: bar ? go_x4_fused<4, 1>(a, nb, c->s16, c->xs, s) : go_x4_fused<4, 0>(a, nb, c->s16, c->xs, s); return true;
         default: return false;
         }
     }
@@ -2407,10 +2469,10 @@ static bool x4_vel(const swpc3d_ctx *c, K3<float> a, hipStream_t s)
             }
         }
         switch (vby) {
-        case 1: go_x4_velz<1>(a, nb, c->s16, s); return true;
-        case 2: go_x4_velz<2>(a, nb, c->s16, s); return true;
-        case 4: go_x4_velz<4>(a, nb, c->s16, s); return true;
-        case 8: go_x4_velz<8>(a, nb, c->s16, s); return true;
+        case 1: go_x4_velz<1>(a, nb, c->s16, c->xs, s); return true;
+        case 2: go_x4_velz<2>(a, nb, c->s16, c->xs, s); return true;
+        case 4: go_x4_velz<4>(a, nb, c->s16, c->xs, s); return true;
+        case 8: go_x4_velz<8>(a, nb, c->s16, c->xs, s); return true;
         default: return false;
         }
     }
@@ -3977,10 +4039,10 @@ int swpc3d_set_tuning(swpc3d_ctx *c, const int *p, int n)
     clear_error();
     if (c) graph_drop(c);  // the captured steps would launch stale arguments
     if (!c || (n > 0 && !p)) return fail(SWPC_EINVAL, "null argument");
-    int v[22] = {c->s_by, c->s_wpe, c->s_split, c->v_by, c->v_wpe, c->xcd, c->s_kc, c->v_kc, c->x4, c->x4_by, c->pml4,
+    int v[23] = {c->s_by, c->s_wpe, c->s_split, c->v_by, c->v_wpe, c->xcd, c->s_kc, c->v_kc, c->x4, c->x4_by, c->pml4,
                  c->x4_form, c->x4_vby, c->conc, c->x4_vz, c->x4_sz, c->graph_k, c->sr, c->r_shift, c->pml_rows,
-                 c->pml_xcd, c->x4_vlds};
-    for (int q = 0; q < n && q < 22; q++) v[q] = p[q];
+                 c->pml_xcd, c->x4_vlds, c->xs};
+    for (int q = 0; q < n && q < 23; q++) v[q] = p[q];
     if (v[18] < -24 || v[18] > 24) return fail(SWPC_EINVAL, "memory-variable scale shift must be in -24..24");
     if (v[16] < 0 || v[16] > 1024) return fail(SWPC_EINVAL, "steps per graph must be in 0..1024");
     if (v[14] < 0 || v[15] < 0) return fail(SWPC_EINVAL, "negative z-chunk");
@@ -3997,6 +4059,7 @@ int swpc3d_set_tuning(swpc3d_ctx *c, const int *p, int n)
     c->pml_rows = v[19] != 0;
     c->pml_xcd = v[20] != 0;
     c->x4_vlds = v[21] != 0;
+    c->xs = v[22] != 0;
     if (v[18] != c->r_shift) {
         // the stored memory variables are rescaled with the new scale
         if (int r = rescale_r(c, std::ldexp(1.0f, c->r_shift - v[18]))) return r;

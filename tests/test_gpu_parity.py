@@ -290,3 +290,33 @@ def test_3d_lds_staged_velocity_is_exact():
     for o in out[1:]:
         for x, y in zip(out[0], o):
             assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp32_r16"])
+@pytest.mark.parametrize("dims", [(71, 45, 39, 6), (300, 40, 36, 8)])
+def test_3d_x_neighbours_by_shuffles_is_exact(prec, dims):
+    """The 4-wide stress and velocity kernels with the x neighbours of a
+    thread's cells taken from the adjacent lanes by wavefront shuffles
+    (tuning entry 22) give the same bits as with loads, for every field
+    over 40 steps: interior rows whose first and last threads hold cells
+    of the absorber (na not a multiple of 4) and x tiles of one and of two
+    waves."""
+    nx, ny, nz, na = dims
+    nt = 40
+    case = problem.case3d(nx, ny, nz, 0.5, 0.04, nt, na=na, src_depth=4.0, nst_side=3, zbeg=-1.5,
+                          rmed=(0.05, 2.0, 0.5), rmed_tile=(32, 32, 32), st_spacing=3.0)
+    say(f"3-D x neighbours by shuffles vs loads, {prec}: {nx}x{ny}x{nz} na={na}")
+    # swpc3d_set_tuning entries 0..22: one-pass stress (staged), velocity tile rows 4, chunk 8
+    base = [4, 3, 0, 4, 5, 0, 32, 0, 1, 2, 0, 1, 4, 0, 8, 0, 16, 0, 0, 1, 0, 0, 0]
+    out = []
+    for xs in (0, 1):
+        v = list(base)
+        v[22] = xs
+        with Swpc3D(case.grid, case.medium, case.pml, case.glut, case.stations, 1, nt, precision=prec) as sim:
+            sim.set_tuning(*v)
+            sim.run(1, nt)
+            out.append((sim.receivers()[0], {f: sim.field(f) for f in FIELDS3}))
+    assert np.abs(out[0][1]["Vx"]).max() > 0
+    assert np.array_equal(out[0][0], out[1][0])
+    for f in FIELDS3:
+        assert np.array_equal(out[0][1][f], out[1][1][f]), f
