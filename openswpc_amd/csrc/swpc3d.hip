@@ -2170,7 +2170,7 @@ struct swpc3d_ctx {
     int pml_rows = 1;       // whole-row absorber boxes by row segments (tuning entry 19)
     int pml_xcd = 0;        // PML workgroups XCD-contiguous (tuning entry 20)
     int x4_vlds = 0;        // z-marching velocity with LDS-staged y rows (tuning entry 21)
-    int xs = 0;             // x neighbours of the 4-wide kernels by wavefront shuffles (tuning entry 22)
+    int xs = 2;             // x neighbours by wavefront shuffles: bit 0 stress, bit 1 velocity (tuning entry 22)
     bool ade_sparse = true; // no ADE array has been set from outside since creation
     // The PML kernels run on a second stream, concurrently with the
     // interior kernels of the same half-step (they write disjoint cells and
@@ -2429,10 +2429,11 @@ static bool x4_stress(const swpc3d_ctx *c, K3<float> a, hipStream_t s)
         const unsigned nb = x4_tiles(c, a, c->x4_by);
         if (nb == 0) return true;
         const bool bar = c->x4_form == 1;
+        const bool xs = (c->xs & 1) != 0;
         switch (c->x4_by) {
-        case 1: bar ? go_x4_fused<1, 1>(a, nb, c->s16, c->xs, s) : go_x4_fused<1, 0>(a, nb, c->s16, c->xs, s); return true;
-        case 2: bar ? go_x4_fused<2, 1>(a, nb, c->s16, c->xs, s) : go_x4_fused<2, 0>(a, nb, c->s16, c->xs, s); return true;
-        case 4: bar ? go_x4_fused<4, 1>(a, nb, c->s16, c->xs, s) : go_x4_fused<4, 0>(a, nb, c->s16, c->xs, s); return true;
+        case 1: bar ? go_x4_fused<1, 1>(a, nb, c->s16, xs, s) : go_x4_fused<1, 0>(a, nb, c->s16, xs, s); return true;
+        case 2: bar ? go_x4_fused<2, 1>(a, nb, c->s16, xs, s) : go_x4_fused<2, 0>(a, nb, c->s16, xs, s); return true;
+        case 4: bar ? go_x4_fused<4, 1>(a, nb, c->s16, xs, s) : go_x4_fused<4, 0>(a, nb, c->s16, xs, s); return true;
         default: return false;
         }
     }
@@ -2469,10 +2470,10 @@ static bool x4_vel(const swpc3d_ctx *c, K3<float> a, hipStream_t s)
             }
         }
         switch (vby) {
-        case 1: go_x4_velz<1>(a, nb, c->s16, c->xs, s); return true;
-        case 2: go_x4_velz<2>(a, nb, c->s16, c->xs, s); return true;
-        case 4: go_x4_velz<4>(a, nb, c->s16, c->xs, s); return true;
-        case 8: go_x4_velz<8>(a, nb, c->s16, c->xs, s); return true;
+        case 1: go_x4_velz<1>(a, nb, c->s16, (c->xs & 2) != 0, s); return true;
+        case 2: go_x4_velz<2>(a, nb, c->s16, (c->xs & 2) != 0, s); return true;
+        case 4: go_x4_velz<4>(a, nb, c->s16, (c->xs & 2) != 0, s); return true;
+        case 8: go_x4_velz<8>(a, nb, c->s16, (c->xs & 2) != 0, s); return true;
         default: return false;
         }
     }
@@ -4059,7 +4060,7 @@ int swpc3d_set_tuning(swpc3d_ctx *c, const int *p, int n)
     c->pml_rows = v[19] != 0;
     c->pml_xcd = v[20] != 0;
     c->x4_vlds = v[21] != 0;
-    c->xs = v[22] != 0;
+    c->xs = v[22] & 3;
     if (v[18] != c->r_shift) {
         // the stored memory variables are rescaled with the new scale
         if (int r = rescale_r(c, std::ldexp(1.0f, c->r_shift - v[18]))) return r;
