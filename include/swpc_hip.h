@@ -300,9 +300,9 @@ int swpc3d_memory(swpc3d_ctx *ctx, long long *bytes);
              or 8 from entry [12]);
      [22]    the 4-wide one-pass stress (bit 0) and z-marching velocity
              (bit 1) kernels take the x neighbours of a thread's four
-             cells from the adjacent lanes by wavefront shuffles instead
-             of loads (default 2: the velocity update, 1.4 % faster; the
-             stress update is 0.4 % slower with them, 3 % with reduced
+             cells from the adjacent lanes by DPP wavefront shifts instead
+             of loads (default 2: the velocity update, 1.6 % faster; the
+             stress update is 0.4 % slower with them, 2 % with reduced
              storage). */
 int swpc3d_set_tuning(swpc3d_ctx *ctx, const int *p, int n);
 

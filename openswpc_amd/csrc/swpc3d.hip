@@ -563,7 +563,7 @@ __device__ __forceinline__ Ord order4(const K3<float> &a, int i, int j, int k)
     return o;
 }
 
-// x neighbours by wavefront shuffles: lane l holds cells i0 .. i0+3 and
+// x neighbours by wavefront shifts: lane l holds cells i0 .. i0+3 and
 // lane l - 1 the four before them, so the cells i0-2, i0-1 (left) and
 // i0+4, i0+5 (right) a stencil needs come from the neighbour lanes'
 // registers.  A lane whose neighbour lane holds no interior cell (it has
@@ -582,10 +582,22 @@ __device__ __forceinline__ XNb x4_nb(const K3<float> &a, int i0)
     return n;
 }
 
+// The value of lane l - 1 (from_left) or l + 1 (from_right) by a DPP
+// wavefront shift (wave_shr:1 / wave_shl:1): a VALU move, no LDS crossbar.
+__device__ __forceinline__ float from_left(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float from_right(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+
 // cells i0-2, i0-1 of array p (v holds i0 .. i0+3 of this lane)
 __device__ __forceinline__ float2 x4_left(const XNb &n, const f4v v, const float *p, int c)
 {
-    const float z = __shfl_up(v.z, 1), w = __shfl_up(v.w, 1);
+    const float z = from_left(v.z), w = from_left(v.w);
     if (n.l) return make_float2(z, w);
     return ld2(p + c - 2);
 }
@@ -593,7 +605,7 @@ __device__ __forceinline__ float2 x4_left(const XNb &n, const f4v v, const float
 // cells i0+4, i0+5
 __device__ __forceinline__ float2 x4_right(const XNb &n, const f4v v, const float *p, int c)
 {
-    const float x = __shfl_down(v.x, 1), y = __shfl_down(v.y, 1);
+    const float x = from_right(v.x), y = from_right(v.y);
     if (n.r) return make_float2(x, y);
     return ld2(p + c + 4);
 }
@@ -601,7 +613,7 @@ __device__ __forceinline__ float2 x4_right(const XNb &n, const f4v v, const floa
 // cell i0+4
 __device__ __forceinline__ float x4_right1(const XNb &n, const f4v v, const float *p, int c)
 {
-    const float x = __shfl_down(v.x, 1);
+    const float x = from_right(v.x);
     if (n.r) return x;
     return p[c + 4];
 }
@@ -2170,7 +2182,7 @@ struct swpc3d_ctx {
     int pml_rows = 1;       // whole-row absorber boxes by row segments (tuning entry 19)
     int pml_xcd = 0;        // PML workgroups XCD-contiguous (tuning entry 20)
     int x4_vlds = 0;        // z-marching velocity with LDS-staged y rows (tuning entry 21)
-    int xs = 2;             // x neighbours by wavefront shuffles: bit 0 stress, bit 1 velocity (tuning entry 22)
+    int xs = 2;             // x neighbours by wavefront shifts: bit 0 stress, bit 1 velocity (tuning entry 22)
     bool ade_sparse = true; // no ADE array has been set from outside since creation
     // The PML kernels run on a second stream, concurrently with the
     // interior kernels of the same half-step (they write disjoint cells and
