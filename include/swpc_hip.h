@@ -533,11 +533,11 @@ int swpc_batch_run(swpc_batch *b, int it0, int nsteps, int graph_k);
    every mode):
      bit 0 (default on): FP32 P-SV members with fused steps, no snapshot
            streams and their sources inside the fused kernels;
-     bit 1 (default off): FP32 3-D members with a density, the one-pass
+     bit 1 (default on): FP32 3-D members with a density, the one-pass
            stress and z-marching velocity forms, no snapshots and no body
-           force (measured 0.96x the branches on the 48-run mix, whose 3-D
-           members fill the GPU on their own, and 1.03x on 64 small 3-D
-           grids). */
+           force (measured 1.00x the autotuned branches on the 48-run mix,
+           whose 3-D members fill the GPU on their own, and 1.05x on 64
+           small 3-D grids). */
 int swpc_batch_set_grouping(swpc_batch *b, int mode);
 int swpc_batch_sync(swpc_batch *b);
 

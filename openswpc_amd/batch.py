@@ -41,10 +41,10 @@ class Batch:
     def run(self, it0, nsteps, graph_k=16):
         check(lib().swpc_batch_run(self.h, it0, nsteps, graph_k), "swpc_batch_run")
 
-    def set_grouping(self, psv=True, d3=False):
-        """Qualifying FP32 P-SV members (psv, the default) and FP32 3-D
-        members (d3) in groups, one launch per phase for all members of a
-        group, instead of one graph branch each; results are the same."""
+    def set_grouping(self, psv=True, d3=True):
+        """Qualifying FP32 P-SV members (psv) and FP32 3-D members (d3) in
+        groups, one launch per phase for all members of a group, instead of
+        one graph branch each (both by default); results are the same."""
         check(lib().swpc_batch_set_grouping(self.h, int(bool(psv)) | (2 if d3 else 0)), "swpc_batch_set_grouping")
 
     def sync(self):
@@ -173,26 +173,27 @@ def alg_bytes_per_step(cases):
 def measure(cases, steps, graph_k=16, tuning=None, log=None):
     """Time `steps` steps of every case five ways on fresh contexts:
     'batch' (one graph per graph_k steps of all members, the P-SV members
-    in one group, one launch per phase for all of them), 'batch_3d_group'
-    (the same with the 3-D members in a group too, which uses launch shapes
-    of its own), 'branches' (the
+    in one group and the 3-D members in another, one launch per phase for
+    all members of a group; the 3-D group uses launch shapes of its own),
+    'batch_psv_group' (the P-SV group only, the 3-D members autotuned
+    branches), 'branches' (the
     same graph with every member a branch of its own), 'streams' (each
     member's own run, all enqueued before one wait: the members overlap on
     their own streams) and 'sequential' (one member after the other).
     Returns {mode: seconds} plus the total cells."""
     res = {}
-    for mode in ("batch", "batch_3d_group", "branches", "streams", "sequential"):
+    for mode in ("batch", "batch_psv_group", "branches", "streams", "sequential"):
         sims = []
         for kind, c in cases:
             s = open_case(kind, c)
             # the 3-D group uses launch shapes of its own
-            if tuning and kind == "3d" and size_class(c.grid) in tuning and mode != "batch_3d_group":
+            if tuning and kind == "3d" and size_class(c.grid) in tuning and mode != "batch":
                 s.set_tuning(*tuning[size_class(c.grid)])
             sims.append(s)
         warm = min(graph_k or 16, steps)
-        if mode in ("batch", "batch_3d_group", "branches"):
+        if mode in ("batch", "batch_psv_group", "branches"):
             b = Batch()
-            b.set_grouping(psv=mode != "branches", d3=mode == "batch_3d_group")
+            b.set_grouping(psv=mode != "branches", d3=mode == "batch")
             for s in sims:
                 b.add(s)
             b.run(1, warm, graph_k)
@@ -253,10 +254,10 @@ def _main(argv=None):
            "cells": res["cells"], "timed_steps": res["timed_steps"], "graph_k": a.graph_k,
            "tuned_classes": len(tuning or {}),
            **{f"{m}_cell_updates_per_s": n / res[m]
-              for m in ("batch", "batch_3d_group", "branches", "streams", "sequential")},
+              for m in ("batch", "batch_psv_group", "branches", "streams", "sequential")},
            "speedup_vs_sequential": res["sequential"] / res["batch"],
            "speedup_vs_branches": res["branches"] / res["batch"],
-           "batch_3d_group_vs_branches": res["branches"] / res["batch_3d_group"],
+           "batch_psv_group_vs_branches": res["branches"] / res["batch_psv_group"],
            "alg_bytes_per_step": alg_bytes_per_step(cases),
            "batch_alg_GBs": ab / res["batch"] / 1e9, "batch_hbm_frac": ab / res["batch"] / 8e12}
     print(json.dumps(out))

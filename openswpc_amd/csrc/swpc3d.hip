@@ -2897,7 +2897,7 @@ void batch_done3(swpc3d_ctx *c, int it0, int it1) { note_snapshots(c, it0, it1);
 // rows by 256-cell segments and the x faces by cells); each cell sees the
 // operations of the member's own run whatever its tuning, so results equal
 // the members' own steps bit for bit.
-constexpr int G3_SBY = 4, G3_VBY = 4, G3_VKC = 8;
+constexpr int G3_SBY = 4, G3_VBY = 2, G3_VKC = 16;
 
 struct Member3 {
     K3<float> s, v;          // stress- and velocity-phase arguments (group tilings; pml = the x faces)

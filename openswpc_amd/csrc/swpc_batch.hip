@@ -5,13 +5,12 @@
 // stream, step counter and state, so results equal the members' own runs;
 // the graph removes the per-kernel launch cost of the many small grids
 // and lets the runtime overlap the branches.  With grouping on (the
-// default for P-SV) the members that qualify are not branches: one group
-// advances all FP32 P-SV members (group2_eligible) with one launch per
-// phase (three per step), and, when asked for, another all FP32 3-D
-// members (group3_eligible) with seven launches per step, the workgroups
-// of every member's kernels of a phase end to end.  The P-SV group runs on
-// the batch stream and the 3-D group on a stream of its own, beside the
-// branches.
+// default) the members that qualify are not branches: one group advances
+// all FP32 P-SV members (group2_eligible) with one launch per phase (three
+// per step), and another all FP32 3-D members (group3_eligible) with seven
+// launches per step, the workgroups of every member's kernels of a phase
+// end to end.  The P-SV group runs on the batch stream and the 3-D group
+// on a stream of its own, beside the branches.
 #include "swpc_batch.h"
 
 #include <algorithm>
@@ -43,7 +42,7 @@ struct swpc_batch {
     int graph_k = 0;             // steps in the captured graph
     swpc::DBuf dits;             // device array of the members' step counters
     long long bytes = 0;
-    int grouping = 1;            // groups: bit 0 P-SV, bit 1 3-D members (swpc_batch_set_grouping)
+    int grouping = 3;            // groups: bit 0 P-SV, bit 1 3-D members (swpc_batch_set_grouping)
     swpc::Group2 *group = nullptr;   // the P-SV group of the captured graph
     std::vector<swpc::Group3 *> groups3;   // its 3-D group (none or one)
     hipStream_t stream3 = nullptr;   // the 3-D group's stream

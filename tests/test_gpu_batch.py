@@ -102,7 +102,7 @@ def test_psv_group_equals_branches_and_own_runs():
                 s.run(1, nt)
         else:
             with batch.Batch() as b:
-                b.set_grouping(mode == "group")
+                b.set_grouping(psv=mode == "group", d3=False)
                 for s in sims:
                     b.add(s)
                 b.run(1, nt, 16)
