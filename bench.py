@@ -64,6 +64,8 @@ def main():
     ap.add_argument("--no-c1", action="store_true", help="skip the P-SV 800x400 4000-step leg")
     ap.add_argument("--no-misfit", action="store_true",
                     help="skip the misfit runs (profiling: the timed forms only, no accuracy and no choice)")
+    ap.add_argument("--lab-sponge", type=float, default=None,
+                    help="c5: strip sponge strength of the learned boundary (default: the library's 0.02)")
     ap.add_argument("--config", choices=("c2", "c3", "c5", "sh"), default="c2",
                     help="c2: the 512x512x256 bench line (default); c3: the 1024x1024x512 reduced-storage "
                          "run against an FP64 run of the same scheme (one JSON line of its own); sh: the "
@@ -487,6 +489,8 @@ def run_c5(args, nt=2000, ext=256, ext_z=200, chunk=250, early=700):
                                                               c.ntdec_w, nt, precision=prec)
         if w is not None:
             sim.set_learned_boundary(w)
+            if args.lab_sponge is not None:
+                sim.set_learned_boundary_sponge(args.lab_sponge)
         for it in range(1, nt + 1, chunk):
             sim.run(it, min(chunk, nt - it + 1))
             sim.sync()
@@ -505,7 +509,8 @@ def run_c5(args, nt=2000, ext=256, ext_z=200, chunk=250, early=700):
     out = {"config": {"workload": f"swpc_3d 512x512x256 workload's boundaries vs an echo-free {gb.nx}x{gb.ny}x{gb.nz} "
                                   f"reference, {nt} steps", "nt": nt, "early_steps": early,
                       "reference_grid": [gb.nx, gb.ny, gb.nz],
-                      "reference_own_reflection_s": path / vmax},
+                      "reference_own_reflection_s": path / vmax,
+                      "lab_sponge": 0.02 if args.lab_sponge is None else args.lab_sponge},
            "runs": {}}
     forms = [("pml20_fp32", case, "fp32", None), ("pml20_fp64", case, "fp64", None),
              ("learned_r16", icase, CHOSEN, weights), ("learned_fp32", icase, "fp32", weights),

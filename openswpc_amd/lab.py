@@ -284,13 +284,15 @@ def random_layers(rng):
     return out
 
 
-def train_boundary(nruns=200, nxs=64, nys=64, nzs=48, dx=0.25, dt=0.01, nt=500, cap=2_000_000, steps=6000,
-                   batch=8192, lr=3e-3, noise=0.1, keep=0.012, thr=1e-3, seed=1, log=print):
+def train_boundary(nruns=200, nxs=64, nys=64, nzs=48, dx=0.25, dt=0.01, nt=500, off=OFFSET, cap=2_000_000,
+                   steps=6000, batch=8192, lr=3e-3, noise=0.1, keep=0.012, thr=1e-3, seed=1, log=print):
     """Generate nruns reference runs on the GPU (even runs: the example's
     layered model at a random depth offset; odd runs: a random layered
     model under a free surface; all with a 5 % von Karman perturbation and
     a random double couple), gather their ghost targets, and fit the ten
-    nets.  Returns (params[10][NP], report)."""
+    nets.  Each run's inner nxs x nys x nzs box sits off cells inside its
+    reflecting edges, which must exceed half the distance the fastest wave
+    covers in nt steps.  Returns (params[10][NP], report)."""
     import time
     from concurrent.futures import ThreadPoolExecutor
     t0 = time.time()
@@ -307,14 +309,14 @@ def train_boundary(nruns=200, nxs=64, nys=64, nzs=48, dx=0.25, dt=0.01, nt=500, 
     # the host builds the next cases (FFT random media, layered model)
     # while the GPU runs the current one; medium__setup runs on the GPU
     with ThreadPoolExecutor(max_workers=6) as pool:
-        futs = [pool.submit(wide_case, nxs, nys, nzs, dx, dt, nt, s, zbeg, 0.05, 2.0, layers, OFFSET, REF_NA, True)
+        futs = [pool.submit(wide_case, nxs, nys, nzs, dx, dt, nt, s, zbeg, 0.05, 2.0, layers, off, REF_NA, True)
                 for s, layers, zbeg in specs[:4]]
         for r in range(nruns):
             lc = futs[r].result()
             futs[r] = None
             if r + 4 < nruns:
                 s, layers, zbeg = specs[r + 4]
-                futs.append(pool.submit(wide_case, nxs, nys, nzs, dx, dt, nt, s, zbeg, 0.05, 2.0, layers, OFFSET, REF_NA, True))
+                futs.append(pool.submit(wide_case, nxs, nys, nzs, dx, dt, nt, s, zbeg, 0.05, 2.0, layers, off, REF_NA, True))
             gather_run(tr, lc, quota, keep, thr, specs[r][0])
             del lc
             if log and (r % 20 == 19 or r == nruns - 1):
@@ -330,8 +332,8 @@ def train_boundary(nruns=200, nxs=64, nys=64, nzs=48, dx=0.25, dt=0.01, nt=500, 
             log(f"lab: net {net} ({FACES[net // 2]} phase {net % 2}): {counts[net]} samples, "
                 f"loss {loss[0]:.3e} -> {final[-1]:.3e}")
     tr.close()
-    return P, {"runs": nruns, "counts": counts.tolist(), "final_loss": final, "steps": steps, "batch": batch,
-               "noise": noise, "seconds": time.time() - t0}
+    return P, {"runs": nruns, "box": [nxs, nys, nzs], "nt": nt, "offset": off, "counts": counts.tolist(),
+               "final_loss": final, "steps": steps, "batch": batch, "noise": noise, "seconds": time.time() - t0}
 
 
 def _inner(a, nxs, nys, nzs, off=0):
@@ -424,7 +426,11 @@ def _main(argv=None):
     t = sub.add_parser("train")
     t.add_argument("--runs", type=int, default=200)
     t.add_argument("--steps", type=int, default=6000)
+    t.add_argument("--box", type=int, nargs=3, default=(64, 64, 48), help="inner box of the reference runs (cells)")
     t.add_argument("--nt", type=int, default=500)
+    t.add_argument("--offset", type=int, default=OFFSET,
+                   help="cells between the inner box and the reflecting edges (more than half the distance the "
+                        "fastest wave covers in nt steps)")
     t.add_argument("--noise", type=float, default=0.1)
     t.add_argument("--seed", type=int, default=1)
     t.add_argument("--out", default=str(WEIGHTS))
@@ -434,7 +440,9 @@ def _main(argv=None):
     e.add_argument("--nt", type=int, default=500)
     a = ap.parse_args(argv)
     if a.cmd == "train":
-        P, rep = train_boundary(nruns=a.runs, steps=a.steps, nt=a.nt, noise=a.noise, seed=a.seed)
+        nxs, nys, nzs = a.box
+        P, rep = train_boundary(nruns=a.runs, nxs=nxs, nys=nys, nzs=nzs, nt=a.nt, off=a.offset, steps=a.steps,
+                                noise=a.noise, seed=a.seed)
         save_weights(a.out, P, report=json.dumps(rep))
         print(json.dumps(rep))
         print(json.dumps(evaluate_boundary(P, seed=10_001)))
