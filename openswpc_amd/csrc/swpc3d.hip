@@ -46,11 +46,43 @@ struct Boxes3 {
     unsigned total;
 };
 
+// The 18 memory variables, and apart from them the 6 stresses, are
+// interleaved by tiles of 256 cells (tix): a wave's four cells per lane
+// then read and write one region of 18 KiB (R) or 6 KiB (S) instead of
+// arrays far apart in HBM.  The stress update takes 2247 instead of
+// 2369 us at 512 x 512 x 256 with R tiled, and 2194 with S tiled too.
+// K3::R[q] and STile::p point at component q's slot of tile 0.
+__host__ __device__ __forceinline__ long long rix(int c) { return tix<NR3>(c); }
+
+// Cell c of one stress component in the tiled layout: a cursor that moves
+// by cells (p + c - 2 is cell c - 2, wherever its tile lies) and becomes
+// an address only where it is loaded or stored (ld4, ld2, ld4nt, st4m,
+// at()).  It has no conversion to a pointer, so an address cannot be
+// offset by cells after the tile mapping by mistake.
+#define GLB __attribute__((address_space(1)))
+
+template <typename T>
+struct SCell {
+    T *p;
+    int c;
+    __device__ __forceinline__ SCell operator+(int d) const { return {p, c + d}; }
+    __device__ __forceinline__ SCell operator-(int d) const { return {p, c - d}; }
+    __device__ __forceinline__ T *at() const { return p + tix<NS3>(c); }
+};
+
+// One stress component: s[c] is its value at cell c, s + c its cursor.
+template <typename T>
+struct STile {
+    T *p;
+    __device__ __forceinline__ GLB T &operator[](int c) const { return ((GLB T *)p)[tix<NS3>(c)]; }
+    __device__ __forceinline__ SCell<T> operator+(int c) const { return {p, c}; }
+};
+
 template <typename T>
 struct K3 {
     int px, sxy, nx, ny, nz, na;
     T *V[3];    // Vx Vy Vz
-    T *S[6];    // Sxx Syy Szz Syz Sxz Sxy
+    STile<T> S[6];  // Sxx Syy Szz Syz Sxz Sxy, tiled
     T *R[18];   // Rxx(m) Ryy(m) Rzz(m) Ryz(m) Rxz(m) Rxy(m), m = 0..2
     T *A[18];   // ADE arrays (order of enum swpc3d_field)
     const float *med[10];   // bx by bz lam mu taup taus muyz muxz muxy (the reference's real(SP) arrays)
@@ -135,23 +167,6 @@ __device__ __forceinline__ unsigned short f2bf_sr(float v, unsigned c, unsigned 
     return (unsigned short)((vb + (sr_hash(c, q, vb) & 0xFFFFu)) >> 16);
 }
 
-// The 18 memory variables are interleaved by tiles of 256 cells: cell c
-// lives in tile t = (c + RB) / 256 at offset (c + RB) % 256, and the tile
-// holds its 256 values of R0, then of R1, ... R17 (K3::R[q] points at R_q's
-// slot of tile 0).  A wave's four cells per lane then read and write one
-// 18 KiB region instead of 18 arrays far apart (the stress update takes
-// 2254 instead of 2369 us at 512 x 512 x 256).  With RB = 32 the first cell
-// of every row (element 32 mod 64) starts a 64-element run of its tile, so
-// FP32 and FP16 rows start on 128-byte lines.
-constexpr int RB = 32;
-constexpr int RTILE = 256;
-constexpr int NR3 = 18;
-
-__host__ __device__ __forceinline__ long long rix(int c)
-{
-    const long long t = (long long)((unsigned)(c + RB) >> 8);
-    return t * (NR3 * RTILE) + ((c + RB) & (RTILE - 1));
-}
 
 template <bool S16, typename T>
 __device__ __forceinline__ T ldR(const K3<T> &a, int q, int c)
@@ -237,12 +252,12 @@ __device__ __forceinline__ void stress_column(const K3<T> &a, int bx, int by, in
     const T *__restrict__ Vx = a.V[0];
     const T *__restrict__ Vy = a.V[1];
     const T *__restrict__ Vz = a.V[2];
-    T *__restrict__ Sxx = a.S[0];
-    T *__restrict__ Syy = a.S[1];
-    T *__restrict__ Szz = a.S[2];
-    T *__restrict__ Syz = a.S[3];
-    T *__restrict__ Sxz = a.S[4];
-    T *__restrict__ Sxy = a.S[5];
+    const STile<T> Sxx = a.S[0];
+    const STile<T> Syy = a.S[1];
+    const STile<T> Szz = a.S[2];
+    const STile<T> Syz = a.S[3];
+    const STile<T> Sxz = a.S[4];
+    const STile<T> Sxy = a.S[5];
     constexpr bool NORMAL = PART != 2, SHEAR = PART != 1;
 
     int c = at3(px, sxy, i, j, kb);
@@ -376,12 +391,12 @@ k3_vel_int(const K3<T> a)
     T *__restrict__ Vx = a.V[0];
     T *__restrict__ Vy = a.V[1];
     T *__restrict__ Vz = a.V[2];
-    const T *__restrict__ Sxx = a.S[0];
-    const T *__restrict__ Syy = a.S[1];
-    const T *__restrict__ Szz = a.S[2];
-    const T *__restrict__ Syz = a.S[3];
-    const T *__restrict__ Sxz = a.S[4];
-    const T *__restrict__ Sxy = a.S[5];
+    const STile<T> Sxx = a.S[0];
+    const STile<T> Syy = a.S[1];
+    const STile<T> Szz = a.S[2];
+    const STile<T> Syz = a.S[3];
+    const STile<T> Sxz = a.S[4];
+    const STile<T> Sxy = a.S[5];
 
     int c = at3(px, sxy, i, j, kb);
     T xzm2 = Sxz[c - 2 * sxy], xzm1 = Sxz[c - sxy], xz0 = Sxz[c];
@@ -436,17 +451,31 @@ k3_vel_int(const K3<T> a)
 // k-2..k+2 that a band needs stay in that XCD's L2.
 typedef float f4v __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ f4v ld4(const float *p) { return *(const f4v *)p; }
-__device__ __forceinline__ f4v ld4nt(const float *p) { return __builtin_nontemporal_load((const f4v *)p); }
-__device__ __forceinline__ void st4nt(float *p, f4v v) { __builtin_nontemporal_store(v, (f4v *)p); }
-__device__ __forceinline__ float2 ld2(const float *p) { return *(const float2 *)p; }
+// Every state array lives in global memory: the accesses say so, so that
+// they compile to global_* instructions also where the compiler cannot
+// trace a pointer back to a kernel argument (flat_* would otherwise wait
+// on the LDS counter too).
+__device__ __forceinline__ f4v ld4(const float *p) { return *(const GLB f4v *)p; }
+__device__ __forceinline__ f4v ld4nt(const float *p) { return __builtin_nontemporal_load((const GLB f4v *)p); }
+__device__ __forceinline__ void st4nt(float *p, f4v v) { __builtin_nontemporal_store(v, (GLB f4v *)p); }
+__device__ __forceinline__ float2 ld2(const float *p)
+{
+    typedef float f2v __attribute__((ext_vector_type(2)));
+    const f2v v = *(const GLB f2v *)p;
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ f4v ld4(SCell<float> s) { return ld4(s.at()); }
+__device__ __forceinline__ f4v ld4nt(SCell<float> s) { return ld4nt(s.at()); }
+__device__ __forceinline__ float2 ld2(SCell<float> s) { return ld2(s.at()); }
+__device__ __forceinline__ float ld1(const float *p) { return *(const GLB float *)p; }
+__device__ __forceinline__ float ld1(SCell<float> s) { return ld1(s.at()); }
 
 typedef _Float16 h4v __attribute__((ext_vector_type(4)));
 
 // four FP16 values (8 bytes, non-temporal) widened to FP32
 __device__ __forceinline__ f4v h4(const __half *p)
 {
-    const h4v h = __builtin_nontemporal_load((const h4v *)p);
+    const h4v h = __builtin_nontemporal_load((const GLB h4v *)p);
     return __builtin_convertvector(h, f4v);
 }
 
@@ -454,7 +483,7 @@ __device__ __forceinline__ f4v h4(const __half *p)
 __device__ __forceinline__ void sth4(__half *p, f4v v)
 {
     const h4v h = __builtin_convertvector(v, h4v);
-    __builtin_nontemporal_store(h, (h4v *)p);
+    __builtin_nontemporal_store(h, (GLB h4v *)p);
 }
 
 // medium value of 4 cells of plane k
@@ -484,17 +513,20 @@ __device__ __forceinline__ f4v ldR4(const K3<float> &a, int q, int c)
 }
 
 // masked stores: all four cells, or the valid ones one by one
-__device__ __forceinline__ void st4m(float *p, f4v v, unsigned m)
+__device__ __forceinline__ void st4m(float *q, f4v v, unsigned m)
 {
     if (m == 15u) {
-        st4nt(p, v);
+        st4nt(q, v);
     } else {
+        GLB float *p = (GLB float *)q;
         if (m & 1u) p[0] = v.x;
         if (m & 2u) p[1] = v.y;
         if (m & 4u) p[2] = v.z;
         if (m & 8u) p[3] = v.w;
     }
 }
+
+__device__ __forceinline__ void st4m(SCell<float> s, f4v v, unsigned m) { st4m(s.at(), v, m); }
 
 template <bool S16>
 __device__ __forceinline__ void stR4(const K3<float> &a, int q, int c, f4v v, unsigned m)
@@ -612,8 +644,10 @@ __device__ __forceinline__ float from_right(float v)
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, false));
 }
 
-// cells i0-2, i0-1 of array p (v holds i0 .. i0+3 of this lane)
-__device__ __forceinline__ float2 x4_left(const XNb &n, const f4v v, const float *p, int c)
+// cells i0-2, i0-1 of array p (v holds i0 .. i0+3 of this lane); p is a
+// pointer or a tiled stress component
+template <typename P>
+__device__ __forceinline__ float2 x4_left(const XNb &n, const f4v v, const P &p, int c)
 {
     const float z = from_left(v.z), w = from_left(v.w);
     if (n.l) return make_float2(z, w);
@@ -621,7 +655,8 @@ __device__ __forceinline__ float2 x4_left(const XNb &n, const f4v v, const float
 }
 
 // cells i0+4, i0+5
-__device__ __forceinline__ float2 x4_right(const XNb &n, const f4v v, const float *p, int c)
+template <typename P>
+__device__ __forceinline__ float2 x4_right(const XNb &n, const f4v v, const P &p, int c)
 {
     const float x = from_right(v.x), y = from_right(v.y);
     if (n.r) return make_float2(x, y);
@@ -629,11 +664,12 @@ __device__ __forceinline__ float2 x4_right(const XNb &n, const f4v v, const floa
 }
 
 // cell i0+4
-__device__ __forceinline__ float x4_right1(const XNb &n, const f4v v, const float *p, int c)
+template <typename P>
+__device__ __forceinline__ float x4_right1(const XNb &n, const f4v v, const P &p, int c)
 {
     const float x = from_right(v.x);
     if (n.r) return x;
-    return p[c + 4];
+    return ld1(p + c + 4);
 }
 
 // normal stresses and their memory variables (m_kernel.f90:176-239)
@@ -1101,12 +1137,12 @@ __global__ void __launch_bounds__(64 * BY) k3x_vel(const K3<float> a)
     if (!x4_cell(a, BY, blockIdx.x, i0, j, k, msk)) return;
     const int px = a.px, sxy = a.sxy;
     const int c = at3(px, sxy, i0, j, k);
-    const float *__restrict__ Sxx = a.S[0];
-    const float *__restrict__ Syy = a.S[1];
-    const float *__restrict__ Szz = a.S[2];
-    const float *__restrict__ Syz = a.S[3];
-    const float *__restrict__ Sxz = a.S[4];
-    const float *__restrict__ Sxy = a.S[5];
+    const STile<float> Sxx = a.S[0];
+    const STile<float> Syy = a.S[1];
+    const STile<float> Szz = a.S[2];
+    const STile<float> Syz = a.S[3];
+    const STile<float> Sxz = a.S[4];
+    const STile<float> Sxy = a.S[5];
     const f4v sxx = ld4(Sxx + c);
     const float2 sxxl = ld2(Sxx + c - 2), sxxr = ld2(Sxx + c + 4);
     const f4v syy = ld4(Syy + c), syyp1 = ld4(Syy + c + px), syyp2 = ld4(Syy + c + 2 * px), syym1 = ld4(Syy + c - px);
@@ -1176,12 +1212,12 @@ __device__ __forceinline__ void velz_blk(const K3<float> &a, unsigned b)
     const int ke = min(kb + a.kc - 1, a.nz - a.na);
     if (kb > ke) return;
     const int px = a.px, sxy = a.sxy;
-    const float *__restrict__ Sxx = a.S[0];
-    const float *__restrict__ Syy = a.S[1];
-    const float *__restrict__ Szz = a.S[2];
-    const float *__restrict__ Syz = a.S[3];
-    const float *__restrict__ Sxz = a.S[4];
-    const float *__restrict__ Sxy = a.S[5];
+    const STile<float> Sxx = a.S[0];
+    const STile<float> Syy = a.S[1];
+    const STile<float> Szz = a.S[2];
+    const STile<float> Syz = a.S[3];
+    const STile<float> Sxz = a.S[4];
+    const STile<float> Sxy = a.S[5];
     int4 fs[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) fs[q] = a.surf[(j - 1) * a.nx + (min(max(i0 + q, 1), a.nx) - 1)];
@@ -1300,13 +1336,13 @@ __global__ void __launch_bounds__(64 * BY) k3x_velzl(const K3<float> a)
     const int ke = min(kb + a.kc - 1, a.nz - a.na);
     if (kb > ke) return;
     const int px = a.px, sxy = a.sxy;
-    const float *__restrict__ Sxx = a.S[0];
-    const float *__restrict__ Syy = a.S[1];
-    const float *__restrict__ Szz = a.S[2];
-    const float *__restrict__ Syz = a.S[3];
-    const float *__restrict__ Sxz = a.S[4];
-    const float *__restrict__ Sxy = a.S[5];
-    const float *__restrict__ Sy[3] = {Syy, Sxy, Syz};
+    const STile<float> Sxx = a.S[0];
+    const STile<float> Syy = a.S[1];
+    const STile<float> Szz = a.S[2];
+    const STile<float> Syz = a.S[3];
+    const STile<float> Sxz = a.S[4];
+    const STile<float> Sxy = a.S[5];
+    const STile<float> Sy[3] = {Syy, Sxy, Syz};
     int4 fs[4];
 #pragma unroll
     for (int q = 0; q < 4; q++)
@@ -1566,12 +1602,12 @@ __device__ __forceinline__ void vel_pml_cell(const K3<T> &a, const int i, const 
 {
     const int px = a.px, sxy = a.sxy;
     const int c = at3(px, sxy, i, j, k);
-    const T *__restrict__ Sxx = a.S[0];
-    const T *__restrict__ Syy = a.S[1];
-    const T *__restrict__ Szz = a.S[2];
-    const T *__restrict__ Syz = a.S[3];
-    const T *__restrict__ Sxz = a.S[4];
-    const T *__restrict__ Sxy = a.S[5];
+    const STile<T> Sxx = a.S[0];
+    const STile<T> Syy = a.S[1];
+    const STile<T> Szz = a.S[2];
+    const STile<T> Syz = a.S[3];
+    const STile<T> Sxz = a.S[4];
+    const STile<T> Sxy = a.S[5];
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const T gxc1 = a.gx[0 * nx + i - 1], gxc2 = a.gx[1 * nx + i - 1], gxc3 = a.gx[2 * nx + i - 1], gxc4 = a.gx[3 * nx + i - 1];
     const T gxe1 = a.gx[4 * nx + i - 1], gxe2 = a.gx[5 * nx + i - 1], gxe3 = a.gx[6 * nx + i - 1], gxe4 = a.gx[7 * nx + i - 1];
@@ -1833,12 +1869,12 @@ __global__ void __launch_bounds__(256) k3x_vel_pml(const K3<float> a, const Boxe
     if (!pml4_cell(B, blockIdx.x * 256u + threadIdx.x, i0, j, k, msk)) return;
     const int px = a.px, sxy = a.sxy, ny = a.ny, nz = a.nz;
     const int c = at3(px, sxy, i0, j, k);
-    const float *__restrict__ Sxx = a.S[0];
-    const float *__restrict__ Syy = a.S[1];
-    const float *__restrict__ Szz = a.S[2];
-    const float *__restrict__ Syz = a.S[3];
-    const float *__restrict__ Sxz = a.S[4];
-    const float *__restrict__ Sxy = a.S[5];
+    const STile<float> Sxx = a.S[0];
+    const STile<float> Syy = a.S[1];
+    const STile<float> Szz = a.S[2];
+    const STile<float> Syz = a.S[3];
+    const STile<float> Sxz = a.S[4];
+    const STile<float> Sxy = a.S[5];
     const f4v gxc1 = gx4(a, 0, i0), gxc2 = gx4(a, 1, i0), gxc3 = gx4(a, 2, i0), gxc4 = gx4(a, 3, i0);
     const f4v gxe1 = gx4(a, 4, i0), gxe2 = gx4(a, 5, i0), gxe3 = gx4(a, 6, i0), gxe4 = gx4(a, 7, i0);
     const float gyc1 = a.gy[0 * ny + j - 1], gyc2 = a.gy[1 * ny + j - 1], gyc3 = a.gy[2 * ny + j - 1], gyc4 = a.gy[3 * ny + j - 1];
@@ -2160,9 +2196,10 @@ struct swpc3d_ctx {
     size_t esz;  // bytes per state element (4 or 8)
     hipStream_t stream = nullptr;
     long long bytes = 0;
-    swpc::DBuf V[3], S[6], A[18];
+    swpc::DBuf V[3], A[18];
     swpc::DBuf Rall;        // the 18 memory variables, interleaved by 256-cell tiles (rix)
-    long long rtiles = 0;   // tiles of Rall
+    swpc::DBuf Sall;        // the 6 stresses, interleaved by 256-cell tiles (STile)
+    long long rtiles = 0;   // tiles of Rall and of Sall
     swpc::DBuf med[10];     // bx by bz lam mu taup taus muyz muxz muxy
     swpc::DBuf mbs[10];     // reduced storage: per-plane (base, scale)
     swpc::DBuf cpl[10];     // FP32 medium: per-plane constant values (K3::cpl)
@@ -2236,7 +2273,7 @@ static K3<T> make_args(swpc3d_ctx *c)
     const swpc3d_desc &d = c->d;
     a.px = c->L.px; a.sxy = c->L.sxy; a.nx = d.nx; a.ny = d.ny; a.nz = d.nz; a.na = d.na;
     for (int q = 0; q < 3; q++) a.V[q] = (T *)c->V[q].p;
-    for (int q = 0; q < 6; q++) a.S[q] = (T *)c->S[q].p;
+    for (int q = 0; q < 6; q++) a.S[q].p = (T *)c->Sall.p + (size_t)q * RTILE;
     for (int q = 0; q < 18; q++) a.R[q] = c->s16 ? nullptr : (T *)c->Rall.p + (size_t)q * RTILE;
     for (int q = 0; q < 18; q++) a.A[q] = (T *)c->A[q].p;
     for (int q = 0; q < 10; q++) a.med[q] = (const float *)c->med[q].p;
@@ -2546,7 +2583,7 @@ static LabGrid lab_grid(const swpc3d_ctx *c)
     g.px = c->L.px;
     g.sxy = c->L.sxy;
     for (int q = 0; q < 3; q++) g.F[q] = c->V[q].p;
-    for (int q = 0; q < 6; q++) g.F[3 + q] = c->S[q].p;
+    for (int q = 0; q < 6; q++) g.F[3 + q] = (char *)c->Sall.p + (size_t)q * RTILE * c->esz;
     g.f64 = c->d.precision == SWPC_FP64 ? 1 : 0;
     g.rho = (const float *)c->rho.p;
     g.lam = (const float *)c->med[MLAM].p;
@@ -3153,27 +3190,46 @@ static void dev_to_box(const Lay3 &L, const S *src, D *dst)
 static DBuf *field_buf(swpc3d_ctx *c, int f)
 {
     if (f >= SWPC3D_VX && f <= SWPC3D_VZ) return &c->V[f];
-    if (f >= SWPC3D_SXX && f <= SWPC3D_SXY) return &c->S[f - SWPC3D_SXX];
+    if (f >= SWPC3D_SXX && f <= SWPC3D_SXY) return &c->Sall;
     if (f >= SWPC3D_R0 && f < SWPC3D_ADE0) return &c->Rall;
     if (f >= SWPC3D_ADE0 && f < SWPC3D_NFIELD) return &c->A[f - SWPC3D_ADE0];
     return nullptr;
 }
 
+// A tiled field's region, its component slot and the components per tile;
+// null for a field stored as a plain array.
+static DBuf *field_tiles(swpc3d_ctx *c, int f, size_t &q, size_t &ns)
+{
+    if (f >= SWPC3D_SXX && f <= SWPC3D_SXY) {
+        q = (size_t)(f - SWPC3D_SXX);
+        ns = NS3;
+        return &c->Sall;
+    }
+    if (f >= SWPC3D_R0 && f < SWPC3D_ADE0) {
+        q = (size_t)(f - SWPC3D_R0);
+        ns = NR3;
+        return &c->Rall;
+    }
+    return nullptr;
+}
+
 // Field f's vol elements of el bytes between the device and a contiguous
-// host array: a memory variable is gathered from (scattered to) its slots
-// of the interleaved tiles, the cells before the first and after the last
-// of the box left out (zero on the device).
+// host array: a tiled field is gathered from (scattered to) its slots of
+// the tiles with one strided copy, the cells before the first and after
+// the last of the box left out (zero on the device).
 static int field_read(swpc3d_ctx *c, int f, void *dst, size_t el)
 {
     const size_t n = (size_t)c->L.vol;
-    if (f < SWPC3D_R0 || f >= SWPC3D_ADE0) {
+    size_t q = 0, ns = 0;
+    const DBuf *t = field_tiles(c, f, q, ns);
+    if (!t) {
         SWPC_HIP(hipMemcpy(dst, field_buf(c, f)->p, n * el, hipMemcpyDeviceToHost));
         return SWPC_OK;
     }
-    const size_t q = (size_t)(f - SWPC3D_R0), nt = (size_t)c->rtiles;
+    const size_t nt = (size_t)c->rtiles;
     std::vector<char> tl(nt * RTILE * el);
-    SWPC_HIP(hipMemcpy2D(tl.data(), RTILE * el, (const char *)c->Rall.p + q * RTILE * el, (size_t)NR3 * RTILE * el,
-                         RTILE * el, nt, hipMemcpyDeviceToHost));
+    SWPC_HIP(hipMemcpy2D(tl.data(), RTILE * el, (const char *)t->p + q * RTILE * el, ns * RTILE * el, RTILE * el, nt,
+                         hipMemcpyDeviceToHost));
     memcpy(dst, tl.data() + (size_t)RB * el, n * el);
     return SWPC_OK;
 }
@@ -3181,15 +3237,17 @@ static int field_read(swpc3d_ctx *c, int f, void *dst, size_t el)
 static int field_write(swpc3d_ctx *c, int f, const void *src, size_t el)
 {
     const size_t n = (size_t)c->L.vol;
-    if (f < SWPC3D_R0 || f >= SWPC3D_ADE0) {
+    size_t q = 0, ns = 0;
+    DBuf *t = field_tiles(c, f, q, ns);
+    if (!t) {
         SWPC_HIP(hipMemcpy(field_buf(c, f)->p, src, n * el, hipMemcpyHostToDevice));
         return SWPC_OK;
     }
-    const size_t q = (size_t)(f - SWPC3D_R0), nt = (size_t)c->rtiles;
+    const size_t nt = (size_t)c->rtiles;
     std::vector<char> tl(nt * RTILE * el, 0);
     memcpy(tl.data() + (size_t)RB * el, src, n * el);
-    SWPC_HIP(hipMemcpy2D((char *)c->Rall.p + q * RTILE * el, (size_t)NR3 * RTILE * el, tl.data(), RTILE * el,
-                         RTILE * el, nt, hipMemcpyHostToDevice));
+    SWPC_HIP(hipMemcpy2D((char *)t->p + q * RTILE * el, ns * RTILE * el, tl.data(), RTILE * el, RTILE * el, nt,
+                         hipMemcpyHostToDevice));
     return SWPC_OK;
 }
 
@@ -3263,12 +3321,12 @@ int swpc3d_create(const swpc3d_desc *desc, swpc3d_ctx **out)
     const size_t vb = (size_t)L.vol * c->esz;
     int r;
     for (auto &b : c->V) if ((r = dalloc(b, vb, c->bytes))) return r;
-    for (auto &b : c->S) if ((r = dalloc(b, vb, c->bytes))) return r;
     // 16-bit arrays start 64 bytes into a line: cell i = 1 sits at element
     // IO + 1 = 32, so its rows then begin on 128-byte lines as the FP32
     // arrays' do, and a wave's 4-cell accesses cover whole lines
     const size_t vb16 = (size_t)L.vol * 2, lead16 = 128 - (size_t)(IO + 1) * 2;
     c->rtiles = ((long long)L.vol + RB + RTILE - 1) / RTILE;
+    if ((r = dalloc(c->Sall, (size_t)c->rtiles * NS3 * RTILE * c->esz, c->bytes))) return r;
     if ((r = dalloc(c->Rall, (size_t)c->rtiles * NR3 * RTILE * (c->s16 ? 2 : c->esz), c->bytes))) return r;
     for (auto &b : c->A) if ((r = dalloc(b, c->a16 ? vb16 : vb, c->bytes, c->a16 ? lead16 : 0))) return r;
     for (auto &b : c->med)
@@ -3349,7 +3407,7 @@ int swpc3d_destroy(swpc3d_ctx *c)
     if (!c) return SWPC_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto &b : c->V) dfree(b);
-    for (auto &b : c->S) dfree(b);
+    dfree(c->Sall);
     dfree(c->Rall);
     for (auto &b : c->A) dfree(b);
     for (auto &b : c->med) dfree(b);

@@ -70,6 +70,26 @@ __host__ __device__ inline int at3(int px, int sxy, int i, int j, int k)
     return (k + KO) * sxy + (j + JO) * px + (i + IO);
 }
 
+// Cell tiles of the 3-D state: arrays of per-cell values interleaved by
+// tiles of RTILE cells, N components per tile.  Cell c (the at3 offset)
+// lives in tile (c + RB) / RTILE at slot (c + RB) % RTILE; component q of
+// tile t starts at element (t * N + q) * RTILE.  RB = 32 puts the first
+// cell of every row (element IO + 1 = 32 mod 64) at a slot that is a
+// multiple of 64, so rows start on 128-byte lines, and aligned groups of
+// 2 or 4 cells never straddle two tiles.  64-bit: a tiled region passes
+// 2^32 elements at 1024 x 1024 x 512.
+constexpr int RB = 32;
+constexpr int RTILE = 256;
+constexpr int NS3 = 6;    // the 3-D stresses Sxx Syy Szz Syz Sxz Sxy, tiled together
+constexpr int NR3 = 18;   // the 3-D memory variables, tiled together
+
+template <int N>
+__host__ __device__ __forceinline__ long long tix(int c)
+{
+    const long long t = (long long)((unsigned)(c + RB) >> 8);
+    return t * (N * RTILE) + ((c + RB) & (RTILE - 1));
+}
+
 struct Lay2 {
     int nx, nz;
     int px;       // elements per row (x)

@@ -64,7 +64,8 @@ LabDomain lab_domain(int oi, int oj, int nx, int ny, int nz);
 // What the lab kernels need of a 3-D context.
 struct LabGrid {
     int px, sxy;
-    void *F[9];             // Vx Vy Vz Sxx Syy Szz Syz Sxz Sxy (FP32, or FP64 with f64)
+    void *F[9];             // Vx Vy Vz Sxx Syy Szz Syz Sxz Sxy (FP32, or FP64 with f64); the
+                            // stresses in cell tiles of NS3 components (tix), F[3 + q] at slot q of tile 0
     int f64 = 0;
     const float *rho, *lam, *mu;
     float dt, h[3];

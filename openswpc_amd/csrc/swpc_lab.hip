@@ -90,17 +90,21 @@ static LabDev make_dev(const LabGrid &g, const LabDomain &d)
     return a;
 }
 
+// Element e of field fi in its array: the stresses (fi >= 3) are stored
+// in cell tiles of NS3 components (tix), the velocities as plain arrays.
+__device__ __forceinline__ long long lab_ix(int fi, int e) { return fi >= 3 ? tix<NS3>(e) : (long long)e; }
+
 // Field value at an element, FP32 or FP64 state (T).
 template <typename T>
 __device__ __forceinline__ float lab_ld(const LabDev &a, int fi, int e)
 {
-    return (float)((const T *)a.F[fi])[e];
+    return (float)((const T *)a.F[fi])[lab_ix(fi, e)];
 }
 
 template <typename T>
 __device__ __forceinline__ void lab_st(const LabDev &a, int fi, int e, float v)
 {
-    ((T *)a.F[fi])[e] = (T)v;
+    ((T *)a.F[fi])[lab_ix(fi, e)] = (T)v;
 }
 
 // Address of strip cell d = 1 of column col of face f, and the address
@@ -160,21 +164,22 @@ __global__ void __launch_bounds__(256) k_lab_colmed(const LabDev a, float4 *__re
 // a 16-byte or 8-byte aligned first cell, one 16-byte and one 8-byte
 // access cover the six cells (two cache-line requests per lane instead of
 // six).  The alignment is the same for every column of a face (rows start
-// at multiples of 32 elements).
+// at multiples of 32 elements).  fi is the field (lab_ix): an aligned
+// group of 2 or 4 cells lies in one tile of a tiled field.
 template <typename T>
-__device__ __forceinline__ void lab_strip_load(const T *__restrict__ F, int base, int step, bool xrow, T *v)
+__device__ __forceinline__ void lab_strip_load(const T *__restrict__ F, int fi, int base, int step, bool xrow, T *v)
 {
     if constexpr (sizeof(T) == 4) {
         const int lo = step > 0 ? base : base - (LAB_NS - 1);
         if (xrow && (lo & 1) == 0) {
             float w[LAB_NS];
             if ((lo & 3) == 0) {
-                const float4 A = *(const float4 *)(F + lo);
-                const float2 B = *(const float2 *)(F + lo + 4);
+                const float4 A = *(const float4 *)(F + lab_ix(fi, lo));
+                const float2 B = *(const float2 *)(F + lab_ix(fi, lo + 4));
                 w[0] = A.x; w[1] = A.y; w[2] = A.z; w[3] = A.w; w[4] = B.x; w[5] = B.y;
             } else {
-                const float2 A = *(const float2 *)(F + lo);
-                const float4 B = *(const float4 *)(F + lo + 2);
+                const float2 A = *(const float2 *)(F + lab_ix(fi, lo));
+                const float4 B = *(const float4 *)(F + lab_ix(fi, lo + 2));
                 w[0] = A.x; w[1] = A.y; w[2] = B.x; w[3] = B.y; w[4] = B.z; w[5] = B.w;
             }
 #pragma unroll
@@ -183,13 +188,13 @@ __device__ __forceinline__ void lab_strip_load(const T *__restrict__ F, int base
         }
     }
 #pragma unroll
-    for (int d = 0; d < LAB_NS; d++) v[d] = F[base + d * step];
+    for (int d = 0; d < LAB_NS; d++) v[d] = F[lab_ix(fi, base + d * step)];
 }
 
 // Store v[d] * sp[d] back to the six strip cells (the sponge).
 template <typename T>
-__device__ __forceinline__ void lab_strip_store(T *__restrict__ F, int base, int step, bool xrow, const float *sp,
-                                                const T *v)
+__device__ __forceinline__ void lab_strip_store(T *__restrict__ F, int fi, int base, int step, bool xrow,
+                                                const float *sp, const T *v)
 {
     T u[LAB_NS];
 #pragma unroll
@@ -201,17 +206,17 @@ __device__ __forceinline__ void lab_strip_store(T *__restrict__ F, int base, int
 #pragma unroll
             for (int d = 0; d < LAB_NS; d++) w[d] = step > 0 ? u[d] : u[LAB_NS - 1 - d];
             if ((lo & 3) == 0) {
-                *(float4 *)(F + lo) = make_float4(w[0], w[1], w[2], w[3]);
-                *(float2 *)(F + lo + 4) = make_float2(w[4], w[5]);
+                *(float4 *)(F + lab_ix(fi, lo)) = make_float4(w[0], w[1], w[2], w[3]);
+                *(float2 *)(F + lab_ix(fi, lo + 4)) = make_float2(w[4], w[5]);
             } else {
-                *(float2 *)(F + lo) = make_float2(w[0], w[1]);
-                *(float4 *)(F + lo + 2) = make_float4(w[2], w[3], w[4], w[5]);
+                *(float2 *)(F + lab_ix(fi, lo)) = make_float2(w[0], w[1]);
+                *(float4 *)(F + lab_ix(fi, lo + 2)) = make_float4(w[2], w[3], w[4], w[5]);
             }
             return;
         }
     }
 #pragma unroll
-    for (int d = 0; d < LAB_NS; d++) F[base + d * step] = u[d];
+    for (int d = 0; d < LAB_NS; d++) F[lab_ix(fi, base + d * step)] = u[d];
 }
 
 // Features of one column into x[LAB_NIN]; prev holds the previous ghost
@@ -224,7 +229,7 @@ __device__ __forceinline__ float lab_features(const LabDev &a, int base, int ste
     const float inv_imp = 1.0f / m.z;
     float amax = 0.0f;
 #pragma unroll
-    for (int fi = 0; fi < 9; fi++) lab_strip_load<T>((const T *)a.F[fi], base, step, xrow, raw + fi * LAB_NS);
+    for (int fi = 0; fi < 9; fi++) lab_strip_load<T>((const T *)a.F[fi], fi, base, step, xrow, raw + fi * LAB_NS);
 #pragma unroll
     for (int fi = 0; fi < 9; fi++) {
 #pragma unroll
@@ -423,7 +428,7 @@ __device__ __forceinline__ void lab_fill_body(const LabFillArgs &p)
             const int f0 = p.phase == 0 ? 0 : 3, f1 = p.phase == 0 ? 3 : 9;
 #pragma unroll
             for (int fi = 0; fi < 9; fi++)
-                if (fi >= f0 && fi < f1) lab_strip_store<T>((T *)a.F[fi], base, step, xrow, p.sp, raw + fi * LAB_NS);
+                if (fi >= f0 && fi < f1) lab_strip_store<T>((T *)a.F[fi], fi, base, step, xrow, p.sp, raw + fi * LAB_NS);
         }
     } else {
 #pragma unroll
