@@ -164,22 +164,21 @@ __global__ void __launch_bounds__(256) k_lab_colmed(const LabDev a, float4 *__re
 // a 16-byte or 8-byte aligned first cell, one 16-byte and one 8-byte
 // access cover the six cells (two cache-line requests per lane instead of
 // six).  The alignment is the same for every column of a face (rows start
-// at multiples of 32 elements).  fi is the field (lab_ix): an aligned
-// group of 2 or 4 cells lies in one tile of a tiled field.
+// at multiples of 32 elements).  A velocity component (a plain array).
 template <typename T>
-__device__ __forceinline__ void lab_strip_load(const T *__restrict__ F, int fi, int base, int step, bool xrow, T *v)
+__device__ __forceinline__ void lab_strip_load(const T *__restrict__ F, int base, int step, bool xrow, T *v)
 {
     if constexpr (sizeof(T) == 4) {
         const int lo = step > 0 ? base : base - (LAB_NS - 1);
         if (xrow && (lo & 1) == 0) {
             float w[LAB_NS];
             if ((lo & 3) == 0) {
-                const float4 A = *(const float4 *)(F + lab_ix(fi, lo));
-                const float2 B = *(const float2 *)(F + lab_ix(fi, lo + 4));
+                const float4 A = *(const float4 *)(F + lo);
+                const float2 B = *(const float2 *)(F + lo + 4);
                 w[0] = A.x; w[1] = A.y; w[2] = A.z; w[3] = A.w; w[4] = B.x; w[5] = B.y;
             } else {
-                const float2 A = *(const float2 *)(F + lab_ix(fi, lo));
-                const float4 B = *(const float4 *)(F + lab_ix(fi, lo + 2));
+                const float2 A = *(const float2 *)(F + lo);
+                const float4 B = *(const float4 *)(F + lo + 2);
                 w[0] = A.x; w[1] = A.y; w[2] = B.x; w[3] = B.y; w[4] = B.z; w[5] = B.w;
             }
 #pragma unroll
@@ -188,13 +187,52 @@ __device__ __forceinline__ void lab_strip_load(const T *__restrict__ F, int fi, 
         }
     }
 #pragma unroll
-    for (int d = 0; d < LAB_NS; d++) v[d] = F[lab_ix(fi, base + d * step)];
+    for (int d = 0; d < LAB_NS; d++) v[d] = F[base + d * step];
+}
+
+// The six strip values of each stress component, into v[q * LAB_NS + d]
+// (q = 0..5).  The components are tiled (tix): component q lies q * RTILE
+// elements after component 0 (S0) in every tile, so one set of cell
+// offsets serves all six; an aligned group of 2 or 4 cells lies in one
+// tile.
+template <typename T>
+__device__ __forceinline__ void lab_strip_load_s(const T *__restrict__ S0, int base, int step, bool xrow, T *v)
+{
+    if constexpr (sizeof(T) == 4) {
+        const int lo = step > 0 ? base : base - (LAB_NS - 1);
+        if (xrow && (lo & 1) == 0) {
+            const bool a4 = (lo & 3) == 0;
+            const float *pa = S0 + tix<NS3>(lo), *pb = S0 + tix<NS3>(lo + (a4 ? 4 : 2));
+#pragma unroll
+            for (int q = 0; q < NS3; q++) {
+                float w[LAB_NS];
+                if (a4) {
+                    const float4 A = *(const float4 *)(pa + q * RTILE);
+                    const float2 B = *(const float2 *)(pb + q * RTILE);
+                    w[0] = A.x; w[1] = A.y; w[2] = A.z; w[3] = A.w; w[4] = B.x; w[5] = B.y;
+                } else {
+                    const float2 A = *(const float2 *)(pa + q * RTILE);
+                    const float4 B = *(const float4 *)(pb + q * RTILE);
+                    w[0] = A.x; w[1] = A.y; w[2] = B.x; w[3] = B.y; w[4] = B.z; w[5] = B.w;
+                }
+#pragma unroll
+                for (int d = 0; d < LAB_NS; d++) v[q * LAB_NS + d] = step > 0 ? w[d] : w[LAB_NS - 1 - d];
+            }
+            return;
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < LAB_NS; d++) {
+        const T *p = S0 + tix<NS3>(base + d * step);
+#pragma unroll
+        for (int q = 0; q < NS3; q++) v[q * LAB_NS + d] = p[q * RTILE];
+    }
 }
 
 // Store v[d] * sp[d] back to the six strip cells (the sponge).
 template <typename T>
-__device__ __forceinline__ void lab_strip_store(T *__restrict__ F, int fi, int base, int step, bool xrow,
-                                                const float *sp, const T *v)
+__device__ __forceinline__ void lab_strip_store(T *__restrict__ F, int base, int step, bool xrow, const float *sp,
+                                                const T *v)
 {
     T u[LAB_NS];
 #pragma unroll
@@ -206,17 +244,54 @@ __device__ __forceinline__ void lab_strip_store(T *__restrict__ F, int fi, int b
 #pragma unroll
             for (int d = 0; d < LAB_NS; d++) w[d] = step > 0 ? u[d] : u[LAB_NS - 1 - d];
             if ((lo & 3) == 0) {
-                *(float4 *)(F + lab_ix(fi, lo)) = make_float4(w[0], w[1], w[2], w[3]);
-                *(float2 *)(F + lab_ix(fi, lo + 4)) = make_float2(w[4], w[5]);
+                *(float4 *)(F + lo) = make_float4(w[0], w[1], w[2], w[3]);
+                *(float2 *)(F + lo + 4) = make_float2(w[4], w[5]);
             } else {
-                *(float2 *)(F + lab_ix(fi, lo)) = make_float2(w[0], w[1]);
-                *(float4 *)(F + lab_ix(fi, lo + 2)) = make_float4(w[2], w[3], w[4], w[5]);
+                *(float2 *)(F + lo) = make_float2(w[0], w[1]);
+                *(float4 *)(F + lo + 2) = make_float4(w[2], w[3], w[4], w[5]);
             }
             return;
         }
     }
 #pragma unroll
-    for (int d = 0; d < LAB_NS; d++) F[lab_ix(fi, base + d * step)] = u[d];
+    for (int d = 0; d < LAB_NS; d++) F[base + d * step] = u[d];
+}
+
+// v[q * LAB_NS + d] * sp[d] back to the stress components' strip cells
+// (tiled, as lab_strip_load_s).
+template <typename T>
+__device__ __forceinline__ void lab_strip_store_s(T *__restrict__ S0, int base, int step, bool xrow, const float *sp,
+                                                  const T *v)
+{
+    if constexpr (sizeof(T) == 4) {
+        const int lo = step > 0 ? base : base - (LAB_NS - 1);
+        if (xrow && (lo & 1) == 0) {
+            const bool a4 = (lo & 3) == 0;
+            float *pa = S0 + tix<NS3>(lo), *pb = S0 + tix<NS3>(lo + (a4 ? 4 : 2));
+#pragma unroll
+            for (int q = 0; q < NS3; q++) {
+                float u[LAB_NS], w[LAB_NS];
+#pragma unroll
+                for (int d = 0; d < LAB_NS; d++) u[d] = v[q * LAB_NS + d] * sp[d];
+#pragma unroll
+                for (int d = 0; d < LAB_NS; d++) w[d] = step > 0 ? u[d] : u[LAB_NS - 1 - d];
+                if (a4) {
+                    *(float4 *)(pa + q * RTILE) = make_float4(w[0], w[1], w[2], w[3]);
+                    *(float2 *)(pb + q * RTILE) = make_float2(w[4], w[5]);
+                } else {
+                    *(float2 *)(pa + q * RTILE) = make_float2(w[0], w[1]);
+                    *(float4 *)(pb + q * RTILE) = make_float4(w[2], w[3], w[4], w[5]);
+                }
+            }
+            return;
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < LAB_NS; d++) {
+        T *p = S0 + tix<NS3>(base + d * step);
+#pragma unroll
+        for (int q = 0; q < NS3; q++) p[q * RTILE] = v[q * LAB_NS + d] * (T)sp[d];
+    }
 }
 
 // Features of one column into x[LAB_NIN]; prev holds the previous ghost
@@ -229,7 +304,8 @@ __device__ __forceinline__ float lab_features(const LabDev &a, int base, int ste
     const float inv_imp = 1.0f / m.z;
     float amax = 0.0f;
 #pragma unroll
-    for (int fi = 0; fi < 9; fi++) lab_strip_load<T>((const T *)a.F[fi], fi, base, step, xrow, raw + fi * LAB_NS);
+    for (int fi = 0; fi < 3; fi++) lab_strip_load<T>((const T *)a.F[fi], base, step, xrow, raw + fi * LAB_NS);
+    lab_strip_load_s<T>((const T *)a.F[3], base, step, xrow, raw + 3 * LAB_NS);
 #pragma unroll
     for (int fi = 0; fi < 9; fi++) {
 #pragma unroll
@@ -425,10 +501,12 @@ __device__ __forceinline__ void lab_fill_body(const LabFillArgs &p)
         // strip sponge on this phase's own fields, from the values read
         // above (the column owns its strip cells within the launch)
         if (p.sponge) {
-            const int f0 = p.phase == 0 ? 0 : 3, f1 = p.phase == 0 ? 3 : 9;
+            if (p.phase == 0) {
 #pragma unroll
-            for (int fi = 0; fi < 9; fi++)
-                if (fi >= f0 && fi < f1) lab_strip_store<T>((T *)a.F[fi], fi, base, step, xrow, p.sp, raw + fi * LAB_NS);
+                for (int fi = 0; fi < 3; fi++) lab_strip_store<T>((T *)a.F[fi], base, step, xrow, p.sp, raw + fi * LAB_NS);
+            } else {
+                lab_strip_store_s<T>((T *)a.F[3], base, step, xrow, p.sp, raw + 3 * LAB_NS);
+            }
         }
     } else {
 #pragma unroll
@@ -487,19 +565,29 @@ __device__ __forceinline__ void lab_fill_body(const LabFillArgs &p)
     float h2[LAB_NH];
 #pragma unroll
     for (int h = 0; h < LAB_NH; h++) h2[h] = sH2t[l >> 5][(l & 31) * LH2 + h];
-#pragma unroll 1
-    for (int o = 0; o < nout; o++) {
-        const float *wo = sO + o * LO;
-        float z = wo[LAB_NH + LAB_NIN];
+    // the ghosts are stored once all are formed: a stress ghost's 64-bit
+    // offset in its tile (lab_ix) then needs no registers beside the
+    // features and hidden values
+    float out[LAB_NOUT];
 #pragma unroll
-        for (int h = 0; h < LAB_NH; h++) z = fmaf(wo[h], h2[h], z);
+    for (int o = 0; o < LAB_NOUT; o++) {
+        out[o] = 0.0f;
+        if (o < nout) {
+            const float *wo = sO + o * LO;
+            float z = wo[LAB_NH + LAB_NIN];
 #pragma unroll
-        for (int q = 0; q < LAB_NIN; q++) z = fmaf(wo[LAB_NH + q], x[q], z);
-        const int fi = c_tgt[net][o][0];
-        float v = live ? z * N : 0.0f;
-        if (fi >= 3) v = v * m.z;
-        lab_st<T>(a, fi, gaddr[o], v);
+            for (int h = 0; h < LAB_NH; h++) z = fmaf(wo[h], h2[h], z);
+#pragma unroll
+            for (int q = 0; q < LAB_NIN; q++) z = fmaf(wo[LAB_NH + q], x[q], z);
+            const int fi = c_tgt[net][o][0];
+            float v = live ? z * N : 0.0f;
+            if (fi >= 3) v = v * m.z;
+            out[o] = v;
+        }
     }
+#pragma unroll
+    for (int o = 0; o < LAB_NOUT; o++)
+        if (o < nout) lab_st<T>(a, c_tgt[net][o][0], gaddr[o], out[o]);
 }
 
 // FP32 state: 168 VGPRs and three 4-wave blocks per CU (the LDS of
