@@ -61,6 +61,10 @@ __host__ __device__ __forceinline__ long long rix(int c) { return tix<NR3>(c); }
 // offset by cells after the tile mapping by mistake.
 #define GLB __attribute__((address_space(1)))
 
+// p as a global-memory pointer (see ld4)
+template <typename T>
+__device__ __forceinline__ GLB T *gp(T *p) { return (GLB T *)p; }
+
 template <typename T>
 struct SCell {
     T *p;
@@ -125,13 +129,13 @@ template <bool S16, typename T>
 __device__ __forceinline__ T medv(const K3<T> &a, int q, int c, int k)
 {
     if constexpr (S16) {
-        if (!a.m16) return (T)a.med[q][c];
+        if (!a.m16) return (T)gp(a.med[q])[c];
         const float2 bs = a.mbs[q][k + KO];
         float v = bs.x;
         if (bs.y != 0.0f) v += bs.y * __half2float(a.hmed[q][c]);
         return (T)v;
     } else {
-        return (T)a.med[q][c];
+        return (T)gp(a.med[q])[c];
     }
 }
 
@@ -172,7 +176,7 @@ template <bool S16, typename T>
 __device__ __forceinline__ T ldR(const K3<T> &a, int q, int c)
 {
     if constexpr (S16) return (T)(__half2float(a.hR[q][rix(c)]) * a.r_scale);
-    else return a.R[q][rix(c)];
+    else return gp(a.R[q])[rix(c)];
 }
 
 template <bool S16, typename T>
@@ -182,7 +186,7 @@ __device__ __forceinline__ void stR(const K3<T> &a, int q, int c, T v)
         const float w = fminf(fmaxf((float)v * a.r_inv, -65504.0f), 65504.0f);
         a.hR[q][rix(c)] = a.sr ? f2h_sr(w, (unsigned)c, (unsigned)q) : __float2half_rn(w);
     } else {
-        a.R[q][rix(c)] = v;
+        gp(a.R[q])[rix(c)] = v;
     }
 }
 
@@ -190,7 +194,7 @@ template <bool S16, typename T>
 __device__ __forceinline__ T ldA(const K3<T> &a, int q, int c)
 {
     if constexpr (S16) return (T)__bfloat162float(a.hA[q][c]);
-    else return a.A[q][c];
+    else return gp(a.A[q])[c];
 }
 
 template <bool S16, typename T>
@@ -204,7 +208,7 @@ __device__ __forceinline__ void stA(const K3<T> &a, int q, int c, T v)
             a.hA[q][c] = __float2bfloat16((float)v);
         }
     } else {
-        a.A[q][c] = v;
+        gp(a.A[q])[c] = v;
     }
 }
 
@@ -1506,16 +1510,16 @@ __device__ __forceinline__ void stress_pml_cell(const K3<T> &a, const int i, con
 {
     const int px = a.px, sxy = a.sxy;
     const int c = at3(px, sxy, i, j, k);
-    const T *__restrict__ Vx = a.V[0];
-    const T *__restrict__ Vy = a.V[1];
-    const T *__restrict__ Vz = a.V[2];
+    const GLB T *__restrict__ Vx = gp(a.V[0]);
+    const GLB T *__restrict__ Vy = gp(a.V[1]);
+    const GLB T *__restrict__ Vz = gp(a.V[2]);
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const T gxc1 = a.gx[0 * nx + i - 1], gxc2 = a.gx[1 * nx + i - 1], gxc3 = a.gx[2 * nx + i - 1], gxc4 = a.gx[3 * nx + i - 1];
-    const T gxe1 = a.gx[4 * nx + i - 1], gxe2 = a.gx[5 * nx + i - 1], gxe3 = a.gx[6 * nx + i - 1], gxe4 = a.gx[7 * nx + i - 1];
-    const T gyc1 = a.gy[0 * ny + j - 1], gyc2 = a.gy[1 * ny + j - 1], gyc3 = a.gy[2 * ny + j - 1], gyc4 = a.gy[3 * ny + j - 1];
-    const T gye1 = a.gy[4 * ny + j - 1], gye2 = a.gy[5 * ny + j - 1], gye3 = a.gy[6 * ny + j - 1], gye4 = a.gy[7 * ny + j - 1];
-    const T gzc1 = a.gz[0 * nz + k - 1], gzc2 = a.gz[1 * nz + k - 1], gzc3 = a.gz[2 * nz + k - 1], gzc4 = a.gz[3 * nz + k - 1];
-    const T gze1 = a.gz[4 * nz + k - 1], gze2 = a.gz[5 * nz + k - 1], gze3 = a.gz[6 * nz + k - 1], gze4 = a.gz[7 * nz + k - 1];
+    const T gxc1 = gp(a.gx)[0 * nx + i - 1], gxc2 = gp(a.gx)[1 * nx + i - 1], gxc3 = gp(a.gx)[2 * nx + i - 1], gxc4 = gp(a.gx)[3 * nx + i - 1];
+    const T gxe1 = gp(a.gx)[4 * nx + i - 1], gxe2 = gp(a.gx)[5 * nx + i - 1], gxe3 = gp(a.gx)[6 * nx + i - 1], gxe4 = gp(a.gx)[7 * nx + i - 1];
+    const T gyc1 = gp(a.gy)[0 * ny + j - 1], gyc2 = gp(a.gy)[1 * ny + j - 1], gyc3 = gp(a.gy)[2 * ny + j - 1], gyc4 = gp(a.gy)[3 * ny + j - 1];
+    const T gye1 = gp(a.gy)[4 * ny + j - 1], gye2 = gp(a.gy)[5 * ny + j - 1], gye3 = gp(a.gy)[6 * ny + j - 1], gye4 = gp(a.gy)[7 * ny + j - 1];
+    const T gzc1 = gp(a.gz)[0 * nz + k - 1], gzc2 = gp(a.gz)[1 * nz + k - 1], gzc3 = gp(a.gz)[2 * nz + k - 1], gzc4 = gp(a.gz)[3 * nz + k - 1];
+    const T gze1 = gp(a.gz)[4 * nz + k - 1], gze2 = gp(a.gz)[5 * nz + k - 1], gze3 = gp(a.gz)[6 * nz + k - 1], gze4 = gp(a.gz)[7 * nz + k - 1];
     const T r20x = a.r2[0], r20y = a.r2[1], r20z = a.r2[2];
     const T dt = a.dt;
     // Along an axis where this cell is undamped (d = 0, so g4 = 0 at the
@@ -1534,7 +1538,7 @@ __device__ __forceinline__ void stress_pml_cell(const K3<T> &a, const int i, con
     const T mu0 = medv<S16>(a, MMU, c, k);
     T lam2mu_R;
     if constexpr (S16) lam2mu_R = medv<S16>(a, MLAM, c, k) + 2 * mu0;
-    else lam2mu_R = (T)(a.med[MLAM][c] + 2 * a.med[MMU][c]);
+    else lam2mu_R = (T)(gp(a.med[MLAM])[c] + 2 * gp(a.med[MMU])[c]);
     const T lam_R = lam2mu_R - 2 * mu0;
     const T axvx = lx ? ldA<S16>(a, 0, c) : (T)0, ayvy = ly ? ldA<S16>(a, 4, c) : (T)0;
     const T azvz = lz ? ldA<S16>(a, 8, c) : (T)0;
@@ -1565,7 +1569,7 @@ __device__ __forceinline__ void stress_pml_cell(const K3<T> &a, const int i, con
         mxy = medv<S16>(a, MMUXY, c, k);
     } else {
         // the averaged moduli of averaged_medium from mu and its +1 neighbours
-        const float *__restrict__ M = a.med[MMU];
+        const GLB float *__restrict__ M = gp(a.med[MMU]);
         const float nnn = M[c], pnn = M[c + sxy], npn = M[c + 1], ppn = M[c + sxy + 1];
         const float nnp = M[c + px], npp = M[c + px + 1], pnp = M[c + sxy + px];
         mxz = (T)harm4s(nnn, pnn, npn, ppn);
@@ -1609,12 +1613,12 @@ __device__ __forceinline__ void vel_pml_cell(const K3<T> &a, const int i, const 
     const STile<T> Sxz = a.S[4];
     const STile<T> Sxy = a.S[5];
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const T gxc1 = a.gx[0 * nx + i - 1], gxc2 = a.gx[1 * nx + i - 1], gxc3 = a.gx[2 * nx + i - 1], gxc4 = a.gx[3 * nx + i - 1];
-    const T gxe1 = a.gx[4 * nx + i - 1], gxe2 = a.gx[5 * nx + i - 1], gxe3 = a.gx[6 * nx + i - 1], gxe4 = a.gx[7 * nx + i - 1];
-    const T gyc1 = a.gy[0 * ny + j - 1], gyc2 = a.gy[1 * ny + j - 1], gyc3 = a.gy[2 * ny + j - 1], gyc4 = a.gy[3 * ny + j - 1];
-    const T gye1 = a.gy[4 * ny + j - 1], gye2 = a.gy[5 * ny + j - 1], gye3 = a.gy[6 * ny + j - 1], gye4 = a.gy[7 * ny + j - 1];
-    const T gzc1 = a.gz[0 * nz + k - 1], gzc2 = a.gz[1 * nz + k - 1], gzc3 = a.gz[2 * nz + k - 1], gzc4 = a.gz[3 * nz + k - 1];
-    const T gze1 = a.gz[4 * nz + k - 1], gze2 = a.gz[5 * nz + k - 1], gze3 = a.gz[6 * nz + k - 1], gze4 = a.gz[7 * nz + k - 1];
+    const T gxc1 = gp(a.gx)[0 * nx + i - 1], gxc2 = gp(a.gx)[1 * nx + i - 1], gxc3 = gp(a.gx)[2 * nx + i - 1], gxc4 = gp(a.gx)[3 * nx + i - 1];
+    const T gxe1 = gp(a.gx)[4 * nx + i - 1], gxe2 = gp(a.gx)[5 * nx + i - 1], gxe3 = gp(a.gx)[6 * nx + i - 1], gxe4 = gp(a.gx)[7 * nx + i - 1];
+    const T gyc1 = gp(a.gy)[0 * ny + j - 1], gyc2 = gp(a.gy)[1 * ny + j - 1], gyc3 = gp(a.gy)[2 * ny + j - 1], gyc4 = gp(a.gy)[3 * ny + j - 1];
+    const T gye1 = gp(a.gy)[4 * ny + j - 1], gye2 = gp(a.gy)[5 * ny + j - 1], gye3 = gp(a.gy)[6 * ny + j - 1], gye4 = gp(a.gy)[7 * ny + j - 1];
+    const T gzc1 = gp(a.gz)[0 * nz + k - 1], gzc2 = gp(a.gz)[1 * nz + k - 1], gzc3 = gp(a.gz)[2 * nz + k - 1], gzc4 = gp(a.gz)[3 * nz + k - 1];
+    const T gze1 = gp(a.gz)[4 * nz + k - 1], gze2 = gp(a.gz)[5 * nz + k - 1], gze3 = gp(a.gz)[6 * nz + k - 1], gze4 = gp(a.gz)[7 * nz + k - 1];
     const T r20x = a.r2[0], r20y = a.r2[1], r20z = a.r2[2];
     const T dt = a.dt;
     // undamped axes: see k3_stress_pml
@@ -1640,7 +1644,7 @@ __device__ __forceinline__ void vel_pml_cell(const K3<T> &a, const int i, const 
     T bx, by, bz;
     if (a.rho) {
         // buoyancies 2/(rho + rho') of averaged_medium from the density
-        const float *__restrict__ Rh = a.rho;
+        const GLB float *__restrict__ Rh = gp(a.rho);
         const float r0 = Rh[c];
         bx = (T)(2.0f / (r0 + Rh[c + 1]));
         by = (T)(2.0f / (r0 + Rh[c + px]));
@@ -1650,11 +1654,11 @@ __device__ __forceinline__ void vel_pml_cell(const K3<T> &a, const int i, const 
         by = medv<S16>(a, MBY, c, k);
         bz = medv<S16>(a, MBZ, c, k);
     }
-    a.V[0][c] = a.V[0][c] + bx * (gxe1 * dxSxx + gyc1 * dySxy + gzc1 * dzSxz
+    gp(a.V[0])[c] = gp(a.V[0])[c] + bx * (gxe1 * dxSxx + gyc1 * dySxy + gzc1 * dzSxz
                                   + gxe2 * axsxx + gyc2 * aysxy + gzc2 * azsxz) * dt;
-    a.V[1][c] = a.V[1][c] + by * (gxc1 * dxSxy + gye1 * dySyy + gzc1 * dzSyz
+    gp(a.V[1])[c] = gp(a.V[1])[c] + by * (gxc1 * dxSxy + gye1 * dySyy + gzc1 * dzSyz
                                   + gxc2 * axsxy + gye2 * aysyy + gzc2 * azsyz) * dt;
-    a.V[2][c] = a.V[2][c] + bz * (gxc1 * dxSxz + gyc1 * dySyz + gze1 * dzSzz
+    gp(a.V[2])[c] = gp(a.V[2])[c] + bz * (gxc1 * dxSxz + gyc1 * dySyz + gze1 * dzSzz
                                   + gxc2 * axsxz + gyc2 * aysyz + gze2 * azszz) * dt;
     if (lx) stA<S16>(a, 9, c, gxe3 * axsxx + gxe4 * dxSxx * dt);
     if (ly) stA<S16>(a, 10, c, gyc3 * aysxy + gyc4 * dySxy * dt);

@@ -93,6 +93,45 @@ def test_3d_one_update_random_state(bits):
             assert e <= tol, (f, e)
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64", "fp32_r16"])
+@pytest.mark.parametrize("dims", [(37, 29, 23), (131, 67, 45)])
+def test_3d_field_roundtrip_every_field(prec, dims):
+    """Every field set to distinct random values over the whole memory box
+    (ghost cells included), then all read back: each equals what was set,
+    so no field overlaps another and no cell lands in another's place.
+    The memory variables and the stresses are stored in tiles of 256 cells
+    (18 and 6 components per tile); these grids end in a partial tile and
+    start their rows at different slots.  Values stored in 16 bit
+    (fp32_r16 memory variables) must round-trip once rounded: the second
+    read equals the first exactly, and the first is within FP16 rounding
+    of the value set (inside the FP16 range of the scale)."""
+    nx, ny, nz = dims
+    case = problem.case3d(nx, ny, nz, 0.5, 0.04, 4, na=6, src_depth=4.0, nst_side=2, zbeg=-2.0)
+    say(f"3-D field round trip {prec} {dims}")
+    rng = np.random.default_rng(11)
+    dt = np.float64 if prec == "fp64" else np.float32
+    with Swpc3D(case.grid, case.medium, case.pml, case.glut, case.stations, 1, 4, precision=prec) as sim:
+        want = {}
+        for f in FIELDS3:
+            if prec == "fp32_r16" and f[0] == "R":
+                # inside the FP16 range of this source's scale (65504 * 2^-26,
+                # about 9.8e-4); beyond it the stores saturate by design
+                a = rng.uniform(-4e-4, 4e-4, case.grid.shape).astype(dt)
+            else:
+                a = rng.standard_normal(case.grid.shape).astype(dt)
+            want[f] = a
+            sim.set_field(f, a)
+        for f in FIELDS3:
+            got = sim.field(f)
+            if prec == "fp32_r16" and f[0] == "R":
+                w = want[f].astype(np.float64)
+                assert np.abs(got - w).max() <= 1e-3 * np.abs(w).max(), f
+                sim.set_field(f, got)
+                assert np.array_equal(sim.field(f), got), f
+            else:
+                assert np.array_equal(got, want[f]), f
+
+
 CASES2 = {
     "layered": (200, 120, 0.25, 0.02, 120, 12, 6.0, -2.0, None),
     "ragged": (131, 77, 0.25, 0.02, 90, 9, 4.0, -1.0, None),
