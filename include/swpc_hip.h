@@ -264,8 +264,12 @@ int swpc3d_memory(swpc3d_ctx *ctx, long long *bytes);
      [0..7]  one-cell-per-thread kernels (FP64 always uses them): stress
              tile rows (2/4/8), stress waves-per-SIMD hint (0/3/4), split
              normal/shear stress (0/1), velocity tile rows, velocity waves
-             hint (0/5), XCD-contiguous workgroup order (0/1), stress
-             z-chunk, velocity z-chunk (0 = automatic);
+             hint (0/5), workgroup order [5], stress z-chunk, velocity
+             z-chunk (0 = automatic).  [5]: 0 as launched, 1 XCD-contiguous
+             (one-cell and 4-wide kernels), G = 2..64 the z-marching
+             4-wide velocity update's y tiles in groups of G, group g on
+             XCD g mod 8 (default 2: 593 against 603 us at 512 x 512 x 256;
+             the other kernels as launched);
      [8]     4-cell-per-thread kernels for FP32 and reduced storage (0/1,
              default 1); [9] their tile rows (1/2/4, default 2);
      [10]    4-cell-per-thread PML kernels (0/1, default 0);

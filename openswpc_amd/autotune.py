@@ -23,8 +23,9 @@ VEL = [(4, 0), (2, 0), (8, 0), (4, 5), (8, 5)]
 # 4-wide candidates: (form, z-march chunk) for stress, z-march chunk for velocity
 STRESS4 = [(0, 0), (1, 0), (2, 0), (1, 4), (1, 8), (1, 16)]
 VEL4 = [0, 8, 16, 32]
-# tuning vector defaults of the first eleven entries (see swpc3d_set_tuning)
-BASE = [4, 3, 0, 4, 5, 0, 32, 0, 1, 2, 0]
+# tuning vector defaults of the first eleven entries (see swpc3d_set_tuning;
+# entry 5 = 2: the z-marching velocity update's y tiles in pairs per XCD)
+BASE = [4, 3, 0, 4, 5, 2, 32, 0, 1, 2, 0]
 
 
 def _time(sim, it0, steps, cls):

@@ -115,7 +115,7 @@ struct K3 {
     int kc;   // z-chunk length of the interior kernels
     int ntx, nty, nkc;          // interior tiles along x, y and z chunks
     unsigned nblk, nblk_pad;    // ntx*nty*nkc and its round-up to a multiple of 8
-    int xcd;                    // remap workgroup ids XCD-contiguously
+    int xcd;                    // workgroup order: 0 as launched, 1 XCD-contiguous, >= 2 y-groups (velz_blk)
     int ade_sparse;             // ADE arrays hold zeros wherever their axis is undamped
     Boxes3 pml;
 };
@@ -1194,16 +1194,32 @@ __global__ void __launch_bounds__(64 * BY) k3x_vel(const K3<float> a)
 template <int BY, bool S16, bool RHO, bool XS = false>
 __device__ __forceinline__ void velz_blk(const K3<float> &a, unsigned b)
 {
-    if (a.xcd) {
-        // each XCD walks one contiguous run of tiles, so the y-neighbour
-        // rows of a tile were fetched into the same L2
-        b = (b & 7u) * (a.nblk_pad >> 3) + (b >> 3);
-        if (b >= a.nblk) return;
+    unsigned xt, yt, zc;
+    if (a.xcd >= 2) {
+        // groups of G = a.xcd consecutive y tiles (all x tiles, one
+        // z-chunk), group g on XCD g mod 8 (workgroups are dealt to the
+        // XCDs in turn): the y-halo rows that a group's tiles share stay in
+        // one L2, while the XCDs still work on neighbouring rows
+        const unsigned G = (unsigned)a.xcd, per = G * (unsigned)a.ntx;
+        const unsigned ngy = ((unsigned)a.nty + G - 1u) / G;
+        const unsigned s = b >> 3, gi = (b & 7u) + 8u * (s / per), r = s % per;
+        if (gi >= ngy * (unsigned)a.nkc) return;
+        zc = gi / ngy;
+        yt = (gi % ngy) * G + r / (unsigned)a.ntx;
+        xt = r % (unsigned)a.ntx;
+        if (yt >= (unsigned)a.nty) return;
+    } else {
+        if (a.xcd) {
+            // each XCD walks one contiguous run of tiles, so the y-neighbour
+            // rows of a tile were fetched into the same L2
+            b = (b & 7u) * (a.nblk_pad >> 3) + (b >> 3);
+            if (b >= a.nblk) return;
+        }
+        xt = b % (unsigned)a.ntx;
+        const unsigned t = b / (unsigned)a.ntx;
+        yt = t % (unsigned)a.nty;
+        zc = t / (unsigned)a.nty;
     }
-    const unsigned xt = b % (unsigned)a.ntx;
-    const unsigned t = b / (unsigned)a.ntx;
-    const unsigned yt = t % (unsigned)a.nty;
-    const unsigned zc = t / (unsigned)a.nty;
     const int i0 = a.ti0 + (int)xt * 256 + (int)threadIdx.x * 4;
     const int j = a.na + 1 + (int)yt * BY + (int)threadIdx.y;
     if (j > a.ny - a.na) return;
@@ -2233,7 +2249,7 @@ struct swpc3d_ctx {
     int gxp = 0;
     int ti0 = 1;
     // interior launch configuration (swpc3d_set_tuning)
-    int s_by = 4, s_wpe = 3, s_split = 0, v_by = 4, v_wpe = 5, xcd = 0, s_kc = 32, v_kc = 0;
+    int s_by = 4, s_wpe = 3, s_split = 0, v_by = 4, v_wpe = 5, xcd = 2, s_kc = 32, v_kc = 0;
     int x4 = 1, x4_by = 2;  // 4-wide FP32 / reduced-storage interior kernels
     int x4_form = 0;        // 4-wide stress: 0 normal + shear launches, 1 one pass (staged), 2 one pass (free)
     int x4_vby = 2;         // tile rows of the z-marching velocity kernel (1, 2, 4, 8)
@@ -2322,7 +2338,7 @@ static K3<T> make_args(swpc3d_ctx *c)
     a.dt = t;
     a.ti0 = c->ti0;
     a.pml = c->boxes;
-    a.xcd = c->xcd;
+    a.xcd = c->xcd == 1;  // the y-group orders (>= 2) are the z-marching velocity update's (x4_vel)
     a.ade_sparse = c->ade_sparse ? 1 : 0;
     return a;
 }
@@ -2422,9 +2438,9 @@ static unsigned x4_tiles(const swpc3d_ctx *c, K3<float> &a, int by)
     a.nty = nj > 0 ? (nj + by - 1) / by : 0;
     const int nk = d.nz - d.na;
     a.nkc = nk;
-    a.xcd = c->xcd;
+    a.xcd = c->xcd == 1;
     if (a.ntx <= 0 || a.nty <= 0 || nk <= 0) return 0;
-    if (c->xcd) return 8u * (unsigned)a.ntx * (((unsigned)a.nty + 7u) / 8u) * (unsigned)nk;
+    if (a.xcd) return 8u * (unsigned)a.ntx * (((unsigned)a.nty + 7u) / 8u) * (unsigned)nk;
     return (unsigned)a.ntx * (unsigned)a.nty * (unsigned)nk;
 }
 
@@ -2532,7 +2548,12 @@ static bool x4_vel(const swpc3d_ctx *c, K3<float> a, hipStream_t s)
         if (a.ntx <= 0 || a.nty <= 0 || nk <= 0) return true;
         a.nblk = (unsigned)a.ntx * (unsigned)a.nty * (unsigned)a.nkc;
         a.nblk_pad = (a.nblk + 7u) & ~7u;
-        const unsigned nb = a.xcd ? a.nblk_pad : a.nblk;
+        a.xcd = c->xcd;
+        unsigned nb = a.xcd ? a.nblk_pad : a.nblk;
+        if (a.xcd >= 2) {
+            const unsigned G = (unsigned)a.xcd, ngrp = ((unsigned)a.nty + G - 1u) / G * (unsigned)a.nkc;
+            nb = 8u * G * (unsigned)a.ntx * ((ngrp + 7u) / 8u);
+        }
         if (c->x4_vlds) {
             // the LDS-staged form walks the tiles in order (no XCD remap)
             switch (vby) {
@@ -4173,6 +4194,7 @@ int swpc3d_set_tuning(swpc3d_ctx *c, const int *p, int n)
     for (int q = 0; q < n && q < 23; q++) v[q] = p[q];
     if (v[18] < -24 || v[18] > 24) return fail(SWPC_EINVAL, "memory-variable scale shift must be in -24..24");
     if (v[16] < 0 || v[16] > 1024) return fail(SWPC_EINVAL, "steps per graph must be in 0..1024");
+    if (v[5] < 0 || v[5] > 64) return fail(SWPC_EINVAL, "XCD order must be in 0..64");
     if (v[14] < 0 || v[15] < 0) return fail(SWPC_EINVAL, "negative z-chunk");
     if (v[6] < 0 || v[7] < 0) return fail(SWPC_EINVAL, "negative z-chunk");
     if (v[9] != 1 && v[9] != 2 && v[9] != 4) return fail(SWPC_EINVAL, "4-wide tile rows must be 1, 2 or 4");
@@ -4180,7 +4202,7 @@ int swpc3d_set_tuning(swpc3d_ctx *c, const int *p, int n)
     if (v[12] != 1 && v[12] != 2 && v[12] != 4 && v[12] != 8)
         return fail(SWPC_EINVAL, "z-marching velocity tile rows must be 1, 2, 4 or 8");
     c->s_by = v[0]; c->s_wpe = v[1]; c->s_split = v[2] != 0; c->v_by = v[3]; c->v_wpe = v[4];
-    c->xcd = v[5] != 0; c->s_kc = v[6]; c->v_kc = v[7]; c->x4 = v[8] != 0; c->x4_by = v[9]; c->pml4 = v[10] != 0;
+    c->xcd = v[5]; c->s_kc = v[6]; c->v_kc = v[7]; c->x4 = v[8] != 0; c->x4_by = v[9]; c->pml4 = v[10] != 0;
     c->x4_form = v[11]; c->x4_vby = v[12]; c->conc = v[13] != 0; c->x4_vz = v[14]; c->x4_sz = v[15];
     c->graph_k = v[16];
     c->sr = v[17] != 0;
