@@ -54,17 +54,20 @@ struct Boxes3 {
 // K3::R[q] and STile::p point at component q's slot of tile 0.
 __host__ __device__ __forceinline__ long long rix(int c) { return tix<NR3>(c); }
 
+// Global-memory address space: every state array lives there, and
+// accesses that say so compile to global_* instructions also where the
+// compiler cannot trace a pointer back to a kernel argument.
+#define GLB __attribute__((address_space(1)))
+
+// p as a global-memory pointer
+template <typename T>
+__device__ __forceinline__ GLB T *gp(T *p) { return (GLB T *)p; }
+
 // Cell c of one stress component in the tiled layout: a cursor that moves
 // by cells (p + c - 2 is cell c - 2, wherever its tile lies) and becomes
 // an address only where it is loaded or stored (ld4, ld2, ld4nt, st4m,
 // at()).  It has no conversion to a pointer, so an address cannot be
 // offset by cells after the tile mapping by mistake.
-#define GLB __attribute__((address_space(1)))
-
-// p as a global-memory pointer (see ld4)
-template <typename T>
-__device__ __forceinline__ GLB T *gp(T *p) { return (GLB T *)p; }
-
 template <typename T>
 struct SCell {
     T *p;
@@ -455,10 +458,8 @@ k3_vel_int(const K3<T> a)
 // k-2..k+2 that a band needs stay in that XCD's L2.
 typedef float f4v __attribute__((ext_vector_type(4)));
 
-// Every state array lives in global memory: the accesses say so, so that
-// they compile to global_* instructions also where the compiler cannot
-// trace a pointer back to a kernel argument (flat_* would otherwise wait
-// on the LDS counter too).
+// Vector accesses of state arrays, as global-memory instructions (GLB;
+// flat_* would also wait on the LDS counter).
 __device__ __forceinline__ f4v ld4(const float *p) { return *(const GLB f4v *)p; }
 __device__ __forceinline__ f4v ld4nt(const float *p) { return __builtin_nontemporal_load((const GLB f4v *)p); }
 __device__ __forceinline__ void st4nt(float *p, f4v v) { __builtin_nontemporal_store(v, (GLB f4v *)p); }

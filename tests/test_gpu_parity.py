@@ -1,8 +1,8 @@
 """HIP path vs the CPU oracle, through the C ABI, on seeded inputs.
 
 Bars: FP32 kernels within relative L2 1e-5 of the FP32 oracle on
-receiver traces and wavefields (the plain-kernel tolerance of the build's
-north star); FP64 kernels within 1e-11 of the FP64 oracle.
+receiver traces and wavefields (the tolerance stated for the plain FP32
+kernels); FP64 kernels within 1e-11 of the FP64 oracle.
 """
 import numpy as np
 import pytest
