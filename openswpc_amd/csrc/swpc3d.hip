@@ -3239,19 +3239,77 @@ static DBuf *field_tiles(swpc3d_ctx *c, int f, size_t &q, size_t &ns)
     return nullptr;
 }
 
+// Component slot q of a tiled region (ns components per tile) to and from
+// a plain array of n cells, elements of type E (the bits only).
+template <typename E>
+__global__ void k_tile_gather(const E *__restrict__ t, int ns, int q, E *__restrict__ out, long long n)
+{
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+        const long long u = e + RB;
+        out[e] = t[((u >> 8) * ns + q) * RTILE + (u & (RTILE - 1))];
+    }
+}
+
+template <typename E>
+__global__ void k_tile_scatter(E *__restrict__ t, int ns, int q, const E *__restrict__ in, long long n)
+{
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+        const long long u = e + RB;
+        t[((u >> 8) * ns + q) * RTILE + (u & (RTILE - 1))] = in[e];
+    }
+}
+
+template <typename E>
+static void tile_copy(bool gather, void *tiles, int ns, int q, void *plain, long long n, hipStream_t s)
+{
+    const unsigned nb = (unsigned)std::min<long long>((n + 255) / 256, 65536);
+    if (gather) hipLaunchKernelGGL(k_tile_gather<E>, dim3(nb), dim3(256), 0, s, (const E *)tiles, ns, q, (E *)plain, n);
+    else hipLaunchKernelGGL(k_tile_scatter<E>, dim3(nb), dim3(256), 0, s, (E *)tiles, ns, q, (const E *)plain, n);
+}
+
+// Gather (scatter) a tiled field through a temporary plain device array:
+// one kernel and one contiguous copy.  False when the temporary cannot be
+// allocated (the caller then copies tile by tile).
+static bool tile_copy_dev(swpc3d_ctx *c, bool gather, void *tiles, int ns, int q, void *host, size_t el, int &err)
+{
+    const long long n = c->L.vol;
+    void *tmp = nullptr;
+    err = SWPC_OK;
+    if (hipMalloc(&tmp, (size_t)n * el) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    hipError_t e = hipSuccess;
+    if (!gather) e = hipMemcpy(tmp, host, (size_t)n * el, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        if (el == 8) tile_copy<unsigned long long>(gather, tiles, ns, q, tmp, n, c->stream);
+        else if (el == 4) tile_copy<unsigned>(gather, tiles, ns, q, tmp, n, c->stream);
+        else tile_copy<unsigned short>(gather, tiles, ns, q, tmp, n, c->stream);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    if (e == hipSuccess && gather) e = hipMemcpy(host, tmp, (size_t)n * el, hipMemcpyDeviceToHost);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) err = fail(SWPC_EHIP, "tiled field copy failed: %s", hipGetErrorString(e));
+    return true;
+}
+
 // Field f's vol elements of el bytes between the device and a contiguous
-// host array: a tiled field is gathered from (scattered to) its slots of
-// the tiles with one strided copy, the cells before the first and after
-// the last of the box left out (zero on the device).
+// host array.  A tiled field is gathered from (scattered to) its slots of
+// the tiles on the device; if no temporary array fits, one strided copy
+// moves the tiles' slots (the cells before the first and after the last of
+// the box left out, zero on the device).
 static int field_read(swpc3d_ctx *c, int f, void *dst, size_t el)
 {
     const size_t n = (size_t)c->L.vol;
     size_t q = 0, ns = 0;
-    const DBuf *t = field_tiles(c, f, q, ns);
+    DBuf *t = field_tiles(c, f, q, ns);
     if (!t) {
         SWPC_HIP(hipMemcpy(dst, field_buf(c, f)->p, n * el, hipMemcpyDeviceToHost));
         return SWPC_OK;
     }
+    int err;
+    if (tile_copy_dev(c, true, t->p, (int)ns, (int)q, dst, el, err)) return err;
     const size_t nt = (size_t)c->rtiles;
     std::vector<char> tl(nt * RTILE * el);
     SWPC_HIP(hipMemcpy2D(tl.data(), RTILE * el, (const char *)t->p + q * RTILE * el, ns * RTILE * el, RTILE * el, nt,
@@ -3269,6 +3327,8 @@ static int field_write(swpc3d_ctx *c, int f, const void *src, size_t el)
         SWPC_HIP(hipMemcpy(field_buf(c, f)->p, src, n * el, hipMemcpyHostToDevice));
         return SWPC_OK;
     }
+    int err;
+    if (tile_copy_dev(c, false, t->p, (int)ns, (int)q, const_cast<void *>(src), el, err)) return err;
     const size_t nt = (size_t)c->rtiles;
     std::vector<char> tl(nt * RTILE * el, 0);
     memcpy(tl.data() + (size_t)RB * el, src, n * el);
