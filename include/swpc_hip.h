@@ -307,7 +307,14 @@ int swpc3d_memory(swpc3d_ctx *ctx, long long *bytes);
              cells from the adjacent lanes by DPP wavefront shifts instead
              of loads (default 2: the velocity update, 1.6 % faster; the
              stress update is 0.4 % slower with them, 2 % with reduced
-             storage). */
+             storage);
+     [23]    learned boundary: strip cache of the ghost fill (0/1, default
+             1).  Each phase stores its own fields' strips, as they stand
+             after its sponge, in a compact array, and the other phase
+             reads them from there instead of the fields (the half-step
+             between them leaves those fields alone); dropped whenever a
+             field is set or a single update is launched outside
+             swpc3d_step/swpc3d_run.  Results are the same bit for bit. */
 int swpc3d_set_tuning(swpc3d_ctx *ctx, const int *p, int n);
 
 /* ------------------------------------------------------------------ */

@@ -2230,6 +2230,11 @@ struct swpc3d_ctx {
     swpc::DBuf lab_w;       // its ten nets as the fill kernel's LDS images (BF16 values, nearest even)
     swpc::DBuf lab_colmed;  // per-column medium features
     swpc::LabDomain lab_dom;
+    // strip cache of the fill (9 fields x 6 strip cells x every column): a
+    // phase reads the other phase's strips from it while lab_ok[phase]
+    swpc::DBuf lab_cache;
+    bool lab_ok[2] = {false, false};
+    bool lab_cache_on = true;   // tuning entry 23
     float lab_sponge = swpc::LAB_SPONGE;  // strip sponge strength (swpc3d_set_lab_sponge)
     bool s16 = false;       // memory variables in 16 bit (SWPC_FP32_S16, SWPC_FP32_R16)
     bool m16 = false;       // medium in 16 bit too (SWPC_FP32_S16)
@@ -2629,8 +2634,12 @@ static int launch_lab(swpc3d_ctx *c, int phase)
     const int cls = phase == 0 ? 1 : 3;
     if (int r = prof_begin(c->prof, cls, c->stream, &e0)) return r;
     if (int r = lab_fill(lab_grid(c), c->lab_dom, c->lab_w.p, (const float4 *)c->lab_colmed.p, phase, c->lab_sponge,
-                         c->stream))
+                         c->lab_cache_on ? c->lab_cache.p : nullptr, c->lab_ok[phase], c->stream))
         return r;
+    // this phase's strips are now cached for the other phase; the other
+    // phase's cached strips go stale with the coming half-step
+    c->lab_ok[1 - phase] = c->lab_cache_on && c->lab_cache.p != nullptr;
+    c->lab_ok[phase] = false;
     return prof_end(c->prof, cls, c->stream, e0);
 }
 
@@ -2894,6 +2903,15 @@ static void graph_drop(swpc3d_ctx *c)
     c->graph = nullptr;
 }
 
+// Fields changed, or may have, outside the fixed order of a step: the
+// learned-boundary fill's strip cache is not to be read, and a captured
+// step sequence that would read it is dropped.
+static void lab_stale(swpc3d_ctx *c)
+{
+    if (c->lab_ok[0] || c->lab_ok[1]) graph_drop(c);
+    c->lab_ok[0] = c->lab_ok[1] = false;
+}
+
 // Capture graph_k steps (kernels and counter advances) once; the graph is
 // dropped whenever a setter changes what the steps launch.
 static int graph_ready(swpc3d_ctx *c)
@@ -2904,6 +2922,7 @@ static int graph_ready(swpc3d_ctx *c)
     for (int q = 0; q < c->graph_k && r == SWPC_OK; q++) r = step_dev_any(c);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (r != SWPC_OK || e != hipSuccess || !g) c->lab_ok[0] = c->lab_ok[1] = false;
     if (r != SWPC_OK) {
         if (g) (void)hipGraphDestroy(g);
         return r;
@@ -3501,6 +3520,7 @@ int swpc3d_destroy(swpc3d_ctx *c)
     for (auto &b : c->cpl) dfree(b);
     dfree(c->rho);
     dfree(c->lab_w);
+    dfree(c->lab_cache);
     dfree(c->lab_colmed);
     DBuf *rest[] = {&c->surf, &c->kob, &c->gx, &c->gy, &c->gz, &c->gxv, &c->src, &c->m6, &c->sdrop, &c->bsrc, &c->f3,
                     &c->fdt, &c->st, &c->uacc, &c->wvel, &c->wdisp, &c->vmax};
@@ -3723,6 +3743,7 @@ int swpc3d_set_lab(swpc3d_ctx *c, const float *params)
 {
     clear_error();
     if (c) graph_drop(c);  // the captured steps would launch stale arguments
+    if (c) lab_stale(c);
     if (c && !params) {
         c->lab = false;
         return SWPC_OK;
@@ -3744,6 +3765,9 @@ int swpc3d_set_lab(swpc3d_ctx *c, const float *params)
     SWPC_HIP(hipMemcpy(c->lab_w.p, img.data(), img.size(), hipMemcpyHostToDevice));
     c->lab_dom = lab_domain(0, 0, c->d.nx, c->d.ny, c->d.nz);
     if (int r = lab_colmed(lab_grid(c), c->lab_dom, c->lab_colmed, c->bytes, c->stream)) return r;
+    const size_t cache_bytes = (size_t)LAB_NF * (size_t)c->lab_dom.col0[5] * c->esz;
+    if (c->lab_cache.bytes != cache_bytes)
+        if (int r = dalloc(c->lab_cache, cache_bytes, c->bytes)) return r;
     SWPC_HIP(hipStreamSynchronize(c->stream));
     c->lab = true;
     return SWPC_OK;
@@ -3948,14 +3972,20 @@ int swpc3d_update_stress(swpc3d_ctx *c)
 {
     clear_error();
     if (int r = ready(c)) return r;
-    return fp64(c) ? launch_stress<double>(c) : launch_stress<float>(c);
+    lab_stale(c);
+    const int r = fp64(c) ? launch_stress<double>(c) : launch_stress<float>(c);
+    lab_stale(c);
+    return r;
 }
 
 int swpc3d_update_vel(swpc3d_ctx *c)
 {
     clear_error();
     if (int r = ready(c)) return r;
-    return fp64(c) ? launch_vel<double>(c) : launch_vel<float>(c);
+    lab_stale(c);
+    const int r = fp64(c) ? launch_vel<double>(c) : launch_vel<float>(c);
+    lab_stale(c);
+    return r;
 }
 
 int swpc3d_stressglut(swpc3d_ctx *c, int it)
@@ -3964,7 +3994,10 @@ int swpc3d_stressglut(swpc3d_ctx *c, int it)
     if (int r = ready(c)) return r;
     if (c->nsrc > 0 && (it < 1 || it > c->nt_src)) return fail(SWPC_EINVAL, "stressglut: it=%d outside 1..%d", it, c->nt_src);
     if (int r = set_step(c, it)) return r;
-    return fp64(c) ? launch_glut<double>(c) : launch_glut<float>(c);
+    lab_stale(c);
+    const int r = fp64(c) ? launch_glut<double>(c) : launch_glut<float>(c);
+    lab_stale(c);
+    return r;
 }
 
 int swpc3d_bodyforce(swpc3d_ctx *c, int it)
@@ -3973,7 +4006,10 @@ int swpc3d_bodyforce(swpc3d_ctx *c, int it)
     if (int r = ready(c)) return r;
     if (c->nbf > 0 && (it < 1 || it > c->nt_bf)) return fail(SWPC_EINVAL, "bodyforce: it=%d outside 1..%d", it, c->nt_bf);
     if (int r = set_step(c, it)) return r;
-    return fp64(c) ? launch_bf<double>(c) : launch_bf<float>(c);
+    lab_stale(c);
+    const int r = fp64(c) ? launch_bf<double>(c) : launch_bf<float>(c);
+    lab_stale(c);
+    return r;
 }
 
 int swpc3d_store_receivers(swpc3d_ctx *c, int it)
@@ -4150,6 +4186,7 @@ int swpc3d_set_field(swpc3d_ctx *c, int f, const double *in)
 {
     clear_error();
     if (c) graph_drop(c);  // the captured steps would launch stale arguments
+    if (c) lab_stale(c);   // the fill's strip cache no longer holds the fields
     if (!c || !in) return fail(SWPC_EINVAL, "null argument");
     DBuf *b = field_buf(c, f);
     if (!b) return fail(SWPC_EINVAL, "unknown field %d", f);
@@ -4249,10 +4286,10 @@ int swpc3d_set_tuning(swpc3d_ctx *c, const int *p, int n)
     clear_error();
     if (c) graph_drop(c);  // the captured steps would launch stale arguments
     if (!c || (n > 0 && !p)) return fail(SWPC_EINVAL, "null argument");
-    int v[23] = {c->s_by, c->s_wpe, c->s_split, c->v_by, c->v_wpe, c->xcd, c->s_kc, c->v_kc, c->x4, c->x4_by, c->pml4,
+    int v[24] = {c->s_by, c->s_wpe, c->s_split, c->v_by, c->v_wpe, c->xcd, c->s_kc, c->v_kc, c->x4, c->x4_by, c->pml4,
                  c->x4_form, c->x4_vby, c->conc, c->x4_vz, c->x4_sz, c->graph_k, c->sr, c->r_shift, c->pml_rows,
-                 c->pml_xcd, c->x4_vlds, c->xs};
-    for (int q = 0; q < n && q < 23; q++) v[q] = p[q];
+                 c->pml_xcd, c->x4_vlds, c->xs, c->lab_cache_on ? 1 : 0};
+    for (int q = 0; q < n && q < 24; q++) v[q] = p[q];
     if (v[18] < -24 || v[18] > 24) return fail(SWPC_EINVAL, "memory-variable scale shift must be in -24..24");
     if (v[16] < 0 || v[16] > 1024) return fail(SWPC_EINVAL, "steps per graph must be in 0..1024");
     if (v[5] < 0 || v[5] > 64) return fail(SWPC_EINVAL, "XCD order must be in 0..64");
@@ -4271,6 +4308,8 @@ int swpc3d_set_tuning(swpc3d_ctx *c, const int *p, int n)
     c->pml_xcd = v[20] != 0;
     c->x4_vlds = v[21] != 0;
     c->xs = v[22] & 3;
+    if (c->lab_cache_on != (v[23] != 0)) lab_stale(c);
+    c->lab_cache_on = v[23] != 0;
     if (v[18] != c->r_shift) {
         // the stored memory variables are rescaled with the new scale
         if (int r = rescale_r(c, std::ldexp(1.0f, c->r_shift - v[18]))) return r;

@@ -297,15 +297,36 @@ __device__ __forceinline__ void lab_strip_store_s(T *__restrict__ S0, int base, 
 // Features of one column into x[LAB_NIN]; prev holds the previous ghost
 // values (stresses already divided by rho*vp).  raw[LAB_NF] receives the
 // strip values as stored (field-major).  Returns N (0: no wave).
+// Field fi's strip value d of global column g in the strip cache
+// ([field][cell][column], so a wave's 64 columns are 64 consecutive values).
+template <typename T>
+__device__ __forceinline__ T *lab_cache_at(T *cache, long long ncol, int fi, int d, long long g)
+{
+    return cache + (long long)(fi * LAB_NS + d) * ncol + g;
+}
+
+// The strips of the velocities (vc) or the stresses (sc) come from the
+// strip cache of column g instead of the fields.
 template <typename T>
 __device__ __forceinline__ float lab_features(const LabDev &a, int base, int step, bool xrow, float4 m,
-                                              const float *prev, float *x, T *raw)
+                                              const float *prev, float *x, T *raw, const T *cache, long long ncol,
+                                              long long g, bool vc, bool sc)
 {
     const float inv_imp = 1.0f / m.z;
     float amax = 0.0f;
+    if (vc) {
 #pragma unroll
-    for (int fi = 0; fi < 3; fi++) lab_strip_load<T>((const T *)a.F[fi], base, step, xrow, raw + fi * LAB_NS);
-    lab_strip_load_s<T>((const T *)a.F[3], base, step, xrow, raw + 3 * LAB_NS);
+        for (int q = 0; q < 3 * LAB_NS; q++) raw[q] = *lab_cache_at(cache, ncol, q / LAB_NS, q % LAB_NS, g);
+    } else {
+#pragma unroll
+        for (int fi = 0; fi < 3; fi++) lab_strip_load<T>((const T *)a.F[fi], base, step, xrow, raw + fi * LAB_NS);
+    }
+    if (sc) {
+#pragma unroll
+        for (int q = 3 * LAB_NS; q < LAB_NF; q++) raw[q] = *lab_cache_at(cache, ncol, q / LAB_NS, q % LAB_NS, g);
+    } else {
+        lab_strip_load_s<T>((const T *)a.F[3], base, step, xrow, raw + 3 * LAB_NS);
+    }
 #pragma unroll
     for (int fi = 0; fi < 9; fi++) {
 #pragma unroll
@@ -383,6 +404,9 @@ struct LabFillArgs {
     int nb0;
     float sp[LAB_NS];          // strip sponge factor of cell d = 1..6 (all 1: no sponge)
     int sponge;                // any factor below 1
+    void *cache;               // strip cache (T [9][LAB_NS][ncol]) or null
+    long long ncol;            // columns of all faces
+    int use_cache;             // the other phase's strips come from the cache
 };
 
 // tanh(z) = 1 - 2/(1 + e^(2z)) with the hardware exponential and a fast
@@ -497,7 +521,22 @@ __device__ __forceinline__ void lab_fill_body(const LabFillArgs &p)
             prev[q] = fi >= 3 ? v / m.z : v;
         }
         T raw[LAB_NF];
-        N = lab_features<T>(a, base, step, xrow, m, prev, x, raw);
+        // Only the x faces' columns are cached: their strips lie along rows,
+        // so the fields' strips cost a scattered line per field, where the
+        // other faces read theirs in coalesced rows.  And a column is cached
+        // only if no later face of the phase (faces run x pair, y pair,
+        // bottom) rescales its strip cells with its sponge: away from the y
+        // faces' and the bottom's strips.
+        bool own = false;
+        if (f < 2) {
+            const int jj = col % a.ny + 1, kk = col / a.ny + 1;
+            own = jj > LAB_NS && jj <= a.ny - LAB_NS && kk <= a.nz - LAB_NS;
+        }
+        T *cache = own ? (T *)p.cache : nullptr;
+        const long long g = a.col0[f] + col;
+        const bool use = cache && p.use_cache;
+        N = lab_features<T>(a, base, step, xrow, m, prev, x, raw, cache, p.ncol, g, use && p.phase == 1,
+                            use && p.phase == 0);
         // strip sponge on this phase's own fields, from the values read
         // above (the column owns its strip cells within the launch)
         if (p.sponge) {
@@ -507,6 +546,16 @@ __device__ __forceinline__ void lab_fill_body(const LabFillArgs &p)
             } else {
                 lab_strip_store_s<T>((T *)a.F[3], base, step, xrow, p.sp, raw + 3 * LAB_NS);
             }
+        }
+        // this phase's strips as they now stand, for the other phase: the
+        // half-step between them leaves these fields alone
+        if (cache) {
+            const int q0 = p.phase == 0 ? 0 : 3 * LAB_NS, q1 = p.phase == 0 ? 3 * LAB_NS : LAB_NF;
+#pragma unroll
+            for (int q = 0; q < LAB_NF; q++)
+                if (q >= q0 && q < q1)
+                    *lab_cache_at(cache, p.ncol, q / LAB_NS, q % LAB_NS, g) =
+                        p.sponge ? raw[q] * (T)p.sp[q % LAB_NS] : raw[q];
         }
     } else {
 #pragma unroll
@@ -653,11 +702,14 @@ void lab_fill_image(const float *w, unsigned char *img)
 }
 
 int lab_fill(const LabGrid &g, const LabDomain &dom, const void *img, const float4 *colmed, int phase, float sponge,
-             hipStream_t s)
+             void *cache, bool use_cache, hipStream_t s)
 {
     LabFillArgs p;
     lab_sponge_factors(sponge, p.sp);
     p.sponge = sponge > 0.0f;
+    p.cache = cache;
+    p.ncol = dom.col0[5];
+    p.use_cache = use_cache ? 1 : 0;
     p.a = make_dev(g, dom);
     p.img = (const uint4 *)img;
     p.colmed = colmed;
@@ -756,7 +808,7 @@ __global__ void __launch_bounds__(256) k_lab_gather(const LabGatherArgs p)
     }
     float x[LAB_NIN];
     T raw[LAB_NF];
-    const float N = lab_features<T>(a, base, step, false, m, prev, x, raw);
+    const float N = lab_features<T>(a, base, step, false, m, prev, x, raw, (const T *)nullptr, 0, 0, false, false);
 #pragma unroll
     for (int q = 0; q < LAB_NOUT; q++) pv[q] = t[q];
     if (!(N > 0.0f) || !(m.y > 0.0f)) return;

@@ -162,6 +162,36 @@ def test_lab_step_matches_oracle(precision, sponge, nxs):
     assert np.abs(fields["Vx"][:, 2, :]).max() > 0
 
 
+@pytest.mark.parametrize("precision,sponge,nxs", [("fp32", None, 24), ("fp32", 0.0, 25), ("fp64", None, 24),
+                                                  ("fp32_r16", None, 25)])
+def test_lab_strip_cache_is_exact(precision, sponge, nxs):
+    """The fill's strip cache (tuning entry 23, default on): the other
+    phase's strips read from the cache instead of the fields give the same
+    fields and traces bit for bit, over graph replays (sim.run), single
+    steps, and steps after a field was set (which drops the cache)."""
+    nt = 70
+    lc = lab.wide_case(nxs, 20, 18, 0.25, 0.01, nt, 6, zbeg=-0.75, off=12)
+    c = lab.inner_case(lc, 0, nst_side=3)
+    P = _demo_params(1)
+    base = [4, 3, 0, 4, 5, 2, 32, 0, 1, 2, 0, 1, 4, 0, 8, 0, 16, 0, 0, 1, 0, 0, 2]
+    out = []
+    for cache in (0, 1):
+        with Swpc3D(c.grid, c.medium, c.pml, c.glut, c.stations, 1, nt, precision=precision) as sim:
+            sim.set_tuning(*(base + [cache]))
+            sim.set_learned_boundary(P)
+            if sponge is not None:
+                sim.set_learned_boundary_sponge(sponge)
+            sim.run(1, 40)
+            sim.run(41, 1)
+            sim.set_field("Sxz", sim.field("Sxz"))
+            sim.run(42, nt - 41)
+            vel, _ = sim.receivers()
+            out.append([vel] + [sim.field(n) for n in ("Vx", "Vz", "Sxx", "Sxz", "Syz")])
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+    assert np.abs(out[1][0]).max() > 0
+
+
 def test_lab_requires_na0():
     c = problem.case3d(40, 36, 32, 0.25, 0.01, 10, na=8, nst_side=2, src_depth=3.0, zbeg=-0.75)
     with Swpc3D(c.grid, c.medium, c.pml, c.glut, None, 1, 10) as sim:
