@@ -2230,8 +2230,9 @@ struct swpc3d_ctx {
     swpc::DBuf lab_w;       // its ten nets as the fill kernel's LDS images (BF16 values, nearest even)
     swpc::DBuf lab_colmed;  // per-column medium features
     swpc::LabDomain lab_dom;
-    // strip cache of the fill (9 fields x 6 strip cells x every column): a
-    // phase reads the other phase's strips from it while lab_ok[phase]
+    // strip cache of the fill (9 fields x 6 strip cells x the x faces'
+    // columns): a phase reads the other phase's strips from it while
+    // lab_ok[phase]
     swpc::DBuf lab_cache;
     bool lab_ok[2] = {false, false};
     bool lab_cache_on = true;   // tuning entry 23
@@ -3765,7 +3766,7 @@ int swpc3d_set_lab(swpc3d_ctx *c, const float *params)
     SWPC_HIP(hipMemcpy(c->lab_w.p, img.data(), img.size(), hipMemcpyHostToDevice));
     c->lab_dom = lab_domain(0, 0, c->d.nx, c->d.ny, c->d.nz);
     if (int r = lab_colmed(lab_grid(c), c->lab_dom, c->lab_colmed, c->bytes, c->stream)) return r;
-    const size_t cache_bytes = (size_t)LAB_NF * (size_t)c->lab_dom.col0[5] * c->esz;
+    const size_t cache_bytes = (size_t)LAB_NF * (size_t)c->lab_dom.col0[2] * c->esz;
     if (c->lab_cache.bytes != cache_bytes)
         if (int r = dalloc(c->lab_cache, cache_bytes, c->bytes)) return r;
     SWPC_HIP(hipStreamSynchronize(c->stream));

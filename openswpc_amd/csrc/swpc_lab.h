@@ -83,7 +83,8 @@ void lab_fill_image(const float *w, unsigned char *img);
 // phase 1) at strip cell d = 1..6 of every column are then multiplied by
 // 1 - sponge ((7 - d)/6)^2 (lab_sponge_factors, formed in FP64 and
 // rounded to FP32).
-// With a strip cache (cache: the state's type, [9][LAB_NS][dom.col0[5]]),
+// With a strip cache (cache: the state's type, [9][LAB_NS][dom.col0[2]], the
+// x faces' columns),
 // the phase stores its own fields' strips there as they stand after the
 // sponge, and with use_cache reads the other phase's strips from it
 // instead of the fields (valid while nothing but the half-step between

@@ -405,7 +405,7 @@ struct LabFillArgs {
     float sp[LAB_NS];          // strip sponge factor of cell d = 1..6 (all 1: no sponge)
     int sponge;                // any factor below 1
     void *cache;               // strip cache (T [9][LAB_NS][ncol]) or null
-    long long ncol;            // columns of all faces
+    long long ncol;            // columns of the x faces (the cache's length)
     int use_cache;             // the other phase's strips come from the cache
 };
 
@@ -708,7 +708,7 @@ int lab_fill(const LabGrid &g, const LabDomain &dom, const void *img, const floa
     lab_sponge_factors(sponge, p.sp);
     p.sponge = sponge > 0.0f;
     p.cache = cache;
-    p.ncol = dom.col0[5];
+    p.ncol = dom.col0[2];   // the x faces' columns (the cached ones) come first
     p.use_cache = use_cache ? 1 : 0;
     p.a = make_dev(g, dom);
     p.img = (const uint4 *)img;
