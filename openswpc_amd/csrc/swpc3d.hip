@@ -3084,6 +3084,8 @@ struct Group3 {
     int nm = 0;
 };
 
+long long group3_cells(const swpc3d_ctx *c) { return (long long)c->d.nx * c->d.ny * c->d.nz; }
+
 bool group3_eligible(const swpc3d_ctx *c)
 {
     return c && ready(c) == SWPC_OK && !fp64(c) && !c->s16 && !c->m16 && c->rho.p && c->x4 &&

@@ -40,6 +40,9 @@ void group2_free(Group2 *g);
 // steps bit for bit.
 struct Group3;
 bool group3_eligible(const swpc3d_ctx *c);
+long long group3_cells(const swpc3d_ctx *c);   // nx * ny * nz
+// 3-D members above this size stay branches (about 128^3 cells)
+constexpr long long GROUP3_MAX_CELLS = 1LL << 21;
 int group3_build(const std::vector<swpc3d_ctx *> &cs, Group3 **out, long long &bytes);
 int group3_step(Group3 *g, hipStream_t s);
 void group3_free(Group3 *g);

@@ -98,12 +98,14 @@ static int batch_capture(swpc_batch *b, int k)
         for (size_t q = 0; q < b->mem.size(); q++)
             if (b->mem[q].c2) grouped[q] = 0;
     }
-    // the 3-D group: every qualifying 3-D member
+    // the 3-D group: every qualifying 3-D member of at most GROUP3_MAX_CELLS
+    // cells (larger ones fill the GPU with their own launches, whose tuned
+    // tile shapes the group's fixed ones would give up)
     if (b->grouping & 2) {
         std::vector<swpc3d_ctx *> g3;
         std::vector<size_t> idx;
         for (size_t q = 0; q < b->mem.size(); q++)
-            if (b->mem[q].c3 && group3_eligible(b->mem[q].c3)) {
+            if (b->mem[q].c3 && group3_eligible(b->mem[q].c3) && group3_cells(b->mem[q].c3) <= GROUP3_MAX_CELLS) {
                 g3.push_back(b->mem[q].c3);
                 idx.push_back(q);
             }
