@@ -544,11 +544,12 @@ int swpc_batch_run(swpc_batch *b, int it0, int nsteps, int graph_k);
    every mode):
      bit 0 (default on): FP32 P-SV members with fused steps, no snapshot
            streams and their sources inside the fused kernels;
-     bit 1 (default on): FP32 3-D members with a density, the one-pass
-           stress and z-marching velocity forms, no snapshots and no body
-           force (measured 1.00x the autotuned branches on the 48-run mix,
-           whose 3-D members fill the GPU on their own, and 1.05x on 64
-           small 3-D grids). */
+     bit 1 (default on): FP32 3-D members of at most 2^21 cells with a
+           density, the one-pass stress and z-marching velocity forms, no
+           snapshots and no body force (larger members, which fill the GPU
+           on their own, stay branches; measured 1.05x the branches on 64
+           small 3-D grids, within 0.5 % of the autotuned branches on the
+           48-run mix). */
 int swpc_batch_set_grouping(swpc_batch *b, int mode);
 int swpc_batch_sync(swpc_batch *b);
 
