@@ -2988,6 +2988,10 @@ int batch_step3(swpc3d_ctx *c) { return step_dev_any(c); }
 
 void batch_done3(swpc3d_ctx *c, int it0, int it1) { note_snapshots(c, it0, it1); }
 
+// The learned-boundary fill's strip cache flags were advanced by steps that
+// were captured but will not run.
+void batch_abort3(swpc3d_ctx *c) { lab_stale(c); }
+
 // ---------------------------------------------------------------------------
 // Groups of 3-D runs in one launch per phase (batches)
 // ---------------------------------------------------------------------------

@@ -19,6 +19,7 @@ int batch_member3(swpc3d_ctx *c, BatchMember &m);   // fails unless the member c
 int batch_check3(swpc3d_ctx *c, int it0, int n);    // its sources cover steps it0..it0+n-1
 int batch_step3(swpc3d_ctx *c);                     // enqueue one step on its stream
 void batch_done3(swpc3d_ctx *c, int it0, int it1);  // host bookkeeping after steps it0..it1
+void batch_abort3(swpc3d_ctx *c);                   // captured steps that will not run: forget their effects
 
 int batch_member2(swpc2d_ctx *c, BatchMember &m);
 int batch_check2(swpc2d_ctx *c, int it0, int n);

@@ -148,6 +148,8 @@ static int batch_capture(swpc_batch *b, int k)
     const hipError_t ee = hipStreamEndCapture(b->stream, &g);
     if (r != SWPC_OK || e != hipSuccess || ee != hipSuccess || !g) {
         if (g) (void)hipGraphDestroy(g);
+        for (auto &x : b->mem)
+            if (x.c3) batch_abort3(x.c3);
         if (r != SWPC_OK) return r;
         return fail(SWPC_EHIP, "batch graph capture failed: %s", hipGetErrorString(e != hipSuccess ? e : ee));
     }
@@ -155,6 +157,8 @@ static int batch_capture(swpc_batch *b, int k)
     const hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
     if (ei != hipSuccess) {
         (void)hipGraphDestroy(g);
+        for (auto &x : b->mem)
+            if (x.c3) batch_abort3(x.c3);
         return fail(SWPC_EHIP, "batch graph instantiation failed: %s", hipGetErrorString(ei));
     }
     b->graph = g;
